@@ -1,0 +1,126 @@
+// C ABI over cf_oracle for the Python tests (ctypes) and the CPU baseline leg of
+// bench.py.  Test infrastructure only: the product library never links this.
+#include "cf_oracle.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+using namespace cforacle;
+
+extern "C" {
+
+uint64_t cfo_wyhash(const void* key, size_t len, uint64_t seed) { return wyhash(key, len, seed, kWySalt); }
+uint64_t cfo_lmer_hash(uint64_t lmer) { return lmer_hash(lmer); }
+
+int64_t cfo_minimizers(const char* frag, size_t len, int km1, int l, int brute, uint64_t* out,
+                       size_t cap) {
+    std::vector<uint64_t> H;
+    if (brute) minimizer_hashes_brute(frag, len, km1, l, H);
+    else minimizer_hashes_literal(frag, len, km1, l, H);
+    if (H.size() > cap) return -int64_t(H.size());
+    std::memcpy(out, H.data(), H.size() * 8);
+    return int64_t(H.size());
+}
+
+// Records -> sequence bytes concatenated, rec_off[n+1].
+int64_t cfo_parse(const char* buf, size_t n, int fastq, char* seq_out, uint64_t* rec_off,
+                  size_t cap_bytes, size_t cap_recs) {
+    const auto recs = fastq ? parse_fastq(buf, n) : parse_fasta(buf, n);
+    size_t tot = 0;
+    for (const auto& r : recs) tot += r.size();
+    if (recs.size() > cap_recs || tot > cap_bytes) return -1;
+    size_t p = 0;
+    rec_off[0] = 0;
+    for (size_t i = 0; i < recs.size(); ++i) {
+        std::memcpy(seq_out + p, recs[i].data(), recs[i].size());
+        p += recs[i].size();
+        rec_off[i + 1] = p;
+    }
+    return int64_t(recs.size());
+}
+
+int64_t cfo_fragments(const char* seq, const uint64_t* rec_off, size_t n_rec, int k,
+                      uint64_t* beg_out, uint64_t* len_out, size_t cap) {
+    size_t c = 0;
+    for (size_t r = 0; r < n_rec; ++r) {
+        const auto fr = fragments(seq + rec_off[r], rec_off[r + 1] - rec_off[r], k);
+        for (const auto& f : fr) {
+            if (c < cap) beg_out[c] = rec_off[r] + f.first, len_out[c] = f.second;
+            ++c;
+        }
+    }
+    return c > cap ? -int64_t(c) : int64_t(c);
+}
+
+// Partition all records.  Outputs per weak super k-mer: g, packed attribute,
+// W label words, global position of its first base. Emission order.
+int64_t cfo_partition(const char* seq, const uint64_t* rec_off, size_t n_rec, int k, int l,
+                      uint32_t G, int scan_form, uint32_t* g_out, uint16_t* att_out,
+                      uint64_t* label_out, uint64_t* pos_out, size_t cap) {
+    const unsigned W = label_words(k, l);
+    size_t c = 0;
+    std::vector<SuperKmer> sk;
+    for (size_t r = 0; r < n_rec; ++r) {
+        const char* rs = seq + rec_off[r];
+        for (const auto& f : fragments(rs, rec_off[r + 1] - rec_off[r], k)) {
+            sk.clear();
+            if (scan_form) partition_fragment_scan(rs + f.first, f.second, k, l, G, sk);
+            else partition_fragment(rs + f.first, f.second, k, l, G, sk);
+            for (const auto& s : sk) {
+                if (c < cap) {
+                    g_out[c] = s.g;
+                    att_out[c] = att_pack(s.len, s.l_disc, s.r_disc);
+                    encode_label(rs + f.first + s.off, s.len, W, label_out + c * W);
+                    pos_out[c] = rec_off[r] + f.first + s.off;
+                }
+                ++c;
+            }
+        }
+    }
+    return c > cap ? -int64_t(c) : int64_t(c);
+}
+
+// Construct all subgraphs from bucketed records (bucket_off[G+1], records of
+// bucket g at [bucket_off[g], bucket_off[g+1])).  mode 0 = literal, 1 = closed
+// form.  Output sorted by (g, key); keys as 4 words (Kmer layout, w[0] low).
+int64_t cfo_construct(int k, int l, uint32_t G, int mode, const uint64_t* bucket_off,
+                      const uint16_t* att, const uint64_t* labels, int threads, uint32_t* g_out,
+                      uint64_t* key_out, uint32_t* ef_out, uint8_t* st_out, size_t cap) {
+    const unsigned W = label_words(k, l);
+    if (threads < 1) threads = 1;
+    std::vector<std::vector<VertexState>> per_t(threads);
+    std::atomic<uint32_t> next{0};
+    auto work = [&](int t) {
+        while (true) {
+            const uint32_t g = next.fetch_add(1);
+            if (g >= G) break;
+            const uint64_t b = bucket_off[g], e = bucket_off[g + 1];
+            if (b == e) continue;
+            if (mode == 0) construct_literal(k, W, g, e - b, att + b, labels + b * W, per_t[t]);
+            else construct_closed_form(k, W, g, e - b, att + b, labels + b * W, per_t[t]);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < threads; ++t) th.emplace_back(work, t);
+    for (auto& x : th) x.join();
+    std::vector<VertexState> all;
+    for (auto& v : per_t) all.insert(all.end(), v.begin(), v.end());
+    sort_states(k, all);
+    if (all.size() > cap) return -int64_t(all.size());
+    for (size_t i = 0; i < all.size(); ++i) {
+        g_out[i] = all[i].g;
+        std::memcpy(key_out + 4 * i, all[i].key.w.data(), 32);
+        ef_out[i] = all[i].st.e_f;
+        st_out[i] = all[i].st.status;
+    }
+    return int64_t(all.size());
+}
+
+uint32_t cfo_edge_at(uint32_t e_f, int side, int f_th) { return edge_at(e_f, side, f_th); }
+
+}  // extern "C"
