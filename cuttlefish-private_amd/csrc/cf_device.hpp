@@ -1,0 +1,178 @@
+// Device-side primitives shared by the partition and construct kernels (gfx950).
+//
+// Encodings (all integer work):
+//  * base codes A=0 C=1 G=2 T=3 (include/DNA.hpp:17-25); E (empty) = 5, N = 4.
+//  * input stream: 2-bit bases packed MSB-first, 32 per 64-bit word: base j at
+//    bits 63-2(j%32) .. 62-2(j%32) of word j/32.
+//  * super k-mer label: W words MSB-aligned, word W-1 holds bases 0..31
+//    (include/Super_Kmer_Chunk.hpp:394-405, include/Subgraph.hpp:403-411).
+//  * k-mer keys: Kmer<k> layout (include/Kmer.hpp:45-54), KW = ceil(k/32) words,
+//    w[0] = least-significant 64 bits, base 0 of the k-mer most significant.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cfgpu {
+
+constexpr uint8_t BASE_N = 4;
+constexpr uint8_t BASE_E = 5;
+
+// Cuttlefish-salted wyhash constants for 8-byte keys with seed 0
+// (include/Minimizer_Utility.hpp:26, 97-107); seed0 = wymix(s0, s1).
+struct WyConst {
+    uint64_t s0, s1, seed0;
+};
+
+__device__ __forceinline__ uint64_t lmer_hash(uint64_t x, const WyConst& c) {
+    // wyhash(&x, 8, 0, salt) (include/wyhash/wyhash.h:118-142) specialised to
+    // len == 8: a = rot32(x), b = x, then two 64x64->128 multiply-folds.
+    uint64_t a = ((x << 32) | (x >> 32)) ^ c.s1;
+    uint64_t b = x ^ c.seed0;
+    uint64_t lo = a * b, hi = __umul64hi(a, b);
+    a = lo ^ c.s0 ^ 8ull;
+    b = hi ^ c.s1;
+    return (a * b) ^ __umul64hi(a, b);
+}
+
+// Reverse the order of the 32 2-bit groups of a word.
+__device__ __forceinline__ uint64_t rev2(uint64_t x) {
+    const uint64_t r = __builtin_bitreverse64(x);
+    return ((r >> 1) & 0x5555555555555555ull) | ((r & 0x5555555555555555ull) << 1);
+}
+
+// Reverse complement of an MSB-first l-mer of 2l bits held in the low bits.
+__device__ __forceinline__ uint64_t lmer_rc(uint64_t x, unsigned l) {
+    const uint64_t mask = l == 32 ? ~0ull : ((1ull << (2 * l)) - 1);
+    return (rev2(x) >> (64 - 2 * l)) ^ mask;
+}
+
+// 64 bases starting at base `pos` of a forward-ordered MSB-first stream whose
+// word 0 holds bases [base0, base0+32).  `words` must hold word idx+1.
+__device__ __forceinline__ uint64_t stream_get64(const uint64_t* words, uint64_t pos) {
+    const uint64_t wi = pos >> 5;
+    const unsigned sh = unsigned(pos & 31) << 1;
+    const uint64_t hi = words[wi];
+    return sh ? (hi << sh) | (words[wi + 1] >> (64 - sh)) : hi;
+}
+
+// 64 bases starting at base `pos` of an MSB-aligned label of W words.  Words
+// past the label's end read as zero.
+__device__ __forceinline__ uint64_t label_get64(const uint64_t* L, unsigned W, unsigned pos) {
+    const unsigned wi = pos >> 5;
+    const unsigned sh = (pos & 31) << 1;
+    const uint64_t hi = wi < W ? L[W - 1 - wi] : 0;
+    if (!sh) return hi;
+    const uint64_t lo = wi + 1 < W ? L[W - 2 - wi] : 0;
+    return (hi << sh) | (lo >> (64 - sh));
+}
+
+__device__ __forceinline__ uint8_t label_base(const uint64_t* L, unsigned W, unsigned i) {
+    return uint8_t((L[W - 1 - (i >> 5)] >> (62 - ((i & 31) << 1))) & 3);
+}
+
+__device__ __forceinline__ uint8_t comp_base(uint8_t b) { return b < 4 ? uint8_t(3 - b) : b; }
+
+// ---------------------------------------------------------------------------
+// Multi-word k-mers (KW words, runtime k with 32(KW-1) < k <= 32 KW).
+template <int KW>
+struct Kmer {
+    uint64_t w[KW];
+};
+
+template <int KW>
+__device__ __forceinline__ bool km_less(const Kmer<KW>& a, const Kmer<KW>& b) {
+#pragma unroll
+    for (int i = KW - 1; i > 0; --i)
+        if (a.w[i] != b.w[i]) return a.w[i] < b.w[i];
+    return a.w[0] < b.w[0];
+}
+
+template <int KW>
+__device__ __forceinline__ bool km_eq(const Kmer<KW>& a, const Kmer<KW>& b) {
+    bool e = true;
+#pragma unroll
+    for (int i = 0; i < KW; ++i) e &= a.w[i] == b.w[i];
+    return e;
+}
+
+// k-mer at label offset i (bases i..i+k-1).
+template <int KW>
+__device__ __forceinline__ Kmer<KW> km_from_label(const uint64_t* L, unsigned W, unsigned i, unsigned k) {
+    Kmer<KW> x;
+    const unsigned kt = k - 32 * (KW - 1);  // bases in the top word (1..31 for odd k)
+    x.w[KW - 1] = label_get64(L, W, i) >> (64 - 2 * kt);
+#pragma unroll
+    for (int j = KW - 2; j >= 0; --j) x.w[j] = label_get64(L, W, i + kt + 32 * (KW - 2 - j));
+    return x;
+}
+
+// Reverse complement (include/Kmer.hpp:479-506).
+template <int KW>
+__device__ __forceinline__ Kmer<KW> km_rc(const Kmer<KW>& x, unsigned k) {
+    Kmer<KW> r;
+    // Reverse all 32*KW groups, then shift right by s = 64*KW - 2k bits.
+    uint64_t t[KW];
+#pragma unroll
+    for (int i = 0; i < KW; ++i) t[i] = rev2(x.w[KW - 1 - i]);
+    const unsigned s = 64 * KW - 2 * k;  // 2..62, never 0 (k odd)
+#pragma unroll
+    for (int i = 0; i < KW; ++i) {
+        uint64_t v = t[i] >> s;
+        if (i + 1 < KW) v |= t[i + 1] << (64 - s);
+        r.w[i] = ~v;
+    }
+    const unsigned kt = k - 32 * (KW - 1);
+    r.w[KW - 1] &= (1ull << (2 * kt)) - 1;
+    return r;
+}
+
+// k-mer rolled one base backward: prepend base b, drop the last base.
+template <int KW>
+__device__ __forceinline__ Kmer<KW> km_prepend(const Kmer<KW>& x, uint8_t b, unsigned k) {
+    Kmer<KW> r;
+#pragma unroll
+    for (int i = 0; i < KW; ++i) {
+        uint64_t v = x.w[i] >> 2;
+        if (i + 1 < KW) v |= x.w[i + 1] << 62;
+        r.w[i] = v;
+    }
+    const unsigned kt = k - 32 * (KW - 1);
+    r.w[KW - 1] |= uint64_t(b) << (2 * (kt - 1));
+    return r;
+}
+
+// Table hash (any hash is admissible: the reference's unitig extraction is
+// independent of table iteration order, include/Kmer_Hasher.hpp:12-21).
+__device__ __forceinline__ uint64_t fmix64(uint64_t h) {
+    h ^= h >> 33;
+    h *= 0xff51afd7ed558ccdull;
+    h ^= h >> 33;
+    h *= 0xc4ceb9fe1a85ec53ull;
+    h ^= h >> 33;
+    return h;
+}
+
+template <int KW>
+__device__ __forceinline__ uint64_t km_hash(const Kmer<KW>& x) {
+    uint64_t h = x.w[0];
+#pragma unroll
+    for (int i = 1; i < KW; ++i) h = fmix64(h) ^ x.w[i];
+    return fmix64(h);
+}
+
+// Saturating 4-bit counters (include/State_Config.hpp:207-217): add one to the
+// front field `fb` (if < 4) and the back field `bb` (if < 4).
+__device__ __forceinline__ uint32_t ef_add(uint32_t e, uint8_t fb, uint8_t bb) {
+    if (fb < 4) {
+        const unsigned off = 4u * fb;
+        if (((e >> off) & 0xF) < 0xF) e += 1u << off;
+    }
+    if (bb < 4) {
+        const unsigned off = 16u + 4u * bb;
+        if (((e >> off) & 0xF) < 0xF) e += 1u << off;
+    }
+    return e;
+}
+
+}  // namespace cfgpu

@@ -1,0 +1,104 @@
+// Kernel launch interface between the C-ABI layer (cfgpu.cpp) and the HIP
+// kernels.  Every launcher validates nothing: the callers check shapes first.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cf_device.hpp"
+
+namespace cfgpu {
+
+constexpr int P_TILE = 2048;            // (k-1)-mer positions per partition workgroup
+constexpr int P_WG = 256;               // partition workgroup size
+constexpr uint32_t G_INVALID = 0xFFFFFFFFu;
+constexpr int N_FEAT = 19;              // 16-bin minimizer histogram + 3 size features
+
+struct PartitionArgs {
+    const uint64_t* packed;    // MSB-first 2-bit stream, padded with >= 2 zero words
+    uint64_t n_words;          // readable words of `packed` (incl. padding)
+    uint64_t n_bases;
+    const uint64_t* frag_off;  // n_frag + 1 base offsets, frag_off[n_frag] == n_bases
+    uint64_t n_frag;
+    uint32_t k, l, w;          // w = k - l: (k-1)-mers per piece at most
+    uint32_t g_mask, g_bits;   // G - 1, log2 G
+    WyConst wy;
+};
+
+// Pass 1: per-position minimizer hash -> (g | bin << 16), and per-tile last
+// g-change position (+1, 0 = none).
+void launch_minimizer_g(const PartitionArgs& a, uint32_t* gbin, uint64_t* tile_lc, hipStream_t s);
+
+// Pass 2 (emit = false): per-bucket record counts.  Pass 3 (emit = true):
+// scatter records into buckets at cursor[g] (pre-set to the bucket offsets).
+void launch_pieces(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc,
+                   uint32_t W, bool emit, uint32_t* bucket_cnt, uint64_t* cursor,
+                   uint32_t* att, uint64_t* labels, uint8_t* mbin, hipStream_t s);
+
+// Exclusive scan of G counts -> off[G+1]; also copies off into cursor.
+void launch_bucket_scan(const uint32_t* cnt, uint32_t G, uint64_t* off, uint64_t* cursor,
+                        hipStream_t s);
+
+// Per-bucket statistics: [n_rec, n_kmer_instances, n_bases, 0] as u64 and the
+// MLP feature vector (N_FEAT floats).
+void launch_bucket_features(const uint64_t* off, const uint32_t* att, const uint8_t* mbin,
+                            uint32_t G, uint32_t k, uint64_t* stats, float* feat, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+struct ConstructArgs {
+    const uint32_t* att;        // bucketed records: len | l_disc<<8 | r_disc<<9 | g<<16
+    const uint64_t* labels;     // W words per record
+    uint32_t W, k;
+    const uint64_t* bucket_off; // G+1 record offsets
+    const uint64_t* tbl_ptr;    // per bucket table base (device address)
+    const uint32_t* tbl_log2;   // per bucket log2(capacity)
+    const uint32_t* tbl_limit;  // per bucket max distinct keys before overflow
+    uint32_t* n_vert;           // per bucket distinct vertex count
+    uint32_t* overflow;         // per bucket overflow flag
+    const uint32_t* item_bucket;  // work items: bucket id
+    const uint32_t* item_first;   //   first record (relative to the bucket)
+    const uint32_t* item_count;   //   record count
+    uint32_t n_items;
+    uint32_t* queue_head;       // persistent mode: next item
+};
+
+// Entry stride in bytes for KW key words: KW u64 key words, u32 e_f, u32 status.
+inline uint32_t entry_bytes(uint32_t KW) { return 8 * KW + 8; }
+
+void launch_clear_tables(uint8_t* tables, uint64_t n_slots, uint32_t KW, hipStream_t s);
+// Non-persistent: one workgroup per work item in [item0, item0 + n).
+void launch_construct(const ConstructArgs& a, uint32_t KW, uint32_t item0, uint32_t n,
+                      uint32_t wg, hipStream_t s);
+// Persistent: `grid` workgroups pull items from queue_head until exhausted.
+void launch_construct_persistent(const ConstructArgs& a, uint32_t KW, uint32_t grid, uint32_t wg,
+                                 hipStream_t s);
+void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+// n_vert[b] = overflow[b] = 0 for every b in list[0..n).
+void launch_zero_buckets(const uint32_t* list, uint32_t n, uint32_t* n_vert, uint32_t* overflow,
+                         hipStream_t s);
+
+// Compact the occupied slots of bucket tables into key/e_f/status/dfa arrays.
+// out_cursor[i] receives the number of occupied entries of buckets[i]; they
+// are written from out_off[i] up to (excluding) out_off[i+1].
+void launch_export(const uint64_t* tbl_ptr, const uint32_t* tbl_log2,
+                   uint32_t KW, const uint32_t* buckets, uint32_t n_buckets,
+                   const uint64_t* out_off, uint32_t* out_cursor, uint64_t* keys_out,
+                   uint32_t* ef_out, uint8_t* st_out, uint8_t* dfa_out, uint32_t cutoff,
+                   hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// FP32 MLP bucket-size predictor: N_FEAT -> H -> H -> 1 (ReLU), predicts
+// log(distinct vertices / k-mer instances) of a bucket.
+constexpr int MLP_H = 64;
+constexpr int MLP_PARAMS = N_FEAT * MLP_H + MLP_H + MLP_H * MLP_H + MLP_H + MLP_H + 1;
+
+constexpr int MLP_TRAIN_MAX = 128;  // training buckets (one workgroup holds all activations)
+int mlp_param_floats();             // MLP_PARAMS + 2 * N_FEAT (standardisation)
+// `adam` holds 2 * MLP_PARAMS floats of optimiser state.
+void launch_mlp_train(float* params, const float* feat, const uint32_t* idx, const float* target,
+                      uint32_t n, uint32_t epochs, float lr, uint32_t seed, float* adam,
+                      float* loss_out, hipStream_t s);
+void launch_mlp_infer(const float* params, const float* feat, uint32_t n, float* pred,
+                      hipStream_t s);
+
+}  // namespace cfgpu

@@ -1,0 +1,883 @@
+// libcfgpu: C ABI (include/cfgpu.h) over the gfx950 partition / construct kernels.
+//
+// Orchestration only: input checks, device buffers, table sizing, work-queue
+// construction, hipGraph capture and timing.  All per-base / per-k-mer work runs
+// in the HIP kernels (partition.hip, construct.hip, mlp.hip).
+#include "../../include/cfgpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "cf_kernels.hpp"
+
+using namespace cfgpu;
+
+namespace {
+
+constexpr uint32_t G_FIXED = 16384;
+constexpr uint64_t SAMPLE_TARGET = MLP_TRAIN_MAX;
+constexpr uint32_t MAX_ROUNDS = 8;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t ensure(size_t bytes) {
+        bytes = std::max<size_t>(bytes, 256);
+        if (bytes <= n) return hipSuccess;
+        release();
+        const hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        n = bytes;
+        return hipSuccess;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+uint64_t wymix_host(uint64_t a, uint64_t b) {
+    const unsigned __int128 r = static_cast<unsigned __int128>(a) * b;
+    return uint64_t(r) ^ uint64_t(r >> 64);
+}
+
+uint32_t ceil_log2(uint64_t x) {
+    uint32_t r = 0;
+    while ((1ull << r) < x) ++r;
+    return r;
+}
+
+struct Event {
+    hipEvent_t e = nullptr;
+    Event() { (void)hipEventCreate(&e); }
+    ~Event() { if (e) (void)hipEventDestroy(e); }
+};
+
+}  // namespace
+
+struct cfgpu_handle {
+    cfgpu_params prm{};
+    uint32_t KW = 1, W = 1, w = 1, G = G_FIXED, g_bits = 14;
+    WyConst wy{};
+    int cus = 256;
+    hipStream_t stream = nullptr;
+    std::string err;
+    cfgpu_tuning tune{0.6f, 256, 2048, 8, 1.15f, 1};
+
+    // partition state
+    DevBuf packed, frag, gbin, tile_lc, bcnt, boff, cursor, att, labels, mbin, stats, feat;
+    uint64_t n_bases = 0, n_frag = 0, n_records = 0;
+    bool partitioned = false;
+    std::vector<uint64_t> h_boff, h_stats;
+
+    // construct state
+    DevBuf d_tbl_ptr, d_log2, d_limit, d_nvert, d_overflow, d_items_b, d_items_f, d_items_c,
+        d_qhead, d_round_list;
+    std::vector<std::unique_ptr<DevBuf>> pools;  // one per sizing round, reused across calls
+    uint32_t round_idx = 0;
+    std::vector<uint64_t> h_tbl_ptr;
+    std::vector<uint32_t> h_log2, h_limit, h_nvert, h_flags;
+    std::vector<float> h_pred;
+    bool constructed = false;
+
+    // MLP
+    DevBuf mlp_params, mlp_adam, mlp_idx, mlp_target, mlp_pred, mlp_loss;
+
+    // export scratch
+    DevBuf ex_keys, ex_ef, ex_st, ex_dfa, ex_off, ex_cursor, ex_list;
+
+    // graph cache
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint64_t> graph_key;
+
+    cfgpu_run_stats rs{};
+
+    ~cfgpu_handle() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int fail(cfgpu_handle* h, int code, const char* fmt, ...) {
+    if (h) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        h->err = buf;
+    }
+    return code;
+}
+
+#define HIPCK(h, x)                                                                         \
+    do {                                                                                    \
+        const hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess)                                                               \
+            return fail((h), e_ == hipErrorOutOfMemory ? CFGPU_E_NOMEM : CFGPU_E_HIP,       \
+                        "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+#define CK(x)                    \
+    do {                         \
+        const int r_ = (x);      \
+        if (r_ != CFGPU_OK) return r_; \
+    } while (0)
+
+double elapsed_ms(const Event& a, const Event& b) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a.e, b.e);
+    return ms;
+}
+
+ConstructArgs construct_args(cfgpu_handle* h) {
+    ConstructArgs a{};
+    a.att = h->att.as<uint32_t>();
+    a.labels = h->labels.as<uint64_t>();
+    a.W = h->W;
+    a.k = h->prm.k;
+    a.bucket_off = h->boff.as<uint64_t>();
+    a.tbl_ptr = h->d_tbl_ptr.as<uint64_t>();
+    a.tbl_log2 = h->d_log2.as<uint32_t>();
+    a.tbl_limit = h->d_limit.as<uint32_t>();
+    a.n_vert = h->d_nvert.as<uint32_t>();
+    a.overflow = h->d_overflow.as<uint32_t>();
+    a.item_bucket = h->d_items_b.as<uint32_t>();
+    a.item_first = h->d_items_f.as<uint32_t>();
+    a.item_count = h->d_items_c.as<uint32_t>();
+    a.queue_head = h->d_qhead.as<uint32_t>();
+    return a;
+}
+
+uint64_t bucket_records(const cfgpu_handle* h, uint32_t b) { return h->h_stats[4 * b + 0]; }
+uint64_t bucket_kmers(const cfgpu_handle* h, uint32_t b) { return h->h_stats[4 * b + 1]; }
+
+// Capacity (log2) that can hold every k-mer instance of the bucket at the
+// target load: a table of this size can never overflow.
+uint32_t upper_log2(const cfgpu_handle* h, uint32_t b) {
+    const double need = double(std::max<uint64_t>(bucket_kmers(h, b), 8)) / h->tune.load_factor;
+    return std::max<uint32_t>(4, ceil_log2(uint64_t(std::ceil(need))));
+}
+
+uint32_t limit_for(uint32_t log2) { return uint32_t(((1ull << log2) * 7) / 8); }
+
+// One sizing round: allocate a pool for `buckets` with capacities `log2s`,
+// upload pointers / limits, zero their counters, clear their tables and build
+// the work-item list in the given order.  Returns the number of work items.
+int prepare_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
+                  const std::vector<uint32_t>& log2s, uint32_t& n_items, uint64_t& n_slots_out,
+                  DevBuf*& pool_out) {
+    const uint32_t eb = entry_bytes(h->KW);
+    uint64_t n_slots = 0;
+    for (uint32_t l2 : log2s) n_slots += 1ull << l2;
+    if (h->pools.size() <= h->round_idx) h->pools.emplace_back(new DevBuf());
+    DevBuf* pool = h->pools[h->round_idx++].get();
+    HIPCK(h, pool->ensure(n_slots * eb));
+    uint64_t off = 0;
+    std::vector<uint32_t> ib, ifirst, icnt;
+    const uint32_t chunk = std::max<uint32_t>(h->tune.workgroup, h->tune.chunk_records);
+    for (size_t i = 0; i < buckets.size(); ++i) {
+        const uint32_t b = buckets[i];
+        h->h_tbl_ptr[b] = reinterpret_cast<uint64_t>(pool->as<uint8_t>() + off * eb);
+        h->h_log2[b] = log2s[i];
+        h->h_limit[b] = limit_for(log2s[i]);
+        off += 1ull << log2s[i];
+        const uint64_t nr = bucket_records(h, b);
+        for (uint64_t f = 0; f < nr; f += chunk) {
+            ib.push_back(b);
+            ifirst.push_back(uint32_t(f));
+            icnt.push_back(uint32_t(std::min<uint64_t>(chunk, nr - f)));
+        }
+    }
+    n_items = uint32_t(ib.size());
+    HIPCK(h, h->d_items_b.ensure(ib.size() * 4));
+    HIPCK(h, h->d_items_f.ensure(ib.size() * 4));
+    HIPCK(h, h->d_items_c.ensure(ib.size() * 4));
+    HIPCK(h, h->d_round_list.ensure(buckets.size() * 4));
+    const size_t G = h->G;
+    HIPCK(h, hipMemcpyAsync(h->d_tbl_ptr.p, h->h_tbl_ptr.data(), G * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCK(h, hipMemcpyAsync(h->d_log2.p, h->h_log2.data(), G * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCK(h, hipMemcpyAsync(h->d_limit.p, h->h_limit.data(), G * 4, hipMemcpyHostToDevice, h->stream));
+    if (n_items) {
+        HIPCK(h, hipMemcpyAsync(h->d_items_b.p, ib.data(), ib.size() * 4, hipMemcpyHostToDevice, h->stream));
+        HIPCK(h, hipMemcpyAsync(h->d_items_f.p, ifirst.data(), ib.size() * 4, hipMemcpyHostToDevice, h->stream));
+        HIPCK(h, hipMemcpyAsync(h->d_items_c.p, icnt.data(), ib.size() * 4, hipMemcpyHostToDevice, h->stream));
+    }
+    if (!buckets.empty())
+        HIPCK(h, hipMemcpyAsync(h->d_round_list.p, buckets.data(), buckets.size() * 4,
+                                hipMemcpyHostToDevice, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));  // host vectors above are reused
+    n_slots_out = n_slots;
+    pool_out = pool;
+    return CFGPU_OK;
+}
+
+// Zero the per-bucket counters of a round and clear its tables (kernels, so
+// the sequence can be captured in a graph).
+void enqueue_round_reset(cfgpu_handle* h, uint32_t n_buckets, DevBuf* pool, uint64_t n_slots) {
+    launch_zero_buckets(h->d_round_list.as<uint32_t>(), n_buckets, h->d_nvert.as<uint32_t>(),
+                        h->d_overflow.as<uint32_t>(), h->stream);
+    launch_clear_tables(pool->as<uint8_t>(), n_slots, h->KW, h->stream);
+}
+
+int finish_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
+                 std::vector<uint32_t>& overflowed) {
+    const size_t G = h->G;
+    std::vector<uint32_t> ovf(G);
+    HIPCK(h, hipMemcpyAsync(ovf.data(), h->d_overflow.p, G * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipMemcpyAsync(h->h_nvert.data(), h->d_nvert.p, G * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    overflowed.clear();
+    for (uint32_t b : buckets) {
+        if (ovf[b]) {
+            overflowed.push_back(b);
+            h->h_flags[b] |= CFGPU_BUCKET_OVERFLOWED;
+            h->h_flags[b] &= ~uint32_t(CFGPU_BUCKET_BUILT);
+        } else {
+            h->h_flags[b] |= CFGPU_BUCKET_BUILT;
+        }
+    }
+    return CFGPU_OK;
+}
+
+// Plain path: one launch per bucket, in bucket order.
+int run_round_plain(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
+                    const std::vector<uint32_t>& log2s, std::vector<uint32_t>& overflowed) {
+    uint32_t n_items = 0;
+    uint64_t n_slots = 0;
+    DevBuf* pool = nullptr;
+    CK(prepare_round(h, buckets, log2s, n_items, n_slots, pool));
+    enqueue_round_reset(h, uint32_t(buckets.size()), pool, n_slots);
+    const ConstructArgs a = construct_args(h);
+    const uint32_t chunk = std::max<uint32_t>(h->tune.workgroup, h->tune.chunk_records);
+    uint32_t item = 0;
+    for (uint32_t b : buckets) {
+        const uint32_t n = uint32_t((bucket_records(h, b) + chunk - 1) / chunk);
+        ConstructArgs ab = a;
+        ab.n_items = n_items;
+        launch_construct(ab, h->KW, item, n, h->tune.workgroup, h->stream);
+        item += n;
+    }
+    HIPCK(h, hipGetLastError());
+    return finish_round(h, buckets, overflowed);
+}
+
+// Batched path: reset + persistent work-queue launch, captured in a hipGraph.
+int run_round_batched(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
+                      const std::vector<uint32_t>& log2s, std::vector<uint32_t>& overflowed,
+                      bool use_graph) {
+    uint32_t n_items = 0;
+    uint64_t n_slots = 0;
+    DevBuf* pool = nullptr;
+    CK(prepare_round(h, buckets, log2s, n_items, n_slots, pool));
+    ConstructArgs a = construct_args(h);
+    a.n_items = n_items;
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(n_items, uint32_t(h->cus) * h->tune.wgs_per_cu));
+    auto enqueue = [&]() {
+        launch_fill_u32(h->d_qhead.as<uint32_t>(), 0u, 1, h->stream);
+        enqueue_round_reset(h, uint32_t(buckets.size()), pool, n_slots);
+        launch_construct_persistent(a, h->KW, grid, h->tune.workgroup, h->stream);
+    };
+    if (!use_graph || !h->tune.use_graph) {
+        // direct launches; the construct kernel alone is timed with events
+        Event k0, k1;
+        launch_fill_u32(h->d_qhead.as<uint32_t>(), 0u, 1, h->stream);
+        enqueue_round_reset(h, uint32_t(buckets.size()), pool, n_slots);
+        HIPCK(h, hipEventRecord(k0.e, h->stream));
+        launch_construct_persistent(a, h->KW, grid, h->tune.workgroup, h->stream);
+        HIPCK(h, hipEventRecord(k1.e, h->stream));
+        HIPCK(h, hipGetLastError());
+        const int r = finish_round(h, buckets, overflowed);
+        if (use_graph) h->rs.t_construct_kernel_ms = elapsed_ms(k0, k1);
+        return r;
+    }
+    // The graph captures kernel arguments (device pointers and sizes); array
+    // contents are read when it runs.  Re-capture when any captured value moves.
+    const void* ptrs[] = {pool->p, h->att.p, h->labels.p, h->boff.p, h->d_tbl_ptr.p, h->d_log2.p,
+                          h->d_limit.p, h->d_nvert.p, h->d_overflow.p, h->d_items_b.p,
+                          h->d_items_f.p, h->d_items_c.p, h->d_qhead.p, h->d_round_list.p};
+    std::vector<uint64_t> key = {n_slots, n_items, grid, h->KW, h->W, h->prm.k, h->tune.workgroup,
+                                 buckets.size()};
+    for (const void* q : ptrs) key.push_back(reinterpret_cast<uint64_t>(q));
+    if (!h->graph_exec || key != h->graph_key) {
+        if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec), h->graph_exec = nullptr;
+        if (h->graph) (void)hipGraphDestroy(h->graph), h->graph = nullptr;
+        HIPCK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        enqueue();
+        HIPCK(h, hipStreamEndCapture(h->stream, &h->graph));
+        HIPCK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+        h->graph_key = key;
+    }
+    HIPCK(h, hipGraphLaunch(h->graph_exec, h->stream));
+    return finish_round(h, buckets, overflowed);
+}
+
+// Rebuild overflowed buckets with never-overflowing capacities until done.
+int fallback(cfgpu_handle* h, std::vector<uint32_t> todo, bool plain_growth) {
+    for (uint32_t round = 0; !todo.empty(); ++round) {
+        if (round >= MAX_ROUNDS) return fail(h, CFGPU_E_OVERFLOW, "table overflow persists after %u rounds", round);
+        ++h->rs.n_fallback_rounds;
+        std::vector<uint32_t> l2(todo.size());
+        for (size_t i = 0; i < todo.size(); ++i) {
+            const uint32_t b = todo[i];
+            l2[i] = plain_growth ? std::min(upper_log2(h, b), h->h_log2[b] + 2) : upper_log2(h, b);
+        }
+        std::vector<uint32_t> ovf;
+        if (plain_growth) CK(run_round_plain(h, todo, l2, ovf));
+        else CK(run_round_batched(h, todo, l2, ovf, false));
+        todo.swap(ovf);
+    }
+    return CFGPU_OK;
+}
+
+std::vector<uint32_t> nonempty_buckets(const cfgpu_handle* h) {
+    std::vector<uint32_t> v;
+    for (uint32_t b = 0; b < h->G; ++b)
+        if (bucket_records(h, b)) v.push_back(b);
+    return v;
+}
+
+void order_largest_first(std::vector<uint32_t>& v, const std::vector<double>& cost) {
+    std::stable_sort(v.begin(), v.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+}
+
+int construct_impl(cfgpu_handle* h, uint32_t mode) {
+    const size_t G = h->G;
+    h->round_idx = 0;
+    h->h_tbl_ptr.assign(G, 0);
+    h->h_log2.assign(G, 0);
+    h->h_limit.assign(G, 0);
+    h->h_nvert.assign(G, 0);
+    h->h_flags.assign(G, 0);
+    h->h_pred.assign(G, 0.f);
+    HIPCK(h, h->d_tbl_ptr.ensure(G * 8));
+    HIPCK(h, h->d_log2.ensure(G * 4));
+    HIPCK(h, h->d_limit.ensure(G * 4));
+    HIPCK(h, h->d_nvert.ensure(G * 4));
+    HIPCK(h, h->d_overflow.ensure(G * 4));
+    HIPCK(h, h->d_qhead.ensure(4));
+    launch_fill_u32(h->d_nvert.as<uint32_t>(), 0u, G, h->stream);
+    launch_fill_u32(h->d_overflow.as<uint32_t>(), 0u, G, h->stream);
+
+    cfgpu_run_stats& rs = h->rs;
+    rs.n_overflowed = rs.n_fallback_rounds = rs.n_samples = 0;
+    rs.t_sample_ms = rs.t_train_ms = rs.t_infer_ms = rs.t_construct_main_ms = rs.t_fallback_ms = 0;
+    rs.mlp_loss = rs.mispredict_mean_abs_log = rs.mispredict_under_frac = rs.fallback_rate = 0;
+    rs.t_construct_kernel_ms = 0;
+
+    std::vector<uint32_t> all = nonempty_buckets(h);
+    rs.n_buckets_nonempty = uint32_t(all.size());
+    Event e0, e1, e2, e3, e4, e5;
+    HIPCK(h, hipEventRecord(e0.e, h->stream));
+
+    std::vector<uint32_t> ovf;
+    std::vector<uint32_t> main_set = all;
+    std::vector<uint32_t> main_l2(all.size());
+    std::vector<double> cost(G, 0.0);
+    for (uint32_t b : all) cost[b] = double(bucket_kmers(h, b));
+
+    if (mode == CFGPU_MODE_PLAIN) {
+        // grow-and-retry: start at a quarter of the k-mer instances
+        for (size_t i = 0; i < all.size(); ++i) {
+            const uint32_t b = all[i];
+            const double guess = double(std::max<uint64_t>(bucket_kmers(h, b) / 4, 8)) / h->tune.load_factor;
+            main_l2[i] = std::min(upper_log2(h, b), std::max<uint32_t>(4, ceil_log2(uint64_t(guess))));
+        }
+        HIPCK(h, hipEventRecord(e2.e, h->stream));
+        CK(run_round_plain(h, main_set, main_l2, ovf));
+        HIPCK(h, hipEventRecord(e3.e, h->stream));
+        rs.n_overflowed = uint32_t(ovf.size());
+        CK(fallback(h, ovf, true));
+    } else if (mode == CFGPU_MODE_BATCHED) {
+        order_largest_first(main_set, cost);
+        for (size_t i = 0; i < main_set.size(); ++i) main_l2[i] = upper_log2(h, main_set[i]);
+        HIPCK(h, hipEventRecord(e2.e, h->stream));
+        CK(run_round_batched(h, main_set, main_l2, ovf, true));
+        HIPCK(h, hipEventRecord(e3.e, h->stream));
+        rs.n_overflowed = uint32_t(ovf.size());
+        CK(fallback(h, ovf, false));
+    } else if (mode == CFGPU_MODE_BATCHED_LEARNED) {
+        // 1. exact tables for an evenly spread sample of buckets: their distinct
+        //    counts are the training labels (and their tables are final).
+        std::vector<uint32_t> sample, rest;
+        const size_t n_s = std::min<size_t>(SAMPLE_TARGET, all.size() / 4);
+        if (n_s >= 8) {
+            const double step = double(all.size()) / double(n_s);
+            std::vector<char> in(all.size(), 0);
+            for (size_t i = 0; i < n_s; ++i) in[size_t(i * step)] = 1;
+            for (size_t i = 0; i < all.size(); ++i) (in[i] ? sample : rest).push_back(all[i]);
+        } else {
+            rest = all;
+        }
+        rs.n_samples = uint32_t(sample.size());
+        if (!sample.empty()) {
+            std::vector<uint32_t> l2(sample.size());
+            for (size_t i = 0; i < sample.size(); ++i) l2[i] = upper_log2(h, sample[i]);
+            order_largest_first(sample, cost);
+            for (size_t i = 0; i < sample.size(); ++i) l2[i] = upper_log2(h, sample[i]);
+            std::vector<uint32_t> so;
+            CK(run_round_batched(h, sample, l2, so, false));
+            CK(fallback(h, so, false));
+            for (uint32_t b : sample) h->h_flags[b] |= CFGPU_BUCKET_SAMPLE;
+        }
+        HIPCK(h, hipEventRecord(e1.e, h->stream));
+        // 2. train on (features, log(distinct / instances)), predict all buckets.
+        bool have_model = false;
+        if (sample.size() >= 8) {
+            std::vector<uint32_t> idx(sample.begin(), sample.end());
+            std::vector<float> tgt(idx.size());
+            for (size_t i = 0; i < idx.size(); ++i)
+                tgt[i] = float(std::log(double(std::max<uint32_t>(1, h->h_nvert[idx[i]])) /
+                                        double(std::max<uint64_t>(1, bucket_kmers(h, idx[i])))));
+            HIPCK(h, h->mlp_params.ensure(size_t(mlp_param_floats()) * 4));
+            HIPCK(h, h->mlp_adam.ensure(size_t(2 * MLP_PARAMS) * 4));
+            HIPCK(h, h->mlp_idx.ensure(idx.size() * 4));
+            HIPCK(h, h->mlp_target.ensure(idx.size() * 4));
+            HIPCK(h, h->mlp_pred.ensure(G * 4));
+            HIPCK(h, h->mlp_loss.ensure(4));
+            HIPCK(h, hipMemcpyAsync(h->mlp_idx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, h->stream));
+            HIPCK(h, hipMemcpyAsync(h->mlp_target.p, tgt.data(), tgt.size() * 4, hipMemcpyHostToDevice, h->stream));
+            launch_mlp_train(h->mlp_params.as<float>(), h->feat.as<float>(), h->mlp_idx.as<uint32_t>(),
+                             h->mlp_target.as<float>(), uint32_t(idx.size()), 300, 3e-3f, 12345u,
+                             h->mlp_adam.as<float>(), h->mlp_loss.as<float>(), h->stream);
+            HIPCK(h, hipEventRecord(e4.e, h->stream));
+            launch_mlp_infer(h->mlp_params.as<float>(), h->feat.as<float>(), uint32_t(G),
+                             h->mlp_pred.as<float>(), h->stream);
+            std::vector<float> pred(G);
+            float loss = 0.f;
+            HIPCK(h, hipMemcpyAsync(pred.data(), h->mlp_pred.p, G * 4, hipMemcpyDeviceToHost, h->stream));
+            HIPCK(h, hipMemcpyAsync(&loss, h->mlp_loss.p, 4, hipMemcpyDeviceToHost, h->stream));
+            HIPCK(h, hipEventRecord(e5.e, h->stream));
+            HIPCK(h, hipStreamSynchronize(h->stream));
+            rs.mlp_loss = loss;
+            rs.t_train_ms = elapsed_ms(e1, e4);
+            rs.t_infer_ms = elapsed_ms(e4, e5);
+            for (uint32_t b : rest) {
+                const double km = double(bucket_kmers(h, b));
+                const double p = std::isfinite(pred[b]) ? km * std::exp(std::min(0.0, double(pred[b]))) : km;
+                h->h_pred[b] = float(std::max(1.0, p));
+            }
+            have_model = true;
+        }
+        // 3. size and order the remaining buckets by the prediction.
+        main_set = rest;
+        std::vector<double> pcost(G, 0.0);
+        for (uint32_t b : rest) pcost[b] = have_model ? double(h->h_pred[b]) : cost[b];
+        order_largest_first(main_set, pcost);
+        main_l2.resize(main_set.size());
+        for (size_t i = 0; i < main_set.size(); ++i) {
+            const uint32_t b = main_set[i];
+            if (!have_model) {
+                main_l2[i] = upper_log2(h, b);
+                continue;
+            }
+            const double need = double(h->h_pred[b]) * h->tune.margin / h->tune.load_factor;
+            main_l2[i] = std::min(upper_log2(h, b), std::max<uint32_t>(4, ceil_log2(uint64_t(std::ceil(need)))));
+        }
+        HIPCK(h, hipEventRecord(e2.e, h->stream));
+        CK(run_round_batched(h, main_set, main_l2, ovf, true));
+        HIPCK(h, hipEventRecord(e3.e, h->stream));
+        rs.n_overflowed = uint32_t(ovf.size());
+        rs.fallback_rate = main_set.empty() ? 0.0 : double(ovf.size()) / double(main_set.size());
+        CK(fallback(h, ovf, false));
+    } else {
+        return fail(h, CFGPU_E_ARG, "unknown construct mode %u", mode);
+    }
+    Event ef;
+    HIPCK(h, hipEventRecord(ef.e, h->stream));
+    HIPCK(h, hipMemcpyAsync(h->h_nvert.data(), h->d_nvert.p, G * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+
+    if (mode == CFGPU_MODE_BATCHED_LEARNED) {
+        rs.t_sample_ms = elapsed_ms(e0, e1);
+        double sum = 0;
+        uint64_t n = 0, under = 0;
+        for (uint32_t b : main_set) {
+            if (h->h_pred[b] <= 0.f || !h->h_nvert[b]) continue;
+            sum += std::fabs(std::log(double(h->h_pred[b]) / double(h->h_nvert[b])));
+            under += double(h->h_pred[b]) < double(h->h_nvert[b]);
+            ++n;
+        }
+        rs.mispredict_mean_abs_log = n ? sum / double(n) : 0.0;
+        rs.mispredict_under_frac = n ? double(under) / double(n) : 0.0;
+    }
+    rs.t_construct_main_ms = elapsed_ms(e2, e3);
+    rs.t_fallback_ms = elapsed_ms(e3, ef);
+    rs.t_construct_ms = elapsed_ms(e0, ef);
+    rs.n_vertices = 0;
+    rs.table_slots = 0;
+    for (uint32_t b : all) {
+        rs.n_vertices += h->h_nvert[b];
+        rs.table_slots += 1ull << h->h_log2[b];
+        if (!(h->h_flags[b] & CFGPU_BUCKET_BUILT))
+            return fail(h, CFGPU_E_OVERFLOW, "bucket %u was not built", b);
+    }
+    rs.table_bytes = rs.table_slots * entry_bytes(h->KW);
+    h->constructed = true;
+    return CFGPU_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int cfgpu_open(const cfgpu_params* p, cfgpu_handle** out) {
+    if (!p || !out) return CFGPU_E_ARG;
+    *out = nullptr;
+    const uint32_t k = p->k, l = p->l;
+    if (k < 3 || k > 127 || !(k & 1) || l < 1 || l > 32 || l >= k || p->cutoff < 1 ||
+        p->cutoff > 15 || p->subgraph_count != G_FIXED || 2 * k - l > 255)
+        return CFGPU_E_ARG;
+    std::unique_ptr<cfgpu_handle> h(new cfgpu_handle());
+    h->prm = *p;
+    h->KW = (k + 31) / 32;
+    h->W = ((2 * (k - 1) - l + 2) + 31) / 32;
+    h->w = k - l;
+    h->G = G_FIXED;
+    h->g_bits = 14;
+    static const uint64_t salt[4] = {4167021922371662411ull, 7320285940802167691ull,
+                                     14307255741305819987ull, 10859488101230029397ull};
+    h->wy = WyConst{salt[0], salt[1], wymix_host(salt[0], salt[1])};
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= p->device || p->device < 0)
+        return CFGPU_E_HIP;
+    if (hipSetDevice(p->device) != hipSuccess) return CFGPU_E_HIP;
+    if (hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess)
+        h->cus = 256;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return CFGPU_E_HIP;
+    *out = h.release();
+    return CFGPU_OK;
+}
+
+int cfgpu_close(cfgpu_handle* h) {
+    if (!h) return CFGPU_E_ARG;
+    (void)hipSetDevice(h->prm.device);
+    (void)hipStreamSynchronize(h->stream);
+    delete h;
+    return CFGPU_OK;
+}
+
+int cfgpu_last_error(const cfgpu_handle* h, char* buf, size_t n) {
+    if (!h || !buf || !n) return CFGPU_E_ARG;
+    std::snprintf(buf, n, "%s", h->err.c_str());
+    return CFGPU_OK;
+}
+
+int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
+                    const uint64_t* frag_off, uint64_t n_frag, uint32_t flags) {
+    if (!h) return CFGPU_E_ARG;
+    HIPCK(h, hipSetDevice(h->prm.device));
+    const uint32_t k = h->prm.k;
+    if ((n_frag && !packed) || (!frag_off && n_frag) || (flags & ~uint32_t(CFGPU_INPUT_DEVICE)))
+        return fail(h, CFGPU_E_ARG, "partition: bad pointers or flags");
+    const bool dev = flags & CFGPU_INPUT_DEVICE;
+    if (!dev) {
+        if (n_frag == 0 && n_bases != 0) return fail(h, CFGPU_E_ARG, "partition: bases without fragments");
+        if (n_frag) {
+            if (frag_off[0] != 0 || frag_off[n_frag] != n_bases)
+                return fail(h, CFGPU_E_ARG, "partition: frag_off must run from 0 to n_bases");
+            for (uint64_t f = 0; f < n_frag; ++f)
+                if (frag_off[f + 1] < frag_off[f] + k + 1)
+                    return fail(h, CFGPU_E_ARG, "partition: fragment %llu shorter than k+1",
+                                (unsigned long long)f);
+        }
+    }
+    h->partitioned = h->constructed = false;
+    const uint32_t G = h->G;
+    const uint64_t n_words = (n_bases + 31) / 32 + 2;
+    const uint64_t tiles = (n_bases + P_TILE - 1) / P_TILE;
+    const uint64_t* d_packed = packed;
+    const uint64_t* d_frag = frag_off;
+    if (!dev) {
+        HIPCK(h, h->packed.ensure(n_words * 8));
+        HIPCK(h, h->frag.ensure((n_frag + 1) * 8));
+        if (n_words > 2)
+            HIPCK(h, hipMemcpyAsync(h->packed.p, packed, (n_words - 2) * 8, hipMemcpyHostToDevice, h->stream));
+        HIPCK(h, hipMemsetAsync(h->packed.as<uint64_t>() + (n_words - 2), 0, 16, h->stream));
+        HIPCK(h, hipMemcpyAsync(h->frag.p, frag_off, (n_frag + 1) * 8, hipMemcpyHostToDevice, h->stream));
+        d_packed = h->packed.as<uint64_t>();
+        d_frag = h->frag.as<uint64_t>();
+    }
+    HIPCK(h, h->gbin.ensure(n_bases * 4));
+    HIPCK(h, h->tile_lc.ensure(tiles * 8));
+    HIPCK(h, h->bcnt.ensure(G * 4));
+    HIPCK(h, h->boff.ensure((G + 1) * 8));
+    HIPCK(h, h->cursor.ensure(G * 8));
+    HIPCK(h, h->stats.ensure(size_t(G) * 4 * 8));
+    HIPCK(h, h->feat.ensure(size_t(G) * N_FEAT * 4));
+
+    PartitionArgs a{};
+    a.packed = d_packed;
+    a.n_words = n_words;
+    a.n_bases = n_bases;
+    a.frag_off = d_frag;
+    a.n_frag = n_frag;
+    a.k = k;
+    a.l = h->prm.l;
+    a.w = h->w;
+    a.g_mask = G - 1;
+    a.g_bits = h->g_bits;
+    a.wy = h->wy;
+
+    Event e0, e1, e2, e3, e4, e5;
+    HIPCK(h, hipEventRecord(e0.e, h->stream));
+    launch_fill_u32(h->bcnt.as<uint32_t>(), 0u, G, h->stream);
+    if (n_frag) launch_minimizer_g(a, h->gbin.as<uint32_t>(), h->tile_lc.as<uint64_t>(), h->stream);
+    HIPCK(h, hipEventRecord(e1.e, h->stream));
+    if (n_frag)
+        launch_pieces(a, h->gbin.as<uint32_t>(), h->tile_lc.as<uint64_t>(), h->W, false,
+                      h->bcnt.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr, h->stream);
+    launch_bucket_scan(h->bcnt.as<uint32_t>(), G, h->boff.as<uint64_t>(), h->cursor.as<uint64_t>(), h->stream);
+    HIPCK(h, hipEventRecord(e2.e, h->stream));
+    uint64_t n_rec = 0;
+    HIPCK(h, hipMemcpyAsync(&n_rec, h->boff.as<uint64_t>() + G, 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    HIPCK(h, h->att.ensure(n_rec * 4));
+    HIPCK(h, h->labels.ensure(n_rec * h->W * 8));
+    HIPCK(h, h->mbin.ensure(n_rec));
+    HIPCK(h, hipEventRecord(e3.e, h->stream));
+    if (n_frag)
+        launch_pieces(a, h->gbin.as<uint32_t>(), h->tile_lc.as<uint64_t>(), h->W, true, nullptr,
+                      h->cursor.as<uint64_t>(), h->att.as<uint32_t>(), h->labels.as<uint64_t>(),
+                      h->mbin.as<uint8_t>(), h->stream);
+    HIPCK(h, hipEventRecord(e4.e, h->stream));
+    launch_bucket_features(h->boff.as<uint64_t>(), h->att.as<uint32_t>(), h->mbin.as<uint8_t>(), G, k,
+                           h->stats.as<uint64_t>(), h->feat.as<float>(), h->stream);
+    HIPCK(h, hipEventRecord(e5.e, h->stream));
+    HIPCK(h, hipGetLastError());
+    h->h_boff.resize(G + 1);
+    h->h_stats.resize(size_t(G) * 4);
+    HIPCK(h, hipMemcpyAsync(h->h_boff.data(), h->boff.p, (G + 1) * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipMemcpyAsync(h->h_stats.data(), h->stats.p, size_t(G) * 32, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+
+    h->n_bases = n_bases;
+    h->n_frag = n_frag;
+    h->n_records = n_rec;
+    cfgpu_run_stats& rs = h->rs;
+    rs = cfgpu_run_stats{};
+    rs.n_bases = n_bases;
+    rs.n_fragments = n_frag;
+    rs.n_records = n_rec;
+    for (uint32_t b = 0; b < G; ++b) {
+        rs.n_kmers += h->h_stats[4 * b + 1];
+        rs.n_edges += h->h_stats[4 * b + 1] - h->h_stats[4 * b + 0];
+    }
+    rs.t_minimizer_ms = elapsed_ms(e0, e1);
+    rs.t_count_ms = elapsed_ms(e1, e2);
+    rs.t_emit_ms = elapsed_ms(e3, e4);
+    rs.t_features_ms = elapsed_ms(e4, e5);
+    rs.t_partition_ms = elapsed_ms(e0, e5);
+    h->partitioned = true;
+    return CFGPU_OK;
+}
+
+int cfgpu_bucket_stats(cfgpu_handle* h, cfgpu_bucket_stat* out, uint32_t n) {
+    if (!h || !out || n < h->G) return CFGPU_E_ARG;
+    if (!h->partitioned) return fail(h, CFGPU_E_STATE, "bucket_stats before partition");
+    for (uint32_t b = 0; b < h->G; ++b) {
+        cfgpu_bucket_stat& s = out[b];
+        s.n_records = h->h_stats[4 * b + 0];
+        s.n_kmers = h->h_stats[4 * b + 1];
+        s.n_bases = h->h_stats[4 * b + 2];
+        const bool c = h->constructed && s.n_records;
+        s.n_vertices = c ? h->h_nvert[b] : 0;
+        s.capacity = c ? (1ull << h->h_log2[b]) : 0;
+        s.predicted = c ? h->h_pred[b] : 0.f;
+        s.flags = c ? h->h_flags[b] : 0;
+    }
+    return CFGPU_OK;
+}
+
+int cfgpu_records_copy(cfgpu_handle* h, uint32_t* att, uint64_t* labels, uint64_t* bucket_off,
+                       uint64_t cap_records, uint64_t* n_records) {
+    if (!h) return CFGPU_E_ARG;
+    if (!h->partitioned) return fail(h, CFGPU_E_STATE, "records_copy before partition");
+    HIPCK(h, hipSetDevice(h->prm.device));
+    if (n_records) *n_records = h->n_records;
+    if (cap_records < h->n_records) return fail(h, CFGPU_E_ARG, "records_copy: capacity too small");
+    if (att && h->n_records)
+        HIPCK(h, hipMemcpyAsync(att, h->att.p, h->n_records * 4, hipMemcpyDeviceToHost, h->stream));
+    if (labels && h->n_records)
+        HIPCK(h, hipMemcpyAsync(labels, h->labels.p, h->n_records * h->W * 8, hipMemcpyDeviceToHost, h->stream));
+    if (bucket_off) std::memcpy(bucket_off, h->h_boff.data(), (h->G + 1) * 8);
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    return CFGPU_OK;
+}
+
+int cfgpu_construct(cfgpu_handle* h, uint32_t mode) {
+    if (!h) return CFGPU_E_ARG;
+    if (!h->partitioned) return fail(h, CFGPU_E_STATE, "construct before partition");
+    HIPCK(h, hipSetDevice(h->prm.device));
+    h->constructed = false;
+    return construct_impl(h, mode);
+}
+
+int cfgpu_table_view_get(cfgpu_handle* h, uint32_t g, cfgpu_table_view* out) {
+    if (!h || !out || g >= h->G) return CFGPU_E_ARG;
+    if (!h->constructed) return fail(h, CFGPU_E_STATE, "table view before construct");
+    out->entries = reinterpret_cast<const void*>(h->h_tbl_ptr[g]);
+    out->capacity = h->h_tbl_ptr[g] ? (1ull << h->h_log2[g]) : 0;
+    out->entry_bytes = entry_bytes(h->KW);
+    out->key_words = h->KW;
+    out->n_vertices = h->h_nvert[g];
+    return CFGPU_OK;
+}
+
+int cfgpu_table_copy(cfgpu_handle* h, const uint32_t* buckets, uint32_t n_buckets, uint64_t* keys,
+                     uint32_t* e_f, uint8_t* status, uint8_t* dfa, uint32_t* bucket_n, uint64_t cap,
+                     uint64_t* n_out) {
+    if (!h || (n_buckets && !buckets)) return CFGPU_E_ARG;
+    if (!h->constructed) return fail(h, CFGPU_E_STATE, "table copy before construct");
+    HIPCK(h, hipSetDevice(h->prm.device));
+    std::vector<uint64_t> off(n_buckets + 1, 0);
+    std::vector<uint32_t> list;
+    for (uint32_t i = 0; i < n_buckets; ++i) {
+        if (buckets[i] >= h->G) return fail(h, CFGPU_E_ARG, "table copy: bucket %u out of range", buckets[i]);
+        off[i + 1] = off[i] + (h->h_tbl_ptr[buckets[i]] ? h->h_nvert[buckets[i]] : 0);
+    }
+    const uint64_t total = off[n_buckets];
+    if (n_out) *n_out = total;
+    if (total > cap) return fail(h, CFGPU_E_ARG, "table copy: capacity %llu < %llu", (unsigned long long)cap,
+                                 (unsigned long long)total);
+    // only buckets with a table take part in the export grid
+    std::vector<uint32_t> ex_b;
+    std::vector<uint64_t> ex_o;
+    for (uint32_t i = 0; i < n_buckets; ++i)
+        if (h->h_tbl_ptr[buckets[i]]) ex_b.push_back(buckets[i]), ex_o.push_back(off[i]);
+    std::vector<uint64_t> ex_end(ex_o.size());
+    for (size_t i = 0, j = 0; i < n_buckets; ++i)
+        if (h->h_tbl_ptr[buckets[i]]) ex_end[j++] = off[i + 1];
+    ex_o.push_back(0);  // room for the end offset of the last bucket
+    const uint32_t KW = h->KW;
+    HIPCK(h, h->ex_keys.ensure(total * KW * 8));
+    HIPCK(h, h->ex_ef.ensure(total * 4));
+    HIPCK(h, h->ex_st.ensure(total));
+    HIPCK(h, h->ex_dfa.ensure(total));
+    HIPCK(h, h->ex_off.ensure((ex_b.size() + 1) * 8));
+    HIPCK(h, h->ex_cursor.ensure(ex_b.size() * 4));
+    HIPCK(h, h->ex_list.ensure(ex_b.size() * 4));
+    if (!ex_b.empty()) {
+        // out_off[i] = start of bucket i, out_off[i+1] = its end: buckets are
+        // exported back to back, so ends are the next starts except the last
+        ex_o.back() = ex_end.back();
+        for (size_t i = 0; i + 1 < ex_b.size(); ++i)
+            if (ex_o[i + 1] != ex_end[i]) return fail(h, CFGPU_E_STATE, "table copy: offsets");
+        HIPCK(h, hipMemcpyAsync(h->ex_off.p, ex_o.data(), ex_o.size() * 8, hipMemcpyHostToDevice, h->stream));
+        HIPCK(h, hipMemcpyAsync(h->ex_list.p, ex_b.data(), ex_b.size() * 4, hipMemcpyHostToDevice, h->stream));
+        launch_fill_u32(h->ex_cursor.as<uint32_t>(), 0u, ex_b.size(), h->stream);
+        launch_export(h->d_tbl_ptr.as<uint64_t>(), h->d_log2.as<uint32_t>(), KW, h->ex_list.as<uint32_t>(),
+                      uint32_t(ex_b.size()), h->ex_off.as<uint64_t>(), h->ex_cursor.as<uint32_t>(),
+                      h->ex_keys.as<uint64_t>(), h->ex_ef.as<uint32_t>(), h->ex_st.as<uint8_t>(),
+                      h->ex_dfa.as<uint8_t>(), h->prm.cutoff, h->stream);
+        HIPCK(h, hipGetLastError());
+    }
+    std::vector<uint32_t> cnt(ex_b.size());
+    if (!ex_b.empty())
+        HIPCK(h, hipMemcpyAsync(cnt.data(), h->ex_cursor.p, cnt.size() * 4, hipMemcpyDeviceToHost, h->stream));
+    if (total) {
+        if (keys) HIPCK(h, hipMemcpyAsync(keys, h->ex_keys.p, total * KW * 8, hipMemcpyDeviceToHost, h->stream));
+        if (e_f) HIPCK(h, hipMemcpyAsync(e_f, h->ex_ef.p, total * 4, hipMemcpyDeviceToHost, h->stream));
+        if (status) HIPCK(h, hipMemcpyAsync(status, h->ex_st.p, total, hipMemcpyDeviceToHost, h->stream));
+        if (dfa) HIPCK(h, hipMemcpyAsync(dfa, h->ex_dfa.p, total, hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    size_t j = 0;
+    for (uint32_t i = 0; i < n_buckets; ++i) {
+        uint32_t c = 0;
+        if (h->h_tbl_ptr[buckets[i]]) c = cnt[j++];
+        if (c != off[i + 1] - off[i])
+            return fail(h, CFGPU_E_STATE, "table copy: bucket %u exported %u of %llu entries", buckets[i], c,
+                        (unsigned long long)(off[i + 1] - off[i]));
+        if (bucket_n) bucket_n[i] = c;
+    }
+    return CFGPU_OK;
+}
+
+int cfgpu_release(cfgpu_handle* h, uint32_t g) {
+    if (!h || g >= h->G) return CFGPU_E_ARG;
+    if (!h->constructed) return fail(h, CFGPU_E_STATE, "release before construct");
+    // The table memory belongs to a pool shared with other buckets and is
+    // reclaimed at the next construct; the view of g stops being valid.
+    h->h_tbl_ptr[g] = 0;
+    h->h_flags[g] &= ~uint32_t(CFGPU_BUCKET_BUILT);
+    return CFGPU_OK;
+}
+
+int cfgpu_run_stats_get(cfgpu_handle* h, cfgpu_run_stats* out) {
+    if (!h || !out) return CFGPU_E_ARG;
+    *out = h->rs;
+    return CFGPU_OK;
+}
+
+int cfgpu_tuning_get(cfgpu_handle* h, cfgpu_tuning* out) {
+    if (!h || !out) return CFGPU_E_ARG;
+    *out = h->tune;
+    return CFGPU_OK;
+}
+
+int cfgpu_tuning_set(cfgpu_handle* h, const cfgpu_tuning* in) {
+    if (!h || !in) return CFGPU_E_ARG;
+    if (!(in->load_factor >= 0.2f && in->load_factor <= 0.85f) ||
+        (in->workgroup != 256 && in->workgroup != 512) || in->chunk_records < 64 ||
+        in->chunk_records > (1u << 24) || in->wgs_per_cu < 1 || in->wgs_per_cu > 32 ||
+        !(in->margin >= 0.01f && in->margin <= 4.0f) || in->use_graph > 1)
+        return fail(h, CFGPU_E_ARG, "tuning out of range");
+    h->tune = *in;
+    return CFGPU_OK;
+}
+
+int cfgpu_autotune(cfgpu_handle* h, uint32_t mode, cfgpu_tune_trial* trials_out, uint32_t n_trials,
+                   uint32_t* n_written) {
+    if (!h) return CFGPU_E_ARG;
+    if (!h->partitioned) return fail(h, CFGPU_E_STATE, "autotune before partition");
+    const cfgpu_tuning base = h->tune;
+    std::vector<cfgpu_tuning> grid;
+    for (float lf : {0.5f, 0.65f})
+        for (uint32_t wg : {256u, 512u})
+            for (uint32_t ch : {1024u, 4096u})
+                for (uint32_t wpc : {4u, 8u}) grid.push_back({lf, wg, ch, wpc, base.margin, base.use_graph});
+    cfgpu_tuning best = base;
+    double best_ms = 1e300;
+    uint32_t w = 0;
+    for (const cfgpu_tuning& t : grid) {
+        h->tune = t;
+        const int r = construct_impl(h, mode);
+        if (r != CFGPU_OK) {
+            h->tune = base;
+            return r;
+        }
+        const double ms = h->rs.t_construct_main_ms + h->rs.t_fallback_ms;
+        if (trials_out && w < n_trials) trials_out[w++] = {t, ms};
+        if (ms < best_ms) best_ms = ms, best = t;
+    }
+    if (n_written) *n_written = w;
+    h->tune = best;
+    return construct_impl(h, mode);
+}
+
+}  // extern "C"

@@ -1,0 +1,393 @@
+// Subgraph construct kernels: per weak super k-mer, every k-mer instance is
+// inserted into its subgraph's open-addressing vertex table in HBM and its
+// State_Config is updated (src/Subgraph.cpp:44-108, include/Subgraph.hpp:546-560,
+// include/State_Config.hpp:207-268).
+//
+// Table entry (stride 8*KW + 8 bytes): KW key words (Kmer<k> layout), a u32
+// Edge_Frequency word (8 saturating 4-bit counters) and a u32 status word
+// (Vertex_Status bits: 2 = front-discontinuous, 4 = back-discontinuous).
+//
+// Every 4-bit counter slot (vertex, side, base) is fed by exactly one canonical
+// (k+1)-mer, so the counters are the per-subgraph (k+1)-mer counts; counting
+// saturates at 15 exactly like Edge_Frequency::add_edge, and saturating adds
+// commute, so the atomic update order cannot change any result.
+//
+// Key claim protocol.  The top key word of a canonical k-mer never uses bit 63
+// (k odd => at most 31 bases in the top word), so ~0 marks an empty slot and
+// `top | 1<<63` marks a slot whose lower key words are still being written.
+// KW == 1: one 64-bit CAS.  KW > 1: CAS the top word EMPTY -> top|BUSY, store
+// the lower words, release fence, store top.  A prober that meets BUSY with a
+// matching top word re-reads the same slot on its next loop iteration; the
+// claimer finishes its stores within its own iteration, so lanes of one
+// wavefront never wait on each other inside an iteration.
+#include "cf_kernels.hpp"
+
+#include <algorithm>
+
+namespace cfgpu {
+namespace {
+
+constexpr uint64_t EMPTY = ~0ull;
+constexpr uint64_t BUSY = 1ull << 63;
+constexpr uint32_t MAX_SPINS = 1u << 22;
+constexpr int W_MAX = 8;  // label words: ceil((2k - l)/32) <= 8 for k <= 127
+
+template <int KW>
+__device__ __forceinline__ uint64_t* slot_ptr(uint8_t* tbl, uint64_t s) {
+    return reinterpret_cast<uint64_t*>(tbl + s * uint64_t(8 * KW + 8));
+}
+
+__device__ __forceinline__ uint64_t ld_relaxed(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ld_relaxed32(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Returns the slot holding `key` (inserting it if absent), or -1 when the table
+// has no room (the caller flags the bucket as overflowed).
+template <int KW>
+__device__ int64_t find_or_insert(uint8_t* tbl, uint64_t mask, const Kmer<KW>& key,
+                                  bool& inserted) {
+    uint64_t s = km_hash(key) & mask;
+    inserted = false;
+    if constexpr (KW == 1) {
+        for (uint64_t n = 0; n <= mask; ++n, s = (s + 1) & mask) {
+            uint64_t* kp = slot_ptr<1>(tbl, s);
+            const uint64_t cur = ld_relaxed(kp);
+            if (cur == key.w[0]) return int64_t(s);
+            if (cur == EMPTY) {
+                const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(kp), EMPTY,
+                                               key.w[0]);
+                if (old == EMPTY) {
+                    inserted = true;
+                    return int64_t(s);
+                }
+                if (old == key.w[0]) return int64_t(s);
+            }
+        }
+        return -1;
+    } else {
+        const uint64_t top = key.w[KW - 1];
+        uint32_t spins = 0;
+        uint64_t n = 0;
+        while (n <= mask) {
+            uint64_t* kp = slot_ptr<KW>(tbl, s);
+            uint64_t cur = ld_relaxed(kp + KW - 1);
+            if (cur == EMPTY) {
+                const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(kp + KW - 1),
+                                               EMPTY, top | BUSY);
+                if (old == EMPTY) {
+#pragma unroll
+                    for (int i = 0; i < KW - 1; ++i)
+                        __hip_atomic_store(kp + i, key.w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __threadfence();
+                    __hip_atomic_store(kp + KW - 1, top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    inserted = true;
+                    return int64_t(s);
+                }
+                cur = old;
+            }
+            if ((cur & ~BUSY) == top) {
+                if (cur & BUSY) {
+                    if (++spins > MAX_SPINS) return -1;
+                    continue;  // the claimer publishes within its own iteration
+                }
+                bool eq = true;
+#pragma unroll
+                for (int i = 0; i < KW - 1; ++i) eq &= ld_relaxed(kp + i) == key.w[i];
+                if (eq) return int64_t(s);
+            }
+            ++n;
+            s = (s + 1) & mask;
+        }
+        return -1;
+    }
+}
+
+// Exclusive sum-scan of one u32 per thread across a WG-thread workgroup;
+// returns the exclusive prefix and writes the total to *total.
+template <int WG>
+__device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* s_wave, uint32_t* total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(x, d);
+        if (lane >= d) x += u;
+    }
+    if (lane == 63) s_wave[wid] = x;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int j = 0; j < WG / 64; ++j) {
+        const uint32_t t = s_wave[j];
+        pre += j < wid ? t : 0;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + x - v;
+}
+
+template <int KW, int WG>
+__device__ void construct_item(const ConstructArgs& a, uint32_t item, uint32_t* s_att,
+                               uint64_t* s_lab, uint32_t* s_kofs, uint32_t* s_wave,
+                               uint32_t* s_flag) {
+    const int tid = threadIdx.x;
+    const uint32_t b = a.item_bucket[item];
+    // Skip buckets already known to overflow (they are rebuilt with a larger
+    // table).  One thread reads the flag so the whole workgroup agrees.
+    if (tid == 0) *s_flag = ld_relaxed32(&a.overflow[b]);
+    __syncthreads();
+    const uint32_t skip = *s_flag;
+    __syncthreads();
+    if (skip) return;
+    const uint32_t first = a.item_first[item], cnt = a.item_count[item];
+    const uint32_t W = a.W, k = a.k;
+    const uint64_t rbase = a.bucket_off[b] + first;
+    uint8_t* tbl = reinterpret_cast<uint8_t*>(a.tbl_ptr[b]);
+    const uint64_t mask = (1ull << a.tbl_log2[b]) - 1;
+    const uint32_t limit = a.tbl_limit[b];
+    const int lane = tid & 63;
+
+    for (uint32_t r0 = 0; r0 < cnt; r0 += WG) {
+        const uint32_t ns = min(uint32_t(WG), cnt - r0);
+        uint32_t kc = 0;
+        if (uint32_t(tid) < ns) {
+            const uint32_t at = a.att[rbase + r0 + tid];
+            s_att[tid] = at;
+            kc = (at & 0xFFu) - (k - 1);
+        }
+        for (uint32_t i = tid; i < ns * W; i += WG) s_lab[i] = a.labels[(rbase + r0) * W + i];
+        uint32_t total;
+        const uint32_t ex = block_excl_sum<WG>(kc, s_wave, &total);
+        s_kofs[tid] = ex;
+        if (tid == WG - 1) s_kofs[WG] = ex + kc;
+        __syncthreads();
+
+        for (uint32_t j = tid; j < total; j += WG) {
+            // record r holding k-mer instance j: largest r with s_kofs[r] <= j
+            int lo = 0, hi = int(ns) - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_kofs[mid] <= j) lo = mid;
+                else hi = mid - 1;
+            }
+            const uint32_t at = s_att[lo];
+            const uint32_t i = j - s_kofs[lo];
+            const uint32_t len = at & 0xFFu;
+            const uint64_t* L = s_lab + lo * W;
+
+            const Kmer<KW> v = km_from_label<KW>(L, W, i, k);
+            const Kmer<KW> vb = km_rc<KW>(v, k);
+            const bool is_can = km_less<KW>(v, vb);  // Directed_Vertex::in_canonical_form
+            const uint8_t pred = i > 0 ? label_base(L, W, i - 1) : BASE_E;
+            const uint8_t succ = i + k < len ? label_base(L, W, i + k) : BASE_E;
+            uint8_t front = is_can ? pred : comp_base(succ);
+            uint8_t back = is_can ? succ : comp_base(pred);
+            if (i > 0) {
+                // Self-loop rule (src/Subgraph.cpp:82-83): the previous k-mer
+                // has the same canonical form iff it equals v or rc(v).
+                const Kmer<KW> u = km_prepend<KW>(v, pred, k);
+                if (km_eq<KW>(u, v) || km_eq<KW>(u, vb)) (is_can ? front : back) = BASE_E;
+            }
+            uint32_t disc = 0;  // src/Subgraph.cpp:88-92 with entrance/exit sides
+            if (i == 0 && ((at >> 8) & 1u)) disc |= is_can ? 2u : 4u;
+            if (i + k == len && ((at >> 9) & 1u)) disc |= is_can ? 4u : 2u;
+
+            bool inserted = false;
+            const int64_t slot = find_or_insert<KW>(tbl, mask, is_can ? v : vb, inserted);
+            if (slot < 0) {
+                atomicOr(&a.overflow[b], 1u);
+            } else {
+                uint32_t* ef = reinterpret_cast<uint32_t*>(slot_ptr<KW>(tbl, uint64_t(slot)) + KW);
+                uint32_t old = ld_relaxed32(ef);
+                while (true) {
+                    const uint32_t nw = ef_add(old, front, back);
+                    if (nw == old) break;
+                    const uint32_t prev = atomicCAS(ef, old, nw);
+                    if (prev == old) break;
+                    old = prev;
+                }
+                if (disc) atomicOr(ef + 1, disc);
+            }
+            // distinct-key accounting: one atomic per wavefront
+            const unsigned long long m = __ballot(inserted);
+            if (m) {
+                const int leader = __ffsll((long long)m) - 1;
+                if (lane == leader) {
+                    const uint32_t add = uint32_t(__popcll(m));
+                    const uint32_t before = atomicAdd(&a.n_vert[b], add);
+                    if (before + add > limit) atomicOr(&a.overflow[b], 1u);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int KW, int WG>
+__global__ __launch_bounds__(WG) void k_construct(ConstructArgs a, uint32_t item0, int persistent) {
+    __shared__ uint32_t s_att[WG];
+    __shared__ uint64_t s_lab[WG * W_MAX];
+    __shared__ uint32_t s_kofs[WG + 1];
+    __shared__ uint32_t s_wave[WG / 64];
+    __shared__ uint32_t s_item, s_flag;
+    if (!persistent) {
+        construct_item<KW, WG>(a, item0 + blockIdx.x, s_att, s_lab, s_kofs, s_wave, &s_flag);
+        return;
+    }
+    while (true) {
+        if (threadIdx.x == 0) s_item = atomicAdd(a.queue_head, 1u);
+        __syncthreads();
+        const uint32_t it = s_item;
+        __syncthreads();
+        if (it >= a.n_items) break;
+        construct_item<KW, WG>(a, it, s_att, s_lab, s_kofs, s_wave, &s_flag);
+    }
+}
+
+template <int KW>
+__global__ __launch_bounds__(256) void k_clear(uint8_t* tables, uint64_t n) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint64_t* p = slot_ptr<KW>(tables, i);
+#pragma unroll
+        for (int j = 0; j < KW; ++j) p[j] = EMPTY;
+        p[KW] = 0;  // e_f = 0, status = 0
+    }
+}
+
+__global__ __launch_bounds__(256) void k_zero_buckets(const uint32_t* list, uint32_t n,
+                                                      uint32_t* n_vert, uint32_t* overflow) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        n_vert[list[i]] = 0;
+        overflow[list[i]] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// Edge_Frequency::edge_at (include/State_Config.hpp:232-252).
+__device__ __forceinline__ uint32_t edge_at(uint32_t e_f, unsigned off, uint32_t f_th) {
+    uint32_t c = 0, which = 0;
+#pragma unroll
+    for (uint32_t bse = 0; bse < 4; ++bse)
+        if (((e_f >> (off + 4 * bse)) & 0xFu) >= f_th) ++c, which = bse;
+    return c == 0 ? BASE_E : (c == 1 ? which : BASE_N);
+}
+
+template <int KW>
+__global__ __launch_bounds__(256) void k_export(const uint64_t* tbl_ptr,
+                                                const uint32_t* tbl_log2, const uint32_t* buckets,
+                                                const uint64_t* out_off, uint32_t* out_cursor,
+                                                uint64_t* keys_out, uint32_t* ef_out,
+                                                uint8_t* st_out, uint8_t* dfa_out,
+                                                uint32_t cutoff) {
+    const uint32_t bi = blockIdx.y;
+    const uint32_t b = buckets[bi];
+    const uint64_t cap = 1ull << tbl_log2[b];
+    const uint8_t* tbl = reinterpret_cast<const uint8_t*>(tbl_ptr[b]);
+    const int lane = threadIdx.x & 63;
+    for (uint64_t s = uint64_t(blockIdx.x) * 256 + threadIdx.x; s - threadIdx.x < cap;
+         s += uint64_t(gridDim.x) * 256) {
+        const bool in = s < cap;
+        const uint64_t* p = in ? reinterpret_cast<const uint64_t*>(tbl + s * uint64_t(8 * KW + 8)) : nullptr;
+        const bool occ = in && p[KW - 1] != EMPTY;
+        const unsigned long long m = __ballot(occ);
+        if (!m) continue;
+        uint32_t base = 0;
+        const int leader = __ffsll((long long)m) - 1;
+        if (lane == leader) base = atomicAdd(&out_cursor[bi], uint32_t(__popcll(m)));
+        base = __shfl(base, leader);
+        // out_off has n_buckets + 1 entries: entries past a bucket's expected
+        // count are counted but not written (the host reports the mismatch).
+        const uint32_t rank = uint32_t(__popcll(m & ((1ull << lane) - 1)));
+        const uint64_t o = out_off[bi] + base + rank;
+        if (occ && o < out_off[bi + 1]) {
+#pragma unroll
+            for (int j = 0; j < KW; ++j) keys_out[o * KW + j] = p[j];
+            const uint32_t e_f = uint32_t(p[KW]), st = uint32_t(p[KW] >> 32);
+            ef_out[o] = e_f;
+            st_out[o] = uint8_t(st);
+            dfa_out[o] = uint8_t(edge_at(e_f, 0, cutoff) | (edge_at(e_f, 16, cutoff) << 3) |
+                                 ((st & 2u) << 5) | ((st & 4u) << 5));
+        }
+    }
+}
+
+}  // namespace
+
+void launch_clear_tables(uint8_t* tables, uint64_t n_slots, uint32_t KW, hipStream_t s) {
+    if (!n_slots) return;
+    const uint32_t grid = uint32_t(std::min<uint64_t>((n_slots + 255) / 256, 65536));
+    switch (KW) {
+        case 1: hipLaunchKernelGGL(k_clear<1>, dim3(grid), dim3(256), 0, s, tables, n_slots); break;
+        case 2: hipLaunchKernelGGL(k_clear<2>, dim3(grid), dim3(256), 0, s, tables, n_slots); break;
+        case 3: hipLaunchKernelGGL(k_clear<3>, dim3(grid), dim3(256), 0, s, tables, n_slots); break;
+        default: hipLaunchKernelGGL(k_clear<4>, dim3(grid), dim3(256), 0, s, tables, n_slots); break;
+    }
+}
+
+template <int KW>
+static void launch_kw(const ConstructArgs& a, uint32_t item0, uint32_t grid, uint32_t wg,
+                      int persistent, hipStream_t s) {
+    if (wg == 512)
+        hipLaunchKernelGGL((k_construct<KW, 512>), dim3(grid), dim3(512), 0, s, a, item0, persistent);
+    else
+        hipLaunchKernelGGL((k_construct<KW, 256>), dim3(grid), dim3(256), 0, s, a, item0, persistent);
+}
+
+static void launch_any(const ConstructArgs& a, uint32_t KW, uint32_t item0, uint32_t grid,
+                       uint32_t wg, int persistent, hipStream_t s) {
+    if (!grid) return;
+    switch (KW) {
+        case 1: launch_kw<1>(a, item0, grid, wg, persistent, s); break;
+        case 2: launch_kw<2>(a, item0, grid, wg, persistent, s); break;
+        case 3: launch_kw<3>(a, item0, grid, wg, persistent, s); break;
+        default: launch_kw<4>(a, item0, grid, wg, persistent, s); break;
+    }
+}
+
+void launch_construct(const ConstructArgs& a, uint32_t KW, uint32_t item0, uint32_t n, uint32_t wg,
+                      hipStream_t s) {
+    launch_any(a, KW, item0, n, wg, 0, s);
+}
+
+void launch_construct_persistent(const ConstructArgs& a, uint32_t KW, uint32_t grid, uint32_t wg,
+                                 hipStream_t s) {
+    launch_any(a, KW, 0, grid, wg, 1, s);
+}
+
+void launch_zero_buckets(const uint32_t* list, uint32_t n, uint32_t* n_vert, uint32_t* overflow,
+                         hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_zero_buckets, dim3((n + 255) / 256), dim3(256), 0, s, list, n, n_vert, overflow);
+}
+
+void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
+    if (!n) return;
+    const uint32_t grid = uint32_t(std::min<uint64_t>((n + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(256), 0, s, p, v, n);
+}
+
+void launch_export(const uint64_t* tbl_ptr, const uint32_t* tbl_log2,
+                   uint32_t KW, const uint32_t* buckets, uint32_t n_buckets, const uint64_t* out_off,
+                   uint32_t* out_cursor, uint64_t* keys_out, uint32_t* ef_out, uint8_t* st_out,
+                   uint8_t* dfa_out, uint32_t cutoff, hipStream_t s) {
+    if (!n_buckets) return;
+    const dim3 grid(64, n_buckets);
+    switch (KW) {
+        case 1: hipLaunchKernelGGL(k_export<1>, grid, dim3(256), 0, s, tbl_ptr, tbl_log2, buckets, out_off, out_cursor, keys_out, ef_out, st_out, dfa_out, cutoff); break;
+        case 2: hipLaunchKernelGGL(k_export<2>, grid, dim3(256), 0, s, tbl_ptr, tbl_log2, buckets, out_off, out_cursor, keys_out, ef_out, st_out, dfa_out, cutoff); break;
+        case 3: hipLaunchKernelGGL(k_export<3>, grid, dim3(256), 0, s, tbl_ptr, tbl_log2, buckets, out_off, out_cursor, keys_out, ef_out, st_out, dfa_out, cutoff); break;
+        default: hipLaunchKernelGGL(k_export<4>, grid, dim3(256), 0, s, tbl_ptr, tbl_log2, buckets, out_off, out_cursor, keys_out, ef_out, st_out, dfa_out, cutoff); break;
+    }
+}
+
+}  // namespace cfgpu

@@ -1,0 +1,356 @@
+// Partition kernels: (k-1)-mer minimizer hashing, weak super k-mer split and
+// bucket scatter (src/Graph_Partitioner.cpp:424-614 restated data-parallel).
+//
+// Work is tiled over the concatenated fragment stream: workgroup t owns the
+// (k-1)-mer start positions [t*P_TILE, (t+1)*P_TILE).  A position p is a
+// (k-1)-mer iff [p, p+k-2] lies inside one fragment.  Pass 1 writes, per
+// position, g = H & (G-1) (and 4 more hash bits for the bucket features) where
+// H is the minimum canonical l-mer hash over the (k-1)-mer's w = k-l l-mers.
+// Pieces are then cut at every g change and every w (k-1)-mers from a run's
+// start, which is the length cap `2(k-1)-l` of the reference's sequential loop
+// (src/Graph_Partitioner.cpp:30, 541).
+#include "cf_kernels.hpp"
+
+namespace cfgpu {
+namespace {
+
+constexpr int WIN_MAX = 128;                      // w = k - l <= 126
+constexpr int MAXF = P_TILE / 4 + 8;              // fragments overlapping a tile (len >= 4)
+constexpr int NWORDS = (P_TILE + 2 * WIN_MAX) / 32 + 4;
+constexpr int S_MAX = P_TILE + WIN_MAX;           // l-mer hashes per tile
+constexpr int PER_T = (S_MAX + P_WG - 1) / P_WG;
+
+__global__ __launch_bounds__(P_WG) void k_minimizer_g(PartitionArgs a, uint32_t* __restrict__ gbin,
+                                                      uint64_t* __restrict__ tile_lc) {
+    __shared__ uint64_t s_words[NWORDS];
+    __shared__ uint64_t s_m[S_MAX];
+    __shared__ uint64_t s_frag[MAXF + 1];
+    __shared__ uint32_t s_g[P_TILE + 1];
+    __shared__ unsigned long long s_lc;
+
+    const int tid = threadIdx.x;
+    const int64_t t0 = int64_t(blockIdx.x) * P_TILE;
+    const int64_t q0 = t0 - 1;  // position held at LDS index 0
+    const int64_t N = int64_t(a.n_bases);
+    const unsigned k = a.k, l = a.l, w = a.w;
+    const int S = P_TILE + int(w);  // l-mers q0 .. q0 + P_TILE + w - 1
+
+    // Fragment boundaries overlapping the tile.
+    if (tid == 0) {
+        const uint64_t key = uint64_t(q0 < 0 ? 0 : q0);
+        uint64_t lo = 0, hi = a.n_frag;  // largest f with frag_off[f] <= key
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi + 1) >> 1;
+            if (a.frag_off[mid] <= key) lo = mid;
+            else hi = mid - 1;
+        }
+        s_m[0] = lo;
+        s_lc = 0;
+    }
+    __syncthreads();
+    const uint64_t f0 = s_m[0];
+    __syncthreads();
+    for (int i = tid; i <= MAXF; i += P_WG) {
+        const uint64_t f = f0 + uint64_t(i);
+        s_frag[i] = f <= a.n_frag ? a.frag_off[f] : a.n_bases;
+    }
+    const int64_t wlo = (q0 < 0 ? 0 : q0) >> 5;
+    for (int i = tid; i < NWORDS; i += P_WG) {
+        const uint64_t wi = uint64_t(wlo) + uint64_t(i);
+        s_words[i] = wi < a.n_words ? a.packed[wi] : 0;
+    }
+    __syncthreads();
+
+    // Canonical l-mer hashes (include/Minimizer_Iterator.hpp:291-296).
+    for (int i = tid; i < S; i += P_WG) {
+        const int64_t q = q0 + i;
+        uint64_t h = ~0ull;
+        if (q >= 0 && q + int64_t(l) <= N) {
+            const uint64_t x = stream_get64(s_words, uint64_t(q - (wlo << 5))) >> (64 - 2 * l);
+            const uint64_t h1 = lmer_hash(x, a.wy), h2 = lmer_hash(lmer_rc(x, l), a.wy);
+            h = h1 < h2 ? h1 : h2;
+        }
+        s_m[i] = h;
+    }
+    __syncthreads();
+
+    // Window minimum over w consecutive l-mers by doubling: after the loop
+    // s_m[i] = min(h[i .. i+span-1]) with span the largest power of two <= w.
+    unsigned span = 1;
+    for (; 2 * span <= w; span *= 2) {
+        uint64_t tmp[PER_T];
+#pragma unroll
+        for (int c = 0; c < PER_T; ++c) {
+            const int i = tid + c * P_WG;
+            if (i < S) {
+                uint64_t v = s_m[i];
+                if (i + int(span) < S) {
+                    const uint64_t u = s_m[i + span];
+                    v = u < v ? u : v;
+                }
+                tmp[c] = v;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < PER_T; ++c) {
+            const int i = tid + c * P_WG;
+            if (i < S) s_m[i] = tmp[c];
+        }
+        __syncthreads();
+    }
+
+    for (int i = tid; i <= P_TILE; i += P_WG) {
+        const int64_t p = q0 + i;
+        uint32_t gv = G_INVALID;
+        if (p >= 0 && p < N) {
+            int lo = 0, hi = MAXF - 1;  // largest j with s_frag[j] <= p
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (int64_t(s_frag[mid]) <= p) lo = mid;
+                else hi = mid - 1;
+            }
+            if (p + int64_t(k) - 1 <= int64_t(s_frag[lo + 1])) {
+                const uint64_t m0 = s_m[i], m1 = s_m[i + w - span];
+                const uint64_t H = m0 < m1 ? m0 : m1;
+                gv = uint32_t(H & a.g_mask) | (uint32_t((H >> a.g_bits) & 15u) << 16);
+            }
+        }
+        s_g[i] = gv;
+        if (i >= 1 && p < N) gbin[p] = gv;
+    }
+    __syncthreads();
+
+    unsigned long long my = 0;
+    for (int i = tid + 1; i <= P_TILE; i += P_WG) {
+        const uint32_t cur = s_g[i];
+        if (cur == G_INVALID) continue;
+        const uint32_t prv = s_g[i - 1];
+        if (prv == G_INVALID || ((prv ^ cur) & 0xFFFFu)) my = (unsigned long long)(q0 + i + 1);
+    }
+    if (my) atomicMax(&s_lc, my);
+    __syncthreads();
+    if (tid == 0) tile_lc[blockIdx.x] = s_lc;
+}
+
+// Inclusive max-scan of one u32 per thread across the workgroup.
+__device__ __forceinline__ uint32_t block_incl_max(uint32_t v, uint32_t* s_wave) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(v, d);
+        if (lane >= d) v = u > v ? u : v;
+    }
+    if (lane == 63) s_wave[wid] = v;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (int j = 0; j < wid; ++j) pre = s_wave[j] > pre ? s_wave[j] : pre;
+    __syncthreads();
+    return v > pre ? v : pre;
+}
+
+template <bool EMIT>
+__global__ __launch_bounds__(P_WG) void k_pieces(PartitionArgs a, const uint32_t* __restrict__ gbin,
+                                                 const uint64_t* __restrict__ tile_lc, uint32_t W,
+                                                 uint32_t* __restrict__ bucket_cnt,
+                                                 unsigned long long* __restrict__ cursor,
+                                                 uint32_t* __restrict__ att,
+                                                 uint64_t* __restrict__ labels,
+                                                 uint8_t* __restrict__ mbin) {
+    constexpr int PPT = P_TILE / P_WG;  // positions per thread
+    __shared__ uint32_t s_g[P_TILE + WIN_MAX + 2];
+    __shared__ unsigned long long s_carry;
+    __shared__ uint32_t s_wave[P_WG / 64];
+
+    const int tid = threadIdx.x;
+    const int64_t t0 = int64_t(blockIdx.x) * P_TILE;
+    const int64_t q0 = t0 - 1;
+    const int64_t N = int64_t(a.n_bases);
+    const unsigned k = a.k, w = a.w;
+
+    for (int i = tid; i < P_TILE + int(w) + 2; i += P_WG) {
+        const int64_t p = q0 + i;
+        s_g[i] = (p >= 0 && p < N) ? gbin[p] : G_INVALID;
+    }
+    // Carry-in: last g-change position (+1) before this tile.  A wave looks
+    // back 64 tiles at a time; only runs longer than a tile need more than one
+    // step.
+    if (tid < 64) {
+        int64_t j = int64_t(blockIdx.x) - 1 - tid;
+        unsigned long long found = 0;
+        while (true) {
+            const unsigned long long v = j >= 0 ? tile_lc[j] : 0ull;
+            const unsigned long long m = __ballot(v != 0 || j < 0);
+            if (m) {
+                const int first = __ffsll((long long)m) - 1;
+                found = __shfl(v, first);
+                break;
+            }
+            j -= 64;
+        }
+        if (tid == 0) s_carry = found;
+    }
+    __syncthreads();
+
+    // Change flags for my PPT contiguous positions.
+    const int base_idx = 1 + tid * PPT;
+    uint32_t chg = 0, last_chg = 0;
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+        const int idx = base_idx + j;
+        const uint32_t cur = s_g[idx];
+        if (cur == G_INVALID) continue;
+        const uint32_t prv = s_g[idx - 1];
+        if (prv == G_INVALID || ((prv ^ cur) & 0xFFFFu)) {
+            chg |= 1u << j;
+            last_chg = uint32_t(idx);
+        }
+    }
+    const uint32_t incl = block_incl_max(last_chg, s_wave);
+    const uint32_t excl_t = __shfl_up(incl, 1);
+    uint32_t excl = (threadIdx.x & 63) ? excl_t : 0;
+    if ((threadIdx.x & 63) == 0 && tid > 0) {
+        // first lane of a wave: the exclusive value is the max over earlier waves
+        for (int j = 0; j < (tid >> 6); ++j) excl = s_wave[j] > excl ? s_wave[j] : excl;
+    }
+    int64_t rs = excl ? q0 + int64_t(excl) : (s_carry ? int64_t(s_carry) - 1 : -1);
+
+#pragma unroll 1
+    for (int j = 0; j < PPT; ++j) {
+        const int idx = base_idx + j;
+        const int64_t p = q0 + idx;
+        const uint32_t cur = s_g[idx];
+        if (cur == G_INVALID) continue;
+        const bool c = (chg >> j) & 1u;
+        if (c || rs < 0) rs = p;
+        if (!c && ((p - rs) % int64_t(w)) != 0) continue;
+        // Piece [p, e): (k-1)-mers with this g, at most w of them.
+        const uint32_t g = cur & 0xFFFFu;
+        int e = idx + 1;
+        const int lim = idx + int(w);
+        while (e < lim) {
+            const uint32_t v = s_g[e];
+            if (v == G_INVALID || (v & 0xFFFFu) != g) break;
+            ++e;
+        }
+        const uint32_t nv = s_g[e], pv = s_g[idx - 1];
+        const uint32_t rj = nv != G_INVALID, lj = pv != G_INVALID;
+        const uint32_t l_disc = lj && ((pv & 0xFFFFu) != g);
+        const uint32_t r_disc = rj && ((nv & 0xFFFFu) != g);
+        const uint32_t len = lj + uint32_t(e - idx) + (k - 2) + rj;
+        if constexpr (!EMIT) {
+            atomicAdd(&bucket_cnt[g], 1u);
+        } else {
+            const unsigned long long slot = atomicAdd(&cursor[g], 1ull);
+            att[slot] = len | (l_disc << 8) | (r_disc << 9) | (g << 16);
+            mbin[slot] = uint8_t((cur >> 16) & 15u);
+            const uint64_t start = uint64_t(p) - lj;
+            for (uint32_t wj = 0; wj < W; ++wj) {
+                uint64_t x = 0;
+                if (32 * wj < len) {
+                    x = stream_get64(a.packed, start + 32ull * wj);
+                    const uint32_t nb = len - 32 * wj;
+                    if (nb < 32) x &= ~0ull << (64 - 2 * nb);
+                }
+                labels[slot * W + (W - 1 - wj)] = x;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_bucket_scan(const uint32_t* __restrict__ cnt, uint32_t G,
+                                                      uint64_t* __restrict__ off,
+                                                      uint64_t* __restrict__ cursor) {
+    __shared__ uint64_t s[1024];
+    const int tid = threadIdx.x;
+    const uint32_t per = (G + 1023) / 1024;
+    const uint32_t b0 = tid * per, b1 = min(G, b0 + per);
+    uint64_t sum = 0;
+    for (uint32_t i = b0; i < b1; ++i) sum += cnt[i];
+    s[tid] = sum;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const uint64_t v = tid >= d ? s[tid - d] : 0;
+        __syncthreads();
+        s[tid] += v;
+        __syncthreads();
+    }
+    uint64_t run = s[tid] - sum;
+    for (uint32_t i = b0; i < b1; ++i) {
+        off[i] = run;
+        cursor[i] = run;
+        run += cnt[i];
+    }
+    if (tid == 1023) off[G] = s[1023];
+}
+
+__global__ __launch_bounds__(256) void k_bucket_features(const uint64_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ att,
+                                                         const uint8_t* __restrict__ mbin,
+                                                         uint32_t k, uint64_t* __restrict__ stats,
+                                                         float* __restrict__ feat) {
+    __shared__ uint32_t s_hist[16];
+    __shared__ unsigned long long s_km, s_bases;
+    const uint32_t b = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (tid < 16) s_hist[tid] = 0;
+    if (tid == 0) s_km = 0, s_bases = 0;
+    __syncthreads();
+    const uint64_t r0 = off[b], r1 = off[b + 1];
+    unsigned long long km = 0, bases = 0;
+    for (uint64_t r = r0 + tid; r < r1; r += 256) {
+        const uint32_t len = att[r] & 0xFFu;
+        km += len - (k - 1);
+        bases += len;
+        atomicAdd(&s_hist[mbin[r] & 15u], 1u);
+    }
+    atomicAdd(&s_km, km);
+    atomicAdd(&s_bases, bases);
+    __syncthreads();
+    if (tid == 0) {
+        const uint64_t n = r1 - r0;
+        stats[4 * b + 0] = n;
+        stats[4 * b + 1] = s_km;
+        stats[4 * b + 2] = s_bases;
+        stats[4 * b + 3] = 0;
+        float* f = feat + size_t(b) * N_FEAT;
+        for (int i = 0; i < 16; ++i) f[i] = log1pf(float(s_hist[i]));
+        f[16] = log1pf(float(n));
+        f[17] = log1pf(float(s_km));
+        f[18] = n ? float(s_bases) / float(n) / float(2 * k) : 0.f;
+    }
+}
+
+}  // namespace
+
+void launch_minimizer_g(const PartitionArgs& a, uint32_t* gbin, uint64_t* tile_lc, hipStream_t s) {
+    const uint64_t tiles = (a.n_bases + P_TILE - 1) / P_TILE;
+    if (!tiles) return;
+    hipLaunchKernelGGL(k_minimizer_g, dim3(uint32_t(tiles)), dim3(P_WG), 0, s, a, gbin, tile_lc);
+}
+
+void launch_pieces(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc,
+                   uint32_t W, bool emit, uint32_t* bucket_cnt, uint64_t* cursor, uint32_t* att,
+                   uint64_t* labels, uint8_t* mbin, hipStream_t s) {
+    const uint64_t tiles = (a.n_bases + P_TILE - 1) / P_TILE;
+    if (!tiles) return;
+    auto* cur = reinterpret_cast<unsigned long long*>(cursor);
+    if (emit)
+        hipLaunchKernelGGL(k_pieces<true>, dim3(uint32_t(tiles)), dim3(P_WG), 0, s, a, gbin, tile_lc,
+                           W, bucket_cnt, cur, att, labels, mbin);
+    else
+        hipLaunchKernelGGL(k_pieces<false>, dim3(uint32_t(tiles)), dim3(P_WG), 0, s, a, gbin,
+                           tile_lc, W, bucket_cnt, cur, att, labels, mbin);
+}
+
+void launch_bucket_scan(const uint32_t* cnt, uint32_t G, uint64_t* off, uint64_t* cursor,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, s, cnt, G, off, cursor);
+}
+
+void launch_bucket_features(const uint64_t* off, const uint32_t* att, const uint8_t* mbin,
+                            uint32_t G, uint32_t k, uint64_t* stats, float* feat, hipStream_t s) {
+    hipLaunchKernelGGL(k_bucket_features, dim3(G), dim3(256), 0, s, off, att, mbin, k, stats, feat);
+}
+
+}  // namespace cfgpu

@@ -1,0 +1,354 @@
+"""ctypes binding of libcfgpu (include/cfgpu.h, include/cfhost.h).
+
+The binding is thin: every call goes straight to the C ABI, which runs the
+gfx950 kernels.  There is no CPU fallback; a missing library or GPU raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "libcfgpu.so")
+
+u64p = ctypes.POINTER(ctypes.c_uint64)
+u32p = ctypes.POINTER(ctypes.c_uint32)
+u8p = ctypes.POINTER(ctypes.c_uint8)
+
+MODE_PLAIN, MODE_BATCHED, MODE_BATCHED_LEARNED = 0, 1, 2
+MODES = {"plain": MODE_PLAIN, "batched": MODE_BATCHED, "learned": MODE_BATCHED_LEARNED}
+G = 16384
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_uint32), ("l", ctypes.c_uint32), ("cutoff", ctypes.c_uint32),
+                ("subgraph_count", ctypes.c_uint32), ("device", ctypes.c_int32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class BucketStat(ctypes.Structure):
+    _fields_ = [("n_records", ctypes.c_uint64), ("n_kmers", ctypes.c_uint64),
+                ("n_bases", ctypes.c_uint64), ("n_vertices", ctypes.c_uint64),
+                ("capacity", ctypes.c_uint64), ("predicted", ctypes.c_float),
+                ("flags", ctypes.c_uint32)]
+
+
+class TableView(ctypes.Structure):
+    _fields_ = [("entries", ctypes.c_void_p), ("capacity", ctypes.c_uint64),
+                ("entry_bytes", ctypes.c_uint32), ("key_words", ctypes.c_uint32),
+                ("n_vertices", ctypes.c_uint64)]
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_uint64) for n in ("n_bases", "n_fragments", "n_records", "n_kmers",
+                                                 "n_edges", "n_vertices", "table_slots",
+                                                 "table_bytes")]
+                + [(n, ctypes.c_uint32) for n in ("n_buckets_nonempty", "n_overflowed",
+                                                  "n_fallback_rounds", "n_samples")]
+                + [(n, ctypes.c_double) for n in ("t_minimizer_ms", "t_count_ms", "t_emit_ms",
+                                                  "t_features_ms", "t_partition_ms", "t_sample_ms",
+                                                  "t_train_ms", "t_infer_ms",
+                                                  "t_construct_main_ms", "t_fallback_ms",
+                                                  "t_construct_ms", "t_construct_kernel_ms",
+                                                  "mlp_loss",
+                                                  "mispredict_mean_abs_log",
+                                                  "mispredict_under_frac", "fallback_rate")])
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("load_factor", ctypes.c_float), ("workgroup", ctypes.c_uint32),
+                ("chunk_records", ctypes.c_uint32), ("wgs_per_cu", ctypes.c_uint32),
+                ("margin", ctypes.c_float), ("use_graph", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class TuneTrial(ctypes.Structure):
+    _fields_ = [("tuning", Tuning), ("ms", ctypes.c_double)]
+
+
+# name -> (restype, argtypes); the set of names must equal the header's.
+SIGNATURES = {
+    "cfgpu_open": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_void_p)]),
+    "cfgpu_close": (ctypes.c_int, [ctypes.c_void_p]),
+    "cfgpu_last_error": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]),
+    "cfgpu_partition": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                       ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32]),
+    "cfgpu_bucket_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BucketStat),
+                                          ctypes.c_uint32]),
+    "cfgpu_records_copy": (ctypes.c_int, [ctypes.c_void_p, u32p, u64p, u64p, ctypes.c_uint64,
+                                          u64p]),
+    "cfgpu_construct": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "cfgpu_table_view_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32,
+                                            ctypes.POINTER(TableView)]),
+    "cfgpu_table_copy": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_uint32, u64p, u32p, u8p,
+                                        u8p, u32p, ctypes.c_uint64, u64p]),
+    "cfgpu_release": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "cfgpu_run_stats_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RunStats)]),
+    "cfgpu_tuning_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Tuning)]),
+    "cfgpu_tuning_set": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Tuning)]),
+    "cfgpu_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32,
+                                      ctypes.POINTER(TuneTrial), ctypes.c_uint32, u32p]),
+    "cfhost_input_from_file": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_uint32,
+                                              ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]),
+    "cfhost_input_from_buffer": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int,
+                                                ctypes.c_uint32, ctypes.c_uint32,
+                                                ctypes.POINTER(ctypes.c_void_p)]),
+    "cfhost_input_from_records": (ctypes.c_int, [ctypes.c_char_p, u64p, ctypes.c_uint64,
+                                                 ctypes.c_uint32, ctypes.c_uint32,
+                                                 ctypes.POINTER(ctypes.c_void_p)]),
+    "cfhost_input_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(u64p), u64p, u64p,
+                                        ctypes.POINTER(u64p), u64p, u64p]),
+    "cfhost_input_free": (None, [ctypes.c_void_p]),
+    "cfhost_synth_genome": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_double, ctypes.c_uint32,
+                                           ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64,
+                                           ctypes.c_void_p]),
+    "cfhost_synth_insert_repeats": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64,
+                                                   ctypes.c_uint32, ctypes.c_uint32,
+                                                   ctypes.c_double, ctypes.c_uint64,
+                                                   ctypes.c_void_p]),
+    "cfhost_synth_reads": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint64,
+                                          ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64,
+                                          ctypes.c_uint32, ctypes.c_void_p]),
+    "cfhost_synth_mutate": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_uint32, ctypes.c_uint64,
+                                           ctypes.c_void_p, ctypes.c_uint64, u64p]),
+}
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libcfgpu.so not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+# ---------------------------------------------------------------------------
+class Input:
+    """Packed fragment stream built by the host ingest (cfhost_input)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        L = lib()
+        packed, frag = u64p(), u64p()
+        nw, nb, nf, nr = (ctypes.c_uint64() for _ in range(4))
+        L.cfhost_input_get(handle, ctypes.byref(packed), ctypes.byref(nw), ctypes.byref(nb),
+                           ctypes.byref(frag), ctypes.byref(nf), ctypes.byref(nr))
+        self.n_words, self.n_bases, self.n_frag, self.n_records = nw.value, nb.value, nf.value, nr.value
+        self.packed = np.ctypeslib.as_array(packed, shape=(self.n_words,)) if self.n_words else np.zeros(2, np.uint64)
+        self.frag_off = np.ctypeslib.as_array(frag, shape=(self.n_frag + 1,))
+
+    @classmethod
+    def from_buffer(cls, buf: bytes, fastq: bool, k: int, threads: int = 0):
+        h = ctypes.c_void_p()
+        r = lib().cfhost_input_from_buffer(buf, len(buf), int(fastq), k, threads, ctypes.byref(h))
+        if r != 0:
+            raise RuntimeError(f"cfhost_input_from_buffer failed: {r}")
+        return cls(h)
+
+    @classmethod
+    def from_file(cls, path: str, fastq: bool, k: int, threads: int = 0):
+        h = ctypes.c_void_p()
+        r = lib().cfhost_input_from_file(path.encode(), int(fastq), k, threads, ctypes.byref(h))
+        if r != 0:
+            raise RuntimeError(f"cfhost_input_from_file failed: {r}")
+        return cls(h)
+
+    @classmethod
+    def from_records(cls, seq: bytes, rec_off, k: int, threads: int = 0):
+        ro = np.ascontiguousarray(rec_off, dtype=np.uint64)
+        h = ctypes.c_void_p()
+        r = lib().cfhost_input_from_records(seq, _p(ro, u64p), len(ro) - 1, k, threads,
+                                            ctypes.byref(h))
+        if r != 0:
+            raise RuntimeError(f"cfhost_input_from_records failed: {r}")
+        return cls(h)
+
+    def close(self):
+        if self._h:
+            lib().cfhost_input_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synth_genome(n, repeat_frac=0.0, fam_min=300, fam_max=6000, divergence=0.02, seed=1):
+    out = ctypes.create_string_buffer(n)
+    r = lib().cfhost_synth_genome(n, repeat_frac, fam_min, fam_max, divergence, seed, out)
+    if r != 0:
+        raise RuntimeError("synth_genome failed")
+    return out.raw[:n]
+
+
+def synth_insert_repeats(base: bytes, n_copies, rep_len, sub_rate, seed):
+    n = len(base) + n_copies * rep_len
+    out = ctypes.create_string_buffer(n)
+    r = lib().cfhost_synth_insert_repeats(base, len(base), n_copies, rep_len, sub_rate, seed, out)
+    if r != 0:
+        raise RuntimeError("synth_insert_repeats failed")
+    return out.raw[:n]
+
+
+def synth_reads(genome: bytes, n_reads, read_len, sub_rate, seed, threads=0):
+    out = ctypes.create_string_buffer(n_reads * read_len)
+    r = lib().cfhost_synth_reads(genome, len(genome), n_reads, read_len, sub_rate, seed, threads, out)
+    if r != 0:
+        raise RuntimeError("synth_reads failed")
+    return out.raw[: n_reads * read_len]
+
+
+def synth_mutate(base: bytes, snp_rate, indel_rate, indel_max, seed):
+    cap = int(len(base) * (1 + indel_rate * indel_max) + 1024)
+    out = ctypes.create_string_buffer(cap)
+    n = ctypes.c_uint64()
+    r = lib().cfhost_synth_mutate(base, len(base), snp_rate, indel_rate, indel_max, seed, out, cap,
+                                  ctypes.byref(n))
+    if r != 0:
+        raise RuntimeError("synth_mutate failed")
+    return out.raw[: n.value]
+
+
+# ---------------------------------------------------------------------------
+class CfgpuError(RuntimeError):
+    pass
+
+
+class Handle:
+    """One cfgpu handle: partition -> construct -> table views."""
+
+    def __init__(self, k, l=20, cutoff=1, device=0):
+        self.k, self.l, self.cutoff = k, l, cutoff
+        self.kw = (k + 31) // 32
+        self.W = ((2 * (k - 1) - l + 2) + 31) // 32
+        p = Params(k, l, cutoff, G, device, 0)
+        h = ctypes.c_void_p()
+        r = lib().cfgpu_open(ctypes.byref(p), ctypes.byref(h))
+        if r != 0:
+            raise CfgpuError(f"cfgpu_open failed ({r}) for k={k} l={l} cutoff={cutoff}")
+        self._h = h
+
+    def _check(self, r, what):
+        if r != 0:
+            buf = ctypes.create_string_buffer(512)
+            lib().cfgpu_last_error(self._h, buf, 512)
+            raise CfgpuError(f"{what} failed ({r}): {buf.value.decode(errors='replace')}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cfgpu_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def partition(self, inp: Input):
+        self._check(lib().cfgpu_partition(self._h, inp.packed.ctypes.data, inp.n_bases,
+                                          inp.frag_off.ctypes.data, inp.n_frag, 0), "partition")
+
+    def partition_device(self, packed_ptr, n_bases, frag_ptr, n_frag):
+        self._check(lib().cfgpu_partition(self._h, packed_ptr, n_bases, frag_ptr, n_frag, 1),
+                    "partition")
+
+    def construct(self, mode="learned"):
+        self._check(lib().cfgpu_construct(self._h, MODES.get(mode, mode)), "construct")
+
+    def bucket_stats(self):
+        arr = (BucketStat * G)()
+        self._check(lib().cfgpu_bucket_stats(self._h, arr, G), "bucket_stats")
+        out = {n: np.array([getattr(s, n) for s in arr]) for n, _ in BucketStat._fields_}
+        return out
+
+    def records(self):
+        n = ctypes.c_uint64()
+        lib().cfgpu_records_copy(self._h, None, None, None, 0, ctypes.byref(n))
+        n = n.value
+        att = np.zeros(max(n, 1), np.uint32)
+        lab = np.zeros(max(n, 1) * self.W, np.uint64)
+        off = np.zeros(G + 1, np.uint64)
+        m = ctypes.c_uint64()
+        self._check(lib().cfgpu_records_copy(self._h, _p(att, u32p), _p(lab, u64p), _p(off, u64p),
+                                             n, ctypes.byref(m)), "records_copy")
+        return att[:n], lab[: n * self.W].reshape(n, self.W), off
+
+    def tables(self, buckets=None):
+        """All occupied entries: dict g, keys (n x kw), e_f, status, dfa."""
+        stats = self.bucket_stats()
+        if buckets is None:
+            buckets = np.nonzero(stats["n_records"])[0]
+        b = np.ascontiguousarray(buckets, dtype=np.uint32)
+        total = int(stats["n_vertices"][b].sum()) if len(b) else 0
+        keys = np.zeros(max(total, 1) * self.kw, np.uint64)
+        ef = np.zeros(max(total, 1), np.uint32)
+        st = np.zeros(max(total, 1), np.uint8)
+        dfa = np.zeros(max(total, 1), np.uint8)
+        bn = np.zeros(max(len(b), 1), np.uint32)
+        n = ctypes.c_uint64()
+        self._check(lib().cfgpu_table_copy(self._h, _p(b, u32p), len(b), _p(keys, u64p),
+                                           _p(ef, u32p), _p(st, u8p), _p(dfa, u8p), _p(bn, u32p),
+                                           max(total, 1), ctypes.byref(n)), "table_copy")
+        total = n.value
+        g = np.repeat(b, bn[: len(b)])
+        return {"g": g, "keys": keys[: total * self.kw].reshape(total, self.kw), "e_f": ef[:total],
+                "status": st[:total], "dfa": dfa[:total]}
+
+    def table_view(self, g):
+        v = TableView()
+        self._check(lib().cfgpu_table_view_get(self._h, g, ctypes.byref(v)), "table_view_get")
+        return v
+
+    def release(self, g):
+        self._check(lib().cfgpu_release(self._h, g), "release")
+
+    def run_stats(self):
+        s = RunStats()
+        self._check(lib().cfgpu_run_stats_get(self._h, ctypes.byref(s)), "run_stats")
+        return s.as_dict()
+
+    def tuning(self):
+        t = Tuning()
+        self._check(lib().cfgpu_tuning_get(self._h, ctypes.byref(t)), "tuning_get")
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = Tuning()
+        lib().cfgpu_tuning_get(self._h, ctypes.byref(t))
+        for k_, v in kw.items():
+            setattr(t, k_, v)
+        self._check(lib().cfgpu_tuning_set(self._h, ctypes.byref(t)), "tuning_set")
+
+    def autotune(self, mode="learned", n=64):
+        arr = (TuneTrial * n)()
+        w = ctypes.c_uint32()
+        self._check(lib().cfgpu_autotune(self._h, MODES.get(mode, mode), arr, n, ctypes.byref(w)),
+                    "autotune")
+        return [dict(arr[i].tuning.as_dict(), ms=arr[i].ms) for i in range(w.value)]
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,189 @@
+/*
+ * cfgpu.h — C ABI of libcfgpu, the MI355X (gfx950) implementation of Cuttlefish 3's
+ * minimizer-partitioned subgraph construction.
+ *
+ * It replaces, behind the reference's C++ class contract, this call sequence of
+ * dBG_Contractor::construct (src/dBG_Contractor.cpp:46-61):
+ *
+ *   Graph_Partitioner<k,...>(G, logistics, l).partition();   -> cfgpu_partition
+ *   G.finalize();                                             -> (buckets stay in HBM)
+ *   G.process(): per g  Subgraph(B,G,op_buf,space).construct() -> cfgpu_construct
+ *                        b.remove()                            -> cfgpu_release
+ *                        contract() reads the table            -> cfgpu_table_view_get /
+ *                                                                 cfgpu_table_copy
+ *
+ * Conventions: every function returns CFGPU_OK (0) or a negative CFGPU_E_* code and
+ * never throws; the message of the last failure is available from cfgpu_last_error.
+ * A handle is used from one host thread at a time.  The handle owns all device
+ * memory; host buffers passed in are only read during the call.
+ */
+#ifndef CFGPU_H
+#define CFGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CFGPU_OK = 0,
+    CFGPU_E_ARG = -1,      /* invalid argument or input shape */
+    CFGPU_E_HIP = -2,      /* HIP runtime error (includes "no GPU") */
+    CFGPU_E_NOMEM = -3,    /* device allocation failed */
+    CFGPU_E_STATE = -4,    /* call out of order (e.g. construct before partition) */
+    CFGPU_E_OVERFLOW = -5  /* table overflow not recovered (internal retry cap hit) */
+};
+
+/* Build parameters.  k, l and cutoff carry the meaning of `cuttlefish build -k`,
+ * `--min-len` and `-c` (src/commands.cpp:38-106, include/Build_Params.hpp):
+ * odd k with l < k <= 127, 1 <= l <= 32, 1 <= cutoff <= 15.  subgraph_count must be
+ * 16384, the reference's fixed subgraph count (include/Atlas.hpp:33-35). */
+typedef struct cfgpu_params {
+    uint32_t k;
+    uint32_t l;
+    uint32_t cutoff;
+    uint32_t subgraph_count;
+    int32_t device;
+    uint32_t reserved;
+} cfgpu_params;
+
+typedef struct cfgpu_handle cfgpu_handle;
+
+/* Opens a handle on `device`.  Replaces the construction of Subgraphs_Manager
+ * (src/Subgraphs_Manager.cpp:26-52) and Edge_Frequency::set_edge_threshold
+ * (src/State_Config.cpp:10-14, called at src/dBG_Contractor.cpp:24). */
+int cfgpu_open(const cfgpu_params* params, cfgpu_handle** out);
+int cfgpu_close(cfgpu_handle* h);
+/* Copies the last error message (NUL-terminated, truncated to n bytes). */
+int cfgpu_last_error(const cfgpu_handle* h, char* buf, size_t n);
+
+/* Input: the fragments of the input (maximal ACGT runs of length >= k+1,
+ * src/Graph_Partitioner.cpp:476-501) concatenated into one stream of 2-bit bases
+ * (A=0 C=1 G=2 T=3) packed MSB-first, 32 per word: base j at bits 63-2(j%32)..62-2(j%32)
+ * of word j/32.  frag_off[n_frag+1] gives each fragment's first base; frag_off[0] == 0
+ * and frag_off[n_frag] == n_bases.  With CFGPU_INPUT_DEVICE the two pointers are device
+ * memory and `packed` must hold ceil(n_bases/32)+2 readable words (the last two zero).
+ *
+ * Replaces Graph_Partitioner::partition()/process_chunk (src/Graph_Partitioner.cpp:45,
+ * 424-614) and Subgraphs_Manager::add_super_kmer/finalize
+ * (include/Subgraphs_Manager.hpp:93, 113; src/Subgraphs_Manager.cpp:84-95): the weak
+ * super k-mers are left in 16384 device-resident buckets in the reference record
+ * format (Super_Kmer_Attributes<false> + MSB-aligned label). */
+enum { CFGPU_INPUT_DEVICE = 1 };
+int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
+                    const uint64_t* frag_off, uint64_t n_frag, uint32_t flags);
+
+typedef struct cfgpu_bucket_stat {
+    uint64_t n_records;   /* weak super k-mers in the bucket */
+    uint64_t n_kmers;     /* k-mer instances: sum(len - k + 1) */
+    uint64_t n_bases;     /* sum of record lengths */
+    uint64_t n_vertices;  /* distinct canonical k-mers (after construct) */
+    uint64_t capacity;    /* table slots (after construct) */
+    float predicted;      /* MLP-predicted distinct vertices (learned mode) */
+    uint32_t flags;       /* CFGPU_BUCKET_* */
+} cfgpu_bucket_stat;
+enum {
+    CFGPU_BUCKET_BUILT = 1,       /* table holds the bucket's final states */
+    CFGPU_BUCKET_OVERFLOWED = 2,  /* first-attempt table was too small and was rebuilt */
+    CFGPU_BUCKET_SAMPLE = 4       /* used as an MLP training sample */
+};
+/* Per-subgraph statistics (the reference logs these at src/Subgraphs_Manager.cpp:264-305).
+ * `out` holds n >= 16384 entries. */
+int cfgpu_bucket_stats(cfgpu_handle* h, cfgpu_bucket_stat* out, uint32_t n);
+
+/* Copies every bucket's records to the host: att[n_records] (16-bit attribute pack of
+ * Super_Kmer_Attributes<false> in the low half, g in the high half, the reference's
+ * in-memory layout, include/Super_Kmer_Attributes.hpp:25-55), labels[n_records*W]
+ * (W = ceil((2(k-1)-l+2)/32) words per record, include/Super_Kmer_Chunk.hpp:394-405;
+ * bits past the record length are zero) and bucket_off[16385].  Record order inside
+ * a bucket is unspecified, as in the reference (parallel workers append). */
+int cfgpu_records_copy(cfgpu_handle* h, uint32_t* att, uint64_t* labels, uint64_t* bucket_off,
+                       uint64_t cap_records, uint64_t* n_records);
+
+/* Builds every non-empty subgraph's vertex table: Subgraph::construct
+ * (src/Subgraph.cpp:44-108) for all buckets, scheduled as in Subgraphs_Manager::process
+ * (src/Subgraphs_Manager.cpp:113-255).  Modes (results are identical in all modes):
+ *  PLAIN            one launch per bucket in bucket order, tables grown and retried;
+ *  BATCHED          one persistent work-queue launch (hipGraph) over all buckets,
+ *                   largest first, tables sized to the exact upper bound;
+ *  BATCHED_LEARNED  as BATCHED with tables sized and ordered by the MLP prediction,
+ *                   undersized tables detected exactly and rebuilt. */
+enum { CFGPU_MODE_PLAIN = 0, CFGPU_MODE_BATCHED = 1, CFGPU_MODE_BATCHED_LEARNED = 2 };
+int cfgpu_construct(cfgpu_handle* h, uint32_t mode);
+
+/* Device view of subgraph g's table: the contract() side of HT_Router
+ * (include/Subgraph.hpp:336-358).  Each entry is key_words u64 key words (Kmer<k>
+ * layout, include/Kmer.hpp:45-54: word 0 least significant), a u32 Edge_Frequency word
+ * and a u32 status word (State_Config<false>, include/State_Config.hpp:24-163).  An
+ * empty entry has all-ones in key word key_words-1.  Valid until cfgpu_release(g),
+ * the next cfgpu_partition/cfgpu_construct, or cfgpu_close. */
+typedef struct cfgpu_table_view {
+    const void* entries;
+    uint64_t capacity;
+    uint32_t entry_bytes;
+    uint32_t key_words;
+    uint64_t n_vertices;
+} cfgpu_table_view;
+int cfgpu_table_view_get(cfgpu_handle* h, uint32_t g, cfgpu_table_view* out);
+
+/* Copies the occupied entries of the listed buckets to host arrays, bucket after
+ * bucket (entry order inside a bucket unspecified): keys[n*key_words], e_f[n],
+ * status[n] and dfa[n] = edge_at(front) | edge_at(back)<<3 | front_disc<<6 |
+ * back_disc<<7 under the handle's cutoff (Edge_Frequency::edge_at,
+ * include/State_Config.hpp:232-252: 0..3 a base, 4 N, 5 E).  bucket_n[i] receives
+ * the count for buckets[i].  Any output pointer may be NULL. */
+int cfgpu_table_copy(cfgpu_handle* h, const uint32_t* buckets, uint32_t n_buckets,
+                     uint64_t* keys, uint32_t* e_f, uint8_t* status, uint8_t* dfa,
+                     uint32_t* bucket_n, uint64_t cap, uint64_t* n_out);
+
+/* Releases subgraph g's table view (Super_Kmer_Bucket::remove,
+ * src/Subgraphs_Manager.cpp:158, frees bucket files; here the view becomes invalid). */
+int cfgpu_release(cfgpu_handle* h, uint32_t g);
+
+/* Counters and timings of the last partition / construct. */
+typedef struct cfgpu_run_stats {
+    uint64_t n_bases, n_fragments, n_records, n_kmers, n_edges, n_vertices;
+    uint64_t table_slots, table_bytes;
+    uint32_t n_buckets_nonempty, n_overflowed, n_fallback_rounds, n_samples;
+    double t_minimizer_ms, t_count_ms, t_emit_ms, t_features_ms, t_partition_ms;
+    double t_sample_ms, t_train_ms, t_infer_ms, t_construct_main_ms, t_fallback_ms;
+    double t_construct_ms;
+    double t_construct_kernel_ms;     /* persistent construct kernel alone, main round
+                                         (recorded when use_graph == 0) */
+    double mlp_loss;                  /* final training MSE (log space) */
+    double mispredict_mean_abs_log;   /* mean |log(pred/true)| over predicted buckets */
+    double mispredict_under_frac;     /* fraction of predicted buckets with pred < true */
+    double fallback_rate;             /* overflowed / predicted buckets */
+} cfgpu_run_stats;
+int cfgpu_run_stats_get(cfgpu_handle* h, cfgpu_run_stats* out);
+
+/* Launch-shape knobs (defaults chosen by cfgpu_autotune or built-in). */
+typedef struct cfgpu_tuning {
+    float load_factor;        /* target table load for sized tables (0.3 .. 0.8) */
+    uint32_t workgroup;       /* construct workgroup size: 256 or 512 */
+    uint32_t chunk_records;   /* records per work item (LDS tiles of `workgroup`) */
+    uint32_t wgs_per_cu;      /* persistent grid = CUs * wgs_per_cu */
+    float margin;             /* learned sizing: capacity = pred * margin / load_factor;
+                                 below 1 under-sizes on purpose (exercises the exact
+                                 overflow fallback; results do not change) */
+    uint32_t use_graph;       /* 1: batched launches are captured and replayed as a hipGraph */
+} cfgpu_tuning;
+int cfgpu_tuning_get(cfgpu_handle* h, cfgpu_tuning* out);
+int cfgpu_tuning_set(cfgpu_handle* h, const cfgpu_tuning* in);
+/* Times the batched construct on the current partition for a small grid of tunings
+ * (results never change) and keeps the fastest.  `trials_out` (may be NULL) receives
+ * up to n_trials (tuning, ms) pairs. */
+typedef struct cfgpu_tune_trial {
+    cfgpu_tuning tuning;
+    double ms;
+} cfgpu_tune_trial;
+int cfgpu_autotune(cfgpu_handle* h, uint32_t mode, cfgpu_tune_trial* trials_out,
+                   uint32_t n_trials, uint32_t* n_written);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFGPU_H */

@@ -1,0 +1,64 @@
+/*
+ * cfhost.h — host-side ingest and synthetic inputs for libcfgpu.
+ *
+ * Ingest restates the reference's record -> fragment rules and 2-bit encoding on the
+ * host, in parallel, and produces exactly the stream cfgpu_partition consumes:
+ *   records  FASTA: '>' header lines, sequence lines concatenated, '\n' line ends only
+ *            (RabbitFX include/RabbitFX/io/Formater.cpp:63-130); FASTQ: 4-line records,
+ *            sequence = line 2 (Formater.cpp:238-263);
+ *   fragments maximal runs of A/C/G/T (either case) of length >= k+1 inside a record
+ *            (src/Graph_Partitioner.cpp:476-501, DNA_Utility::is_DNA_base);
+ *   bases    A=0 C=1 G=2 T=3 (include/DNA.hpp:17-25), packed MSB-first 32 per word.
+ */
+#ifndef CFHOST_H
+#define CFHOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cfhost_input cfhost_input;
+
+/* Parse a FASTA (is_fastq = 0) or FASTQ (is_fastq = 1) file / buffer. */
+int cfhost_input_from_file(const char* path, int is_fastq, uint32_t k, uint32_t threads,
+                           cfhost_input** out);
+int cfhost_input_from_buffer(const char* buf, uint64_t n, int is_fastq, uint32_t k,
+                             uint32_t threads, cfhost_input** out);
+/* Records given directly: record r is seq[rec_off[r], rec_off[r+1]). */
+int cfhost_input_from_records(const char* seq, const uint64_t* rec_off, uint64_t n_rec, uint32_t k,
+                              uint32_t threads, cfhost_input** out);
+/* packed holds n_words = ceil(n_bases/32) + 2 words (the last two zero). */
+int cfhost_input_get(const cfhost_input* in, const uint64_t** packed, uint64_t* n_words,
+                     uint64_t* n_bases, const uint64_t** frag_off, uint64_t* n_frag,
+                     uint64_t* n_records);
+void cfhost_input_free(cfhost_input* in);
+
+/* Seeded synthetic sequences (ACGT only).  All are deterministic in `seed`
+ * regardless of `threads`. */
+/* iid genome of length len; then repeat families (lengths log-uniform in
+ * [fam_min, fam_max]) copied with per-copy iid substitutions at `divergence`
+ * over positions until repeat_frac * len bases are covered. */
+int cfhost_synth_genome(uint64_t len, double repeat_frac, uint32_t fam_min, uint32_t fam_max,
+                        double divergence, uint64_t seed, char* out);
+/* base + n_copies inserted copies of one random rep_len repeat, each copy with
+ * iid substitutions at sub_rate; out holds base_len + n_copies * rep_len. */
+int cfhost_synth_insert_repeats(const char* base, uint64_t base_len, uint32_t n_copies,
+                                uint32_t rep_len, double sub_rate, uint64_t seed, char* out);
+/* n_reads reads of read_len from uniform starts on either strand (50/50) with iid
+ * substitutions at sub_rate, written back to back into out[n_reads*read_len]. */
+int cfhost_synth_reads(const char* genome, uint64_t glen, uint64_t n_reads, uint32_t read_len,
+                       double sub_rate, uint64_t seed, uint32_t threads, char* out);
+/* A derivative of base with SNPs at snp_rate and indels (1..indel_max bp, half
+ * insertions, half deletions) at indel_rate per base. */
+int cfhost_synth_mutate(const char* base, uint64_t len, double snp_rate, double indel_rate,
+                        uint32_t indel_max, uint64_t seed, char* out, uint64_t cap,
+                        uint64_t* out_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFHOST_H */

@@ -123,4 +123,103 @@ int64_t cfo_construct(int k, int l, uint32_t G, int mode, const uint64_t* bucket
 
 uint32_t cfo_edge_at(uint32_t e_f, int side, int f_th) { return edge_at(e_f, side, f_th); }
 
+// CPU baseline: the reference's pipeline shape on `threads` workers.
+//  1. partition: records split into `threads` contiguous ranges (the reference
+//     parallelises across input files, src/Graph_Partitioner.cpp:121-184; a
+//     single read file is split here so every core partitions);
+//  2. bucketize the weak super k-mers by g (Atlas / Super_Kmer_Bucket);
+//  3. construct every subgraph with a per-worker hash map, buckets handed out
+//     largest first (src/Subgraphs_Manager.cpp:222-255).
+// out[0..5] = partition ms, bucketize ms, construct ms, k-mer instances,
+//             weak super k-mers, distinct (g, vertex) states.
+int cfo_pipeline(const char* seq, const uint64_t* rec_off, size_t n_rec, int k, int l, int threads,
+                 double* out) {
+    using clk = std::chrono::steady_clock;
+    const unsigned W = label_words(k, l);
+    const uint32_t G = 16384;
+    if (threads < 1) threads = 1;
+    struct Part { std::vector<uint32_t> g; std::vector<uint16_t> att; std::vector<uint64_t> lab; };
+    std::vector<Part> parts(threads);
+    const auto t0 = clk::now();
+    {
+        std::vector<std::thread> th;
+        for (int t = 0; t < threads; ++t)
+            th.emplace_back([&, t]() {
+                Part& P = parts[t];
+                std::vector<SuperKmer> sk;
+                const size_t r0 = n_rec * t / threads, r1 = n_rec * (t + 1) / threads;
+                for (size_t r = r0; r < r1; ++r) {
+                    const char* rs = seq + rec_off[r];
+                    for (const auto& f : fragments(rs, rec_off[r + 1] - rec_off[r], k)) {
+                        sk.clear();
+                        partition_fragment(rs + f.first, f.second, k, l, G, sk);
+                        for (const auto& x : sk) {
+                            P.g.push_back(x.g);
+                            P.att.push_back(att_pack(x.len, x.l_disc, x.r_disc));
+                            const size_t o = P.lab.size();
+                            P.lab.resize(o + W);
+                            encode_label(rs + f.first + x.off, x.len, W, &P.lab[o]);
+                        }
+                    }
+                }
+            });
+        for (auto& x : th) x.join();
+    }
+    const auto t1 = clk::now();
+    std::vector<uint64_t> off(G + 1, 0);
+    for (const auto& P : parts) for (uint32_t g : P.g) ++off[g + 1];
+    for (uint32_t g = 0; g < G; ++g) off[g + 1] += off[g];
+    const uint64_t n_sk = off[G];
+    std::vector<uint16_t> att(n_sk);
+    std::vector<uint64_t> lab(n_sk * W);
+    {
+        std::vector<uint64_t> cur(off.begin(), off.end() - 1);
+        for (const auto& P : parts)
+            for (size_t i = 0; i < P.g.size(); ++i) {
+                const uint64_t d = cur[P.g[i]]++;
+                att[d] = P.att[i];
+                std::memcpy(&lab[d * W], &P.lab[i * W], W * 8);
+            }
+    }
+    parts.clear();
+    const auto t2 = clk::now();
+    std::vector<uint32_t> order(G);
+    for (uint32_t g = 0; g < G; ++g) order[g] = g;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return off[a + 1] - off[a] > off[b + 1] - off[b];
+    });
+    std::atomic<uint32_t> next{0};
+    std::atomic<uint64_t> states{0};
+    {
+        std::vector<std::thread> th;
+        for (int t = 0; t < threads; ++t)
+            th.emplace_back([&]() {
+                std::vector<VertexState> vs;
+                uint64_t mine = 0;
+                while (true) {
+                    const uint32_t i = next.fetch_add(1);
+                    if (i >= G) break;
+                    const uint32_t g = order[i];
+                    if (off[g + 1] == off[g]) continue;
+                    vs.clear();
+                    construct_literal(k, W, g, off[g + 1] - off[g], &att[off[g]], &lab[off[g] * W], vs);
+                    mine += vs.size();
+                }
+                states += mine;
+            });
+        for (auto& x : th) x.join();
+    }
+    const auto t3 = clk::now();
+    uint64_t n_km = 0;
+    for (uint64_t i = 0; i < n_sk; ++i) n_km += (att[i] & 0xFF) - (k - 1);
+    auto ms = [](clk::duration d) { return std::chrono::duration<double, std::milli>(d).count(); };
+    out[0] = ms(t1 - t0);
+    out[1] = ms(t2 - t1);
+    out[2] = ms(t3 - t2);
+    out[3] = double(n_km);
+    out[4] = double(n_sk);
+    out[5] = double(states.load());
+    return 0;
+}
+
 }  // extern "C"

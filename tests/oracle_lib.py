@@ -49,6 +49,9 @@ def lib():
         L.cfo_construct.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int, u64p,
                                     u16p, u64p, ctypes.c_int, u32p, u64p, u32p, u8p,
                                     ctypes.c_size_t]
+        L.cfo_pipeline.restype = ctypes.c_int
+        L.cfo_pipeline.argtypes = [ctypes.c_char_p, u64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.cfo_edge_at.restype = ctypes.c_uint32
         L.cfo_edge_at.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
         _LIB = L
@@ -136,3 +139,13 @@ def construct(k, l, bucket_off, att, labels, G=16384, mode=0, threads=8):
 
 def edge_at(e_f, side, f_th):
     return lib().cfo_edge_at(int(e_f), side, f_th)
+
+
+def pipeline(seq: bytes, rec_off, k, l, threads):
+    """Timed CPU pipeline (partition, bucketize, construct) on `threads` workers."""
+    ro = np.ascontiguousarray(rec_off, dtype=np.uint64)
+    out = (ctypes.c_double * 6)()
+    r = lib().cfo_pipeline(seq, _ptr(ro, u64p), len(ro) - 1, k, l, threads, out)
+    assert r == 0
+    return {"partition_ms": out[0], "bucketize_ms": out[1], "construct_ms": out[2],
+            "n_kmers": int(out[3]), "n_records": int(out[4]), "n_states": int(out[5])}

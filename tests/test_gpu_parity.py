@@ -1,0 +1,226 @@
+"""GPU parity: the HIP partition and construct, called through the C ABI,
+against the CPU oracle on the same seeded inputs.
+
+Bar: bit-exact.  Partition: the per-bucket multiset of records (attribute pack
+and label) equals the oracle's.  Construct: the sorted (g, canonical k-mer,
+Edge_Frequency word, status) rows equal the oracle's literal restatement of
+Subgraph::construct, and the exported DFA byte equals edge_at/disc under the
+cutoff.  All three construct modes must agree."""
+import sys
+
+import numpy as np
+import pytest
+
+import cfgpu as C
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rows_sorted(cols):
+    m = np.stack([np.asarray(c, dtype=np.uint64) for c in cols], axis=1)
+    order = np.lexsort(m.T[::-1])
+    return m[order]
+
+
+def oracle_records(buf, fastq, k, l):
+    seq, ro = O.parse(buf, fastq)
+    part = O.partition(seq, ro, k, l)
+    return seq, ro, part
+
+
+def check_records(h, part, k, l):
+    att, lab, off = h.records()
+    W = O.label_words(k, l)
+    n = len(part["g"])
+    assert len(att) == n, (len(att), n)
+    g_gpu = (att >> 16).astype(np.uint64)
+    # records sit in their own bucket
+    if n:
+        bucket_of = np.repeat(np.arange(C.G), np.diff(off.astype(np.int64)))
+        np.testing.assert_array_equal(bucket_of, g_gpu)
+    gpu = rows_sorted([g_gpu, att & 0xFFFF] + [lab[:, j] for j in range(W)])
+    ora = rows_sorted([part["g"], part["att"]] + [part["labels"][:, j] for j in range(W)])
+    np.testing.assert_array_equal(gpu, ora)
+    return att, lab, off
+
+
+def dfa_from(ef, st, cutoff):
+    def side(off):
+        f = np.stack([(ef >> (off + 4 * b)) & 0xF for b in range(4)], axis=1) >= cutoff
+        c = f.sum(axis=1)
+        which = np.argmax(f, axis=1)
+        return np.where(c == 0, 5, np.where(c == 1, which, 4)).astype(np.uint32)
+    return (side(0) | (side(16) << 3) | ((st.astype(np.uint32) & 2) << 5) |
+            ((st.astype(np.uint32) & 4) << 5)).astype(np.uint8)
+
+
+def check_tables(h, part, k, l, cutoff, want=None):
+    kw = (k + 31) // 32
+    t = h.tables()
+    if want is None:
+        off, att, lab = O.bucketize(part)
+        want = O.construct(k, l, off, att, lab, mode=0)
+    n = len(want["g"])
+    assert len(t["g"]) == n, (len(t["g"]), n)
+    gpu = rows_sorted([t["g"]] + [t["keys"][:, j] for j in range(kw - 1, -1, -1)] + [t["e_f"], t["status"], t["dfa"]])
+    ora = rows_sorted([want["g"]] + [want["keys"][:, j] for j in range(kw - 1, -1, -1)] +
+                      [want["e_f"], want["status"], dfa_from(want["e_f"], want["status"], cutoff)])
+    np.testing.assert_array_equal(gpu, ora)
+    stats = h.bucket_stats()
+    nz = stats["n_records"] > 0
+    assert ((stats["flags"][nz] & 1) == 1).all()
+    np.testing.assert_array_equal(np.bincount(want["g"], minlength=C.G)[nz], stats["n_vertices"][nz])
+    return want
+
+
+def run_case(name, buf, fastq, k, l, cutoff, modes=("plain", "batched", "learned")):
+    say(f"[case] {name}: k={k} l={l} cutoff={cutoff} bytes={len(buf)}")
+    seq, ro, part = oracle_records(buf, fastq, k, l)
+    inp = C.Input.from_buffer(buf, fastq, k)
+    want = None
+    with C.Handle(k, l, cutoff) as h:
+        say(f"[case] {name}: partition n_bases={inp.n_bases} n_frag={inp.n_frag}")
+        h.partition(inp)
+        check_records(h, part, k, l)
+        rs = h.run_stats()
+        assert rs["n_records"] == len(part["g"])
+        lens = (part["att"] & 0xFF).astype(np.int64)
+        assert rs["n_kmers"] == int((lens - (k - 1)).sum())
+        beg, ln = O.fragments(seq, ro, k)
+        assert rs["n_kmers"] == int((ln.astype(np.int64) - k + 1).sum()) + int((lens > 0).sum()) - len(ln)
+        for mode in modes:
+            say(f"[case] {name}: construct mode={mode}")
+            h.construct(mode)
+            want = check_tables(h, part, k, l, cutoff, want)
+    return want
+
+
+def fasta(recs, width=80):
+    return b"".join(b">r%d\n%s\n" % (i, b"\n".join(r[j:j + width] for j in range(0, len(r), width)))
+                    for i, r in enumerate(recs))
+
+
+def fastq(reads, read_len):
+    n = len(reads) // read_len
+    q = b"I" * read_len
+    return b"".join(b"@q%d\n%s\n+\n%s\n" % (i, reads[i * read_len:(i + 1) * read_len], q) for i in range(n))
+
+
+# --------------------------------------------------------------------------
+def test_reference_mode_repeats_k31():
+    base = C.synth_genome(200000, 0.0, 300, 6000, 0.0, seed=11)
+    g = C.synth_insert_repeats(base, 40, 1000, 0.01, seed=12)
+    run_case("ref-repeats-k31", fasta([g]), False, 31, 20, 1)
+
+
+def test_read_mode_k31_cutoff2():
+    g = C.synth_genome(60000, 0.1, 300, 2000, 0.02, seed=21)
+    reads = C.synth_reads(g, 8000, 150, 0.005, seed=22)
+    run_case("reads-k31-c2", fastq(reads, 150), True, 31, 20, 2)
+
+
+def test_read_mode_k63_cutoff2():
+    g = C.synth_genome(40000, 0.2, 300, 2000, 0.01, seed=31)
+    reads = C.synth_reads(g, 5000, 150, 0.01, seed=32)
+    run_case("reads-k63-c2", fastq(reads, 150), True, 63, 20, 2)
+
+
+def test_multi_reference_k127():
+    base = C.synth_genome(30000, 0.0, 300, 6000, 0.0, seed=41)
+    gens = [C.synth_mutate(base, 0.005, 0.01, 10, seed=42 + i) for i in range(4)]
+    run_case("multiref-k127", fasta(gens), False, 127, 20, 1)
+
+
+@pytest.mark.parametrize("k,l", [(45, 20), (95, 24), (21, 11)])
+def test_other_word_counts(k, l):
+    g = C.synth_genome(20000, 0.2, 200, 1500, 0.02, seed=k)
+    reads = C.synth_reads(g, 2000, 150, 0.01, seed=k + 1)
+    run_case(f"reads-k{k}", fastq(reads, 150), True, k, l, 2)
+
+
+def test_edge_cases_homopolymers_tandem_lowercase_N():
+    rng = np.random.default_rng(5)
+    r = bytes(rng.choice(list(b"ACGT"), 3000))
+    recs = [r[:1000] + b"A" * 5000 + r[1000:1500] + b"T" * 400 + (b"CAG" * 300) + b"N" * 5 + r[1500:].lower(),
+            b"G" * 40, r[:31], r[:32], b"AACTGACATGTCAGTT" * 20, b"NNNN", b""]
+    run_case("edges-k31", fasta(recs), False, 31, 20, 1)
+    run_case("edges-k15", fasta(recs), False, 15, 7, 3)
+
+
+def test_long_homopolymer_run_crosses_many_tiles():
+    # one g-run longer than 64 partition tiles: the carry-in look-back takes
+    # more than one step
+    g = C.synth_genome(2000, 0.0, 300, 6000, 0.0, seed=3)
+    run_case("long-run", fasta([g + b"A" * 150000 + g]), False, 31, 20, 1, modes=("learned",))
+
+
+def test_reference_fixtures():
+    import os
+    d = os.path.join(os.path.dirname(__file__), "golden", "ref_data")
+    for name, fq, k, l in (("rob-palindrome.fa", False, 15, 7), ("refs2.fa", False, 9, 5),
+                           ("reads.fq", True, 5, 3), ("cycles.fa", False, 3, 2)):
+        with open(os.path.join(d, name), "rb") as f:
+            run_case(name, f.read(), fq, k, l, 1)
+
+
+def test_empty_input():
+    say("[case] empty")
+    inp = C.Input.from_buffer(b">x\nNNNN\n", False, 31)
+    assert inp.n_frag == 0
+    with C.Handle(31, 20, 1) as h:
+        h.partition(inp)
+        for mode in ("plain", "batched", "learned"):
+            h.construct(mode)
+            t = h.tables()
+            assert len(t["g"]) == 0
+
+
+def test_overflow_fallback_is_exact():
+    # margin < 1 under-sizes the learned tables on purpose: every overflow is
+    # detected and the bucket rebuilt, so the result cannot change.
+    g = C.synth_genome(80000, 0.1, 300, 2000, 0.02, seed=51)
+    reads = C.synth_reads(g, 10000, 150, 0.005, seed=52)
+    buf = fastq(reads, 150)
+    k, l = 31, 20
+    seq, ro, part = oracle_records(buf, True, k, l)
+    inp = C.Input.from_buffer(buf, True, k)
+    say("[case] overflow-fallback")
+    with C.Handle(k, l, 2) as h:
+        h.partition(inp)
+        h.set_tuning(margin=0.25, load_factor=0.8)
+        h.construct("learned")
+        rs = h.run_stats()
+        say(f"[case] overflowed={rs['n_overflowed']} rounds={rs['n_fallback_rounds']}")
+        assert rs["n_overflowed"] > 0 and rs["n_fallback_rounds"] >= 1
+        check_tables(h, part, k, l, 2)
+        stats = h.bucket_stats()
+        assert ((stats["flags"] & 2) > 0).sum() == rs["n_overflowed"]
+
+
+def test_repeat_calls_are_idempotent_and_release():
+    g = C.synth_genome(50000, 0.2, 300, 2000, 0.02, seed=61)
+    reads = C.synth_reads(g, 6000, 150, 0.01, seed=62)
+    inp = C.Input.from_buffer(fastq(reads, 150), True, 31)
+    say("[case] idempotent")
+    with C.Handle(31, 20, 2) as h:
+        h.partition(inp)
+        h.construct("learned")
+        a = h.tables()
+        h.partition(inp)
+        h.construct("learned")
+        b = h.tables()
+        np.testing.assert_array_equal(rows_sorted([a["g"], a["keys"][:, 0], a["e_f"], a["status"]]),
+                                      rows_sorted([b["g"], b["keys"][:, 0], b["e_f"], b["status"]]))
+        nz = np.nonzero(h.bucket_stats()["n_records"])[0]
+        v = h.table_view(int(nz[0]))
+        assert v.capacity >= v.n_vertices > 0 and v.key_words == 1 and v.entry_bytes == 16
+        h.release(int(nz[0]))
+        assert h.table_view(int(nz[0])).capacity == 0
+        assert len(h.tables(buckets=[int(nz[0])])["g"]) == 0
+        assert len(h.tables(buckets=[int(nz[1])])["g"]) > 0
