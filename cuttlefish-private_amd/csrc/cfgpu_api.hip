@@ -729,6 +729,58 @@ int cfgpu_records_copy(cfgpu_handle* h, uint32_t* att, uint64_t* labels, uint64_
     return CFGPU_OK;
 }
 
+int cfgpu_load_buckets(cfgpu_handle* h, const uint32_t* att, const uint64_t* labels,
+                       const uint64_t* bucket_off) {
+    if (!h) return CFGPU_E_ARG;
+    if (!att || !labels || !bucket_off) return fail(h, CFGPU_E_ARG, "load_buckets: null pointer");
+    HIPCK(h, hipSetDevice(h->prm.device));
+    const uint32_t G = h->G, k = h->prm.k, l = h->prm.l;
+    if (bucket_off[0] != 0) return fail(h, CFGPU_E_ARG, "load_buckets: bucket_off[0] != 0");
+    const uint32_t max_len = std::min<uint32_t>(255, 2 * k - l);
+    for (uint32_t g = 0; g < G; ++g) {
+        if (bucket_off[g + 1] < bucket_off[g]) return fail(h, CFGPU_E_ARG, "load_buckets: offsets decrease");
+        for (uint64_t r = bucket_off[g]; r < bucket_off[g + 1]; ++r) {
+            const uint32_t a = att[r], len = a & 0xFFu;
+            if ((a >> 16) != g || len < k || len > max_len || ((a >> 10) & 0x3Fu))
+                return fail(h, CFGPU_E_ARG, "load_buckets: record %llu of bucket %u is malformed",
+                            (unsigned long long)r, g);
+        }
+    }
+    h->partitioned = h->constructed = false;
+    const uint64_t n = bucket_off[G];
+    HIPCK(h, h->att.ensure(n * 4));
+    HIPCK(h, h->labels.ensure(n * h->W * 8));
+    HIPCK(h, h->mbin.ensure(n));
+    HIPCK(h, h->boff.ensure((G + 1) * 8));
+    HIPCK(h, h->stats.ensure(size_t(G) * 4 * 8));
+    HIPCK(h, h->feat.ensure(size_t(G) * N_FEAT * 4));
+    if (n) {
+        HIPCK(h, hipMemcpyAsync(h->att.p, att, n * 4, hipMemcpyHostToDevice, h->stream));
+        HIPCK(h, hipMemcpyAsync(h->labels.p, labels, n * h->W * 8, hipMemcpyHostToDevice, h->stream));
+        HIPCK(h, hipMemsetAsync(h->mbin.p, 0, n, h->stream));  // no minimizer bins for loaded records
+    }
+    HIPCK(h, hipMemcpyAsync(h->boff.p, bucket_off, (G + 1) * 8, hipMemcpyHostToDevice, h->stream));
+    launch_bucket_features(h->boff.as<uint64_t>(), h->att.as<uint32_t>(), h->mbin.as<uint8_t>(), G, k,
+                           h->stats.as<uint64_t>(), h->feat.as<float>(), h->stream);
+    HIPCK(h, hipGetLastError());
+    h->h_boff.assign(bucket_off, bucket_off + G + 1);
+    h->h_stats.resize(size_t(G) * 4);
+    HIPCK(h, hipMemcpyAsync(h->h_stats.data(), h->stats.p, size_t(G) * 32, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    h->n_bases = 0;
+    h->n_frag = 0;
+    h->n_records = n;
+    cfgpu_run_stats& rs = h->rs;
+    rs = cfgpu_run_stats{};
+    rs.n_records = n;
+    for (uint32_t b = 0; b < G; ++b) {
+        rs.n_kmers += h->h_stats[4 * b + 1];
+        rs.n_edges += h->h_stats[4 * b + 1] - h->h_stats[4 * b + 0];
+    }
+    h->partitioned = true;
+    return CFGPU_OK;
+}
+
 int cfgpu_construct(cfgpu_handle* h, uint32_t mode) {
     if (!h) return CFGPU_E_ARG;
     if (!h->partitioned) return fail(h, CFGPU_E_STATE, "construct before partition");

@@ -163,4 +163,43 @@ int cfhost_synth_mutate(const char* base, uint64_t len, double snp_rate, double 
     return 0;
 }
 
+int cfhost_synth_bucket_workload(uint32_t n_buckets, uint64_t n_records, double zipf_s, uint32_t k,
+                                 uint32_t l, uint64_t seed, uint32_t* att, uint64_t* labels,
+                                 uint64_t* bucket_off) {
+    const uint32_t G = 16384;
+    if (!att || !labels || !bucket_off || n_buckets == 0 || n_buckets > G || k < 3 || l >= k ||
+        2 * k - l > 255)
+        return -1;
+    const uint32_t W = ((2 * (k - 1) - l + 2) + 31) / 32;
+    // record counts per bucket: largest remainder of n_records * w_b / sum(w)
+    std::vector<double> w(n_buckets);
+    double tot = 0;
+    for (uint32_t b = 0; b < n_buckets; ++b) tot += (w[b] = std::pow(double(b + 1), -zipf_s));
+    std::vector<uint64_t> cnt(n_buckets);
+    uint64_t assigned = 0;
+    for (uint32_t b = 0; b < n_buckets; ++b) assigned += (cnt[b] = uint64_t(double(n_records) * w[b] / tot));
+    for (uint32_t b = 0; assigned < n_records; b = (b + 1) % n_buckets) ++cnt[b], ++assigned;
+    bucket_off[0] = 0;
+    for (uint32_t g = 0; g < G; ++g) bucket_off[g + 1] = bucket_off[g] + (g < n_buckets ? cnt[g] : 0);
+    const uint32_t max_len = 2 * k - l;
+    std::vector<uint8_t> src;
+    for (uint32_t b = 0; b < n_buckets; ++b) {
+        Rng r(seed * 0x9E3779B97F4A7C15ull + b);
+        const uint64_t slen = std::max<uint64_t>(2 * max_len, 2 * cnt[b]);
+        src.resize(slen);
+        for (auto& x : src) x = uint8_t(r.below(4));
+        for (uint64_t i = bucket_off[b]; i < bucket_off[b + 1]; ++i) {
+            const uint32_t len = k + uint32_t(r.below(max_len - k + 1));
+            const uint64_t pos = r.below(slen - len + 1);
+            const bool ld = r.below(8) == 0, rd = r.below(8) == 0;
+            att[i] = len | (uint32_t(ld) << 8) | (uint32_t(rd) << 9) | (b << 16);
+            uint64_t* L = labels + i * W;
+            for (uint32_t j = 0; j < W; ++j) L[j] = 0;
+            for (uint32_t t = 0; t < len; ++t)
+                L[W - 1 - t / 32] |= uint64_t(src[pos + t]) << (62 - 2 * (t % 32));
+        }
+    }
+    return 0;
+}
+
 }  // extern "C"

@@ -82,6 +82,7 @@ SIGNATURES = {
                                           ctypes.c_uint32]),
     "cfgpu_records_copy": (ctypes.c_int, [ctypes.c_void_p, u32p, u64p, u64p, ctypes.c_uint64,
                                           u64p]),
+    "cfgpu_load_buckets": (ctypes.c_int, [ctypes.c_void_p, u32p, u64p, u64p]),
     "cfgpu_construct": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     "cfgpu_table_view_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32,
                                             ctypes.POINTER(TableView)]),
@@ -114,6 +115,10 @@ SIGNATURES = {
     "cfhost_synth_reads": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint64,
                                           ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64,
                                           ctypes.c_uint32, ctypes.c_void_p]),
+    "cfhost_synth_bucket_workload": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64,
+                                                    ctypes.c_double, ctypes.c_uint32,
+                                                    ctypes.c_uint32, ctypes.c_uint64, u32p, u64p,
+                                                    u64p]),
     "cfhost_synth_mutate": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_uint32, ctypes.c_uint64,
                                            ctypes.c_void_p, ctypes.c_uint64, u64p]),
@@ -229,6 +234,18 @@ def synth_mutate(base: bytes, snp_rate, indel_rate, indel_max, seed):
     return out.raw[: n.value]
 
 
+def synth_bucket_workload(n_buckets, n_records, zipf_s, k, l, seed):
+    W = ((2 * (k - 1) - l + 2) + 31) // 32
+    att = np.zeros(max(n_records, 1), np.uint32)
+    lab = np.zeros(max(n_records, 1) * W, np.uint64)
+    off = np.zeros(G + 1, np.uint64)
+    r = lib().cfhost_synth_bucket_workload(n_buckets, n_records, zipf_s, k, l, seed, _p(att, u32p),
+                                           _p(lab, u64p), _p(off, u64p))
+    if r != 0:
+        raise RuntimeError("synth_bucket_workload failed")
+    return att[:n_records], lab[: n_records * W].reshape(n_records, W), off
+
+
 # ---------------------------------------------------------------------------
 class CfgpuError(RuntimeError):
     pass
@@ -272,6 +289,17 @@ class Handle:
     def partition_device(self, packed_ptr, n_bases, frag_ptr, n_frag):
         self._check(lib().cfgpu_partition(self._h, packed_ptr, n_bases, frag_ptr, n_frag, 1),
                     "partition")
+
+    def load_buckets(self, att, labels, bucket_off):
+        a = np.ascontiguousarray(att, dtype=np.uint32)
+        lb = np.ascontiguousarray(labels, dtype=np.uint64).reshape(-1)
+        if lb.size == 0:
+            lb = np.zeros(1, np.uint64)
+        if a.size == 0:
+            a = np.zeros(1, np.uint32)
+        o = np.ascontiguousarray(bucket_off, dtype=np.uint64)
+        self._check(lib().cfgpu_load_buckets(self._h, _p(a, u32p), _p(lb, u64p), _p(o, u64p)),
+                    "load_buckets")
 
     def construct(self, mode="learned"):
         self._check(lib().cfgpu_construct(self._h, MODES.get(mode, mode)), "construct")

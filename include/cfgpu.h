@@ -102,6 +102,14 @@ int cfgpu_bucket_stats(cfgpu_handle* h, cfgpu_bucket_stat* out, uint32_t n);
 int cfgpu_records_copy(cfgpu_handle* h, uint32_t* att, uint64_t* labels, uint64_t* bucket_off,
                        uint64_t cap_records, uint64_t* n_records);
 
+/* Loads bucketed weak super k-mer records (the same layout cfgpu_records_copy returns) in
+ * place of a partition: the reader side of the reference's bucket files
+ * (Super_Kmer_Bucket::Iterator, include/Super_Kmer_Bucket.hpp:144-175).  Every record of
+ * bucket g must carry g in its attribute's high half and a length in [k, 2k-l]; bits of
+ * a label past the record length are ignored.  Host pointers. */
+int cfgpu_load_buckets(cfgpu_handle* h, const uint32_t* att, const uint64_t* labels,
+                       const uint64_t* bucket_off);
+
 /* Builds every non-empty subgraph's vertex table: Subgraph::construct
  * (src/Subgraph.cpp:44-108) for all buckets, scheduled as in Subgraphs_Manager::process
  * (src/Subgraphs_Manager.cpp:113-255).  Modes (results are identical in all modes):

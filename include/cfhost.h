@@ -57,6 +57,15 @@ int cfhost_synth_mutate(const char* base, uint64_t len, double snp_rate, double 
                         uint32_t indel_max, uint64_t seed, char* out, uint64_t cap,
                         uint64_t* out_len);
 
+/* Bucketed records for scheduling stress: n_records weak super k-mers over buckets
+ * 0..n_buckets-1 with counts proportional to (rank+1)^-zipf_s (rank = bucket id), each a
+ * substring (length k..2k-l) of a per-bucket random sequence twice as long as the
+ * bucket's record count, disc flags at random.  Layout as cfgpu_load_buckets takes:
+ * att[n_records], labels[n_records*W] (W = ceil((2k-l)/32)), bucket_off[16385]. */
+int cfhost_synth_bucket_workload(uint32_t n_buckets, uint64_t n_records, double zipf_s,
+                                 uint32_t k, uint32_t l, uint64_t seed, uint32_t* att,
+                                 uint64_t* labels, uint64_t* bucket_off);
+
 #ifdef __cplusplus
 }
 #endif

@@ -224,3 +224,34 @@ def test_repeat_calls_are_idempotent_and_release():
         assert h.table_view(int(nz[0])).capacity == 0
         assert len(h.tables(buckets=[int(nz[0])])["g"]) == 0
         assert len(h.tables(buckets=[int(nz[1])])["g"]) > 0
+
+
+def test_zipf_bucket_workload_loaded_records():
+    # Zipf(1.1)-sized buckets loaded directly (the reader side of the bucket
+    # format): the persistent queue must split the giant buckets and still
+    # produce the oracle's states.
+    for k, l in ((31, 20), (63, 20)):
+        att, lab, off = C.synth_bucket_workload(512, 60000, 1.1, k, l, seed=k)
+        say(f"[case] zipf-loaded k={k} records={len(att)} largest={int(np.diff(off.astype(np.int64)).max())}")
+        want = O.construct(k, l, off, (att & 0xFFFF).astype(np.uint16), lab)
+        with C.Handle(k, l, 1) as h:
+            h.load_buckets(att, lab, off)
+            for mode in ("plain", "batched", "learned"):
+                say(f"[case] zipf-loaded k={k} mode={mode}")
+                h.construct(mode)
+                check_tables(h, None, k, l, 1, want)
+
+
+def test_load_buckets_rejects_malformed_records():
+    att, lab, off = C.synth_bucket_workload(8, 100, 1.1, 31, 20, seed=1)
+    with C.Handle(31, 20, 1) as h:
+        bad = att.copy()
+        bad[5] = (bad[5] & 0xFFFF) | (9 << 16)         # wrong bucket id
+        with pytest.raises(C.CfgpuError):
+            h.load_buckets(bad, lab, off)
+        bad = att.copy()
+        bad[7] = (int(bad[7]) & 0xFFFFFF00) | 30         # shorter than k
+        with pytest.raises(C.CfgpuError):
+            h.load_buckets(bad, lab, off)
+        with pytest.raises(C.CfgpuError):
+            h.construct("learned")                       # nothing loaded
