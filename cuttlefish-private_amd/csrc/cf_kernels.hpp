@@ -51,7 +51,7 @@ struct ConstructArgs {
     uint32_t W, k;
     const uint64_t* bucket_off; // G+1 record offsets
     const uint64_t* tbl_ptr;    // per bucket table base (device address)
-    const uint32_t* tbl_log2;   // per bucket log2(capacity)
+    const uint64_t* tbl_cap;    // per bucket capacity (slots)
     const uint32_t* tbl_limit;  // per bucket max distinct keys before overflow
     uint32_t* n_vert;           // per bucket distinct vertex count
     uint32_t* overflow;         // per bucket overflow flag
@@ -80,7 +80,7 @@ void launch_zero_buckets(const uint32_t* list, uint32_t n, uint32_t* n_vert, uin
 // Compact the occupied slots of bucket tables into key/e_f/status/dfa arrays.
 // out_cursor[i] receives the number of occupied entries of buckets[i]; they
 // are written from out_off[i] up to (excluding) out_off[i+1].
-void launch_export(const uint64_t* tbl_ptr, const uint32_t* tbl_log2,
+void launch_export(const uint64_t* tbl_ptr, const uint64_t* tbl_cap,
                    uint32_t KW, const uint32_t* buckets, uint32_t n_buckets,
                    const uint64_t* out_off, uint32_t* out_cursor, uint64_t* keys_out,
                    uint32_t* ef_out, uint8_t* st_out, uint8_t* dfa_out, uint32_t cutoff,

@@ -60,10 +60,11 @@ uint64_t wymix_host(uint64_t a, uint64_t b) {
     return uint64_t(r) ^ uint64_t(r >> 64);
 }
 
-uint32_t ceil_log2(uint64_t x) {
-    uint32_t r = 0;
-    while ((1ull << r) < x) ++r;
-    return r;
+// Table capacities are multiples of 64 slots (any size works: the kernels
+// reduce hashes with a multiply-shift).
+uint64_t round_cap(double slots) {
+    const uint64_t c = uint64_t(std::ceil(std::max(64.0, slots)));
+    return (c + 63) & ~uint64_t(63);
 }
 
 struct Event {
@@ -90,12 +91,13 @@ struct cfgpu_handle {
     std::vector<uint64_t> h_boff, h_stats;
 
     // construct state
-    DevBuf d_tbl_ptr, d_log2, d_limit, d_nvert, d_overflow, d_items_b, d_items_f, d_items_c,
+    DevBuf d_tbl_ptr, d_cap, d_limit, d_nvert, d_overflow, d_items_b, d_items_f, d_items_c,
         d_qhead, d_round_list;
     std::vector<std::unique_ptr<DevBuf>> pools;  // one per sizing round, reused across calls
     uint32_t round_idx = 0;
     std::vector<uint64_t> h_tbl_ptr;
-    std::vector<uint32_t> h_log2, h_limit, h_nvert, h_flags;
+    std::vector<uint64_t> h_cap;
+    std::vector<uint32_t> h_limit, h_nvert, h_flags;
     std::vector<float> h_pred;
     bool constructed = false;
 
@@ -161,7 +163,7 @@ ConstructArgs construct_args(cfgpu_handle* h) {
     a.k = h->prm.k;
     a.bucket_off = h->boff.as<uint64_t>();
     a.tbl_ptr = h->d_tbl_ptr.as<uint64_t>();
-    a.tbl_log2 = h->d_log2.as<uint32_t>();
+    a.tbl_cap = h->d_cap.as<uint64_t>();
     a.tbl_limit = h->d_limit.as<uint32_t>();
     a.n_vert = h->d_nvert.as<uint32_t>();
     a.overflow = h->d_overflow.as<uint32_t>();
@@ -175,36 +177,35 @@ ConstructArgs construct_args(cfgpu_handle* h) {
 uint64_t bucket_records(const cfgpu_handle* h, uint32_t b) { return h->h_stats[4 * b + 0]; }
 uint64_t bucket_kmers(const cfgpu_handle* h, uint32_t b) { return h->h_stats[4 * b + 1]; }
 
-// Capacity (log2) that can hold every k-mer instance of the bucket at the
-// target load: a table of this size can never overflow.
-uint32_t upper_log2(const cfgpu_handle* h, uint32_t b) {
-    const double need = double(std::max<uint64_t>(bucket_kmers(h, b), 8)) / h->tune.load_factor;
-    return std::max<uint32_t>(4, ceil_log2(uint64_t(std::ceil(need))));
+// Capacity that holds every k-mer instance of the bucket at the target load:
+// such a table never overflows by load (a probe chain past MAX_PROBES is
+// still detected and rebuilt at twice the size).
+uint64_t upper_cap(const cfgpu_handle* h, uint32_t b) {
+    return round_cap(double(std::max<uint64_t>(bucket_kmers(h, b), 8)) / h->tune.load_factor);
 }
 
-uint32_t limit_for(uint32_t log2) { return uint32_t(((1ull << log2) * 7) / 8); }
+uint32_t limit_for(uint64_t cap) { return uint32_t(std::min<uint64_t>((cap * 7) / 8, 0xFFFFFFFFull)); }
 
-// One sizing round: allocate a pool for `buckets` with capacities `log2s`,
+// One sizing round: allocate a pool for `buckets` with capacities `caps`,
 // upload pointers / limits, zero their counters, clear their tables and build
 // the work-item list in the given order.  Returns the number of work items.
 int prepare_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
-                  const std::vector<uint32_t>& log2s, uint32_t& n_items, uint64_t& n_slots_out,
-                  DevBuf*& pool_out) {
+                  const std::vector<uint64_t>& caps, uint32_t& n_items, uint64_t& n_slots_out,
+                  DevBuf*& pool_out, uint32_t chunk) {
     const uint32_t eb = entry_bytes(h->KW);
     uint64_t n_slots = 0;
-    for (uint32_t l2 : log2s) n_slots += 1ull << l2;
+    for (uint64_t c : caps) n_slots += c;
     if (h->pools.size() <= h->round_idx) h->pools.emplace_back(new DevBuf());
     DevBuf* pool = h->pools[h->round_idx++].get();
     HIPCK(h, pool->ensure(n_slots * eb));
     uint64_t off = 0;
     std::vector<uint32_t> ib, ifirst, icnt;
-    const uint32_t chunk = std::max<uint32_t>(h->tune.workgroup, h->tune.chunk_records);
     for (size_t i = 0; i < buckets.size(); ++i) {
         const uint32_t b = buckets[i];
         h->h_tbl_ptr[b] = reinterpret_cast<uint64_t>(pool->as<uint8_t>() + off * eb);
-        h->h_log2[b] = log2s[i];
-        h->h_limit[b] = limit_for(log2s[i]);
-        off += 1ull << log2s[i];
+        h->h_cap[b] = caps[i];
+        h->h_limit[b] = limit_for(caps[i]);
+        off += caps[i];
         const uint64_t nr = bucket_records(h, b);
         for (uint64_t f = 0; f < nr; f += chunk) {
             ib.push_back(b);
@@ -219,7 +220,7 @@ int prepare_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
     HIPCK(h, h->d_round_list.ensure(buckets.size() * 4));
     const size_t G = h->G;
     HIPCK(h, hipMemcpyAsync(h->d_tbl_ptr.p, h->h_tbl_ptr.data(), G * 8, hipMemcpyHostToDevice, h->stream));
-    HIPCK(h, hipMemcpyAsync(h->d_log2.p, h->h_log2.data(), G * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCK(h, hipMemcpyAsync(h->d_cap.p, h->h_cap.data(), G * 8, hipMemcpyHostToDevice, h->stream));
     HIPCK(h, hipMemcpyAsync(h->d_limit.p, h->h_limit.data(), G * 4, hipMemcpyHostToDevice, h->stream));
     if (n_items) {
         HIPCK(h, hipMemcpyAsync(h->d_items_b.p, ib.data(), ib.size() * 4, hipMemcpyHostToDevice, h->stream));
@@ -265,14 +266,16 @@ int finish_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
 
 // Plain path: one launch per bucket, in bucket order.
 int run_round_plain(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
-                    const std::vector<uint32_t>& log2s, std::vector<uint32_t>& overflowed) {
+                    const std::vector<uint64_t>& caps, std::vector<uint32_t>& overflowed) {
+    // one workgroup per LDS tile of records, so each bucket's launch spreads
+    // over as many CUs as the bucket has tiles
+    const uint32_t chunk = h->tune.workgroup;
     uint32_t n_items = 0;
     uint64_t n_slots = 0;
     DevBuf* pool = nullptr;
-    CK(prepare_round(h, buckets, log2s, n_items, n_slots, pool));
+    CK(prepare_round(h, buckets, caps, n_items, n_slots, pool, chunk));
     enqueue_round_reset(h, uint32_t(buckets.size()), pool, n_slots);
     const ConstructArgs a = construct_args(h);
-    const uint32_t chunk = std::max<uint32_t>(h->tune.workgroup, h->tune.chunk_records);
     uint32_t item = 0;
     for (uint32_t b : buckets) {
         const uint32_t n = uint32_t((bucket_records(h, b) + chunk - 1) / chunk);
@@ -287,12 +290,13 @@ int run_round_plain(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
 
 // Batched path: reset + persistent work-queue launch, captured in a hipGraph.
 int run_round_batched(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
-                      const std::vector<uint32_t>& log2s, std::vector<uint32_t>& overflowed,
+                      const std::vector<uint64_t>& caps, std::vector<uint32_t>& overflowed,
                       bool use_graph) {
+    const uint32_t chunk = std::max<uint32_t>(h->tune.workgroup, h->tune.chunk_records);
     uint32_t n_items = 0;
     uint64_t n_slots = 0;
     DevBuf* pool = nullptr;
-    CK(prepare_round(h, buckets, log2s, n_items, n_slots, pool));
+    CK(prepare_round(h, buckets, caps, n_items, n_slots, pool, chunk));
     ConstructArgs a = construct_args(h);
     a.n_items = n_items;
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(n_items, uint32_t(h->cus) * h->tune.wgs_per_cu));
@@ -316,7 +320,7 @@ int run_round_batched(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
     }
     // The graph captures kernel arguments (device pointers and sizes); array
     // contents are read when it runs.  Re-capture when any captured value moves.
-    const void* ptrs[] = {pool->p, h->att.p, h->labels.p, h->boff.p, h->d_tbl_ptr.p, h->d_log2.p,
+    const void* ptrs[] = {pool->p, h->att.p, h->labels.p, h->boff.p, h->d_tbl_ptr.p, h->d_cap.p,
                           h->d_limit.p, h->d_nvert.p, h->d_overflow.p, h->d_items_b.p,
                           h->d_items_f.p, h->d_items_c.p, h->d_qhead.p, h->d_round_list.p};
     std::vector<uint64_t> key = {n_slots, n_items, grid, h->KW, h->W, h->prm.k, h->tune.workgroup,
@@ -340,10 +344,14 @@ int fallback(cfgpu_handle* h, std::vector<uint32_t> todo, bool plain_growth) {
     for (uint32_t round = 0; !todo.empty(); ++round) {
         if (round >= MAX_ROUNDS) return fail(h, CFGPU_E_OVERFLOW, "table overflow persists after %u rounds", round);
         ++h->rs.n_fallback_rounds;
-        std::vector<uint32_t> l2(todo.size());
+        std::vector<uint64_t> l2(todo.size());
         for (size_t i = 0; i < todo.size(); ++i) {
             const uint32_t b = todo[i];
-            l2[i] = plain_growth ? std::min(upper_log2(h, b), h->h_log2[b] + 2) : upper_log2(h, b);
+            // plain: grow 4x (bounded by the never-overflowing size); otherwise
+            // that size, doubled if the last attempt already had it
+            const uint64_t ub = upper_cap(h, b);
+            l2[i] = plain_growth ? std::min(ub, round_cap(4.0 * double(h->h_cap[b])))
+                                 : (h->h_cap[b] >= ub ? round_cap(2.0 * double(h->h_cap[b])) : ub);
         }
         std::vector<uint32_t> ovf;
         if (plain_growth) CK(run_round_plain(h, todo, l2, ovf));
@@ -368,13 +376,13 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
     const size_t G = h->G;
     h->round_idx = 0;
     h->h_tbl_ptr.assign(G, 0);
-    h->h_log2.assign(G, 0);
+    h->h_cap.assign(G, 0);
     h->h_limit.assign(G, 0);
     h->h_nvert.assign(G, 0);
     h->h_flags.assign(G, 0);
     h->h_pred.assign(G, 0.f);
     HIPCK(h, h->d_tbl_ptr.ensure(G * 8));
-    HIPCK(h, h->d_log2.ensure(G * 4));
+    HIPCK(h, h->d_cap.ensure(G * 8));
     HIPCK(h, h->d_limit.ensure(G * 4));
     HIPCK(h, h->d_nvert.ensure(G * 4));
     HIPCK(h, h->d_overflow.ensure(G * 4));
@@ -395,7 +403,7 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
 
     std::vector<uint32_t> ovf;
     std::vector<uint32_t> main_set = all;
-    std::vector<uint32_t> main_l2(all.size());
+    std::vector<uint64_t> main_l2(all.size());
     std::vector<double> cost(G, 0.0);
     for (uint32_t b : all) cost[b] = double(bucket_kmers(h, b));
 
@@ -404,7 +412,7 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
         for (size_t i = 0; i < all.size(); ++i) {
             const uint32_t b = all[i];
             const double guess = double(std::max<uint64_t>(bucket_kmers(h, b) / 4, 8)) / h->tune.load_factor;
-            main_l2[i] = std::min(upper_log2(h, b), std::max<uint32_t>(4, ceil_log2(uint64_t(guess))));
+            main_l2[i] = std::min(upper_cap(h, b), round_cap(guess));
         }
         HIPCK(h, hipEventRecord(e2.e, h->stream));
         CK(run_round_plain(h, main_set, main_l2, ovf));
@@ -413,7 +421,7 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
         CK(fallback(h, ovf, true));
     } else if (mode == CFGPU_MODE_BATCHED) {
         order_largest_first(main_set, cost);
-        for (size_t i = 0; i < main_set.size(); ++i) main_l2[i] = upper_log2(h, main_set[i]);
+        for (size_t i = 0; i < main_set.size(); ++i) main_l2[i] = upper_cap(h, main_set[i]);
         HIPCK(h, hipEventRecord(e2.e, h->stream));
         CK(run_round_batched(h, main_set, main_l2, ovf, true));
         HIPCK(h, hipEventRecord(e3.e, h->stream));
@@ -434,10 +442,9 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
         }
         rs.n_samples = uint32_t(sample.size());
         if (!sample.empty()) {
-            std::vector<uint32_t> l2(sample.size());
-            for (size_t i = 0; i < sample.size(); ++i) l2[i] = upper_log2(h, sample[i]);
+            std::vector<uint64_t> l2(sample.size());
             order_largest_first(sample, cost);
-            for (size_t i = 0; i < sample.size(); ++i) l2[i] = upper_log2(h, sample[i]);
+            for (size_t i = 0; i < sample.size(); ++i) l2[i] = upper_cap(h, sample[i]);
             std::vector<uint32_t> so;
             CK(run_round_batched(h, sample, l2, so, false));
             CK(fallback(h, so, false));
@@ -461,7 +468,7 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
             HIPCK(h, hipMemcpyAsync(h->mlp_idx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, h->stream));
             HIPCK(h, hipMemcpyAsync(h->mlp_target.p, tgt.data(), tgt.size() * 4, hipMemcpyHostToDevice, h->stream));
             launch_mlp_train(h->mlp_params.as<float>(), h->feat.as<float>(), h->mlp_idx.as<uint32_t>(),
-                             h->mlp_target.as<float>(), uint32_t(idx.size()), 300, 3e-3f, 12345u,
+                             h->mlp_target.as<float>(), uint32_t(idx.size()), 120, 3e-3f, 12345u,
                              h->mlp_adam.as<float>(), h->mlp_loss.as<float>(), h->stream);
             HIPCK(h, hipEventRecord(e4.e, h->stream));
             launch_mlp_infer(h->mlp_params.as<float>(), h->feat.as<float>(), uint32_t(G),
@@ -491,11 +498,11 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
         for (size_t i = 0; i < main_set.size(); ++i) {
             const uint32_t b = main_set[i];
             if (!have_model) {
-                main_l2[i] = upper_log2(h, b);
+                main_l2[i] = upper_cap(h, b);
                 continue;
             }
             const double need = double(h->h_pred[b]) * h->tune.margin / h->tune.load_factor;
-            main_l2[i] = std::min(upper_log2(h, b), std::max<uint32_t>(4, ceil_log2(uint64_t(std::ceil(need)))));
+            main_l2[i] = std::min(upper_cap(h, b), round_cap(need));
         }
         HIPCK(h, hipEventRecord(e2.e, h->stream));
         CK(run_round_batched(h, main_set, main_l2, ovf, true));
@@ -531,7 +538,7 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
     rs.table_slots = 0;
     for (uint32_t b : all) {
         rs.n_vertices += h->h_nvert[b];
-        rs.table_slots += 1ull << h->h_log2[b];
+        rs.table_slots += h->h_cap[b];
         if (!(h->h_flags[b] & CFGPU_BUCKET_BUILT))
             return fail(h, CFGPU_E_OVERFLOW, "bucket %u was not built", b);
     }
@@ -706,7 +713,7 @@ int cfgpu_bucket_stats(cfgpu_handle* h, cfgpu_bucket_stat* out, uint32_t n) {
         s.n_bases = h->h_stats[4 * b + 2];
         const bool c = h->constructed && s.n_records;
         s.n_vertices = c ? h->h_nvert[b] : 0;
-        s.capacity = c ? (1ull << h->h_log2[b]) : 0;
+        s.capacity = c ? h->h_cap[b] : 0;
         s.predicted = c ? h->h_pred[b] : 0.f;
         s.flags = c ? h->h_flags[b] : 0;
     }
@@ -793,7 +800,7 @@ int cfgpu_table_view_get(cfgpu_handle* h, uint32_t g, cfgpu_table_view* out) {
     if (!h || !out || g >= h->G) return CFGPU_E_ARG;
     if (!h->constructed) return fail(h, CFGPU_E_STATE, "table view before construct");
     out->entries = reinterpret_cast<const void*>(h->h_tbl_ptr[g]);
-    out->capacity = h->h_tbl_ptr[g] ? (1ull << h->h_log2[g]) : 0;
+    out->capacity = h->h_tbl_ptr[g] ? h->h_cap[g] : 0;
     out->entry_bytes = entry_bytes(h->KW);
     out->key_words = h->KW;
     out->n_vertices = h->h_nvert[g];
@@ -842,7 +849,7 @@ int cfgpu_table_copy(cfgpu_handle* h, const uint32_t* buckets, uint32_t n_bucket
         HIPCK(h, hipMemcpyAsync(h->ex_off.p, ex_o.data(), ex_o.size() * 8, hipMemcpyHostToDevice, h->stream));
         HIPCK(h, hipMemcpyAsync(h->ex_list.p, ex_b.data(), ex_b.size() * 4, hipMemcpyHostToDevice, h->stream));
         launch_fill_u32(h->ex_cursor.as<uint32_t>(), 0u, ex_b.size(), h->stream);
-        launch_export(h->d_tbl_ptr.as<uint64_t>(), h->d_log2.as<uint32_t>(), KW, h->ex_list.as<uint32_t>(),
+        launch_export(h->d_tbl_ptr.as<uint64_t>(), h->d_cap.as<uint64_t>(), KW, h->ex_list.as<uint32_t>(),
                       uint32_t(ex_b.size()), h->ex_off.as<uint64_t>(), h->ex_cursor.as<uint32_t>(),
                       h->ex_keys.as<uint64_t>(), h->ex_ef.as<uint32_t>(), h->ex_st.as<uint8_t>(),
                       h->ex_dfa.as<uint8_t>(), h->prm.cutoff, h->stream);

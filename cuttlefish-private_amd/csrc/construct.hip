@@ -30,11 +30,20 @@ namespace {
 constexpr uint64_t EMPTY = ~0ull;
 constexpr uint64_t BUSY = 1ull << 63;
 constexpr uint32_t MAX_SPINS = 1u << 22;
+constexpr uint32_t MAX_PROBES = 1024;  // longer probe chains count as overflow
 constexpr int W_MAX = 8;  // label words: ceil((2k - l)/32) <= 8 for k <= 127
 
+// Table memory is addressed through global-address-space pointers so probes
+// and atomics compile to global_* (not flat_*) instructions.
+typedef __attribute__((address_space(1))) uint8_t g_u8;
+typedef __attribute__((address_space(1))) uint64_t g_u64;
+typedef __attribute__((address_space(1))) uint32_t g_u32;
+typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
+typedef __attribute__((address_space(1))) u32x4 g_u32x4;
+
 template <int KW>
-__device__ __forceinline__ uint64_t* slot_ptr(uint8_t* tbl, uint64_t s) {
-    return reinterpret_cast<uint64_t*>(tbl + s * uint64_t(8 * KW + 8));
+__device__ __forceinline__ g_u64* slot_ptr(g_u8* tbl, uint64_t s) {
+    return reinterpret_cast<g_u64*>(tbl + s * uint64_t(8 * KW + 8));
 }
 
 __device__ __forceinline__ uint64_t ld_relaxed(const uint64_t* p) {
@@ -43,65 +52,82 @@ __device__ __forceinline__ uint64_t ld_relaxed(const uint64_t* p) {
 __device__ __forceinline__ uint32_t ld_relaxed32(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ uint64_t ld_coherent(g_u64* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool cas64(g_u64* p, uint64_t& expected, uint64_t desired) {
+    return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED,
+                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
-// Returns the slot holding `key` (inserting it if absent), or -1 when the table
-// has no room (the caller flags the bucket as overflowed).
+// Home slot of a hash in a table of `cap` slots (multiply-shift range
+// reduction, so capacities need not be powers of two).
+__device__ __forceinline__ uint64_t home_slot(uint64_t h, uint64_t cap) { return __umul64hi(h, cap); }
+
+struct Probe {
+    int64_t slot;      // -1: no room within MAX_PROBES
+    uint32_t ef_seen;  // Edge_Frequency word as loaded (may lag the true value)
+    bool inserted;
+};
+
+// Finds `key`'s slot, inserting the key if absent.  Plain loads may return a
+// stale line: a slot's key only ever changes from EMPTY to a key, and counters
+// only grow, so a stale read can only show EMPTY (the CAS then returns the
+// truth) or a smaller counter word (the counter CAS then fails and retries).
 template <int KW>
-__device__ int64_t find_or_insert(uint8_t* tbl, uint64_t mask, const Kmer<KW>& key,
-                                  bool& inserted) {
-    uint64_t s = km_hash(key) & mask;
-    inserted = false;
+__device__ Probe find_or_insert(g_u8* tbl, uint64_t cap, const Kmer<KW>& key) {
+    uint64_t s = home_slot(km_hash(key), cap);
+    const uint64_t lim = cap < MAX_PROBES ? cap : MAX_PROBES;
     if constexpr (KW == 1) {
-        for (uint64_t n = 0; n <= mask; ++n, s = (s + 1) & mask) {
-            uint64_t* kp = slot_ptr<1>(tbl, s);
-            const uint64_t cur = ld_relaxed(kp);
-            if (cur == key.w[0]) return int64_t(s);
+        for (uint64_t n = 0; n < lim; ++n) {
+            g_u64* kp = slot_ptr<1>(tbl, s);
+            const u32x4 e = *reinterpret_cast<g_u32x4*>(kp);  // key, e_f, status
+            const uint64_t cur = uint64_t(e.x) | (uint64_t(e.y) << 32);
+            if (cur == key.w[0]) return {int64_t(s), e.z, false};
             if (cur == EMPTY) {
-                const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(kp), EMPTY,
-                                               key.w[0]);
-                if (old == EMPTY) {
-                    inserted = true;
-                    return int64_t(s);
-                }
-                if (old == key.w[0]) return int64_t(s);
+                uint64_t exp = EMPTY;
+                if (cas64(kp, exp, key.w[0])) return {int64_t(s), 0u, true};
+                if (exp == key.w[0]) return {int64_t(s), 0u, false};
             }
+            s = s + 1 == cap ? 0 : s + 1;
         }
-        return -1;
+        return {-1, 0u, false};
     } else {
         const uint64_t top = key.w[KW - 1];
         uint32_t spins = 0;
         uint64_t n = 0;
-        while (n <= mask) {
-            uint64_t* kp = slot_ptr<KW>(tbl, s);
-            uint64_t cur = ld_relaxed(kp + KW - 1);
+        bool fresh = false;  // re-read a claimed slot coherently
+        while (n < lim) {
+            g_u64* kp = slot_ptr<KW>(tbl, s);
+            uint64_t cur = fresh ? ld_coherent(kp + KW - 1) : kp[KW - 1];
+            fresh = false;
             if (cur == EMPTY) {
-                const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(kp + KW - 1),
-                                               EMPTY, top | BUSY);
-                if (old == EMPTY) {
+                uint64_t exp = EMPTY;
+                if (cas64(kp + KW - 1, exp, top | BUSY)) {
 #pragma unroll
                     for (int i = 0; i < KW - 1; ++i)
                         __hip_atomic_store(kp + i, key.w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     __threadfence();
                     __hip_atomic_store(kp + KW - 1, top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    inserted = true;
-                    return int64_t(s);
+                    return {int64_t(s), 0u, true};
                 }
-                cur = old;
+                cur = exp;
             }
             if ((cur & ~BUSY) == top) {
                 if (cur & BUSY) {
-                    if (++spins > MAX_SPINS) return -1;
+                    if (++spins > MAX_SPINS) return {-1, 0u, false};
+                    fresh = true;
                     continue;  // the claimer publishes within its own iteration
                 }
                 bool eq = true;
 #pragma unroll
-                for (int i = 0; i < KW - 1; ++i) eq &= ld_relaxed(kp + i) == key.w[i];
-                if (eq) return int64_t(s);
+                for (int i = 0; i < KW - 1; ++i) eq &= ld_coherent(kp + i) == key.w[i];
+                if (eq) return {int64_t(s), uint32_t(kp[KW]), false};
             }
             ++n;
-            s = (s + 1) & mask;
+            s = s + 1 == cap ? 0 : s + 1;
         }
-        return -1;
+        return {-1, 0u, false};
     }
 }
 
@@ -146,12 +172,19 @@ __device__ void construct_item(const ConstructArgs& a, uint32_t item, uint32_t* 
     const uint32_t first = a.item_first[item], cnt = a.item_count[item];
     const uint32_t W = a.W, k = a.k;
     const uint64_t rbase = a.bucket_off[b] + first;
-    uint8_t* tbl = reinterpret_cast<uint8_t*>(a.tbl_ptr[b]);
-    const uint64_t mask = (1ull << a.tbl_log2[b]) - 1;
+    g_u8* tbl = reinterpret_cast<g_u8*>(a.tbl_ptr[b]);
+    const uint64_t cap = a.tbl_cap[b];
     const uint32_t limit = a.tbl_limit[b];
     const int lane = tid & 63;
 
     for (uint32_t r0 = 0; r0 < cnt; r0 += WG) {
+        if (r0) {  // stop early once the bucket is known to overflow
+            if (tid == 0) *s_flag = ld_relaxed32(&a.overflow[b]);
+            __syncthreads();
+            const uint32_t ovf = *s_flag;
+            __syncthreads();
+            if (ovf) return;
+        }
         const uint32_t ns = min(uint32_t(WG), cnt - r0);
         uint32_t kc = 0;
         if (uint32_t(tid) < ns) {
@@ -196,24 +229,25 @@ __device__ void construct_item(const ConstructArgs& a, uint32_t item, uint32_t* 
             if (i == 0 && ((at >> 8) & 1u)) disc |= is_can ? 2u : 4u;
             if (i + k == len && ((at >> 9) & 1u)) disc |= is_can ? 4u : 2u;
 
-            bool inserted = false;
-            const int64_t slot = find_or_insert<KW>(tbl, mask, is_can ? v : vb, inserted);
-            if (slot < 0) {
+            const Probe pr = find_or_insert<KW>(tbl, cap, is_can ? v : vb);
+            if (pr.slot < 0) {
                 atomicOr(&a.overflow[b], 1u);
             } else {
-                uint32_t* ef = reinterpret_cast<uint32_t*>(slot_ptr<KW>(tbl, uint64_t(slot)) + KW);
-                uint32_t old = ld_relaxed32(ef);
+                g_u32* ef = reinterpret_cast<g_u32*>(slot_ptr<KW>(tbl, uint64_t(pr.slot)) + KW);
+                // Saturating update; counters never decrease, so a (possibly
+                // stale) word that is already saturated needs no atomic.
+                uint32_t old = pr.ef_seen;
                 while (true) {
                     const uint32_t nw = ef_add(old, front, back);
                     if (nw == old) break;
-                    const uint32_t prev = atomicCAS(ef, old, nw);
-                    if (prev == old) break;
-                    old = prev;
+                    if (__hip_atomic_compare_exchange_strong(ef, &old, nw, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT))
+                        break;
                 }
-                if (disc) atomicOr(ef + 1, disc);
+                if (disc) __hip_atomic_fetch_or(ef + 1, disc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             // distinct-key accounting: one atomic per wavefront
-            const unsigned long long m = __ballot(inserted);
+            const unsigned long long m = __ballot(pr.inserted);
             if (m) {
                 const int leader = __ffsll((long long)m) - 1;
                 if (lane == leader) {
@@ -252,7 +286,7 @@ template <int KW>
 __global__ __launch_bounds__(256) void k_clear(uint8_t* tables, uint64_t n) {
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-        uint64_t* p = slot_ptr<KW>(tables, i);
+        g_u64* p = slot_ptr<KW>((g_u8*)(tables), i);
 #pragma unroll
         for (int j = 0; j < KW; ++j) p[j] = EMPTY;
         p[KW] = 0;  // e_f = 0, status = 0
@@ -284,14 +318,14 @@ __device__ __forceinline__ uint32_t edge_at(uint32_t e_f, unsigned off, uint32_t
 
 template <int KW>
 __global__ __launch_bounds__(256) void k_export(const uint64_t* tbl_ptr,
-                                                const uint32_t* tbl_log2, const uint32_t* buckets,
+                                                const uint64_t* tbl_cap, const uint32_t* buckets,
                                                 const uint64_t* out_off, uint32_t* out_cursor,
                                                 uint64_t* keys_out, uint32_t* ef_out,
                                                 uint8_t* st_out, uint8_t* dfa_out,
                                                 uint32_t cutoff) {
     const uint32_t bi = blockIdx.y;
     const uint32_t b = buckets[bi];
-    const uint64_t cap = 1ull << tbl_log2[b];
+    const uint64_t cap = tbl_cap[b];
     const uint8_t* tbl = reinterpret_cast<const uint8_t*>(tbl_ptr[b]);
     const int lane = threadIdx.x & 63;
     for (uint64_t s = uint64_t(blockIdx.x) * 256 + threadIdx.x; s - threadIdx.x < cap;
@@ -376,17 +410,17 @@ void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(256), 0, s, p, v, n);
 }
 
-void launch_export(const uint64_t* tbl_ptr, const uint32_t* tbl_log2,
+void launch_export(const uint64_t* tbl_ptr, const uint64_t* tbl_cap,
                    uint32_t KW, const uint32_t* buckets, uint32_t n_buckets, const uint64_t* out_off,
                    uint32_t* out_cursor, uint64_t* keys_out, uint32_t* ef_out, uint8_t* st_out,
                    uint8_t* dfa_out, uint32_t cutoff, hipStream_t s) {
     if (!n_buckets) return;
     const dim3 grid(64, n_buckets);
     switch (KW) {
-        case 1: hipLaunchKernelGGL(k_export<1>, grid, dim3(256), 0, s, tbl_ptr, tbl_log2, buckets, out_off, out_cursor, keys_out, ef_out, st_out, dfa_out, cutoff); break;
-        case 2: hipLaunchKernelGGL(k_export<2>, grid, dim3(256), 0, s, tbl_ptr, tbl_log2, buckets, out_off, out_cursor, keys_out, ef_out, st_out, dfa_out, cutoff); break;
-        case 3: hipLaunchKernelGGL(k_export<3>, grid, dim3(256), 0, s, tbl_ptr, tbl_log2, buckets, out_off, out_cursor, keys_out, ef_out, st_out, dfa_out, cutoff); break;
-        default: hipLaunchKernelGGL(k_export<4>, grid, dim3(256), 0, s, tbl_ptr, tbl_log2, buckets, out_off, out_cursor, keys_out, ef_out, st_out, dfa_out, cutoff); break;
+        case 1: hipLaunchKernelGGL(k_export<1>, grid, dim3(256), 0, s, tbl_ptr, tbl_cap, buckets, out_off, out_cursor, keys_out, ef_out, st_out, dfa_out, cutoff); break;
+        case 2: hipLaunchKernelGGL(k_export<2>, grid, dim3(256), 0, s, tbl_ptr, tbl_cap, buckets, out_off, out_cursor, keys_out, ef_out, st_out, dfa_out, cutoff); break;
+        case 3: hipLaunchKernelGGL(k_export<3>, grid, dim3(256), 0, s, tbl_ptr, tbl_cap, buckets, out_off, out_cursor, keys_out, ef_out, st_out, dfa_out, cutoff); break;
+        default: hipLaunchKernelGGL(k_export<4>, grid, dim3(256), 0, s, tbl_ptr, tbl_cap, buckets, out_off, out_cursor, keys_out, ef_out, st_out, dfa_out, cutoff); break;
     }
 }
 

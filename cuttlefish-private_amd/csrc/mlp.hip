@@ -75,28 +75,38 @@ __global__ __launch_bounds__(TW) void k_mlp_train(float* __restrict__ gparams,
     for (int i = tid; i < int(n) * F; i += TW) X[i] = (X[i] - P[OMEAN + i % F]) * P[OISTD + i % F];
     __syncthreads();
 
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    // Output bias starts at the mean target: the network then fits deviations
+    // from the average distinct/instance ratio, and weight decay keeps those
+    // small when the training buckets do not support them.
+    if (tid == 0) {
+        float m = 0.f;
+        for (uint32_t s = 0; s < n; ++s) m += target[s];
+        P[OB3] = n ? m / float(n) : 0.f;
+    }
+    __syncthreads();
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f, wd = 1e-3f;
     float b1t = 1.f, b2t = 1.f;
+    const int NH = int(n) * H;
     for (uint32_t ep = 0; ep < epochs; ++ep) {
-        // forward + output gradient (sample-parallel)
+        // forward, (sample, neuron)-parallel
+        for (int q = tid; q < NH; q += TW) {
+            const int sm = q / H, j = q % H;
+            float acc = P[OB1 + j];
+            for (int f = 0; f < F; ++f) acc += P[OW1 + j * F + f] * X[sm * F + f];
+            H1[q] = acc > 0.f ? acc : 0.f;
+        }
+        __syncthreads();
+        for (int q = tid; q < NH; q += TW) {
+            const int sm = q / H, j = q % H;
+            float acc = P[OB2 + j];
+            for (int i = 0; i < H; ++i) acc += P[OW2 + j * H + i] * H1[sm * H + i];
+            B[q] = acc > 0.f ? acc : 0.f;
+        }
+        __syncthreads();
         float my_loss = 0.f;
         if (uint32_t(tid) < n) {
-            const float* x = X + tid * F;
-            float* h1 = H1 + tid * H;
-            float* h2 = B + tid * H;
-            for (int j = 0; j < H; ++j) {
-                float acc = P[OB1 + j];
-                for (int f = 0; f < F; ++f) acc += P[OW1 + j * F + f] * x[f];
-                h1[j] = acc > 0.f ? acc : 0.f;
-            }
             float y = P[OB3];
-            for (int j = 0; j < H; ++j) {
-                float acc = P[OB2 + j];
-                for (int i = 0; i < H; ++i) acc += P[OW2 + j * H + i] * h1[i];
-                acc = acc > 0.f ? acc : 0.f;
-                h2[j] = acc;
-                y += P[OW3 + j] * acc;
-            }
+            for (int j = 0; j < H; ++j) y += P[OW3 + j] * B[tid * H + j];
             const float d = y - target[tid];
             my_loss = d * d;
             dy[tid] = 2.f * d / float(n);
@@ -105,39 +115,33 @@ __global__ __launch_bounds__(TW) void k_mlp_train(float* __restrict__ gparams,
         // dw3, db3 (weight-parallel)
         for (int j = tid; j < H + 1; j += TW) {
             float g = 0.f;
-            for (uint32_t s = 0; s < n; ++s) g += dy[s] * (j < H ? B[s * H + j] : 1.f);
+            for (uint32_t sm = 0; sm < n; ++sm) g += dy[sm] * (j < H ? B[sm * H + j] : 1.f);
             Gr[j < H ? OW3 + j : OB3] = g;
         }
         __syncthreads();
-        // dh2 in place of h2 (sample-parallel)
-        if (uint32_t(tid) < n) {
-            float* h2 = B + tid * H;
-            for (int j = 0; j < H; ++j) h2[j] = h2[j] > 0.f ? dy[tid] * P[OW3 + j] : 0.f;
-        }
+        // dh2 in place of h2
+        for (int q = tid; q < NH; q += TW) B[q] = B[q] > 0.f ? dy[q / H] * P[OW3 + q % H] : 0.f;
         __syncthreads();
         // dW2, db2
         for (int w = tid; w < H * H + H; w += TW) {
             float g = 0.f;
             if (w < H * H) {
                 const int j = w / H, i = w % H;
-                for (uint32_t s = 0; s < n; ++s) g += B[s * H + j] * H1[s * H + i];
+                for (uint32_t sm = 0; sm < n; ++sm) g += B[sm * H + j] * H1[sm * H + i];
                 Gr[OW2 + w] = g;
             } else {
                 const int j = w - H * H;
-                for (uint32_t s = 0; s < n; ++s) g += B[s * H + j];
+                for (uint32_t sm = 0; sm < n; ++sm) g += B[sm * H + j];
                 Gr[OB2 + j] = g;
             }
         }
         __syncthreads();
-        // dh1 in place of h1
-        if (uint32_t(tid) < n) {
-            float* h1 = H1 + tid * H;
-            const float* d2 = B + tid * H;
-            for (int i = 0; i < H; ++i) {
-                float acc = 0.f;
-                for (int j = 0; j < H; ++j) acc += d2[j] * P[OW2 + j * H + i];
-                h1[i] = h1[i] > 0.f ? acc : 0.f;
-            }
+        // dh1 in place of h1 (each (sample, i) reads only its own h1)
+        for (int q = tid; q < NH; q += TW) {
+            const int sm = q / H, i = q % H;
+            float acc = 0.f;
+            for (int j = 0; j < H; ++j) acc += B[sm * H + j] * P[OW2 + j * H + i];
+            H1[q] = H1[q] > 0.f ? acc : 0.f;
         }
         __syncthreads();
         // dW1, db1
@@ -145,16 +149,16 @@ __global__ __launch_bounds__(TW) void k_mlp_train(float* __restrict__ gparams,
             float g = 0.f;
             if (w < H * F) {
                 const int i = w / F, f = w % F;
-                for (uint32_t s = 0; s < n; ++s) g += H1[s * H + i] * X[s * F + f];
+                for (uint32_t sm = 0; sm < n; ++sm) g += H1[sm * H + i] * X[sm * F + f];
                 Gr[OW1 + w] = g;
             } else {
                 const int i = w - H * F;
-                for (uint32_t s = 0; s < n; ++s) g += H1[s * H + i];
+                for (uint32_t sm = 0; sm < n; ++sm) g += H1[sm * H + i];
                 Gr[OB1 + i] = g;
             }
         }
         __syncthreads();
-        // Adam
+        // Adam with decoupled weight decay on the weight matrices
         b1t *= b1;
         b2t *= b2;
         for (int i = tid; i < MLP_PARAMS; i += TW) {
@@ -163,9 +167,9 @@ __global__ __launch_bounds__(TW) void k_mlp_train(float* __restrict__ gparams,
             const float v = b2 * adam[MLP_PARAMS + i] + (1.f - b2) * g * g;
             adam[i] = m;
             adam[MLP_PARAMS + i] = v;
-            P[i] -= lr * (m / (1.f - b1t)) / (sqrtf(v / (1.f - b2t)) + eps);
+            const bool is_w = i < OB1 || (i >= OW2 && i < OB2) || (i >= OW3 && i < OB3);
+            P[i] -= lr * ((m / (1.f - b1t)) / (sqrtf(v / (1.f - b2t)) + eps) + (is_w ? wd * P[i] : 0.f));
         }
-        // loss (last epoch only)
         if (ep + 1 == epochs) {
             float v = my_loss;
             for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
