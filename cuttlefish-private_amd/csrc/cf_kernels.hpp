@@ -72,6 +72,12 @@ void launch_construct(const ConstructArgs& a, uint32_t KW, uint32_t item0, uint3
 // Persistent: `grid` workgroups pull items from queue_head until exhausted.
 void launch_construct_persistent(const ConstructArgs& a, uint32_t KW, uint32_t grid, uint32_t wg,
                                  hipStream_t s);
+// LDS-resident construct: one work item = one hash pass of one bucket
+// (item_first = pass index, item_count = number of passes P; the bucket's
+// table holds P * lds_capacity(KW) slots).  Persistent over queue_head.
+constexpr int LDS_WG = 512;
+constexpr int lds_capacity(int KW) { return KW == 1 ? 8192 : KW == 2 ? 5120 : KW == 3 ? 4096 : 3072; }
+void launch_construct_lds(const ConstructArgs& a, uint32_t KW, uint32_t grid, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 // n_vert[b] = overflow[b] = 0 for every b in list[0..n).
 void launch_zero_buckets(const uint32_t* list, uint32_t n, uint32_t* n_vert, uint32_t* overflow,

@@ -26,6 +26,7 @@ namespace {
 constexpr uint32_t G_FIXED = 16384;
 constexpr uint64_t SAMPLE_TARGET = MLP_TRAIN_MAX;
 constexpr uint32_t MAX_ROUNDS = 8;
+constexpr double LDS_LOAD = 0.7;  // target fill of an LDS pass table (overflow at 7/8)
 
 struct DevBuf {
     void* p = nullptr;
@@ -82,7 +83,7 @@ struct cfgpu_handle {
     int cus = 256;
     hipStream_t stream = nullptr;
     std::string err;
-    cfgpu_tuning tune{0.6f, 256, 2048, 8, 1.15f, 1};
+    cfgpu_tuning tune{0.6f, 256, 2048, 8, 1.15f, 1, 32};
 
     // partition state
     DevBuf packed, frag, gbin, tile_lc, bcnt, boff, cursor, att, labels, mbin, stats, feat;
@@ -191,7 +192,7 @@ uint32_t limit_for(uint64_t cap) { return uint32_t(std::min<uint64_t>((cap * 7) 
 // the work-item list in the given order.  Returns the number of work items.
 int prepare_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
                   const std::vector<uint64_t>& caps, uint32_t& n_items, uint64_t& n_slots_out,
-                  DevBuf*& pool_out, uint32_t chunk) {
+                  DevBuf*& pool_out, uint32_t chunk, uint32_t lds_cap = 0) {
     const uint32_t eb = entry_bytes(h->KW);
     uint64_t n_slots = 0;
     for (uint64_t c : caps) n_slots += c;
@@ -206,6 +207,11 @@ int prepare_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
         h->h_cap[b] = caps[i];
         h->h_limit[b] = limit_for(caps[i]);
         off += caps[i];
+        if (lds_cap) {  // one item per hash pass: (bucket, pass, passes)
+            const uint32_t P = uint32_t(caps[i] / lds_cap);
+            for (uint32_t q = 0; q < P; ++q) ib.push_back(b), ifirst.push_back(q), icnt.push_back(P);
+            continue;
+        }
         const uint64_t nr = bucket_records(h, b);
         for (uint64_t f = 0; f < nr; f += chunk) {
             ib.push_back(b);
@@ -289,33 +295,43 @@ int run_round_plain(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
 }
 
 // Batched path: reset + persistent work-queue launch, captured in a hipGraph.
+// Batched path: one persistent work-queue launch for the round, optionally
+// captured in a hipGraph.  lds = true builds each bucket in LDS (caps are
+// passes * lds_capacity, one work item per pass); otherwise record chunks
+// insert into HBM tables with global atomics.
 int run_round_batched(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
                       const std::vector<uint64_t>& caps, std::vector<uint32_t>& overflowed,
-                      bool use_graph) {
+                      bool use_graph, bool lds = false) {
     const uint32_t chunk = std::max<uint32_t>(h->tune.workgroup, h->tune.chunk_records);
+    const uint32_t C = uint32_t(lds_capacity(int(h->KW)));
     uint32_t n_items = 0;
     uint64_t n_slots = 0;
     DevBuf* pool = nullptr;
-    CK(prepare_round(h, buckets, caps, n_items, n_slots, pool, chunk));
+    CK(prepare_round(h, buckets, caps, n_items, n_slots, pool, chunk, lds ? C : 0));
     ConstructArgs a = construct_args(h);
     a.n_items = n_items;
-    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(n_items, uint32_t(h->cus) * h->tune.wgs_per_cu));
-    auto enqueue = [&]() {
+    const uint32_t per_cu = lds ? 1u : h->tune.wgs_per_cu;  // the LDS kernel fills a CU's LDS
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(n_items, uint32_t(h->cus) * per_cu));
+    auto reset = [&]() {
         launch_fill_u32(h->d_qhead.as<uint32_t>(), 0u, 1, h->stream);
-        enqueue_round_reset(h, uint32_t(buckets.size()), pool, n_slots);
-        launch_construct_persistent(a, h->KW, grid, h->tune.workgroup, h->stream);
+        launch_zero_buckets(h->d_round_list.as<uint32_t>(), uint32_t(buckets.size()), h->d_nvert.as<uint32_t>(),
+                            h->d_overflow.as<uint32_t>(), h->stream);
+        if (!lds) launch_clear_tables(pool->as<uint8_t>(), n_slots, h->KW, h->stream);  // LDS passes write every slot
+    };
+    auto kernel = [&]() {
+        if (lds) launch_construct_lds(a, h->KW, grid, h->stream);
+        else launch_construct_persistent(a, h->KW, grid, h->tune.workgroup, h->stream);
     };
     if (!use_graph || !h->tune.use_graph) {
         // direct launches; the construct kernel alone is timed with events
         Event k0, k1;
-        launch_fill_u32(h->d_qhead.as<uint32_t>(), 0u, 1, h->stream);
-        enqueue_round_reset(h, uint32_t(buckets.size()), pool, n_slots);
+        reset();
         HIPCK(h, hipEventRecord(k0.e, h->stream));
-        launch_construct_persistent(a, h->KW, grid, h->tune.workgroup, h->stream);
+        kernel();
         HIPCK(h, hipEventRecord(k1.e, h->stream));
         HIPCK(h, hipGetLastError());
         const int r = finish_round(h, buckets, overflowed);
-        if (use_graph) h->rs.t_construct_kernel_ms = elapsed_ms(k0, k1);
+        if (use_graph) h->rs.t_construct_kernel_ms += elapsed_ms(k0, k1);
         return r;
     }
     // The graph captures kernel arguments (device pointers and sizes); array
@@ -324,19 +340,48 @@ int run_round_batched(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
                           h->d_limit.p, h->d_nvert.p, h->d_overflow.p, h->d_items_b.p,
                           h->d_items_f.p, h->d_items_c.p, h->d_qhead.p, h->d_round_list.p};
     std::vector<uint64_t> key = {n_slots, n_items, grid, h->KW, h->W, h->prm.k, h->tune.workgroup,
-                                 buckets.size()};
+                                 buckets.size(), uint64_t(lds)};
     for (const void* q : ptrs) key.push_back(reinterpret_cast<uint64_t>(q));
     if (!h->graph_exec || key != h->graph_key) {
         if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec), h->graph_exec = nullptr;
         if (h->graph) (void)hipGraphDestroy(h->graph), h->graph = nullptr;
         HIPCK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        enqueue();
+        reset();
+        kernel();
         HIPCK(h, hipStreamEndCapture(h->stream, &h->graph));
         HIPCK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
         h->graph_key = key;
     }
     HIPCK(h, hipGraphLaunch(h->graph_exec, h->stream));
     return finish_round(h, buckets, overflowed);
+}
+
+// Split a set of buckets (with distinct-count estimates) between the LDS
+// kernel (estimated passes <= max_passes) and the global-atomic kernel, run
+// both, and collect the overflowed buckets.
+int run_main(cfgpu_handle* h, const std::vector<uint32_t>& order, const std::vector<double>& est,
+             double margin, std::vector<uint32_t>& overflowed) {
+    const double C = double(lds_capacity(int(h->KW)));
+    std::vector<uint32_t> lds_b, glob_b;
+    std::vector<uint64_t> lds_c, glob_c;
+    for (uint32_t b : order) {
+        const double need = std::max(1.0, est[b] * margin);
+        const uint32_t P = uint32_t(std::ceil(need / (C * LDS_LOAD)));
+        if (P <= h->tune.max_passes) {
+            lds_b.push_back(b);
+            lds_c.push_back(uint64_t(P) * uint64_t(C));
+        } else {
+            glob_b.push_back(b);
+            glob_c.push_back(std::min(upper_cap(h, b), round_cap(need / h->tune.load_factor)));
+        }
+    }
+    std::vector<uint32_t> o1, o2;
+    if (!glob_b.empty()) CK(run_round_batched(h, glob_b, glob_c, o1, !lds_b.empty() ? false : true));
+    if (!lds_b.empty()) CK(run_round_batched(h, lds_b, lds_c, o2, true, true));
+    overflowed = o1;
+    overflowed.insert(overflowed.end(), o2.begin(), o2.end());
+    h->rs.n_lds_buckets = uint32_t(lds_b.size());
+    return CFGPU_OK;
 }
 
 // Rebuild overflowed buckets with never-overflowing capacities until done.
@@ -395,6 +440,7 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
     rs.t_sample_ms = rs.t_train_ms = rs.t_infer_ms = rs.t_construct_main_ms = rs.t_fallback_ms = 0;
     rs.mlp_loss = rs.mispredict_mean_abs_log = rs.mispredict_under_frac = rs.fallback_rate = 0;
     rs.t_construct_kernel_ms = 0;
+    rs.n_lds_buckets = 0;
 
     std::vector<uint32_t> all = nonempty_buckets(h);
     rs.n_buckets_nonempty = uint32_t(all.size());
@@ -421,9 +467,8 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
         CK(fallback(h, ovf, true));
     } else if (mode == CFGPU_MODE_BATCHED) {
         order_largest_first(main_set, cost);
-        for (size_t i = 0; i < main_set.size(); ++i) main_l2[i] = upper_cap(h, main_set[i]);
         HIPCK(h, hipEventRecord(e2.e, h->stream));
-        CK(run_round_batched(h, main_set, main_l2, ovf, true));
+        CK(run_main(h, main_set, cost, 1.0, ovf));
         HIPCK(h, hipEventRecord(e3.e, h->stream));
         rs.n_overflowed = uint32_t(ovf.size());
         CK(fallback(h, ovf, false));
@@ -504,8 +549,10 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
             const double need = double(h->h_pred[b]) * h->tune.margin / h->tune.load_factor;
             main_l2[i] = std::min(upper_cap(h, b), round_cap(need));
         }
+        std::vector<double> est(G, 0.0);
+        for (uint32_t b : main_set) est[b] = have_model ? double(h->h_pred[b]) : cost[b];
         HIPCK(h, hipEventRecord(e2.e, h->stream));
-        CK(run_round_batched(h, main_set, main_l2, ovf, true));
+        CK(run_main(h, main_set, est, have_model ? double(h->tune.margin) : 1.0, ovf));
         HIPCK(h, hipEventRecord(e3.e, h->stream));
         rs.n_overflowed = uint32_t(ovf.size());
         rs.fallback_rate = main_set.empty() ? 0.0 : double(ovf.size()) / double(main_set.size());
@@ -904,7 +951,7 @@ int cfgpu_tuning_set(cfgpu_handle* h, const cfgpu_tuning* in) {
     if (!(in->load_factor >= 0.2f && in->load_factor <= 0.85f) ||
         (in->workgroup != 256 && in->workgroup != 512) || in->chunk_records < 64 ||
         in->chunk_records > (1u << 24) || in->wgs_per_cu < 1 || in->wgs_per_cu > 32 ||
-        !(in->margin >= 0.01f && in->margin <= 4.0f) || in->use_graph > 1)
+        !(in->margin >= 0.01f && in->margin <= 4.0f) || in->use_graph > 1 || in->max_passes > 4096)
         return fail(h, CFGPU_E_ARG, "tuning out of range");
     h->tune = *in;
     return CFGPU_OK;
@@ -919,7 +966,7 @@ int cfgpu_autotune(cfgpu_handle* h, uint32_t mode, cfgpu_tune_trial* trials_out,
     for (float lf : {0.5f, 0.65f})
         for (uint32_t wg : {256u, 512u})
             for (uint32_t ch : {1024u, 4096u})
-                for (uint32_t wpc : {4u, 8u}) grid.push_back({lf, wg, ch, wpc, base.margin, base.use_graph});
+                for (uint32_t wpc : {4u, 8u}) grid.push_back({lf, wg, ch, wpc, base.margin, base.use_graph, base.max_passes});
     cfgpu_tuning best = base;
     double best_ms = 1e300;
     uint32_t w = 0;

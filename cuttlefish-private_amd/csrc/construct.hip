@@ -20,6 +20,7 @@
 // matching top word re-reads the same slot on its next loop iteration; the
 // claimer finishes its stores within its own iteration, so lanes of one
 // wavefront never wait on each other inside an iteration.
+#include "cf_instance.hpp"
 #include "cf_kernels.hpp"
 
 #include <algorithm>
@@ -131,31 +132,6 @@ __device__ Probe find_or_insert(g_u8* tbl, uint64_t cap, const Kmer<KW>& key) {
     }
 }
 
-// Exclusive sum-scan of one u32 per thread across a WG-thread workgroup;
-// returns the exclusive prefix and writes the total to *total.
-template <int WG>
-__device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* s_wave, uint32_t* total) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t u = __shfl_up(x, d);
-        if (lane >= d) x += u;
-    }
-    if (lane == 63) s_wave[wid] = x;
-    __syncthreads();
-    uint32_t pre = 0, tot = 0;
-#pragma unroll
-    for (int j = 0; j < WG / 64; ++j) {
-        const uint32_t t = s_wave[j];
-        pre += j < wid ? t : 0;
-        tot += t;
-    }
-    __syncthreads();
-    *total = tot;
-    return pre + x - v;
-}
-
 template <int KW, int WG>
 __device__ void construct_item(const ConstructArgs& a, uint32_t item, uint32_t* s_att,
                                uint64_t* s_lab, uint32_t* s_kofs, uint32_t* s_wave,
@@ -186,50 +162,11 @@ __device__ void construct_item(const ConstructArgs& a, uint32_t item, uint32_t* 
             if (ovf) return;
         }
         const uint32_t ns = min(uint32_t(WG), cnt - r0);
-        uint32_t kc = 0;
-        if (uint32_t(tid) < ns) {
-            const uint32_t at = a.att[rbase + r0 + tid];
-            s_att[tid] = at;
-            kc = (at & 0xFFu) - (k - 1);
-        }
-        for (uint32_t i = tid; i < ns * W; i += WG) s_lab[i] = a.labels[(rbase + r0) * W + i];
-        uint32_t total;
-        const uint32_t ex = block_excl_sum<WG>(kc, s_wave, &total);
-        s_kofs[tid] = ex;
-        if (tid == WG - 1) s_kofs[WG] = ex + kc;
-        __syncthreads();
-
+        const uint32_t total = stage_records<WG>(a.att, a.labels, rbase + r0, ns, W, k, s_att, s_lab,
+                                                 s_kofs, s_wave);
         for (uint32_t j = tid; j < total; j += WG) {
-            // record r holding k-mer instance j: largest r with s_kofs[r] <= j
-            int lo = 0, hi = int(ns) - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_kofs[mid] <= j) lo = mid;
-                else hi = mid - 1;
-            }
-            const uint32_t at = s_att[lo];
-            const uint32_t i = j - s_kofs[lo];
-            const uint32_t len = at & 0xFFu;
-            const uint64_t* L = s_lab + lo * W;
-
-            const Kmer<KW> v = km_from_label<KW>(L, W, i, k);
-            const Kmer<KW> vb = km_rc<KW>(v, k);
-            const bool is_can = km_less<KW>(v, vb);  // Directed_Vertex::in_canonical_form
-            const uint8_t pred = i > 0 ? label_base(L, W, i - 1) : BASE_E;
-            const uint8_t succ = i + k < len ? label_base(L, W, i + k) : BASE_E;
-            uint8_t front = is_can ? pred : comp_base(succ);
-            uint8_t back = is_can ? succ : comp_base(pred);
-            if (i > 0) {
-                // Self-loop rule (src/Subgraph.cpp:82-83): the previous k-mer
-                // has the same canonical form iff it equals v or rc(v).
-                const Kmer<KW> u = km_prepend<KW>(v, pred, k);
-                if (km_eq<KW>(u, v) || km_eq<KW>(u, vb)) (is_can ? front : back) = BASE_E;
-            }
-            uint32_t disc = 0;  // src/Subgraph.cpp:88-92 with entrance/exit sides
-            if (i == 0 && ((at >> 8) & 1u)) disc |= is_can ? 2u : 4u;
-            if (i + k == len && ((at >> 9) & 1u)) disc |= is_can ? 4u : 2u;
-
-            const Probe pr = find_or_insert<KW>(tbl, cap, is_can ? v : vb);
+            const Inst<KW> in = decode_instance<KW>(s_att, s_lab, s_kofs, ns, W, k, j);
+            const Probe pr = find_or_insert<KW>(tbl, cap, in.key);
             if (pr.slot < 0) {
                 atomicOr(&a.overflow[b], 1u);
             } else {
@@ -238,13 +175,13 @@ __device__ void construct_item(const ConstructArgs& a, uint32_t item, uint32_t* 
                 // stale) word that is already saturated needs no atomic.
                 uint32_t old = pr.ef_seen;
                 while (true) {
-                    const uint32_t nw = ef_add(old, front, back);
+                    const uint32_t nw = ef_add(old, in.front, in.back);
                     if (nw == old) break;
                     if (__hip_atomic_compare_exchange_strong(ef, &old, nw, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                              __HIP_MEMORY_SCOPE_AGENT))
                         break;
                 }
-                if (disc) __hip_atomic_fetch_or(ef + 1, disc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (in.disc) __hip_atomic_fetch_or(ef + 1, in.disc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             // distinct-key accounting: one atomic per wavefront
             const unsigned long long m = __ballot(pr.inserted);

@@ -1,0 +1,171 @@
+// LDS-resident subgraph construct.
+//
+// Device-scope atomics on HBM-resident tables are serviced past the per-XCD L2,
+// so a table built with global CAS costs a memory transaction per k-mer
+// instance.  This kernel instead builds one bucket's table inside a workgroup's
+// LDS and writes the finished table to HBM with plain, coalesced stores.
+//
+// A bucket whose distinct k-mers do not fit one LDS table is split by hash into
+// P passes: pass p keeps the k-mers with pass_of(hash) == p, and every pass
+// re-reads the bucket's records (cheap: they are L2 / Infinity-Cache resident
+// across consecutive passes).  Pass p's LDS table becomes slots
+// [p*C, (p+1)*C) of the bucket's HBM table.  One work item = one pass.
+//
+// The per-instance semantics are those of Subgraph::construct
+// (src/Subgraph.cpp:44-108) via decode_instance; counters saturate exactly as
+// Edge_Frequency::add_edge (include/State_Config.hpp:207-217).  A pass whose
+// table fills past 7/8 flags the bucket as overflowed; the host rebuilds it on
+// the global path, so capacity choices never change results.
+#include "cf_instance.hpp"
+#include "cf_kernels.hpp"
+
+namespace cfgpu {
+namespace {
+
+constexpr uint64_t EMPTY = ~0ull;
+constexpr uint64_t BUSY = 1ull << 63;
+constexpr int LWG = LDS_WG;
+
+template <int KW>
+__global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
+    constexpr int C = lds_capacity(KW);
+    constexpr int WMAX = 2 * KW;  // label words for k <= 32*KW
+    __shared__ uint64_t s_key[C * KW];
+    __shared__ uint32_t s_ef[C];
+    __shared__ uint32_t s_st[C];
+    __shared__ uint32_t s_att[LWG];
+    __shared__ uint64_t s_lab[LWG * WMAX];
+    __shared__ uint32_t s_kofs[LWG + 1];
+    __shared__ uint32_t s_wave[LWG / 64];
+    __shared__ uint32_t s_item, s_cnt, s_ovf;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const uint32_t W = a.W, k = a.k;
+    constexpr uint32_t LIMIT = (uint32_t(C) * 7u) / 8u;
+
+    while (true) {
+        __syncthreads();  // every thread is done with the previous item's flags
+        if (tid == 0) {
+            s_item = atomicAdd(a.queue_head, 1u);
+            s_cnt = 0;
+            s_ovf = 0;
+        }
+        __syncthreads();
+        const uint32_t item = s_item;
+        if (item >= a.n_items) break;
+        const uint32_t b = a.item_bucket[item];
+        const uint32_t p = a.item_first[item];     // pass index
+        const uint32_t P = a.item_count[item];     // passes of this bucket
+        if (tid == 0) s_ovf = __hip_atomic_load(&a.overflow[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int i = tid; i < C * KW; i += LWG) s_key[i] = EMPTY;
+        for (int i = tid; i < C; i += LWG) s_ef[i] = 0u, s_st[i] = 0u;
+        __syncthreads();
+        if (s_ovf) continue;  // bucket already flagged by another pass
+
+        const uint64_t r_begin = a.bucket_off[b], r_end = a.bucket_off[b + 1];
+        for (uint64_t r0 = r_begin; r0 < r_end; r0 += LWG) {
+            const uint32_t ns = uint32_t(min(uint64_t(LWG), r_end - r0));
+            const uint32_t total = stage_records<LWG>(a.att, a.labels, r0, ns, W, k, s_att, s_lab, s_kofs, s_wave);
+            for (uint32_t j = tid; j < total; j += LWG) {
+                const Inst<KW> in = decode_instance<KW>(s_att, s_lab, s_kofs, ns, W, k, j);
+                const uint64_t h = km_hash<KW>(in.key);
+                if (uint32_t((uint64_t(uint32_t(h >> 32)) * P) >> 32) != p) continue;
+                uint32_t s = uint32_t((uint64_t(uint32_t(h)) * uint32_t(C)) >> 32);
+                bool inserted = false;
+                int slot = -1;
+                if constexpr (KW == 1) {
+                    for (int n = 0; n < C; ++n) {
+                        const uint64_t cur = __hip_atomic_load(&s_key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (cur == in.key.w[0]) { slot = int(s); break; }
+                        if (cur == EMPTY) {
+                            const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(&s_key[s]), EMPTY, in.key.w[0]);
+                            if (old == EMPTY) { inserted = true; slot = int(s); break; }
+                            if (old == in.key.w[0]) { slot = int(s); break; }
+                        }
+                        s = s + 1 == uint32_t(C) ? 0 : s + 1;
+                    }
+                } else {
+                    const uint64_t top = in.key.w[KW - 1];
+                    for (int n = 0; n < C;) {
+                        uint64_t* kp = &s_key[s * KW];
+                        uint64_t cur = __hip_atomic_load(kp + KW - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (cur == EMPTY) {
+                            const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(kp + KW - 1), EMPTY, top | BUSY);
+                            if (old == EMPTY) {
+#pragma unroll
+                                for (int w = 0; w < KW - 1; ++w)
+                                    __hip_atomic_store(kp + w, in.key.w[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                __atomic_thread_fence(__ATOMIC_RELEASE);
+                                __hip_atomic_store(kp + KW - 1, top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                inserted = true;
+                                slot = int(s);
+                                break;
+                            }
+                            cur = old;
+                        }
+                        if ((cur & ~BUSY) == top) {
+                            if (cur & BUSY) continue;  // claimer publishes within its own iteration
+                            bool eq = true;
+#pragma unroll
+                            for (int w = 0; w < KW - 1; ++w)
+                                eq &= __hip_atomic_load(kp + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == in.key.w[w];
+                            if (eq) { slot = int(s); break; }
+                        }
+                        ++n;
+                        s = s + 1 == uint32_t(C) ? 0 : s + 1;
+                    }
+                }
+                if (slot < 0) {
+                    s_ovf = 1;
+                } else {
+                    uint32_t old = s_ef[slot];
+                    while (true) {
+                        const uint32_t nw = ef_add(old, in.front, in.back);
+                        if (nw == old) break;
+                        const uint32_t prev = atomicCAS(&s_ef[slot], old, nw);
+                        if (prev == old) break;
+                        old = prev;
+                    }
+                    if (in.disc) atomicOr(&s_st[slot], in.disc);
+                }
+                const unsigned long long m = __ballot(inserted);
+                if (m && lane == __ffsll((long long)m) - 1) {
+                    const uint32_t add = uint32_t(__popcll(m));
+                    if (atomicAdd(&s_cnt, add) + add > LIMIT) s_ovf = 1;
+                }
+            }
+            __syncthreads();
+            if (s_ovf) break;
+        }
+        __syncthreads();
+        if (s_ovf) {
+            if (tid == 0) atomicOr(&a.overflow[b], 1u);
+            continue;
+        }
+        // Write the finished pass table to slots [p*C, (p+1)*C) of the bucket.
+        typedef __attribute__((address_space(1))) uint64_t g_u64;
+        g_u64* out = reinterpret_cast<g_u64*>(a.tbl_ptr[b]) + uint64_t(p) * C * (KW + 1);
+        for (int i = tid; i < C; i += LWG) {
+            g_u64* e = out + uint64_t(i) * (KW + 1);
+#pragma unroll
+            for (int w = 0; w < KW; ++w) e[w] = s_key[i * KW + w];
+            e[KW] = uint64_t(s_ef[i]) | (uint64_t(s_st[i]) << 32);
+        }
+        if (tid == 0) atomicAdd(&a.n_vert[b], s_cnt);
+    }
+}
+
+}  // namespace
+
+void launch_construct_lds(const ConstructArgs& a, uint32_t KW, uint32_t grid, hipStream_t s) {
+    if (!grid) return;
+    switch (KW) {
+        case 1: hipLaunchKernelGGL(k_construct_lds<1>, dim3(grid), dim3(LWG), 0, s, a); break;
+        case 2: hipLaunchKernelGGL(k_construct_lds<2>, dim3(grid), dim3(LWG), 0, s, a); break;
+        case 3: hipLaunchKernelGGL(k_construct_lds<3>, dim3(grid), dim3(LWG), 0, s, a); break;
+        default: hipLaunchKernelGGL(k_construct_lds<4>, dim3(grid), dim3(LWG), 0, s, a); break;
+    }
+}
+
+}  // namespace cfgpu

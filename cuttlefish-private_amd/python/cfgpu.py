@@ -49,9 +49,9 @@ class RunStats(ctypes.Structure):
                                                   "t_features_ms", "t_partition_ms", "t_sample_ms",
                                                   "t_train_ms", "t_infer_ms",
                                                   "t_construct_main_ms", "t_fallback_ms",
-                                                  "t_construct_ms", "t_construct_kernel_ms",
-                                                  "mlp_loss",
-                                                  "mispredict_mean_abs_log",
+                                                  "t_construct_ms", "t_construct_kernel_ms")]
+                + [(n, ctypes.c_uint32) for n in ("n_lds_buckets", "reserved")]
+                + [(n, ctypes.c_double) for n in ("mlp_loss", "mispredict_mean_abs_log",
                                                   "mispredict_under_frac", "fallback_rate")])
 
     def as_dict(self):
@@ -61,7 +61,8 @@ class RunStats(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("load_factor", ctypes.c_float), ("workgroup", ctypes.c_uint32),
                 ("chunk_records", ctypes.c_uint32), ("wgs_per_cu", ctypes.c_uint32),
-                ("margin", ctypes.c_float), ("use_graph", ctypes.c_uint32)]
+                ("margin", ctypes.c_float), ("use_graph", ctypes.c_uint32),
+                ("max_passes", ctypes.c_uint32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

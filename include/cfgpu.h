@@ -158,8 +158,10 @@ typedef struct cfgpu_run_stats {
     double t_minimizer_ms, t_count_ms, t_emit_ms, t_features_ms, t_partition_ms;
     double t_sample_ms, t_train_ms, t_infer_ms, t_construct_main_ms, t_fallback_ms;
     double t_construct_ms;
-    double t_construct_kernel_ms;     /* persistent construct kernel alone, main round
+    double t_construct_kernel_ms;     /* persistent construct kernels alone, main round
                                          (recorded when use_graph == 0) */
+    uint32_t n_lds_buckets;           /* main-round buckets built in LDS */
+    uint32_t reserved;
     double mlp_loss;                  /* final training MSE (log space) */
     double mispredict_mean_abs_log;   /* mean |log(pred/true)| over predicted buckets */
     double mispredict_under_frac;     /* fraction of predicted buckets with pred < true */
@@ -177,6 +179,9 @@ typedef struct cfgpu_tuning {
                                  below 1 under-sizes on purpose (exercises the exact
                                  overflow fallback; results do not change) */
     uint32_t use_graph;       /* 1: batched launches are captured and replayed as a hipGraph */
+    uint32_t max_passes;      /* batched paths: buckets needing at most this many LDS hash
+                                 passes are built in LDS, larger ones with global atomics
+                                 (0: always global) */
 } cfgpu_tuning;
 int cfgpu_tuning_get(cfgpu_handle* h, cfgpu_tuning* out);
 int cfgpu_tuning_set(cfgpu_handle* h, const cfgpu_tuning* in);

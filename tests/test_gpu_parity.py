@@ -236,10 +236,21 @@ def test_zipf_bucket_workload_loaded_records():
         want = O.construct(k, l, off, (att & 0xFFFF).astype(np.uint16), lab)
         with C.Handle(k, l, 1) as h:
             h.load_buckets(att, lab, off)
-            for mode in ("plain", "batched", "learned"):
-                say(f"[case] zipf-loaded k={k} mode={mode}")
-                h.construct(mode)
-                check_tables(h, None, k, l, 1, want)
+            h.construct("plain")
+            check_tables(h, None, k, l, 1, want)
+            # max_passes routes buckets between the LDS kernel and the
+            # global-atomic kernel: all LDS, mixed, all global
+            for mp in (32, 2, 0):
+                h.set_tuning(max_passes=mp)
+                for mode in ("batched", "learned"):
+                    say(f"[case] zipf-loaded k={k} mode={mode} max_passes={mp}")
+                    h.construct(mode)
+                    check_tables(h, None, k, l, 1, want)
+                    rs = h.run_stats()
+                    if mp == 0:
+                        assert rs["n_lds_buckets"] == 0
+                    if mp == 32 and mode == "batched":
+                        assert rs["n_lds_buckets"] > 0
 
 
 def test_load_buckets_rejects_malformed_records():
