@@ -193,12 +193,35 @@ def test_overflow_fallback_is_exact():
     say("[case] overflow-fallback")
     with C.Handle(k, l, 2) as h:
         h.partition(inp)
-        h.set_tuning(margin=0.25, load_factor=0.8)
+        # global-atomic tables only (every LDS pass table of these small
+        # buckets would have room whatever the estimate)
+        h.set_tuning(margin=0.25, load_factor=0.8, max_passes=0)
         h.construct("learned")
         rs = h.run_stats()
         say(f"[case] overflowed={rs['n_overflowed']} rounds={rs['n_fallback_rounds']}")
         assert rs["n_overflowed"] > 0 and rs["n_fallback_rounds"] >= 1
+        assert rs["n_lds_buckets"] == 0
         check_tables(h, part, k, l, 2)
+        stats = h.bucket_stats()
+        assert ((stats["flags"] & 2) > 0).sum() == rs["n_overflowed"]
+
+
+def test_lds_overflow_fallback_is_exact():
+    # Large Zipf buckets with under-estimated distinct counts get too few LDS
+    # passes; the full pass table is detected and the bucket rebuilt on the
+    # global path.
+    k, l = 31, 20
+    att, lab, off = C.synth_bucket_workload(256, 60000, 1.1, k, l, seed=7)
+    want = O.construct(k, l, off, (att & 0xFFFF).astype(np.uint16), lab)
+    say("[case] lds-overflow")
+    with C.Handle(k, l, 1) as h:
+        h.load_buckets(att, lab, off)
+        h.set_tuning(margin=0.2, max_passes=4096)
+        h.construct("learned")
+        rs = h.run_stats()
+        say(f"[case] lds-overflow overflowed={rs['n_overflowed']} lds={rs['n_lds_buckets']}")
+        assert rs["n_lds_buckets"] > 0 and rs["n_overflowed"] > 0 and rs["n_fallback_rounds"] >= 1
+        check_tables(h, None, k, l, 1, want)
         stats = h.bucket_stats()
         assert ((stats["flags"] & 2) > 0).sum() == rs["n_overflowed"]
 
