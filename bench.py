@@ -204,7 +204,8 @@ def main():
                    "n_kmers": n_km, "n_records": rs["n_records"], "n_vertices": rs["n_vertices"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_construct (persistent, main round)", "kernel_ms": kms,
+                     "kernel": "construct main round (k_construct_lds, plus k_construct for buckets "
+                               "too large for LDS passes)", "kernel_ms": kms,
                      "algorithmic_bytes": kbytes},
         "cpu_baseline": cpu,
         "paths": {
@@ -227,6 +228,7 @@ def main():
                     "mispredict_mean_abs_log": rs["mispredict_mean_abs_log"],
                     "mispredict_under_frac": rs["mispredict_under_frac"],
                     "fallback_rate": rs["fallback_rate"], "overflowed": rs["n_overflowed"],
+                    "lds_buckets": rs["n_lds_buckets"],
                     "table_bytes": rs["table_bytes"]},
     }
     print(json.dumps(out), flush=True)

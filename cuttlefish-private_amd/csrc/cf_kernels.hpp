@@ -75,8 +75,10 @@ void launch_construct_persistent(const ConstructArgs& a, uint32_t KW, uint32_t g
 // LDS-resident construct: one work item = one hash pass of one bucket
 // (item_first = pass index, item_count = number of passes P; the bucket's
 // table holds P * lds_capacity(KW) slots).  Persistent over queue_head.
-constexpr int LDS_WG = 512;
-constexpr int lds_capacity(int KW) { return KW == 1 ? 8192 : KW == 2 ? 5120 : KW == 3 ? 4096 : 3072; }
+// The LDS holds only the pass table: 8*KW key bytes + 4 e_f bytes + 1 status
+// byte per slot, sized to fill the 160 KiB of a gfx950 CU.
+constexpr int LDS_WG = 1024;
+constexpr int lds_capacity(int KW) { return KW == 1 ? 12544 : KW == 2 ? 7744 : KW == 3 ? 5632 : 4416; }
 void launch_construct_lds(const ConstructArgs& a, uint32_t KW, uint32_t grid, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 // n_vert[b] = overflow[b] = 0 for every b in list[0..n).

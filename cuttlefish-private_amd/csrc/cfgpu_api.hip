@@ -26,7 +26,7 @@ namespace {
 constexpr uint32_t G_FIXED = 16384;
 constexpr uint64_t SAMPLE_TARGET = MLP_TRAIN_MAX;
 constexpr uint32_t MAX_ROUNDS = 8;
-constexpr double LDS_LOAD = 0.7;  // target fill of an LDS pass table (overflow at 7/8)
+constexpr double LDS_LOAD = 0.8;  // target fill of an LDS pass table (overflow at 7/8)
 
 struct DevBuf {
     void* p = nullptr;

@@ -32,7 +32,6 @@ constexpr uint64_t EMPTY = ~0ull;
 constexpr uint64_t BUSY = 1ull << 63;
 constexpr uint32_t MAX_SPINS = 1u << 22;
 constexpr uint32_t MAX_PROBES = 1024;  // longer probe chains count as overflow
-constexpr int W_MAX = 8;  // label words: ceil((2k - l)/32) <= 8 for k <= 127
 
 // Table memory is addressed through global-address-space pointers so probes
 // and atomics compile to global_* (not flat_*) instructions.
@@ -47,9 +46,6 @@ __device__ __forceinline__ g_u64* slot_ptr(g_u8* tbl, uint64_t s) {
     return reinterpret_cast<g_u64*>(tbl + s * uint64_t(8 * KW + 8));
 }
 
-__device__ __forceinline__ uint64_t ld_relaxed(const uint64_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ __forceinline__ uint32_t ld_relaxed32(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -133,18 +129,12 @@ __device__ Probe find_or_insert(g_u8* tbl, uint64_t cap, const Kmer<KW>& key) {
 }
 
 template <int KW, int WG>
-__device__ void construct_item(const ConstructArgs& a, uint32_t item, uint32_t* s_att,
-                               uint64_t* s_lab, uint32_t* s_kofs, uint32_t* s_wave,
-                               uint32_t* s_flag) {
+__device__ void construct_item(const ConstructArgs& a, uint32_t item) {
     const int tid = threadIdx.x;
     const uint32_t b = a.item_bucket[item];
-    // Skip buckets already known to overflow (they are rebuilt with a larger
-    // table).  One thread reads the flag so the whole workgroup agrees.
-    if (tid == 0) *s_flag = ld_relaxed32(&a.overflow[b]);
-    __syncthreads();
-    const uint32_t skip = *s_flag;
-    __syncthreads();
-    if (skip) return;
+    // Buckets already known to overflow are skipped (they are rebuilt with a
+    // larger table); each thread polls the flag between its records.
+    if (ld_relaxed32(&a.overflow[b])) return;
     const uint32_t first = a.item_first[item], cnt = a.item_count[item];
     const uint32_t W = a.W, k = a.k;
     const uint64_t rbase = a.bucket_off[b] + first;
@@ -153,19 +143,13 @@ __device__ void construct_item(const ConstructArgs& a, uint32_t item, uint32_t* 
     const uint32_t limit = a.tbl_limit[b];
     const int lane = tid & 63;
 
-    for (uint32_t r0 = 0; r0 < cnt; r0 += WG) {
-        if (r0) {  // stop early once the bucket is known to overflow
-            if (tid == 0) *s_flag = ld_relaxed32(&a.overflow[b]);
-            __syncthreads();
-            const uint32_t ovf = *s_flag;
-            __syncthreads();
-            if (ovf) return;
-        }
-        const uint32_t ns = min(uint32_t(WG), cnt - r0);
-        const uint32_t total = stage_records<WG>(a.att, a.labels, rbase + r0, ns, W, k, s_att, s_lab,
-                                                 s_kofs, s_wave);
-        for (uint32_t j = tid; j < total; j += WG) {
-            const Inst<KW> in = decode_instance<KW>(s_att, s_lab, s_kofs, ns, W, k, j);
+    // One thread walks one record (RecordWalker rolls the k-mer per base).
+    for (uint32_t r = tid; r < cnt; r += WG) {
+        if (r >= uint32_t(WG) && ld_relaxed32(&a.overflow[b])) break;
+        RecordWalker<KW, 2 * KW> rw;
+        rw.load(a.att, a.labels, rbase + r, W, k);
+        for (uint32_t t = 0; t < rw.n; ++t) {
+            const Inst<KW> in = rw.next();
             const Probe pr = find_or_insert<KW>(tbl, cap, in.key);
             if (pr.slot < 0) {
                 atomicOr(&a.overflow[b], 1u);
@@ -194,19 +178,14 @@ __device__ void construct_item(const ConstructArgs& a, uint32_t item, uint32_t* 
                 }
             }
         }
-        __syncthreads();
     }
 }
 
 template <int KW, int WG>
 __global__ __launch_bounds__(WG) void k_construct(ConstructArgs a, uint32_t item0, int persistent) {
-    __shared__ uint32_t s_att[WG];
-    __shared__ uint64_t s_lab[WG * W_MAX];
-    __shared__ uint32_t s_kofs[WG + 1];
-    __shared__ uint32_t s_wave[WG / 64];
-    __shared__ uint32_t s_item, s_flag;
+    __shared__ uint32_t s_item;
     if (!persistent) {
-        construct_item<KW, WG>(a, item0 + blockIdx.x, s_att, s_lab, s_kofs, s_wave, &s_flag);
+        construct_item<KW, WG>(a, item0 + blockIdx.x);
         return;
     }
     while (true) {
@@ -215,7 +194,7 @@ __global__ __launch_bounds__(WG) void k_construct(ConstructArgs a, uint32_t item
         const uint32_t it = s_item;
         __syncthreads();
         if (it >= a.n_items) break;
-        construct_item<KW, WG>(a, it, s_att, s_lab, s_kofs, s_wave, &s_flag);
+        construct_item<KW, WG>(a, it);
     }
 }
 

@@ -11,8 +11,13 @@
 // across consecutive passes).  Pass p's LDS table becomes slots
 // [p*C, (p+1)*C) of the bucket's HBM table.  One work item = one pass.
 //
+// Each thread walks whole records (RecordWalker: the k-mer and its reverse
+// complement are rolled one base per instance), so no staging barriers or
+// per-instance searches are needed and the LDS holds only the table: u64 key
+// words, the u32 Edge_Frequency word and one Vertex_Status byte per slot.
+//
 // The per-instance semantics are those of Subgraph::construct
-// (src/Subgraph.cpp:44-108) via decode_instance; counters saturate exactly as
+// (src/Subgraph.cpp:44-108); counters saturate exactly as
 // Edge_Frequency::add_edge (include/State_Config.hpp:207-217).  A pass whose
 // table fills past 7/8 flags the bucket as overflowed; the host rebuilds it on
 // the global path, so capacity choices never change results.
@@ -26,17 +31,13 @@ constexpr uint64_t EMPTY = ~0ull;
 constexpr uint64_t BUSY = 1ull << 63;
 constexpr int LWG = LDS_WG;
 
-template <int KW>
+template <int KW, int WMAX>
 __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
     constexpr int C = lds_capacity(KW);
-    constexpr int WMAX = 2 * KW;  // label words for k <= 32*KW
+    static_assert(C % 64 == 0, "capacity");
     __shared__ uint64_t s_key[C * KW];
     __shared__ uint32_t s_ef[C];
-    __shared__ uint32_t s_st[C];
-    __shared__ uint32_t s_att[LWG];
-    __shared__ uint64_t s_lab[LWG * WMAX];
-    __shared__ uint32_t s_kofs[LWG + 1];
-    __shared__ uint32_t s_wave[LWG / 64];
+    __shared__ uint32_t s_st[C / 4];  // one Vertex_Status byte per slot
     __shared__ uint32_t s_item, s_cnt, s_ovf;
 
     const int tid = threadIdx.x;
@@ -59,16 +60,18 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
         const uint32_t P = a.item_count[item];     // passes of this bucket
         if (tid == 0) s_ovf = __hip_atomic_load(&a.overflow[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         for (int i = tid; i < C * KW; i += LWG) s_key[i] = EMPTY;
-        for (int i = tid; i < C; i += LWG) s_ef[i] = 0u, s_st[i] = 0u;
+        for (int i = tid; i < C; i += LWG) s_ef[i] = 0u;
+        for (int i = tid; i < C / 4; i += LWG) s_st[i] = 0u;
         __syncthreads();
         if (s_ovf) continue;  // bucket already flagged by another pass
 
         const uint64_t r_begin = a.bucket_off[b], r_end = a.bucket_off[b + 1];
-        for (uint64_t r0 = r_begin; r0 < r_end; r0 += LWG) {
-            const uint32_t ns = uint32_t(min(uint64_t(LWG), r_end - r0));
-            const uint32_t total = stage_records<LWG>(a.att, a.labels, r0, ns, W, k, s_att, s_lab, s_kofs, s_wave);
-            for (uint32_t j = tid; j < total; j += LWG) {
-                const Inst<KW> in = decode_instance<KW>(s_att, s_lab, s_kofs, ns, W, k, j);
+        for (uint64_t r = r_begin + tid; r < r_end; r += LWG) {
+            if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+            RecordWalker<KW, WMAX> rw;
+            rw.load(a.att, a.labels, r, W, k);
+            for (uint32_t t = 0; t < rw.n; ++t) {
+                const Inst<KW> in = rw.next();
                 const uint64_t h = km_hash<KW>(in.key);
                 if (uint32_t((uint64_t(uint32_t(h >> 32)) * P) >> 32) != p) continue;
                 uint32_t s = uint32_t((uint64_t(uint32_t(h)) * uint32_t(C)) >> 32);
@@ -127,7 +130,7 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                         if (prev == old) break;
                         old = prev;
                     }
-                    if (in.disc) atomicOr(&s_st[slot], in.disc);
+                    if (in.disc) atomicOr(&s_st[slot >> 2], in.disc << (8 * (slot & 3)));
                 }
                 const unsigned long long m = __ballot(inserted);
                 if (m && lane == __ffsll((long long)m) - 1) {
@@ -135,8 +138,6 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                     if (atomicAdd(&s_cnt, add) + add > LIMIT) s_ovf = 1;
                 }
             }
-            __syncthreads();
-            if (s_ovf) break;
         }
         __syncthreads();
         if (s_ovf) {
@@ -150,7 +151,8 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
             g_u64* e = out + uint64_t(i) * (KW + 1);
 #pragma unroll
             for (int w = 0; w < KW; ++w) e[w] = s_key[i * KW + w];
-            e[KW] = uint64_t(s_ef[i]) | (uint64_t(s_st[i]) << 32);
+            const uint32_t st = (s_st[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+            e[KW] = uint64_t(s_ef[i]) | (uint64_t(st) << 32);
         }
         if (tid == 0) atomicAdd(&a.n_vert[b], s_cnt);
     }
@@ -160,11 +162,12 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
 
 void launch_construct_lds(const ConstructArgs& a, uint32_t KW, uint32_t grid, hipStream_t s) {
     if (!grid) return;
+    // label words: W = ceil((2k - l) / 32) <= 2 * KW for l >= 1
     switch (KW) {
-        case 1: hipLaunchKernelGGL(k_construct_lds<1>, dim3(grid), dim3(LWG), 0, s, a); break;
-        case 2: hipLaunchKernelGGL(k_construct_lds<2>, dim3(grid), dim3(LWG), 0, s, a); break;
-        case 3: hipLaunchKernelGGL(k_construct_lds<3>, dim3(grid), dim3(LWG), 0, s, a); break;
-        default: hipLaunchKernelGGL(k_construct_lds<4>, dim3(grid), dim3(LWG), 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_construct_lds<1, 2>), dim3(grid), dim3(LWG), 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_construct_lds<2, 4>), dim3(grid), dim3(LWG), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_construct_lds<3, 6>), dim3(grid), dim3(LWG), 0, s, a); break;
+        default: hipLaunchKernelGGL((k_construct_lds<4, 8>), dim3(grid), dim3(LWG), 0, s, a); break;
     }
 }
 

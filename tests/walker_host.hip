@@ -1,0 +1,64 @@
+// Host build of the construct kernels' per-record walker (RecordWalker) and
+// saturating counter update (ef_add), for CPU parity tests against the oracle.
+// Test infrastructure: compiled by tests/test_walker_cpu.py, never shipped.
+//
+// stdin:  u32 k, u32 W, u64 n, then u32 att[n] (g in the high 16 bits) and
+//         u64 labels[n * W].
+// stdout: u64 rows, then per (g, canonical k-mer) sorted by (g, key):
+//         u32 g, u64 key[4], u32 e_f, u32 status.
+#include <hip/hip_runtime.h>
+#undef __device__
+#define __device__ __host__ __attribute__((device))
+#include "../cuttlefish-private_amd/csrc/cf_instance.hpp"
+
+#include <array>
+#include <cstdio>
+#include <map>
+#include <vector>
+
+using namespace cfgpu;
+
+using Row = std::pair<uint32_t, uint32_t>;
+using Key = std::pair<uint32_t, std::array<uint64_t, 4>>;
+
+template <int KW>
+static void walk_all(uint32_t k, uint32_t W, const std::vector<uint32_t>& att,
+                     const std::vector<uint64_t>& lab, std::map<Key, Row>& out) {
+    for (size_t r = 0; r < att.size(); ++r) {
+        RecordWalker<KW, 2 * KW> rw;
+        rw.load(att.data(), lab.data(), r, W, k);
+        for (uint32_t t = 0; t < rw.n; ++t) {
+            const Inst<KW> in = rw.next();
+            Key key{att[r] >> 16, {0, 0, 0, 0}};
+            for (int w = 0; w < KW; ++w) key.second[w] = in.key.w[w];
+            Row& v = out[key];
+            v.first = ef_add(v.first, in.front, in.back);
+            v.second |= in.disc;
+        }
+    }
+}
+
+int main() {
+    uint32_t k, W;
+    uint64_t n;
+    if (fread(&k, 4, 1, stdin) != 1 || fread(&W, 4, 1, stdin) != 1 || fread(&n, 8, 1, stdin) != 1) return 2;
+    std::vector<uint32_t> att(n);
+    std::vector<uint64_t> lab(n * W);
+    if (fread(att.data(), 4, n, stdin) != n || fread(lab.data(), 8, n * W, stdin) != n * W) return 2;
+    std::map<Key, Row> out;
+    switch ((k + 31) / 32) {
+        case 1: walk_all<1>(k, W, att, lab, out); break;
+        case 2: walk_all<2>(k, W, att, lab, out); break;
+        case 3: walk_all<3>(k, W, att, lab, out); break;
+        default: walk_all<4>(k, W, att, lab, out); break;
+    }
+    const uint64_t rows = out.size();
+    fwrite(&rows, 8, 1, stdout);
+    for (const auto& [key, v] : out) {
+        fwrite(&key.first, 4, 1, stdout);
+        fwrite(key.second.data(), 8, 4, stdout);
+        fwrite(&v.first, 4, 1, stdout);
+        fwrite(&v.second, 4, 1, stdout);
+    }
+    return 0;
+}
