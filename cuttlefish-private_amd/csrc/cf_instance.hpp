@@ -26,17 +26,25 @@ struct RecordWalker {
     Kmer<KW> v, vb, prev;  // current k-mer, its reverse complement, previous k-mer
     uint32_t len, k, kt, i, n, at;
 
-    // Load record r of a bucket array with W label words per record (word
-    // W-1 holds bases 0..31).  Every index below is static, so the walker
-    // stays in registers.
-    __device__ __forceinline__ void load(const uint32_t* att, const uint64_t* labels, uint64_t r,
-                                         uint32_t W, uint32_t k_) {
-        at = att[r];
+    // Fetch record r of a bucket array with W label words per record (word
+    // W-1 holds bases 0..31) into registers: att and the label words with
+    // bases 0..31 first.
+    __device__ __forceinline__ static void fetch(const uint32_t* att, const uint64_t* labels, uint64_t r,
+                                                 uint32_t W, uint32_t& at_out, uint64_t (&L)[WMAX]) {
+        at_out = att[r];
+#pragma unroll
+        for (int q = 0; q < WMAX; ++q) L[q] = uint32_t(q) < W ? labels[r * W + (W - 1 - q)] : 0;
+    }
+
+    // Start walking a fetched record.  Every index below is static, so the
+    // walker stays in registers.
+    __device__ __forceinline__ void init(uint32_t at_in, const uint64_t (&L)[WMAX], uint32_t k_) {
+        at = at_in;
         len = at & 0xFFu;
         k = k_;
         kt = k - 32 * (KW - 1);
 #pragma unroll
-        for (int q = 0; q < WMAX; ++q) S[q] = uint32_t(q) < W ? labels[r * W + (W - 1 - q)] : 0;
+        for (int q = 0; q < WMAX; ++q) S[q] = L[q];
         // First k-mer: the top word holds bases 0..kt-1, word j < KW-1 holds
         // the 32 bases from kt + 32(KW-2-j).
         v.w[KW - 1] = S[0] >> (64 - 2 * kt);
@@ -59,6 +67,14 @@ struct RecordWalker {
         }
         i = 0;
         n = len - k + 1;
+    }
+
+    __device__ __forceinline__ void load(const uint32_t* att, const uint64_t* labels, uint64_t r,
+                                         uint32_t W, uint32_t k_) {
+        uint32_t a;
+        uint64_t L[WMAX];
+        fetch(att, labels, r, W, a, L);
+        init(a, L, k_);
     }
 
     // Decode the current instance and advance.

@@ -11,10 +11,11 @@
 // across consecutive passes).  Pass p's LDS table becomes slots
 // [p*C, (p+1)*C) of the bucket's HBM table.  One work item = one pass.
 //
-// Each thread walks whole records (RecordWalker: the k-mer and its reverse
-// complement are rolled one base per instance), so no staging barriers or
-// per-instance searches are needed and the LDS holds only the table: u64 key
-// words, the u32 Edge_Frequency word and one Vertex_Status byte per slot.
+// Each lane walks a stream of whole records (RecordWalker: the k-mer and its
+// reverse complement are rolled one base per instance), so no staging
+// barriers or per-instance searches are needed and the LDS holds only the
+// table: u64 key words, the u32 Edge_Frequency word and one Vertex_Status byte
+// per slot.
 //
 // The per-instance semantics are those of Subgraph::construct
 // (src/Subgraph.cpp:44-108); counters saturate exactly as
@@ -31,6 +32,74 @@ constexpr uint64_t EMPTY = ~0ull;
 constexpr uint64_t BUSY = 1ull << 63;
 constexpr int LWG = LDS_WG;
 
+// Find or insert `key` in an LDS table of C slots (KW u64 words each), probing
+// groups of 4 slots: the 4 top words are read together, and a group fills
+// from its first slot on (claims take the first empty slot; entries are never
+// removed), so a group with an empty slot ends an unsuccessful search.
+// Returns the slot, or -1 if every group is full.
+template <int KW, int C>
+__device__ __forceinline__ int lds_find_or_insert(uint64_t* s_key, const Kmer<KW>& key, uint64_t h,
+                                                  bool& inserted) {
+    constexpr uint32_t NG = uint32_t(C) / 4;
+    uint32_t g = uint32_t((uint64_t(uint32_t(h)) * NG) >> 32);
+    const uint64_t top = key.w[KW - 1];
+    inserted = false;
+    for (uint32_t n = 0; n < NG;) {
+        uint64_t cur[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            cur[j] = __hip_atomic_load(&s_key[(g * 4 + j) * KW + KW - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        int empty = 4;
+        bool busy = false;
+#pragma unroll
+        for (int j = 3; j >= 0; --j)
+            if (cur[j] == EMPTY) empty = j;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= empty) break;
+            if ((cur[j] & ~BUSY) != top) continue;
+            if constexpr (KW == 1) {
+                return int(g * 4 + j);
+            } else {
+                if (cur[j] & BUSY) {
+                    busy = true;
+                    continue;
+                }
+                const uint64_t* kp = &s_key[(g * 4 + j) * KW];
+                bool eq = true;
+#pragma unroll
+                for (int w = 0; w < KW - 1; ++w)
+                    eq &= __hip_atomic_load(kp + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == key.w[w];
+                if (eq) return int(g * 4 + j);
+            }
+        }
+        if (busy) continue;  // a matching top word is being published: re-read
+        if (empty == 4) {    // full group without the key: next group
+            g = g + 1 == NG ? 0 : g + 1;
+            ++n;
+            continue;
+        }
+        const uint32_t slot = g * 4 + uint32_t(empty);
+        uint64_t* kp = &s_key[slot * KW];
+        const uint64_t claim = KW == 1 ? top : (top | BUSY);
+        const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(kp + KW - 1), EMPTY, claim);
+        if (old == EMPTY) {
+            if constexpr (KW > 1) {
+#pragma unroll
+                for (int w = 0; w < KW - 1; ++w)
+                    __hip_atomic_store(kp + w, key.w[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __atomic_thread_fence(__ATOMIC_RELEASE);
+                __hip_atomic_store(kp + KW - 1, top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            inserted = true;
+            return int(slot);
+        }
+        if (KW == 1 && old == top) return int(slot);
+        // lost the slot to another key: re-read the group
+    }
+    return -1;
+}
+
 template <int KW, int WMAX>
 __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
     constexpr int C = lds_capacity(KW);
@@ -44,6 +113,7 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
     const int lane = tid & 63;
     const uint32_t W = a.W, k = a.k;
     constexpr uint32_t LIMIT = (uint32_t(C) * 7u) / 8u;
+    using Walker = RecordWalker<KW, WMAX>;
 
     while (true) {
         __syncthreads();  // every thread is done with the previous item's flags
@@ -65,64 +135,28 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
         __syncthreads();
         if (s_ovf) continue;  // bucket already flagged by another pass
 
-        const uint64_t r_begin = a.bucket_off[b], r_end = a.bucket_off[b + 1];
-        for (uint64_t r = r_begin + tid; r < r_end; r += LWG) {
-            if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
-            RecordWalker<KW, WMAX> rw;
+        // Each lane walks a stream of records r, r + LWG, ... and prefetches
+        // the next one while walking the current one, so lanes stay busy
+        // whatever the record lengths.
+        const uint64_t r_end = a.bucket_off[b + 1];
+        uint64_t r = a.bucket_off[b] + tid;
+        Walker rw;
+        uint32_t at_next = 0;
+        uint64_t lab_next[WMAX];
+        if (r < r_end) {
             rw.load(a.att, a.labels, r, W, k);
-            for (uint32_t t = 0; t < rw.n; ++t) {
-                const Inst<KW> in = rw.next();
-                const uint64_t h = km_hash<KW>(in.key);
-                if (uint32_t((uint64_t(uint32_t(h >> 32)) * P) >> 32) != p) continue;
-                uint32_t s = uint32_t((uint64_t(uint32_t(h)) * uint32_t(C)) >> 32);
-                bool inserted = false;
-                int slot = -1;
-                if constexpr (KW == 1) {
-                    for (int n = 0; n < C; ++n) {
-                        const uint64_t cur = __hip_atomic_load(&s_key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        if (cur == in.key.w[0]) { slot = int(s); break; }
-                        if (cur == EMPTY) {
-                            const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(&s_key[s]), EMPTY, in.key.w[0]);
-                            if (old == EMPTY) { inserted = true; slot = int(s); break; }
-                            if (old == in.key.w[0]) { slot = int(s); break; }
-                        }
-                        s = s + 1 == uint32_t(C) ? 0 : s + 1;
-                    }
-                } else {
-                    const uint64_t top = in.key.w[KW - 1];
-                    for (int n = 0; n < C;) {
-                        uint64_t* kp = &s_key[s * KW];
-                        uint64_t cur = __hip_atomic_load(kp + KW - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        if (cur == EMPTY) {
-                            const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(kp + KW - 1), EMPTY, top | BUSY);
-                            if (old == EMPTY) {
-#pragma unroll
-                                for (int w = 0; w < KW - 1; ++w)
-                                    __hip_atomic_store(kp + w, in.key.w[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                                __atomic_thread_fence(__ATOMIC_RELEASE);
-                                __hip_atomic_store(kp + KW - 1, top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                                inserted = true;
-                                slot = int(s);
-                                break;
-                            }
-                            cur = old;
-                        }
-                        if ((cur & ~BUSY) == top) {
-                            if (cur & BUSY) continue;  // claimer publishes within its own iteration
-                            bool eq = true;
-#pragma unroll
-                            for (int w = 0; w < KW - 1; ++w)
-                                eq &= __hip_atomic_load(kp + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == in.key.w[w];
-                            if (eq) { slot = int(s); break; }
-                        }
-                        ++n;
-                        s = s + 1 == uint32_t(C) ? 0 : s + 1;
-                    }
-                }
+            if (r + LWG < r_end) Walker::fetch(a.att, a.labels, r + LWG, W, at_next, lab_next);
+        }
+        while (r < r_end) {
+            const Inst<KW> in = rw.next();
+            const uint64_t h = km_hash<KW>(in.key);
+            bool inserted = false;
+            if (uint32_t((uint64_t(uint32_t(h >> 32)) * P) >> 32) == p) {
+                const int slot = lds_find_or_insert<KW, C>(s_key, in.key, h, inserted);
                 if (slot < 0) {
                     s_ovf = 1;
                 } else {
-                    uint32_t old = s_ef[slot];
+                    uint32_t old = inserted ? 0u : s_ef[slot];
                     while (true) {
                         const uint32_t nw = ef_add(old, in.front, in.back);
                         if (nw == old) break;
@@ -132,11 +166,18 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                     }
                     if (in.disc) atomicOr(&s_st[slot >> 2], in.disc << (8 * (slot & 3)));
                 }
-                const unsigned long long m = __ballot(inserted);
-                if (m && lane == __ffsll((long long)m) - 1) {
-                    const uint32_t add = uint32_t(__popcll(m));
-                    if (atomicAdd(&s_cnt, add) + add > LIMIT) s_ovf = 1;
-                }
+            }
+            const unsigned long long m = __ballot(inserted);
+            if (m && lane == __ffsll((long long)m) - 1) {
+                const uint32_t add = uint32_t(__popcll(m));
+                if (atomicAdd(&s_cnt, add) + add > LIMIT) s_ovf = 1;
+            }
+            if (rw.i == rw.n) {  // record done: start the prefetched one
+                r += LWG;
+                if (r >= r_end) break;
+                if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+                rw.init(at_next, lab_next, k);
+                if (r + LWG < r_end) Walker::fetch(a.att, a.labels, r + LWG, W, at_next, lab_next);
             }
         }
         __syncthreads();
