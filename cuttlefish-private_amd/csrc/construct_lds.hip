@@ -32,23 +32,30 @@ constexpr uint64_t EMPTY = ~0ull;
 constexpr uint64_t BUSY = 1ull << 63;
 constexpr int LWG = LDS_WG;
 
+struct LdsProbe {
+    int slot;         // -1: every group is full
+    uint32_t ef;      // the slot's Edge_Frequency word as read while probing (may be stale)
+    bool inserted;
+};
+
 // Find or insert `key` in an LDS table of C slots (KW u64 words each), probing
-// groups of 4 slots: the 4 top words are read together, and a group fills
-// from its first slot on (claims take the first empty slot; entries are never
-// removed), so a group with an empty slot ends an unsuccessful search.
-// Returns the slot, or -1 if every group is full.
+// groups of 4 slots: the 4 top words and the 4 e_f words are read together,
+// and a group fills from its first slot on (claims take the first empty slot;
+// entries are never removed), so a group with an empty slot ends an
+// unsuccessful search.
 template <int KW, int C>
-__device__ __forceinline__ int lds_find_or_insert(uint64_t* s_key, const Kmer<KW>& key, uint64_t h,
-                                                  bool& inserted) {
+__device__ __forceinline__ LdsProbe lds_find_or_insert(uint64_t* s_key, const uint32_t* s_ef,
+                                                       const Kmer<KW>& key, uint64_t h) {
+    typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
     constexpr uint32_t NG = uint32_t(C) / 4;
     uint32_t g = uint32_t((uint64_t(uint32_t(h)) * NG) >> 32);
     const uint64_t top = key.w[KW - 1];
-    inserted = false;
     for (uint32_t n = 0; n < NG;) {
         uint64_t cur[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j)
             cur[j] = __hip_atomic_load(&s_key[(g * 4 + j) * KW + KW - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const u32x4 ef4 = *reinterpret_cast<const u32x4*>(&s_ef[g * 4]);
         int empty = 4;
         bool busy = false;
 #pragma unroll
@@ -59,7 +66,7 @@ __device__ __forceinline__ int lds_find_or_insert(uint64_t* s_key, const Kmer<KW
             if (j >= empty) break;
             if ((cur[j] & ~BUSY) != top) continue;
             if constexpr (KW == 1) {
-                return int(g * 4 + j);
+                return {int(g * 4 + j), ef4[j], false};
             } else {
                 if (cur[j] & BUSY) {
                     busy = true;
@@ -70,7 +77,7 @@ __device__ __forceinline__ int lds_find_or_insert(uint64_t* s_key, const Kmer<KW
 #pragma unroll
                 for (int w = 0; w < KW - 1; ++w)
                     eq &= __hip_atomic_load(kp + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == key.w[w];
-                if (eq) return int(g * 4 + j);
+                if (eq) return {int(g * 4 + j), ef4[j], false};
             }
         }
         if (busy) continue;  // a matching top word is being published: re-read
@@ -91,13 +98,12 @@ __device__ __forceinline__ int lds_find_or_insert(uint64_t* s_key, const Kmer<KW
                 __atomic_thread_fence(__ATOMIC_RELEASE);
                 __hip_atomic_store(kp + KW - 1, top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
-            inserted = true;
-            return int(slot);
+            return {int(slot), 0u, true};
         }
-        if (KW == 1 && old == top) return int(slot);
+        if (KW == 1 && old == top) return {int(slot), 0u, false};
         // lost the slot to another key: re-read the group
     }
-    return -1;
+    return {-1, 0u, false};
 }
 
 template <int KW, int WMAX>
@@ -105,12 +111,11 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
     constexpr int C = lds_capacity(KW);
     static_assert(C % 64 == 0, "capacity");
     __shared__ uint64_t s_key[C * KW];
-    __shared__ uint32_t s_ef[C];
+    __shared__ __attribute__((aligned(16))) uint32_t s_ef[C];  // read 4 at a time
     __shared__ uint32_t s_st[C / 4];  // one Vertex_Status byte per slot
     __shared__ uint32_t s_item, s_cnt, s_ovf;
 
     const int tid = threadIdx.x;
-    const int lane = tid & 63;
     const uint32_t W = a.W, k = a.k;
     constexpr uint32_t LIMIT = (uint32_t(C) * 7u) / 8u;
     using Walker = RecordWalker<KW, WMAX>;
@@ -147,32 +152,34 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
             rw.load(a.att, a.labels, r, W, k);
             if (r + LWG < r_end) Walker::fetch(a.att, a.labels, r + LWG, W, at_next, lab_next);
         }
+        uint32_t my_ins = 0;  // keys this lane inserted since its last flush into s_cnt
         while (r < r_end) {
             const Inst<KW> in = rw.next();
             const uint64_t h = km_hash<KW>(in.key);
-            bool inserted = false;
             if (uint32_t((uint64_t(uint32_t(h >> 32)) * P) >> 32) == p) {
-                const int slot = lds_find_or_insert<KW, C>(s_key, in.key, h, inserted);
-                if (slot < 0) {
+                const LdsProbe pr = lds_find_or_insert<KW, C>(s_key, s_ef, in.key, h);
+                if (pr.slot < 0) {
                     s_ovf = 1;
                 } else {
-                    uint32_t old = inserted ? 0u : s_ef[slot];
+                    // Saturating update from the word seen while probing (a
+                    // stale value only costs a CAS retry).
+                    uint32_t old = pr.ef;
                     while (true) {
                         const uint32_t nw = ef_add(old, in.front, in.back);
                         if (nw == old) break;
-                        const uint32_t prev = atomicCAS(&s_ef[slot], old, nw);
+                        const uint32_t prev = atomicCAS(&s_ef[pr.slot], old, nw);
                         if (prev == old) break;
                         old = prev;
                     }
-                    if (in.disc) atomicOr(&s_st[slot >> 2], in.disc << (8 * (slot & 3)));
+                    if (in.disc) atomicOr(&s_st[pr.slot >> 2], in.disc << (8 * (pr.slot & 3)));
+                    my_ins += pr.inserted;
                 }
             }
-            const unsigned long long m = __ballot(inserted);
-            if (m && lane == __ffsll((long long)m) - 1) {
-                const uint32_t add = uint32_t(__popcll(m));
-                if (atomicAdd(&s_cnt, add) + add > LIMIT) s_ovf = 1;
-            }
-            if (rw.i == rw.n) {  // record done: start the prefetched one
+            if (rw.i == rw.n) {  // record done: account inserts, start the prefetched record
+                if (my_ins) {
+                    if (atomicAdd(&s_cnt, my_ins) + my_ins > LIMIT) s_ovf = 1;
+                    my_ins = 0;
+                }
                 r += LWG;
                 if (r >= r_end) break;
                 if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
