@@ -33,20 +33,16 @@ constexpr uint64_t BUSY = 1ull << 63;
 constexpr int LWG = LDS_WG;
 
 struct LdsProbe {
-    int slot;         // -1: every group is full
-    uint32_t ef;      // the slot's Edge_Frequency word as read while probing (may be stale)
+    int slot;  // -1: every group is full
     bool inserted;
 };
 
 // Find or insert `key` in an LDS table of C slots (KW u64 words each), probing
-// groups of 4 slots: the 4 top words and the 4 e_f words are read together,
-// and a group fills from its first slot on (claims take the first empty slot;
-// entries are never removed), so a group with an empty slot ends an
-// unsuccessful search.
+// groups of 4 slots: the 4 top words are read together, and a group fills
+// from its first slot on (claims take the first empty slot; entries are never
+// removed), so a group with an empty slot ends an unsuccessful search.
 template <int KW, int C>
-__device__ __forceinline__ LdsProbe lds_find_or_insert(uint64_t* s_key, const uint32_t* s_ef,
-                                                       const Kmer<KW>& key, uint64_t h) {
-    typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
+__device__ __forceinline__ LdsProbe lds_find_or_insert(uint64_t* s_key, const Kmer<KW>& key, uint64_t h) {
     constexpr uint32_t NG = uint32_t(C) / 4;
     uint32_t g = uint32_t((uint64_t(uint32_t(h)) * NG) >> 32);
     const uint64_t top = key.w[KW - 1];
@@ -55,7 +51,6 @@ __device__ __forceinline__ LdsProbe lds_find_or_insert(uint64_t* s_key, const ui
 #pragma unroll
         for (int j = 0; j < 4; ++j)
             cur[j] = __hip_atomic_load(&s_key[(g * 4 + j) * KW + KW - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        const u32x4 ef4 = *reinterpret_cast<const u32x4*>(&s_ef[g * 4]);
         int empty = 4;
         bool busy = false;
 #pragma unroll
@@ -66,7 +61,7 @@ __device__ __forceinline__ LdsProbe lds_find_or_insert(uint64_t* s_key, const ui
             if (j >= empty) break;
             if ((cur[j] & ~BUSY) != top) continue;
             if constexpr (KW == 1) {
-                return {int(g * 4 + j), ef4[j], false};
+                return {int(g * 4 + j), false};
             } else {
                 if (cur[j] & BUSY) {
                     busy = true;
@@ -77,7 +72,7 @@ __device__ __forceinline__ LdsProbe lds_find_or_insert(uint64_t* s_key, const ui
 #pragma unroll
                 for (int w = 0; w < KW - 1; ++w)
                     eq &= __hip_atomic_load(kp + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == key.w[w];
-                if (eq) return {int(g * 4 + j), ef4[j], false};
+                if (eq) return {int(g * 4 + j), false};
             }
         }
         if (busy) continue;  // a matching top word is being published: re-read
@@ -98,12 +93,12 @@ __device__ __forceinline__ LdsProbe lds_find_or_insert(uint64_t* s_key, const ui
                 __atomic_thread_fence(__ATOMIC_RELEASE);
                 __hip_atomic_store(kp + KW - 1, top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
-            return {int(slot), 0u, true};
+            return {int(slot), true};
         }
-        if (KW == 1 && old == top) return {int(slot), 0u, false};
+        if (KW == 1 && old == top) return {int(slot), false};
         // lost the slot to another key: re-read the group
     }
-    return {-1, 0u, false};
+    return {-1, false};
 }
 
 template <int KW, int WMAX>
@@ -111,7 +106,7 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
     constexpr int C = lds_capacity(KW);
     static_assert(C % 64 == 0, "capacity");
     __shared__ uint64_t s_key[C * KW];
-    __shared__ __attribute__((aligned(16))) uint32_t s_ef[C];  // read 4 at a time
+    __shared__ uint32_t s_ef[C];
     __shared__ uint32_t s_st[C / 4];  // one Vertex_Status byte per slot
     __shared__ uint32_t s_item, s_cnt, s_ovf;
 
@@ -157,13 +152,13 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
             const Inst<KW> in = rw.next();
             const uint64_t h = km_hash<KW>(in.key);
             if (uint32_t((uint64_t(uint32_t(h >> 32)) * P) >> 32) == p) {
-                const LdsProbe pr = lds_find_or_insert<KW, C>(s_key, s_ef, in.key, h);
+                const LdsProbe pr = lds_find_or_insert<KW, C>(s_key, in.key, h);
                 if (pr.slot < 0) {
                     s_ovf = 1;
                 } else {
-                    // Saturating update from the word seen while probing (a
-                    // stale value only costs a CAS retry).
-                    uint32_t old = pr.ef;
+                    // Saturating update (a fresh slot starts at 0; a stale
+                    // value only costs a CAS retry).
+                    uint32_t old = pr.inserted ? 0u : s_ef[pr.slot];
                     while (true) {
                         const uint32_t nw = ef_add(old, in.front, in.back);
                         if (nw == old) break;
