@@ -229,6 +229,8 @@ def main():
                     "mispredict_under_frac": rs["mispredict_under_frac"],
                     "fallback_rate": rs["fallback_rate"], "overflowed": rs["n_overflowed"],
                     "lds_buckets": rs["n_lds_buckets"],
+                    "predictor": {0: "none", 1: "mlp", 2: "constant ratio"}.get(rs["predictor"]),
+                    "holdout_err_mlp": rs["holdout_err_mlp"], "holdout_err_const": rs["holdout_err_const"],
                     "table_bytes": rs["table_bytes"]},
     }
     print(json.dumps(out), flush=True)

@@ -24,7 +24,7 @@ using namespace cfgpu;
 namespace {
 
 constexpr uint32_t G_FIXED = 16384;
-constexpr uint64_t SAMPLE_TARGET = MLP_TRAIN_MAX;
+constexpr uint64_t SAMPLE_TARGET = MLP_TRAIN_MAX + MLP_TRAIN_MAX / 3;  // 3/4 train, 1/4 held out
 constexpr uint32_t MAX_ROUNDS = 8;
 constexpr double LDS_LOAD = 0.8;  // target fill of an LDS pass table (overflow at 7/8)
 
@@ -439,6 +439,8 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
     rs.n_overflowed = rs.n_fallback_rounds = rs.n_samples = 0;
     rs.t_sample_ms = rs.t_train_ms = rs.t_infer_ms = rs.t_construct_main_ms = rs.t_fallback_ms = 0;
     rs.mlp_loss = rs.mispredict_mean_abs_log = rs.mispredict_under_frac = rs.fallback_rate = 0;
+    rs.holdout_err_mlp = rs.holdout_err_const = 0;
+    rs.predictor = 0;
     rs.t_construct_kernel_ms = 0;
     rs.n_lds_buckets = 0;
 
@@ -499,11 +501,24 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
         // 2. train on (features, log(distinct / instances)), predict all buckets.
         bool have_model = false;
         if (sample.size() >= 8) {
-            std::vector<uint32_t> idx(sample.begin(), sample.end());
+            // Every fourth sample bucket is held out to choose between the MLP
+            // and a constant distinct/instance ratio (the training mean): with
+            // few informative features the network can memorise its training
+            // buckets and predict worse than the mean.
+            auto log_ratio = [&](uint32_t b) {
+                return std::log(double(std::max<uint32_t>(1, h->h_nvert[b])) /
+                                double(std::max<uint64_t>(1, bucket_kmers(h, b))));
+            };
+            std::vector<uint32_t> idx, hold;
+            for (size_t i = 0; i < sample.size(); ++i)
+                (i % 4 == 3 || idx.size() == MLP_TRAIN_MAX ? hold : idx).push_back(sample[i]);
             std::vector<float> tgt(idx.size());
-            for (size_t i = 0; i < idx.size(); ++i)
-                tgt[i] = float(std::log(double(std::max<uint32_t>(1, h->h_nvert[idx[i]])) /
-                                        double(std::max<uint64_t>(1, bucket_kmers(h, idx[i])))));
+            double mean_t = 0.0;
+            for (size_t i = 0; i < idx.size(); ++i) {
+                tgt[i] = float(log_ratio(idx[i]));
+                mean_t += tgt[i];
+            }
+            mean_t /= double(idx.size());
             HIPCK(h, h->mlp_params.ensure(size_t(mlp_param_floats()) * 4));
             HIPCK(h, h->mlp_adam.ensure(size_t(2 * MLP_PARAMS) * 4));
             HIPCK(h, h->mlp_idx.ensure(idx.size() * 4));
@@ -527,9 +542,21 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
             rs.mlp_loss = loss;
             rs.t_train_ms = elapsed_ms(e1, e4);
             rs.t_infer_ms = elapsed_ms(e4, e5);
+            auto mlp_log = [&](uint32_t b) {
+                return std::isfinite(pred[b]) ? std::min(0.0, double(pred[b])) : 0.0;
+            };
+            double err_mlp = 0.0, err_const = 0.0;
+            for (uint32_t b : hold) {
+                err_mlp += std::fabs(mlp_log(b) - log_ratio(b));
+                err_const += std::fabs(std::min(0.0, mean_t) - log_ratio(b));
+            }
+            rs.holdout_err_mlp = hold.empty() ? 0.0 : err_mlp / double(hold.size());
+            rs.holdout_err_const = hold.empty() ? 0.0 : err_const / double(hold.size());
+            const bool use_mlp = err_mlp <= err_const;
+            rs.predictor = use_mlp ? 1u : 2u;
             for (uint32_t b : rest) {
                 const double km = double(bucket_kmers(h, b));
-                const double p = std::isfinite(pred[b]) ? km * std::exp(std::min(0.0, double(pred[b]))) : km;
+                const double p = km * std::exp(use_mlp ? mlp_log(b) : std::min(0.0, mean_t));
                 h->h_pred[b] = float(std::max(1.0, p));
             }
             have_model = true;

@@ -50,9 +50,10 @@ class RunStats(ctypes.Structure):
                                                   "t_train_ms", "t_infer_ms",
                                                   "t_construct_main_ms", "t_fallback_ms",
                                                   "t_construct_ms", "t_construct_kernel_ms")]
-                + [(n, ctypes.c_uint32) for n in ("n_lds_buckets", "reserved")]
+                + [(n, ctypes.c_uint32) for n in ("n_lds_buckets", "predictor")]
                 + [(n, ctypes.c_double) for n in ("mlp_loss", "mispredict_mean_abs_log",
-                                                  "mispredict_under_frac", "fallback_rate")])
+                                                  "mispredict_under_frac", "fallback_rate",
+                                                  "holdout_err_mlp", "holdout_err_const")])
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

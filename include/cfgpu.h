@@ -161,11 +161,13 @@ typedef struct cfgpu_run_stats {
     double t_construct_kernel_ms;     /* persistent construct kernels alone, main round
                                          (recorded when use_graph == 0) */
     uint32_t n_lds_buckets;           /* main-round buckets built in LDS */
-    uint32_t reserved;
+    uint32_t predictor;               /* sizing used: 0 none, 1 MLP, 2 constant ratio */
     double mlp_loss;                  /* final training MSE (log space) */
     double mispredict_mean_abs_log;   /* mean |log(pred/true)| over predicted buckets */
     double mispredict_under_frac;     /* fraction of predicted buckets with pred < true */
     double fallback_rate;             /* overflowed / predicted buckets */
+    double holdout_err_mlp;           /* mean |log error| on held-out sample buckets: MLP */
+    double holdout_err_const;         /*   ... and the constant mean-ratio predictor */
 } cfgpu_run_stats;
 int cfgpu_run_stats_get(cfgpu_handle* h, cfgpu_run_stats* out);
 

@@ -270,6 +270,8 @@ def test_zipf_bucket_workload_loaded_records():
                     h.construct(mode)
                     check_tables(h, None, k, l, 1, want)
                     rs = h.run_stats()
+                    if mode == "learned":
+                        assert rs["predictor"] in (1, 2) and rs["n_samples"] >= 8
                     if mp == 0:
                         assert rs["n_lds_buckets"] == 0
                     if mp == 32 and mode == "batched":
