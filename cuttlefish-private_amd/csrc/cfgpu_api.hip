@@ -25,6 +25,9 @@ namespace {
 
 constexpr uint32_t G_FIXED = 16384;
 constexpr uint64_t SAMPLE_TARGET = MLP_TRAIN_MAX + MLP_TRAIN_MAX / 3;  // 3/4 train, 1/4 held out
+// The MLP is trained only when the constant mean ratio misses the held-out
+// buckets by more than this (mean |log error|).
+constexpr double CONST_RATIO_GOOD = 0.05;
 constexpr uint32_t MAX_ROUNDS = 8;
 constexpr double LDS_LOAD = 0.8;  // target fill of an LDS pass table (overflow at 7/8)
 
@@ -83,7 +86,7 @@ struct cfgpu_handle {
     int cus = 256;
     hipStream_t stream = nullptr;
     std::string err;
-    cfgpu_tuning tune{0.6f, 256, 2048, 8, 1.15f, 1, 32};
+    cfgpu_tuning tune{0.6f, 256, 2048, 8, 1.15f, 1, 32, 0};
 
     // partition state
     DevBuf packed, frag, gbin, tile_lc, bcnt, boff, cursor, att, labels, mbin, stats, feat;
@@ -519,45 +522,49 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
                 mean_t += tgt[i];
             }
             mean_t /= double(idx.size());
-            HIPCK(h, h->mlp_params.ensure(size_t(mlp_param_floats()) * 4));
-            HIPCK(h, h->mlp_adam.ensure(size_t(2 * MLP_PARAMS) * 4));
-            HIPCK(h, h->mlp_idx.ensure(idx.size() * 4));
-            HIPCK(h, h->mlp_target.ensure(idx.size() * 4));
-            HIPCK(h, h->mlp_pred.ensure(G * 4));
-            HIPCK(h, h->mlp_loss.ensure(4));
-            HIPCK(h, hipMemcpyAsync(h->mlp_idx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, h->stream));
-            HIPCK(h, hipMemcpyAsync(h->mlp_target.p, tgt.data(), tgt.size() * 4, hipMemcpyHostToDevice, h->stream));
-            launch_mlp_train(h->mlp_params.as<float>(), h->feat.as<float>(), h->mlp_idx.as<uint32_t>(),
-                             h->mlp_target.as<float>(), uint32_t(idx.size()), 120, 3e-3f, 12345u,
-                             h->mlp_adam.as<float>(), h->mlp_loss.as<float>(), h->stream);
-            HIPCK(h, hipEventRecord(e4.e, h->stream));
-            launch_mlp_infer(h->mlp_params.as<float>(), h->feat.as<float>(), uint32_t(G),
-                             h->mlp_pred.as<float>(), h->stream);
-            std::vector<float> pred(G);
-            float loss = 0.f;
-            HIPCK(h, hipMemcpyAsync(pred.data(), h->mlp_pred.p, G * 4, hipMemcpyDeviceToHost, h->stream));
-            HIPCK(h, hipMemcpyAsync(&loss, h->mlp_loss.p, 4, hipMemcpyDeviceToHost, h->stream));
-            HIPCK(h, hipEventRecord(e5.e, h->stream));
-            HIPCK(h, hipStreamSynchronize(h->stream));
-            rs.mlp_loss = loss;
-            rs.t_train_ms = elapsed_ms(e1, e4);
-            rs.t_infer_ms = elapsed_ms(e4, e5);
-            auto mlp_log = [&](uint32_t b) {
-                return std::isfinite(pred[b]) ? std::min(0.0, double(pred[b])) : 0.0;
-            };
-            double err_mlp = 0.0, err_const = 0.0;
-            for (uint32_t b : hold) {
-                err_mlp += std::fabs(mlp_log(b) - log_ratio(b));
-                err_const += std::fabs(std::min(0.0, mean_t) - log_ratio(b));
-            }
-            rs.holdout_err_mlp = hold.empty() ? 0.0 : err_mlp / double(hold.size());
+            double err_const = 0.0;
+            for (uint32_t b : hold) err_const += std::fabs(std::min(0.0, mean_t) - log_ratio(b));
             rs.holdout_err_const = hold.empty() ? 0.0 : err_const / double(hold.size());
-            const bool use_mlp = err_mlp <= err_const;
+            std::vector<float> pred(G, 0.f);
+            bool use_mlp = false;
+            const uint32_t force = h->tune.predictor;
+            if (force == 1 || (force == 0 && (hold.empty() || rs.holdout_err_const > CONST_RATIO_GOOD))) {
+                HIPCK(h, h->mlp_params.ensure(size_t(mlp_param_floats()) * 4));
+                HIPCK(h, h->mlp_adam.ensure(size_t(2 * MLP_PARAMS) * 4));
+                HIPCK(h, h->mlp_idx.ensure(idx.size() * 4));
+                HIPCK(h, h->mlp_target.ensure(idx.size() * 4));
+                HIPCK(h, h->mlp_pred.ensure(G * 4));
+                HIPCK(h, h->mlp_loss.ensure(4));
+                HIPCK(h, hipMemcpyAsync(h->mlp_idx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, h->stream));
+                HIPCK(h, hipMemcpyAsync(h->mlp_target.p, tgt.data(), tgt.size() * 4, hipMemcpyHostToDevice, h->stream));
+                launch_mlp_train(h->mlp_params.as<float>(), h->feat.as<float>(), h->mlp_idx.as<uint32_t>(),
+                                 h->mlp_target.as<float>(), uint32_t(idx.size()), 120, 3e-3f, 12345u,
+                                 h->mlp_adam.as<float>(), h->mlp_loss.as<float>(), h->stream);
+                HIPCK(h, hipEventRecord(e4.e, h->stream));
+                launch_mlp_infer(h->mlp_params.as<float>(), h->feat.as<float>(), uint32_t(G),
+                                 h->mlp_pred.as<float>(), h->stream);
+                float loss = 0.f;
+                HIPCK(h, hipMemcpyAsync(pred.data(), h->mlp_pred.p, G * 4, hipMemcpyDeviceToHost, h->stream));
+                HIPCK(h, hipMemcpyAsync(&loss, h->mlp_loss.p, 4, hipMemcpyDeviceToHost, h->stream));
+                HIPCK(h, hipEventRecord(e5.e, h->stream));
+                HIPCK(h, hipStreamSynchronize(h->stream));
+                rs.mlp_loss = loss;
+                rs.t_train_ms = elapsed_ms(e1, e4);
+                rs.t_infer_ms = elapsed_ms(e4, e5);
+                auto mlp_log = [&](uint32_t b) {
+                    return std::isfinite(pred[b]) ? std::min(0.0, double(pred[b])) : 0.0;
+                };
+                double err_mlp = 0.0;
+                for (uint32_t b : hold) err_mlp += std::fabs(mlp_log(b) - log_ratio(b));
+                rs.holdout_err_mlp = hold.empty() ? 0.0 : err_mlp / double(hold.size());
+                use_mlp = force == 1 || hold.empty() || err_mlp <= err_const;
+            }
             rs.predictor = use_mlp ? 1u : 2u;
             for (uint32_t b : rest) {
                 const double km = double(bucket_kmers(h, b));
-                const double p = km * std::exp(use_mlp ? mlp_log(b) : std::min(0.0, mean_t));
-                h->h_pred[b] = float(std::max(1.0, p));
+                const double lr = use_mlp ? (std::isfinite(pred[b]) ? std::min(0.0, double(pred[b])) : 0.0)
+                                          : std::min(0.0, mean_t);
+                h->h_pred[b] = float(std::max(1.0, km * std::exp(lr)));
             }
             have_model = true;
         }
@@ -978,7 +985,8 @@ int cfgpu_tuning_set(cfgpu_handle* h, const cfgpu_tuning* in) {
     if (!(in->load_factor >= 0.2f && in->load_factor <= 0.85f) ||
         (in->workgroup != 256 && in->workgroup != 512) || in->chunk_records < 64 ||
         in->chunk_records > (1u << 24) || in->wgs_per_cu < 1 || in->wgs_per_cu > 32 ||
-        !(in->margin >= 0.01f && in->margin <= 4.0f) || in->use_graph > 1 || in->max_passes > 4096)
+        !(in->margin >= 0.01f && in->margin <= 4.0f) || in->use_graph > 1 || in->max_passes > 4096 ||
+        in->predictor > 2)
         return fail(h, CFGPU_E_ARG, "tuning out of range");
     h->tune = *in;
     return CFGPU_OK;
@@ -993,7 +1001,7 @@ int cfgpu_autotune(cfgpu_handle* h, uint32_t mode, cfgpu_tune_trial* trials_out,
     for (float lf : {0.5f, 0.65f})
         for (uint32_t wg : {256u, 512u})
             for (uint32_t ch : {1024u, 4096u})
-                for (uint32_t wpc : {4u, 8u}) grid.push_back({lf, wg, ch, wpc, base.margin, base.use_graph, base.max_passes});
+                for (uint32_t wpc : {4u, 8u}) grid.push_back({lf, wg, ch, wpc, base.margin, base.use_graph, base.max_passes, base.predictor});
     cfgpu_tuning best = base;
     double best_ms = 1e300;
     uint32_t w = 0;

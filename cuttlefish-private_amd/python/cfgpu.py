@@ -63,7 +63,7 @@ class Tuning(ctypes.Structure):
     _fields_ = [("load_factor", ctypes.c_float), ("workgroup", ctypes.c_uint32),
                 ("chunk_records", ctypes.c_uint32), ("wgs_per_cu", ctypes.c_uint32),
                 ("margin", ctypes.c_float), ("use_graph", ctypes.c_uint32),
-                ("max_passes", ctypes.c_uint32)]
+                ("max_passes", ctypes.c_uint32), ("predictor", ctypes.c_uint32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

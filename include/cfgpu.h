@@ -184,6 +184,8 @@ typedef struct cfgpu_tuning {
     uint32_t max_passes;      /* batched paths: buckets needing at most this many LDS hash
                                  passes are built in LDS, larger ones with global atomics
                                  (0: always global) */
+    uint32_t predictor;       /* learned sizing: 0 chooses on held-out sample buckets,
+                                 1 always the MLP, 2 always the constant mean ratio */
 } cfgpu_tuning;
 int cfgpu_tuning_get(cfgpu_handle* h, cfgpu_tuning* out);
 int cfgpu_tuning_set(cfgpu_handle* h, const cfgpu_tuning* in);

@@ -195,9 +195,10 @@ def test_overflow_fallback_is_exact():
         h.partition(inp)
         # global-atomic tables only (every LDS pass table of these small
         # buckets would have room whatever the estimate)
-        h.set_tuning(margin=0.25, load_factor=0.8, max_passes=0)
+        h.set_tuning(margin=0.25, load_factor=0.8, max_passes=0, predictor=1)
         h.construct("learned")
         rs = h.run_stats()
+        assert rs["predictor"] == 1 and np.isfinite(rs["mlp_loss"]) and rs["t_train_ms"] > 0
         say(f"[case] overflowed={rs['n_overflowed']} rounds={rs['n_fallback_rounds']}")
         assert rs["n_overflowed"] > 0 and rs["n_fallback_rounds"] >= 1
         assert rs["n_lds_buckets"] == 0
@@ -216,9 +217,10 @@ def test_lds_overflow_fallback_is_exact():
     say("[case] lds-overflow")
     with C.Handle(k, l, 1) as h:
         h.load_buckets(att, lab, off)
-        h.set_tuning(margin=0.2, max_passes=4096)
+        h.set_tuning(margin=0.2, max_passes=4096, predictor=2)
         h.construct("learned")
         rs = h.run_stats()
+        assert rs["predictor"] == 2
         say(f"[case] lds-overflow overflowed={rs['n_overflowed']} lds={rs['n_lds_buckets']}")
         assert rs["n_lds_buckets"] > 0 and rs["n_overflowed"] > 0 and rs["n_fallback_rounds"] >= 1
         check_tables(h, None, k, l, 1, want)
@@ -291,3 +293,5 @@ def test_load_buckets_rejects_malformed_records():
             h.load_buckets(bad, lab, off)
         with pytest.raises(C.CfgpuError):
             h.construct("learned")                       # nothing loaded
+        with pytest.raises(C.CfgpuError):
+            h.set_tuning(predictor=3)                    # no such predictor
