@@ -29,14 +29,26 @@ struct PartitionArgs {
 // g-change position (+1, 0 = none).
 void launch_minimizer_g(const PartitionArgs& a, uint32_t* gbin, uint64_t* tile_lc, hipStream_t s);
 
-// Pass 2 (emit = false): per-bucket record counts.  Pass 3 (emit = true):
-// scatter records into buckets at cursor[g] (pre-set to the bucket offsets).
-void launch_pieces(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc,
-                   uint32_t W, bool emit, uint32_t* bucket_cnt, uint64_t* cursor,
-                   uint32_t* att, uint64_t* labels, uint8_t* mbin, hipStream_t s);
-
-// Exclusive scan of G counts -> off[G+1]; also copies off into cursor.
-void launch_bucket_scan(const uint32_t* cnt, uint32_t G, uint64_t* off, uint64_t* cursor,
+// Records are bucketed in two levels.  Level 1 splits by coarse bin
+// g >> FINE_BITS over big tiles of P_SUB partition tiles; level 2 splits each
+// coarse bin into its 2^FINE_BITS buckets.
+constexpr int FINE_BITS = 6;
+constexpr int N_COARSE = 256;
+constexpr int P_SUB = 32;
+static_assert((N_COARSE << FINE_BITS) == 16384, "coarse x fine bins cover the 16384 subgraphs");
+uint64_t pieces_big_tiles(uint64_t n_bases);
+// Pass 2: record counts per (coarse bin, big tile) in hist[N_COARSE][n_big],
+// scanned in place; coff[N_COARSE+1] = coarse bin offsets (coff[N_COARSE] =
+// records); ctot[N_COARSE] scratch.
+void launch_pieces_hist(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc,
+                        uint32_t* hist, uint64_t* ctot, uint64_t* coff, hipStream_t s);
+// Pass 3: records into the level-1 arrays, grouped by coarse bin.
+void launch_pieces_emit(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc,
+                        const uint32_t* hist, const uint64_t* coff, uint32_t W, uint32_t* att1,
+                        uint64_t* lab1, uint8_t* mbin1, hipStream_t s);
+// Level 2: level-1 records into per-bucket order; boff[G+1] bucket offsets.
+void launch_bucket_fine(const uint32_t* att1, const uint64_t* lab1, const uint8_t* mbin1, const uint64_t* coff,
+                        uint32_t W, uint64_t* boff, uint32_t* att, uint64_t* labels, uint8_t* mbin,
                         hipStream_t s);
 
 // Per-bucket statistics: [n_rec, n_kmer_instances, n_bases, 0] as u64 and the

@@ -9,6 +9,13 @@
 // Pieces are then cut at every g change and every w (k-1)-mers from a run's
 // start, which is the length cap `2(k-1)-l` of the reference's sequential loop
 // (src/Graph_Partitioner.cpp:30, 541).
+//
+// Records reach their buckets in two levels, with no per-record global
+// atomics: big tiles (P_SUB sub-tiles) count their records per coarse bin
+// (g >> FINE_BITS); a scan gives each (coarse bin, big tile) a contiguous run,
+// which the emit pass fills; then one workgroup per coarse bin moves its
+// records into the 2^FINE_BITS final buckets.  Each output run is written by
+// a single workgroup, so HBM lines are written whole.
 #include "cf_kernels.hpp"
 
 namespace cfgpu {
@@ -149,34 +156,31 @@ __device__ __forceinline__ uint32_t block_incl_max(uint32_t v, uint32_t* s_wave)
     return v > pre ? v : pre;
 }
 
-template <bool EMIT>
-__global__ __launch_bounds__(P_WG) void k_pieces(PartitionArgs a, const uint32_t* __restrict__ gbin,
-                                                 const uint64_t* __restrict__ tile_lc, uint32_t W,
-                                                 uint32_t* __restrict__ bucket_cnt,
-                                                 unsigned long long* __restrict__ cursor,
-                                                 uint32_t* __restrict__ att,
-                                                 uint64_t* __restrict__ labels,
-                                                 uint8_t* __restrict__ mbin) {
+// Weak super k-mers of sub-tile `tile` (positions [tile*P_TILE, +P_TILE)):
+// calls emit(g, mb, len, l_disc, r_disc, start) once per piece, where mb is
+// the 4 hash bits above g and start the first base of the record.  Every
+// thread of the workgroup must call it (it synchronises).  The run start
+// before the sub-tile comes from tile_lc: a wave looks back 64 tiles at a
+// time; only runs longer than a tile need more than one step.
+template <class Emit>
+__device__ __forceinline__ void sub_tile_pieces(const PartitionArgs& a, const uint32_t* __restrict__ gbin,
+                                                const uint64_t* __restrict__ tile_lc, uint64_t tile,
+                                                uint32_t* s_g, unsigned long long* s_carry,
+                                                uint32_t* s_wave, Emit&& emit) {
     constexpr int PPT = P_TILE / P_WG;  // positions per thread
-    __shared__ uint32_t s_g[P_TILE + WIN_MAX + 2];
-    __shared__ unsigned long long s_carry;
-    __shared__ uint32_t s_wave[P_WG / 64];
-
     const int tid = threadIdx.x;
-    const int64_t t0 = int64_t(blockIdx.x) * P_TILE;
+    const int64_t t0 = int64_t(tile) * P_TILE;
     const int64_t q0 = t0 - 1;
     const int64_t N = int64_t(a.n_bases);
     const unsigned k = a.k, w = a.w;
 
+    __syncthreads();  // the previous sub-tile is done with s_g
     for (int i = tid; i < P_TILE + int(w) + 2; i += P_WG) {
         const int64_t p = q0 + i;
         s_g[i] = (p >= 0 && p < N) ? gbin[p] : G_INVALID;
     }
-    // Carry-in: last g-change position (+1) before this tile.  A wave looks
-    // back 64 tiles at a time; only runs longer than a tile need more than one
-    // step.
     if (tid < 64) {
-        int64_t j = int64_t(blockIdx.x) - 1 - tid;
+        int64_t j = int64_t(tile) - 1 - tid;
         unsigned long long found = 0;
         while (true) {
             const unsigned long long v = j >= 0 ? tile_lc[j] : 0ull;
@@ -188,7 +192,7 @@ __global__ __launch_bounds__(P_WG) void k_pieces(PartitionArgs a, const uint32_t
             }
             j -= 64;
         }
-        if (tid == 0) s_carry = found;
+        if (tid == 0) *s_carry = found;
     }
     __syncthreads();
 
@@ -213,7 +217,8 @@ __global__ __launch_bounds__(P_WG) void k_pieces(PartitionArgs a, const uint32_t
         // first lane of a wave: the exclusive value is the max over earlier waves
         for (int j = 0; j < (tid >> 6); ++j) excl = s_wave[j] > excl ? s_wave[j] : excl;
     }
-    int64_t rs = excl ? q0 + int64_t(excl) : (s_carry ? int64_t(s_carry) - 1 : -1);
+    const unsigned long long carry = *s_carry;
+    int64_t rs = excl ? q0 + int64_t(excl) : (carry ? int64_t(carry) - 1 : -1);
 
 #pragma unroll 1
     for (int j = 0; j < PPT; ++j) {
@@ -238,35 +243,42 @@ __global__ __launch_bounds__(P_WG) void k_pieces(PartitionArgs a, const uint32_t
         const uint32_t l_disc = lj && ((pv & 0xFFFFu) != g);
         const uint32_t r_disc = rj && ((nv & 0xFFFFu) != g);
         const uint32_t len = lj + uint32_t(e - idx) + (k - 2) + rj;
-        if constexpr (!EMIT) {
-            atomicAdd(&bucket_cnt[g], 1u);
-        } else {
-            const unsigned long long slot = atomicAdd(&cursor[g], 1ull);
-            att[slot] = len | (l_disc << 8) | (r_disc << 9) | (g << 16);
-            mbin[slot] = uint8_t((cur >> 16) & 15u);
-            const uint64_t start = uint64_t(p) - lj;
-            for (uint32_t wj = 0; wj < W; ++wj) {
-                uint64_t x = 0;
-                if (32 * wj < len) {
-                    x = stream_get64(a.packed, start + 32ull * wj);
-                    const uint32_t nb = len - 32 * wj;
-                    if (nb < 32) x &= ~0ull << (64 - 2 * nb);
-                }
-                labels[slot * W + (W - 1 - wj)] = x;
-            }
-        }
+        emit(g, (cur >> 16) & 15u, len, l_disc, r_disc, uint64_t(p) - lj);
     }
 }
 
-__global__ __launch_bounds__(1024) void k_bucket_scan(const uint32_t* __restrict__ cnt, uint32_t G,
-                                                      uint64_t* __restrict__ off,
-                                                      uint64_t* __restrict__ cursor) {
+// Pass 2: per big tile (P_SUB sub-tiles), record counts per coarse bin
+// (g >> FINE_BITS) into hist[c * n_big + t].  No global atomics.
+__global__ __launch_bounds__(P_WG) void k_pieces_hist(PartitionArgs a, const uint32_t* __restrict__ gbin,
+                                                      const uint64_t* __restrict__ tile_lc,
+                                                      uint64_t n_tiles, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t s_g[P_TILE + WIN_MAX + 2];
+    __shared__ unsigned long long s_carry;
+    __shared__ uint32_t s_wave[P_WG / 64];
+    __shared__ uint32_t s_hist[N_COARSE];
+    const uint64_t big = blockIdx.x, n_big = gridDim.x;
+    for (int i = threadIdx.x; i < N_COARSE; i += P_WG) s_hist[i] = 0;
+    const uint64_t t_end = min(n_tiles, (big + 1) * P_SUB);
+    for (uint64_t t = big * P_SUB; t < t_end; ++t)
+        sub_tile_pieces(a, gbin, tile_lc, t, s_g, &s_carry, s_wave,
+                        [&](uint32_t g, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t) {
+                            atomicAdd(&s_hist[g >> FINE_BITS], 1u);
+                        });
+    __syncthreads();
+    for (int i = threadIdx.x; i < N_COARSE; i += P_WG) hist[uint64_t(i) * n_big + big] = s_hist[i];
+}
+
+// Per coarse bin c (one workgroup each): exclusive scan of hist[c][*] over
+// the big tiles, in place, and the bin's total into ctot[c].
+__global__ __launch_bounds__(1024) void k_hist_scan(uint32_t* __restrict__ hist, uint64_t n_big,
+                                                    uint64_t* __restrict__ ctot) {
     __shared__ uint64_t s[1024];
     const int tid = threadIdx.x;
-    const uint32_t per = (G + 1023) / 1024;
-    const uint32_t b0 = tid * per, b1 = min(G, b0 + per);
+    uint32_t* row = hist + uint64_t(blockIdx.x) * n_big;
+    const uint64_t per = (n_big + 1023) / 1024;
+    const uint64_t b0 = min(n_big, uint64_t(tid) * per), b1 = min(n_big, b0 + per);
     uint64_t sum = 0;
-    for (uint32_t i = b0; i < b1; ++i) sum += cnt[i];
+    for (uint64_t i = b0; i < b1; ++i) sum += row[i];
     s[tid] = sum;
     __syncthreads();
     for (int d = 1; d < 1024; d <<= 1) {
@@ -276,12 +288,106 @@ __global__ __launch_bounds__(1024) void k_bucket_scan(const uint32_t* __restrict
         __syncthreads();
     }
     uint64_t run = s[tid] - sum;
-    for (uint32_t i = b0; i < b1; ++i) {
-        off[i] = run;
-        cursor[i] = run;
-        run += cnt[i];
+    for (uint64_t i = b0; i < b1; ++i) {
+        const uint32_t c = row[i];
+        row[i] = uint32_t(run);  // a coarse bin holds < 2^32 records
+        run += c;
     }
-    if (tid == 1023) off[G] = s[1023];
+    if (tid == 1023) ctot[blockIdx.x] = s[1023];
+}
+
+// Exclusive scan of the N_COARSE bin totals -> coff[0..N_COARSE].
+__global__ __launch_bounds__(N_COARSE) void k_coarse_scan(const uint64_t* __restrict__ ctot,
+                                                          uint64_t* __restrict__ coff) {
+    __shared__ uint64_t s[N_COARSE];
+    const int tid = threadIdx.x;
+    s[tid] = ctot[tid];
+    __syncthreads();
+    for (int d = 1; d < N_COARSE; d <<= 1) {
+        const uint64_t v = tid >= d ? s[tid - d] : 0;
+        __syncthreads();
+        s[tid] += v;
+        __syncthreads();
+    }
+    coff[tid + 1] = s[tid];
+    if (tid == 0) coff[0] = 0;
+}
+
+// Pass 3: the same pieces again, each big tile writing its records of coarse
+// bin c as one contiguous run at coff[c] + hist[c][t] (level-1 arrays).
+__global__ __launch_bounds__(P_WG) void k_pieces_emit(PartitionArgs a, const uint32_t* __restrict__ gbin,
+                                                      const uint64_t* __restrict__ tile_lc, uint64_t n_tiles,
+                                                      const uint32_t* __restrict__ hist,
+                                                      const uint64_t* __restrict__ coff, uint32_t W,
+                                                      uint32_t* __restrict__ att, uint64_t* __restrict__ labels,
+                                                      uint8_t* __restrict__ mbin) {
+    __shared__ uint32_t s_g[P_TILE + WIN_MAX + 2];
+    __shared__ unsigned long long s_carry;
+    __shared__ uint32_t s_wave[P_WG / 64];
+    __shared__ uint64_t s_base[N_COARSE];
+    __shared__ uint32_t s_cur[N_COARSE];
+    const uint64_t big = blockIdx.x, n_big = gridDim.x;
+    for (int i = threadIdx.x; i < N_COARSE; i += P_WG) {
+        s_base[i] = coff[i] + hist[uint64_t(i) * n_big + big];
+        s_cur[i] = 0;
+    }
+    const uint64_t t_end = min(n_tiles, (big + 1) * P_SUB);
+    for (uint64_t t = big * P_SUB; t < t_end; ++t)
+        sub_tile_pieces(a, gbin, tile_lc, t, s_g, &s_carry, s_wave,
+                        [&](uint32_t g, uint32_t mb, uint32_t len, uint32_t l_disc, uint32_t r_disc,
+                            uint64_t start) {
+                            const uint32_t c = g >> FINE_BITS;
+                            const uint64_t slot = s_base[c] + atomicAdd(&s_cur[c], 1u);
+                            att[slot] = len | (l_disc << 8) | (r_disc << 9) | (g << 16);
+                            mbin[slot] = uint8_t(mb);
+                            for (uint32_t wj = 0; wj < W; ++wj) {
+                                uint64_t x = 0;
+                                if (32 * wj < len) {
+                                    x = stream_get64(a.packed, start + 32ull * wj);
+                                    const uint32_t nb = len - 32 * wj;
+                                    if (nb < 32) x &= ~0ull << (64 - 2 * nb);
+                                }
+                                labels[slot * W + (W - 1 - wj)] = x;
+                            }
+                        });
+}
+
+// Level 2: one workgroup per coarse bin moves its records into the final
+// per-g order (the bin's 2^FINE_BITS buckets) and writes their offsets.
+// Records of one bucket are written in increasing slots by one workgroup.
+__global__ __launch_bounds__(1024) void k_bucket_fine(const uint32_t* __restrict__ att1,
+                                                      const uint64_t* __restrict__ lab1,
+                                                      const uint8_t* __restrict__ mbin1,
+                                                      const uint64_t* __restrict__ coff, uint32_t W,
+                                                      uint64_t* __restrict__ boff, uint32_t* __restrict__ att,
+                                                      uint64_t* __restrict__ labels, uint8_t* __restrict__ mbin) {
+    constexpr uint32_t NF = 1u << FINE_BITS;
+    __shared__ uint32_t s_cnt[NF];
+    __shared__ unsigned long long s_cur[NF];
+    const uint32_t c = blockIdx.x;
+    const int tid = threadIdx.x;
+    const uint64_t r0 = coff[c], r1 = coff[c + 1];
+    for (uint32_t i = tid; i < NF; i += 1024) s_cnt[i] = 0;
+    __syncthreads();
+    for (uint64_t r = r0 + tid; r < r1; r += 1024) atomicAdd(&s_cnt[(att1[r] >> 16) & (NF - 1)], 1u);
+    __syncthreads();
+    if (tid == 0) {
+        uint64_t run = r0;
+        for (uint32_t f = 0; f < NF; ++f) {
+            boff[c * NF + f] = run;
+            s_cur[f] = run;
+            run += s_cnt[f];
+        }
+        if (c + 1 == gridDim.x) boff[(c + 1) * NF] = r1;
+    }
+    __syncthreads();
+    for (uint64_t r = r0 + tid; r < r1; r += 1024) {
+        const uint32_t at = att1[r];
+        const uint64_t slot = atomicAdd(&s_cur[(at >> 16) & (NF - 1)], 1ull);
+        att[slot] = at;
+        mbin[slot] = mbin1[r];
+        for (uint32_t wj = 0; wj < W; ++wj) labels[slot * W + wj] = lab1[r * W + wj];
+    }
 }
 
 __global__ __launch_bounds__(256) void k_bucket_features(const uint64_t* __restrict__ off,
@@ -329,23 +435,36 @@ void launch_minimizer_g(const PartitionArgs& a, uint32_t* gbin, uint64_t* tile_l
     hipLaunchKernelGGL(k_minimizer_g, dim3(uint32_t(tiles)), dim3(P_WG), 0, s, a, gbin, tile_lc);
 }
 
-void launch_pieces(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc,
-                   uint32_t W, bool emit, uint32_t* bucket_cnt, uint64_t* cursor, uint32_t* att,
-                   uint64_t* labels, uint8_t* mbin, hipStream_t s) {
-    const uint64_t tiles = (a.n_bases + P_TILE - 1) / P_TILE;
-    if (!tiles) return;
-    auto* cur = reinterpret_cast<unsigned long long*>(cursor);
-    if (emit)
-        hipLaunchKernelGGL(k_pieces<true>, dim3(uint32_t(tiles)), dim3(P_WG), 0, s, a, gbin, tile_lc,
-                           W, bucket_cnt, cur, att, labels, mbin);
-    else
-        hipLaunchKernelGGL(k_pieces<false>, dim3(uint32_t(tiles)), dim3(P_WG), 0, s, a, gbin,
-                           tile_lc, W, bucket_cnt, cur, att, labels, mbin);
+uint64_t pieces_big_tiles(uint64_t n_bases) {
+    const uint64_t tiles = (n_bases + P_TILE - 1) / P_TILE;
+    return (tiles + P_SUB - 1) / P_SUB;
 }
 
-void launch_bucket_scan(const uint32_t* cnt, uint32_t G, uint64_t* off, uint64_t* cursor,
+void launch_pieces_hist(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc,
+                        uint32_t* hist, uint64_t* ctot, uint64_t* coff, hipStream_t s) {
+    const uint64_t tiles = (a.n_bases + P_TILE - 1) / P_TILE;
+    const uint64_t n_big = pieces_big_tiles(a.n_bases);
+    if (!n_big) return;
+    hipLaunchKernelGGL(k_pieces_hist, dim3(uint32_t(n_big)), dim3(P_WG), 0, s, a, gbin, tile_lc, tiles, hist);
+    hipLaunchKernelGGL(k_hist_scan, dim3(N_COARSE), dim3(1024), 0, s, hist, n_big, ctot);
+    hipLaunchKernelGGL(k_coarse_scan, dim3(1), dim3(N_COARSE), 0, s, ctot, coff);
+}
+
+void launch_pieces_emit(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc,
+                        const uint32_t* hist, const uint64_t* coff, uint32_t W, uint32_t* att1,
+                        uint64_t* lab1, uint8_t* mbin1, hipStream_t s) {
+    const uint64_t tiles = (a.n_bases + P_TILE - 1) / P_TILE;
+    const uint64_t n_big = pieces_big_tiles(a.n_bases);
+    if (!n_big) return;
+    hipLaunchKernelGGL(k_pieces_emit, dim3(uint32_t(n_big)), dim3(P_WG), 0, s, a, gbin, tile_lc, tiles, hist,
+                       coff, W, att1, lab1, mbin1);
+}
+
+void launch_bucket_fine(const uint32_t* att1, const uint64_t* lab1, const uint8_t* mbin1, const uint64_t* coff,
+                        uint32_t W, uint64_t* boff, uint32_t* att, uint64_t* labels, uint8_t* mbin,
                         hipStream_t s) {
-    hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, s, cnt, G, off, cursor);
+    hipLaunchKernelGGL(k_bucket_fine, dim3(N_COARSE), dim3(1024), 0, s, att1, lab1, mbin1, coff, W, boff, att,
+                       labels, mbin);
 }
 
 void launch_bucket_features(const uint64_t* off, const uint32_t* att, const uint8_t* mbin,
