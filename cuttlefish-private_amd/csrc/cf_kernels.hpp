@@ -30,20 +30,22 @@ struct PartitionArgs {
 void launch_minimizer_g(const PartitionArgs& a, uint32_t* gbin, uint64_t* tile_lc, hipStream_t s);
 
 // Records are bucketed in two levels.  Level 1 splits by coarse bin
-// g >> FINE_BITS over big tiles of P_SUB partition tiles; level 2 splits each
-// coarse bin into its 2^FINE_BITS buckets.
+// g >> FINE_BITS: the input is cut into n_span contiguous spans (one
+// E_WG-thread workgroup each, walking E_TILE positions at a time); level 2
+// splits each coarse bin into its 2^FINE_BITS buckets.
 constexpr int FINE_BITS = 6;
 constexpr int N_COARSE = 256;
-constexpr int P_SUB = 32;
+constexpr int E_WG = 1024;
+constexpr int E_TILE = 8192;
 static_assert((N_COARSE << FINE_BITS) == 16384, "coarse x fine bins cover the 16384 subgraphs");
-uint64_t pieces_big_tiles(uint64_t n_bases);
-// Pass 2: record counts per (coarse bin, big tile) in hist[N_COARSE][n_big],
+uint32_t pieces_spans(uint64_t n_bases, uint32_t n_cu);
+// Pass 2: record counts per (coarse bin, span) in hist[N_COARSE][n_span],
 // scanned in place; coff[N_COARSE+1] = coarse bin offsets (coff[N_COARSE] =
 // records); ctot[N_COARSE] scratch.
-void launch_pieces_hist(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc,
+void launch_pieces_hist(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, uint32_t n_span,
                         uint32_t* hist, uint64_t* ctot, uint64_t* coff, hipStream_t s);
 // Pass 3: records into the level-1 arrays, grouped by coarse bin.
-void launch_pieces_emit(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc,
+void launch_pieces_emit(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, uint32_t n_span,
                         const uint32_t* hist, const uint64_t* coff, uint32_t W, uint32_t* att1,
                         uint64_t* lab1, uint8_t* mbin1, hipStream_t s);
 // Level 2: level-1 records into per-bucket order; boff[G+1] bucket offsets.

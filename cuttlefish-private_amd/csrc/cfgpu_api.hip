@@ -711,10 +711,10 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
         d_packed = h->packed.as<uint64_t>();
         d_frag = h->frag.as<uint64_t>();
     }
-    const uint64_t n_big = pieces_big_tiles(n_bases);
+    const uint32_t n_span = pieces_spans(n_bases, uint32_t(h->cus));
     HIPCK(h, h->gbin.ensure(n_bases * 4));
     HIPCK(h, h->tile_lc.ensure(tiles * 8));
-    HIPCK(h, h->hist.ensure(size_t(N_COARSE) * n_big * 4));
+    HIPCK(h, h->hist.ensure(size_t(N_COARSE) * n_span * 4));
     HIPCK(h, h->ctot.ensure(size_t(N_COARSE) * 8));
     HIPCK(h, h->coff.ensure(size_t(N_COARSE + 1) * 8));
     HIPCK(h, h->boff.ensure((G + 1) * 8));
@@ -740,7 +740,7 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     if (n_frag) launch_minimizer_g(a, h->gbin.as<uint32_t>(), h->tile_lc.as<uint64_t>(), h->stream);
     HIPCK(h, hipEventRecord(e1.e, h->stream));
     if (n_frag)
-        launch_pieces_hist(a, h->gbin.as<uint32_t>(), h->tile_lc.as<uint64_t>(), h->hist.as<uint32_t>(),
+        launch_pieces_hist(a, h->gbin.as<uint32_t>(), h->tile_lc.as<uint64_t>(), n_span, h->hist.as<uint32_t>(),
                            h->ctot.as<uint64_t>(), h->coff.as<uint64_t>(), h->stream);
     HIPCK(h, hipEventRecord(e2.e, h->stream));
     uint64_t n_rec = 0;
@@ -754,7 +754,7 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     HIPCK(h, h->mbin.ensure(n_rec));
     HIPCK(h, hipEventRecord(e3.e, h->stream));
     if (n_frag)
-        launch_pieces_emit(a, h->gbin.as<uint32_t>(), h->tile_lc.as<uint64_t>(), h->hist.as<uint32_t>(),
+        launch_pieces_emit(a, h->gbin.as<uint32_t>(), h->tile_lc.as<uint64_t>(), n_span, h->hist.as<uint32_t>(),
                            h->coff.as<uint64_t>(), h->W, h->att1.as<uint32_t>(), h->lab1.as<uint64_t>(),
                            h->mbin1.as<uint8_t>(), h->stream);
     launch_bucket_fine(h->att1.as<uint32_t>(), h->lab1.as<uint64_t>(), h->mbin1.as<uint8_t>(),

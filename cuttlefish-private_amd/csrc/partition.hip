@@ -16,6 +16,8 @@
 // which the emit pass fills; then one workgroup per coarse bin moves its
 // records into the 2^FINE_BITS final buckets.  Each output run is written by
 // a single workgroup, so HBM lines are written whole.
+#include <algorithm>
+
 #include "cf_kernels.hpp"
 
 namespace cfgpu {
@@ -156,31 +158,32 @@ __device__ __forceinline__ uint32_t block_incl_max(uint32_t v, uint32_t* s_wave)
     return v > pre ? v : pre;
 }
 
-// Weak super k-mers of sub-tile `tile` (positions [tile*P_TILE, +P_TILE)):
-// calls emit(g, mb, len, l_disc, r_disc, start) once per piece, where mb is
-// the 4 hash bits above g and start the first base of the record.  Every
-// thread of the workgroup must call it (it synchronises).  The run start
-// before the sub-tile comes from tile_lc: a wave looks back 64 tiles at a
-// time; only runs longer than a tile need more than one step.
-template <class Emit>
-__device__ __forceinline__ void sub_tile_pieces(const PartitionArgs& a, const uint32_t* __restrict__ gbin,
-                                                const uint64_t* __restrict__ tile_lc, uint64_t tile,
-                                                uint32_t* s_g, unsigned long long* s_carry,
-                                                uint32_t* s_wave, Emit&& emit) {
-    constexpr int PPT = P_TILE / P_WG;  // positions per thread
+// Weak super k-mers with their first (k-1)-mer in [t0, t0 + TILE) (t0 a
+// multiple of P_TILE): calls emit(g, mb, len, l_disc, r_disc, start) once per
+// piece, where mb is the 4 hash bits above g and start the first base of the
+// record.  Every thread of the WG-thread workgroup must call it (it
+// synchronises).  The run start before t0 comes from tile_lc: a wave looks
+// back 64 minimizer tiles at a time; only runs longer than a tile need more
+// than one step.
+template <int WG, int TILE, class Emit>
+__device__ __forceinline__ void tile_pieces(const PartitionArgs& a, const uint32_t* __restrict__ gbin,
+                                            const uint64_t* __restrict__ tile_lc, uint64_t t0,
+                                            uint32_t* s_g, unsigned long long* s_carry,
+                                            uint32_t* s_wave, Emit&& emit) {
+    constexpr int PPT = TILE / WG;  // positions per thread
+    static_assert(PPT <= 32 && TILE % P_TILE == 0, "tile shape");
     const int tid = threadIdx.x;
-    const int64_t t0 = int64_t(tile) * P_TILE;
-    const int64_t q0 = t0 - 1;
+    const int64_t q0 = int64_t(t0) - 1;
     const int64_t N = int64_t(a.n_bases);
     const unsigned k = a.k, w = a.w;
 
-    __syncthreads();  // the previous sub-tile is done with s_g
-    for (int i = tid; i < P_TILE + int(w) + 2; i += P_WG) {
+    __syncthreads();  // the previous tile is done with s_g
+    for (int i = tid; i < TILE + int(w) + 2; i += WG) {
         const int64_t p = q0 + i;
         s_g[i] = (p >= 0 && p < N) ? gbin[p] : G_INVALID;
     }
     if (tid < 64) {
-        int64_t j = int64_t(tile) - 1 - tid;
+        int64_t j = int64_t(t0 / P_TILE) - 1 - tid;
         unsigned long long found = 0;
         while (true) {
             const unsigned long long v = j >= 0 ? tile_lc[j] : 0ull;
@@ -247,25 +250,36 @@ __device__ __forceinline__ void sub_tile_pieces(const PartitionArgs& a, const ui
     }
 }
 
-// Pass 2: per big tile (P_SUB sub-tiles), record counts per coarse bin
-// (g >> FINE_BITS) into hist[c * n_big + t].  No global atomics.
-__global__ __launch_bounds__(P_WG) void k_pieces_hist(PartitionArgs a, const uint32_t* __restrict__ gbin,
+// Span s of the input (E_TILE tiles [s*per, (s+1)*per), one workgroup each)
+// for the level-1 passes.
+struct Span {
+    uint64_t t_begin, t_end;  // E_TILE tile indices
+};
+__device__ __forceinline__ Span span_of(uint64_t n_bases) {
+    const uint64_t n_t = (n_bases + E_TILE - 1) / E_TILE;
+    const uint64_t per = (n_t + gridDim.x - 1) / gridDim.x;
+    const uint64_t b = min(n_t, uint64_t(blockIdx.x) * per);
+    return {b, min(n_t, b + per)};
+}
+
+// Pass 2: records per coarse bin (g >> FINE_BITS) of each span into
+// hist[c * n_span + span].  No global atomics.
+__global__ __launch_bounds__(E_WG) void k_pieces_hist(PartitionArgs a, const uint32_t* __restrict__ gbin,
                                                       const uint64_t* __restrict__ tile_lc,
-                                                      uint64_t n_tiles, uint32_t* __restrict__ hist) {
-    __shared__ uint32_t s_g[P_TILE + WIN_MAX + 2];
+                                                      uint32_t* __restrict__ hist) {
+    __shared__ uint32_t s_g[E_TILE + WIN_MAX + 2];
     __shared__ unsigned long long s_carry;
-    __shared__ uint32_t s_wave[P_WG / 64];
+    __shared__ uint32_t s_wave[E_WG / 64];
     __shared__ uint32_t s_hist[N_COARSE];
-    const uint64_t big = blockIdx.x, n_big = gridDim.x;
-    for (int i = threadIdx.x; i < N_COARSE; i += P_WG) s_hist[i] = 0;
-    const uint64_t t_end = min(n_tiles, (big + 1) * P_SUB);
-    for (uint64_t t = big * P_SUB; t < t_end; ++t)
-        sub_tile_pieces(a, gbin, tile_lc, t, s_g, &s_carry, s_wave,
-                        [&](uint32_t g, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t) {
-                            atomicAdd(&s_hist[g >> FINE_BITS], 1u);
-                        });
+    for (int i = threadIdx.x; i < N_COARSE; i += E_WG) s_hist[i] = 0;
+    const Span sp = span_of(a.n_bases);
+    for (uint64_t t = sp.t_begin; t < sp.t_end; ++t)
+        tile_pieces<E_WG, E_TILE>(a, gbin, tile_lc, t * E_TILE, s_g, &s_carry, s_wave,
+                                  [&](uint32_t g, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t) {
+                                      atomicAdd(&s_hist[g >> FINE_BITS], 1u);
+                                  });
     __syncthreads();
-    for (int i = threadIdx.x; i < N_COARSE; i += P_WG) hist[uint64_t(i) * n_big + big] = s_hist[i];
+    for (int i = threadIdx.x; i < N_COARSE; i += E_WG) hist[uint64_t(i) * gridDim.x + blockIdx.x] = s_hist[i];
 }
 
 // Per coarse bin c (one workgroup each): exclusive scan of hist[c][*] over
@@ -313,43 +327,40 @@ __global__ __launch_bounds__(N_COARSE) void k_coarse_scan(const uint64_t* __rest
     if (tid == 0) coff[0] = 0;
 }
 
-// Pass 3: the same pieces again, each big tile writing its records of coarse
-// bin c as one contiguous run at coff[c] + hist[c][t] (level-1 arrays).
-__global__ __launch_bounds__(P_WG) void k_pieces_emit(PartitionArgs a, const uint32_t* __restrict__ gbin,
-                                                      const uint64_t* __restrict__ tile_lc, uint64_t n_tiles,
+// Pass 3: the same pieces again; span s appends its records of coarse bin c
+// to the run starting at coff[c] + hist[c][s] (level-1 arrays).  A span is a
+// contiguous 1/n_span of the input, so each run is written front to back by
+// one workgroup and its HBM lines are completed in L2 before eviction.
+__global__ __launch_bounds__(E_WG) void k_pieces_emit(PartitionArgs a, const uint32_t* __restrict__ gbin,
+                                                      const uint64_t* __restrict__ tile_lc,
                                                       const uint32_t* __restrict__ hist,
                                                       const uint64_t* __restrict__ coff, uint32_t W,
                                                       uint32_t* __restrict__ att, uint64_t* __restrict__ labels,
                                                       uint8_t* __restrict__ mbin) {
-    __shared__ uint32_t s_g[P_TILE + WIN_MAX + 2];
+    __shared__ uint32_t s_g[E_TILE + WIN_MAX + 2];
     __shared__ unsigned long long s_carry;
-    __shared__ uint32_t s_wave[P_WG / 64];
-    __shared__ uint64_t s_base[N_COARSE];
-    __shared__ uint32_t s_cur[N_COARSE];
-    const uint64_t big = blockIdx.x, n_big = gridDim.x;
-    for (int i = threadIdx.x; i < N_COARSE; i += P_WG) {
-        s_base[i] = coff[i] + hist[uint64_t(i) * n_big + big];
-        s_cur[i] = 0;
-    }
-    const uint64_t t_end = min(n_tiles, (big + 1) * P_SUB);
-    for (uint64_t t = big * P_SUB; t < t_end; ++t)
-        sub_tile_pieces(a, gbin, tile_lc, t, s_g, &s_carry, s_wave,
-                        [&](uint32_t g, uint32_t mb, uint32_t len, uint32_t l_disc, uint32_t r_disc,
-                            uint64_t start) {
-                            const uint32_t c = g >> FINE_BITS;
-                            const uint64_t slot = s_base[c] + atomicAdd(&s_cur[c], 1u);
-                            att[slot] = len | (l_disc << 8) | (r_disc << 9) | (g << 16);
-                            mbin[slot] = uint8_t(mb);
-                            for (uint32_t wj = 0; wj < W; ++wj) {
-                                uint64_t x = 0;
-                                if (32 * wj < len) {
-                                    x = stream_get64(a.packed, start + 32ull * wj);
-                                    const uint32_t nb = len - 32 * wj;
-                                    if (nb < 32) x &= ~0ull << (64 - 2 * nb);
-                                }
-                                labels[slot * W + (W - 1 - wj)] = x;
-                            }
-                        });
+    __shared__ uint32_t s_wave[E_WG / 64];
+    __shared__ unsigned long long s_cur[N_COARSE];
+    for (int i = threadIdx.x; i < N_COARSE; i += E_WG)
+        s_cur[i] = coff[i] + hist[uint64_t(i) * gridDim.x + blockIdx.x];
+    const Span sp = span_of(a.n_bases);
+    for (uint64_t t = sp.t_begin; t < sp.t_end; ++t)
+        tile_pieces<E_WG, E_TILE>(a, gbin, tile_lc, t * E_TILE, s_g, &s_carry, s_wave,
+                                  [&](uint32_t g, uint32_t mb, uint32_t len, uint32_t l_disc, uint32_t r_disc,
+                                      uint64_t start) {
+                                      const uint64_t slot = atomicAdd(&s_cur[g >> FINE_BITS], 1ull);
+                                      att[slot] = len | (l_disc << 8) | (r_disc << 9) | (g << 16);
+                                      mbin[slot] = uint8_t(mb);
+                                      for (uint32_t wj = 0; wj < W; ++wj) {
+                                          uint64_t x = 0;
+                                          if (32 * wj < len) {
+                                              x = stream_get64(a.packed, start + 32ull * wj);
+                                              const uint32_t nb = len - 32 * wj;
+                                              if (nb < 32) x &= ~0ull << (64 - 2 * nb);
+                                          }
+                                          labels[slot * W + (W - 1 - wj)] = x;
+                                      }
+                                  });
 }
 
 // Level 2: one workgroup per coarse bin moves its records into the final
@@ -435,29 +446,25 @@ void launch_minimizer_g(const PartitionArgs& a, uint32_t* gbin, uint64_t* tile_l
     hipLaunchKernelGGL(k_minimizer_g, dim3(uint32_t(tiles)), dim3(P_WG), 0, s, a, gbin, tile_lc);
 }
 
-uint64_t pieces_big_tiles(uint64_t n_bases) {
-    const uint64_t tiles = (n_bases + P_TILE - 1) / P_TILE;
-    return (tiles + P_SUB - 1) / P_SUB;
+uint32_t pieces_spans(uint64_t n_bases, uint32_t n_cu) {
+    const uint64_t n_t = (n_bases + E_TILE - 1) / E_TILE;
+    return uint32_t(std::min<uint64_t>(n_t, n_cu));
 }
 
-void launch_pieces_hist(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc,
+void launch_pieces_hist(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, uint32_t n_span,
                         uint32_t* hist, uint64_t* ctot, uint64_t* coff, hipStream_t s) {
-    const uint64_t tiles = (a.n_bases + P_TILE - 1) / P_TILE;
-    const uint64_t n_big = pieces_big_tiles(a.n_bases);
-    if (!n_big) return;
-    hipLaunchKernelGGL(k_pieces_hist, dim3(uint32_t(n_big)), dim3(P_WG), 0, s, a, gbin, tile_lc, tiles, hist);
-    hipLaunchKernelGGL(k_hist_scan, dim3(N_COARSE), dim3(1024), 0, s, hist, n_big, ctot);
+    if (!n_span) return;
+    hipLaunchKernelGGL(k_pieces_hist, dim3(n_span), dim3(E_WG), 0, s, a, gbin, tile_lc, hist);
+    hipLaunchKernelGGL(k_hist_scan, dim3(N_COARSE), dim3(1024), 0, s, hist, uint64_t(n_span), ctot);
     hipLaunchKernelGGL(k_coarse_scan, dim3(1), dim3(N_COARSE), 0, s, ctot, coff);
 }
 
-void launch_pieces_emit(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc,
+void launch_pieces_emit(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, uint32_t n_span,
                         const uint32_t* hist, const uint64_t* coff, uint32_t W, uint32_t* att1,
                         uint64_t* lab1, uint8_t* mbin1, hipStream_t s) {
-    const uint64_t tiles = (a.n_bases + P_TILE - 1) / P_TILE;
-    const uint64_t n_big = pieces_big_tiles(a.n_bases);
-    if (!n_big) return;
-    hipLaunchKernelGGL(k_pieces_emit, dim3(uint32_t(n_big)), dim3(P_WG), 0, s, a, gbin, tile_lc, tiles, hist,
-                       coff, W, att1, lab1, mbin1);
+    if (!n_span) return;
+    hipLaunchKernelGGL(k_pieces_emit, dim3(n_span), dim3(E_WG), 0, s, a, gbin, tile_lc, hist, coff, W, att1,
+                       lab1, mbin1);
 }
 
 void launch_bucket_fine(const uint32_t* att1, const uint64_t* lab1, const uint8_t* mbin1, const uint64_t* coff,
