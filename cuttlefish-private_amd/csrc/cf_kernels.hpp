@@ -38,20 +38,23 @@ constexpr int N_COARSE = 256;
 constexpr int E_WG = 1024;
 constexpr int E_TILE = 8192;
 static_assert((N_COARSE << FINE_BITS) == 16384, "coarse x fine bins cover the 16384 subgraphs");
-uint32_t pieces_spans(uint64_t n_bases, uint32_t n_cu);
-// Pass 2: record counts per (coarse bin, span) in hist[N_COARSE][n_span],
+constexpr int H_SPLIT = 4;  // count spans per emit span
+struct PieceSpans {
+    uint64_t per_count, per_emit;  // E_TILE tiles per count / emit span
+    uint32_t n_count, n_emit;      // spans (workgroups)
+};
+PieceSpans pieces_spans(uint64_t n_bases, uint32_t n_cu);
+// Pass 2: record counts per (coarse bin, count span) in hist[N_COARSE][n_count],
 // scanned in place; coff[N_COARSE+1] = coarse bin offsets (coff[N_COARSE] =
 // records); ctot[N_COARSE] scratch.
-void launch_pieces_hist(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, uint32_t n_span,
+void launch_pieces_hist(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, const PieceSpans& ps,
                         uint32_t* hist, uint64_t* ctot, uint64_t* coff, hipStream_t s);
-// Pass 3: records into the level-1 arrays, grouped by coarse bin.
-void launch_pieces_emit(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, uint32_t n_span,
-                        const uint32_t* hist, const uint64_t* coff, uint32_t W, uint32_t* att1,
-                        uint64_t* lab1, uint8_t* mbin1, hipStream_t s);
+// Pass 3: records into the level-1 array (W + 1 words each), grouped by coarse bin.
+void launch_pieces_emit(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, const PieceSpans& ps,
+                        const uint32_t* hist, const uint64_t* coff, uint32_t W, uint64_t* rec1, hipStream_t s);
 // Level 2: level-1 records into per-bucket order; boff[G+1] bucket offsets.
-void launch_bucket_fine(const uint32_t* att1, const uint64_t* lab1, const uint8_t* mbin1, const uint64_t* coff,
-                        uint32_t W, uint64_t* boff, uint32_t* att, uint64_t* labels, uint8_t* mbin,
-                        hipStream_t s);
+void launch_bucket_fine(const uint64_t* rec1, const uint64_t* coff, uint32_t W, uint64_t* boff, uint32_t* att,
+                        uint64_t* labels, uint8_t* mbin, hipStream_t s);
 
 // Per-bucket statistics: [n_rec, n_kmer_instances, n_bases, 0] as u64 and the
 // MLP feature vector (N_FEAT floats).

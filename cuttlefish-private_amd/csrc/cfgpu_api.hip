@@ -90,7 +90,7 @@ struct cfgpu_handle {
 
     // partition state
     DevBuf packed, frag, gbin, tile_lc, boff, att, labels, mbin, stats, feat;
-    DevBuf hist, ctot, coff, att1, lab1, mbin1;  // level-1 bucketing
+    DevBuf hist, ctot, coff, rec1;  // level-1 bucketing
     uint64_t n_bases = 0, n_frag = 0, n_records = 0;
     bool partitioned = false;
     std::vector<uint64_t> h_boff, h_stats;
@@ -711,10 +711,10 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
         d_packed = h->packed.as<uint64_t>();
         d_frag = h->frag.as<uint64_t>();
     }
-    const uint32_t n_span = pieces_spans(n_bases, uint32_t(h->cus));
+    const PieceSpans ps = pieces_spans(n_bases, uint32_t(h->cus));
     HIPCK(h, h->gbin.ensure(n_bases * 4));
     HIPCK(h, h->tile_lc.ensure(tiles * 8));
-    HIPCK(h, h->hist.ensure(size_t(N_COARSE) * n_span * 4));
+    HIPCK(h, h->hist.ensure(size_t(N_COARSE) * ps.n_count * 4));
     HIPCK(h, h->ctot.ensure(size_t(N_COARSE) * 8));
     HIPCK(h, h->coff.ensure(size_t(N_COARSE + 1) * 8));
     HIPCK(h, h->boff.ensure((G + 1) * 8));
@@ -740,26 +740,22 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     if (n_frag) launch_minimizer_g(a, h->gbin.as<uint32_t>(), h->tile_lc.as<uint64_t>(), h->stream);
     HIPCK(h, hipEventRecord(e1.e, h->stream));
     if (n_frag)
-        launch_pieces_hist(a, h->gbin.as<uint32_t>(), h->tile_lc.as<uint64_t>(), n_span, h->hist.as<uint32_t>(),
+        launch_pieces_hist(a, h->gbin.as<uint32_t>(), h->tile_lc.as<uint64_t>(), ps, h->hist.as<uint32_t>(),
                            h->ctot.as<uint64_t>(), h->coff.as<uint64_t>(), h->stream);
     HIPCK(h, hipEventRecord(e2.e, h->stream));
     uint64_t n_rec = 0;
     HIPCK(h, hipMemcpyAsync(&n_rec, h->coff.as<uint64_t>() + N_COARSE, 8, hipMemcpyDeviceToHost, h->stream));
     HIPCK(h, hipStreamSynchronize(h->stream));
-    HIPCK(h, h->att1.ensure(n_rec * 4));
-    HIPCK(h, h->lab1.ensure(n_rec * h->W * 8));
-    HIPCK(h, h->mbin1.ensure(n_rec));
+    HIPCK(h, h->rec1.ensure(n_rec * (h->W + 1) * 8));
     HIPCK(h, h->att.ensure(n_rec * 4));
     HIPCK(h, h->labels.ensure(n_rec * h->W * 8));
     HIPCK(h, h->mbin.ensure(n_rec));
     HIPCK(h, hipEventRecord(e3.e, h->stream));
     if (n_frag)
-        launch_pieces_emit(a, h->gbin.as<uint32_t>(), h->tile_lc.as<uint64_t>(), n_span, h->hist.as<uint32_t>(),
-                           h->coff.as<uint64_t>(), h->W, h->att1.as<uint32_t>(), h->lab1.as<uint64_t>(),
-                           h->mbin1.as<uint8_t>(), h->stream);
-    launch_bucket_fine(h->att1.as<uint32_t>(), h->lab1.as<uint64_t>(), h->mbin1.as<uint8_t>(),
-                       h->coff.as<uint64_t>(), h->W, h->boff.as<uint64_t>(), h->att.as<uint32_t>(),
-                       h->labels.as<uint64_t>(), h->mbin.as<uint8_t>(), h->stream);
+        launch_pieces_emit(a, h->gbin.as<uint32_t>(), h->tile_lc.as<uint64_t>(), ps, h->hist.as<uint32_t>(),
+                           h->coff.as<uint64_t>(), h->W, h->rec1.as<uint64_t>(), h->stream);
+    launch_bucket_fine(h->rec1.as<uint64_t>(), h->coff.as<uint64_t>(), h->W, h->boff.as<uint64_t>(),
+                       h->att.as<uint32_t>(), h->labels.as<uint64_t>(), h->mbin.as<uint8_t>(), h->stream);
     HIPCK(h, hipEventRecord(e4.e, h->stream));
     launch_bucket_features(h->boff.as<uint64_t>(), h->att.as<uint32_t>(), h->mbin.as<uint8_t>(), G, k,
                            h->stats.as<uint64_t>(), h->feat.as<float>(), h->stream);

@@ -158,10 +158,15 @@ __device__ __forceinline__ uint32_t block_incl_max(uint32_t v, uint32_t* s_wave)
     return v > pre ? v : pre;
 }
 
+// Packed words staged per tile: records start at t0 - 1 or later and end
+// before t0 + TILE + 256 (len <= 255), and stream_get64 reads one word ahead.
+#define TILE_WORDS(TILE) ((TILE) / 32 + 16)
+
 // Weak super k-mers with their first (k-1)-mer in [t0, t0 + TILE) (t0 a
 // multiple of P_TILE): calls emit(g, mb, len, l_disc, r_disc, start) once per
 // piece, where mb is the 4 hash bits above g and start the first base of the
-// record.  Every thread of the WG-thread workgroup must call it (it
+// record.  With s_words, the tile's packed words (from word t0/32 - 1) are
+// staged there first.  Every thread of the WG-thread workgroup must call it (it
 // synchronises).  The run start before t0 comes from tile_lc: a wave looks
 // back 64 minimizer tiles at a time; only runs longer than a tile need more
 // than one step.
@@ -169,7 +174,7 @@ template <int WG, int TILE, class Emit>
 __device__ __forceinline__ void tile_pieces(const PartitionArgs& a, const uint32_t* __restrict__ gbin,
                                             const uint64_t* __restrict__ tile_lc, uint64_t t0,
                                             uint32_t* s_g, unsigned long long* s_carry,
-                                            uint32_t* s_wave, Emit&& emit) {
+                                            uint32_t* s_wave, uint64_t* s_words, Emit&& emit) {
     constexpr int PPT = TILE / WG;  // positions per thread
     static_assert(PPT <= 32 && TILE % P_TILE == 0, "tile shape");
     const int tid = threadIdx.x;
@@ -177,10 +182,15 @@ __device__ __forceinline__ void tile_pieces(const PartitionArgs& a, const uint32
     const int64_t N = int64_t(a.n_bases);
     const unsigned k = a.k, w = a.w;
 
-    __syncthreads();  // the previous tile is done with s_g
+    __syncthreads();  // the previous tile is done with s_g (and s_words)
     for (int i = tid; i < TILE + int(w) + 2; i += WG) {
         const int64_t p = q0 + i;
         s_g[i] = (p >= 0 && p < N) ? gbin[p] : G_INVALID;
+    }
+    if (s_words) {  // packed words of every record starting in the tile
+        const uint64_t wlo = t0 ? t0 / 32 - 1 : 0;
+        for (int i = tid; i < TILE_WORDS(TILE); i += WG)
+            s_words[i] = wlo + i < a.n_words ? a.packed[wlo + i] : 0;
     }
     if (tid < 64) {
         int64_t j = int64_t(t0 / P_TILE) - 1 - tid;
@@ -250,31 +260,29 @@ __device__ __forceinline__ void tile_pieces(const PartitionArgs& a, const uint32
     }
 }
 
-// Span s of the input (E_TILE tiles [s*per, (s+1)*per), one workgroup each)
-// for the level-1 passes.
+// Span of workgroup blockIdx.x: E_TILE tiles [b * per, (b + 1) * per).
 struct Span {
-    uint64_t t_begin, t_end;  // E_TILE tile indices
+    uint64_t t_begin, t_end;
 };
-__device__ __forceinline__ Span span_of(uint64_t n_bases) {
+__device__ __forceinline__ Span span_of(uint64_t n_bases, uint64_t per) {
     const uint64_t n_t = (n_bases + E_TILE - 1) / E_TILE;
-    const uint64_t per = (n_t + gridDim.x - 1) / gridDim.x;
     const uint64_t b = min(n_t, uint64_t(blockIdx.x) * per);
     return {b, min(n_t, b + per)};
 }
 
-// Pass 2: records per coarse bin (g >> FINE_BITS) of each span into
-// hist[c * n_span + span].  No global atomics.
+// Pass 2: records per coarse bin (g >> FINE_BITS) of each count span (per
+// tiles) into hist[c * gridDim.x + span].  No global atomics.
 __global__ __launch_bounds__(E_WG) void k_pieces_hist(PartitionArgs a, const uint32_t* __restrict__ gbin,
-                                                      const uint64_t* __restrict__ tile_lc,
+                                                      const uint64_t* __restrict__ tile_lc, uint64_t per,
                                                       uint32_t* __restrict__ hist) {
     __shared__ uint32_t s_g[E_TILE + WIN_MAX + 2];
     __shared__ unsigned long long s_carry;
     __shared__ uint32_t s_wave[E_WG / 64];
     __shared__ uint32_t s_hist[N_COARSE];
     for (int i = threadIdx.x; i < N_COARSE; i += E_WG) s_hist[i] = 0;
-    const Span sp = span_of(a.n_bases);
+    const Span sp = span_of(a.n_bases, per);
     for (uint64_t t = sp.t_begin; t < sp.t_end; ++t)
-        tile_pieces<E_WG, E_TILE>(a, gbin, tile_lc, t * E_TILE, s_g, &s_carry, s_wave,
+        tile_pieces<E_WG, E_TILE>(a, gbin, tile_lc, t * E_TILE, s_g, &s_carry, s_wave, nullptr,
                                   [&](uint32_t g, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t) {
                                       atomicAdd(&s_hist[g >> FINE_BITS], 1u);
                                   });
@@ -327,48 +335,51 @@ __global__ __launch_bounds__(N_COARSE) void k_coarse_scan(const uint64_t* __rest
     if (tid == 0) coff[0] = 0;
 }
 
-// Pass 3: the same pieces again; span s appends its records of coarse bin c
-// to the run starting at coff[c] + hist[c][s] (level-1 arrays).  A span is a
-// contiguous 1/n_span of the input, so each run is written front to back by
-// one workgroup and its HBM lines are completed in L2 before eviction.
+// Pass 3: the same pieces again; emit span s (H_SPLIT count spans) appends
+// its records of coarse bin c to the run starting at coff[c] +
+// hist[c][s * H_SPLIT].  A span is a contiguous part of the input, so each
+// run is written front to back by one workgroup.  Level-1 records are one
+// array of structures, W + 1 words each: att | mbin << 32, then the label.
 __global__ __launch_bounds__(E_WG) void k_pieces_emit(PartitionArgs a, const uint32_t* __restrict__ gbin,
-                                                      const uint64_t* __restrict__ tile_lc,
-                                                      const uint32_t* __restrict__ hist,
+                                                      const uint64_t* __restrict__ tile_lc, uint64_t per,
+                                                      uint32_t n_hist, const uint32_t* __restrict__ hist,
                                                       const uint64_t* __restrict__ coff, uint32_t W,
-                                                      uint32_t* __restrict__ att, uint64_t* __restrict__ labels,
-                                                      uint8_t* __restrict__ mbin) {
+                                                      uint64_t* __restrict__ rec) {
     __shared__ uint32_t s_g[E_TILE + WIN_MAX + 2];
     __shared__ unsigned long long s_carry;
     __shared__ uint32_t s_wave[E_WG / 64];
+    __shared__ uint64_t s_words[TILE_WORDS(E_TILE)];
     __shared__ unsigned long long s_cur[N_COARSE];
     for (int i = threadIdx.x; i < N_COARSE; i += E_WG)
-        s_cur[i] = coff[i] + hist[uint64_t(i) * gridDim.x + blockIdx.x];
-    const Span sp = span_of(a.n_bases);
-    for (uint64_t t = sp.t_begin; t < sp.t_end; ++t)
-        tile_pieces<E_WG, E_TILE>(a, gbin, tile_lc, t * E_TILE, s_g, &s_carry, s_wave,
+        s_cur[i] = coff[i] + hist[uint64_t(i) * n_hist + uint64_t(blockIdx.x) * H_SPLIT];
+    const Span sp = span_of(a.n_bases, per);
+    for (uint64_t t = sp.t_begin; t < sp.t_end; ++t) {
+        const uint64_t t0 = t * E_TILE;
+        const uint64_t base0 = t0 ? t0 - 32 : 0;  // first base of s_words[0]
+        tile_pieces<E_WG, E_TILE>(a, gbin, tile_lc, t0, s_g, &s_carry, s_wave, s_words,
                                   [&](uint32_t g, uint32_t mb, uint32_t len, uint32_t l_disc, uint32_t r_disc,
                                       uint64_t start) {
                                       const uint64_t slot = atomicAdd(&s_cur[g >> FINE_BITS], 1ull);
-                                      att[slot] = len | (l_disc << 8) | (r_disc << 9) | (g << 16);
-                                      mbin[slot] = uint8_t(mb);
+                                      uint64_t* out = rec + slot * (W + 1);
+                                      out[0] = uint64_t(len | (l_disc << 8) | (r_disc << 9) | (g << 16)) |
+                                               (uint64_t(mb) << 32);
                                       for (uint32_t wj = 0; wj < W; ++wj) {
                                           uint64_t x = 0;
                                           if (32 * wj < len) {
-                                              x = stream_get64(a.packed, start + 32ull * wj);
+                                              x = stream_get64(s_words, start - base0 + 32ull * wj);
                                               const uint32_t nb = len - 32 * wj;
                                               if (nb < 32) x &= ~0ull << (64 - 2 * nb);
                                           }
-                                          labels[slot * W + (W - 1 - wj)] = x;
+                                          out[1 + (W - 1 - wj)] = x;
                                       }
                                   });
+    }
 }
 
 // Level 2: one workgroup per coarse bin moves its records into the final
 // per-g order (the bin's 2^FINE_BITS buckets) and writes their offsets.
 // Records of one bucket are written in increasing slots by one workgroup.
-__global__ __launch_bounds__(1024) void k_bucket_fine(const uint32_t* __restrict__ att1,
-                                                      const uint64_t* __restrict__ lab1,
-                                                      const uint8_t* __restrict__ mbin1,
+__global__ __launch_bounds__(1024) void k_bucket_fine(const uint64_t* __restrict__ rec,
                                                       const uint64_t* __restrict__ coff, uint32_t W,
                                                       uint64_t* __restrict__ boff, uint32_t* __restrict__ att,
                                                       uint64_t* __restrict__ labels, uint8_t* __restrict__ mbin) {
@@ -378,9 +389,10 @@ __global__ __launch_bounds__(1024) void k_bucket_fine(const uint32_t* __restrict
     const uint32_t c = blockIdx.x;
     const int tid = threadIdx.x;
     const uint64_t r0 = coff[c], r1 = coff[c + 1];
+    const uint64_t RW = W + 1;
     for (uint32_t i = tid; i < NF; i += 1024) s_cnt[i] = 0;
     __syncthreads();
-    for (uint64_t r = r0 + tid; r < r1; r += 1024) atomicAdd(&s_cnt[(att1[r] >> 16) & (NF - 1)], 1u);
+    for (uint64_t r = r0 + tid; r < r1; r += 1024) atomicAdd(&s_cnt[(uint32_t(rec[r * RW]) >> 16) & (NF - 1)], 1u);
     __syncthreads();
     if (tid == 0) {
         uint64_t run = r0;
@@ -393,11 +405,12 @@ __global__ __launch_bounds__(1024) void k_bucket_fine(const uint32_t* __restrict
     }
     __syncthreads();
     for (uint64_t r = r0 + tid; r < r1; r += 1024) {
-        const uint32_t at = att1[r];
+        const uint64_t h = rec[r * RW];
+        const uint32_t at = uint32_t(h);
         const uint64_t slot = atomicAdd(&s_cur[(at >> 16) & (NF - 1)], 1ull);
         att[slot] = at;
-        mbin[slot] = mbin1[r];
-        for (uint32_t wj = 0; wj < W; ++wj) labels[slot * W + wj] = lab1[r * W + wj];
+        mbin[slot] = uint8_t(h >> 32);
+        for (uint32_t wj = 0; wj < W; ++wj) labels[slot * W + wj] = rec[r * RW + 1 + wj];
     }
 }
 
@@ -446,32 +459,36 @@ void launch_minimizer_g(const PartitionArgs& a, uint32_t* gbin, uint64_t* tile_l
     hipLaunchKernelGGL(k_minimizer_g, dim3(uint32_t(tiles)), dim3(P_WG), 0, s, a, gbin, tile_lc);
 }
 
-uint32_t pieces_spans(uint64_t n_bases, uint32_t n_cu) {
+PieceSpans pieces_spans(uint64_t n_bases, uint32_t n_cu) {
+    PieceSpans ps{0, 0, 0, 0};
     const uint64_t n_t = (n_bases + E_TILE - 1) / E_TILE;
-    return uint32_t(std::min<uint64_t>(n_t, n_cu));
+    if (!n_t) return ps;
+    const uint64_t n_h = std::min<uint64_t>(n_t, uint64_t(n_cu) * H_SPLIT);
+    ps.per_count = (n_t + n_h - 1) / n_h;
+    ps.n_count = uint32_t((n_t + ps.per_count - 1) / ps.per_count);
+    ps.per_emit = ps.per_count * H_SPLIT;
+    ps.n_emit = uint32_t((n_t + ps.per_emit - 1) / ps.per_emit);
+    return ps;
 }
 
-void launch_pieces_hist(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, uint32_t n_span,
+void launch_pieces_hist(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, const PieceSpans& ps,
                         uint32_t* hist, uint64_t* ctot, uint64_t* coff, hipStream_t s) {
-    if (!n_span) return;
-    hipLaunchKernelGGL(k_pieces_hist, dim3(n_span), dim3(E_WG), 0, s, a, gbin, tile_lc, hist);
-    hipLaunchKernelGGL(k_hist_scan, dim3(N_COARSE), dim3(1024), 0, s, hist, uint64_t(n_span), ctot);
+    if (!ps.n_count) return;
+    hipLaunchKernelGGL(k_pieces_hist, dim3(ps.n_count), dim3(E_WG), 0, s, a, gbin, tile_lc, ps.per_count, hist);
+    hipLaunchKernelGGL(k_hist_scan, dim3(N_COARSE), dim3(1024), 0, s, hist, uint64_t(ps.n_count), ctot);
     hipLaunchKernelGGL(k_coarse_scan, dim3(1), dim3(N_COARSE), 0, s, ctot, coff);
 }
 
-void launch_pieces_emit(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, uint32_t n_span,
-                        const uint32_t* hist, const uint64_t* coff, uint32_t W, uint32_t* att1,
-                        uint64_t* lab1, uint8_t* mbin1, hipStream_t s) {
-    if (!n_span) return;
-    hipLaunchKernelGGL(k_pieces_emit, dim3(n_span), dim3(E_WG), 0, s, a, gbin, tile_lc, hist, coff, W, att1,
-                       lab1, mbin1);
+void launch_pieces_emit(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, const PieceSpans& ps,
+                        const uint32_t* hist, const uint64_t* coff, uint32_t W, uint64_t* rec1, hipStream_t s) {
+    if (!ps.n_emit) return;
+    hipLaunchKernelGGL(k_pieces_emit, dim3(ps.n_emit), dim3(E_WG), 0, s, a, gbin, tile_lc, ps.per_emit,
+                       ps.n_count, hist, coff, W, rec1);
 }
 
-void launch_bucket_fine(const uint32_t* att1, const uint64_t* lab1, const uint8_t* mbin1, const uint64_t* coff,
-                        uint32_t W, uint64_t* boff, uint32_t* att, uint64_t* labels, uint8_t* mbin,
-                        hipStream_t s) {
-    hipLaunchKernelGGL(k_bucket_fine, dim3(N_COARSE), dim3(1024), 0, s, att1, lab1, mbin1, coff, W, boff, att,
-                       labels, mbin);
+void launch_bucket_fine(const uint64_t* rec1, const uint64_t* coff, uint32_t W, uint64_t* boff, uint32_t* att,
+                        uint64_t* labels, uint8_t* mbin, hipStream_t s) {
+    hipLaunchKernelGGL(k_bucket_fine, dim3(N_COARSE), dim3(1024), 0, s, rec1, coff, W, boff, att, labels, mbin);
 }
 
 void launch_bucket_features(const uint64_t* off, const uint32_t* att, const uint8_t* mbin,
