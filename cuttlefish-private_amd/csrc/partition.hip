@@ -35,6 +35,7 @@ __global__ __launch_bounds__(P_WG) void k_minimizer_g(PartitionArgs a, uint32_t*
     __shared__ uint64_t s_m[S_MAX];
     __shared__ uint64_t s_frag[MAXF + 1];
     __shared__ uint32_t s_g[P_TILE + 1];
+    __shared__ uint8_t s_inv[P_TILE + 1];
     __shared__ unsigned long long s_lc;
 
     const int tid = threadIdx.x;
@@ -44,18 +45,29 @@ __global__ __launch_bounds__(P_WG) void k_minimizer_g(PartitionArgs a, uint32_t*
     const unsigned k = a.k, l = a.l, w = a.w;
     const int S = P_TILE + int(w);  // l-mers q0 .. q0 + P_TILE + w - 1
 
-    // Fragment boundaries overlapping the tile.
-    if (tid == 0) {
+    // Fragment containing max(q0, 0): a 64-ary search by wave 0 (a few
+    // dependent loads instead of log2(n_frag)).
+    if (tid < 64) {
         const uint64_t key = uint64_t(q0 < 0 ? 0 : q0);
-        uint64_t lo = 0, hi = a.n_frag;  // largest f with frag_off[f] <= key
-        while (lo < hi) {
-            const uint64_t mid = (lo + hi + 1) >> 1;
-            if (a.frag_off[mid] <= key) lo = mid;
-            else hi = mid - 1;
+        uint64_t lo = 0, hi = a.n_frag;  // largest f in [lo, hi] with frag_off[f] <= key
+        while (hi - lo >= 64) {
+            const uint64_t step = (hi - lo + 63) / 64;
+            const uint64_t idx = lo + uint64_t(tid) * step;
+            const bool le = idx <= hi && a.frag_off[idx] <= key;
+            const unsigned long long m = __ballot(le);  // lane 0 always set
+            const uint64_t last = uint64_t(63 - __clzll((long long)m));
+            lo = lo + last * step;
+            hi = min(hi, lo + step - 1);
         }
-        s_m[0] = lo;
-        s_lc = 0;
+        const uint64_t idx = lo + uint64_t(tid);
+        const bool le = idx <= hi && a.frag_off[idx] <= key;
+        const unsigned long long m = __ballot(le);
+        if (tid == 0) {
+            s_m[0] = lo + uint64_t(63 - __clzll((long long)m));
+            s_lc = 0;
+        }
     }
+    for (int i = tid; i <= P_TILE; i += P_WG) s_inv[i] = 0;
     __syncthreads();
     const uint64_t f0 = s_m[0];
     __syncthreads();
@@ -67,6 +79,14 @@ __global__ __launch_bounds__(P_WG) void k_minimizer_g(PartitionArgs a, uint32_t*
     for (int i = tid; i < NWORDS; i += P_WG) {
         const uint64_t wi = uint64_t(wlo) + uint64_t(i);
         s_words[i] = wi < a.n_words ? a.packed[wi] : 0;
+    }
+    __syncthreads();
+    // Position p starts a (k-1)-mer iff no fragment boundary b has
+    // p < b <= p + k - 2: each boundary marks [b - k + 2, b - 1].
+    for (int j = tid; j <= MAXF; j += P_WG) {
+        const int64_t b = int64_t(s_frag[j]);
+        const int64_t lo = max(int64_t(0), b - int64_t(k) + 2 - q0), hi = min(int64_t(P_TILE), b - 1 - q0);
+        for (int64_t i = lo; i <= hi; ++i) s_inv[i] = 1;
     }
     __syncthreads();
 
@@ -113,13 +133,7 @@ __global__ __launch_bounds__(P_WG) void k_minimizer_g(PartitionArgs a, uint32_t*
         const int64_t p = q0 + i;
         uint32_t gv = G_INVALID;
         if (p >= 0 && p < N) {
-            int lo = 0, hi = MAXF - 1;  // largest j with s_frag[j] <= p
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (int64_t(s_frag[mid]) <= p) lo = mid;
-                else hi = mid - 1;
-            }
-            if (p + int64_t(k) - 1 <= int64_t(s_frag[lo + 1])) {
+            if (!s_inv[i]) {
                 const uint64_t m0 = s_m[i], m1 = s_m[i + w - span];
                 const uint64_t H = m0 < m1 ? m0 : m1;
                 gv = uint32_t(H & a.g_mask) | (uint32_t((H >> a.g_bits) & 15u) << 16);
@@ -158,6 +172,31 @@ __device__ __forceinline__ uint32_t block_incl_max(uint32_t v, uint32_t* s_wave)
     return v > pre ? v : pre;
 }
 
+// Exclusive sum-scan of one u32 per thread across a WG-thread workgroup;
+// writes the total to *total.
+template <int WG>
+__device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* s_wave, uint32_t* total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(x, d);
+        if (lane >= d) x += u;
+    }
+    __syncthreads();  // s_wave may still be read by a previous scan
+    if (lane == 63) s_wave[wid] = x;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int j = 0; j < WG / 64; ++j) {
+        const uint32_t t = s_wave[j];
+        pre += j < wid ? t : 0;
+        tot += t;
+    }
+    *total = tot;
+    return pre + x - v;
+}
+
 // Packed words staged per tile: records start at t0 - 1 or later and end
 // before t0 + TILE + 256 (len <= 255), and stream_get64 reads one word ahead.
 #define TILE_WORDS(TILE) ((TILE) / 32 + 16)
@@ -166,7 +205,8 @@ __device__ __forceinline__ uint32_t block_incl_max(uint32_t v, uint32_t* s_wave)
 // multiple of P_TILE): calls emit(g, mb, len, l_disc, r_disc, start) once per
 // piece, where mb is the 4 hash bits above g and start the first base of the
 // record.  With s_words, the tile's packed words (from word t0/32 - 1) are
-// staged there first.  Every thread of the WG-thread workgroup must call it (it
+// staged there first.  Pieces are first collected in s_desc (TILE entries),
+// so emit runs on every lane instead of inside the per-position loop.  Every thread of the WG-thread workgroup must call it (it
 // synchronises).  The run start before t0 comes from tile_lc: a wave looks
 // back 64 minimizer tiles at a time; only runs longer than a tile need more
 // than one step.
@@ -174,7 +214,8 @@ template <int WG, int TILE, class Emit>
 __device__ __forceinline__ void tile_pieces(const PartitionArgs& a, const uint32_t* __restrict__ gbin,
                                             const uint64_t* __restrict__ tile_lc, uint64_t t0,
                                             uint32_t* s_g, unsigned long long* s_carry,
-                                            uint32_t* s_wave, uint64_t* s_words, Emit&& emit) {
+                                            uint32_t* s_wave, uint64_t* s_words, uint64_t* s_desc,
+                                            Emit&& emit) {
     constexpr int PPT = TILE / WG;  // positions per thread
     static_assert(PPT <= 32 && TILE % P_TILE == 0, "tile shape");
     const int tid = threadIdx.x;
@@ -231,32 +272,72 @@ __device__ __forceinline__ void tile_pieces(const PartitionArgs& a, const uint32
         for (int j = 0; j < (tid >> 6); ++j) excl = s_wave[j] > excl ? s_wave[j] : excl;
     }
     const unsigned long long carry = *s_carry;
-    int64_t rs = excl ? q0 + int64_t(excl) : (carry ? int64_t(carry) - 1 : -1);
+    const int64_t rs0 = excl ? q0 + int64_t(excl) : (carry ? int64_t(carry) - 1 : -1);
 
-#pragma unroll 1
-    for (int j = 0; j < PPT; ++j) {
-        const int idx = base_idx + j;
-        const int64_t p = q0 + idx;
-        const uint32_t cur = s_g[idx];
-        if (cur == G_INVALID) continue;
-        const bool c = (chg >> j) & 1u;
-        if (c || rs < 0) rs = p;
-        if (!c && ((p - rs) % int64_t(w)) != 0) continue;
-        // Piece [p, e): (k-1)-mers with this g, at most w of them.
-        const uint32_t g = cur & 0xFFFFu;
-        int e = idx + 1;
-        const int lim = idx + int(w);
-        while (e < lim) {
-            const uint32_t v = s_g[e];
-            if (v == G_INVALID || (v & 0xFFFFu) != g) break;
-            ++e;
+    // A piece starts at every change and every w (k-1)-mers after one.  The
+    // phase (p - run start) mod w is carried from position to position; only
+    // the thread's first position in a run needs a division.
+    auto phase_start = [&](int64_t p) -> uint32_t { return rs0 < 0 ? 0u : uint32_t(uint64_t(p - rs0) % w); };
+    uint32_t n_mine = 0;
+    {
+        bool have = false;
+        uint32_t ph = 0;
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            const int idx = base_idx + j;
+            if (s_g[idx] == G_INVALID) {
+                have = false;
+                continue;
+            }
+            const bool c = (chg >> j) & 1u;
+            ph = c ? 0u : (have ? (ph + 1 == w ? 0u : ph + 1) : phase_start(q0 + idx));
+            have = true;
+            n_mine += ph == 0;
         }
-        const uint32_t nv = s_g[e], pv = s_g[idx - 1];
-        const uint32_t rj = nv != G_INVALID, lj = pv != G_INVALID;
-        const uint32_t l_disc = lj && ((pv & 0xFFFFu) != g);
-        const uint32_t r_disc = rj && ((nv & 0xFFFFu) != g);
-        const uint32_t len = lj + uint32_t(e - idx) + (k - 2) + rj;
-        emit(g, (cur >> 16) & 15u, len, l_disc, r_disc, uint64_t(p) - lj);
+    }
+    uint32_t n_tile;
+    const uint32_t my_off = block_excl_sum<WG>(n_mine, s_wave, &n_tile);
+    {
+        bool have = false;
+        uint32_t ph = 0, o = my_off;
+#pragma unroll 1
+        for (int j = 0; j < PPT; ++j) {
+            const int idx = base_idx + j;
+            const uint32_t cur = s_g[idx];
+            if (cur == G_INVALID) {
+                have = false;
+                continue;
+            }
+            const bool c = (chg >> j) & 1u;
+            ph = c ? 0u : (have ? (ph + 1 == w ? 0u : ph + 1) : phase_start(q0 + idx));
+            have = true;
+            if (ph) continue;
+            // Piece [idx, e): (k-1)-mers with this g, at most w of them.
+            const uint32_t g = cur & 0xFFFFu;
+            int e = idx + 1;
+            const int lim = idx + int(w);
+            while (e < lim) {
+                const uint32_t v = s_g[e];
+                if (v == G_INVALID || (v & 0xFFFFu) != g) break;
+                ++e;
+            }
+            const uint32_t nv = s_g[e], pv = s_g[idx - 1];
+            const uint32_t rj = nv != G_INVALID, lj = pv != G_INVALID;
+            const uint32_t l_disc = lj && ((pv & 0xFFFFu) != g);
+            const uint32_t r_disc = rj && ((nv & 0xFFFFu) != g);
+            const uint32_t len = lj + uint32_t(e - idx) + (k - 2) + rj;
+            s_desc[o++] = uint64_t(idx) | (uint64_t(g) << 16) | (uint64_t((cur >> 16) & 15u) << 32) |
+                          (uint64_t(len) << 36) | (uint64_t(l_disc) << 44) | (uint64_t(r_disc) << 45) |
+                          (uint64_t(lj) << 46);
+        }
+    }
+    __syncthreads();
+    // Every piece of the tile, one lane each.
+    for (uint32_t i = tid; i < n_tile; i += WG) {
+        const uint64_t d = s_desc[i];
+        const uint32_t idx = uint32_t(d & 0xFFFFu), lj = uint32_t(d >> 46) & 1u;
+        emit(uint32_t(d >> 16) & 0xFFFFu, uint32_t(d >> 32) & 15u, uint32_t(d >> 36) & 0xFFu,
+             uint32_t(d >> 44) & 1u, uint32_t(d >> 45) & 1u, uint64_t(q0 + int64_t(idx)) - lj);
     }
 }
 
@@ -279,10 +360,11 @@ __global__ __launch_bounds__(E_WG) void k_pieces_hist(PartitionArgs a, const uin
     __shared__ unsigned long long s_carry;
     __shared__ uint32_t s_wave[E_WG / 64];
     __shared__ uint32_t s_hist[N_COARSE];
+    __shared__ uint64_t s_desc[E_TILE];
     for (int i = threadIdx.x; i < N_COARSE; i += E_WG) s_hist[i] = 0;
     const Span sp = span_of(a.n_bases, per);
     for (uint64_t t = sp.t_begin; t < sp.t_end; ++t)
-        tile_pieces<E_WG, E_TILE>(a, gbin, tile_lc, t * E_TILE, s_g, &s_carry, s_wave, nullptr,
+        tile_pieces<E_WG, E_TILE>(a, gbin, tile_lc, t * E_TILE, s_g, &s_carry, s_wave, nullptr, s_desc,
                                   [&](uint32_t g, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t) {
                                       atomicAdd(&s_hist[g >> FINE_BITS], 1u);
                                   });
@@ -350,13 +432,14 @@ __global__ __launch_bounds__(E_WG) void k_pieces_emit(PartitionArgs a, const uin
     __shared__ uint32_t s_wave[E_WG / 64];
     __shared__ uint64_t s_words[TILE_WORDS(E_TILE)];
     __shared__ unsigned long long s_cur[N_COARSE];
+    __shared__ uint64_t s_desc[E_TILE];
     for (int i = threadIdx.x; i < N_COARSE; i += E_WG)
         s_cur[i] = coff[i] + hist[uint64_t(i) * n_hist + uint64_t(blockIdx.x) * H_SPLIT];
     const Span sp = span_of(a.n_bases, per);
     for (uint64_t t = sp.t_begin; t < sp.t_end; ++t) {
         const uint64_t t0 = t * E_TILE;
         const uint64_t base0 = t0 ? t0 - 32 : 0;  // first base of s_words[0]
-        tile_pieces<E_WG, E_TILE>(a, gbin, tile_lc, t0, s_g, &s_carry, s_wave, s_words,
+        tile_pieces<E_WG, E_TILE>(a, gbin, tile_lc, t0, s_g, &s_carry, s_wave, s_words, s_desc,
                                   [&](uint32_t g, uint32_t mb, uint32_t len, uint32_t l_disc, uint32_t r_disc,
                                       uint64_t start) {
                                       const uint64_t slot = atomicAdd(&s_cur[g >> FINE_BITS], 1ull);
