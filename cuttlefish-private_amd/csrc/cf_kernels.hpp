@@ -30,17 +30,19 @@ struct PartitionArgs {
 void launch_minimizer_g(const PartitionArgs& a, uint32_t* gbin, uint64_t* tile_lc, hipStream_t s);
 
 // Records are bucketed in two levels.  Level 1 splits by coarse bin
-// g >> FINE_BITS: the input is cut into n_span contiguous spans (one
-// E_WG-thread workgroup each, walking E_TILE positions at a time); level 2
-// splits each coarse bin into its 2^FINE_BITS buckets.
+// g >> FINE_BITS: the input is cut into contiguous spans (count spans, and
+// emit spans of H_SPLIT count spans, one workgroup each); level 2 splits each
+// coarse bin into its 2^FINE_BITS buckets.
 constexpr int FINE_BITS = 6;
 constexpr int N_COARSE = 256;
-constexpr int E_WG = 1024;
-constexpr int E_TILE = 8192;
+constexpr int E_WG = 512;          // emit workgroup, E_TILE positions per step
+constexpr int E_TILE = 4096;
+constexpr int E_SPANS_PER_CU = 3;  // emit spans (workgroups) per CU
+constexpr int H_WG = 256;          // count workgroup, one minimizer tile per step
 static_assert((N_COARSE << FINE_BITS) == 16384, "coarse x fine bins cover the 16384 subgraphs");
 constexpr int H_SPLIT = 4;  // count spans per emit span
 struct PieceSpans {
-    uint64_t per_count, per_emit;  // E_TILE tiles per count / emit span
+    uint64_t per_count, per_emit;  // minimizer tiles (P_TILE positions) per count / emit span
     uint32_t n_count, n_emit;      // spans (workgroups)
 };
 PieceSpans pieces_spans(uint64_t n_bases, uint32_t n_cu);

@@ -201,38 +201,50 @@ __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* s_wave,
 // before t0 + TILE + 256 (len <= 255), and stream_get64 reads one word ahead.
 #define TILE_WORDS(TILE) ((TILE) / 32 + 16)
 
+// The g values of a tile's positions q0 .. q0 + TILE + w + 1 (q0 = t0 - 1),
+// loaded into registers one tile ahead and stored to LDS when the tile starts.
+template <int WG, int TILE>
+struct GAhead {
+    static constexpr int N = (TILE + WIN_MAX + 2 + WG - 1) / WG;
+    uint32_t v[N];
+    __device__ __forceinline__ void load(const PartitionArgs& a, const uint32_t* __restrict__ gbin, uint64_t t0) {
+        const int64_t q0 = int64_t(t0) - 1, n = int64_t(a.n_bases);
+        const int span = TILE + int(a.w) + 2;
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+            const int i = int(threadIdx.x) + c * WG;
+            const int64_t p = q0 + i;
+            v[c] = (i < span && p >= 0 && p < n) ? gbin[p] : G_INVALID;
+        }
+    }
+    __device__ __forceinline__ void store(uint32_t* s_g) const {
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+            const int i = int(threadIdx.x) + c * WG;
+            if (i < TILE + WIN_MAX + 2) s_g[i] = v[c];
+        }
+    }
+};
+
 // Weak super k-mers with their first (k-1)-mer in [t0, t0 + TILE) (t0 a
-// multiple of P_TILE): calls emit(g, mb, len, l_disc, r_disc, start) once per
-// piece, where mb is the 4 hash bits above g and start the first base of the
-// record.  With s_words, the tile's packed words (from word t0/32 - 1) are
-// staged there first.  Pieces are first collected in s_desc (TILE entries),
-// so emit runs on every lane instead of inside the per-position loop.  Every thread of the WG-thread workgroup must call it (it
-// synchronises).  The run start before t0 comes from tile_lc: a wave looks
-// back 64 minimizer tiles at a time; only runs longer than a tile need more
-// than one step.
-template <int WG, int TILE, class Emit>
-__device__ __forceinline__ void tile_pieces(const PartitionArgs& a, const uint32_t* __restrict__ gbin,
-                                            const uint64_t* __restrict__ tile_lc, uint64_t t0,
-                                            uint32_t* s_g, unsigned long long* s_carry,
-                                            uint32_t* s_wave, uint64_t* s_words, uint64_t* s_desc,
-                                            Emit&& emit) {
+// multiple of P_TILE), whose g values are in s_g: calls emit(g, mb, len,
+// l_disc, r_disc, start) once per piece, where mb is the 4 hash bits above g
+// and start the first base of the record.  Every thread of the WG-thread
+// workgroup must call it (it synchronises).  With COLLECT the pieces are first
+// gathered in s_desc (TILE entries), so emit runs on every lane instead of
+// inside the per-position loop.  The run start before t0 comes from tile_lc:
+// a wave looks back 64 minimizer tiles at a time; only runs longer than a
+// tile need more than one step.
+template <int WG, int TILE, bool COLLECT, class Emit>
+__device__ __forceinline__ void tile_pieces(const PartitionArgs& a, const uint64_t* __restrict__ tile_lc,
+                                            uint64_t t0, const uint32_t* s_g, unsigned long long* s_carry,
+                                            uint32_t* s_wave, uint64_t* s_desc, Emit&& emit) {
     constexpr int PPT = TILE / WG;  // positions per thread
     static_assert(PPT <= 32 && TILE % P_TILE == 0, "tile shape");
     const int tid = threadIdx.x;
     const int64_t q0 = int64_t(t0) - 1;
-    const int64_t N = int64_t(a.n_bases);
     const unsigned k = a.k, w = a.w;
 
-    __syncthreads();  // the previous tile is done with s_g (and s_words)
-    for (int i = tid; i < TILE + int(w) + 2; i += WG) {
-        const int64_t p = q0 + i;
-        s_g[i] = (p >= 0 && p < N) ? gbin[p] : G_INVALID;
-    }
-    if (s_words) {  // packed words of every record starting in the tile
-        const uint64_t wlo = t0 ? t0 / 32 - 1 : 0;
-        for (int i = tid; i < TILE_WORDS(TILE); i += WG)
-            s_words[i] = wlo + i < a.n_words ? a.packed[wlo + i] : 0;
-    }
     if (tid < 64) {
         int64_t j = int64_t(t0 / P_TILE) - 1 - tid;
         unsigned long long found = 0;
@@ -248,7 +260,6 @@ __device__ __forceinline__ void tile_pieces(const PartitionArgs& a, const uint32
         }
         if (tid == 0) *s_carry = found;
     }
-    __syncthreads();
 
     // Change flags for my PPT contiguous positions.
     const int base_idx = 1 + tid * PPT;
@@ -264,7 +275,7 @@ __device__ __forceinline__ void tile_pieces(const PartitionArgs& a, const uint32
             last_chg = uint32_t(idx);
         }
     }
-    const uint32_t incl = block_incl_max(last_chg, s_wave);
+    const uint32_t incl = block_incl_max(last_chg, s_wave);  // its barriers publish s_carry
     const uint32_t excl_t = __shfl_up(incl, 1);
     uint32_t excl = (threadIdx.x & 63) ? excl_t : 0;
     if ((threadIdx.x & 63) == 0 && tid > 0) {
@@ -278,8 +289,9 @@ __device__ __forceinline__ void tile_pieces(const PartitionArgs& a, const uint32
     // phase (p - run start) mod w is carried from position to position; only
     // the thread's first position in a run needs a division.
     auto phase_start = [&](int64_t p) -> uint32_t { return rs0 < 0 ? 0u : uint32_t(uint64_t(p - rs0) % w); };
-    uint32_t n_mine = 0;
-    {
+    uint32_t o = 0, n_tile = 0;
+    if constexpr (COLLECT) {
+        uint32_t n_mine = 0;
         bool have = false;
         uint32_t ph = 0;
 #pragma unroll
@@ -294,12 +306,11 @@ __device__ __forceinline__ void tile_pieces(const PartitionArgs& a, const uint32
             have = true;
             n_mine += ph == 0;
         }
+        o = block_excl_sum<WG>(n_mine, s_wave, &n_tile);
     }
-    uint32_t n_tile;
-    const uint32_t my_off = block_excl_sum<WG>(n_mine, s_wave, &n_tile);
     {
         bool have = false;
-        uint32_t ph = 0, o = my_off;
+        uint32_t ph = 0;
 #pragma unroll 1
         for (int j = 0; j < PPT; ++j) {
             const int idx = base_idx + j;
@@ -326,50 +337,62 @@ __device__ __forceinline__ void tile_pieces(const PartitionArgs& a, const uint32
             const uint32_t l_disc = lj && ((pv & 0xFFFFu) != g);
             const uint32_t r_disc = rj && ((nv & 0xFFFFu) != g);
             const uint32_t len = lj + uint32_t(e - idx) + (k - 2) + rj;
-            s_desc[o++] = uint64_t(idx) | (uint64_t(g) << 16) | (uint64_t((cur >> 16) & 15u) << 32) |
-                          (uint64_t(len) << 36) | (uint64_t(l_disc) << 44) | (uint64_t(r_disc) << 45) |
-                          (uint64_t(lj) << 46);
+            if constexpr (COLLECT)
+                s_desc[o++] = uint64_t(idx) | (uint64_t(g) << 16) | (uint64_t((cur >> 16) & 15u) << 32) |
+                              (uint64_t(len) << 36) | (uint64_t(l_disc) << 44) | (uint64_t(r_disc) << 45) |
+                              (uint64_t(lj) << 46);
+            else
+                emit(g, (cur >> 16) & 15u, len, l_disc, r_disc, uint64_t(q0 + int64_t(idx)) - lj);
         }
     }
-    __syncthreads();
-    // Every piece of the tile, one lane each.
-    for (uint32_t i = tid; i < n_tile; i += WG) {
-        const uint64_t d = s_desc[i];
-        const uint32_t idx = uint32_t(d & 0xFFFFu), lj = uint32_t(d >> 46) & 1u;
-        emit(uint32_t(d >> 16) & 0xFFFFu, uint32_t(d >> 32) & 15u, uint32_t(d >> 36) & 0xFFu,
-             uint32_t(d >> 44) & 1u, uint32_t(d >> 45) & 1u, uint64_t(q0 + int64_t(idx)) - lj);
+    if constexpr (COLLECT) {
+        __syncthreads();
+        // Every piece of the tile, one lane each.
+        for (uint32_t i = tid; i < n_tile; i += WG) {
+            const uint64_t d = s_desc[i];
+            const uint32_t idx = uint32_t(d & 0xFFFFu), lj = uint32_t(d >> 46) & 1u;
+            emit(uint32_t(d >> 16) & 0xFFFFu, uint32_t(d >> 32) & 15u, uint32_t(d >> 36) & 0xFFu,
+                 uint32_t(d >> 44) & 1u, uint32_t(d >> 45) & 1u, uint64_t(q0 + int64_t(idx)) - lj);
+        }
     }
 }
 
-// Span of workgroup blockIdx.x: E_TILE tiles [b * per, (b + 1) * per).
+// Span of workgroup blockIdx.x: minimizer tiles (P_TILE positions)
+// [b * per, (b + 1) * per).
 struct Span {
     uint64_t t_begin, t_end;
 };
 __device__ __forceinline__ Span span_of(uint64_t n_bases, uint64_t per) {
-    const uint64_t n_t = (n_bases + E_TILE - 1) / E_TILE;
+    const uint64_t n_t = (n_bases + P_TILE - 1) / P_TILE;
     const uint64_t b = min(n_t, uint64_t(blockIdx.x) * per);
     return {b, min(n_t, b + per)};
 }
 
-// Pass 2: records per coarse bin (g >> FINE_BITS) of each count span (per
-// tiles) into hist[c * gridDim.x + span].  No global atomics.
-__global__ __launch_bounds__(E_WG) void k_pieces_hist(PartitionArgs a, const uint32_t* __restrict__ gbin,
+// Pass 2: records per coarse bin (g >> FINE_BITS) of each count span into
+// hist[c * gridDim.x + span].  No global atomics.
+__global__ __launch_bounds__(H_WG) void k_pieces_hist(PartitionArgs a, const uint32_t* __restrict__ gbin,
                                                       const uint64_t* __restrict__ tile_lc, uint64_t per,
                                                       uint32_t* __restrict__ hist) {
-    __shared__ uint32_t s_g[E_TILE + WIN_MAX + 2];
+    __shared__ uint32_t s_g[P_TILE + WIN_MAX + 2];
     __shared__ unsigned long long s_carry;
-    __shared__ uint32_t s_wave[E_WG / 64];
+    __shared__ uint32_t s_wave[H_WG / 64];
     __shared__ uint32_t s_hist[N_COARSE];
-    __shared__ uint64_t s_desc[E_TILE];
-    for (int i = threadIdx.x; i < N_COARSE; i += E_WG) s_hist[i] = 0;
+    for (int i = threadIdx.x; i < N_COARSE; i += H_WG) s_hist[i] = 0;
     const Span sp = span_of(a.n_bases, per);
-    for (uint64_t t = sp.t_begin; t < sp.t_end; ++t)
-        tile_pieces<E_WG, E_TILE>(a, gbin, tile_lc, t * E_TILE, s_g, &s_carry, s_wave, nullptr, s_desc,
-                                  [&](uint32_t g, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t) {
-                                      atomicAdd(&s_hist[g >> FINE_BITS], 1u);
-                                  });
+    GAhead<H_WG, P_TILE> ga;
+    if (sp.t_begin < sp.t_end) ga.load(a, gbin, sp.t_begin * P_TILE);
+    for (uint64_t t = sp.t_begin; t < sp.t_end; ++t) {
+        __syncthreads();  // the previous tile is done with s_g
+        ga.store(s_g);
+        if (t + 1 < sp.t_end) ga.load(a, gbin, (t + 1) * P_TILE);
+        __syncthreads();
+        tile_pieces<H_WG, P_TILE, false>(a, tile_lc, t * P_TILE, s_g, &s_carry, s_wave, nullptr,
+                                         [&](uint32_t g, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t) {
+                                             atomicAdd(&s_hist[g >> FINE_BITS], 1u);
+                                         });
+    }
     __syncthreads();
-    for (int i = threadIdx.x; i < N_COARSE; i += E_WG) hist[uint64_t(i) * gridDim.x + blockIdx.x] = s_hist[i];
+    for (int i = threadIdx.x; i < N_COARSE; i += H_WG) hist[uint64_t(i) * gridDim.x + blockIdx.x] = s_hist[i];
 }
 
 // Per coarse bin c (one workgroup each): exclusive scan of hist[c][*] over
@@ -435,27 +458,37 @@ __global__ __launch_bounds__(E_WG) void k_pieces_emit(PartitionArgs a, const uin
     __shared__ uint64_t s_desc[E_TILE];
     for (int i = threadIdx.x; i < N_COARSE; i += E_WG)
         s_cur[i] = coff[i] + hist[uint64_t(i) * n_hist + uint64_t(blockIdx.x) * H_SPLIT];
-    const Span sp = span_of(a.n_bases, per);
-    for (uint64_t t = sp.t_begin; t < sp.t_end; ++t) {
-        const uint64_t t0 = t * E_TILE;
-        const uint64_t base0 = t0 ? t0 - 32 : 0;  // first base of s_words[0]
-        tile_pieces<E_WG, E_TILE>(a, gbin, tile_lc, t0, s_g, &s_carry, s_wave, s_words, s_desc,
-                                  [&](uint32_t g, uint32_t mb, uint32_t len, uint32_t l_disc, uint32_t r_disc,
-                                      uint64_t start) {
-                                      const uint64_t slot = atomicAdd(&s_cur[g >> FINE_BITS], 1ull);
-                                      uint64_t* out = rec + slot * (W + 1);
-                                      out[0] = uint64_t(len | (l_disc << 8) | (r_disc << 9) | (g << 16)) |
-                                               (uint64_t(mb) << 32);
-                                      for (uint32_t wj = 0; wj < W; ++wj) {
-                                          uint64_t x = 0;
-                                          if (32 * wj < len) {
-                                              x = stream_get64(s_words, start - base0 + 32ull * wj);
-                                              const uint32_t nb = len - 32 * wj;
-                                              if (nb < 32) x &= ~0ull << (64 - 2 * nb);
-                                          }
-                                          out[1 + (W - 1 - wj)] = x;
-                                      }
-                                  });
+    const Span sp = span_of(a.n_bases, per);  // per is a multiple of E_TILE / P_TILE
+    constexpr uint64_t STEP = E_TILE / P_TILE;
+    GAhead<E_WG, E_TILE> ga;
+    if (sp.t_begin < sp.t_end) ga.load(a, gbin, sp.t_begin * P_TILE);
+    for (uint64_t t = sp.t_begin; t < sp.t_end; t += STEP) {
+        const uint64_t t0 = t * P_TILE;
+        __syncthreads();  // the previous tile is done with s_g, s_words and s_desc
+        ga.store(s_g);
+        if (t + STEP < sp.t_end) ga.load(a, gbin, (t + STEP) * P_TILE);
+        const uint64_t wlo = t0 ? t0 / 32 - 1 : 0;  // s_words[0] holds bases wlo*32 ..
+        for (int i = threadIdx.x; i < TILE_WORDS(E_TILE); i += E_WG)
+            s_words[i] = wlo + i < a.n_words ? a.packed[wlo + i] : 0;
+        __syncthreads();
+        const uint64_t base0 = wlo * 32;
+        tile_pieces<E_WG, E_TILE, true>(a, tile_lc, t0, s_g, &s_carry, s_wave, s_desc,
+                                        [&](uint32_t g, uint32_t mb, uint32_t len, uint32_t l_disc,
+                                            uint32_t r_disc, uint64_t start) {
+                                            const uint64_t slot = atomicAdd(&s_cur[g >> FINE_BITS], 1ull);
+                                            uint64_t* out = rec + slot * (W + 1);
+                                            out[0] = uint64_t(len | (l_disc << 8) | (r_disc << 9) | (g << 16)) |
+                                                     (uint64_t(mb) << 32);
+                                            for (uint32_t wj = 0; wj < W; ++wj) {
+                                                uint64_t x = 0;
+                                                if (32 * wj < len) {
+                                                    x = stream_get64(s_words, start - base0 + 32ull * wj);
+                                                    const uint32_t nb = len - 32 * wj;
+                                                    if (nb < 32) x &= ~0ull << (64 - 2 * nb);
+                                                }
+                                                out[1 + (W - 1 - wj)] = x;
+                                            }
+                                        });
     }
 }
 
@@ -544,20 +577,24 @@ void launch_minimizer_g(const PartitionArgs& a, uint32_t* gbin, uint64_t* tile_l
 
 PieceSpans pieces_spans(uint64_t n_bases, uint32_t n_cu) {
     PieceSpans ps{0, 0, 0, 0};
-    const uint64_t n_t = (n_bases + E_TILE - 1) / E_TILE;
+    const uint64_t n_t = (n_bases + P_TILE - 1) / P_TILE;
     if (!n_t) return ps;
-    const uint64_t n_h = std::min<uint64_t>(n_t, uint64_t(n_cu) * H_SPLIT);
-    ps.per_count = (n_t + n_h - 1) / n_h;
-    ps.n_count = uint32_t((n_t + ps.per_count - 1) / ps.per_count);
-    ps.per_emit = ps.per_count * H_SPLIT;
+    // emit spans: E_SPANS_PER_CU per CU, each a multiple of the emit tile
+    // (E_TILE / P_TILE minimizer tiles) and of H_SPLIT count spans
+    constexpr uint64_t Q = uint64_t(E_TILE / P_TILE) * H_SPLIT;
+    const uint64_t n_e = std::max<uint64_t>(1, std::min<uint64_t>(uint64_t(n_cu) * E_SPANS_PER_CU, (n_t + Q - 1) / Q));
+    ps.per_emit = (n_t + n_e - 1) / n_e;
+    ps.per_emit = (ps.per_emit + Q - 1) / Q * Q;
+    ps.per_count = ps.per_emit / H_SPLIT;
     ps.n_emit = uint32_t((n_t + ps.per_emit - 1) / ps.per_emit);
+    ps.n_count = uint32_t((n_t + ps.per_count - 1) / ps.per_count);
     return ps;
 }
 
 void launch_pieces_hist(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, const PieceSpans& ps,
                         uint32_t* hist, uint64_t* ctot, uint64_t* coff, hipStream_t s) {
     if (!ps.n_count) return;
-    hipLaunchKernelGGL(k_pieces_hist, dim3(ps.n_count), dim3(E_WG), 0, s, a, gbin, tile_lc, ps.per_count, hist);
+    hipLaunchKernelGGL(k_pieces_hist, dim3(ps.n_count), dim3(H_WG), 0, s, a, gbin, tile_lc, ps.per_count, hist);
     hipLaunchKernelGGL(k_hist_scan, dim3(N_COARSE), dim3(1024), 0, s, hist, uint64_t(ps.n_count), ctot);
     hipLaunchKernelGGL(k_coarse_scan, dim3(1), dim3(N_COARSE), 0, s, ctot, coff);
 }
