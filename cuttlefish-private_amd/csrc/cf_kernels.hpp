@@ -37,7 +37,7 @@ constexpr int FINE_BITS = 6;
 constexpr int N_COARSE = 256;
 constexpr int E_WG = 512;          // emit workgroup, E_TILE positions per step
 constexpr int E_TILE = 4096;
-constexpr int E_SPANS_PER_CU = 3;  // emit spans (workgroups) per CU
+constexpr int E_SPANS_PER_CU = 2;  // emit spans (workgroups) per CU
 constexpr int H_WG = 256;          // count workgroup, one minimizer tile per step
 static_assert((N_COARSE << FINE_BITS) == 16384, "coarse x fine bins cover the 16384 subgraphs");
 constexpr int H_SPLIT = 4;  // count spans per emit span

@@ -232,34 +232,17 @@ struct GAhead {
 // and start the first base of the record.  Every thread of the WG-thread
 // workgroup must call it (it synchronises).  With COLLECT the pieces are first
 // gathered in s_desc (TILE entries), so emit runs on every lane instead of
-// inside the per-position loop.  The run start before t0 comes from tile_lc:
-// a wave looks back 64 minimizer tiles at a time; only runs longer than a
-// tile need more than one step.
+// inside the per-position loop.  `carry` is the last g change (+1) before t0
+// (0: none); the same value for the next tile is returned.
 template <int WG, int TILE, bool COLLECT, class Emit>
-__device__ __forceinline__ void tile_pieces(const PartitionArgs& a, const uint64_t* __restrict__ tile_lc,
-                                            uint64_t t0, const uint32_t* s_g, unsigned long long* s_carry,
-                                            uint32_t* s_wave, uint64_t* s_desc, Emit&& emit) {
+__device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a, uint64_t t0,
+                                                          unsigned long long carry, const uint32_t* s_g,
+                                                          uint32_t* s_wave, uint64_t* s_desc, Emit&& emit) {
     constexpr int PPT = TILE / WG;  // positions per thread
     static_assert(PPT <= 32 && TILE % P_TILE == 0, "tile shape");
     const int tid = threadIdx.x;
     const int64_t q0 = int64_t(t0) - 1;
     const unsigned k = a.k, w = a.w;
-
-    if (tid < 64) {
-        int64_t j = int64_t(t0 / P_TILE) - 1 - tid;
-        unsigned long long found = 0;
-        while (true) {
-            const unsigned long long v = j >= 0 ? tile_lc[j] : 0ull;
-            const unsigned long long m = __ballot(v != 0 || j < 0);
-            if (m) {
-                const int first = __ffsll((long long)m) - 1;
-                found = __shfl(v, first);
-                break;
-            }
-            j -= 64;
-        }
-        if (tid == 0) *s_carry = found;
-    }
 
     // Change flags for my PPT contiguous positions.
     const int base_idx = 1 + tid * PPT;
@@ -275,14 +258,18 @@ __device__ __forceinline__ void tile_pieces(const PartitionArgs& a, const uint64
             last_chg = uint32_t(idx);
         }
     }
-    const uint32_t incl = block_incl_max(last_chg, s_wave);  // its barriers publish s_carry
+    const uint32_t incl = block_incl_max(last_chg, s_wave);
     const uint32_t excl_t = __shfl_up(incl, 1);
     uint32_t excl = (threadIdx.x & 63) ? excl_t : 0;
     if ((threadIdx.x & 63) == 0 && tid > 0) {
         // first lane of a wave: the exclusive value is the max over earlier waves
         for (int j = 0; j < (tid >> 6); ++j) excl = s_wave[j] > excl ? s_wave[j] : excl;
     }
-    const unsigned long long carry = *s_carry;
+    // the tile's last change, for the next tile's carry (every wave's
+    // maximum is in s_wave until the next scan)
+    uint32_t tile_last = 0;
+    for (int j = 0; j < WG / 64; ++j) tile_last = s_wave[j] > tile_last ? s_wave[j] : tile_last;
+    const unsigned long long carry_out = tile_last ? (unsigned long long)(q0 + int64_t(tile_last) + 1) : carry;
     const int64_t rs0 = excl ? q0 + int64_t(excl) : (carry ? int64_t(carry) - 1 : -1);
 
     // A piece starts at every change and every w (k-1)-mers after one.  The
@@ -355,6 +342,20 @@ __device__ __forceinline__ void tile_pieces(const PartitionArgs& a, const uint64
                  uint32_t(d >> 44) & 1u, uint32_t(d >> 45) & 1u, uint64_t(q0 + int64_t(idx)) - lj);
         }
     }
+    return carry_out;
+}
+
+// Last g change (+1) before minimizer tile t, from tile_lc: a wave looks back
+// 64 tiles at a time; only runs longer than a tile need more than one step.
+// Called by wave 0; the value is returned to every lane of that wave.
+__device__ __forceinline__ unsigned long long carry_before(const uint64_t* __restrict__ tile_lc, uint64_t t) {
+    int64_t j = int64_t(t) - 1 - int64_t(threadIdx.x & 63);
+    while (true) {
+        const unsigned long long v = j >= 0 ? tile_lc[j] : 0ull;
+        const unsigned long long m = __ballot(v != 0 || j < 0);
+        if (m) return __shfl(v, __ffsll((long long)m) - 1);
+        j -= 64;
+    }
 }
 
 // Span of workgroup blockIdx.x: minimizer tiles (P_TILE positions)
@@ -380,16 +381,24 @@ __global__ __launch_bounds__(H_WG) void k_pieces_hist(PartitionArgs a, const uin
     for (int i = threadIdx.x; i < N_COARSE; i += H_WG) s_hist[i] = 0;
     const Span sp = span_of(a.n_bases, per);
     GAhead<H_WG, P_TILE> ga;
-    if (sp.t_begin < sp.t_end) ga.load(a, gbin, sp.t_begin * P_TILE);
+    if (sp.t_begin < sp.t_end) {
+        ga.load(a, gbin, sp.t_begin * P_TILE);
+        if (threadIdx.x < 64) {
+            const unsigned long long c = carry_before(tile_lc, sp.t_begin);
+            if (threadIdx.x == 0) s_carry = c;
+        }
+    }
+    __syncthreads();
+    unsigned long long carry = s_carry;
     for (uint64_t t = sp.t_begin; t < sp.t_end; ++t) {
         __syncthreads();  // the previous tile is done with s_g
         ga.store(s_g);
         if (t + 1 < sp.t_end) ga.load(a, gbin, (t + 1) * P_TILE);
         __syncthreads();
-        tile_pieces<H_WG, P_TILE, false>(a, tile_lc, t * P_TILE, s_g, &s_carry, s_wave, nullptr,
-                                         [&](uint32_t g, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t) {
-                                             atomicAdd(&s_hist[g >> FINE_BITS], 1u);
-                                         });
+        carry = tile_pieces<H_WG, P_TILE, false>(a, t * P_TILE, carry, s_g, s_wave, nullptr,
+                                                 [&](uint32_t g, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t) {
+                                                     atomicAdd(&s_hist[g >> FINE_BITS], 1u);
+                                                 });
     }
     __syncthreads();
     for (int i = threadIdx.x; i < N_COARSE; i += H_WG) hist[uint64_t(i) * gridDim.x + blockIdx.x] = s_hist[i];
@@ -456,23 +465,43 @@ __global__ __launch_bounds__(E_WG) void k_pieces_emit(PartitionArgs a, const uin
     __shared__ uint64_t s_words[TILE_WORDS(E_TILE)];
     __shared__ unsigned long long s_cur[N_COARSE];
     __shared__ uint64_t s_desc[E_TILE];
+    static_assert(TILE_WORDS(E_TILE) <= E_WG, "one packed word per thread");
     for (int i = threadIdx.x; i < N_COARSE; i += E_WG)
         s_cur[i] = coff[i] + hist[uint64_t(i) * n_hist + uint64_t(blockIdx.x) * H_SPLIT];
     const Span sp = span_of(a.n_bases, per);  // per is a multiple of E_TILE / P_TILE
     constexpr uint64_t STEP = E_TILE / P_TILE;
+    // g values and packed words of the next tile are loaded while the
+    // current one is processed
     GAhead<E_WG, E_TILE> ga;
-    if (sp.t_begin < sp.t_end) ga.load(a, gbin, sp.t_begin * P_TILE);
+    uint64_t word_next = 0;
+    auto load_words = [&](uint64_t t0) {
+        const uint64_t wlo = t0 ? t0 / 32 - 1 : 0;
+        const uint64_t wi = wlo + threadIdx.x;
+        word_next = threadIdx.x < TILE_WORDS(E_TILE) && wi < a.n_words ? a.packed[wi] : 0;
+    };
+    if (sp.t_begin < sp.t_end) {
+        ga.load(a, gbin, sp.t_begin * P_TILE);
+        load_words(sp.t_begin * P_TILE);
+        if (threadIdx.x < 64) {
+            const unsigned long long c = carry_before(tile_lc, sp.t_begin);
+            if (threadIdx.x == 0) s_carry = c;
+        }
+    }
+    __syncthreads();
+    unsigned long long carry = s_carry;
     for (uint64_t t = sp.t_begin; t < sp.t_end; t += STEP) {
         const uint64_t t0 = t * P_TILE;
         __syncthreads();  // the previous tile is done with s_g, s_words and s_desc
         ga.store(s_g);
-        if (t + STEP < sp.t_end) ga.load(a, gbin, (t + STEP) * P_TILE);
-        const uint64_t wlo = t0 ? t0 / 32 - 1 : 0;  // s_words[0] holds bases wlo*32 ..
-        for (int i = threadIdx.x; i < TILE_WORDS(E_TILE); i += E_WG)
-            s_words[i] = wlo + i < a.n_words ? a.packed[wlo + i] : 0;
+        if (threadIdx.x < TILE_WORDS(E_TILE)) s_words[threadIdx.x] = word_next;
+        if (t + STEP < sp.t_end) {
+            ga.load(a, gbin, (t + STEP) * P_TILE);
+            load_words((t + STEP) * P_TILE);
+        }
         __syncthreads();
+        const uint64_t wlo = t0 ? t0 / 32 - 1 : 0;  // s_words[0] holds bases wlo*32 ..
         const uint64_t base0 = wlo * 32;
-        tile_pieces<E_WG, E_TILE, true>(a, tile_lc, t0, s_g, &s_carry, s_wave, s_desc,
+        carry = tile_pieces<E_WG, E_TILE, true>(a, t0, carry, s_g, s_wave, s_desc,
                                         [&](uint32_t g, uint32_t mb, uint32_t len, uint32_t l_disc,
                                             uint32_t r_disc, uint64_t start) {
                                             const uint64_t slot = atomicAdd(&s_cur[g >> FINE_BITS], 1ull);
