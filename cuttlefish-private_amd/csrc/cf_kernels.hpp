@@ -35,12 +35,12 @@ void launch_minimizer_g(const PartitionArgs& a, uint32_t* gbin, uint64_t* tile_l
 // coarse bin into its 2^FINE_BITS buckets.
 constexpr int FINE_BITS = 6;
 constexpr int N_COARSE = 256;
-constexpr int E_WG = 512;          // emit workgroup, E_TILE positions per step
-constexpr int E_TILE = 4096;
-constexpr int E_SPANS_PER_CU = 2;  // emit spans (workgroups) per CU
+constexpr int E_WG = 256;          // emit workgroup, E_TILE positions per step
+constexpr int E_TILE = 2048;
+constexpr int E_SPANS_PER_CU = 8;  // emit spans (workgroups) per CU
 constexpr int H_WG = 256;          // count workgroup, one minimizer tile per step
 static_assert((N_COARSE << FINE_BITS) == 16384, "coarse x fine bins cover the 16384 subgraphs");
-constexpr int H_SPLIT = 4;  // count spans per emit span
+constexpr int H_SPLIT = 1;  // count spans per emit span
 struct PieceSpans {
     uint64_t per_count, per_emit;  // minimizer tiles (P_TILE positions) per count / emit span
     uint32_t n_count, n_emit;      // spans (workgroups)
