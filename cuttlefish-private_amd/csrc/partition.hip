@@ -273,43 +273,33 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
     const int64_t rs0 = excl ? q0 + int64_t(excl) : (carry ? int64_t(carry) - 1 : -1);
 
     // A piece starts at every change and every w (k-1)-mers after one.  The
-    // phase (p - run start) mod w is carried from position to position; only
-    // the thread's first position in a run needs a division.
-    auto phase_start = [&](int64_t p) -> uint32_t { return rs0 < 0 ? 0u : uint32_t(uint64_t(p - rs0) % w); };
+    // phase (p - run start) mod w is carried from position to position; a
+    // valid position after an invalid one always starts a run (it has a change
+    // flag), so only the thread's first position can need a division.
+    uint32_t ph0 = 0;
+    if (rs0 >= 0 && !(chg & 1u) && s_g[base_idx] != G_INVALID)
+        ph0 = uint32_t(uint64_t(q0 + base_idx - rs0) % w);
     uint32_t o = 0, n_tile = 0;
     if constexpr (COLLECT) {
         uint32_t n_mine = 0;
-        bool have = false;
-        uint32_t ph = 0;
+        uint32_t ph = ph0;
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
-            const int idx = base_idx + j;
-            if (s_g[idx] == G_INVALID) {
-                have = false;
-                continue;
-            }
-            const bool c = (chg >> j) & 1u;
-            ph = c ? 0u : (have ? (ph + 1 == w ? 0u : ph + 1) : phase_start(q0 + idx));
-            have = true;
-            n_mine += ph == 0;
+            if (j) ph = ph + 1 == w ? 0u : ph + 1;
+            if ((chg >> j) & 1u) ph = 0;
+            n_mine += ph == 0 && s_g[base_idx + j] != G_INVALID;
         }
         o = block_excl_sum<WG>(n_mine, s_wave, &n_tile);
     }
     {
-        bool have = false;
-        uint32_t ph = 0;
+        uint32_t ph = ph0;
 #pragma unroll 1
         for (int j = 0; j < PPT; ++j) {
+            if (j) ph = ph + 1 == w ? 0u : ph + 1;
+            if ((chg >> j) & 1u) ph = 0;
             const int idx = base_idx + j;
             const uint32_t cur = s_g[idx];
-            if (cur == G_INVALID) {
-                have = false;
-                continue;
-            }
-            const bool c = (chg >> j) & 1u;
-            ph = c ? 0u : (have ? (ph + 1 == w ? 0u : ph + 1) : phase_start(q0 + idx));
-            have = true;
-            if (ph) continue;
+            if (cur == G_INVALID || ph) continue;
             // Piece [idx, e): (k-1)-mers with this g, at most w of them.
             const uint32_t g = cur & 0xFFFFu;
             int e = idx + 1;
