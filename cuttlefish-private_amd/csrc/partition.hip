@@ -197,6 +197,28 @@ __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* s_wave,
     return pre + x - v;
 }
 
+// Per thread: min of v over the threads after it in a WG-thread workgroup
+// (tail for the last thread).
+template <int WG>
+__device__ __forceinline__ uint32_t block_excl_suffix_min(uint32_t v, uint32_t* s_wave, uint32_t tail) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_down(x, d);
+        if (lane + d < 64) x = u < x ? u : x;
+    }
+    __syncthreads();  // s_wave may still be read by a previous scan
+    if (lane == 0) s_wave[wid] = x;
+    __syncthreads();
+    uint32_t after = tail;
+#pragma unroll
+    for (int j = 0; j < WG / 64; ++j)
+        if (j > wid) after = s_wave[j] < after ? s_wave[j] : after;
+    const uint32_t nxt = __shfl_down(x, 1);
+    return lane < 63 ? (nxt < after ? nxt : after) : after;
+}
+
 // Packed words staged per tile: records start at t0 - 1 or later and end
 // before t0 + TILE + 256 (len <= 255), and stream_get64 reads one word ahead.
 #define TILE_WORDS(TILE) ((TILE) / 32 + 16)
@@ -272,6 +294,33 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
     const unsigned long long carry_out = tile_last ? (unsigned long long)(q0 + int64_t(tile_last) + 1) : carry;
     const int64_t rs0 = excl ? q0 + int64_t(excl) : (carry ? int64_t(carry) - 1 : -1);
 
+    // Run boundaries (a change flag or an invalid position) and, per thread,
+    // the first boundary after its positions: a piece starting at idx ends at
+    // min(next boundary, idx + w).
+    uint32_t bnd = chg;
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) bnd |= uint32_t(s_g[base_idx + j] == G_INVALID) << j;
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
+    uint32_t tail = NONE;  // first boundary in (TILE, TILE + w + 1]
+    if (tid < 64) {
+        for (int i0 = TILE + 1; i0 <= TILE + int(w) + 1 && tail == NONE; i0 += 64) {
+            const int i = i0 + tid;
+            bool b = false;
+            if (i <= TILE + int(w) + 1) {
+                const uint32_t cur = s_g[i], prv = s_g[i - 1];
+                b = cur == G_INVALID || prv == G_INVALID || ((prv ^ cur) & 0xFFFFu);
+            }
+            const unsigned long long m = __ballot(b);
+            if (m) tail = uint32_t(i0 + __ffsll((long long)m) - 1);
+        }
+    }
+    __shared__ uint32_t s_tail;
+    if (tid == 0) s_tail = tail;
+    __syncthreads();
+    tail = s_tail;
+    const uint32_t next_bnd = block_excl_suffix_min<WG>(bnd ? uint32_t(base_idx + __ffs(int(bnd)) - 1) : NONE,
+                                                        s_wave, tail);
+
     // A piece starts at every change and every w (k-1)-mers after one.  The
     // phase (p - run start) mod w is carried from position to position; a
     // valid position after an invalid one always starts a run (it has a change
@@ -302,13 +351,9 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
             if (cur == G_INVALID || ph) continue;
             // Piece [idx, e): (k-1)-mers with this g, at most w of them.
             const uint32_t g = cur & 0xFFFFu;
-            int e = idx + 1;
-            const int lim = idx + int(w);
-            while (e < lim) {
-                const uint32_t v = s_g[e];
-                if (v == G_INVALID || (v & 0xFFFFu) != g) break;
-                ++e;
-            }
+            const uint32_t later = bnd >> (j + 1);  // my boundaries after position j
+            const uint32_t nb = later ? uint32_t(idx) + uint32_t(__ffs(int(later))) : next_bnd;
+            const int e = int(min(nb, uint32_t(idx) + w));
             const uint32_t nv = s_g[e], pv = s_g[idx - 1];
             const uint32_t rj = nv != G_INVALID, lj = pv != G_INVALID;
             const uint32_t l_disc = lj && ((pv & 0xFFFFu) != g);
