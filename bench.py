@@ -161,8 +161,9 @@ def main():
     stk = h.bucket_stats()
     h.set_tuning(use_graph=1)
     main = (stk["flags"] & 4) == 0
-    n_sk_m = int(stk["n_records"][main].sum())
-    n_km_m = int(stk["n_kmers"][main].sum())
+    # the records the main round walks (distinct records after dedup)
+    n_sk_m = int(stk["n_records_walked"][main].sum())
+    n_km_m = int(stk["n_kmers_walked"][main].sum())
     n_e_m = n_km_m - n_sk_m
     kbytes = algorithmic_bytes(args.k, args.l, n_sk_m, n_e_m, n_km_m)
     kms = rk["t_construct_kernel_ms"]
@@ -251,7 +252,8 @@ def main():
                       "count": rs["t_count_ms"], "emit": rs["t_emit_ms"], "features": rs["t_features_ms"],
                       "construct": rs["t_construct_ms"], "sample": rs["t_sample_ms"],
                       "train": rs["t_train_ms"], "infer": rs["t_infer_ms"],
-                      "construct_main": rs["t_construct_main_ms"], "fallback": rs["t_fallback_ms"]},
+                      "construct_main": rs["t_construct_main_ms"], "fallback": rs["t_fallback_ms"],
+                      "dedup": rs["t_dedup_ms"]},
         "partition_roofline": {"achieved": pbytes / (part_ms / 1000.0) / 1e9 if part_ms else 0,
                                "unit": "GB/s", "algorithmic_bytes": pbytes},
         "learned": {"samples": rs["n_samples"], "mlp_loss": rs["mlp_loss"],
@@ -259,6 +261,8 @@ def main():
                     "mispredict_under_frac": rs["mispredict_under_frac"],
                     "fallback_rate": rs["fallback_rate"], "overflowed": rs["n_overflowed"],
                     "lds_buckets": rs["n_lds_buckets"],
+                    "records_walked": rs["n_records_walked"], "dedup_ms": rs["t_dedup_ms"],
+                    "dedup_fallback_buckets": rs["n_dedup_fallback"],
                     "predictor": {0: "none", 1: "mlp", 2: "constant ratio"}.get(rs["predictor"]),
                     "holdout_err_mlp": rs["holdout_err_mlp"], "holdout_err_const": rs["holdout_err_const"],
                     "table_bytes": rs["table_bytes"]},

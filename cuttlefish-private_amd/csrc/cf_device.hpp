@@ -161,16 +161,21 @@ __device__ __forceinline__ uint64_t km_hash(const Kmer<KW>& x) {
     return fmix64(h);
 }
 
-// Saturating 4-bit counters (include/State_Config.hpp:207-217): add one to the
-// front field `fb` (if < 4) and the back field `bb` (if < 4).
-__device__ __forceinline__ uint32_t ef_add(uint32_t e, uint8_t fb, uint8_t bb) {
+// Saturating 4-bit counters (include/State_Config.hpp:207-217): add m to the
+// front field `fb` (if < 4) and the back field `bb` (if < 4), each field
+// saturating at 15 (m single adds give the same word).
+__device__ __forceinline__ uint32_t ef_add(uint32_t e, uint8_t fb, uint8_t bb, uint32_t m = 1) {
     if (fb < 4) {
         const unsigned off = 4u * fb;
-        if (((e >> off) & 0xF) < 0xF) e += 1u << off;
+        const uint32_t f = (e >> off) & 0xFu;
+        const uint32_t nf = f + m < 15u ? f + m : 15u;
+        e += (nf - f) << off;
     }
     if (bb < 4) {
         const unsigned off = 16u + 4u * bb;
-        if (((e >> off) & 0xF) < 0xF) e += 1u << off;
+        const uint32_t f = (e >> off) & 0xFu;
+        const uint32_t nf = f + m < 15u ? f + m : 15u;
+        e += (nf - f) << off;
     }
     return e;
 }

@@ -79,6 +79,8 @@ struct ConstructArgs {
     const uint32_t* item_count;   //   record count
     uint32_t n_items;
     uint32_t* queue_head;       // persistent mode: next item
+    const uint32_t* mult;       // per record multiplicity (null: every record counts once)
+    const uint64_t* bucket_end; // per bucket end of its records (null: bucket_off[b + 1])
 };
 
 // Entry stride in bytes for KW key words: KW u64 key words, u32 e_f, u32 status.
@@ -112,6 +114,35 @@ void launch_export(const uint64_t* tbl_ptr, const uint64_t* tbl_cap,
                    const uint64_t* out_off, uint32_t* out_cursor, uint64_t* keys_out,
                    uint32_t* ef_out, uint8_t* st_out, uint8_t* dfa_out, uint32_t cutoff,
                    hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Exact record dedup per bucket (dedup.hip): distinct (len, disc bits, label)
+// records with their multiplicities, written to the bucket's own region of
+// the out_* arrays; out_count[b] distinct records.  One work item = one hash
+// pass of one bucket; the LDS table holds dedup_capacity(W) records.
+constexpr int DD_WG = 1024;
+constexpr int dedup_capacity(int W) { return ((163840 - 2048) / (8 * (W + 1) + 4)) & ~63; }
+struct DedupArgs {
+    const uint32_t* att;
+    const uint64_t* labels;
+    const uint64_t* bucket_off;
+    uint32_t* overflow;      // per bucket: a pass overflowed (records are then copied)
+    uint32_t* out_count;     // per bucket distinct records (zeroed before the launch)
+    unsigned long long* out_kmers;  // per bucket k-mer instances of those records (zeroed)
+    uint32_t k;
+    uint32_t* out_att;
+    uint64_t* out_labels;
+    uint32_t* out_mult;
+    const uint32_t* item_bucket;
+    const uint32_t* item_pass;
+    const uint32_t* item_passes;
+    uint32_t n_items;
+    uint32_t* queue_head;
+};
+void launch_dedup(const DedupArgs& a, uint32_t W, uint32_t grid, hipStream_t s);
+// Buckets in list[0..n): records copied unchanged with multiplicity 1.
+void launch_dedup_copy(const DedupArgs& a, const uint32_t* list, uint32_t n, uint32_t W, hipStream_t s);
+void launch_dedup_end(const uint64_t* off, const uint32_t* cnt, uint32_t G, uint64_t* end, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // FP32 MLP bucket-size predictor: N_FEAT -> H -> H -> 1 (ReLU), predicts

@@ -86,7 +86,7 @@ struct cfgpu_handle {
     int cus = 256;
     hipStream_t stream = nullptr;
     std::string err;
-    cfgpu_tuning tune{0.6f, 256, 2048, 8, 1.15f, 1, 32, 0};
+    cfgpu_tuning tune{0.6f, 256, 2048, 8, 1.15f, 1, 32, 0, 1};
 
     // partition state
     DevBuf packed, frag, gbin, tile_lc, boff, att, labels, mbin, stats, feat;
@@ -105,6 +105,12 @@ struct cfgpu_handle {
     std::vector<uint32_t> h_limit, h_nvert, h_flags;
     std::vector<float> h_pred;
     bool constructed = false;
+
+    // record dedup: distinct records and multiplicities in each bucket's region
+    DevBuf dd_att, dd_lab, dd_mult, dd_cnt, dd_km, dd_end, dd_ovf, dd_ib, dd_ip, dd_iP, dd_list;
+    bool dd_active = false;
+    std::vector<uint32_t> h_ddcnt;
+    std::vector<uint64_t> h_ddkm;
 
     // MLP
     DevBuf mlp_params, mlp_adam, mlp_idx, mlp_target, mlp_pred, mlp_loss;
@@ -176,10 +182,20 @@ ConstructArgs construct_args(cfgpu_handle* h) {
     a.item_first = h->d_items_f.as<uint32_t>();
     a.item_count = h->d_items_c.as<uint32_t>();
     a.queue_head = h->d_qhead.as<uint32_t>();
+    if (h->dd_active) {
+        a.att = h->dd_att.as<uint32_t>();
+        a.labels = h->dd_lab.as<uint64_t>();
+        a.mult = h->dd_mult.as<uint32_t>();
+        a.bucket_end = h->dd_end.as<uint64_t>();
+    }
     return a;
 }
 
 uint64_t bucket_records(const cfgpu_handle* h, uint32_t b) { return h->h_stats[4 * b + 0]; }
+// Records the construct kernels walk: the distinct ones after dedup.
+uint64_t walked_records(const cfgpu_handle* h, uint32_t b) {
+    return h->dd_active ? h->h_ddcnt[b] : h->h_stats[4 * b + 0];
+}
 uint64_t bucket_kmers(const cfgpu_handle* h, uint32_t b) { return h->h_stats[4 * b + 1]; }
 
 // Capacity that holds every k-mer instance of the bucket at the target load:
@@ -216,7 +232,7 @@ int prepare_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
             for (uint32_t q = 0; q < P; ++q) ib.push_back(b), ifirst.push_back(q), icnt.push_back(P);
             continue;
         }
-        const uint64_t nr = bucket_records(h, b);
+        const uint64_t nr = walked_records(h, b);
         for (uint64_t f = 0; f < nr; f += chunk) {
             ib.push_back(b);
             ifirst.push_back(uint32_t(f));
@@ -421,6 +437,138 @@ void order_largest_first(std::vector<uint32_t>& v, const std::vector<double>& co
     std::stable_sort(v.begin(), v.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
 }
 
+// Record dedup before the batched constructs.  A probe set of evenly spread
+// buckets is deduplicated with passes sized for all-distinct records; when
+// its records repeat less than DEDUP_MIN_GAIN times (auto mode) the
+// construct walks the original records.  Otherwise the other buckets get
+// passes sized from the measured distinct fraction, and buckets whose pass
+// tables overflowed keep their records unchanged (copied, multiplicity 1).
+constexpr double DEDUP_LOAD = 0.8;
+constexpr double DEDUP_MIN_GAIN = 1.33;
+constexpr size_t DEDUP_PROBES = 64;
+
+int dedup_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets, double distinct_frac) {
+    const double C = double(dedup_capacity(int(h->W))) * DEDUP_LOAD;
+    std::vector<uint32_t> ib, ip, iP;
+    for (uint32_t b : buckets) {
+        const double need = std::max(1.0, double(bucket_records(h, b)) * distinct_frac);
+        const uint32_t P = uint32_t(std::min(4096.0, std::ceil(need / C)));
+        for (uint32_t q = 0; q < P; ++q) ib.push_back(b), ip.push_back(q), iP.push_back(P);
+    }
+    if (ib.empty()) return CFGPU_OK;
+    HIPCK(h, h->dd_ib.ensure(ib.size() * 4));
+    HIPCK(h, h->dd_ip.ensure(ib.size() * 4));
+    HIPCK(h, h->dd_iP.ensure(ib.size() * 4));
+    HIPCK(h, hipMemcpyAsync(h->dd_ib.p, ib.data(), ib.size() * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCK(h, hipMemcpyAsync(h->dd_ip.p, ip.data(), ip.size() * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCK(h, hipMemcpyAsync(h->dd_iP.p, iP.data(), iP.size() * 4, hipMemcpyHostToDevice, h->stream));
+    launch_fill_u32(h->d_qhead.as<uint32_t>(), 0u, 1, h->stream);
+    DedupArgs a{};
+    a.att = h->att.as<uint32_t>();
+    a.labels = h->labels.as<uint64_t>();
+    a.bucket_off = h->boff.as<uint64_t>();
+    a.overflow = h->dd_ovf.as<uint32_t>();
+    a.out_count = h->dd_cnt.as<uint32_t>();
+    a.out_kmers = h->dd_km.as<unsigned long long>();
+    a.k = h->prm.k;
+    a.out_att = h->dd_att.as<uint32_t>();
+    a.out_labels = h->dd_lab.as<uint64_t>();
+    a.out_mult = h->dd_mult.as<uint32_t>();
+    a.item_bucket = h->dd_ib.as<uint32_t>();
+    a.item_pass = h->dd_ip.as<uint32_t>();
+    a.item_passes = h->dd_iP.as<uint32_t>();
+    a.n_items = uint32_t(ib.size());
+    a.queue_head = h->d_qhead.as<uint32_t>();
+    launch_dedup(a, h->W, std::min<uint32_t>(a.n_items, uint32_t(h->cus)), h->stream);
+    HIPCK(h, hipGetLastError());
+    return CFGPU_OK;
+}
+
+int dedup_buckets(cfgpu_handle* h, const std::vector<uint32_t>& all) {
+    h->dd_active = false;
+    cfgpu_run_stats& rs = h->rs;
+    rs.n_records_walked = h->n_records;
+    rs.t_dedup_ms = 0;
+    if (!h->tune.dedup || all.empty()) return CFGPU_OK;
+    const size_t G = h->G;
+    const uint64_t n = std::max<uint64_t>(h->n_records, 1);
+    HIPCK(h, h->dd_att.ensure(n * 4));
+    HIPCK(h, h->dd_lab.ensure(n * h->W * 8));
+    HIPCK(h, h->dd_mult.ensure(n * 4));
+    HIPCK(h, h->dd_cnt.ensure(G * 4));
+    HIPCK(h, h->dd_km.ensure(G * 8));
+    HIPCK(h, h->dd_end.ensure(G * 8));
+    HIPCK(h, h->dd_ovf.ensure(G * 4));
+    HIPCK(h, h->dd_list.ensure(G * 4));
+    Event e0, e1;
+    HIPCK(h, hipEventRecord(e0.e, h->stream));
+    launch_fill_u32(h->dd_cnt.as<uint32_t>(), 0u, G, h->stream);
+    launch_fill_u32(h->dd_km.as<uint32_t>(), 0u, 2 * G, h->stream);
+    launch_fill_u32(h->dd_ovf.as<uint32_t>(), 0u, G, h->stream);
+
+    std::vector<uint32_t> probe, rest;
+    if (all.size() >= 4 * DEDUP_PROBES) {
+        const double step = double(all.size()) / double(DEDUP_PROBES);
+        std::vector<char> in(all.size(), 0);
+        for (size_t i = 0; i < DEDUP_PROBES; ++i) in[size_t(i * step)] = 1;
+        for (size_t i = 0; i < all.size(); ++i) (in[i] ? probe : rest).push_back(all[i]);
+    } else {
+        probe = all;
+    }
+    CK(dedup_round(h, probe, 1.0));
+    std::vector<uint32_t> cnt(G), ovf(G);
+    HIPCK(h, hipMemcpyAsync(cnt.data(), h->dd_cnt.p, G * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipMemcpyAsync(ovf.data(), h->dd_ovf.p, G * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    double d = 0, t = 0;
+    for (uint32_t b : probe)
+        if (!ovf[b]) d += cnt[b], t += double(bucket_records(h, b));
+    const double frac = t > 0 ? d / t : 1.0;
+    if (h->tune.dedup == 1 && frac * DEDUP_MIN_GAIN > 1.0) {
+        HIPCK(h, hipEventRecord(e1.e, h->stream));
+        HIPCK(h, hipStreamSynchronize(h->stream));
+        rs.t_dedup_ms = elapsed_ms(e0, e1);
+        return CFGPU_OK;  // records barely repeat: walk them as they are
+    }
+    // margin < 1 under-sizes on purpose here too (exercises the copy fallback)
+    CK(dedup_round(h, rest, std::min(1.0, frac * 1.3) * std::min(1.0, double(h->tune.margin))));
+    HIPCK(h, hipMemcpyAsync(ovf.data(), h->dd_ovf.p, G * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    std::vector<uint32_t> list;
+    for (uint32_t b : all)
+        if (ovf[b]) list.push_back(b);
+    if (!list.empty()) {
+        HIPCK(h, hipMemcpyAsync(h->dd_list.p, list.data(), list.size() * 4, hipMemcpyHostToDevice, h->stream));
+        DedupArgs a{};
+        a.att = h->att.as<uint32_t>();
+        a.labels = h->labels.as<uint64_t>();
+        a.bucket_off = h->boff.as<uint64_t>();
+        a.out_count = h->dd_cnt.as<uint32_t>();
+        a.out_kmers = h->dd_km.as<unsigned long long>();
+        a.k = h->prm.k;
+        a.out_att = h->dd_att.as<uint32_t>();
+        a.out_labels = h->dd_lab.as<uint64_t>();
+        a.out_mult = h->dd_mult.as<uint32_t>();
+        launch_dedup_copy(a, h->dd_list.as<uint32_t>(), uint32_t(list.size()), h->W, h->stream);
+    }
+    launch_dedup_end(h->boff.as<uint64_t>(), h->dd_cnt.as<uint32_t>(), uint32_t(G), h->dd_end.as<uint64_t>(),
+                     h->stream);
+    h->h_ddcnt.resize(G);
+    h->h_ddkm.resize(G);
+    HIPCK(h, hipMemcpyAsync(h->h_ddcnt.data(), h->dd_cnt.p, G * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipMemcpyAsync(h->h_ddkm.data(), h->dd_km.p, G * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipEventRecord(e1.e, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    HIPCK(h, hipGetLastError());
+    uint64_t walked = 0;
+    for (uint32_t b = 0; b < G; ++b) walked += h->h_ddcnt[b];
+    rs.n_records_walked = walked;
+    rs.n_dedup_fallback = uint32_t(list.size());
+    rs.t_dedup_ms = elapsed_ms(e0, e1);
+    h->dd_active = true;
+    return CFGPU_OK;
+}
+
 int construct_impl(cfgpu_handle* h, uint32_t mode) {
     const size_t G = h->G;
     h->round_idx = 0;
@@ -450,6 +598,11 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
 
     std::vector<uint32_t> all = nonempty_buckets(h);
     rs.n_buckets_nonempty = uint32_t(all.size());
+    h->dd_active = false;
+    rs.n_records_walked = h->n_records;
+    rs.n_dedup_fallback = 0;
+    rs.t_dedup_ms = 0;
+    if (mode != CFGPU_MODE_PLAIN) CK(dedup_buckets(h, all));
     Event e0, e1, e2, e3, e4, e5;
     HIPCK(h, hipEventRecord(e0.e, h->stream));
 
@@ -801,6 +954,9 @@ int cfgpu_bucket_stats(cfgpu_handle* h, cfgpu_bucket_stat* out, uint32_t n) {
         s.capacity = c ? h->h_cap[b] : 0;
         s.predicted = c ? h->h_pred[b] : 0.f;
         s.flags = c ? h->h_flags[b] : 0;
+        const bool dd = c && h->dd_active;
+        s.n_records_walked = dd ? h->h_ddcnt[b] : (c ? s.n_records : 0);
+        s.n_kmers_walked = dd ? h->h_ddkm[b] : (c ? s.n_kmers : 0);
     }
     return CFGPU_OK;
 }
@@ -990,7 +1146,7 @@ int cfgpu_tuning_set(cfgpu_handle* h, const cfgpu_tuning* in) {
         (in->workgroup != 256 && in->workgroup != 512) || in->chunk_records < 64 ||
         in->chunk_records > (1u << 24) || in->wgs_per_cu < 1 || in->wgs_per_cu > 32 ||
         !(in->margin >= 0.01f && in->margin <= 4.0f) || in->use_graph > 1 || in->max_passes > 4096 ||
-        in->predictor > 2)
+        in->predictor > 2 || in->dedup > 2)
         return fail(h, CFGPU_E_ARG, "tuning out of range");
     h->tune = *in;
     return CFGPU_OK;
@@ -1005,7 +1161,7 @@ int cfgpu_autotune(cfgpu_handle* h, uint32_t mode, cfgpu_tune_trial* trials_out,
     for (float lf : {0.5f, 0.65f})
         for (uint32_t wg : {256u, 512u})
             for (uint32_t ch : {1024u, 4096u})
-                for (uint32_t wpc : {4u, 8u}) grid.push_back({lf, wg, ch, wpc, base.margin, base.use_graph, base.max_passes, base.predictor});
+                for (uint32_t wpc : {4u, 8u}) grid.push_back({lf, wg, ch, wpc, base.margin, base.use_graph, base.max_passes, base.predictor, base.dedup});
     cfgpu_tuning best = base;
     double best_ms = 1e300;
     uint32_t w = 0;

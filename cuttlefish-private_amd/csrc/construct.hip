@@ -148,6 +148,7 @@ __device__ void construct_item(const ConstructArgs& a, uint32_t item) {
         if (r >= uint32_t(WG) && ld_relaxed32(&a.overflow[b])) break;
         RecordWalker<KW, 2 * KW> rw;
         rw.load(a.att, a.labels, rbase + r, W, k);
+        const uint32_t m = a.mult ? a.mult[rbase + r] : 1u;
         for (uint32_t t = 0; t < rw.n; ++t) {
             const Inst<KW> in = rw.next();
             const Probe pr = find_or_insert<KW>(tbl, cap, in.key);
@@ -159,7 +160,7 @@ __device__ void construct_item(const ConstructArgs& a, uint32_t item) {
                 // stale) word that is already saturated needs no atomic.
                 uint32_t old = pr.ef_seen;
                 while (true) {
-                    const uint32_t nw = ef_add(old, in.front, in.back);
+                    const uint32_t nw = ef_add(old, in.front, in.back, m);
                     if (nw == old) break;
                     if (__hip_atomic_compare_exchange_strong(ef, &old, nw, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                              __HIP_MEMORY_SCOPE_AGENT))

@@ -19,7 +19,8 @@
 //
 // The per-instance semantics are those of Subgraph::construct
 // (src/Subgraph.cpp:44-108); counters saturate exactly as
-// Edge_Frequency::add_edge (include/State_Config.hpp:207-217).  A pass whose
+// Edge_Frequency::add_edge (include/State_Config.hpp:207-217).  A record with
+// multiplicity m (deduplicated records) adds m to each counter it touches.  A pass whose
 // table fills past 7/8 flags the bucket as overflowed; the host rebuilds it on
 // the global path, so capacity choices never change results.
 #include "cf_instance.hpp"
@@ -138,14 +139,18 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
         // Each lane walks a stream of records r, r + LWG, ... and prefetches
         // the next one while walking the current one, so lanes stay busy
         // whatever the record lengths.
-        const uint64_t r_end = a.bucket_off[b + 1];
+        const uint64_t r_end = a.bucket_end ? a.bucket_end[b] : a.bucket_off[b + 1];
         uint64_t r = a.bucket_off[b] + tid;
         Walker rw;
-        uint32_t at_next = 0;
+        uint32_t at_next = 0, m = 1, m_next = 1;
         uint64_t lab_next[WMAX];
         if (r < r_end) {
             rw.load(a.att, a.labels, r, W, k);
-            if (r + LWG < r_end) Walker::fetch(a.att, a.labels, r + LWG, W, at_next, lab_next);
+            if (a.mult) m = a.mult[r];
+            if (r + LWG < r_end) {
+                Walker::fetch(a.att, a.labels, r + LWG, W, at_next, lab_next);
+                if (a.mult) m_next = a.mult[r + LWG];
+            }
         }
         uint32_t my_ins = 0;  // keys this lane inserted since its last flush into s_cnt
         while (r < r_end) {
@@ -160,7 +165,7 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                     // value only costs a CAS retry).
                     uint32_t old = pr.inserted ? 0u : s_ef[pr.slot];
                     while (true) {
-                        const uint32_t nw = ef_add(old, in.front, in.back);
+                        const uint32_t nw = ef_add(old, in.front, in.back, m);
                         if (nw == old) break;
                         const uint32_t prev = atomicCAS(&s_ef[pr.slot], old, nw);
                         if (prev == old) break;
@@ -179,7 +184,11 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                 if (r >= r_end) break;
                 if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
                 rw.init(at_next, lab_next, k);
-                if (r + LWG < r_end) Walker::fetch(a.att, a.labels, r + LWG, W, at_next, lab_next);
+                m = m_next;
+                if (r + LWG < r_end) {
+                    Walker::fetch(a.att, a.labels, r + LWG, W, at_next, lab_next);
+                    if (a.mult) m_next = a.mult[r + LWG];
+                }
             }
         }
         __syncthreads();

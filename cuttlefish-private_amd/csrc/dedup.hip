@@ -1,0 +1,231 @@
+// Exact weak super k-mer deduplication per bucket.
+//
+// Two records of one bucket with the same length, discontinuity bits and
+// label yield the same k-mer instances with the same neighbours and marks in
+// Subgraph::construct (src/Subgraph.cpp:57-105).  Counters saturate
+// (include/State_Config.hpp:207-217), so adding a record m times equals adding
+// min(15, count + m) once, and status bits are ORed.  Reads sequenced at
+// coverage c repeat most records ~c/2 times per strand, so construct walks the
+// distinct records with their multiplicities.
+//
+// One work item = one hash pass of one bucket: records whose key hash selects
+// the pass are counted in an LDS table (key = the label words with bases past
+// `len` cleared, then len | l_disc | r_disc as the top word), and the distinct
+// records are appended to the bucket's region of the output arrays.  A pass
+// whose table fills past 7/8 flags the bucket; the host then copies its
+// records unchanged (multiplicity 1), so results never depend on the sizing.
+#include "cf_kernels.hpp"
+
+namespace cfgpu {
+namespace {
+
+constexpr uint64_t EMPTY = ~0ull;
+constexpr uint64_t BUSY = 1ull << 63;
+
+// Find or insert a KR-word key (word KR-1 is the top word, < 2^63) in an LDS
+// table of C slots by 4-slot group probing; the claim protocol of the
+// construct tables (top word claimed with BUSY, lower words written, top word
+// published).  Returns the slot or -1 when every group is full.
+template <int KR, int C>
+__device__ __forceinline__ int dd_find_or_insert(uint64_t* s_key, const uint64_t (&key)[KR], uint64_t h,
+                                                 bool& inserted) {
+    constexpr uint32_t NG = uint32_t(C) / 4;
+    uint32_t g = uint32_t((uint64_t(uint32_t(h)) * NG) >> 32);
+    const uint64_t top = key[KR - 1];
+    inserted = false;
+    for (uint32_t n = 0; n < NG;) {
+        uint64_t cur[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            cur[j] = __hip_atomic_load(&s_key[(g * 4 + j) * KR + KR - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        int empty = 4;
+        bool busy = false;
+#pragma unroll
+        for (int j = 3; j >= 0; --j)
+            if (cur[j] == EMPTY) empty = j;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= empty) break;
+            if ((cur[j] & ~BUSY) != top) continue;
+            if (cur[j] & BUSY) {
+                busy = true;
+                continue;
+            }
+            const uint64_t* kp = &s_key[(g * 4 + j) * KR];
+            bool eq = true;
+#pragma unroll
+            for (int w = 0; w < KR - 1; ++w)
+                eq &= __hip_atomic_load(kp + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == key[w];
+            if (eq) return int(g * 4 + j);
+        }
+        if (busy) continue;
+        if (empty == 4) {
+            g = g + 1 == NG ? 0 : g + 1;
+            ++n;
+            continue;
+        }
+        const uint32_t slot = g * 4 + uint32_t(empty);
+        uint64_t* kp = &s_key[slot * KR];
+        if (atomicCAS(reinterpret_cast<unsigned long long*>(kp + KR - 1), EMPTY, top | BUSY) == EMPTY) {
+#pragma unroll
+            for (int w = 0; w < KR - 1; ++w)
+                __hip_atomic_store(kp + w, key[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __atomic_thread_fence(__ATOMIC_RELEASE);
+            __hip_atomic_store(kp + KR - 1, top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            inserted = true;
+            return int(slot);
+        }
+    }
+    return -1;
+}
+
+template <int W>
+__global__ __launch_bounds__(DD_WG) void k_dedup(DedupArgs a) {
+    constexpr int KR = W + 1;
+    constexpr int C = dedup_capacity(W);
+    static_assert(C % 64 == 0, "capacity");
+    __shared__ uint64_t s_key[C * KR];
+    __shared__ uint32_t s_cnt[C];
+    __shared__ uint32_t s_item, s_n, s_ovf, s_base;
+    __shared__ unsigned long long s_km;
+    const int tid = threadIdx.x;
+    constexpr uint32_t LIMIT = (uint32_t(C) * 7u) / 8u;
+
+    while (true) {
+        __syncthreads();
+        if (tid == 0) {
+            s_item = atomicAdd(a.queue_head, 1u);
+            s_n = 0;
+            s_ovf = 0;
+        }
+        __syncthreads();
+        const uint32_t item = s_item;
+        if (item >= a.n_items) break;
+        const uint32_t b = a.item_bucket[item], p = a.item_pass[item], P = a.item_passes[item];
+        if (tid == 0) s_ovf = __hip_atomic_load(&a.overflow[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int i = tid; i < C * KR; i += DD_WG) s_key[i] = EMPTY;
+        for (int i = tid; i < C; i += DD_WG) s_cnt[i] = 0;
+        __syncthreads();
+        if (s_ovf) continue;
+
+        const uint64_t r0 = a.bucket_off[b], r1 = a.bucket_off[b + 1];
+        uint32_t my_ins = 0, seen = 0;
+        for (uint64_t r = r0 + tid; r < r1; r += DD_WG) {
+            if ((++seen & 7u) == 0) {  // account inserts every 8 records
+                if (my_ins && atomicAdd(&s_n, my_ins) + my_ins > LIMIT) s_ovf = 1;
+                my_ins = 0;
+                if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+            }
+            const uint32_t at = a.att[r];
+            const uint32_t len = at & 0xFFu;
+            uint64_t key[KR];
+#pragma unroll
+            for (int q = 0; q < W; ++q) {  // word W-1-i holds bases 32i .. 32i+31
+                const int i = W - 1 - q;
+                const int nb = int(len) - 32 * i;
+                const uint64_t x = a.labels[r * W + q];
+                key[q] = nb >= 32 ? x : (nb <= 0 ? 0ull : x & (~0ull << (64 - 2 * nb)));
+            }
+            key[KR - 1] = at & 0x3FFu;
+            uint64_t h = key[0];
+#pragma unroll
+            for (int q = 1; q < KR; ++q) h = fmix64(h) ^ key[q];
+            h = fmix64(h);
+            if (uint32_t((uint64_t(uint32_t(h >> 32)) * P) >> 32) != p) continue;
+            bool inserted;
+            const int slot = dd_find_or_insert<KR, C>(s_key, key, h, inserted);
+            if (slot < 0) {
+                s_ovf = 1;
+                break;
+            }
+            atomicAdd(&s_cnt[slot], 1u);
+            my_ins += inserted;
+        }
+        if (my_ins && atomicAdd(&s_n, my_ins) + my_ins > LIMIT) s_ovf = 1;
+        __syncthreads();
+        if (s_ovf) {
+            if (tid == 0) atomicOr(&a.overflow[b], 1u);
+            continue;
+        }
+        // Append the pass's distinct records to the bucket's output region.
+        if (tid == 0) {
+            s_base = atomicAdd(&a.out_count[b], s_n);
+            s_n = 0;
+            s_km = 0;
+        }
+        __syncthreads();
+        unsigned long long my_km = 0;
+        const uint64_t base = r0 + s_base;
+        const uint32_t g16 = uint32_t(b) << 16;
+        for (int i = tid; i < C; i += DD_WG) {
+            const uint32_t c = s_cnt[i];
+            if (!c) continue;
+            const uint64_t o = base + atomicAdd(&s_n, 1u);
+            const uint64_t* kp = &s_key[i * KR];
+            my_km += (kp[KR - 1] & 0xFFu) - (a.k - 1);
+            a.out_att[o] = uint32_t(kp[KR - 1]) | g16;
+            a.out_mult[o] = c;
+#pragma unroll
+            for (int q = 0; q < W; ++q) a.out_labels[o * W + q] = kp[q];
+        }
+        if (my_km) atomicAdd(&s_km, my_km);
+        __syncthreads();
+        if (tid == 0 && s_km) atomicAdd(&a.out_kmers[b], s_km);
+    }
+}
+
+// Copy the records of flagged buckets unchanged (multiplicity 1).
+__global__ __launch_bounds__(256) void k_dedup_copy(DedupArgs a, const uint32_t* __restrict__ list, uint32_t W) {
+    const uint32_t b = list[blockIdx.x];
+    const uint64_t r0 = a.bucket_off[b], r1 = a.bucket_off[b + 1];
+    __shared__ unsigned long long s_km;
+    if (threadIdx.x == 0) s_km = 0;
+    __syncthreads();
+    unsigned long long km = 0;
+    for (uint64_t r = r0 + threadIdx.x; r < r1; r += 256) {
+        const uint32_t at = a.att[r];
+        a.out_att[r] = at;
+        a.out_mult[r] = 1u;
+        km += (at & 0xFFu) - (a.k - 1);
+        for (uint32_t q = 0; q < W; ++q) a.out_labels[r * W + q] = a.labels[r * W + q];
+    }
+    atomicAdd(&s_km, km);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a.out_count[b] = uint32_t(r1 - r0);
+        a.out_kmers[b] = s_km;
+    }
+}
+
+// bucket_end[b] = bucket_off[b] + count[b] for every b.
+__global__ __launch_bounds__(256) void k_dedup_end(const uint64_t* __restrict__ off, const uint32_t* __restrict__ cnt,
+                                                   uint32_t G, uint64_t* __restrict__ end) {
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b < G) end[b] = off[b] + cnt[b];
+}
+
+}  // namespace
+
+void launch_dedup(const DedupArgs& a, uint32_t W, uint32_t grid, hipStream_t s) {
+    if (!grid) return;
+    switch (W) {
+        case 1: hipLaunchKernelGGL(k_dedup<1>, dim3(grid), dim3(DD_WG), 0, s, a); break;
+        case 2: hipLaunchKernelGGL(k_dedup<2>, dim3(grid), dim3(DD_WG), 0, s, a); break;
+        case 3: hipLaunchKernelGGL(k_dedup<3>, dim3(grid), dim3(DD_WG), 0, s, a); break;
+        case 4: hipLaunchKernelGGL(k_dedup<4>, dim3(grid), dim3(DD_WG), 0, s, a); break;
+        case 5: hipLaunchKernelGGL(k_dedup<5>, dim3(grid), dim3(DD_WG), 0, s, a); break;
+        case 6: hipLaunchKernelGGL(k_dedup<6>, dim3(grid), dim3(DD_WG), 0, s, a); break;
+        case 7: hipLaunchKernelGGL(k_dedup<7>, dim3(grid), dim3(DD_WG), 0, s, a); break;
+        default: hipLaunchKernelGGL(k_dedup<8>, dim3(grid), dim3(DD_WG), 0, s, a); break;
+    }
+}
+
+void launch_dedup_copy(const DedupArgs& a, const uint32_t* list, uint32_t n, uint32_t W, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_dedup_copy, dim3(n), dim3(256), 0, s, a, list, W);
+}
+
+void launch_dedup_end(const uint64_t* off, const uint32_t* cnt, uint32_t G, uint64_t* end, hipStream_t s) {
+    hipLaunchKernelGGL(k_dedup_end, dim3((G + 255) / 256), dim3(256), 0, s, off, cnt, G, end);
+}
+
+}  // namespace cfgpu

@@ -30,7 +30,8 @@ class BucketStat(ctypes.Structure):
     _fields_ = [("n_records", ctypes.c_uint64), ("n_kmers", ctypes.c_uint64),
                 ("n_bases", ctypes.c_uint64), ("n_vertices", ctypes.c_uint64),
                 ("capacity", ctypes.c_uint64), ("predicted", ctypes.c_float),
-                ("flags", ctypes.c_uint32)]
+                ("flags", ctypes.c_uint32), ("n_records_walked", ctypes.c_uint64),
+                ("n_kmers_walked", ctypes.c_uint64)]
 
 
 class TableView(ctypes.Structure):
@@ -63,7 +64,8 @@ class Tuning(ctypes.Structure):
     _fields_ = [("load_factor", ctypes.c_float), ("workgroup", ctypes.c_uint32),
                 ("chunk_records", ctypes.c_uint32), ("wgs_per_cu", ctypes.c_uint32),
                 ("margin", ctypes.c_float), ("use_graph", ctypes.c_uint32),
-                ("max_passes", ctypes.c_uint32), ("predictor", ctypes.c_uint32)]
+                ("max_passes", ctypes.c_uint32), ("predictor", ctypes.c_uint32),
+                ("dedup", ctypes.c_uint32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -83,6 +83,8 @@ typedef struct cfgpu_bucket_stat {
     uint64_t capacity;    /* table slots (after construct) */
     float predicted;      /* MLP-predicted distinct vertices (learned mode) */
     uint32_t flags;       /* CFGPU_BUCKET_* */
+    uint64_t n_records_walked;  /* records the last construct walked (distinct after dedup) */
+    uint64_t n_kmers_walked;    /* their k-mer instances */
 } cfgpu_bucket_stat;
 enum {
     CFGPU_BUCKET_BUILT = 1,       /* table holds the bucket's final states */
@@ -168,6 +170,10 @@ typedef struct cfgpu_run_stats {
     double fallback_rate;             /* overflowed / predicted buckets */
     double holdout_err_mlp;           /* mean |log error| on held-out sample buckets: MLP */
     double holdout_err_const;         /*   ... and the constant mean-ratio predictor */
+    uint64_t n_records_walked;        /* records the construct kernels walked (distinct after dedup) */
+    uint32_t n_dedup_fallback;        /* buckets whose dedup tables overflowed (walked as they are) */
+    uint32_t reserved2;
+    double t_dedup_ms;                /* record dedup, incl. its probe round */
 } cfgpu_run_stats;
 int cfgpu_run_stats_get(cfgpu_handle* h, cfgpu_run_stats* out);
 
@@ -178,14 +184,18 @@ typedef struct cfgpu_tuning {
     uint32_t chunk_records;   /* records per work item (LDS tiles of `workgroup`) */
     uint32_t wgs_per_cu;      /* persistent grid = CUs * wgs_per_cu */
     float margin;             /* learned sizing: capacity = pred * margin / load_factor;
-                                 below 1 under-sizes on purpose (exercises the exact
-                                 overflow fallback; results do not change) */
+                                 below 1 under-sizes on purpose, tables and dedup passes
+                                 (exercises the exact overflow fallbacks; results do not
+                                 change) */
     uint32_t use_graph;       /* 1: batched launches are captured and replayed as a hipGraph */
     uint32_t max_passes;      /* batched paths: buckets needing at most this many LDS hash
                                  passes are built in LDS, larger ones with global atomics
                                  (0: always global) */
     uint32_t predictor;       /* learned sizing: 0 chooses on held-out sample buckets,
                                  1 always the MLP, 2 always the constant mean ratio */
+    uint32_t dedup;           /* batched paths: walk each bucket's distinct records once with
+                                 their multiplicity.  0 off, 1 when sampled buckets repeat
+                                 records at least 1.33x, 2 always (results never change) */
 } cfgpu_tuning;
 int cfgpu_tuning_get(cfgpu_handle* h, cfgpu_tuning* out);
 int cfgpu_tuning_set(cfgpu_handle* h, const cfgpu_tuning* in);

@@ -217,10 +217,14 @@ def test_lds_overflow_fallback_is_exact():
     say("[case] lds-overflow")
     with C.Handle(k, l, 1) as h:
         h.load_buckets(att, lab, off)
-        h.set_tuning(margin=0.2, max_passes=4096, predictor=2)
+        h.set_tuning(margin=0.2, max_passes=4096, predictor=2, dedup=2)
         h.construct("learned")
         rs = h.run_stats()
         assert rs["predictor"] == 2
+        # random labels never repeat, and passes sized at 20 % overflow: every
+        # bucket but the probes keeps its records through the copy fallback
+        say(f"[case] lds-overflow dedup fallback={rs['n_dedup_fallback']} walked={rs['n_records_walked']}")
+        assert rs["n_dedup_fallback"] > 0 and rs["n_records_walked"] == len(att)
         say(f"[case] lds-overflow overflowed={rs['n_overflowed']} lds={rs['n_lds_buckets']}")
         assert rs["n_lds_buckets"] > 0 and rs["n_overflowed"] > 0 and rs["n_fallback_rounds"] >= 1
         check_tables(h, None, k, l, 1, want)
@@ -249,6 +253,32 @@ def test_repeat_calls_are_idempotent_and_release():
         assert h.table_view(int(nz[0])).capacity == 0
         assert len(h.tables(buckets=[int(nz[0])])["g"]) == 0
         assert len(h.tables(buckets=[int(nz[1])])["g"]) > 0
+
+
+def test_record_dedup_is_exact():
+    # High-coverage reads repeat most weak super k-mers: the deduplicated walk
+    # (always / auto) must give the same tables as walking every record.
+    g = C.synth_genome(40000, 0.1, 300, 2000, 0.02, seed=71)
+    reads = C.synth_reads(g, 20000, 150, 0.002, seed=72)
+    buf = fastq(reads, 150)
+    k, l = 31, 20
+    seq, ro, part = oracle_records(buf, True, k, l)
+    inp = C.Input.from_buffer(buf, True, k)
+    say("[case] dedup")
+    with C.Handle(k, l, 2) as h:
+        h.partition(inp)
+        want = None
+        for dd in (0, 2, 1):
+            h.set_tuning(dedup=dd)
+            for mode in ("batched", "learned"):
+                say(f"[case] dedup={dd} mode={mode}")
+                h.construct(mode)
+                want = check_tables(h, part, k, l, 2, want)
+                rs = h.run_stats()
+                if dd == 0:
+                    assert rs["n_records_walked"] == rs["n_records"]
+                else:
+                    assert rs["n_records_walked"] < rs["n_records"] / 2, rs["n_records_walked"]
 
 
 def test_zipf_bucket_workload_loaded_records():
