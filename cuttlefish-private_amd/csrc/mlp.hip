@@ -168,18 +168,20 @@ __global__ __launch_bounds__(TW) void k_mlp_train(float* __restrict__ gparams,
             }
         }
         __syncthreads();
-        // dw3, db3: 16 lanes per unit, 8 samples each
+        // dw3: 16 lanes per unit, 8 samples each; db3 by the first wave
         {
             const int j = tid >> 4, part = tid & 15;
-            if (j <= H) {
-                float g = 0.f;
+            float g = 0.f;
 #pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    const int sm = part * 8 + c;
-                    g += dy[sm] * (j < H ? B[sm * H + j] : 1.f);
-                }
-                g = group_sum(g, 16);
-                if (part == 0) Gr[j < H ? OW3 + j : OB3] = g;
+            for (int c = 0; c < 8; ++c) {
+                const int sm = part * 8 + c;
+                g += dy[sm] * B[sm * H + j];
+            }
+            g = group_sum(g, 16);
+            if (part == 0) Gr[OW3 + j] = g;
+            if (tid < 64) {
+                const float gb = group_sum(dy[tid] + dy[tid + 64], 64);
+                if (tid == 0) Gr[OB3] = gb;
             }
         }
         __syncthreads();
