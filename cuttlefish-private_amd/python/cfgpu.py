@@ -54,7 +54,9 @@ class RunStats(ctypes.Structure):
                 + [(n, ctypes.c_uint32) for n in ("n_lds_buckets", "predictor")]
                 + [(n, ctypes.c_double) for n in ("mlp_loss", "mispredict_mean_abs_log",
                                                   "mispredict_under_frac", "fallback_rate",
-                                                  "holdout_err_mlp", "holdout_err_const")])
+                                                  "holdout_err_mlp", "holdout_err_const")]
+                + [("n_records_walked", ctypes.c_uint64), ("n_dedup_fallback", ctypes.c_uint32),
+                   ("reserved2", ctypes.c_uint32), ("t_dedup_ms", ctypes.c_double)])
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -109,3 +109,28 @@ def test_synth_is_seeded_and_thread_independent():
     assert 0.8 * 50000 < len(m) < 1.2 * 50000
     rep = C.synth_insert_repeats(g1[:40000], 200, 1000, 0.01, 5)
     assert len(rep) == 40000 + 200 * 1000
+
+
+def _c_struct_fields(name):
+    """Field names, in order, of `typedef struct name {...} name;` in include/cfgpu.h."""
+    import re
+    text = open(os.path.join(REPO, "include", "cfgpu.h")).read()
+    body = text[text.index("typedef struct " + name + " {"):]
+    body = body[body.index("{") + 1:body.index("} " + name + ";")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        _, names = decl.split(None, 1) if not decl.startswith(("const ", "struct ")) else decl.rsplit(None, 1)
+        fields += [n.strip().lstrip("*") for n in names.split(",")]
+    return fields
+
+
+@pytest.mark.parametrize("cname,pyname", [("cfgpu_run_stats", "RunStats"), ("cfgpu_bucket_stat", "BucketStat"),
+                                          ("cfgpu_tuning", "Tuning"), ("cfgpu_params", "Params"),
+                                          ("cfgpu_table_view", "TableView")])
+def test_ctypes_structs_mirror_the_header(cname, pyname):
+    py = [n for n, _ in getattr(C, pyname)._fields_]
+    assert py == _c_struct_fields(cname)
