@@ -91,7 +91,7 @@ __device__ __forceinline__ LdsProbe lds_find_or_insert(uint64_t* s_key, const Km
 #pragma unroll
                 for (int w = 0; w < KW - 1; ++w)
                     __hip_atomic_store(kp + w, key.w[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                __atomic_thread_fence(__ATOMIC_RELEASE);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // LDS table: the workgroup is the only observer
                 __hip_atomic_store(kp + KW - 1, top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
             return {int(slot), true};
