@@ -266,18 +266,28 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
     const int64_t q0 = int64_t(t0) - 1;
     const unsigned k = a.k, w = a.w;
 
-    // Change flags for my PPT contiguous positions.
+    // My PPT contiguous positions' g values (and the one before) in
+    // registers: gv[j + 1] is position base_idx + j, read with 128-bit loads.
+    static_assert(PPT == 8, "register layout below");
+    typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
     const int base_idx = 1 + tid * PPT;
+    uint32_t gv[PPT + 1];
+    {
+        const u32x4* row = reinterpret_cast<const u32x4*>(s_g + tid * PPT);
+        const u32x4 r0 = row[0], r1 = row[1], r2 = row[2];
+        gv[0] = r0.x; gv[1] = r0.y; gv[2] = r0.z; gv[3] = r0.w;
+        gv[4] = r1.x; gv[5] = r1.y; gv[6] = r1.z; gv[7] = r1.w;
+        gv[8] = r2.x;
+    }
+    // Change flags for my positions.
     uint32_t chg = 0, last_chg = 0;
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
-        const int idx = base_idx + j;
-        const uint32_t cur = s_g[idx];
+        const uint32_t cur = gv[j + 1], prv = gv[j];
         if (cur == G_INVALID) continue;
-        const uint32_t prv = s_g[idx - 1];
         if (prv == G_INVALID || ((prv ^ cur) & 0xFFFFu)) {
             chg |= 1u << j;
-            last_chg = uint32_t(idx);
+            last_chg = uint32_t(base_idx + j);
         }
     }
     const uint32_t incl = block_incl_max(last_chg, s_wave);
@@ -299,7 +309,7 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
     // min(next boundary, idx + w).
     uint32_t bnd = chg;
 #pragma unroll
-    for (int j = 0; j < PPT; ++j) bnd |= uint32_t(s_g[base_idx + j] == G_INVALID) << j;
+    for (int j = 0; j < PPT; ++j) bnd |= uint32_t(gv[j + 1] == G_INVALID) << j;
     constexpr uint32_t NONE = 0xFFFFFFFFu;
     uint32_t tail = NONE;  // first boundary in (TILE, TILE + w + 1]
     if (tid < 64) {
@@ -326,7 +336,7 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
     // valid position after an invalid one always starts a run (it has a change
     // flag), so only the thread's first position can need a division.
     uint32_t ph0 = 0;
-    if (rs0 >= 0 && !(chg & 1u) && s_g[base_idx] != G_INVALID)
+    if (rs0 >= 0 && !(chg & 1u) && gv[1] != G_INVALID)
         ph0 = uint32_t(uint64_t(q0 + base_idx - rs0) % w);
     uint32_t o = 0, n_tile = 0;
     if constexpr (COLLECT) {
@@ -336,25 +346,25 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
         for (int j = 0; j < PPT; ++j) {
             if (j) ph = ph + 1 == w ? 0u : ph + 1;
             if ((chg >> j) & 1u) ph = 0;
-            n_mine += ph == 0 && s_g[base_idx + j] != G_INVALID;
+            n_mine += ph == 0 && gv[j + 1] != G_INVALID;
         }
         o = block_excl_sum<WG>(n_mine, s_wave, &n_tile);
     }
     {
         uint32_t ph = ph0;
-#pragma unroll 1
+#pragma unroll
         for (int j = 0; j < PPT; ++j) {
             if (j) ph = ph + 1 == w ? 0u : ph + 1;
             if ((chg >> j) & 1u) ph = 0;
             const int idx = base_idx + j;
-            const uint32_t cur = s_g[idx];
+            const uint32_t cur = gv[j + 1];
             if (cur == G_INVALID || ph) continue;
             // Piece [idx, e): (k-1)-mers with this g, at most w of them.
             const uint32_t g = cur & 0xFFFFu;
             const uint32_t later = bnd >> (j + 1);  // my boundaries after position j
             const uint32_t nb = later ? uint32_t(idx) + uint32_t(__ffs(int(later))) : next_bnd;
             const int e = int(min(nb, uint32_t(idx) + w));
-            const uint32_t nv = s_g[e], pv = s_g[idx - 1];
+            const uint32_t nv = s_g[e], pv = gv[j];
             const uint32_t rj = nv != G_INVALID, lj = pv != G_INVALID;
             const uint32_t l_disc = lj && ((pv & 0xFFFFu) != g);
             const uint32_t r_disc = rj && ((nv & 0xFFFFu) != g);
@@ -409,7 +419,7 @@ __device__ __forceinline__ Span span_of(uint64_t n_bases, uint64_t per) {
 __global__ __launch_bounds__(H_WG) void k_pieces_hist(PartitionArgs a, const uint32_t* __restrict__ gbin,
                                                       const uint64_t* __restrict__ tile_lc, uint64_t per,
                                                       uint32_t* __restrict__ hist) {
-    __shared__ uint32_t s_g[P_TILE + WIN_MAX + 2];
+    __shared__ __attribute__((aligned(16))) uint32_t s_g[P_TILE + WIN_MAX + 2];
     __shared__ unsigned long long s_carry;
     __shared__ uint32_t s_wave[H_WG / 64];
     __shared__ uint32_t s_hist[N_COARSE];
@@ -489,12 +499,12 @@ __global__ __launch_bounds__(N_COARSE) void k_coarse_scan(const uint64_t* __rest
 // hist[c][s * H_SPLIT].  A span is a contiguous part of the input, so each
 // run is written front to back by one workgroup.  Level-1 records are one
 // array of structures, W + 1 words each: att | mbin << 32, then the label.
-__global__ __launch_bounds__(E_WG) void k_pieces_emit(PartitionArgs a, const uint32_t* __restrict__ gbin,
+__global__ __launch_bounds__(E_WG, 4) void k_pieces_emit(PartitionArgs a, const uint32_t* __restrict__ gbin,
                                                       const uint64_t* __restrict__ tile_lc, uint64_t per,
                                                       uint32_t n_hist, const uint32_t* __restrict__ hist,
                                                       const uint64_t* __restrict__ coff, uint32_t W,
                                                       uint64_t* __restrict__ rec) {
-    __shared__ uint32_t s_g[E_TILE + WIN_MAX + 2];
+    __shared__ __attribute__((aligned(16))) uint32_t s_g[E_TILE + WIN_MAX + 2];
     __shared__ unsigned long long s_carry;
     __shared__ uint32_t s_wave[E_WG / 64];
     __shared__ uint64_t s_words[TILE_WORDS(E_TILE)];
