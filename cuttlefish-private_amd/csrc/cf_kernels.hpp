@@ -55,18 +55,8 @@ void launch_pieces_hist(const PartitionArgs& a, const uint32_t* gbin, const uint
 void launch_pieces_emit(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, const PieceSpans& ps,
                         const uint32_t* hist, const uint64_t* coff, uint32_t W, uint64_t* rec1, hipStream_t s);
 // Level 2: level-1 records into per-bucket order; boff[G+1] bucket offsets.
-// Coarse bin c is cut into slices of at most F_SLICE records (slices
-// [slice_first[c], slice_first[c+1]), each with its record range and bin);
-// fcnt holds 2^FINE_BITS counts per slice.
-constexpr int F_WG = 512;
-constexpr uint64_t F_SLICE = 32768;
-void launch_fine_count(const uint64_t* rec1, uint32_t W, const uint64_t* slice_begin, const uint64_t* slice_end,
-                       uint32_t n_slices, uint32_t* fcnt, hipStream_t s);
-void launch_fine_scan(uint32_t* fcnt, const uint32_t* slice_first, const uint64_t* coff, uint64_t* boff,
-                      hipStream_t s);
-void launch_fine_scatter(const uint64_t* rec1, uint32_t W, const uint64_t* slice_begin, const uint64_t* slice_end,
-                         const uint32_t* slice_bin, uint32_t n_slices, const uint32_t* fcnt, const uint64_t* coff,
-                         uint32_t* att, uint64_t* labels, uint8_t* mbin, hipStream_t s);
+void launch_bucket_fine(const uint64_t* rec1, const uint64_t* coff, uint32_t W, uint64_t* boff, uint32_t* att,
+                        uint64_t* labels, uint8_t* mbin, hipStream_t s);
 
 // Per-bucket statistics: [n_rec, n_kmer_instances, n_bases, 0] as u64 and the
 // MLP feature vector (N_FEAT floats).

@@ -91,8 +91,6 @@ struct cfgpu_handle {
     // partition state
     DevBuf packed, frag, gbin, tile_lc, boff, att, labels, mbin, stats, feat;
     DevBuf hist, ctot, coff, rec1;  // level-1 bucketing
-    DevBuf sl_begin, sl_end, sl_bin, sl_first, fcnt;  // level-2 slices
-    std::vector<uint64_t> h_coff;
     uint64_t n_bases = 0, n_frag = 0, n_records = 0;
     bool partitioned = false;
     std::vector<uint64_t> h_boff, h_stats;
@@ -898,10 +896,9 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
         launch_pieces_hist(a, h->gbin.as<uint32_t>(), h->tile_lc.as<uint64_t>(), ps, h->hist.as<uint32_t>(),
                            h->ctot.as<uint64_t>(), h->coff.as<uint64_t>(), h->stream);
     HIPCK(h, hipEventRecord(e2.e, h->stream));
-    h->h_coff.resize(N_COARSE + 1);
-    HIPCK(h, hipMemcpyAsync(h->h_coff.data(), h->coff.p, (N_COARSE + 1) * 8, hipMemcpyDeviceToHost, h->stream));
+    uint64_t n_rec = 0;
+    HIPCK(h, hipMemcpyAsync(&n_rec, h->coff.as<uint64_t>() + N_COARSE, 8, hipMemcpyDeviceToHost, h->stream));
     HIPCK(h, hipStreamSynchronize(h->stream));
-    const uint64_t n_rec = h->h_coff[N_COARSE];
     HIPCK(h, h->rec1.ensure(n_rec * (h->W + 1) * 8));
     HIPCK(h, h->att.ensure(n_rec * 4));
     HIPCK(h, h->labels.ensure(n_rec * h->W * 8));
@@ -910,40 +907,8 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     if (n_frag)
         launch_pieces_emit(a, h->gbin.as<uint32_t>(), h->tile_lc.as<uint64_t>(), ps, h->hist.as<uint32_t>(),
                            h->coff.as<uint64_t>(), h->W, h->rec1.as<uint64_t>(), h->stream);
-    {
-        // level-2 slices: each coarse bin cut into runs of at most F_SLICE records
-        std::vector<uint64_t> sb, se;
-        std::vector<uint32_t> sbin, sfirst(N_COARSE + 1, 0);
-        for (uint32_t c = 0; c < uint32_t(N_COARSE); ++c) {
-            sfirst[c] = uint32_t(sb.size());
-            for (uint64_t r = h->h_coff[c]; r < h->h_coff[c + 1]; r += F_SLICE) {
-                sb.push_back(r);
-                se.push_back(std::min<uint64_t>(r + F_SLICE, h->h_coff[c + 1]));
-                sbin.push_back(c);
-            }
-        }
-        sfirst[N_COARSE] = uint32_t(sb.size());
-        const size_t ns = std::max<size_t>(sb.size(), 1);
-        HIPCK(h, h->sl_begin.ensure(ns * 8));
-        HIPCK(h, h->sl_end.ensure(ns * 8));
-        HIPCK(h, h->sl_bin.ensure(ns * 4));
-        HIPCK(h, h->sl_first.ensure((N_COARSE + 1) * 4));
-        HIPCK(h, h->fcnt.ensure(ns * (size_t(1) << FINE_BITS) * 4));
-        if (!sb.empty()) {
-            HIPCK(h, hipMemcpyAsync(h->sl_begin.p, sb.data(), sb.size() * 8, hipMemcpyHostToDevice, h->stream));
-            HIPCK(h, hipMemcpyAsync(h->sl_end.p, se.data(), se.size() * 8, hipMemcpyHostToDevice, h->stream));
-            HIPCK(h, hipMemcpyAsync(h->sl_bin.p, sbin.data(), sbin.size() * 4, hipMemcpyHostToDevice, h->stream));
-        }
-        HIPCK(h, hipMemcpyAsync(h->sl_first.p, sfirst.data(), sfirst.size() * 4, hipMemcpyHostToDevice, h->stream));
-        const uint32_t n_sl = uint32_t(sb.size());
-        launch_fine_count(h->rec1.as<uint64_t>(), h->W, h->sl_begin.as<uint64_t>(), h->sl_end.as<uint64_t>(), n_sl,
-                          h->fcnt.as<uint32_t>(), h->stream);
-        launch_fine_scan(h->fcnt.as<uint32_t>(), h->sl_first.as<uint32_t>(), h->coff.as<uint64_t>(),
-                         h->boff.as<uint64_t>(), h->stream);
-        launch_fine_scatter(h->rec1.as<uint64_t>(), h->W, h->sl_begin.as<uint64_t>(), h->sl_end.as<uint64_t>(),
-                            h->sl_bin.as<uint32_t>(), n_sl, h->fcnt.as<uint32_t>(), h->coff.as<uint64_t>(),
-                            h->att.as<uint32_t>(), h->labels.as<uint64_t>(), h->mbin.as<uint8_t>(), h->stream);
-    }
+    launch_bucket_fine(h->rec1.as<uint64_t>(), h->coff.as<uint64_t>(), h->W, h->boff.as<uint64_t>(),
+                       h->att.as<uint32_t>(), h->labels.as<uint64_t>(), h->mbin.as<uint8_t>(), h->stream);
     HIPCK(h, hipEventRecord(e4.e, h->stream));
     launch_bucket_features(h->boff.as<uint64_t>(), h->att.as<uint32_t>(), h->mbin.as<uint8_t>(), G, k,
                            h->stats.as<uint64_t>(), h->feat.as<float>(), h->stream);

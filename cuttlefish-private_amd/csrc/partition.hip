@@ -566,72 +566,35 @@ __global__ __launch_bounds__(E_WG, 4) void k_pieces_emit(PartitionArgs a, const 
     }
 }
 
-// Level 2 moves each coarse bin's records into its 2^FINE_BITS buckets.  A
-// coarse bin is cut into slices of F_SLICE records (slice list built by the
-// host); pass A counts fine digits per slice, a scan turns them into each
-// slice's start in every bucket, and pass B writes the records: every slice
-// appends to 2^FINE_BITS runs, each written front to back by one workgroup.
-__global__ __launch_bounds__(F_WG) void k_fine_count(const uint64_t* __restrict__ rec, uint32_t W,
-                                                     const uint64_t* __restrict__ slice_begin,
-                                                     const uint64_t* __restrict__ slice_end,
-                                                     uint32_t* __restrict__ fcnt) {
+// Level 2: one workgroup per coarse bin moves its records into the final
+// per-g order (the bin's 2^FINE_BITS buckets) and writes their offsets.
+// Records of one bucket are written in increasing slots by one workgroup.
+__global__ __launch_bounds__(1024) void k_bucket_fine(const uint64_t* __restrict__ rec,
+                                                      const uint64_t* __restrict__ coff, uint32_t W,
+                                                      uint64_t* __restrict__ boff, uint32_t* __restrict__ att,
+                                                      uint64_t* __restrict__ labels, uint8_t* __restrict__ mbin) {
     constexpr uint32_t NF = 1u << FINE_BITS;
     __shared__ uint32_t s_cnt[NF];
-    const uint32_t sl = blockIdx.x;
-    for (uint32_t i = threadIdx.x; i < NF; i += F_WG) s_cnt[i] = 0;
-    __syncthreads();
-    const uint64_t RW = W + 1;
-    for (uint64_t r = slice_begin[sl] + threadIdx.x; r < slice_end[sl]; r += F_WG)
-        atomicAdd(&s_cnt[(uint32_t(rec[r * RW]) >> 16) & (NF - 1)], 1u);
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < NF; i += F_WG) fcnt[uint64_t(sl) * NF + i] = s_cnt[i];
-}
-
-// Per coarse bin c: slices [slice_first[c], slice_first[c+1]).  Turns fcnt
-// into each slice's first slot per bucket and writes the bucket offsets.
-__global__ __launch_bounds__(64) void k_fine_scan(uint32_t* __restrict__ fcnt,
-                                                  const uint32_t* __restrict__ slice_first,
-                                                  const uint64_t* __restrict__ coff,
-                                                  uint64_t* __restrict__ boff) {
-    constexpr uint32_t NF = 1u << FINE_BITS;
-    static_assert(NF == 64, "one lane per fine bucket");
-    const uint32_t c = blockIdx.x, f = threadIdx.x;
-    const uint32_t s0 = slice_first[c], s1 = slice_first[c + 1];
-    uint32_t tot = 0;
-    for (uint32_t sl = s0; sl < s1; ++sl) tot += fcnt[uint64_t(sl) * NF + f];
-    // exclusive scan of the bucket totals across the wave
-    uint32_t x = tot;
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t u = __shfl_up(x, d);
-        if (int(f) >= d) x += u;
-    }
-    const uint64_t start = coff[c] + (x - tot);
-    boff[uint64_t(c) * NF + f] = start;
-    if (c + 1 == gridDim.x && f == NF - 1) boff[uint64_t(c + 1) * NF] = coff[c + 1];
-    uint64_t run = start;
-    for (uint32_t sl = s0; sl < s1; ++sl) {
-        const uint32_t n = fcnt[uint64_t(sl) * NF + f];
-        fcnt[uint64_t(sl) * NF + f] = uint32_t(run - coff[c]);  // offset within the coarse bin
-        run += n;
-    }
-}
-
-__global__ __launch_bounds__(F_WG) void k_fine_scatter(const uint64_t* __restrict__ rec, uint32_t W,
-                                                       const uint64_t* __restrict__ slice_begin,
-                                                       const uint64_t* __restrict__ slice_end,
-                                                       const uint32_t* __restrict__ slice_bin,
-                                                       const uint32_t* __restrict__ fcnt,
-                                                       const uint64_t* __restrict__ coff,
-                                                       uint32_t* __restrict__ att, uint64_t* __restrict__ labels,
-                                                       uint8_t* __restrict__ mbin) {
-    constexpr uint32_t NF = 1u << FINE_BITS;
     __shared__ unsigned long long s_cur[NF];
-    const uint32_t sl = blockIdx.x;
-    const uint64_t base = coff[slice_bin[sl]];
-    for (uint32_t i = threadIdx.x; i < NF; i += F_WG) s_cur[i] = base + fcnt[uint64_t(sl) * NF + i];
-    __syncthreads();
+    const uint32_t c = blockIdx.x;
+    const int tid = threadIdx.x;
+    const uint64_t r0 = coff[c], r1 = coff[c + 1];
     const uint64_t RW = W + 1;
-    for (uint64_t r = slice_begin[sl] + threadIdx.x; r < slice_end[sl]; r += F_WG) {
+    for (uint32_t i = tid; i < NF; i += 1024) s_cnt[i] = 0;
+    __syncthreads();
+    for (uint64_t r = r0 + tid; r < r1; r += 1024) atomicAdd(&s_cnt[(uint32_t(rec[r * RW]) >> 16) & (NF - 1)], 1u);
+    __syncthreads();
+    if (tid == 0) {
+        uint64_t run = r0;
+        for (uint32_t f = 0; f < NF; ++f) {
+            boff[c * NF + f] = run;
+            s_cur[f] = run;
+            run += s_cnt[f];
+        }
+        if (c + 1 == gridDim.x) boff[(c + 1) * NF] = r1;
+    }
+    __syncthreads();
+    for (uint64_t r = r0 + tid; r < r1; r += 1024) {
         const uint64_t h = rec[r * RW];
         const uint32_t at = uint32_t(h);
         const uint64_t slot = atomicAdd(&s_cur[(at >> 16) & (NF - 1)], 1ull);
@@ -717,23 +680,9 @@ void launch_pieces_emit(const PartitionArgs& a, const uint32_t* gbin, const uint
                        ps.n_count, hist, coff, W, rec1);
 }
 
-void launch_fine_count(const uint64_t* rec1, uint32_t W, const uint64_t* slice_begin, const uint64_t* slice_end,
-                       uint32_t n_slices, uint32_t* fcnt, hipStream_t s) {
-    if (n_slices)
-        hipLaunchKernelGGL(k_fine_count, dim3(n_slices), dim3(F_WG), 0, s, rec1, W, slice_begin, slice_end, fcnt);
-}
-
-void launch_fine_scan(uint32_t* fcnt, const uint32_t* slice_first, const uint64_t* coff, uint64_t* boff,
-                      hipStream_t s) {
-    hipLaunchKernelGGL(k_fine_scan, dim3(N_COARSE), dim3(64), 0, s, fcnt, slice_first, coff, boff);
-}
-
-void launch_fine_scatter(const uint64_t* rec1, uint32_t W, const uint64_t* slice_begin, const uint64_t* slice_end,
-                         const uint32_t* slice_bin, uint32_t n_slices, const uint32_t* fcnt, const uint64_t* coff,
-                         uint32_t* att, uint64_t* labels, uint8_t* mbin, hipStream_t s) {
-    if (n_slices)
-        hipLaunchKernelGGL(k_fine_scatter, dim3(n_slices), dim3(F_WG), 0, s, rec1, W, slice_begin, slice_end,
-                           slice_bin, fcnt, coff, att, labels, mbin);
+void launch_bucket_fine(const uint64_t* rec1, const uint64_t* coff, uint32_t W, uint64_t* boff, uint32_t* att,
+                        uint64_t* labels, uint8_t* mbin, hipStream_t s) {
+    hipLaunchKernelGGL(k_bucket_fine, dim3(N_COARSE), dim3(1024), 0, s, rec1, coff, W, boff, att, labels, mbin);
 }
 
 void launch_bucket_features(const uint64_t* off, const uint32_t* att, const uint8_t* mbin,
