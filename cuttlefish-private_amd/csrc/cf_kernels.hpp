@@ -41,6 +41,9 @@ constexpr int E_SPANS_PER_CU = 8;  // emit spans (workgroups) per CU
 constexpr int H_WG = 256;          // count workgroup, one minimizer tile per step
 static_assert((N_COARSE << FINE_BITS) == 16384, "coarse x fine bins cover the 16384 subgraphs");
 constexpr int H_SPLIT = 1;  // count spans per emit span
+// Level-1 record: header word (att | mbin << 32) and W label words, padded to
+// an even number of words so records are 16-byte aligned.
+__host__ __device__ constexpr uint32_t l1_words(uint32_t W) { return (W + 2) & ~1u; }
 struct PieceSpans {
     uint64_t per_count, per_emit;  // minimizer tiles (P_TILE positions) per count / emit span
     uint32_t n_count, n_emit;      // spans (workgroups)

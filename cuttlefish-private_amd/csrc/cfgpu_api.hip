@@ -899,7 +899,7 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     uint64_t n_rec = 0;
     HIPCK(h, hipMemcpyAsync(&n_rec, h->coff.as<uint64_t>() + N_COARSE, 8, hipMemcpyDeviceToHost, h->stream));
     HIPCK(h, hipStreamSynchronize(h->stream));
-    HIPCK(h, h->rec1.ensure(n_rec * (h->W + 1) * 8));
+    HIPCK(h, h->rec1.ensure(n_rec * l1_words(h->W) * 8));
     HIPCK(h, h->att.ensure(n_rec * 4));
     HIPCK(h, h->labels.ensure(n_rec * h->W * 8));
     HIPCK(h, h->mbin.ensure(n_rec));

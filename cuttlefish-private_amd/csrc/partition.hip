@@ -23,6 +23,9 @@
 namespace cfgpu {
 namespace {
 
+typedef __attribute__((ext_vector_type(2))) uint64_t u64x2;
+
+
 constexpr int WIN_MAX = 128;                      // w = k - l <= 126
 constexpr int MAXF = P_TILE / 4 + 8;              // fragments overlapping a tile (len >= 4)
 constexpr int NWORDS = (P_TILE + 2 * WIN_MAX) / 32 + 4;
@@ -498,7 +501,8 @@ __global__ __launch_bounds__(N_COARSE) void k_coarse_scan(const uint64_t* __rest
 // its records of coarse bin c to the run starting at coff[c] +
 // hist[c][s * H_SPLIT].  A span is a contiguous part of the input, so each
 // run is written front to back by one workgroup.  Level-1 records are one
-// array of structures, W + 1 words each: att | mbin << 32, then the label.
+// array of structures, l1_words(W) words each: att | mbin << 32, then the
+// label, padded to whole 16-byte pairs.
 __global__ __launch_bounds__(E_WG, 4) void k_pieces_emit(PartitionArgs a, const uint32_t* __restrict__ gbin,
                                                       const uint64_t* __restrict__ tile_lc, uint64_t per,
                                                       uint32_t n_hist, const uint32_t* __restrict__ hist,
@@ -550,18 +554,23 @@ __global__ __launch_bounds__(E_WG, 4) void k_pieces_emit(PartitionArgs a, const 
                                         [&](uint32_t g, uint32_t mb, uint32_t len, uint32_t l_disc,
                                             uint32_t r_disc, uint64_t start) {
                                             const uint64_t slot = atomicAdd(&s_cur[g >> FINE_BITS], 1ull);
-                                            uint64_t* out = rec + slot * (W + 1);
-                                            out[0] = uint64_t(len | (l_disc << 8) | (r_disc << 9) | (g << 16)) |
-                                                     (uint64_t(mb) << 32);
-                                            for (uint32_t wj = 0; wj < W; ++wj) {
+                                            // record = RW words (16-byte aligned): header, then
+                                            // the label words, written two words at a time
+                                            u64x2* out = reinterpret_cast<u64x2*>(rec + slot * l1_words(W));
+                                            uint64_t prev = uint64_t(len | (l_disc << 8) | (r_disc << 9) | (g << 16)) |
+                                                            (uint64_t(mb) << 32);
+                                            for (uint32_t q = 0; q < W; ++q) {
+                                                const uint32_t wj = W - 1 - q;  // label word q holds bases 32wj ..
                                                 uint64_t x = 0;
                                                 if (32 * wj < len) {
                                                     x = stream_get64(s_words, start - base0 + 32ull * wj);
                                                     const uint32_t nb = len - 32 * wj;
                                                     if (nb < 32) x &= ~0ull << (64 - 2 * nb);
                                                 }
-                                                out[1 + (W - 1 - wj)] = x;
+                                                if (q & 1) prev = x;
+                                                else out[q / 2] = u64x2{prev, x};
                                             }
+                                            if (!(W & 1)) out[W / 2] = u64x2{prev, 0ull};
                                         });
     }
 }
@@ -579,7 +588,7 @@ __global__ __launch_bounds__(1024) void k_bucket_fine(const uint64_t* __restrict
     const uint32_t c = blockIdx.x;
     const int tid = threadIdx.x;
     const uint64_t r0 = coff[c], r1 = coff[c + 1];
-    const uint64_t RW = W + 1;
+    const uint64_t RW = l1_words(W);
     for (uint32_t i = tid; i < NF; i += 1024) s_cnt[i] = 0;
     __syncthreads();
     for (uint64_t r = r0 + tid; r < r1; r += 1024) atomicAdd(&s_cnt[(uint32_t(rec[r * RW]) >> 16) & (NF - 1)], 1u);
@@ -595,12 +604,18 @@ __global__ __launch_bounds__(1024) void k_bucket_fine(const uint64_t* __restrict
     }
     __syncthreads();
     for (uint64_t r = r0 + tid; r < r1; r += 1024) {
-        const uint64_t h = rec[r * RW];
-        const uint32_t at = uint32_t(h);
+        const u64x2* in = reinterpret_cast<const u64x2*>(rec + r * RW);
+        const u64x2 p0 = in[0];
+        const uint32_t at = uint32_t(p0.x);
         const uint64_t slot = atomicAdd(&s_cur[(at >> 16) & (NF - 1)], 1ull);
         att[slot] = at;
-        mbin[slot] = uint8_t(h >> 32);
-        for (uint32_t wj = 0; wj < W; ++wj) labels[slot * W + wj] = rec[r * RW + 1 + wj];
+        mbin[slot] = uint8_t(p0.x >> 32);
+        labels[slot * W] = p0.y;
+        for (uint32_t q = 1; q < W; q += 2) {
+            const u64x2 pq = in[(q + 1) / 2];
+            labels[slot * W + q] = pq.x;
+            if (q + 1 < W) labels[slot * W + q + 1] = pq.y;
+        }
     }
 }
 
