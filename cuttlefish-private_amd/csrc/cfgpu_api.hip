@@ -443,7 +443,7 @@ void order_largest_first(std::vector<uint32_t>& v, const std::vector<double>& co
 // construct walks the original records.  Otherwise the other buckets get
 // passes sized from the measured distinct fraction, and buckets whose pass
 // tables overflowed keep their records unchanged (copied, multiplicity 1).
-constexpr double DEDUP_LOAD = 0.8;
+constexpr double DEDUP_LOAD = 0.85;
 constexpr double DEDUP_MIN_GAIN = 1.33;
 constexpr size_t DEDUP_PROBES = 64;
 
@@ -531,7 +531,7 @@ int dedup_buckets(cfgpu_handle* h, const std::vector<uint32_t>& all) {
         return CFGPU_OK;  // records barely repeat: walk them as they are
     }
     // margin < 1 under-sizes on purpose here too (exercises the copy fallback)
-    CK(dedup_round(h, rest, std::min(1.0, frac * 1.3) * std::min(1.0, double(h->tune.margin))));
+    CK(dedup_round(h, rest, std::min(1.0, frac * 1.1) * std::min(1.0, double(h->tune.margin))));
     HIPCK(h, hipMemcpyAsync(ovf.data(), h->dd_ovf.p, G * 4, hipMemcpyDeviceToHost, h->stream));
     HIPCK(h, hipStreamSynchronize(h->stream));
     std::vector<uint32_t> list;
