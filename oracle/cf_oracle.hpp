@@ -18,6 +18,9 @@
 //   construct_literal         — closed form (canonical (k+1)-mer counting with the
 //                               self-loop / homopolymer exceptions) and a
 //                               brute-force global graph over all (k+1)-mers.
+//   contract_subgraph         — lm-tigs glued across subgraphs plus trivial unitigs
+//                               equal the brute-force global maximal unitigs;
+//                               discontinuity-edge endpoints match the labels.
 #pragma once
 
 #include <array>
@@ -152,5 +155,24 @@ void construct_closed_form(unsigned k, unsigned W, uint32_t g, size_t n, const u
 
 // Sort helper: by (g, key).
 void sort_states(unsigned k, std::vector<VertexState>& v);
+
+// One unitig extracted by Subgraph::contract (src/Subgraph.cpp:184-270).
+enum : uint32_t { UNITIG_CYCLE = 1, UNITIG_LEFT_EXIT = 2, UNITIG_RIGHT_EXIT = 4 };
+struct Unitig {
+    std::string label;       // trivial: the reference's output orientation; lm-tig: v_hat's
+    uint32_t flags = 0;      // UNITIG_*; an exit flag marks an lm-tig end at a discontinuity
+    uint64_t n_vertices = 0;
+    Km u, v;                 // lm-tig: discontinuity-edge endpoints (left, right) ...
+    unsigned u_side = 2, v_side = 2;  // ... and their sides (0 front, 1 back, 2 phi)
+};
+struct ContractResult {
+    std::vector<Unitig> unitigs;
+    uint64_t isolated = 0, phantom = 0, trivial = 0, icc = 0, disc_edges = 0, vertex_count = 0, label_sz = 0;
+};
+// Subgraph::contract over one subgraph's vertex states (cf_oracle_contract.cpp).
+void contract_subgraph(unsigned k, unsigned f_th, const VertexState* v, size_t n, ContractResult& out);
+// Orients an lm-tig so that its first k characters are at most those of its
+// reverse complement (ends swapped accordingly); cycles are left as they are.
+void normalise_unitig(unsigned k, Unitig& u);
 
 }  // namespace cforacle

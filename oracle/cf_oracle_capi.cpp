@@ -222,4 +222,72 @@ int cfo_pipeline(const char* seq, const uint64_t* rec_off, size_t n_rec, int k, 
     return 0;
 }
 
+// Subgraph::contract over vertex states sorted by g (as cfo_construct returns
+// them).  Per bucket g, counts[7g..7g+6] = isolated, phantom, trivial unitigs,
+// cycles, lm-tigs (discontinuity edges), vertices in unitigs, label bytes.  Per
+// unitig: rec_g, rec_flags (1 cycle, 2 left exit, 4 right exit), rec_off /
+// rec_len into `labels`, ends[8i..8i+7] = the left and right discontinuity
+// vertices (4 words each, zero for phi), sides[2i..2i+1] (0 front, 1 back,
+// 2 phi).  `normalise` orients lm-tigs canonically.  Unitigs of a bucket are
+// listed in the order of their walks.  Returns the unitig count, or -1 when a
+// capacity is short (*lab_need then holds the label bytes needed).
+int64_t cfo_contract(int k, int f_th, uint32_t G, size_t n, const uint32_t* g, const uint64_t* keys,
+                     const uint32_t* e_f, const uint8_t* status, int normalise, int threads, uint64_t* counts,
+                     uint32_t* rec_g, uint32_t* rec_flags, uint64_t* rec_off, uint32_t* rec_len,
+                     uint64_t* ends, uint8_t* sides, size_t rec_cap, char* labels, size_t lab_cap,
+                     uint64_t* lab_need) {
+    std::vector<VertexState> v(n);
+    for (size_t i = 0; i < n; ++i) {
+        v[i].g = g[i];
+        std::memcpy(v[i].key.w.data(), keys + 4 * i, 32);
+        v[i].st.e_f = e_f[i];
+        v[i].st.status = status[i];
+    }
+    std::vector<size_t> beg(G + 1, n);
+    for (size_t i = n; i-- > 0;) beg[g[i]] = i;
+    for (uint32_t b = G; b-- > 0;) beg[b] = std::min(beg[b], beg[b + 1]);
+    std::vector<ContractResult> res(G);
+    if (threads < 1) threads = 1;
+    std::atomic<uint32_t> next{0};
+    auto work = [&]() {
+        while (true) {
+            const uint32_t b = next.fetch_add(1);
+            if (b >= G) break;
+            if (beg[b] == beg[b + 1]) continue;
+            contract_subgraph(k, f_th, v.data() + beg[b], beg[b + 1] - beg[b], res[b]);
+            if (normalise)
+                for (auto& u : res[b].unitigs) normalise_unitig(k, u);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < threads; ++t) th.emplace_back(work);
+    for (auto& x : th) x.join();
+    size_t nr = 0, nl = 0;
+    for (uint32_t b = 0; b < G; ++b) {
+        const auto& r = res[b];
+        const uint64_t c[7] = {r.isolated, r.phantom, r.trivial, r.icc, r.disc_edges, r.vertex_count, r.label_sz};
+        std::memcpy(counts + 7 * size_t(b), c, sizeof c);
+        nr += r.unitigs.size();
+        for (const auto& u : r.unitigs) nl += u.label.size();
+    }
+    *lab_need = nl;
+    if (nr > rec_cap || nl > lab_cap) return -1;
+    size_t i = 0, o = 0;
+    for (uint32_t b = 0; b < G; ++b)
+        for (const auto& u : res[b].unitigs) {
+            rec_g[i] = b;
+            rec_flags[i] = u.flags;
+            rec_off[i] = o;
+            rec_len[i] = uint32_t(u.label.size());
+            std::memcpy(labels + o, u.label.data(), u.label.size());
+            o += u.label.size();
+            std::memcpy(ends + 8 * i, (u.u_side == 2 ? Km{} : u.u).w.data(), 32);
+            std::memcpy(ends + 8 * i + 4, (u.v_side == 2 ? Km{} : u.v).w.data(), 32);
+            sides[2 * i] = uint8_t(u.u_side);
+            sides[2 * i + 1] = uint8_t(u.v_side);
+            ++i;
+        }
+    return int64_t(nr);
+}
+
 }  // extern "C"

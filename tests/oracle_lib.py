@@ -54,6 +54,10 @@ def lib():
                                    ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.cfo_edge_at.restype = ctypes.c_uint32
         L.cfo_edge_at.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
+        L.cfo_contract.restype = ctypes.c_int64
+        L.cfo_contract.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_size_t, u32p, u64p,
+                                   u32p, u8p, ctypes.c_int, ctypes.c_int, u64p, u32p, u32p, u64p, u32p,
+                                   u64p, u8p, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t, u64p]
         _LIB = L
     return _LIB
 
@@ -135,6 +139,42 @@ def construct(k, l, bucket_off, att, labels, G=16384, mode=0, threads=8):
     assert n >= 0, n
     return {"g": g[:n].copy(), "keys": keys[: 4 * n].reshape(n, 4).copy(), "e_f": ef[:n].copy(),
             "status": st[:n].copy()}
+
+
+CONTRACT_COUNTS = ("isolated", "phantom", "trivial", "icc", "disc_edges", "vertex_count", "label_sz")
+
+
+def contract(k, f_th, states, G=16384, normalise=True, threads=8):
+    """Subgraph::contract of every subgraph over `states` (as construct returns
+    them): per-bucket counters (G x 7, CONTRACT_COUNTS) and the unitigs: g,
+    flags (1 cycle, 2 left exit, 4 right exit), labels (str), ends (n x 8 key
+    words, left then right) and sides (n x 2; 0 front, 1 back, 2 phi)."""
+    L = lib()
+    n = len(states["g"])
+    g = np.ascontiguousarray(states["g"], dtype=np.uint32)
+    keys = np.ascontiguousarray(states["keys"], dtype=np.uint64).reshape(-1)
+    ef = np.ascontiguousarray(states["e_f"], dtype=np.uint32)
+    st = np.ascontiguousarray(states["status"], dtype=np.uint8)
+    counts = np.zeros(G * 7, dtype=np.uint64)
+    cap = max(16, n)
+    rg = np.zeros(cap, np.uint32)
+    rf = np.zeros(cap, np.uint32)
+    ro = np.zeros(cap, np.uint64)
+    rl = np.zeros(cap, np.uint32)
+    ends = np.zeros(cap * 8, np.uint64)
+    sides = np.zeros(cap * 2, np.uint8)
+    lab_cap = max(64, n * k)
+    labels = ctypes.create_string_buffer(lab_cap)
+    need = ctypes.c_uint64()
+    r = L.cfo_contract(k, f_th, G, n, _ptr(g, u32p), _ptr(keys, u64p), _ptr(ef, u32p), _ptr(st, u8p),
+                       int(normalise), threads, _ptr(counts, u64p), _ptr(rg, u32p), _ptr(rf, u32p),
+                       _ptr(ro, u64p), _ptr(rl, u32p), _ptr(ends, u64p), _ptr(sides, u8p), cap, labels,
+                       lab_cap, ctypes.byref(need))
+    assert r >= 0, (r, need.value)
+    raw = labels.raw
+    return {"counts": counts.reshape(G, 7), "g": rg[:r].copy(), "flags": rf[:r].copy(),
+            "labels": [raw[int(o):int(o) + int(ln)].decode() for o, ln in zip(ro[:r], rl[:r])],
+            "ends": ends[: 8 * r].reshape(r, 8).copy(), "sides": sides[: 2 * r].reshape(r, 2).copy()}
 
 
 def edge_at(e_f, side, f_th):

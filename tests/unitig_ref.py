@@ -1,7 +1,8 @@
 """Reference maximal unitigs for small inputs, computed from global brute-force
 side counts (every (k+1)-mer instance of every fragment, no partitioning), with
 the reference's walk (include/Subgraph.hpp:414-543) and output orientation
-(include/Unitig_Collator.hpp:306-400).  Test infrastructure."""
+(include/Unitig_Collator.hpp:306-400).  Vertices in no edge at the cutoff are
+not part of the graph and yield no unitig.  Test infrastructure."""
 import oracle_lib as O
 
 COMP = {"A": "T", "C": "G", "G": "C", "T": "A"}
@@ -53,7 +54,7 @@ def canonical_cycle(lab, k):
     for c in (cyc, rc(cyc)):
         for st in range(n):
             r = c[st:] + c[:st]
-            r += r[: k - 1]
+            r = (r * (1 + (k - 1 + n - 1) // n))[: n + k - 1]  # periodic: cycles shorter than k - 1 too
             if best is None or r[:k] < best[:k] or (r[:k] == best[:k] and r < best):
                 best = r
     return best
@@ -89,6 +90,8 @@ def unitigs(seq, ro, k, f_th):
     for v in order:
         if v in visited:
             continue
+        if not edges(cnt, v, 0, f_th) and not edges(cnt, v, 1, f_th):
+            continue  # isolated: in no edge at the cutoff, never output (src/Subgraph.cpp:206-214)
         visited.add(v)
         back, cyc = walk(v, 1)
         if cyc:
