@@ -180,4 +180,14 @@ __device__ __forceinline__ uint32_t ef_add(uint32_t e, uint8_t fb, uint8_t bb, u
     return e;
 }
 
+// Edge_Frequency::edge_at (include/State_Config.hpp:232-252) of the side whose
+// counters start at bit `off` (0 front, 16 back): E, the one base, or N.
+__device__ __forceinline__ uint32_t edge_at(uint32_t e_f, unsigned off, uint32_t f_th) {
+    uint32_t c = 0, which = 0;
+#pragma unroll
+    for (uint32_t bse = 0; bse < 4; ++bse)
+        if (((e_f >> (off + 4 * bse)) & 0xFu) >= f_th) ++c, which = bse;
+    return c == 0 ? BASE_E : (c == 1 ? which : BASE_N);
+}
+
 }  // namespace cfgpu

@@ -148,6 +148,49 @@ void launch_dedup_copy(const DedupArgs& a, const uint32_t* list, uint32_t n, uin
 void launch_dedup_end(const uint64_t* off, const uint32_t* cnt, uint32_t G, uint64_t* end, hipStream_t s);
 
 // ---------------------------------------------------------------------------
+// Subgraph::contract on the device tables (contract.hip).  Every built
+// subgraph's vertices in no edge at the cutoff are counted; the others go to
+// a per-subgraph table (LDS when they fit, else a per-workgroup global
+// scratch region), and every maximal unitig of the subgraph is written as an
+// ASCII label with a record.
+constexpr int CT_WG = 1024;
+// LDS table slots: 8*KW key bytes + 4 state bytes + 4 label-offset bytes each.
+constexpr int ct_lds_capacity(int KW) { return KW == 1 ? 9216 : KW == 2 ? 6144 : KW == 3 ? 4608 : 3648; }
+// Global scratch table slots for a subgraph of nv vertices (load <= 0.7).
+__host__ __device__ constexpr uint64_t ct_global_capacity(uint64_t nv) { return ((nv * 10 / 7 + 64) + 63) & ~63ull; }
+struct UnitigRec {             // layout of cfgpu_unitig
+    uint64_t label_off;
+    uint32_t label_len;
+    uint16_t g;
+    uint16_t flags;            // 1 cycle, 2 left end exits, 4 right end exits
+};
+enum : uint32_t { CT_ISOLATED, CT_PHANTOM, CT_TRIVIAL, CT_ICC, CT_DISC_EDGES, CT_VERTICES, CT_LABEL_BYTES, CT_FLAGS };
+struct ContractArgs {
+    const uint64_t* tbl_ptr;       // construct tables (entry_bytes(KW) per slot)
+    const uint64_t* tbl_cap;
+    const uint32_t* n_vert;        // per bucket distinct vertices (global scratch sizing)
+    const uint32_t* list;          // buckets for the LDS pass
+    uint32_t n_list;
+    uint32_t k, cutoff;
+    uint32_t* queue_head;          // [0] LDS pass, [1] global pass
+    uint32_t* ovf_list;            // buckets the LDS pass could not hold
+    uint32_t* ovf_count;
+    uint8_t* scratch;              // global pass: scratch_per_wg bytes per workgroup
+    uint64_t scratch_per_wg;
+    uint32_t* counts;              // per bucket 8 counters (CT_*), written by the pass that finishes it
+    UnitigRec* recs;
+    uint64_t rec_cap;
+    char* labels;
+    uint64_t lab_cap;
+    unsigned long long* cursors;   // [0] records, [1] label bytes (totals, even past capacity)
+    uint32_t* retry;               // set when the output capacity was short
+    uint32_t* error;               // set on an inconsistent table
+};
+// Two passes: LDS tables for every listed bucket that fits, then the global
+// scratch tables for the rest (their count is read on the device).
+void launch_contract(const ContractArgs& a, uint32_t KW, uint32_t grid_lds, uint32_t grid_global, hipStream_t s);
+
+// ---------------------------------------------------------------------------
 // FP32 MLP bucket-size predictor: N_FEAT -> H -> H -> 1 (ReLU), predicts
 // log(distinct vertices / k-mer instances) of a bucket.
 constexpr int MLP_H = 64;

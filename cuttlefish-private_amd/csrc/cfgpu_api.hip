@@ -118,6 +118,11 @@ struct cfgpu_handle {
     // export scratch
     DevBuf ex_keys, ex_ef, ex_st, ex_dfa, ex_off, ex_cursor, ex_list;
 
+    // contract: per-subgraph counters, unitig records and labels
+    DevBuf ct_list, ct_ovf, ct_counts, ct_ctl, ct_scratch, ct_recs, ct_labels;
+    uint64_t n_unitigs = 0, n_ubytes = 0;
+    bool contracted = false;
+
     // graph cache
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -848,7 +853,7 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
                                 (unsigned long long)f);
         }
     }
-    h->partitioned = h->constructed = false;
+    h->partitioned = h->constructed = h->contracted = false;
     const uint32_t G = h->G;
     const uint64_t n_words = (n_bases + 31) / 32 + 2;
     const uint64_t tiles = (n_bases + P_TILE - 1) / P_TILE;
@@ -994,7 +999,7 @@ int cfgpu_load_buckets(cfgpu_handle* h, const uint32_t* att, const uint64_t* lab
                             (unsigned long long)r, g);
         }
     }
-    h->partitioned = h->constructed = false;
+    h->partitioned = h->constructed = h->contracted = false;
     const uint64_t n = bucket_off[G];
     HIPCK(h, h->att.ensure(n * 4));
     HIPCK(h, h->labels.ensure(n * h->W * 8));
@@ -1033,8 +1038,137 @@ int cfgpu_construct(cfgpu_handle* h, uint32_t mode) {
     if (!h) return CFGPU_E_ARG;
     if (!h->partitioned) return fail(h, CFGPU_E_STATE, "construct before partition");
     HIPCK(h, hipSetDevice(h->prm.device));
-    h->constructed = false;
+    h->constructed = h->contracted = false;
     return construct_impl(h, mode);
+}
+
+static_assert(sizeof(cfgpu_unitig) == sizeof(UnitigRec) && sizeof(cfgpu_unitig) == 16, "cfgpu_unitig layout");
+static_assert(sizeof(cfgpu_contract_stat) == 32, "cfgpu_contract_stat layout");
+
+int cfgpu_contract(cfgpu_handle* h) {
+    if (!h) return CFGPU_E_ARG;
+    if (!h->constructed) return fail(h, CFGPU_E_STATE, "contract before construct");
+    HIPCK(h, hipSetDevice(h->prm.device));
+    h->contracted = false;
+    std::vector<uint32_t> list;
+    uint64_t sum_nv = 0;
+    uint32_t max_nv = 0;
+    for (uint32_t g = 0; g < h->G; ++g)
+        if (h->h_tbl_ptr[g] && h->h_nvert[g]) {
+            list.push_back(g);
+            sum_nv += h->h_nvert[g];
+            max_nv = std::max(max_nv, h->h_nvert[g]);
+        }
+    const uint32_t KW = h->KW, n_list = uint32_t(list.size());
+    const uint32_t grid = std::min<uint32_t>(n_list, uint32_t(h->cus));  // one 1024-thread workgroup per CU
+    const uint64_t per_wg = ct_global_capacity(max_nv) * uint64_t(8 * KW + 8);
+    HIPCK(h, h->ct_list.ensure(n_list * 4ull));
+    HIPCK(h, h->ct_ovf.ensure(n_list * 4ull));
+    HIPCK(h, h->ct_counts.ensure(size_t(h->G) * sizeof(cfgpu_contract_stat)));
+    HIPCK(h, h->ct_ctl.ensure(64));
+    HIPCK(h, h->ct_scratch.ensure(per_wg * std::max<uint32_t>(grid, 1)));
+    if (n_list) HIPCK(h, hipMemcpyAsync(h->ct_list.p, list.data(), n_list * 4ull, hipMemcpyHostToDevice, h->stream));
+    // output sized for short unitigs; re-run once at the exact size when short
+    uint64_t rec_cap = std::max<uint64_t>(4096, sum_nv / 2), lab_cap = std::max<uint64_t>(1 << 20, sum_nv * 8);
+    Event e0, e1;
+    HIPCK(h, hipEventRecord(e0.e, h->stream));
+    uint32_t ctl[16];
+    uint32_t run = 0;
+    for (;;) {
+        ++run;
+        HIPCK(h, h->ct_recs.ensure(rec_cap * sizeof(cfgpu_unitig)));
+        HIPCK(h, h->ct_labels.ensure(lab_cap));
+        launch_fill_u32(h->ct_counts.as<uint32_t>(), 0u, uint64_t(h->G) * 8, h->stream);
+        launch_fill_u32(h->ct_ctl.as<uint32_t>(), 0u, 16, h->stream);
+        ContractArgs a{};
+        a.tbl_ptr = h->d_tbl_ptr.as<uint64_t>();
+        a.tbl_cap = h->d_cap.as<uint64_t>();
+        a.n_vert = h->d_nvert.as<uint32_t>();
+        a.list = h->ct_list.as<uint32_t>();
+        a.n_list = n_list;
+        a.k = h->prm.k;
+        a.cutoff = h->prm.cutoff;
+        uint32_t* ctl_d = h->ct_ctl.as<uint32_t>();
+        a.queue_head = ctl_d;       // [0], [1]
+        a.ovf_count = ctl_d + 2;
+        a.retry = ctl_d + 3;
+        a.error = ctl_d + 4;
+        a.cursors = reinterpret_cast<unsigned long long*>(ctl_d + 8);  // [8..11]
+        a.ovf_list = h->ct_ovf.as<uint32_t>();
+        a.scratch = h->ct_scratch.as<uint8_t>();
+        a.scratch_per_wg = per_wg;
+        a.counts = h->ct_counts.as<uint32_t>();
+        a.recs = h->ct_recs.as<UnitigRec>();
+        a.rec_cap = rec_cap;
+        a.labels = h->ct_labels.as<char>();
+        a.lab_cap = lab_cap;
+        launch_contract(a, KW, grid, grid, h->stream);
+        HIPCK(h, hipGetLastError());
+        HIPCK(h, hipMemcpyAsync(ctl, h->ct_ctl.p, sizeof ctl, hipMemcpyDeviceToHost, h->stream));
+        HIPCK(h, hipStreamSynchronize(h->stream));
+        if (ctl[4]) return fail(h, CFGPU_E_STATE, "contract: inconsistent subgraph table");
+        uint64_t cur[2];
+        std::memcpy(cur, ctl + 8, sizeof cur);
+        if (!ctl[3]) {
+            h->n_unitigs = cur[0];
+            h->n_ubytes = cur[1];
+            h->rs.n_contract_global = ctl[2];
+            break;
+        }
+        if (run == 2) return fail(h, CFGPU_E_OVERFLOW, "contract: output still short after resizing");
+        rec_cap = cur[0];
+        lab_cap = cur[1];
+    }
+    HIPCK(h, hipEventRecord(e1.e, h->stream));
+    HIPCK(h, hipEventSynchronize(e1.e));
+    h->rs.t_contract_ms = elapsed_ms(e0, e1);
+    h->rs.n_unitigs = h->n_unitigs;
+    h->rs.n_unitig_bytes = h->n_ubytes;
+    h->rs.n_contract_runs = run;
+    h->contracted = true;
+    return CFGPU_OK;
+}
+
+int cfgpu_contract_stats(cfgpu_handle* h, cfgpu_contract_stat* out, uint32_t n) {
+    if (!h || (n && !out) || n > h->G) return CFGPU_E_ARG;
+    if (!h->contracted) return fail(h, CFGPU_E_STATE, "contract stats before contract");
+    HIPCK(h, hipSetDevice(h->prm.device));
+    if (n) {
+        HIPCK(h, hipMemcpyAsync(out, h->ct_counts.p, size_t(n) * sizeof(cfgpu_contract_stat), hipMemcpyDeviceToHost,
+                                h->stream));
+        HIPCK(h, hipStreamSynchronize(h->stream));
+    }
+    return CFGPU_OK;
+}
+
+int cfgpu_unitigs_view(cfgpu_handle* h, const cfgpu_unitig** recs, const char** labels, uint64_t* n_unitigs,
+                       uint64_t* n_bytes) {
+    if (!h) return CFGPU_E_ARG;
+    if (!h->contracted) return fail(h, CFGPU_E_STATE, "unitigs before contract");
+    if (recs) *recs = h->ct_recs.as<const cfgpu_unitig>();
+    if (labels) *labels = h->ct_labels.as<const char>();
+    if (n_unitigs) *n_unitigs = h->n_unitigs;
+    if (n_bytes) *n_bytes = h->n_ubytes;
+    return CFGPU_OK;
+}
+
+int cfgpu_unitigs_copy(cfgpu_handle* h, cfgpu_unitig* recs, uint64_t rec_cap, char* labels, uint64_t lab_cap,
+                       uint64_t* n_unitigs, uint64_t* n_bytes) {
+    if (!h) return CFGPU_E_ARG;
+    if (!h->contracted) return fail(h, CFGPU_E_STATE, "unitigs before contract");
+    if (n_unitigs) *n_unitigs = h->n_unitigs;
+    if (n_bytes) *n_bytes = h->n_ubytes;
+    if ((recs && rec_cap < h->n_unitigs) || (labels && lab_cap < h->n_ubytes))
+        return fail(h, CFGPU_E_ARG, "unitigs copy: capacity %llu / %llu < %llu / %llu", (unsigned long long)rec_cap,
+                    (unsigned long long)lab_cap, (unsigned long long)h->n_unitigs, (unsigned long long)h->n_ubytes);
+    HIPCK(h, hipSetDevice(h->prm.device));
+    if (recs && h->n_unitigs)
+        HIPCK(h, hipMemcpyAsync(recs, h->ct_recs.p, h->n_unitigs * sizeof(cfgpu_unitig), hipMemcpyDeviceToHost,
+                                h->stream));
+    if (labels && h->n_ubytes)
+        HIPCK(h, hipMemcpyAsync(labels, h->ct_labels.p, h->n_ubytes, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    return CFGPU_OK;
 }
 
 int cfgpu_table_view_get(cfgpu_handle* h, uint32_t g, cfgpu_table_view* out) {

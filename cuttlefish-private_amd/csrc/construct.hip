@@ -224,15 +224,6 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint32_t v, uint6
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
-// Edge_Frequency::edge_at (include/State_Config.hpp:232-252).
-__device__ __forceinline__ uint32_t edge_at(uint32_t e_f, unsigned off, uint32_t f_th) {
-    uint32_t c = 0, which = 0;
-#pragma unroll
-    for (uint32_t bse = 0; bse < 4; ++bse)
-        if (((e_f >> (off + 4 * bse)) & 0xFu) >= f_th) ++c, which = bse;
-    return c == 0 ? BASE_E : (c == 1 ? which : BASE_N);
-}
-
 template <int KW>
 __global__ __launch_bounds__(256) void k_export(const uint64_t* tbl_ptr,
                                                 const uint64_t* tbl_cap, const uint32_t* buckets,

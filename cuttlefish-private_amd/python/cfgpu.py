@@ -56,10 +56,26 @@ class RunStats(ctypes.Structure):
                                                   "mispredict_under_frac", "fallback_rate",
                                                   "holdout_err_mlp", "holdout_err_const")]
                 + [("n_records_walked", ctypes.c_uint64), ("n_dedup_fallback", ctypes.c_uint32),
-                   ("reserved2", ctypes.c_uint32), ("t_dedup_ms", ctypes.c_double)])
+                   ("reserved2", ctypes.c_uint32), ("t_dedup_ms", ctypes.c_double),
+                   ("t_contract_ms", ctypes.c_double), ("n_unitigs", ctypes.c_uint64),
+                   ("n_unitig_bytes", ctypes.c_uint64), ("n_contract_global", ctypes.c_uint32),
+                   ("n_contract_runs", ctypes.c_uint32)])
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class Unitig(ctypes.Structure):
+    _fields_ = [("label_off", ctypes.c_uint64), ("label_len", ctypes.c_uint32),
+                ("g", ctypes.c_uint16), ("flags", ctypes.c_uint16)]
+
+
+CONTRACT_STAT_FIELDS = ("isolated", "phantom", "trivial", "icc", "disc_edges", "vertices",
+                        "label_bytes", "flags")
+
+
+class ContractStat(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in CONTRACT_STAT_FIELDS]
 
 
 class Tuning(ctypes.Structure):
@@ -95,6 +111,13 @@ SIGNATURES = {
     "cfgpu_table_copy": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_uint32, u64p, u32p, u8p,
                                         u8p, u32p, ctypes.c_uint64, u64p]),
     "cfgpu_release": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "cfgpu_contract": (ctypes.c_int, [ctypes.c_void_p]),
+    "cfgpu_contract_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ContractStat),
+                                            ctypes.c_uint32]),
+    "cfgpu_unitigs_view": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                          ctypes.POINTER(ctypes.c_void_p), u64p, u64p]),
+    "cfgpu_unitigs_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Unitig), ctypes.c_uint64,
+                                          ctypes.c_char_p, ctypes.c_uint64, u64p, u64p]),
     "cfgpu_run_stats_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RunStats)]),
     "cfgpu_tuning_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Tuning)]),
     "cfgpu_tuning_set": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Tuning)]),
@@ -356,6 +379,32 @@ class Handle:
 
     def release(self, g):
         self._check(lib().cfgpu_release(self._h, g), "release")
+
+    def contract(self):
+        """Subgraph::contract of every built subgraph on the device."""
+        self._check(lib().cfgpu_contract(self._h), "contract")
+
+    def contract_stats(self):
+        """Per-subgraph counters (G x 8, CONTRACT_STAT_FIELDS order)."""
+        arr = np.zeros((G, 8), np.uint32)
+        self._check(lib().cfgpu_contract_stats(self._h, arr.ctypes.data_as(ctypes.POINTER(ContractStat)), G),
+                    "contract_stats")
+        return arr
+
+    def unitigs(self):
+        """Unitigs of the last contract: dict g, flags (1 cycle, 2 left exit, 4 right exit), labels (str)."""
+        n, nb = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(lib().cfgpu_unitigs_copy(self._h, None, 0, None, 0, ctypes.byref(n), ctypes.byref(nb)),
+                    "unitigs_copy")
+        recs = (Unitig * max(n.value, 1))()
+        buf = ctypes.create_string_buffer(max(nb.value, 1))
+        self._check(lib().cfgpu_unitigs_copy(self._h, recs, n.value, buf, nb.value, ctypes.byref(n),
+                                             ctypes.byref(nb)), "unitigs_copy")
+        raw = buf.raw
+        m = n.value
+        return {"g": np.array([recs[i].g for i in range(m)], np.uint32),
+                "flags": np.array([recs[i].flags for i in range(m)], np.uint32),
+                "labels": [raw[recs[i].label_off:recs[i].label_off + recs[i].label_len].decode() for i in range(m)]}
 
     def run_stats(self):
         s = RunStats()

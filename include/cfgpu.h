@@ -152,7 +152,47 @@ int cfgpu_table_copy(cfgpu_handle* h, const uint32_t* buckets, uint32_t n_bucket
  * src/Subgraphs_Manager.cpp:158, frees bucket files; here the view becomes invalid). */
 int cfgpu_release(cfgpu_handle* h, uint32_t g);
 
-/* Counters and timings of the last partition / construct. */
+/* Subgraph::contract (src/Subgraph.cpp:184-270; extract_maximal_unitig and
+ * walk_unitig, include/Subgraph.hpp:414-543) of every subgraph the last
+ * cfgpu_construct built and that was not released: the maximal unitigs of each
+ * subgraph's graph at the handle's cutoff, as ASCII labels on the device.
+ *   - a trivial maximal unitig (no end at a discontinuity) in the orientation
+ *     and rotation the reference prints it (Maximal_Unitig_Scratch::finalize,
+ *     include/Maximal_Unitig_Scratch.hpp:198-222; FASTA_Record.hpp:142-146);
+ *   - an lm-tig (an end exits through a discontinuity vertex): the label with
+ *     its first k characters at most its reverse complement's, and the flags
+ *     of the ends that exit.  Its discontinuity edge (Discontinuity_Graph::
+ *     add_edge, include/Discontinuity_Graph.hpp:144-152) joins the exiting ends:
+ *     the left end is the canonical form of the first k-mer, on its back side
+ *     if that k-mer is canonical, else its front; the right end is the last
+ *     k-mer's canonical form, on its front side if canonical, else its back;
+ *     an end that does not exit is phi.
+ * Vertices in no edge at the cutoff are counted (isolated; phantom when one
+ * side is discontinuous) and produce no unitig.  Unitig order is unspecified. */
+typedef struct cfgpu_unitig {
+    uint64_t label_off;   /* byte offset of the label */
+    uint32_t label_len;   /* vertices + k - 1 */
+    uint16_t g;           /* subgraph */
+    uint16_t flags;       /* CFGPU_UNITIG_* */
+} cfgpu_unitig;
+enum { CFGPU_UNITIG_CYCLE = 1, CFGPU_UNITIG_LEFT_EXIT = 2, CFGPU_UNITIG_RIGHT_EXIT = 4 };
+int cfgpu_contract(cfgpu_handle* h);
+/* Per-subgraph counters of the last contract (Subgraph's isolated, trivial_mtig_c,
+ * icc_count, disc_edge_c, the vertices in unitigs and the label bytes;
+ * include/Subgraph.hpp:178-186).  flags: 1 LDS table, 2 global table, 0 not contracted. */
+typedef struct cfgpu_contract_stat {
+    uint32_t isolated, phantom, trivial, icc, disc_edges, vertices, label_bytes, flags;
+} cfgpu_contract_stat;
+int cfgpu_contract_stats(cfgpu_handle* h, cfgpu_contract_stat* out, uint32_t n);
+/* Device pointers to the unitig records and labels (valid until the next
+ * cfgpu_partition / cfgpu_construct / cfgpu_contract or cfgpu_close). */
+int cfgpu_unitigs_view(cfgpu_handle* h, const cfgpu_unitig** recs, const char** labels,
+                       uint64_t* n_unitigs, uint64_t* n_bytes);
+/* Copies the records and labels to host arrays (label_off indexes `labels`). */
+int cfgpu_unitigs_copy(cfgpu_handle* h, cfgpu_unitig* recs, uint64_t rec_cap, char* labels,
+                       uint64_t lab_cap, uint64_t* n_unitigs, uint64_t* n_bytes);
+
+/* Counters and timings of the last partition / construct / contract. */
 typedef struct cfgpu_run_stats {
     uint64_t n_bases, n_fragments, n_records, n_kmers, n_edges, n_vertices;
     uint64_t table_slots, table_bytes;
@@ -174,6 +214,11 @@ typedef struct cfgpu_run_stats {
     uint32_t n_dedup_fallback;        /* buckets whose dedup tables overflowed (walked as they are) */
     uint32_t reserved2;
     double t_dedup_ms;                /* record dedup, incl. its probe round */
+    /* last cfgpu_contract */
+    double t_contract_ms;             /* both passes, incl. an output re-run when it was short */
+    uint64_t n_unitigs, n_unitig_bytes;
+    uint32_t n_contract_global;       /* subgraphs too large for an LDS table */
+    uint32_t n_contract_runs;         /* 1, or 2 after growing the output */
 } cfgpu_run_stats;
 int cfgpu_run_stats_get(cfgpu_handle* h, cfgpu_run_stats* out);
 

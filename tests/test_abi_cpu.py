@@ -130,7 +130,8 @@ def _c_struct_fields(name):
 
 @pytest.mark.parametrize("cname,pyname", [("cfgpu_run_stats", "RunStats"), ("cfgpu_bucket_stat", "BucketStat"),
                                           ("cfgpu_tuning", "Tuning"), ("cfgpu_params", "Params"),
-                                          ("cfgpu_table_view", "TableView")])
+                                          ("cfgpu_table_view", "TableView"), ("cfgpu_unitig", "Unitig"),
+                                          ("cfgpu_contract_stat", "ContractStat")])
 def test_ctypes_structs_mirror_the_header(cname, pyname):
     py = [n for n, _ in getattr(C, pyname)._fields_]
     assert py == _c_struct_fields(cname)

@@ -1,0 +1,117 @@
+"""GPU parity of Subgraph::contract on the device-built tables (cfgpu_contract)
+against the oracle's restatement (oracle/cf_oracle_contract.cpp), itself
+pinned against brute-force global unitigs in test_oracle_contract.py.
+
+Bar: bit-exact.  Per subgraph, the multiset of (label, flags) equals the
+oracle's (trivial unitigs in the reference's output orientation, lm-tigs
+oriented canonically, cycles rotated to their smallest vertex), and the
+counters (isolated, phantom, trivial, cycles, discontinuity edges, vertices in
+unitigs, label bytes) are equal."""
+import numpy as np
+import pytest
+
+import cfgpu as C
+import oracle_lib as O
+from test_gpu_parity import fastq, oracle_records, say
+from test_oracle import rand_seq, rc
+from test_oracle_contract import inputs as mixed_input
+
+pytestmark = pytest.mark.gpu
+
+
+def compare(name, got, stats, want):
+    np.testing.assert_array_equal(stats[:, :7].astype(np.uint64), want["counts"][:, :7])
+    built = stats[:, 7] != 0
+    assert not (stats[:, 7] & 4).any(), "inconsistent table flagged"
+    assert built[want["counts"][:, 5] + want["counts"][:, 0] > 0].all()
+    gl = sorted(zip(got["g"].tolist(), got["labels"], got["flags"].tolist()))
+    wl = sorted(zip(want["g"].tolist(), want["labels"], want["flags"].tolist()))
+    say(f"[case] {name}: unitigs={len(gl)} oracle={len(wl)}")
+    assert len(gl) == len(wl)
+    assert gl == wl
+
+
+def run(name, buf, fq, k, l, cutoff, modes=("learned",)):
+    say(f"[case] {name}: k={k} l={l} cutoff={cutoff} bytes={len(buf)}")
+    seq, ro, part = oracle_records(buf, fq, k, l)
+    off, att, lab = O.bucketize(part)
+    want = O.contract(k, cutoff, O.construct(k, l, off, att, lab), normalise=True)
+    inp = C.Input.from_buffer(buf, fq, k)
+    with C.Handle(k, l, cutoff) as h:
+        h.partition(inp)
+        for mode in modes:
+            say(f"[case] {name}: construct {mode} + contract")
+            h.construct(mode)
+            h.contract()
+            compare(name, h.unitigs(), h.contract_stats(), want)
+            rs = h.run_stats()
+            assert rs["n_unitigs"] == len(want["labels"])
+    return want
+
+
+@pytest.mark.parametrize("k,l,cutoff", [(31, 20, 1), (31, 20, 2), (15, 7, 1), (63, 20, 1)])
+def test_contract_mixed_small(k, l, cutoff):
+    rng = np.random.default_rng(7 + k + cutoff)
+    run("mixed", mixed_input(rng), False, k, l, cutoff, modes=("plain", "batched", "learned"))
+
+
+def test_contract_tandem_cycles():
+    rng = np.random.default_rng(21)
+    recs = [rand_seq(rng, p) * (1 + 100 // p) for p in range(2, 10) for _ in range(4)]
+    buf = "".join(f">c{i}\n{r}\n" for i, r in enumerate(recs)).encode()
+    for k, l in ((15, 7), (31, 20)):
+        want = run("cycles", buf, False, k, l, 1)
+        assert (want["flags"] & 1).sum() >= 20
+
+
+def test_contract_reads_cutoff2():
+    g = C.synth_genome(300_000, 0.2, 300, 3000, 0.02, seed=31)
+    reads = C.synth_reads(g, 60_000, 150, 0.005, seed=32)
+    run("reads", fastq(reads, 150), True, 31, 20, 2, modes=("batched", "learned"))
+
+
+def test_contract_reference_k127():
+    base = C.synth_genome(200_000, 0.1, 300, 2000, 0.02, seed=33)
+    recs = [base] + [C.synth_mutate(base, 0.005, 0.01, 10, seed=34 + i) for i in range(3)]
+    buf = b"".join(b">r%d\n%s\n" % (i, r) for i, r in enumerate(recs))
+    run("k127", buf, False, 127, 20, 1)
+
+
+def test_contract_large_subgraphs_use_global_tables():
+    # Zipf-sized buckets of random records: the largest hold far more vertices
+    # than an LDS table, so the global-scratch pass takes them.
+    for k, l in ((31, 20), (63, 20)):
+        att, lab, off = C.synth_bucket_workload(256, 60000, 1.1, k, l, seed=40 + k)
+        st = O.construct(k, l, off, (att & 0xFFFF).astype(np.uint16), lab)
+        want = O.contract(k, 1, st, normalise=True)
+        say(f"[case] zipf-contract k={k} vertices={len(st['g'])}")
+        with C.Handle(k, l, 1) as h:
+            h.load_buckets(att, lab, off)
+            h.construct("batched")
+            h.contract()
+            stats = h.contract_stats()
+            compare("zipf", h.unitigs(), stats, want)
+            assert (stats[:, 7] == 2).sum() > 0 and (stats[:, 7] == 1).sum() > 0
+            assert h.run_stats()["n_contract_global"] == (stats[:, 7] == 2).sum()
+
+
+def test_contract_after_release_skips_subgraph():
+    rng = np.random.default_rng(3)
+    buf = mixed_input(rng)
+    k, l = 31, 20
+    inp = C.Input.from_buffer(buf, False, k)
+    with C.Handle(k, l, 1) as h:
+        h.partition(inp)
+        h.construct("batched")
+        h.contract()
+        before = h.contract_stats()
+        g = int(np.nonzero(before[:, 5])[0][0])
+        h.release(g)
+        h.contract()
+        after = h.contract_stats()
+        assert after[g].sum() == 0
+        keep = np.ones(len(before), bool)
+        keep[g] = False
+        np.testing.assert_array_equal(after[keep], before[keep])
+        got = h.unitigs()
+        assert g not in set(got["g"].tolist())
