@@ -171,6 +171,23 @@ def main():
     pbytes = inp.n_bases / 4 + rs["n_records"] * (4 + 8 * (((2 * (args.k - 1) - args.l + 2) + 31) // 32))
     part_ms = rs["t_minimizer_ms"] + rs["t_count_ms"] + rs["t_emit_ms"]
 
+    # Subgraph::contract of every subgraph on the tables just built (the first
+    # call sizes its output buffers)
+    log("contract")
+    h.contract()
+    ct = []
+    for _ in range(3):
+        h.contract()
+        ct.append(h.run_stats()["t_contract_ms"])
+    rc = h.run_stats()
+    cst = h.contract_stats()
+    ct_ms = sorted(ct)[len(ct) // 2]
+    contract = {"ms": ct_ms, "unitigs": rc["n_unitigs"], "label_bytes": rc["n_unitig_bytes"],
+                "global_table_subgraphs": rc["n_contract_global"], "runs": rc["n_contract_runs"],
+                "isolated": int(cst[:, 0].sum()), "phantom": int(cst[:, 1].sum()),
+                "trivial": int(cst[:, 2].sum()), "cycles": int(cst[:, 3].sum()),
+                "lmtigs": int(cst[:, 4].sum()), "vertices_in_unitigs": int(cst[:, 5].sum())}
+
     # host buffers in (the packed reads cross PCIe inside the step)
     hip.sync()
     t0 = time.perf_counter()
@@ -245,6 +262,9 @@ def main():
             "batched_upper_bound": {"ms_per_step": ms_batched, "kmers_per_s": n_km / (ms_batched / 1000.0),
                                     "same_vertex_counts": same_batched},
             "speedup_vs_plain": ms_plain / ms,
+            "with_contract": {"ms_per_step": ms + ct_ms, "kmers_per_s": n_km / ((ms + ct_ms) / 1000.0),
+                              "note": "the step plus Subgraph::contract of every subgraph (unitig labels on "
+                                      "the device)"},
             "host_input": {"ms_per_step": ms_host, "kmers_per_s": n_km / (ms_host / 1000.0),
                            "note": "cfgpu_partition from host buffers: includes the H2D copy of the packed reads"},
         },
@@ -254,6 +274,7 @@ def main():
                       "train": rs["t_train_ms"], "infer": rs["t_infer_ms"],
                       "construct_main": rs["t_construct_main_ms"], "fallback": rs["t_fallback_ms"],
                       "dedup": rs["t_dedup_ms"]},
+        "contract": contract,
         "partition_roofline": {"achieved": pbytes / (part_ms / 1000.0) / 1e9 if part_ms else 0,
                                "unit": "GB/s", "algorithmic_bytes": pbytes},
         "learned": {"samples": rs["n_samples"], "mlp_loss": rs["mlp_loss"],

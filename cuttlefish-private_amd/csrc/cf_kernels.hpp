@@ -154,8 +154,9 @@ void launch_dedup_end(const uint64_t* off, const uint32_t* cnt, uint32_t G, uint
 // scratch region), and every maximal unitig of the subgraph is written as an
 // ASCII label with a record.
 constexpr int CT_WG = 1024;
-// LDS table slots: 8*KW key bytes + 4 state bytes + 4 label-offset bytes each.
-constexpr int ct_lds_capacity(int KW) { return KW == 1 ? 9216 : KW == 2 ? 6144 : KW == 3 ? 4608 : 3648; }
+// LDS table slots: 8*KW key bytes + 4 state bytes + 2 x 4 list-ranking bytes each
+// (a slot's two directed states are numbered below 2^14).
+constexpr int ct_lds_capacity(int KW) { return KW == 1 ? 7360 : KW == 2 ? 5248 : KW == 3 ? 4096 : 3328; }
 // Global scratch table slots for a subgraph of nv vertices (load <= 0.7).
 __host__ __device__ constexpr uint64_t ct_global_capacity(uint64_t nv) { return ((nv * 10 / 7 + 64) + 63) & ~63ull; }
 struct UnitigRec {             // layout of cfgpu_unitig

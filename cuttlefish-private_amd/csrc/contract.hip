@@ -17,12 +17,17 @@
 //   link    each side that continues to a neighbour is marked (one lookup per
 //           side; the edge back through the same side, a palindromic
 //           (k+1)-mer, ends the unitig there);
-//   paths   every side that does not continue is a path end; a thread walks
-//           from it to the other end, and the end whose orientation is
-//           canonical (first k-mer below the reverse complement's,
-//           include/Maximal_Unitig_Scratch.hpp:136-138, 198-217) writes it;
-//   cycles  what no path reached lies on cycles; each is walked once and
-//           written from its smallest vertex in canonical form
+//   paths   LDS tables (k_contract_rank): list ranking by pointer jumping over
+//           the directed states (vertex, exit side) gives every vertex both
+//           ends of its path and its distance to each, in log2(length) rounds;
+//           the path is written in the orientation whose first k-mer is below
+//           its reverse complement's (include/Maximal_Unitig_Scratch.hpp:
+//           136-138, 198-217), each vertex storing its own base.
+//           Global tables (k_contract_walk, subgraphs too large for LDS): a
+//           thread walks from every path end to the other, and the end whose
+//           orientation is canonical writes the path in a second walk;
+//   cycles  what no path holds lies on cycles; each is walked once and written
+//           from its smallest vertex in canonical form
 //           (Maximal_Unitig_Scratch::finalize, :213-219; FASTA_Record.hpp:142-146).
 //
 // Labels are ASCII in that orientation: a trivial unitig exactly as the
@@ -39,10 +44,11 @@ constexpr uint64_t EMPTY = ~0ull;
 // state word: bits 0-7 the DFA byte (edge_at front | back << 3 | front-disc << 6 | back-disc << 7)
 constexpr uint32_t M_LINK0 = 1u << 8;    // the front side continues the unitig (bit 9: the back side)
 constexpr uint32_t M_VIS = 1u << 10;     // on a path walked from one of its ends
-constexpr uint32_t M_WRITER = 1u << 11;  // writes its unitig (label offset in aux)
+constexpr uint32_t M_WRITER = 1u << 11;  // writes its unitig (label offset in aux / the ranking word)
 constexpr uint32_t M_CYCLE = 1u << 12;   // ... a cycle, from this vertex out of its back side
 constexpr uint32_t M_START1 = 1u << 13;  // ... a path whose end side at this vertex is the back
-constexpr int CT_CAND = 1024;
+constexpr uint32_t M_ONCYC = 1u << 14;   // lies on a cycle
+constexpr int CT_CAND = 512;
 
 template <int KW>
 struct CtTab {
@@ -81,9 +87,10 @@ __device__ __forceinline__ Kmer<KW> tab_key(const CtTab<KW>& t, uint32_t i) {
     return x;
 }
 
-// Unique keys: claim the top word, then fill the rest (read only after a barrier).
+// Unique keys: claim the top word, then fill the rest (read only after a
+// barrier).  Returns the slot, or -1 when the table is full.
 template <int KW>
-__device__ __forceinline__ bool tab_insert(const CtTab<KW>& t, const Kmer<KW>& x, uint32_t meta) {
+__device__ __forceinline__ int tab_insert(const CtTab<KW>& t, const Kmer<KW>& x, uint32_t meta) {
     uint32_t s = slot_of(km_hash(x), t.cap);
     for (uint32_t n = 0; n < t.cap; ++n) {
         uint64_t* kp = &t.key[size_t(s) * KW];
@@ -91,11 +98,11 @@ __device__ __forceinline__ bool tab_insert(const CtTab<KW>& t, const Kmer<KW>& x
 #pragma unroll
             for (int w = 0; w < KW - 1; ++w) kp[w] = x.w[w];
             t.meta[s] = meta;
-            return true;
+            return int(s);
         }
         s = s + 1 == t.cap ? 0 : s + 1;
     }
-    return false;
+    return -1;
 }
 
 template <int KW>
@@ -135,23 +142,420 @@ __device__ __forceinline__ int walk_step(const CtTab<KW>& t, uint32_t m, uint32_
     return tab_find(t, can ? z : zb);
 }
 
+// ASCII of base code b (A C G T).
+__device__ __forceinline__ uint32_t base_char(uint32_t b) { return (0x54474341u >> (8 * b)) & 0xFFu; }
+
+// z's k bases as ASCII at out: bytes up to the first 4-byte boundary, then
+// 4-byte stores, then the remaining bytes.
 template <int KW>
 __device__ __forceinline__ void put_kmer(char* out, const Kmer<KW>& z, uint32_t k) {
-    uint32_t p = 0;
+    const uint32_t head = (4u - uint32_t(reinterpret_cast<uintptr_t>(out) & 3u)) & 3u;
+    uint32_t p = 0, acc = 0;
 #pragma unroll
     for (int w = KW - 1; w >= 0; --w) {  // word KW-1 holds the first k - 32(KW-1) bases
         const uint32_t nb = w == KW - 1 ? k - 32 * (KW - 1) : 32u;
         const uint64_t v = z.w[w];
-        for (uint32_t q = 0; q < nb; ++q) out[p++] = "ACGT"[(v >> (2 * (nb - 1 - q))) & 3u];
+        for (uint32_t q = 0; q < nb; ++q, ++p) {
+            const uint32_t c = base_char(uint32_t(v >> (2 * (nb - 1 - q))) & 3u);
+            if (p < head) {
+                out[p] = char(c);
+                continue;
+            }
+            const uint32_t at = (p - head) & 3u;
+            acc = at ? acc | (c << (8 * at)) : c;
+            if (at == 3) *reinterpret_cast<uint32_t*>(out + p - 3) = acc;
+        }
+    }
+    const uint32_t tail = (k - head) & 3u;  // k > head: k >= 15
+    for (uint32_t j = 0; j < tail; ++j) out[k - tail + j] = char(acc >> (8 * j));
+}
+
+// Reads bucket b's construct table: vertices in no edge at the cutoff are
+// counted in s_cnt, the others inserted with their DFA byte (and their slot
+// appended to `list` when given).  Sets *s_ovf when more than `limit` do not
+// fit.  Several slots per thread are loaded before any is used, so each thread
+// keeps several HBM reads in flight.
+template <int KW>
+__device__ __forceinline__ void load_bucket(const ContractArgs& a, uint32_t b, const CtTab<KW>& t, uint32_t limit,
+                                            uint32_t* s_n, uint32_t* s_ovf, uint32_t* s_cnt, uint16_t* list) {
+    constexpr int U = KW <= 2 ? 8 : 4;
+    const uint64_t* tbl = reinterpret_cast<const uint64_t*>(a.tbl_ptr[b]);
+    const uint64_t tcap = a.tbl_cap[b];
+    uint32_t iso = 0, ph = 0;
+    for (uint64_t s0 = threadIdx.x; s0 < tcap; s0 += uint64_t(U) * CT_WG) {
+        uint64_t e[U][KW + 1];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t s = s0 + uint64_t(u) * CT_WG;
+            if (s < tcap) {
+#pragma unroll
+                for (int w = 0; w <= KW; ++w) e[u][w] = tbl[s * (KW + 1) + w];
+            } else {
+                e[u][KW - 1] = EMPTY;
+            }
+        }
+        bool stop = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (e[u][KW - 1] == EMPTY || stop) continue;
+            const uint32_t ef = uint32_t(e[u][KW]), st = uint32_t(e[u][KW] >> 32);
+            const uint32_t f = edge_at(ef, 0, a.cutoff), bk = edge_at(ef, 16, a.cutoff);
+            if (f == BASE_E && bk == BASE_E) {
+                ++iso;
+                ph += (st & 6u) != 0;
+                continue;
+            }
+            Kmer<KW> x;
+#pragma unroll
+            for (int w = 0; w < KW; ++w) x.w[w] = e[u][w];
+            const uint32_t m = f | (bk << 3) | ((st & 2u) << 5) | ((st & 4u) << 5);
+            const uint32_t d = atomicAdd(s_n, 1u);
+            const int slot = d < limit ? tab_insert(t, x, m) : -1;
+            if (slot < 0) {
+                *s_ovf = 1;
+                stop = true;
+            } else if (list) {
+                list[d] = uint16_t(slot);
+            }
+        }
+        if (stop || __hip_atomic_load(s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+    }
+    if (iso) atomicAdd(&s_cnt[CT_ISOLATED], iso);
+    if (ph) atomicAdd(&s_cnt[CT_PHANTOM], ph);
+}
+
+// Side s of slot i (key x, rc xr, state m): the directed state it continues to
+// as (slot << 1 | exit side there), or -1 when the unitig ends there.  Sets
+// *err on an inconsistent table.
+template <int KW>
+__device__ __forceinline__ int link_of(const CtTab<KW>& t, uint32_t i, uint32_t s, const Kmer<KW>& x,
+                                       const Kmer<KW>& xr, uint32_t m, uint32_t k, uint32_t* err) {
+    if (side_edge(m, s) >= 4) return -1;
+    Kmer<KW> z = s ? x : xr, zb = s ? xr : x;
+    uint32_t side = s, bb;
+    const int j = walk_step(t, m, k, z, zb, side, bb);
+    if (j < 0) {
+        *err = 1;
+        return -1;
+    }
+    const uint32_t ent = 1u - side;
+    const uint32_t ej = side_edge(ld_meta(&t.meta[j]), ent);
+    if (ej == BASE_E) *err = 1;                  // the edge is counted at both ends
+    if (ej >= 4) return -1;                      // branching entrance: crossed
+    if (uint32_t(j) == i && ent == s) return -1; // back through the same side
+    return int((uint32_t(j) << 1) | side);
+}
+
+// Thread 0: every cycle among the candidate slots (or all slots marked M_ONCYC
+// when the list overflowed), walked once from a vertex out of its back side;
+// the smallest vertex in canonical form writes it.  Its label offset goes to
+// *off_of(slot).
+template <int KW, class OffOf>
+__device__ void cycles_serial(const CtTab<KW>& t, uint32_t k, const uint32_t* cand, uint32_t ncand, uint32_t* s_bytes,
+                              uint32_t* s_nrec, uint32_t* s_cnt, uint32_t* s_err, OffOf off_of) {
+    const bool all = ncand > uint32_t(CT_CAND);
+    const uint32_t nc = all ? t.cap : ncand;
+    for (uint32_t q = 0; q < nc; ++q) {
+        const uint32_t c = all ? q : cand[q];
+        if (t.key[size_t(c) * KW + KW - 1] == EMPTY || !(t.meta[c] & M_ONCYC) || (t.meta[c] & (M_VIS | M_WRITER)))
+            continue;
+        Kmer<KW> z = tab_key(t, c), zb = km_rc(z, k), mn = z;
+        uint32_t side = 1, cur = c, n = 1, ms = c, bb;
+        t.meta[c] |= M_VIS;
+        for (uint32_t step = 0; step < t.cap; ++step) {
+            const uint32_t m = t.meta[cur];
+            if (!(m & (M_LINK0 << side))) {  // every side on a cycle continues
+                *s_err = 1;
+                break;
+            }
+            const int j = walk_step(t, m, k, z, zb, side, bb);
+            if (j < 0) {
+                *s_err = 1;
+                break;
+            }
+            if (uint32_t(j) == c) break;
+            t.meta[j] |= M_VIS;
+            Kmer<KW> key;  // canonical form
+#pragma unroll
+            for (int w = 0; w < KW; ++w) key.w[w] = side ? z.w[w] : zb.w[w];
+            if (km_less(key, mn)) mn = key, ms = uint32_t(j);
+            cur = uint32_t(j);
+            ++n;
+        }
+        const uint32_t bytes = n + k - 1;
+        *off_of(ms) = *s_bytes;
+        *s_bytes += bytes;
+        t.meta[ms] |= M_WRITER | M_CYCLE;
+        *s_nrec += 1;
+        s_cnt[CT_TRIVIAL] += 1;
+        s_cnt[CT_ICC] += 1;
+        s_cnt[CT_VERTICES] += n;
+        s_cnt[CT_LABEL_BYTES] += bytes;
     }
 }
 
-template <int KW, bool GLOBAL>
-__global__ __launch_bounds__(CT_WG) void k_contract(ContractArgs a) {
-    constexpr uint32_t CL = GLOBAL ? 1u : uint32_t(ct_lds_capacity(KW));
+// Writes the cycle whose smallest vertex is slot i (canonical, leaving through
+// its back side) at out[0 .. n + k - 1), never past `end`.
+template <int KW>
+__device__ __forceinline__ uint32_t write_cycle(const CtTab<KW>& t, uint32_t i, uint32_t k, char* out, uint64_t off,
+                                                uint64_t end, uint32_t* err) {
+    Kmer<KW> z = tab_key(t, i), zb = km_rc(z, k);
+    if (off + k > end) {
+        atomicOr(err, 1u);
+        return 0;
+    }
+    put_kmer(out, z, k);
+    uint32_t side = 1, cur = i, n = 1, bb;
+    for (uint32_t step = 0; step < t.cap; ++step) {
+        const uint32_t m = t.meta[cur];
+        if (!(m & (M_LINK0 << side))) break;
+        const int j = walk_step(t, m, k, z, zb, side, bb);
+        if (j < 0 || uint32_t(j) == i) break;
+        if (off + k + n > end) {
+            atomicOr(err, 1u);
+            break;
+        }
+        out[k - 1 + n] = char(base_char(bb));
+        cur = uint32_t(j);
+        ++n;
+    }
+    return n;
+}
+
+// list-ranking word: directed state (slot << 1 | exit side) in the low 14 bits,
+// steps to the state it points at in the high 18 bits
+constexpr uint32_t JP_BITS = 14, JP_MASK = (1u << JP_BITS) - 1, JP_DMAX = (1u << (32 - JP_BITS)) - 1;
+__device__ __forceinline__ uint32_t jp_pack(uint32_t p, uint32_t d) { return p | (min(d, JP_DMAX) << JP_BITS); }
+constexpr uint32_t RANK_ROUNDS = 16;  // > log2(2 * LDS capacity): every path converges
+static_assert(2 * ct_lds_capacity(1) <= (1 << JP_BITS), "states fit the ranking word");
+
+template <int KW>
+__global__ __launch_bounds__(CT_WG) void k_contract_rank(ContractArgs a) {
+    constexpr uint32_t CL = uint32_t(ct_lds_capacity(KW));
+    constexpr uint32_t NV = CL * 4 / 5;                // vertices an LDS table takes
+    constexpr uint32_t NQ = (NV + CT_WG - 1) / CT_WG;  // vertices per thread
     __shared__ uint64_t s_key[CL * KW];
+    __shared__ uint16_t s_list[NV];  // slots in insertion order
     __shared__ uint32_t s_meta[CL];
-    __shared__ uint32_t s_aux[CL];
+    __shared__ uint32_t s_jp[2 * CL];
+    __shared__ uint32_t s_cand[CT_CAND];
+    __shared__ uint32_t s_cnt[8];
+    __shared__ uint32_t s_chg[RANK_ROUNDS];
+    __shared__ uint32_t s_item, s_n, s_ovf, s_ncand, s_bytes, s_nrec, s_rctr, s_err, s_skip;
+    __shared__ unsigned long long s_brec, s_bbyte;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t k = a.k;
+    const uint32_t kt = k - 32 * (KW - 1);
+    CtTab<KW> t;
+    t.key = s_key;
+    t.meta = s_meta;
+    t.aux = nullptr;
+    t.cap = CL;
+
+    while (true) {
+        __syncthreads();
+        if (tid == 0) {
+            s_item = atomicAdd(&a.queue_head[0], 1u);
+            s_n = s_ovf = s_ncand = s_bytes = s_nrec = s_rctr = s_err = s_skip = 0;
+        }
+        if (tid < 8) s_cnt[tid] = 0;
+        if (tid < RANK_ROUNDS) s_chg[tid] = 0;
+        __syncthreads();
+        if (s_item >= a.n_list) break;
+        const uint32_t b = a.list[s_item];
+        for (uint32_t i = tid; i < CL; i += CT_WG) {
+            s_key[size_t(i) * KW + KW - 1] = EMPTY;
+            s_meta[i] = 0;
+        }
+        __syncthreads();
+        load_bucket(a, b, t, NV, &s_n, &s_ovf, s_cnt, s_list);
+        __syncthreads();
+        if (s_ovf) {
+            if (tid == 0) a.ovf_list[atomicAdd(a.ovf_count, 1u)] = b;
+            continue;
+        }
+
+        // this thread's vertices: list entries tid + q * CT_WG, bit q of occ
+        const uint32_t nv = s_n;
+        uint32_t occ = 0, slot[NQ];
+#pragma unroll
+        for (uint32_t q = 0; q < NQ; ++q) {
+            const uint32_t d = tid + q * CT_WG;
+            slot[q] = d < nv ? s_list[d] : 0u;
+            if (d < nv) occ |= 1u << q;
+        }
+
+        // link: each directed state points at its successor (1 step) or at itself (an end)
+#pragma unroll
+        for (uint32_t q = 0; q < NQ; ++q) {
+            if (!((occ >> q) & 1u)) continue;
+            const uint32_t i = slot[q];
+            const Kmer<KW> x = tab_key(t, i);
+            const Kmer<KW> xr = km_rc(x, k);
+            const uint32_t m = ld_meta(&s_meta[i]);
+            uint32_t add = 0;
+            for (uint32_t sd = 0; sd < 2; ++sd) {
+                const int nx = link_of(t, i, sd, x, xr, m, k, &s_err);
+                s_jp[2 * i + sd] = nx < 0 ? jp_pack(2 * i + sd, 0) : jp_pack(uint32_t(nx), 1);
+                if (nx >= 0) add |= M_LINK0 << sd;
+            }
+            if (add) atomicOr(&s_meta[i], add);
+        }
+        __syncthreads();
+
+        // rank: pointer jumping until no path state moves (cycle states never
+        // settle; they stop counting once their distance passes the vertex count)
+        for (uint32_t r = 0; r < RANK_ROUNDS; ++r) {
+            uint32_t nw[NQ][2];
+            bool any = false;
+#pragma unroll
+            for (uint32_t q = 0; q < NQ; ++q) {
+                const uint32_t i = slot[q];
+#pragma unroll
+                for (uint32_t sd = 0; sd < 2; ++sd) {
+                    nw[q][sd] = 0;
+                    if (!((occ >> q) & 1u)) continue;
+                    const uint32_t x = 2 * i + sd, w = s_jp[x];
+                    const uint32_t p = w & JP_MASK;
+                    nw[q][sd] = w;
+                    if (p == x) continue;
+                    const uint32_t w2 = s_jp[p];
+                    const uint32_t p2 = w2 & JP_MASK, nd = (w >> JP_BITS) + (w2 >> JP_BITS);
+                    nw[q][sd] = jp_pack(p2, nd);
+                    any |= p2 != p && nd < s_n;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t q = 0; q < NQ; ++q)
+                if ((occ >> q) & 1u) {
+                    const uint32_t i = slot[q];
+                    s_jp[2 * i] = nw[q][0];
+                    s_jp[2 * i + 1] = nw[q][1];
+                }
+            if (any) s_chg[r] = 1;
+            __syncthreads();
+            if (!s_chg[r]) break;
+        }
+
+        // per vertex: the ends of its path, the output orientation and its position
+        uint32_t info[NQ], wn[NQ], wfl[NQ];  // info: A | pos << 13 | fwd << 26 | path << 27
+#pragma unroll
+        for (uint32_t q = 0; q < NQ; ++q) {
+            info[q] = wn[q] = wfl[q] = 0;
+            if (!((occ >> q) & 1u)) continue;
+            const uint32_t i = slot[q];
+            const uint32_t wf = s_jp[2 * i], wb = s_jp[2 * i + 1];
+            const uint32_t Ff = wf & JP_MASK, Fb = wb & JP_MASK;
+            if (s_jp[Fb] != jp_pack(Fb, 0)) {  // not a path end: the vertex lies on a cycle
+                atomicOr(&s_meta[i], M_ONCYC);
+                const uint32_t c = atomicAdd(&s_ncand, 1u);
+                if (c < uint32_t(CT_CAND)) s_cand[c] = i;
+                continue;
+            }
+            const uint32_t df = wf >> JP_BITS, db = wb >> JP_BITS;
+            const uint32_t L = Ff >> 1, sL = Ff & 1u, R = Fb >> 1, sR = Fb & 1u;
+            const Kmer<KW> cL = tab_key(t, L), cR = tab_key(t, R);
+            // first k-mer of the path read with this vertex canonical, and the
+            // reverse complement of its last k-mer
+            const Kmer<KW> first = sL == 0 ? cL : km_rc(cL, k);
+            const Kmer<KW> rlast = sR == 1 ? km_rc(cR, k) : cR;
+            const bool fwd = km_less(first, rlast);
+            const uint32_t pos = fwd ? df : db;
+            const uint32_t A = fwd ? L : R;
+            info[q] = A | (pos << 13) | (uint32_t(fwd) << 26) | (1u << 27);
+            if (pos == 0) {
+                const uint32_t sA = fwd ? sL : sR, B = fwd ? R : L, sB = fwd ? sR : sL;
+                wn[q] = df + db + 1;
+                wfl[q] = (side_exits(s_meta[A], sA) ? 2u : 0u) | (side_exits(s_meta[B], sB) ? 4u : 0u);
+            }
+        }
+        __syncthreads();  // every ranking word read
+
+        // the first vertex of each path reserves its label (offset in its front word)
+        {
+            uint32_t nrec = 0, verts = 0, lbytes = 0, triv = 0, disc = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < NQ; ++q) {
+                if (!info[q] || ((info[q] >> 13) & 0x1FFFu)) continue;
+                const uint32_t i = slot[q];
+                const uint32_t bytes = wn[q] + k - 1;
+                s_jp[2 * i] = atomicAdd(&s_bytes, bytes);
+                ++nrec;
+                verts += wn[q];
+                lbytes += bytes;
+                disc += wfl[q] != 0;
+                triv += wfl[q] == 0;
+            }
+            if (nrec) {
+                atomicAdd(&s_nrec, nrec);
+                atomicAdd(&s_cnt[CT_VERTICES], verts);
+                atomicAdd(&s_cnt[CT_LABEL_BYTES], lbytes);
+                atomicAdd(&s_cnt[CT_TRIVIAL], triv);
+                atomicAdd(&s_cnt[CT_DISC_EDGES], disc);
+            }
+        }
+        __syncthreads();
+        if (tid == 0 && s_ncand)
+            cycles_serial(t, k, s_cand, s_ncand, &s_bytes, &s_nrec, s_cnt, &s_err,
+                          [&](uint32_t slot) { return &s_jp[2 * slot]; });
+        __syncthreads();
+
+        // reserve the bucket's output, publish its counters
+        if (tid == 0) {
+            s_brec = atomicAdd(&a.cursors[0], (unsigned long long)s_nrec);
+            s_bbyte = atomicAdd(&a.cursors[1], (unsigned long long)s_bytes);
+            s_skip = s_brec + s_nrec > a.rec_cap || s_bbyte + s_bytes > a.lab_cap;
+            if (s_skip) atomicOr(a.retry, 1u);
+            if (s_err) atomicOr(a.error, 1u);
+            s_cnt[CT_FLAGS] = 1u | (s_err ? 4u : 0u);
+        }
+        __syncthreads();
+        if (tid < 8) a.counts[size_t(b) * 8 + tid] = s_cnt[tid];
+        if (s_skip) continue;
+
+        // write: every path vertex stores its base, the first one its k-mer and the record
+        const uint64_t lab_end = s_bbyte + s_bytes;
+#pragma unroll
+        for (uint32_t q = 0; q < NQ; ++q) {
+            if (!((occ >> q) & 1u)) continue;
+            const uint32_t i = slot[q];
+            if (!info[q]) {
+                if ((s_meta[i] & (M_WRITER | M_CYCLE)) != (M_WRITER | M_CYCLE)) continue;
+                const uint64_t off = s_bbyte + s_jp[2 * i];
+                const uint32_t n = write_cycle(t, i, k, a.labels + off, off, lab_end, a.error);
+                const uint32_t ri = atomicAdd(&s_rctr, 1u);
+                if (ri < s_nrec) a.recs[s_brec + ri] = UnitigRec{off, n + k - 1, uint16_t(b), 1};
+                continue;
+            }
+            const uint32_t A = info[q] & 0x1FFFu, pos = (info[q] >> 13) & 0x1FFFu;
+            const bool fwd = (info[q] >> 26) & 1u;
+            const uint64_t off = s_bbyte + s_jp[2 * A];
+            const Kmer<KW> x = tab_key(t, i);
+            if (pos == 0) {
+                if (off + wn[q] + k - 1 > lab_end) {
+                    atomicOr(a.error, 1u);
+                    continue;
+                }
+                put_kmer(a.labels + off, fwd ? x : km_rc(x, k), k);
+                const uint32_t ri = atomicAdd(&s_rctr, 1u);
+                if (ri < s_nrec) a.recs[s_brec + ri] = UnitigRec{off, wn[q] + k - 1, uint16_t(b), uint16_t(wfl[q])};
+            } else {
+                const uint32_t c = fwd ? uint32_t(x.w[0] & 3u) : 3u - uint32_t((x.w[KW - 1] >> (2 * (kt - 1))) & 3u);
+                if (off + pos + k - 1 >= lab_end) {
+                    atomicOr(a.error, 1u);
+                    continue;
+                }
+                a.labels[off + pos + k - 1] = char(base_char(c));
+            }
+        }
+    }
+}
+
+// Subgraphs whose vertices do not fit an LDS table: a table in this
+// workgroup's global scratch region, sized by the subgraph's vertex count.
+template <int KW>
+__global__ __launch_bounds__(CT_WG) void k_contract_walk(ContractArgs a) {
     __shared__ uint32_t s_cand[CT_CAND];
     __shared__ uint32_t s_cnt[8];
     __shared__ uint32_t s_item, s_n, s_ovf, s_ncand, s_bytes, s_nrec, s_rctr, s_err, s_skip;
@@ -162,104 +566,42 @@ __global__ __launch_bounds__(CT_WG) void k_contract(ContractArgs a) {
     while (true) {
         __syncthreads();
         if (tid == 0) {
-            s_item = atomicAdd(&a.queue_head[GLOBAL ? 1 : 0], 1u);
+            s_item = atomicAdd(&a.queue_head[1], 1u);
             s_n = s_ovf = s_ncand = s_bytes = s_nrec = s_rctr = s_err = s_skip = 0;
         }
         if (tid < 8) s_cnt[tid] = 0;
         __syncthreads();
-        const uint32_t n_items =
-            GLOBAL ? __hip_atomic_load(a.ovf_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : a.n_list;
-        if (s_item >= n_items) break;
-        const uint32_t b = GLOBAL ? a.ovf_list[s_item] : a.list[s_item];
+        if (s_item >= __hip_atomic_load(a.ovf_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        const uint32_t b = a.ovf_list[s_item];
 
         CtTab<KW> t;
-        uint32_t limit;
-        if constexpr (GLOBAL) {
-            const uint32_t cap = uint32_t(ct_global_capacity(a.n_vert[b]));
-            uint8_t* base = a.scratch + uint64_t(blockIdx.x) * a.scratch_per_wg;
-            t.key = reinterpret_cast<uint64_t*>(base);
-            t.meta = reinterpret_cast<uint32_t*>(base + uint64_t(cap) * KW * 8);
-            t.aux = t.meta + cap;
-            t.cap = cap;
-            limit = cap;  // holds every vertex of the bucket
-        } else {
-            t.key = s_key;
-            t.meta = s_meta;
-            t.aux = s_aux;
-            t.cap = CL;
-            limit = CL * 4 / 5;
-        }
+        const uint32_t cap = uint32_t(ct_global_capacity(a.n_vert[b]));
+        uint8_t* base = a.scratch + uint64_t(blockIdx.x) * a.scratch_per_wg;
+        t.key = reinterpret_cast<uint64_t*>(base);
+        t.meta = reinterpret_cast<uint32_t*>(base + uint64_t(cap) * KW * 8);
+        t.aux = t.meta + cap;
+        t.cap = cap;
         for (uint32_t i = tid; i < t.cap; i += CT_WG) {
             t.key[size_t(i) * KW + KW - 1] = EMPTY;
             t.meta[i] = 0;
         }
         __syncthreads();
-
-        // load: isolated vertices counted, the others into the table
-        {
-            const uint8_t* tbl = reinterpret_cast<const uint8_t*>(a.tbl_ptr[b]);
-            const uint64_t tcap = a.tbl_cap[b];
-            uint32_t iso = 0, ph = 0;
-            for (uint64_t s = tid; s < tcap; s += CT_WG) {
-                const uint64_t* p = reinterpret_cast<const uint64_t*>(tbl + s * uint64_t(8 * KW + 8));
-                Kmer<KW> x;
-#pragma unroll
-                for (int w = 0; w < KW; ++w) x.w[w] = p[w];
-                if (x.w[KW - 1] == EMPTY) continue;
-                const uint64_t es = p[KW];
-                const uint32_t ef = uint32_t(es), st = uint32_t(es >> 32);
-                const uint32_t f = edge_at(ef, 0, a.cutoff), bk = edge_at(ef, 16, a.cutoff);
-                if (f == BASE_E && bk == BASE_E) {
-                    ++iso;
-                    ph += (st & 6u) != 0;
-                    continue;
-                }
-                if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
-                const uint32_t m = f | (bk << 3) | ((st & 2u) << 5) | ((st & 4u) << 5);
-                if (atomicAdd(&s_n, 1u) >= limit || !tab_insert(t, x, m)) {
-                    s_ovf = 1;
-                    break;
-                }
-            }
-            if (iso) atomicAdd(&s_cnt[CT_ISOLATED], iso);
-            if (ph) atomicAdd(&s_cnt[CT_PHANTOM], ph);
-        }
+        load_bucket(a, b, t, cap, &s_n, &s_ovf, s_cnt, nullptr);  // the capacity exceeds the vertex count
         __syncthreads();
         if (s_ovf) {
-            if (tid == 0) {
-                if constexpr (GLOBAL) {
-                    atomicOr(a.error, 1u);
-                } else {
-                    a.ovf_list[atomicAdd(a.ovf_count, 1u)] = b;
-                }
-            }
+            if (tid == 0) atomicOr(a.error, 1u);
             continue;
         }
 
-        // link: sides that continue the unitig
+        // link
         for (uint32_t i = tid; i < t.cap; i += CT_WG) {
             if (t.key[size_t(i) * KW + KW - 1] == EMPTY) continue;
             const Kmer<KW> x = tab_key(t, i);
             const Kmer<KW> xr = km_rc(x, k);
             const uint32_t m = ld_meta(&t.meta[i]);
             uint32_t add = 0;
-            for (uint32_t s = 0; s < 2; ++s) {
-                const uint32_t e = side_edge(m, s);
-                if (e >= 4) continue;
-                Kmer<KW> z = s ? x : xr, zb = s ? xr : x;
-                uint32_t side = s, bb;
-                const int j = walk_step(t, m, k, z, zb, side, bb);
-                if (j < 0) {
-                    s_err = 1;
-                    continue;
-                }
-                const uint32_t ent = 1u - side;
-                const uint32_t ej = side_edge(ld_meta(&t.meta[j]), ent);
-                if (ej == BASE_E) s_err = 1;                 // the edge is counted at both ends
-                if (ej >= 4) continue;                       // branching entrance: crossed
-                if (uint32_t(j) == i && ent == s) continue;  // back through the same side
-                add |= M_LINK0 << s;
-            }
+            for (uint32_t sd = 0; sd < 2; ++sd)
+                if (link_of(t, i, sd, x, xr, m, k, &s_err) >= 0) add |= M_LINK0 << sd;
             if (add) atomicOr(&t.meta[i], add);
         }
         __syncthreads();
@@ -270,13 +612,13 @@ __global__ __launch_bounds__(CT_WG) void k_contract(ContractArgs a) {
             for (uint32_t i = tid; i < t.cap; i += CT_WG) {
                 if (t.key[size_t(i) * KW + KW - 1] == EMPTY) continue;
                 const uint32_t m0 = ld_meta(&t.meta[i]);
-                for (uint32_t s = 0; s < 2; ++s) {
-                    if (m0 & (M_LINK0 << s)) continue;
+                for (uint32_t sd = 0; sd < 2; ++sd) {
+                    if (m0 & (M_LINK0 << sd)) continue;
                     const Kmer<KW> x = tab_key(t, i);
                     const Kmer<KW> xr = km_rc(x, k);
-                    Kmer<KW> z = s ? xr : x, zb = s ? x : xr;  // leave through side 1 - s
+                    Kmer<KW> z = sd ? xr : x, zb = sd ? x : xr;  // leave through side 1 - sd
                     const Kmer<KW> first = z;
-                    uint32_t side = 1u - s, cur = i, n = 1, bb;
+                    uint32_t side = 1u - sd, cur = i, n = 1, bb;
                     atomicOr(&t.meta[i], M_VIS);
                     for (uint32_t step = 0; step < t.cap; ++step) {
                         const uint32_t m = ld_meta(&t.meta[cur]);
@@ -291,10 +633,10 @@ __global__ __launch_bounds__(CT_WG) void k_contract(ContractArgs a) {
                         ++n;
                     }
                     if (!km_less(first, zb)) continue;  // the other end's orientation is canonical
-                    const bool ex = side_exits(m0, s) || side_exits(ld_meta(&t.meta[cur]), side);
+                    const bool ex = side_exits(m0, sd) || side_exits(ld_meta(&t.meta[cur]), side);
                     const uint32_t bytes = n + k - 1;
                     t.aux[i] = atomicAdd(&s_bytes, bytes);
-                    atomicOr(&t.meta[i], M_WRITER | (s ? M_START1 : 0u));
+                    atomicOr(&t.meta[i], M_WRITER | (sd ? M_START1 : 0u));
                     ++nrec;
                     verts += n;
                     lbytes += bytes;
@@ -315,50 +657,14 @@ __global__ __launch_bounds__(CT_WG) void k_contract(ContractArgs a) {
         // cycles: vertices no path reached
         for (uint32_t i = tid; i < t.cap; i += CT_WG) {
             if (t.key[size_t(i) * KW + KW - 1] == EMPTY || (ld_meta(&t.meta[i]) & M_VIS)) continue;
+            atomicOr(&t.meta[i], M_ONCYC);
             const uint32_t c = atomicAdd(&s_ncand, 1u);
-            if (c < CT_CAND) s_cand[c] = i;
+            if (c < uint32_t(CT_CAND)) s_cand[c] = i;
         }
         __syncthreads();
-        if (tid == 0 && s_ncand) {
-            const bool all = s_ncand > uint32_t(CT_CAND);
-            const uint32_t nc = all ? t.cap : s_ncand;
-            for (uint32_t q = 0; q < nc; ++q) {
-                const uint32_t c = all ? q : s_cand[q];
-                if (t.key[size_t(c) * KW + KW - 1] == EMPTY || (t.meta[c] & M_VIS)) continue;
-                Kmer<KW> z = tab_key(t, c), zb = km_rc(z, k), mn = z;
-                uint32_t side = 1, cur = c, n = 1, ms = c, bb;
-                t.meta[c] |= M_VIS;
-                for (uint32_t step = 0; step < t.cap; ++step) {
-                    const uint32_t m = t.meta[cur];
-                    if (!(m & (M_LINK0 << side))) {  // every side on a cycle continues
-                        s_err = 1;
-                        break;
-                    }
-                    const int j = walk_step(t, m, k, z, zb, side, bb);
-                    if (j < 0) {
-                        s_err = 1;
-                        break;
-                    }
-                    if (uint32_t(j) == c) break;
-                    t.meta[j] |= M_VIS;
-                    Kmer<KW> key;  // canonical form
-#pragma unroll
-                    for (int w = 0; w < KW; ++w) key.w[w] = side ? z.w[w] : zb.w[w];
-                    if (km_less(key, mn)) mn = key, ms = uint32_t(j);
-                    cur = uint32_t(j);
-                    ++n;
-                }
-                const uint32_t bytes = n + k - 1;
-                t.aux[ms] = s_bytes;
-                s_bytes += bytes;
-                t.meta[ms] |= M_WRITER | M_CYCLE;
-                s_nrec += 1;
-                s_cnt[CT_TRIVIAL] += 1;
-                s_cnt[CT_ICC] += 1;
-                s_cnt[CT_VERTICES] += n;
-                s_cnt[CT_LABEL_BYTES] += bytes;
-            }
-        }
+        if (tid == 0 && s_ncand)
+            cycles_serial(t, k, s_cand, s_ncand, &s_bytes, &s_nrec, s_cnt, &s_err,
+                          [&](uint32_t slot) { return &t.aux[slot]; });
         __syncthreads();
 
         // reserve the bucket's output, publish its counters
@@ -368,7 +674,7 @@ __global__ __launch_bounds__(CT_WG) void k_contract(ContractArgs a) {
             s_skip = s_brec + s_nrec > a.rec_cap || s_bbyte + s_bytes > a.lab_cap;
             if (s_skip) atomicOr(a.retry, 1u);
             if (s_err) atomicOr(a.error, 1u);
-            s_cnt[CT_FLAGS] = (GLOBAL ? 2u : 1u) | (s_err ? 4u : 0u);
+            s_cnt[CT_FLAGS] = 2u | (s_err ? 4u : 0u);
         }
         __syncthreads();
         if (tid < 8) a.counts[size_t(b) * 8 + tid] = s_cnt[tid];
@@ -381,49 +687,48 @@ __global__ __launch_bounds__(CT_WG) void k_contract(ContractArgs a) {
             if (t.key[size_t(i) * KW + KW - 1] == EMPTY) continue;
             const uint32_t m0 = t.meta[i];
             if (!(m0 & M_WRITER)) continue;
-            const bool cyc = m0 & M_CYCLE;
-            const uint32_t s = cyc ? 0u : ((m0 & M_START1) ? 1u : 0u);
+            const uint64_t off = s_bbyte + t.aux[i];
+            char* out = a.labels + off;
+            if (m0 & M_CYCLE) {
+                const uint32_t n = write_cycle(t, i, k, out, off, lab_end, a.error);
+                const uint32_t ri = atomicAdd(&s_rctr, 1u);
+                if (ri < s_nrec) a.recs[s_brec + ri] = UnitigRec{off, n + k - 1, uint16_t(b), 1};
+                continue;
+            }
+            const uint32_t sd = (m0 & M_START1) ? 1u : 0u;
             const Kmer<KW> x = tab_key(t, i);
             const Kmer<KW> xr = km_rc(x, k);
-            Kmer<KW> z = s ? xr : x, zb = s ? x : xr;
-            const uint64_t off = s_bbyte + t.aux[i];
+            Kmer<KW> z = sd ? xr : x, zb = sd ? x : xr;
             if (off + k > lab_end) {
                 atomicOr(a.error, 1u);
                 continue;
             }
-            char* out = a.labels + off;
             put_kmer(out, z, k);
-            uint32_t side = 1u - s, cur = i, n = 1, bb;
+            uint32_t side = 1u - sd, cur = i, n = 1, bb;
             for (uint32_t step = 0; step < t.cap; ++step) {
                 const uint32_t m = t.meta[cur];
                 if (!(m & (M_LINK0 << side))) break;
                 const int j = walk_step(t, m, k, z, zb, side, bb);
-                if (j < 0 || (cyc && uint32_t(j) == i)) break;
+                if (j < 0) break;
                 if (off + k + n > lab_end) {
                     atomicOr(a.error, 1u);
                     break;
                 }
-                out[k - 1 + n] = "ACGT"[bb];
+                out[k - 1 + n] = char(base_char(bb));
                 cur = uint32_t(j);
                 ++n;
             }
-            uint16_t fl = cyc ? 1 : 0;
-            if (!cyc) fl |= (side_exits(m0, s) ? 2 : 0) | (side_exits(t.meta[cur], side) ? 4 : 0);
-            UnitigRec r;
-            r.label_off = off;
-            r.label_len = n + k - 1;
-            r.g = uint16_t(b);
-            r.flags = fl;
+            const uint16_t fl = (side_exits(m0, sd) ? 2 : 0) | (side_exits(t.meta[cur], side) ? 4 : 0);
             const uint32_t ri = atomicAdd(&s_rctr, 1u);
-            if (ri < s_nrec) a.recs[s_brec + ri] = r;
+            if (ri < s_nrec) a.recs[s_brec + ri] = UnitigRec{off, n + k - 1, uint16_t(b), fl};
         }
     }
 }
 
 template <int KW>
 void launch_kw(const ContractArgs& a, uint32_t grid_lds, uint32_t grid_global, hipStream_t s) {
-    if (grid_lds) hipLaunchKernelGGL((k_contract<KW, false>), dim3(grid_lds), dim3(CT_WG), 0, s, a);
-    if (grid_global) hipLaunchKernelGGL((k_contract<KW, true>), dim3(grid_global), dim3(CT_WG), 0, s, a);
+    if (grid_lds) hipLaunchKernelGGL(k_contract_rank<KW>, dim3(grid_lds), dim3(CT_WG), 0, s, a);
+    if (grid_global) hipLaunchKernelGGL(k_contract_walk<KW>, dim3(grid_global), dim3(CT_WG), 0, s, a);
 }
 
 }  // namespace
