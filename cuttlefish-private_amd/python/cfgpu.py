@@ -123,6 +123,13 @@ SIGNATURES = {
     "cfgpu_tuning_set": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Tuning)]),
     "cfgpu_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32,
                                       ctypes.POINTER(TuneTrial), ctypes.c_uint32, u32p]),
+    "cfhost_cdbg_build": (ctypes.c_int, [ctypes.c_uint32, ctypes.POINTER(Unitig), ctypes.c_uint64,
+                                         ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "cfhost_cdbg_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                       ctypes.POINTER(u64p), u64p]),
+    "cfhost_cdbg_write_fasta": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "cfhost_cdbg_free": (None, [ctypes.c_void_p]),
     "cfhost_input_from_file": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_uint32,
                                               ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]),
     "cfhost_input_from_buffer": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int,
@@ -225,6 +232,36 @@ class Input:
             self.close()
         except Exception:
             pass
+
+
+def cdbg_from_records(k, recs, n, buf, nb, sorted=True):
+    """cfhost_cdbg_build over Unitig records and their label buffer -> list of str."""
+    c = ctypes.c_void_p()
+    r = lib().cfhost_cdbg_build(k, recs, n, buf, nb, int(sorted), ctypes.byref(c))
+    if r != 0:
+        raise RuntimeError(f"cfhost_cdbg_build failed ({r})")
+    try:
+        seq, off, m = ctypes.c_void_p(), u64p(), ctypes.c_uint64()
+        lib().cfhost_cdbg_get(c, ctypes.byref(seq), ctypes.byref(off), ctypes.byref(m))
+        m = m.value
+        o = np.ctypeslib.as_array(off, shape=(m + 1,)).copy() if m else np.zeros(1, np.uint64)
+        raw = ctypes.string_at(seq, int(o[-1])) if m else b""
+        return [raw[int(o[i]):int(o[i + 1])].decode() for i in range(m)]
+    finally:
+        lib().cfhost_cdbg_free(c)
+
+
+def cdbg_from_unitigs(k, g, flags, labels, sorted=True):
+    """cfhost_cdbg_build over unitigs given as arrays (g, flags) and a list of labels."""
+    n = len(labels)
+    recs = (Unitig * max(n, 1))()
+    off = 0
+    for i, lab in enumerate(labels):
+        recs[i].label_off, recs[i].label_len = off, len(lab)
+        recs[i].g, recs[i].flags = int(g[i]), int(flags[i])
+        off += len(lab)
+    buf = "".join(labels).encode()
+    return cdbg_from_records(k, recs, n, buf, len(buf), sorted)
 
 
 def synth_genome(n, repeat_frac=0.0, fam_min=300, fam_max=6000, divergence=0.02, seed=1):
@@ -390,6 +427,22 @@ class Handle:
         self._check(lib().cfgpu_contract_stats(self._h, arr.ctypes.data_as(ctypes.POINTER(ContractStat)), G),
                     "contract_stats")
         return arr
+
+    def unitigs_raw(self):
+        """(Unitig records array, label bytes) of the last contract."""
+        n, nb = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(lib().cfgpu_unitigs_copy(self._h, None, 0, None, 0, ctypes.byref(n), ctypes.byref(nb)),
+                    "unitigs_copy")
+        recs = (Unitig * max(n.value, 1))()
+        buf = ctypes.create_string_buffer(max(nb.value, 1))
+        self._check(lib().cfgpu_unitigs_copy(self._h, recs, n.value, buf, nb.value, ctypes.byref(n),
+                                             ctypes.byref(nb)), "unitigs_copy")
+        return recs, n.value, buf, nb.value
+
+    def cdbg(self, sorted=True):
+        """Maximal unitigs of the whole graph (cfhost_cdbg_build over the last contract)."""
+        recs, n, buf, nb = self.unitigs_raw()
+        return cdbg_from_records(self.k, recs, n, buf, nb, sorted)
 
     def unitigs(self):
         """Unitigs of the last contract: dict g, flags (1 cycle, 2 left exit, 4 right exit), labels (str)."""

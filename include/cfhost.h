@@ -1,5 +1,5 @@
 /*
- * cfhost.h — host-side ingest and synthetic inputs for libcfgpu.
+ * cfhost.h — host-side ingest, synthetic inputs and the compacted graph for libcfgpu.
  *
  * Ingest restates the reference's record -> fragment rules and 2-bit encoding on the
  * host, in parallel, and produces exactly the stream cfgpu_partition consumes:
@@ -15,6 +15,8 @@
 
 #include <stddef.h>
 #include <stdint.h>
+
+#include "cfgpu.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -65,6 +67,25 @@ int cfhost_synth_mutate(const char* base, uint64_t len, double snp_rate, double 
 int cfhost_synth_bucket_workload(uint32_t n_buckets, uint64_t n_records, double zipf_s,
                                  uint32_t k, uint32_t l, uint64_t seed, uint32_t* att,
                                  uint64_t* labels, uint64_t* bucket_off);
+
+/* The compacted de Bruijn graph: every maximal unitig of the whole input, from the
+ * per-subgraph output of cfgpu_contract (records and labels as cfgpu_unitigs_copy
+ * returns them).  Trivial unitigs are kept; lm-tigs are glued at the discontinuity
+ * vertices they share across subgraphs - the contraction of the discontinuity graph and
+ * its expansion into labels (src/Discontinuity_Graph_Contractor.cpp,
+ * src/Contracted_Graph_Expander.cpp) - and written in the form Unitig_Collator gives
+ * maximal unitigs (include/Unitig_Collator.hpp:306-400): a linear unitig with its first
+ * k characters at most its reverse complement's, a cycle rotated to its smallest vertex
+ * in canonical form.  `sorted` != 0 sorts the unitigs.  Returns 0, -1 on bad arguments,
+ * -2 on records that do not glue consistently. */
+typedef struct cfhost_cdbg cfhost_cdbg;
+int cfhost_cdbg_build(uint32_t k, const cfgpu_unitig* recs, uint64_t n_recs, const char* labels,
+                      uint64_t n_bytes, int sorted, cfhost_cdbg** out);
+/* unitig i = seq[off[i], off[i+1]), i < n */
+int cfhost_cdbg_get(const cfhost_cdbg* c, const char** seq, const uint64_t** off, uint64_t* n);
+/* FASTA, one line per unitig, headers are the unitig index. */
+int cfhost_cdbg_write_fasta(const cfhost_cdbg* c, const char* path);
+void cfhost_cdbg_free(cfhost_cdbg* c);
 
 #ifdef __cplusplus
 }
