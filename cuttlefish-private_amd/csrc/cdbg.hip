@@ -1,0 +1,315 @@
+// The compacted de Bruijn graph on the device: the per-subgraph unitigs of
+// cfgpu_contract glued into every maximal unitig of the whole graph.
+//
+// An lm-tig end that exits lies on a discontinuity vertex; the lm-tig of the
+// subgraph owning the vertex's other side (when the vertex is not isolated
+// there) ends at the same vertex on the opposite side.  Gluing lm-tigs at these
+// shared (vertex, side) pairs contracts the discontinuity graph and expands it
+// back into labels (src/Discontinuity_Graph_Contractor.cpp,
+// src/Contracted_Graph_Expander.cpp); a chain ends where a vertex has no lm-tig
+// on the far side (the phantom edges, src/Discontinuity_Graph_Contractor.cpp:
+// 131-190).  Every unitig record is a piece of exactly one chain (trivial
+// unitigs are chains of one piece):
+//
+//   ends    each exiting end is entered in a hash table keyed by its vertex's
+//           canonical k-mer, in the slot half of its side; the other half holds
+//           the partner end;
+//   rank    directed states (piece, end it is entered at) point at the next
+//           piece of the chain; pointer jumping gives every state the last state
+//           of its chain, the pieces in between and their label length;
+//   place   a piece knows both chain ends, hence the orientation the chain is
+//           written in (first k characters at most the reverse complement's,
+//           include/Unitig_Collator.hpp:306-345), its offset in the chain and
+//           the chain's length; the first piece reserves the output;
+//   copy    each piece copies its label, reverse-complemented when read
+//           backwards, to its place (neighbours rewrite the same k shared
+//           characters).
+// Chains that close on themselves (cycles through several subgraphs) are
+// listed and finished on the host.
+#include "cf_kernels.hpp"
+
+namespace cfgpu {
+namespace {
+
+constexpr uint64_t EMPTY = ~0ull;
+constexpr uint64_t BUSY = 1ull << 63;  // never set in a key word: top words hold < 63 bits
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr uint32_t HOPS_SHIFT = 40;  // state word: label characters | pieces << 40
+constexpr uint64_t HOPS_MAX = (1ull << 24) - 1;
+
+__device__ __forceinline__ uint32_t ccode(char c) { return c == 'A' ? 0u : c == 'C' ? 1u : c == 'G' ? 2u : 3u; }
+
+// k characters at s as a k-mer (Kmer layout: word 0 holds the last 32 bases),
+// or their reverse complement.
+template <int KW>
+__device__ __forceinline__ Kmer<KW> enc(const char* s, uint32_t k, bool rc) {
+    Kmer<KW> x;
+#pragma unroll
+    for (int w = 0; w < KW; ++w) x.w[w] = 0;
+    const uint32_t kt = k - 32 * (KW - 1);
+    for (uint32_t i = 0; i < k; ++i) {
+        const uint32_t b = rc ? 3u - ccode(s[k - 1 - i]) : ccode(s[i]);
+#pragma unroll
+        for (int w = KW - 1; w > 0; --w) x.w[w] = (x.w[w] << 2) | (x.w[w - 1] >> 62);
+        x.w[0] = (x.w[0] << 2) | b;
+    }
+    x.w[KW - 1] &= (1ull << (2 * kt)) - 1;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t slot_of(uint64_t h, uint64_t cap) {
+    return uint32_t((uint64_t(uint32_t(h >> 32)) * cap) >> 32);
+}
+
+// The vertex and side of end `at` (0 first k-mer, 1 last) of a label: the
+// canonical k-mer and the side facing the label.
+template <int KW>
+__device__ __forceinline__ Kmer<KW> end_vertex(const char* lab, uint32_t len, uint32_t at, uint32_t k, uint32_t& side) {
+    const char* x = at ? lab + len - k : lab;
+    const Kmer<KW> f = enc<KW>(x, k, false), r = enc<KW>(x, k, true);
+    const bool can = km_less(f, r);
+    side = at ? (can ? 0u : 1u) : (can ? 1u : 0u);
+    return can ? f : r;
+}
+
+template <int KW>
+__global__ __launch_bounds__(256) void k_ends_insert(CdbgArgs a) {
+    for (uint64_t r = uint64_t(blockIdx.x) * 256 + threadIdx.x; r < a.n; r += uint64_t(gridDim.x) * 256) {
+        const UnitigRec u = a.recs[r];
+        for (uint32_t at = 0; at < 2; ++at) {
+            if (!(u.flags & (at ? 4u : 2u))) continue;
+            uint32_t side;
+            const Kmer<KW> x = end_vertex<KW>(a.labels + u.label_off, u.label_len, at, a.k, side);
+            uint32_t s = slot_of(km_hash(x), a.hcap);
+            // claim a slot: the top key word with BUSY set while the lower words
+            // are written (a lane that finds BUSY retries the slot; it never
+            // waits inside this iteration)
+            const uint64_t top = KW == 1 ? x.w[0] : (x.w[KW - 1] | BUSY);
+            for (uint64_t n = 0; n < 2 * a.hcap;) {
+                uint64_t* kp = &a.hkey[uint64_t(s) * KW];
+                uint64_t cur = __hip_atomic_load(kp + KW - 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                if (cur == EMPTY) {
+                    cur = atomicCAS(reinterpret_cast<unsigned long long*>(kp + KW - 1), EMPTY, top);
+                    if (cur == EMPTY) {
+#pragma unroll
+                        for (int w = 0; w < KW - 1; ++w)
+                            __hip_atomic_store(kp + w, x.w[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (KW > 1)
+                            __hip_atomic_store(kp + KW - 1, x.w[KW - 1], __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                        cur = x.w[KW - 1];
+                    } else if (KW > 1) {
+                        continue;  // lost the claim: look again
+                    }
+                }
+                if (KW > 1 && cur == (x.w[KW - 1] | BUSY)) continue;  // being written: look again
+                bool eq = cur == x.w[KW - 1];
+#pragma unroll
+                for (int w = 0; w < KW - 1; ++w)
+                    eq &= eq && __hip_atomic_load(kp + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == x.w[w];
+                if (eq) {
+                    if (atomicCAS(&a.hval[2 * uint64_t(s) + side], NONE, uint32_t(2 * r + at)) != NONE)
+                        atomicOr(a.error, 2u);  // two lm-tigs on one side of a vertex
+                    break;
+                }
+                s = s + 1 == a.hcap ? 0 : s + 1;
+                if (++n == a.hcap) atomicOr(a.error, 1u);
+            }
+        }
+    }
+}
+
+// State 2r + e: piece r entered at end e.  Its successor is the piece across
+// the vertex at r's other end, entered at its matching end.
+template <int KW>
+__global__ __launch_bounds__(256) void k_rank_init(CdbgArgs a) {
+    for (uint64_t r = uint64_t(blockIdx.x) * 256 + threadIdx.x; r < a.n; r += uint64_t(gridDim.x) * 256) {
+        const UnitigRec u = a.recs[r];
+        for (uint32_t e = 0; e < 2; ++e) {
+            const uint32_t out = 1 - e;  // leaves through its other end
+            uint32_t next = NONE;
+            if (u.flags & (out ? 4u : 2u)) {
+                uint32_t side;
+                const Kmer<KW> x = end_vertex<KW>(a.labels + u.label_off, u.label_len, out, a.k, side);
+                uint32_t s = slot_of(km_hash(x), a.hcap);
+                for (uint64_t n = 0; n < a.hcap; ++n) {
+                    const uint64_t* kp = &a.hkey[uint64_t(s) * KW];
+                    if (kp[KW - 1] == EMPTY) break;
+                    bool eq = true;
+#pragma unroll
+                    for (int w = 0; w < KW; ++w) eq &= kp[w] == x.w[w];
+                    if (eq) {
+                        next = a.hval[2 * uint64_t(s) + (1 - side)];
+                        break;
+                    }
+                    s = s + 1 == a.hcap ? 0 : s + 1;
+                }
+            }
+            const uint64_t st = 2 * r + e;
+            if (next == NONE) {
+                a.jt[0][st] = uint32_t(st);
+                a.jw[0][st] = 0;
+            } else {
+                a.jt[0][st] = next;
+                a.jw[0][st] = uint64_t(a.recs[next >> 1].label_len - a.k) | (1ull << HOPS_SHIFT);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rank_round(CdbgArgs a, uint32_t src, uint32_t* changed) {
+    const uint32_t* jt = a.jt[src];
+    const uint64_t* jw = a.jw[src];
+    uint32_t* nt = a.jt[src ^ 1];
+    uint64_t* nw = a.jw[src ^ 1];
+    bool any = false;
+    for (uint64_t st = uint64_t(blockIdx.x) * 256 + threadIdx.x; st < 2 * a.n; st += uint64_t(gridDim.x) * 256) {
+        const uint32_t t = jt[st];
+        const uint64_t w = jw[st];
+        if (t == uint32_t(st)) {
+            nt[st] = t;
+            nw[st] = w;
+            continue;
+        }
+        const uint32_t t2 = jt[t];
+        const uint64_t w2 = jw[t];
+        const uint64_t hops = min((w >> HOPS_SHIFT) + (w2 >> HOPS_SHIFT), HOPS_MAX);
+        const uint64_t chars = (w & ((1ull << HOPS_SHIFT) - 1)) + (w2 & ((1ull << HOPS_SHIFT) - 1));
+        nt[st] = t2;
+        nw[st] = (chars & ((1ull << HOPS_SHIFT) - 1)) | (hops << HOPS_SHIFT);
+        any |= t2 != t && hops < a.n;  // chains settle; cycles stop counting past n pieces
+    }
+    if (any) atomicOr(changed, 1u);
+}
+
+// Per piece: chain ends, orientation, offset and length of its chain; the
+// first piece of each chain reserves the output (wave-aggregated).
+template <int KW>
+__global__ __launch_bounds__(256) void k_place(CdbgArgs a, uint32_t src) {
+    const uint32_t* jt = a.jt[src];
+    const uint64_t* jw = a.jw[src];
+    const int lane = int(threadIdx.x & 63u);
+    for (uint64_t r0 = uint64_t(blockIdx.x) * 256; r0 < a.n; r0 += uint64_t(gridDim.x) * 256) {
+        const uint64_t r = r0 + threadIdx.x;
+        bool head = false;
+        uint64_t len = 0;
+        if (r < a.n) {
+            const UnitigRec u = a.recs[r];
+            const uint32_t E0 = jt[2 * r], E1 = jt[2 * r + 1];
+            const uint64_t w0 = jw[2 * r], w1 = jw[2 * r + 1];
+            if (jt[E0] != E0 || jw[E0] != 0 || jt[E1] != E1 || jw[E1] != 0) {  // on a cycle
+                a.place_head[r] = NONE;
+                atomicAdd(a.n_cycle_pieces, 1u);
+            } else {
+                const uint64_t c0 = w0 & ((1ull << HOPS_SHIFT) - 1), c1 = w1 & ((1ull << HOPS_SHIFT) - 1);
+                const uint64_t h0 = w0 >> HOPS_SHIFT, h1 = w1 >> HOPS_SHIFT;
+                bool fwd = true;  // read the chain with this piece forward
+                if (!(u.flags & 1u)) {
+                    // first piece in that reading: the end state of direction 1,
+                    // read from its unpaired end; last piece: the end of direction 0
+                    const uint32_t jf = E1 >> 1, af = 1u - (E1 & 1u), jl = E0 >> 1, al = E0 & 1u;
+                    const UnitigRec uf = a.recs[jf], ul = a.recs[jl];
+                    const char* lf = a.labels + uf.label_off;
+                    const char* ll = a.labels + ul.label_off;
+                    const Kmer<KW> first = af ? enc<KW>(lf + uf.label_len - a.k, a.k, true) : enc<KW>(lf, a.k, false);
+                    // reverse complement of the last k characters
+                    const Kmer<KW> rlast =
+                        al ? enc<KW>(ll, a.k, false) : enc<KW>(ll + ul.label_len - a.k, a.k, true);
+                    fwd = km_less(first, rlast);
+                }
+                const uint32_t hd = fwd ? (E1 >> 1) : (E0 >> 1);
+                a.place_head[r] = hd;
+                a.place_pos[r] = (fwd ? c1 : c0) | (uint64_t(fwd ? 0u : 1u) << 63);
+                head = (fwd ? h1 : h0) == 0;
+                len = c0 + c1 + u.label_len;
+            }
+        }
+        // heads reserve their chain's bytes: one atomic per wave
+        uint64_t incl = head ? len : 0;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint64_t v = __shfl_up(incl, d);
+            if (lane >= d) incl += v;
+        }
+        const uint64_t tot = __shfl(incl, 63);
+        const unsigned long long hb = __ballot(head);
+        uint64_t base = 0, rbase = 0;
+        if (lane == 0 && tot) {
+            base = atomicAdd(&a.cursors[1], (unsigned long long)tot);
+            rbase = atomicAdd(&a.cursors[0], (unsigned long long)__popcll(hb));
+        }
+        base = __shfl(base, 0);
+        rbase = __shfl(rbase, 0);
+        if (head) {
+            const uint64_t off = base + incl - len;
+            const uint64_t ri = rbase + uint64_t(__popcll(hb & ((1ull << lane) - 1)));
+            a.head_off[r] = off;
+            if (off + len <= a.out_cap && ri < a.out_n_cap) {
+                a.out_rec[2 * ri] = off;
+                a.out_rec[2 * ri + 1] = len;
+            } else {
+                atomicOr(a.retry, 1u);
+            }
+        }
+    }
+}
+
+// One wave per 64 pieces, each lane copying its own piece.
+__global__ __launch_bounds__(256) void k_copy(CdbgArgs a) {
+    for (uint64_t r = uint64_t(blockIdx.x) * 256 + threadIdx.x; r < a.n; r += uint64_t(gridDim.x) * 256) {
+        const uint32_t hd = a.place_head[r];
+        if (hd == NONE) continue;
+        const UnitigRec u = a.recs[r];
+        const uint64_t pos = a.place_pos[r] & ~(1ull << 63);
+        const bool rev = a.place_pos[r] >> 63;
+        const uint64_t hoff = a.head_off[hd];  // all-ones unless the head placed its chain
+        if (hoff >= a.out_cap || pos >= a.out_cap - hoff || a.out_cap - hoff - pos < u.label_len) {
+            atomicOr(a.error, 4u);
+            continue;
+        }
+        const uint64_t off = hoff + pos;
+        const char* src = a.labels + u.label_off;
+        char* dst = a.out + off;
+        if (!rev) {
+            for (uint32_t i = 0; i < u.label_len; ++i) dst[i] = src[i];
+        } else {
+            for (uint32_t i = 0; i < u.label_len; ++i) {
+                const char c = src[u.label_len - 1 - i];
+                dst[i] = c == 'A' ? 'T' : c == 'C' ? 'G' : c == 'G' ? 'C' : 'A';
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cycle_list(CdbgArgs a, uint32_t* list, uint32_t* n) {
+    for (uint64_t r = uint64_t(blockIdx.x) * 256 + threadIdx.x; r < a.n; r += uint64_t(gridDim.x) * 256)
+        if (a.place_head[r] == NONE) list[atomicAdd(n, 1u)] = uint32_t(r);
+}
+
+template <int KW>
+void launch_kw(const CdbgArgs& a, uint32_t grid, int phase, uint32_t src, uint32_t* changed, hipStream_t s) {
+    switch (phase) {
+        case 0: hipLaunchKernelGGL(k_ends_insert<KW>, dim3(grid), dim3(256), 0, s, a); break;
+        case 1: hipLaunchKernelGGL(k_rank_init<KW>, dim3(grid), dim3(256), 0, s, a); break;
+        case 2: hipLaunchKernelGGL(k_rank_round, dim3(grid), dim3(256), 0, s, a, src, changed); break;
+        case 3: hipLaunchKernelGGL(k_place<KW>, dim3(grid), dim3(256), 0, s, a, src); break;
+        default: hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, a); break;
+    }
+}
+
+}  // namespace
+
+void launch_cdbg_phase(const CdbgArgs& a, uint32_t KW, uint32_t grid, int phase, uint32_t src, uint32_t* changed,
+                       hipStream_t s) {
+    switch (KW) {
+        case 1: launch_kw<1>(a, grid, phase, src, changed, s); break;
+        case 2: launch_kw<2>(a, grid, phase, src, changed, s); break;
+        case 3: launch_kw<3>(a, grid, phase, src, changed, s); break;
+        default: launch_kw<4>(a, grid, phase, src, changed, s); break;
+    }
+}
+
+void launch_cdbg_cycle_list(const CdbgArgs& a, uint32_t grid, uint32_t* list, uint32_t* n, hipStream_t s) {
+    hipLaunchKernelGGL(k_cycle_list, dim3(grid), dim3(256), 0, s, a, list, n);
+}
+
+}  // namespace cfgpu

@@ -192,6 +192,38 @@ struct ContractArgs {
 void launch_contract(const ContractArgs& a, uint32_t KW, uint32_t grid_lds, uint32_t grid_global, hipStream_t s);
 
 // ---------------------------------------------------------------------------
+// The compacted graph (cdbg.hip): contract's unitigs glued into the maximal
+// unitigs of the whole graph.  Phases 0 ends into the hash table, 1 chain
+// successors, 2 one pointer-jumping round from jt/jw[src] (sets *changed),
+// 3 placement, 4 label copies.
+struct CdbgArgs {
+    const UnitigRec* recs;
+    uint64_t n;
+    const char* labels;
+    uint32_t k;
+    uint64_t* hkey;             // hcap * KW words, all-ones = free
+    uint32_t* hval;             // 2 per slot: the end on the front / back side (all-ones = none)
+    uint64_t hcap;
+    uint32_t* jt[2];            // per state: the state it points at
+    uint64_t* jw[2];            //   ... label characters | pieces << 40 up to there
+    uint32_t* place_head;       // per piece: first piece of its chain (all-ones: on a cycle)
+    uint64_t* place_pos;        //   ... its offset in the chain | reversed << 63
+    uint64_t* head_off;         // per chain head: output offset
+    char* out;
+    uint64_t out_cap;
+    uint64_t* out_rec;          // (offset, length) per maximal unitig
+    uint64_t out_n_cap;
+    unsigned long long* cursors;  // [0] unitigs, [1] bytes
+    uint32_t* retry;
+    uint32_t* error;
+    uint32_t* n_cycle_pieces;
+};
+void launch_cdbg_phase(const CdbgArgs& a, uint32_t KW, uint32_t grid, int phase, uint32_t src, uint32_t* changed,
+                       hipStream_t s);
+// list[0..*n) = pieces on cycles
+void launch_cdbg_cycle_list(const CdbgArgs& a, uint32_t grid, uint32_t* list, uint32_t* n, hipStream_t s);
+
+// ---------------------------------------------------------------------------
 // FP32 MLP bucket-size predictor: N_FEAT -> H -> H -> 1 (ReLU), predicts
 // log(distinct vertices / k-mer instances) of a bucket.
 constexpr int MLP_H = 64;

@@ -4,6 +4,7 @@
 // construction, hipGraph capture and timing.  All per-base / per-k-mer work runs
 // in the HIP kernels (partition.hip, construct.hip, mlp.hip).
 #include "../../include/cfgpu.h"
+#include "../../include/cfhost.h"
 
 #include <hip/hip_runtime.h>
 
@@ -122,6 +123,15 @@ struct cfgpu_handle {
     DevBuf ct_list, ct_ovf, ct_counts, ct_ctl, ct_scratch, ct_recs, ct_labels;
     uint64_t n_unitigs = 0, n_ubytes = 0;
     bool contracted = false;
+
+    // compacted graph: glue table, list-ranking states, output; cycles through
+    // several subgraphs finished on the host (appended after the device part)
+    DevBuf cd_hkey, cd_hval, cd_jt0, cd_jt1, cd_jw0, cd_jw1, cd_head, cd_pos, cd_hoff, cd_out, cd_rec, cd_ctl,
+        cd_cyc;
+    uint64_t cd_n = 0, cd_bytes = 0;
+    std::string cd_host_seq;
+    std::vector<uint64_t> cd_host_rec;
+    bool cdbg_done = false;
 
     // graph cache
     hipGraph_t graph = nullptr;
@@ -853,7 +863,7 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
                                 (unsigned long long)f);
         }
     }
-    h->partitioned = h->constructed = h->contracted = false;
+    h->partitioned = h->constructed = h->contracted = h->cdbg_done = false;
     const uint32_t G = h->G;
     const uint64_t n_words = (n_bases + 31) / 32 + 2;
     const uint64_t tiles = (n_bases + P_TILE - 1) / P_TILE;
@@ -999,7 +1009,7 @@ int cfgpu_load_buckets(cfgpu_handle* h, const uint32_t* att, const uint64_t* lab
                             (unsigned long long)r, g);
         }
     }
-    h->partitioned = h->constructed = h->contracted = false;
+    h->partitioned = h->constructed = h->contracted = h->cdbg_done = false;
     const uint64_t n = bucket_off[G];
     HIPCK(h, h->att.ensure(n * 4));
     HIPCK(h, h->labels.ensure(n * h->W * 8));
@@ -1038,7 +1048,7 @@ int cfgpu_construct(cfgpu_handle* h, uint32_t mode) {
     if (!h) return CFGPU_E_ARG;
     if (!h->partitioned) return fail(h, CFGPU_E_STATE, "construct before partition");
     HIPCK(h, hipSetDevice(h->prm.device));
-    h->constructed = h->contracted = false;
+    h->constructed = h->contracted = h->cdbg_done = false;
     return construct_impl(h, mode);
 }
 
@@ -1049,7 +1059,7 @@ int cfgpu_contract(cfgpu_handle* h) {
     if (!h) return CFGPU_E_ARG;
     if (!h->constructed) return fail(h, CFGPU_E_STATE, "contract before construct");
     HIPCK(h, hipSetDevice(h->prm.device));
-    h->contracted = false;
+    h->contracted = h->cdbg_done = false;
     std::vector<uint32_t> list;
     uint64_t sum_nv = 0;
     uint32_t max_nv = 0;
@@ -1126,6 +1136,155 @@ int cfgpu_contract(cfgpu_handle* h) {
     h->rs.n_unitig_bytes = h->n_ubytes;
     h->rs.n_contract_runs = run;
     h->contracted = true;
+    return CFGPU_OK;
+}
+
+int cfgpu_cdbg(cfgpu_handle* h) {
+    if (!h) return CFGPU_E_ARG;
+    if (!h->contracted) return fail(h, CFGPU_E_STATE, "cdbg before contract");
+    HIPCK(h, hipSetDevice(h->prm.device));
+    h->cdbg_done = false;
+    h->cd_host_seq.clear();
+    h->cd_host_rec.clear();
+    const uint64_t n = h->n_unitigs;
+    const uint32_t KW = h->KW;
+    Event e0, e1;
+    HIPCK(h, hipEventRecord(e0.e, h->stream));
+    uint32_t ctl[16] = {};
+    uint32_t rounds = 0;
+    if (n) {
+        if (2 * n >= 0xFFFFFFFFull) return fail(h, CFGPU_E_ARG, "cdbg: %llu unitigs", (unsigned long long)n);
+        const uint64_t hcap = round_cap(4.0 * double(n));
+        HIPCK(h, h->cd_hkey.ensure(hcap * KW * 8));
+        HIPCK(h, h->cd_hval.ensure(hcap * 8));
+        HIPCK(h, h->cd_jt0.ensure(2 * n * 4));
+        HIPCK(h, h->cd_jt1.ensure(2 * n * 4));
+        HIPCK(h, h->cd_jw0.ensure(2 * n * 8));
+        HIPCK(h, h->cd_jw1.ensure(2 * n * 8));
+        HIPCK(h, h->cd_head.ensure(n * 4));
+        HIPCK(h, h->cd_pos.ensure(n * 8));
+        HIPCK(h, h->cd_hoff.ensure(n * 8));
+        HIPCK(h, h->cd_out.ensure(h->n_ubytes));  // glued labels share k characters: never longer
+        HIPCK(h, h->cd_rec.ensure(n * 16));
+        HIPCK(h, h->cd_ctl.ensure(64));
+        HIPCK(h, hipMemsetAsync(h->cd_hkey.p, 0xFF, hcap * KW * 8, h->stream));
+        HIPCK(h, hipMemsetAsync(h->cd_hval.p, 0xFF, hcap * 8, h->stream));
+        HIPCK(h, hipMemsetAsync(h->cd_hoff.p, 0xFF, n * 8, h->stream));
+        HIPCK(h, hipMemsetAsync(h->cd_ctl.p, 0, 64, h->stream));
+        uint32_t* c = h->cd_ctl.as<uint32_t>();
+        CdbgArgs a{};
+        a.recs = h->ct_recs.as<const UnitigRec>();
+        a.n = n;
+        a.labels = h->ct_labels.as<const char>();
+        a.k = h->prm.k;
+        a.hkey = h->cd_hkey.as<uint64_t>();
+        a.hval = h->cd_hval.as<uint32_t>();
+        a.hcap = hcap;
+        a.jt[0] = h->cd_jt0.as<uint32_t>();
+        a.jt[1] = h->cd_jt1.as<uint32_t>();
+        a.jw[0] = h->cd_jw0.as<uint64_t>();
+        a.jw[1] = h->cd_jw1.as<uint64_t>();
+        a.place_head = h->cd_head.as<uint32_t>();
+        a.place_pos = h->cd_pos.as<uint64_t>();
+        a.head_off = h->cd_hoff.as<uint64_t>();
+        a.out = h->cd_out.as<char>();
+        a.out_cap = h->n_ubytes;
+        a.out_rec = h->cd_rec.as<uint64_t>();
+        a.out_n_cap = n;
+        a.cursors = reinterpret_cast<unsigned long long*>(c + 8);  // [8..11]
+        a.retry = c + 2;
+        a.error = c + 3;
+        a.n_cycle_pieces = c + 4;
+        const uint32_t grid = uint32_t(std::min<uint64_t>((2 * n + 255) / 256, uint64_t(h->cus) * 8));
+        launch_cdbg_phase(a, KW, grid, 0, 0, nullptr, h->stream);
+        launch_cdbg_phase(a, KW, grid, 1, 0, nullptr, h->stream);
+        uint32_t src = 0;
+        for (rounds = 0; rounds < 64;) {
+            HIPCK(h, hipMemsetAsync(c + 5, 0, 4, h->stream));
+            launch_cdbg_phase(a, KW, grid, 2, src, c + 5, h->stream);
+            uint32_t changed = 0;
+            HIPCK(h, hipMemcpyAsync(&changed, c + 5, 4, hipMemcpyDeviceToHost, h->stream));
+            HIPCK(h, hipStreamSynchronize(h->stream));
+            src ^= 1;
+            ++rounds;
+            if (!changed) break;
+        }
+        launch_cdbg_phase(a, KW, grid, 3, src, nullptr, h->stream);
+        launch_cdbg_phase(a, KW, grid, 4, src, nullptr, h->stream);
+        HIPCK(h, hipGetLastError());
+        HIPCK(h, hipMemcpyAsync(ctl, c, sizeof ctl, hipMemcpyDeviceToHost, h->stream));
+        HIPCK(h, hipStreamSynchronize(h->stream));
+        if (ctl[3]) return fail(h, CFGPU_E_STATE, "cdbg: inconsistent unitig ends (%u)", ctl[3]);
+        if (ctl[2]) return fail(h, CFGPU_E_STATE, "cdbg: output longer than the unitig labels");
+        if (ctl[4]) {  // cycles through several subgraphs: their pieces to the host
+            HIPCK(h, h->cd_cyc.ensure(uint64_t(ctl[4]) * 4));
+            HIPCK(h, hipMemsetAsync(c + 6, 0, 4, h->stream));
+            launch_cdbg_cycle_list(a, grid, h->cd_cyc.as<uint32_t>(), c + 6, h->stream);
+            std::vector<uint32_t> list(ctl[4]);
+            HIPCK(h, hipMemcpyAsync(list.data(), h->cd_cyc.p, list.size() * 4, hipMemcpyDeviceToHost, h->stream));
+            HIPCK(h, hipStreamSynchronize(h->stream));
+            std::vector<cfgpu_unitig> recs(list.size());
+            std::string labs;
+            for (size_t i = 0; i < list.size(); ++i) {
+                HIPCK(h, hipMemcpy(&recs[i], h->ct_recs.as<cfgpu_unitig>() + list[i], sizeof(cfgpu_unitig),
+                                   hipMemcpyDeviceToHost));
+                std::string lab(recs[i].label_len, 'A');
+                HIPCK(h, hipMemcpy(&lab[0], h->ct_labels.as<char>() + recs[i].label_off, lab.size(),
+                                   hipMemcpyDeviceToHost));
+                recs[i].label_off = labs.size();
+                labs += lab;
+            }
+            cfhost_cdbg* cc = nullptr;
+            if (cfhost_cdbg_build(h->prm.k, recs.data(), recs.size(), labs.data(), labs.size(), 0, &cc) != 0)
+                return fail(h, CFGPU_E_STATE, "cdbg: cycles through several subgraphs do not close");
+            const char* seq;
+            const uint64_t* off;
+            uint64_t m;
+            cfhost_cdbg_get(cc, &seq, &off, &m);
+            h->cd_host_seq.assign(seq, off[m]);
+            for (uint64_t i = 0; i < m; ++i) {
+                h->cd_host_rec.push_back(off[i]);
+                h->cd_host_rec.push_back(off[i + 1] - off[i]);
+            }
+            cfhost_cdbg_free(cc);
+        }
+    }
+    uint64_t cur[2] = {0, 0};
+    std::memcpy(cur, ctl + 8, sizeof cur);
+    h->cd_n = cur[0];
+    h->cd_bytes = cur[1];
+    HIPCK(h, hipEventRecord(e1.e, h->stream));
+    HIPCK(h, hipEventSynchronize(e1.e));
+    h->rs.t_cdbg_ms = elapsed_ms(e0, e1);
+    h->rs.n_cdbg_unitigs = h->cd_n + h->cd_host_rec.size() / 2;
+    h->rs.n_cdbg_bytes = h->cd_bytes + h->cd_host_seq.size();
+    h->rs.n_cdbg_rounds = rounds;
+    h->rs.n_cdbg_cycle_pieces = ctl[4];
+    h->cdbg_done = true;
+    return CFGPU_OK;
+}
+
+int cfgpu_cdbg_copy(cfgpu_handle* h, char* seq, uint64_t seq_cap, uint64_t* rec, uint64_t rec_cap, uint64_t* n,
+                    uint64_t* n_bytes) {
+    if (!h) return CFGPU_E_ARG;
+    if (!h->cdbg_done) return fail(h, CFGPU_E_STATE, "cdbg copy before cdbg");
+    const uint64_t tn = h->cd_n + h->cd_host_rec.size() / 2, tb = h->cd_bytes + h->cd_host_seq.size();
+    if (n) *n = tn;
+    if (n_bytes) *n_bytes = tb;
+    if ((seq && seq_cap < tb) || (rec && rec_cap < tn))
+        return fail(h, CFGPU_E_ARG, "cdbg copy: capacity %llu / %llu < %llu / %llu", (unsigned long long)seq_cap,
+                    (unsigned long long)rec_cap, (unsigned long long)tb, (unsigned long long)tn);
+    HIPCK(h, hipSetDevice(h->prm.device));
+    if (seq && h->cd_bytes)
+        HIPCK(h, hipMemcpyAsync(seq, h->cd_out.p, h->cd_bytes, hipMemcpyDeviceToHost, h->stream));
+    if (rec && h->cd_n) HIPCK(h, hipMemcpyAsync(rec, h->cd_rec.p, h->cd_n * 16, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    if (seq && !h->cd_host_seq.empty()) std::memcpy(seq + h->cd_bytes, h->cd_host_seq.data(), h->cd_host_seq.size());
+    if (rec)
+        for (size_t i = 0; i < h->cd_host_rec.size() / 2; ++i) {
+            rec[2 * (h->cd_n + i)] = h->cd_bytes + h->cd_host_rec[2 * i];
+            rec[2 * (h->cd_n + i) + 1] = h->cd_host_rec[2 * i + 1];
+        }
     return CFGPU_OK;
 }
 

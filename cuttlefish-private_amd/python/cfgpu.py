@@ -3,6 +3,7 @@
 The binding is thin: every call goes straight to the C ABI, which runs the
 gfx950 kernels.  There is no CPU fallback; a missing library or GPU raises.
 """
+import builtins
 import ctypes
 import os
 
@@ -59,7 +60,9 @@ class RunStats(ctypes.Structure):
                    ("reserved2", ctypes.c_uint32), ("t_dedup_ms", ctypes.c_double),
                    ("t_contract_ms", ctypes.c_double), ("n_unitigs", ctypes.c_uint64),
                    ("n_unitig_bytes", ctypes.c_uint64), ("n_contract_global", ctypes.c_uint32),
-                   ("n_contract_runs", ctypes.c_uint32)])
+                   ("n_contract_runs", ctypes.c_uint32), ("t_cdbg_ms", ctypes.c_double),
+                   ("n_cdbg_unitigs", ctypes.c_uint64), ("n_cdbg_bytes", ctypes.c_uint64),
+                   ("n_cdbg_rounds", ctypes.c_uint32), ("n_cdbg_cycle_pieces", ctypes.c_uint32)])
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -123,6 +126,9 @@ SIGNATURES = {
     "cfgpu_tuning_set": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Tuning)]),
     "cfgpu_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32,
                                       ctypes.POINTER(TuneTrial), ctypes.c_uint32, u32p]),
+    "cfgpu_cdbg": (ctypes.c_int, [ctypes.c_void_p]),
+    "cfgpu_cdbg_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, u64p, ctypes.c_uint64,
+                                       u64p, u64p]),
     "cfhost_cdbg_build": (ctypes.c_int, [ctypes.c_uint32, ctypes.POINTER(Unitig), ctypes.c_uint64,
                                          ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_void_p)]),
@@ -161,6 +167,7 @@ SIGNATURES = {
 }
 
 _LIB = None
+sorted_fn = builtins.sorted
 
 
 def lib():
@@ -234,8 +241,9 @@ class Input:
             pass
 
 
-def cdbg_from_records(k, recs, n, buf, nb, sorted=True):
-    """cfhost_cdbg_build over Unitig records and their label buffer -> list of str."""
+def cdbg_from_records(k, recs, n, buf, nb, sorted=True, decode=True):
+    """cfhost_cdbg_build over Unitig records and their label buffer -> list of str
+    (decode=False: (unitig count, total label bytes))."""
     c = ctypes.c_void_p()
     r = lib().cfhost_cdbg_build(k, recs, n, buf, nb, int(sorted), ctypes.byref(c))
     if r != 0:
@@ -244,6 +252,8 @@ def cdbg_from_records(k, recs, n, buf, nb, sorted=True):
         seq, off, m = ctypes.c_void_p(), u64p(), ctypes.c_uint64()
         lib().cfhost_cdbg_get(c, ctypes.byref(seq), ctypes.byref(off), ctypes.byref(m))
         m = m.value
+        if not decode:
+            return m, int(off[m]) if m else 0
         o = np.ctypeslib.as_array(off, shape=(m + 1,)).copy() if m else np.zeros(1, np.uint64)
         raw = ctypes.string_at(seq, int(o[-1])) if m else b""
         return [raw[int(o[i]):int(o[i + 1])].decode() for i in range(m)]
@@ -438,6 +448,21 @@ class Handle:
         self._check(lib().cfgpu_unitigs_copy(self._h, recs, n.value, buf, nb.value, ctypes.byref(n),
                                              ctypes.byref(nb)), "unitigs_copy")
         return recs, n.value, buf, nb.value
+
+    def cdbg_device(self, sorted=True, decode=True):
+        """Maximal unitigs of the whole graph glued on the device (cfgpu_cdbg)."""
+        self._check(lib().cfgpu_cdbg(self._h), "cdbg")
+        n, nb = ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(lib().cfgpu_cdbg_copy(self._h, None, 0, None, 0, ctypes.byref(n), ctypes.byref(nb)), "cdbg_copy")
+        if not decode:
+            return n.value, nb.value
+        buf = ctypes.create_string_buffer(max(nb.value, 1))
+        rec = np.zeros(max(2 * n.value, 2), np.uint64)
+        self._check(lib().cfgpu_cdbg_copy(self._h, buf, nb.value, _p(rec, u64p), n.value, ctypes.byref(n),
+                                          ctypes.byref(nb)), "cdbg_copy")
+        raw = buf.raw
+        out = [raw[int(rec[2 * i]):int(rec[2 * i] + rec[2 * i + 1])].decode() for i in range(n.value)]
+        return sorted_fn(out) if sorted else out
 
     def cdbg(self, sorted=True):
         """Maximal unitigs of the whole graph (cfhost_cdbg_build over the last contract)."""

@@ -192,6 +192,19 @@ int cfgpu_unitigs_view(cfgpu_handle* h, const cfgpu_unitig** recs, const char** 
 int cfgpu_unitigs_copy(cfgpu_handle* h, cfgpu_unitig* recs, uint64_t rec_cap, char* labels,
                        uint64_t lab_cap, uint64_t* n_unitigs, uint64_t* n_bytes);
 
+/* The compacted de Bruijn graph on the device: the unitigs of the last cfgpu_contract
+ * glued into every maximal unitig of the whole graph, as cfhost_cdbg_build does on the
+ * host (include/cfhost.h): lm-tigs joined at the discontinuity vertices they share
+ * across subgraphs (src/Discontinuity_Graph_Contractor.cpp,
+ * src/Contracted_Graph_Expander.cpp), each unitig in the form Unitig_Collator writes it
+ * (include/Unitig_Collator.hpp:306-400).  Cycles through several subgraphs, rare, are
+ * finished on the host. */
+int cfgpu_cdbg(cfgpu_handle* h);
+/* Unitig i = seq[rec[2i], rec[2i] + rec[2i+1]), i < *n (order unspecified).  Either
+ * output may be NULL to query *n and *n_bytes. */
+int cfgpu_cdbg_copy(cfgpu_handle* h, char* seq, uint64_t seq_cap, uint64_t* rec, uint64_t rec_cap,
+                    uint64_t* n, uint64_t* n_bytes);
+
 /* Counters and timings of the last partition / construct / contract. */
 typedef struct cfgpu_run_stats {
     uint64_t n_bases, n_fragments, n_records, n_kmers, n_edges, n_vertices;
@@ -219,6 +232,11 @@ typedef struct cfgpu_run_stats {
     uint64_t n_unitigs, n_unitig_bytes;
     uint32_t n_contract_global;       /* subgraphs too large for an LDS table */
     uint32_t n_contract_runs;         /* 1, or 2 after growing the output */
+    /* last cfgpu_cdbg */
+    double t_cdbg_ms;
+    uint64_t n_cdbg_unitigs, n_cdbg_bytes;
+    uint32_t n_cdbg_rounds;           /* pointer-jumping rounds */
+    uint32_t n_cdbg_cycle_pieces;     /* lm-tigs on cycles through several subgraphs (host) */
 } cfgpu_run_stats;
 int cfgpu_run_stats_get(cfgpu_handle* h, cfgpu_run_stats* out);
 

@@ -1,5 +1,6 @@
-"""The compacted de Bruijn graph end to end on the GPU: partition, construct and
-contract on the device, lm-tigs glued on the host (cfhost_cdbg_build).  The
+"""The compacted de Bruijn graph end to end on the GPU: partition, construct,
+contract and the gluing of lm-tigs on the device (cfgpu_cdbg), which must also
+agree with the host gluing (cfhost_cdbg_build).  The
 sorted set of canonical maximal unitigs must equal the brute-force global
 graph's (unitig_ref.py) on small inputs, and the oracle pipeline's (partition,
 construct and contract restated on the CPU, glued the same way) on inputs
@@ -17,11 +18,19 @@ pytestmark = pytest.mark.gpu
 
 
 def gpu_cdbg(buf, fq, k, l, cutoff, mode="learned"):
+    """Glued on the device (cfgpu_cdbg); the host gluing of the same unitigs must agree."""
     with C.Handle(k, l, cutoff) as h:
         h.partition(C.Input.from_buffer(buf, fq, k))
         h.construct(mode)
         h.contract()
-        return h.cdbg()
+        dev = h.cdbg_device()
+        rs = h.run_stats()
+        say(f"[case] cdbg device: unitigs={rs['n_cdbg_unitigs']} rounds={rs['n_cdbg_rounds']} "
+            f"cycle_pieces={rs['n_cdbg_cycle_pieces']}")
+        assert dev == h.cdbg()
+        assert rs["n_cdbg_unitigs"] == len(dev)
+        gpu_cdbg.last_stats = rs
+        return dev
 
 
 def oracle_cdbg(buf, fq, k, l, cutoff):
@@ -40,6 +49,21 @@ def test_cdbg_small_equals_bruteforce(k, l, cutoff):
     seq, ro = O.parse(buf, False)
     got = gpu_cdbg(buf, False, k, l, cutoff, mode="batched")
     assert got == U.unitigs(seq, ro, k, cutoff)
+
+
+def test_cdbg_cycles_within_and_across_subgraphs():
+    # short-period tandem repeats cycle inside one subgraph; long periods pass
+    # through several subgraphs and close on the host
+    rng = np.random.default_rng(95)
+    recs = ["".join("ACGT"[b] for b in rng.integers(0, 4, p)) for p in (3, 4, 7, 40, 55, 70, 90, 120)]
+    recs = [(u * (2 + 300 // len(u))).encode() for u in recs]
+    buf = fasta(recs)
+    seq, ro = O.parse(buf, False)
+    for k, l in ((31, 20), (21, 11)):
+        say(f"[case] cdbg-cycles k={k}")
+        got = gpu_cdbg(buf, False, k, l, 1, mode="batched")
+        assert got == U.unitigs(seq, ro, k, 1)
+        assert gpu_cdbg.last_stats["n_cdbg_cycle_pieces"] > 0
 
 
 def test_cdbg_reference_with_repeat_copies():
