@@ -182,11 +182,31 @@ def main():
     rc = h.run_stats()
     cst = h.contract_stats()
     ct_ms = sorted(ct)[len(ct) // 2]
+    # the compacted graph: lm-tigs glued on the device (the first call sizes its
+    # buffers), then the maximal unitigs copied to the host
+    h.cdbg_device(decode=False)
+    cd = []
+    for _ in range(3):
+        h.cdbg_device(decode=False)
+        cd.append(h.run_stats()["t_cdbg_ms"])
+    rcd = h.run_stats()
+    cdbg_ms = sorted(cd)[len(cd) // 2]
+    hip.sync()
+    t0 = time.perf_counter()
+    n_cdbg, cdbg_bytes = rcd["n_cdbg_unitigs"], rcd["n_cdbg_bytes"]
+    seq_buf = ctypes.create_string_buffer(max(cdbg_bytes, 1))
+    rec_buf = np.zeros(max(2 * n_cdbg, 2), np.uint64)
+    C.lib().cfgpu_cdbg_copy(h._h, seq_buf, cdbg_bytes, rec_buf.ctypes.data_as(C.u64p), n_cdbg, None, None)
+    copy_ms = (time.perf_counter() - t0) * 1000.0
+    del seq_buf, rec_buf
     contract = {"ms": ct_ms, "unitigs": rc["n_unitigs"], "label_bytes": rc["n_unitig_bytes"],
                 "global_table_subgraphs": rc["n_contract_global"], "runs": rc["n_contract_runs"],
                 "isolated": int(cst[:, 0].sum()), "phantom": int(cst[:, 1].sum()),
                 "trivial": int(cst[:, 2].sum()), "cycles": int(cst[:, 3].sum()),
-                "lmtigs": int(cst[:, 4].sum()), "vertices_in_unitigs": int(cst[:, 5].sum())}
+                "lmtigs": int(cst[:, 4].sum()), "vertices_in_unitigs": int(cst[:, 5].sum()),
+                "cdbg": {"maximal_unitigs": n_cdbg, "label_bytes": cdbg_bytes, "device_ms": cdbg_ms,
+                         "rounds": rcd["n_cdbg_rounds"], "cycle_pieces_on_host": rcd["n_cdbg_cycle_pieces"],
+                         "copy_to_host_ms": copy_ms}}
 
     # host buffers in (the packed reads cross PCIe inside the step)
     hip.sync()
@@ -265,6 +285,11 @@ def main():
             "with_contract": {"ms_per_step": ms + ct_ms, "kmers_per_s": n_km / ((ms + ct_ms) / 1000.0),
                               "note": "the step plus Subgraph::contract of every subgraph (unitig labels on "
                                       "the device)"},
+            "to_cdbg": {"ms_per_step": ms + ct_ms + cdbg_ms,
+                        "kmers_per_s": n_km / ((ms + ct_ms + cdbg_ms) / 1000.0),
+                        "with_copy_to_host_ms": ms + ct_ms + cdbg_ms + copy_ms,
+                        "note": "the step, contract and the gluing of lm-tigs into every maximal unitig of the "
+                                "graph, all on the device (the copy of the unitigs to the host apart)"},
             "host_input": {"ms_per_step": ms_host, "kmers_per_s": n_km / (ms_host / 1000.0),
                            "note": "cfgpu_partition from host buffers: includes the H2D copy of the packed reads"},
         },

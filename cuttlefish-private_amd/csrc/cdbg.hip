@@ -40,46 +40,59 @@ constexpr uint64_t HOPS_MAX = (1ull << 24) - 1;
 __device__ __forceinline__ uint32_t ccode(char c) { return c == 'A' ? 0u : c == 'C' ? 1u : c == 'G' ? 2u : 3u; }
 
 // k characters at s as a k-mer (Kmer layout: word 0 holds the last 32 bases),
-// or their reverse complement.
+// f, and its reverse complement, r, in one pass.
+// The characters are read as aligned 4-byte words (the label buffer has at
+// least 4 bytes of slack after its last label).
 template <int KW>
-__device__ __forceinline__ Kmer<KW> enc(const char* s, uint32_t k, bool rc) {
-    Kmer<KW> x;
+__device__ __forceinline__ void enc2(const char* s, uint32_t k, Kmer<KW>& f, Kmer<KW>& r) {
 #pragma unroll
-    for (int w = 0; w < KW; ++w) x.w[w] = 0;
+    for (int w = 0; w < KW; ++w) f.w[w] = r.w[w] = 0;
     const uint32_t kt = k - 32 * (KW - 1);
-    for (uint32_t i = 0; i < k; ++i) {
-        const uint32_t b = rc ? 3u - ccode(s[k - 1 - i]) : ccode(s[i]);
+    const uint32_t* p = reinterpret_cast<const uint32_t*>(reinterpret_cast<uintptr_t>(s) & ~uintptr_t(3));
+    const uint32_t sh = uint32_t(reinterpret_cast<uintptr_t>(s) & 3u);
+    const uint32_t nw = (sh + k + 3) / 4;
+#pragma unroll 4
+    for (uint32_t j = 0; j < nw; ++j) {
+        const uint32_t v = p[j];
 #pragma unroll
-        for (int w = KW - 1; w > 0; --w) x.w[w] = (x.w[w] << 2) | (x.w[w - 1] >> 62);
-        x.w[0] = (x.w[0] << 2) | b;
+        for (uint32_t q = 0; q < 4; ++q) {
+            const uint32_t i = 4 * j + q;  // character i - sh of the k-mer
+            if (i < sh || i >= sh + k) continue;
+            const uint32_t b = ccode(char((v >> (8 * q)) & 0xFFu));
+#pragma unroll
+            for (int w = KW - 1; w > 0; --w) f.w[w] = (f.w[w] << 2) | (f.w[w - 1] >> 62);
+            f.w[0] = (f.w[0] << 2) | b;
+            r = km_prepend(r, uint8_t(3u - b), k);
+        }
     }
-    x.w[KW - 1] &= (1ull << (2 * kt)) - 1;
-    return x;
+    f.w[KW - 1] &= (1ull << (2 * kt)) - 1;
 }
 
 __device__ __forceinline__ uint32_t slot_of(uint64_t h, uint64_t cap) {
     return uint32_t((uint64_t(uint32_t(h >> 32)) * cap) >> 32);
 }
 
-// The vertex and side of end `at` (0 first k-mer, 1 last) of a label: the
-// canonical k-mer and the side facing the label.
-template <int KW>
-__device__ __forceinline__ Kmer<KW> end_vertex(const char* lab, uint32_t len, uint32_t at, uint32_t k, uint32_t& side) {
-    const char* x = at ? lab + len - k : lab;
-    const Kmer<KW> f = enc<KW>(x, k, false), r = enc<KW>(x, k, true);
-    const bool can = km_less(f, r);
-    side = at ? (can ? 0u : 1u) : (can ? 1u : 0u);
-    return can ? f : r;
-}
-
 template <int KW>
 __global__ __launch_bounds__(256) void k_ends_insert(CdbgArgs a) {
+    Kmer<KW>* kend = reinterpret_cast<Kmer<KW>*>(a.kend);
     for (uint64_t r = uint64_t(blockIdx.x) * 256 + threadIdx.x; r < a.n; r += uint64_t(gridDim.x) * 256) {
         const UnitigRec u = a.recs[r];
+        const char* lab = a.labels + u.label_off;
+        Kmer<KW> A0, A1, B0, B1;  // first k characters and their rc; last k characters and their rc
+        enc2<KW>(lab, a.k, A0, A1);
+        enc2<KW>(lab + u.label_len - a.k, a.k, B0, B1);
+        kend[2 * r] = A0;
+        kend[2 * r + 1] = B1;
         for (uint32_t at = 0; at < 2; ++at) {
+            a.end_slot[2 * r + at] = NONE;
             if (!(u.flags & (at ? 4u : 2u))) continue;
-            uint32_t side;
-            const Kmer<KW> x = end_vertex<KW>(a.labels + u.label_off, u.label_len, at, a.k, side);
+            // the end's vertex (canonical k-mer) and the side facing the label:
+            // first k-mer -> back side if canonical; last k-mer -> front side if canonical
+            const bool can = at ? km_less(B0, B1) : km_less(A0, A1);
+            const uint32_t side = at ? (can ? 0u : 1u) : (can ? 1u : 0u);
+            Kmer<KW> x;
+#pragma unroll
+            for (int w = 0; w < KW; ++w) x.w[w] = at ? (can ? B0.w[w] : B1.w[w]) : (can ? A0.w[w] : A1.w[w]);
             uint32_t s = slot_of(km_hash(x), a.hcap);
             // claim a slot: the top key word with BUSY set while the lower words
             // are written (a lane that finds BUSY retries the slot; it never
@@ -87,7 +100,7 @@ __global__ __launch_bounds__(256) void k_ends_insert(CdbgArgs a) {
             const uint64_t top = KW == 1 ? x.w[0] : (x.w[KW - 1] | BUSY);
             for (uint64_t n = 0; n < 2 * a.hcap;) {
                 uint64_t* kp = &a.hkey[uint64_t(s) * KW];
-                uint64_t cur = __hip_atomic_load(kp + KW - 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                uint64_t cur = __hip_atomic_load(kp + KW - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (cur == EMPTY) {
                     cur = atomicCAS(reinterpret_cast<unsigned long long*>(kp + KW - 1), EMPTY, top);
                     if (cur == EMPTY) {
@@ -103,12 +116,14 @@ __global__ __launch_bounds__(256) void k_ends_insert(CdbgArgs a) {
                 }
                 if (KW > 1 && cur == (x.w[KW - 1] | BUSY)) continue;  // being written: look again
                 bool eq = cur == x.w[KW - 1];
+                if (KW > 1 && eq) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // published: its lower words
 #pragma unroll
                 for (int w = 0; w < KW - 1; ++w)
                     eq &= eq && __hip_atomic_load(kp + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == x.w[w];
                 if (eq) {
                     if (atomicCAS(&a.hval[2 * uint64_t(s) + side], NONE, uint32_t(2 * r + at)) != NONE)
                         atomicOr(a.error, 2u);  // two lm-tigs on one side of a vertex
+                    a.end_slot[2 * r + at] = (s << 1) | side;
                     break;
                 }
                 s = s + 1 == a.hcap ? 0 : s + 1;
@@ -120,38 +135,16 @@ __global__ __launch_bounds__(256) void k_ends_insert(CdbgArgs a) {
 
 // State 2r + e: piece r entered at end e.  Its successor is the piece across
 // the vertex at r's other end, entered at its matching end.
-template <int KW>
 __global__ __launch_bounds__(256) void k_rank_init(CdbgArgs a) {
-    for (uint64_t r = uint64_t(blockIdx.x) * 256 + threadIdx.x; r < a.n; r += uint64_t(gridDim.x) * 256) {
-        const UnitigRec u = a.recs[r];
-        for (uint32_t e = 0; e < 2; ++e) {
-            const uint32_t out = 1 - e;  // leaves through its other end
-            uint32_t next = NONE;
-            if (u.flags & (out ? 4u : 2u)) {
-                uint32_t side;
-                const Kmer<KW> x = end_vertex<KW>(a.labels + u.label_off, u.label_len, out, a.k, side);
-                uint32_t s = slot_of(km_hash(x), a.hcap);
-                for (uint64_t n = 0; n < a.hcap; ++n) {
-                    const uint64_t* kp = &a.hkey[uint64_t(s) * KW];
-                    if (kp[KW - 1] == EMPTY) break;
-                    bool eq = true;
-#pragma unroll
-                    for (int w = 0; w < KW; ++w) eq &= kp[w] == x.w[w];
-                    if (eq) {
-                        next = a.hval[2 * uint64_t(s) + (1 - side)];
-                        break;
-                    }
-                    s = s + 1 == a.hcap ? 0 : s + 1;
-                }
-            }
-            const uint64_t st = 2 * r + e;
-            if (next == NONE) {
-                a.jt[0][st] = uint32_t(st);
-                a.jw[0][st] = 0;
-            } else {
-                a.jt[0][st] = next;
-                a.jw[0][st] = uint64_t(a.recs[next >> 1].label_len - a.k) | (1ull << HOPS_SHIFT);
-            }
+    for (uint64_t st = uint64_t(blockIdx.x) * 256 + threadIdx.x; st < 2 * a.n; st += uint64_t(gridDim.x) * 256) {
+        const uint32_t es = a.end_slot[st ^ 1];  // the end it leaves through
+        const uint32_t next = es == NONE ? NONE : a.hval[2 * uint64_t(es >> 1) + (1u - (es & 1u))];
+        if (next == NONE) {
+            a.jt[0][st] = uint32_t(st);
+            a.jw[0][st] = 0;
+        } else {
+            a.jt[0][st] = next;
+            a.jw[0][st] = uint64_t(a.recs[next >> 1].label_len - a.k) | (1ull << HOPS_SHIFT);
         }
     }
 }
@@ -206,14 +199,11 @@ __global__ __launch_bounds__(256) void k_place(CdbgArgs a, uint32_t src) {
                 if (!(u.flags & 1u)) {
                     // first piece in that reading: the end state of direction 1,
                     // read from its unpaired end; last piece: the end of direction 0
+                    // kend[2j] = first k characters of piece j, kend[2j+1] = rc of its last k
+                    const Kmer<KW>* kend = reinterpret_cast<const Kmer<KW>*>(a.kend);
                     const uint32_t jf = E1 >> 1, af = 1u - (E1 & 1u), jl = E0 >> 1, al = E0 & 1u;
-                    const UnitigRec uf = a.recs[jf], ul = a.recs[jl];
-                    const char* lf = a.labels + uf.label_off;
-                    const char* ll = a.labels + ul.label_off;
-                    const Kmer<KW> first = af ? enc<KW>(lf + uf.label_len - a.k, a.k, true) : enc<KW>(lf, a.k, false);
-                    // reverse complement of the last k characters
-                    const Kmer<KW> rlast =
-                        al ? enc<KW>(ll, a.k, false) : enc<KW>(ll + ul.label_len - a.k, a.k, true);
+                    const Kmer<KW> first = kend[2 * uint64_t(jf) + af];
+                    const Kmer<KW> rlast = kend[2 * uint64_t(jl) + (1u - al)];  // rc of the last k characters
                     fwd = km_less(first, rlast);
                 }
                 const uint32_t hd = fwd ? (E1 >> 1) : (E0 >> 1);
@@ -253,9 +243,10 @@ __global__ __launch_bounds__(256) void k_place(CdbgArgs a, uint32_t src) {
     }
 }
 
-// One wave per 64 pieces, each lane copying its own piece.
+// Eight lanes per piece, each copying every eighth character.
 __global__ __launch_bounds__(256) void k_copy(CdbgArgs a) {
-    for (uint64_t r = uint64_t(blockIdx.x) * 256 + threadIdx.x; r < a.n; r += uint64_t(gridDim.x) * 256) {
+    const uint32_t j = threadIdx.x & 7u;
+    for (uint64_t r = (uint64_t(blockIdx.x) * 256 + threadIdx.x) >> 3; r < a.n; r += (uint64_t(gridDim.x) * 256) >> 3) {
         const uint32_t hd = a.place_head[r];
         if (hd == NONE) continue;
         const UnitigRec u = a.recs[r];
@@ -263,16 +254,15 @@ __global__ __launch_bounds__(256) void k_copy(CdbgArgs a) {
         const bool rev = a.place_pos[r] >> 63;
         const uint64_t hoff = a.head_off[hd];  // all-ones unless the head placed its chain
         if (hoff >= a.out_cap || pos >= a.out_cap - hoff || a.out_cap - hoff - pos < u.label_len) {
-            atomicOr(a.error, 4u);
+            if (j == 0) atomicOr(a.error, 4u);
             continue;
         }
-        const uint64_t off = hoff + pos;
         const char* src = a.labels + u.label_off;
-        char* dst = a.out + off;
+        char* dst = a.out + hoff + pos;
         if (!rev) {
-            for (uint32_t i = 0; i < u.label_len; ++i) dst[i] = src[i];
+            for (uint32_t i = j; i < u.label_len; i += 8) dst[i] = src[i];
         } else {
-            for (uint32_t i = 0; i < u.label_len; ++i) {
+            for (uint32_t i = j; i < u.label_len; i += 8) {
                 const char c = src[u.label_len - 1 - i];
                 dst[i] = c == 'A' ? 'T' : c == 'C' ? 'G' : c == 'G' ? 'C' : 'A';
             }
@@ -289,7 +279,7 @@ template <int KW>
 void launch_kw(const CdbgArgs& a, uint32_t grid, int phase, uint32_t src, uint32_t* changed, hipStream_t s) {
     switch (phase) {
         case 0: hipLaunchKernelGGL(k_ends_insert<KW>, dim3(grid), dim3(256), 0, s, a); break;
-        case 1: hipLaunchKernelGGL(k_rank_init<KW>, dim3(grid), dim3(256), 0, s, a); break;
+        case 1: hipLaunchKernelGGL(k_rank_init, dim3(grid), dim3(256), 0, s, a); break;
         case 2: hipLaunchKernelGGL(k_rank_round, dim3(grid), dim3(256), 0, s, a, src, changed); break;
         case 3: hipLaunchKernelGGL(k_place<KW>, dim3(grid), dim3(256), 0, s, a, src); break;
         default: hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, a); break;

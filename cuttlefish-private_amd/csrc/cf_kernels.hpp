@@ -204,6 +204,8 @@ struct CdbgArgs {
     uint64_t* hkey;             // hcap * KW words, all-ones = free
     uint32_t* hval;             // 2 per slot: the end on the front / back side (all-ones = none)
     uint64_t hcap;
+    uint32_t* end_slot;         // per piece end: hash slot << 1 | side, all-ones if it does not exit
+    uint64_t* kend;             // per piece: its first k characters and the rc of its last k (KW words each)
     uint32_t* jt[2];            // per state: the state it points at
     uint64_t* jw[2];            //   ... label characters | pieces << 40 up to there
     uint32_t* place_head;       // per piece: first piece of its chain (all-ones: on a cycle)

@@ -126,8 +126,8 @@ struct cfgpu_handle {
 
     // compacted graph: glue table, list-ranking states, output; cycles through
     // several subgraphs finished on the host (appended after the device part)
-    DevBuf cd_hkey, cd_hval, cd_jt0, cd_jt1, cd_jw0, cd_jw1, cd_head, cd_pos, cd_hoff, cd_out, cd_rec, cd_ctl,
-        cd_cyc;
+    DevBuf cd_hkey, cd_hval, cd_eslot, cd_kend, cd_jt0, cd_jt1, cd_jw0, cd_jw1, cd_head, cd_pos, cd_hoff, cd_out,
+        cd_rec, cd_ctl, cd_cyc;
     uint64_t cd_n = 0, cd_bytes = 0;
     std::string cd_host_seq;
     std::vector<uint64_t> cd_host_rec;
@@ -1087,7 +1087,7 @@ int cfgpu_contract(cfgpu_handle* h) {
     for (;;) {
         ++run;
         HIPCK(h, h->ct_recs.ensure(rec_cap * sizeof(cfgpu_unitig)));
-        HIPCK(h, h->ct_labels.ensure(lab_cap));
+        HIPCK(h, h->ct_labels.ensure(lab_cap + 64));  // slack: labels are read as aligned words
         launch_fill_u32(h->ct_counts.as<uint32_t>(), 0u, uint64_t(h->G) * 8, h->stream);
         launch_fill_u32(h->ct_ctl.as<uint32_t>(), 0u, 16, h->stream);
         ContractArgs a{};
@@ -1157,6 +1157,8 @@ int cfgpu_cdbg(cfgpu_handle* h) {
         const uint64_t hcap = round_cap(4.0 * double(n));
         HIPCK(h, h->cd_hkey.ensure(hcap * KW * 8));
         HIPCK(h, h->cd_hval.ensure(hcap * 8));
+        HIPCK(h, h->cd_eslot.ensure(2 * n * 4));
+        HIPCK(h, h->cd_kend.ensure(2 * n * KW * 8));
         HIPCK(h, h->cd_jt0.ensure(2 * n * 4));
         HIPCK(h, h->cd_jt1.ensure(2 * n * 4));
         HIPCK(h, h->cd_jw0.ensure(2 * n * 8));
@@ -1180,6 +1182,8 @@ int cfgpu_cdbg(cfgpu_handle* h) {
         a.hkey = h->cd_hkey.as<uint64_t>();
         a.hval = h->cd_hval.as<uint32_t>();
         a.hcap = hcap;
+        a.end_slot = h->cd_eslot.as<uint32_t>();
+        a.kend = h->cd_kend.as<uint64_t>();
         a.jt[0] = h->cd_jt0.as<uint32_t>();
         a.jt[1] = h->cd_jt1.as<uint32_t>();
         a.jw[0] = h->cd_jw0.as<uint64_t>();
