@@ -16,7 +16,7 @@
 //           the partner end;
 //   rank    directed states (piece, end it is entered at) point at the next
 //           piece of the chain; pointer jumping gives every state the last state
-//           of its chain, the pieces in between and their label length;
+//           of its chain and the label characters of the pieces up to it;
 //   place   a piece knows both chain ends, hence the orientation the chain is
 //           written in (first k characters at most the reverse complement's,
 //           include/Unitig_Collator.hpp:306-345), its offset in the chain and
@@ -34,8 +34,11 @@ namespace {
 constexpr uint64_t EMPTY = ~0ull;
 constexpr uint64_t BUSY = 1ull << 63;  // never set in a key word: top words hold < 63 bits
 constexpr uint32_t NONE = 0xFFFFFFFFu;
-constexpr uint32_t HOPS_SHIFT = 40;  // state word: label characters | pieces << 40
-constexpr uint64_t HOPS_MAX = (1ull << 24) - 1;
+// list-ranking word: the state pointed at (low 30 bits) | label characters up
+// to it (high 34 bits, saturating)
+constexpr uint32_t JT_BITS = 30;
+constexpr uint64_t JT_MASK = (1ull << JT_BITS) - 1, JC_MAX = (1ull << (64 - JT_BITS)) - 1;
+__device__ __forceinline__ uint64_t jword(uint64_t t, uint64_t c) { return t | (min(c, JC_MAX) << JT_BITS); }
 
 __device__ __forceinline__ uint32_t ccode(char c) { return c == 'A' ? 0u : c == 'C' ? 1u : c == 'G' ? 2u : 3u; }
 
@@ -139,37 +142,26 @@ __global__ __launch_bounds__(256) void k_rank_init(CdbgArgs a) {
     for (uint64_t st = uint64_t(blockIdx.x) * 256 + threadIdx.x; st < 2 * a.n; st += uint64_t(gridDim.x) * 256) {
         const uint32_t es = a.end_slot[st ^ 1];  // the end it leaves through
         const uint32_t next = es == NONE ? NONE : a.hval[2 * uint64_t(es >> 1) + (1u - (es & 1u))];
-        if (next == NONE) {
-            a.jt[0][st] = uint32_t(st);
-            a.jw[0][st] = 0;
-        } else {
-            a.jt[0][st] = next;
-            a.jw[0][st] = uint64_t(a.recs[next >> 1].label_len - a.k) | (1ull << HOPS_SHIFT);
-        }
+        a.jp[st] = next == NONE ? jword(st, 0) : jword(next, a.recs[next >> 1].label_len - a.k);
     }
 }
 
-__global__ __launch_bounds__(256) void k_rank_round(CdbgArgs a, uint32_t src, uint32_t* changed) {
-    const uint32_t* jt = a.jt[src];
-    const uint64_t* jw = a.jw[src];
-    uint32_t* nt = a.jt[src ^ 1];
-    uint64_t* nw = a.jw[src ^ 1];
+// One pointer-jumping pass, in place: a state's word is replaced by the
+// combination with the word of the state it points at.  Words are read and
+// written whole, so every word always names a state further along the chain
+// with the characters up to it, whatever other states have already jumped.
+__global__ __launch_bounds__(256) void k_rank_round(CdbgArgs a, uint32_t* changed) {
     bool any = false;
     for (uint64_t st = uint64_t(blockIdx.x) * 256 + threadIdx.x; st < 2 * a.n; st += uint64_t(gridDim.x) * 256) {
-        const uint32_t t = jt[st];
-        const uint64_t w = jw[st];
-        if (t == uint32_t(st)) {
-            nt[st] = t;
-            nw[st] = w;
-            continue;
-        }
-        const uint32_t t2 = jt[t];
-        const uint64_t w2 = jw[t];
-        const uint64_t hops = min((w >> HOPS_SHIFT) + (w2 >> HOPS_SHIFT), HOPS_MAX);
-        const uint64_t chars = (w & ((1ull << HOPS_SHIFT) - 1)) + (w2 & ((1ull << HOPS_SHIFT) - 1));
-        nt[st] = t2;
-        nw[st] = (chars & ((1ull << HOPS_SHIFT) - 1)) | (hops << HOPS_SHIFT);
-        any |= t2 != t && hops < a.n;  // chains settle; cycles stop counting past n pieces
+        const uint64_t w = __hip_atomic_load(&a.jp[st], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t t = w & JT_MASK;
+        if (t == st) continue;  // chain end
+        const uint64_t w2 = __hip_atomic_load(&a.jp[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t t2 = w2 & JT_MASK;
+        if (t2 == t) continue;  // points at a chain end already
+        const uint64_t c = (w >> JT_BITS) + (w2 >> JT_BITS);
+        __hip_atomic_store(&a.jp[st], jword(t2, c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        any |= c < a.total_chars;  // chains settle; cycles stop counting past every character
     }
     if (any) atomicOr(changed, 1u);
 }
@@ -177,9 +169,8 @@ __global__ __launch_bounds__(256) void k_rank_round(CdbgArgs a, uint32_t src, ui
 // Per piece: chain ends, orientation, offset and length of its chain; the
 // first piece of each chain reserves the output (wave-aggregated).
 template <int KW>
-__global__ __launch_bounds__(256) void k_place(CdbgArgs a, uint32_t src) {
-    const uint32_t* jt = a.jt[src];
-    const uint64_t* jw = a.jw[src];
+__global__ __launch_bounds__(256) void k_place(CdbgArgs a) {
+    const uint64_t* jp = a.jp;
     const int lane = int(threadIdx.x & 63u);
     for (uint64_t r0 = uint64_t(blockIdx.x) * 256; r0 < a.n; r0 += uint64_t(gridDim.x) * 256) {
         const uint64_t r = r0 + threadIdx.x;
@@ -187,14 +178,13 @@ __global__ __launch_bounds__(256) void k_place(CdbgArgs a, uint32_t src) {
         uint64_t len = 0;
         if (r < a.n) {
             const UnitigRec u = a.recs[r];
-            const uint32_t E0 = jt[2 * r], E1 = jt[2 * r + 1];
-            const uint64_t w0 = jw[2 * r], w1 = jw[2 * r + 1];
-            if (jt[E0] != E0 || jw[E0] != 0 || jt[E1] != E1 || jw[E1] != 0) {  // on a cycle
+            const uint64_t w0 = jp[2 * r], w1 = jp[2 * r + 1];
+            const uint64_t E0 = w0 & JT_MASK, E1 = w1 & JT_MASK;
+            if (jp[E0] != E0 || jp[E1] != E1) {  // not chain ends (word = itself, 0 characters): a cycle
                 a.place_head[r] = NONE;
                 atomicAdd(a.n_cycle_pieces, 1u);
             } else {
-                const uint64_t c0 = w0 & ((1ull << HOPS_SHIFT) - 1), c1 = w1 & ((1ull << HOPS_SHIFT) - 1);
-                const uint64_t h0 = w0 >> HOPS_SHIFT, h1 = w1 >> HOPS_SHIFT;
+                const uint64_t c0 = w0 >> JT_BITS, c1 = w1 >> JT_BITS;
                 bool fwd = true;  // read the chain with this piece forward
                 if (!(u.flags & 1u)) {
                     // first piece in that reading: the end state of direction 1,
@@ -206,10 +196,10 @@ __global__ __launch_bounds__(256) void k_place(CdbgArgs a, uint32_t src) {
                     const Kmer<KW> rlast = kend[2 * uint64_t(jl) + (1u - al)];  // rc of the last k characters
                     fwd = km_less(first, rlast);
                 }
-                const uint32_t hd = fwd ? (E1 >> 1) : (E0 >> 1);
+                const uint32_t hd = uint32_t(fwd ? (E1 >> 1) : (E0 >> 1));
                 a.place_head[r] = hd;
                 a.place_pos[r] = (fwd ? c1 : c0) | (uint64_t(fwd ? 0u : 1u) << 63);
-                head = (fwd ? h1 : h0) == 0;
+                head = fwd ? E1 == 2 * r + 1 : E0 == 2 * r;  // nothing before it in that reading
                 len = c0 + c1 + u.label_len;
             }
         }
@@ -280,8 +270,8 @@ void launch_kw(const CdbgArgs& a, uint32_t grid, int phase, uint32_t src, uint32
     switch (phase) {
         case 0: hipLaunchKernelGGL(k_ends_insert<KW>, dim3(grid), dim3(256), 0, s, a); break;
         case 1: hipLaunchKernelGGL(k_rank_init, dim3(grid), dim3(256), 0, s, a); break;
-        case 2: hipLaunchKernelGGL(k_rank_round, dim3(grid), dim3(256), 0, s, a, src, changed); break;
-        case 3: hipLaunchKernelGGL(k_place<KW>, dim3(grid), dim3(256), 0, s, a, src); break;
+        case 2: hipLaunchKernelGGL(k_rank_round, dim3(grid), dim3(256), 0, s, a, changed); break;
+        case 3: hipLaunchKernelGGL(k_place<KW>, dim3(grid), dim3(256), 0, s, a); break;
         default: hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, a); break;
     }
 }

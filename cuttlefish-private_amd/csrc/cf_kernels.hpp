@@ -194,8 +194,8 @@ void launch_contract(const ContractArgs& a, uint32_t KW, uint32_t grid_lds, uint
 // ---------------------------------------------------------------------------
 // The compacted graph (cdbg.hip): contract's unitigs glued into the maximal
 // unitigs of the whole graph.  Phases 0 ends into the hash table, 1 chain
-// successors, 2 one pointer-jumping round from jt/jw[src] (sets *changed),
-// 3 placement, 4 label copies.
+// successors, 2 one pointer-jumping pass (sets *changed), 3 placement, 4 label
+// copies.
 struct CdbgArgs {
     const UnitigRec* recs;
     uint64_t n;
@@ -206,8 +206,8 @@ struct CdbgArgs {
     uint64_t hcap;
     uint32_t* end_slot;         // per piece end: hash slot << 1 | side, all-ones if it does not exit
     uint64_t* kend;             // per piece: its first k characters and the rc of its last k (KW words each)
-    uint32_t* jt[2];            // per state: the state it points at
-    uint64_t* jw[2];            //   ... label characters | pieces << 40 up to there
+    uint64_t* jp;               // per state: the state it points at | label characters up to it << 30
+    uint64_t total_chars;       // label bytes of all pieces
     uint32_t* place_head;       // per piece: first piece of its chain (all-ones: on a cycle)
     uint64_t* place_pos;        //   ... its offset in the chain | reversed << 63
     uint64_t* head_off;         // per chain head: output offset

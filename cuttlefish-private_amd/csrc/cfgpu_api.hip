@@ -126,8 +126,7 @@ struct cfgpu_handle {
 
     // compacted graph: glue table, list-ranking states, output; cycles through
     // several subgraphs finished on the host (appended after the device part)
-    DevBuf cd_hkey, cd_hval, cd_eslot, cd_kend, cd_jt0, cd_jt1, cd_jw0, cd_jw1, cd_head, cd_pos, cd_hoff, cd_out,
-        cd_rec, cd_ctl, cd_cyc;
+    DevBuf cd_hkey, cd_hval, cd_eslot, cd_kend, cd_jp, cd_head, cd_pos, cd_hoff, cd_out, cd_rec, cd_ctl, cd_cyc;
     uint64_t cd_n = 0, cd_bytes = 0;
     std::string cd_host_seq;
     std::vector<uint64_t> cd_host_rec;
@@ -1153,16 +1152,13 @@ int cfgpu_cdbg(cfgpu_handle* h) {
     uint32_t ctl[16] = {};
     uint32_t rounds = 0;
     if (n) {
-        if (2 * n >= 0xFFFFFFFFull) return fail(h, CFGPU_E_ARG, "cdbg: %llu unitigs", (unsigned long long)n);
+        if (2 * n >= (1ull << 30)) return fail(h, CFGPU_E_ARG, "cdbg: %llu unitigs", (unsigned long long)n);
         const uint64_t hcap = round_cap(4.0 * double(n));
         HIPCK(h, h->cd_hkey.ensure(hcap * KW * 8));
         HIPCK(h, h->cd_hval.ensure(hcap * 8));
         HIPCK(h, h->cd_eslot.ensure(2 * n * 4));
         HIPCK(h, h->cd_kend.ensure(2 * n * KW * 8));
-        HIPCK(h, h->cd_jt0.ensure(2 * n * 4));
-        HIPCK(h, h->cd_jt1.ensure(2 * n * 4));
-        HIPCK(h, h->cd_jw0.ensure(2 * n * 8));
-        HIPCK(h, h->cd_jw1.ensure(2 * n * 8));
+        HIPCK(h, h->cd_jp.ensure(2 * n * 8));
         HIPCK(h, h->cd_head.ensure(n * 4));
         HIPCK(h, h->cd_pos.ensure(n * 8));
         HIPCK(h, h->cd_hoff.ensure(n * 8));
@@ -1184,10 +1180,8 @@ int cfgpu_cdbg(cfgpu_handle* h) {
         a.hcap = hcap;
         a.end_slot = h->cd_eslot.as<uint32_t>();
         a.kend = h->cd_kend.as<uint64_t>();
-        a.jt[0] = h->cd_jt0.as<uint32_t>();
-        a.jt[1] = h->cd_jt1.as<uint32_t>();
-        a.jw[0] = h->cd_jw0.as<uint64_t>();
-        a.jw[1] = h->cd_jw1.as<uint64_t>();
+        a.jp = h->cd_jp.as<uint64_t>();
+        a.total_chars = h->n_ubytes;
         a.place_head = h->cd_head.as<uint32_t>();
         a.place_pos = h->cd_pos.as<uint64_t>();
         a.head_off = h->cd_hoff.as<uint64_t>();
@@ -1202,19 +1196,17 @@ int cfgpu_cdbg(cfgpu_handle* h) {
         const uint32_t grid = uint32_t(std::min<uint64_t>((2 * n + 255) / 256, uint64_t(h->cus) * 8));
         launch_cdbg_phase(a, KW, grid, 0, 0, nullptr, h->stream);
         launch_cdbg_phase(a, KW, grid, 1, 0, nullptr, h->stream);
-        uint32_t src = 0;
         for (rounds = 0; rounds < 64;) {
             HIPCK(h, hipMemsetAsync(c + 5, 0, 4, h->stream));
-            launch_cdbg_phase(a, KW, grid, 2, src, c + 5, h->stream);
+            launch_cdbg_phase(a, KW, grid, 2, 0, c + 5, h->stream);
             uint32_t changed = 0;
             HIPCK(h, hipMemcpyAsync(&changed, c + 5, 4, hipMemcpyDeviceToHost, h->stream));
             HIPCK(h, hipStreamSynchronize(h->stream));
-            src ^= 1;
             ++rounds;
             if (!changed) break;
         }
-        launch_cdbg_phase(a, KW, grid, 3, src, nullptr, h->stream);
-        launch_cdbg_phase(a, KW, grid, 4, src, nullptr, h->stream);
+        launch_cdbg_phase(a, KW, grid, 3, 0, nullptr, h->stream);
+        launch_cdbg_phase(a, KW, grid, 4, 0, nullptr, h->stream);
         HIPCK(h, hipGetLastError());
         HIPCK(h, hipMemcpyAsync(ctl, c, sizeof ctl, hipMemcpyDeviceToHost, h->stream));
         HIPCK(h, hipStreamSynchronize(h->stream));
