@@ -3,7 +3,7 @@
  * minimizer-partitioned subgraph construction.
  *
  * It replaces, behind the reference's C++ class contract, this call sequence of
- * dBG_Contractor::construct (src/dBG_Contractor.cpp:46-61):
+ * dBG_Contractor::construct (src/dBG_Contractor.cpp:43-66, 75-113):
  *
  *   Graph_Partitioner<k,...>(G, logistics, l).partition();   -> cfgpu_partition
  *   G.finalize();                                             -> (buckets stay in HBM)
@@ -11,6 +11,9 @@
  *                        b.remove()                            -> cfgpu_release
  *                        contract() reads the table            -> cfgpu_table_view_get /
  *                                                                 cfgpu_table_copy
+ *                        contract() itself                     -> cfgpu_contract
+ *   Discontinuity_Graph_Contractor::contract, Contracted_Graph_Expander::expand,
+ *   Unitig_Collator::collate                                   -> cfgpu_cdbg
  *
  * Conventions: every function returns CFGPU_OK (0) or a negative CFGPU_E_* code and
  * never throws; the message of the last failure is available from cfgpu_last_error.
