@@ -7,12 +7,13 @@
 //   --read | --ref     FASTQ reads / FASTA references
 //   -c/--cutoff C      (k+1)-mer frequency cutoff (default reads 2, refs 1)
 //   -t/--threads T     host threads for the ingest
-//   -o/--output PREFIX output prefix: writes PREFIX.states (binary vertex states)
-// Extra: --mode plain|batched|learned (default learned), --device N.
+//   -o/--output PREFIX output prefix: writes PREFIX.fa, the maximal unitigs
+//                      (src/Build_Params.cpp:72-75, include/File_Extensions.hpp:16)
+// Extra: --mode plain|batched|learned (default learned), --device N, --states (with -o:
+// also PREFIX.states, every (g, canonical k-mer, Edge_Frequency, status) row).
 //
 // Output: the reference's per-subgraph statistics on stderr
-// (src/Subgraphs_Manager.cpp:264-305) and, with -o, every (g, canonical k-mer,
-// Edge_Frequency, status) row.  Exit status EXIT_FAILURE on any error, as the
+// (src/Subgraphs_Manager.cpp:264-305).  Exit status EXIT_FAILURE on any error, as the
 // reference.
 #include <cstdio>
 #include <cstdlib>
@@ -28,7 +29,7 @@ namespace {
 
 void usage() {
     std::cerr << "usage: cf_build (-s FILE... | -l LIST) -k K [--min-len L] (--read|--ref) [-c C] "
-                 "[-t T] [-o PREFIX] [--mode plain|batched|learned] [--device N]\n";
+                 "[-t T] [-o PREFIX [--states]] [--mode plain|batched|learned] [--device N]\n";
 }
 
 }  // namespace
@@ -37,7 +38,7 @@ int main(int argc, char** argv) {
     std::vector<std::string> seqs;
     unsigned k = 27, l = 20, threads = 0;
     int cutoff = -1, device = 0;
-    bool read = false, ref = false;
+    bool read = false, ref = false, states = false;
     std::string out, mode = "learned";
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
@@ -72,6 +73,8 @@ int main(int argc, char** argv) {
             ref = true;
         } else if (a == "--mode") {
             mode = need("--mode");
+        } else if (a == "--states") {
+            states = true;
         } else if (a == "--device") {
             device = std::stoi(need("--device"));
         } else if (a == "-h" || a == "--help") {
@@ -97,8 +100,13 @@ int main(int argc, char** argv) {
         cuttlefish_amd::Subgraphs_Manager G(uint16_t(k), uint16_t(l), uint8_t(cutoff), device);
         G.partition(seqs, read, threads);
         G.finalize();
-        std::FILE* dump = nullptr;
+        G.construct(m);
         if (!out.empty()) {
+            const uint64_t n = G.contract_and_collate(out + ".fa");
+            std::cerr << "Maximal unitig count: " << n << ".\n";
+        }
+        std::FILE* dump = nullptr;
+        if (!out.empty() && states) {
             dump = std::fopen((out + ".states").c_str(), "wb");
             if (!dump) { std::cerr << "cannot write " << out << ".states\n"; return EXIT_FAILURE; }
         }

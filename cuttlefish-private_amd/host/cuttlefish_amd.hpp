@@ -18,6 +18,7 @@
 #pragma once
 
 #include <cstdint>
+#include <cstdio>
 #include <functional>
 #include <stdexcept>
 #include <string>
@@ -142,6 +143,7 @@ public:
 
     // Graph_Partitioner::partition over the input files (one FASTA/FASTQ group).
     void partition(const std::vector<std::string>& paths, bool is_fastq, unsigned threads = 0) {
+        constructed_ = false;
         std::vector<uint64_t> packed, frag{0};
         uint64_t bases = 0;
         for (const std::string& p : paths) {
@@ -172,12 +174,43 @@ public:
         return m;
     }
 
-    // Subgraphs_Manager::process: construct every subgraph on the GPU, then hand
-    // each non-empty one to `contract` (the reference calls Subgraph::contract).
-    void process(const std::function<void(Subgraph&)>& contract,
-                 uint32_t mode = CFGPU_MODE_BATCHED_LEARNED) {
+    // Every Subgraph::construct on the GPU (src/Subgraphs_Manager.cpp:147-160).
+    void construct(uint32_t mode = CFGPU_MODE_BATCHED_LEARNED) {
         check(cfgpu_construct(h_, mode));
         check(cfgpu_bucket_stats(h_, stats_.data(), uint32_t(graph_count())));
+        constructed_ = true;
+    }
+
+    // Every Subgraph::contract, then the discontinuity-graph contraction, expansion and
+    // collation (src/dBG_Contractor.cpp:75-113), on the GPU; the maximal unitigs are
+    // written to `fasta_path` as one FASTA record each.  Returns their count.
+    uint64_t contract_and_collate(const std::string& fasta_path) {
+        if (!constructed_) construct();
+        check(cfgpu_contract(h_));
+        check(cfgpu_cdbg(h_));
+        uint64_t n = 0, nb = 0;
+        check(cfgpu_cdbg_copy(h_, nullptr, 0, nullptr, 0, &n, &nb));
+        std::string seq(nb, '\0');
+        std::vector<uint64_t> rec(2 * n);
+        check(cfgpu_cdbg_copy(h_, &seq[0], nb, rec.data(), n, &n, &nb));
+        std::FILE* f = std::fopen(fasta_path.c_str(), "wb");
+        if (!f) throw std::runtime_error("cannot write " + fasta_path);
+        for (uint64_t i = 0; i < n; ++i) {
+            std::fprintf(f, ">%llu\n", static_cast<unsigned long long>(i));
+            std::fwrite(seq.data() + rec[2 * i], 1, rec[2 * i + 1], f);
+            std::fputc('\n', f);
+        }
+        if (std::fclose(f) != 0) throw std::runtime_error("cannot write " + fasta_path);
+        return n;
+    }
+
+    // Subgraphs_Manager::process: construct every subgraph on the GPU (unless already
+    // constructed), then hand each non-empty one to `contract` (the reference calls
+    // Subgraph::contract) and release it.
+    void process(const std::function<void(Subgraph&)>& contract,
+                 uint32_t mode = CFGPU_MODE_BATCHED_LEARNED) {
+        if (!constructed_) construct(mode);
+        constructed_ = false;
         const unsigned kw = (k_ + 31) / 32;
         for (uint32_t g = 0; g < graph_count(); ++g) {
             if (!stats_[g].n_records) continue;
@@ -231,6 +264,7 @@ private:
     uint16_t k_, l_;
     uint8_t cutoff_;
     cfgpu_handle* h_ = nullptr;
+    bool constructed_ = false;
     std::vector<cfgpu_bucket_stat> stats_;
 };
 

@@ -112,3 +112,22 @@ def test_cdbg_pangenome_k127():
     _, _, want = oracle_cdbg(buf, False, 127, 20, 1)
     got = gpu_cdbg(buf, False, 127, 20, 1)
     assert got == want
+
+
+def test_cf_build_cli_writes_unitigs(tmp_path):
+    # the `cuttlefish build` front end: -o PREFIX writes PREFIX.fa, one record per maximal unitig
+    import os
+    import subprocess
+    rng = np.random.default_rng(97)
+    buf = mixed_input(rng)
+    (tmp_path / "in.fa").write_bytes(buf)
+    exe = os.path.join(os.path.dirname(C.LIB_PATH), "cf_build")
+    say("[case] cf_build CLI")
+    r = subprocess.run([exe, "-s", str(tmp_path / "in.fa"), "-k", "31", "--ref", "-o", str(tmp_path / "out")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = (tmp_path / "out.fa").read_text().split("\n")
+    got = sorted(lines[i + 1] for i in range(0, len(lines) - 1, 2))
+    assert all(lines[i].startswith(">") for i in range(0, len(lines) - 1, 2))
+    seq, ro = O.parse(buf, False)
+    assert got == U.unitigs(seq, ro, 31, 1)
