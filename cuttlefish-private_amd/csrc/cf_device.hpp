@@ -24,15 +24,28 @@ struct WyConst {
     uint64_t s0, s1, seed0;
 };
 
+// Full 64x64 -> 128-bit product from four 32x32+64 multiply-adds
+// (v_mad_u64_u32); no partial sum overflows 64 bits.
+__device__ __forceinline__ void mul128(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
+    const uint64_t a0 = uint32_t(a), a1 = a >> 32, b0 = uint32_t(b), b1 = b >> 32;
+    const uint64_t p00 = a0 * b0;
+    const uint64_t t = a0 * b1 + (p00 >> 32);
+    const uint64_t u = a1 * b0 + uint32_t(t);
+    lo = (u << 32) | uint32_t(p00);
+    hi = a1 * b1 + (t >> 32) + (u >> 32);
+}
+
 __device__ __forceinline__ uint64_t lmer_hash(uint64_t x, const WyConst& c) {
     // wyhash(&x, 8, 0, salt) (include/wyhash/wyhash.h:118-142) specialised to
     // len == 8: a = rot32(x), b = x, then two 64x64->128 multiply-folds.
     uint64_t a = ((x << 32) | (x >> 32)) ^ c.s1;
     uint64_t b = x ^ c.seed0;
-    uint64_t lo = a * b, hi = __umul64hi(a, b);
+    uint64_t lo, hi;
+    mul128(a, b, lo, hi);
     a = lo ^ c.s0 ^ 8ull;
     b = hi ^ c.s1;
-    return (a * b) ^ __umul64hi(a, b);
+    mul128(a, b, lo, hi);
+    return lo ^ hi;
 }
 
 // Reverse the order of the 32 2-bit groups of a word.

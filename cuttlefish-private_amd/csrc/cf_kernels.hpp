@@ -20,6 +20,7 @@ struct PartitionArgs {
     uint64_t n_bases;
     const uint64_t* frag_off;  // n_frag + 1 base offsets, frag_off[n_frag] == n_bases
     uint64_t n_frag;
+    uint64_t* tile_frag;       // per minimizer tile (ceil(n_bases / P_TILE)): its first fragment
     uint32_t k, l, w;          // w = k - l: (k-1)-mers per piece at most
     uint32_t g_mask, g_bits;   // G - 1, log2 G
     WyConst wy;

@@ -90,7 +90,7 @@ struct cfgpu_handle {
     cfgpu_tuning tune{0.6f, 256, 2048, 8, 1.15f, 1, 32, 0, 1};
 
     // partition state
-    DevBuf packed, frag, gbin, tile_lc, boff, att, labels, mbin, stats, feat;
+    DevBuf packed, frag, gbin, tile_lc, tile_frag, boff, att, labels, mbin, stats, feat;
     DevBuf hist, ctot, coff, rec1;  // level-1 bucketing
     uint64_t n_bases = 0, n_frag = 0, n_records = 0;
     bool partitioned = false;
@@ -881,6 +881,7 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     const PieceSpans ps = pieces_spans(n_bases, uint32_t(h->cus));
     HIPCK(h, h->gbin.ensure(n_bases * 4));
     HIPCK(h, h->tile_lc.ensure(tiles * 8));
+    HIPCK(h, h->tile_frag.ensure(tiles * 8));
     HIPCK(h, h->hist.ensure(size_t(N_COARSE) * ps.n_count * 4));
     HIPCK(h, h->ctot.ensure(size_t(N_COARSE) * 8));
     HIPCK(h, h->coff.ensure(size_t(N_COARSE + 1) * 8));
@@ -893,6 +894,7 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     a.n_words = n_words;
     a.n_bases = n_bases;
     a.frag_off = d_frag;
+    a.tile_frag = h->tile_frag.as<uint64_t>();
     a.n_frag = n_frag;
     a.k = k;
     a.l = h->prm.l;

@@ -32,14 +32,28 @@ constexpr int NWORDS = (P_TILE + 2 * WIN_MAX) / 32 + 4;
 constexpr int S_MAX = P_TILE + WIN_MAX;           // l-mer hashes per tile
 constexpr int PER_T = (S_MAX + P_WG - 1) / P_WG;
 
+// tile_frag[t] = the fragment holding position max(t * P_TILE - 1, 0): the
+// largest f with frag_off[f] <= that position.  Fragment f covers the tiles
+// whose position falls in [frag_off[f], frag_off[f+1]) (frag_off[0] = 0,
+// strictly increasing), so every tile is written exactly once.
+__global__ __launch_bounds__(256) void k_tile_frag(const uint64_t* __restrict__ frag_off, uint64_t n_frag,
+                                                   uint64_t tiles, uint64_t* __restrict__ tile_frag) {
+    for (uint64_t f = uint64_t(blockIdx.x) * 256 + threadIdx.x; f < n_frag; f += uint64_t(gridDim.x) * 256) {
+        const uint64_t b = frag_off[f], e = frag_off[f + 1];
+        // tiles t >= 1 with t * P_TILE - 1 in [b, e); tile 0 (position 0) belongs to f = 0
+        const uint64_t t0 = f == 0 ? 0 : (b + P_TILE) / P_TILE;  // ceil((b + 1) / P_TILE)
+        const uint64_t t1 = min((e + P_TILE) / P_TILE, tiles);   // ceil((e + 1) / P_TILE)
+        for (uint64_t t = t0; t < t1; ++t) tile_frag[t] = f;
+    }
+}
+
 __global__ __launch_bounds__(P_WG) void k_minimizer_g(PartitionArgs a, uint32_t* __restrict__ gbin,
                                                       uint64_t* __restrict__ tile_lc) {
     __shared__ uint64_t s_words[NWORDS];
     __shared__ uint64_t s_m[S_MAX];
-    __shared__ uint64_t s_frag[MAXF + 1];
-    __shared__ uint32_t s_g[P_TILE + 1];
-    __shared__ uint8_t s_inv[P_TILE + 1];
+    __shared__ uint32_t s_inv[(P_TILE + 1 + 31) / 32];  // bit i: position q0 + i starts no (k-1)-mer
     __shared__ unsigned long long s_lc;
+    uint32_t* s_g = reinterpret_cast<uint32_t*>(s_m);  // reuses s_m once the minima are read
 
     const int tid = threadIdx.x;
     const int64_t t0 = int64_t(blockIdx.x) * P_TILE;
@@ -48,48 +62,31 @@ __global__ __launch_bounds__(P_WG) void k_minimizer_g(PartitionArgs a, uint32_t*
     const unsigned k = a.k, l = a.l, w = a.w;
     const int S = P_TILE + int(w);  // l-mers q0 .. q0 + P_TILE + w - 1
 
-    // Fragment containing max(q0, 0): a 64-ary search by wave 0 (a few
-    // dependent loads instead of log2(n_frag)).
-    if (tid < 64) {
-        const uint64_t key = uint64_t(q0 < 0 ? 0 : q0);
-        uint64_t lo = 0, hi = a.n_frag;  // largest f in [lo, hi] with frag_off[f] <= key
-        while (hi - lo >= 64) {
-            const uint64_t step = (hi - lo + 63) / 64;
-            const uint64_t idx = lo + uint64_t(tid) * step;
-            const bool le = idx <= hi && a.frag_off[idx] <= key;
-            const unsigned long long m = __ballot(le);  // lane 0 always set
-            const uint64_t last = uint64_t(63 - __clzll((long long)m));
-            lo = lo + last * step;
-            hi = min(hi, lo + step - 1);
-        }
-        const uint64_t idx = lo + uint64_t(tid);
-        const bool le = idx <= hi && a.frag_off[idx] <= key;
-        const unsigned long long m = __ballot(le);
-        if (tid == 0) {
-            s_m[0] = lo + uint64_t(63 - __clzll((long long)m));
-            s_lc = 0;
-        }
-    }
-    for (int i = tid; i <= P_TILE; i += P_WG) s_inv[i] = 0;
-    __syncthreads();
-    const uint64_t f0 = s_m[0];
-    __syncthreads();
-    for (int i = tid; i <= MAXF; i += P_WG) {
-        const uint64_t f = f0 + uint64_t(i);
-        s_frag[i] = f <= a.n_frag ? a.frag_off[f] : a.n_bases;
-    }
+    // Fragments from the one containing max(q0, 0) (found for every tile
+    // beforehand, k_tile_frag) to the one containing the next tile's first
+    // position, plus two: fragments are longer than k, so no later boundary
+    // lies within k - 2 of this tile.
+    if (tid == 0) s_lc = 0;
+    for (int i = tid; i < (P_TILE + 1 + 31) / 32; i += P_WG) s_inv[i] = 0;
+    const uint64_t f0 = a.tile_frag[blockIdx.x];
+    const uint64_t f1 = blockIdx.x + 1 < gridDim.x ? a.tile_frag[blockIdx.x + 1] + 2 : a.n_frag;
     const int64_t wlo = (q0 < 0 ? 0 : q0) >> 5;
     for (int i = tid; i < NWORDS; i += P_WG) {
         const uint64_t wi = uint64_t(wlo) + uint64_t(i);
         s_words[i] = wi < a.n_words ? a.packed[wi] : 0;
     }
-    __syncthreads();
     // Position p starts a (k-1)-mer iff no fragment boundary b has
     // p < b <= p + k - 2: each boundary marks [b - k + 2, b - 1].
-    for (int j = tid; j <= MAXF; j += P_WG) {
-        const int64_t b = int64_t(s_frag[j]);
+    __syncthreads();  // s_inv cleared
+    for (uint64_t f = f0 + uint64_t(tid); f <= min(f1, a.n_frag); f += P_WG) {
+        const int64_t b = int64_t(a.frag_off[f]);
         const int64_t lo = max(int64_t(0), b - int64_t(k) + 2 - q0), hi = min(int64_t(P_TILE), b - 1 - q0);
-        for (int64_t i = lo; i <= hi; ++i) s_inv[i] = 1;
+        for (int64_t wi = lo >> 5; wi <= (hi >> 5) && lo <= hi; ++wi) {  // bits lo..hi, a word at a time
+            const int64_t a0 = max(lo, wi << 5), a1 = min(hi, (wi << 5) + 31);
+            const uint32_t n = uint32_t(a1 - a0 + 1);
+            const uint32_t mask = (n == 32 ? ~0u : ((1u << n) - 1)) << (a0 & 31);
+            atomicOr(&s_inv[wi], mask);
+        }
     }
     __syncthreads();
 
@@ -132,18 +129,26 @@ __global__ __launch_bounds__(P_WG) void k_minimizer_g(PartitionArgs a, uint32_t*
         __syncthreads();
     }
 
-    for (int i = tid; i <= P_TILE; i += P_WG) {
+    constexpr int GPT = (P_TILE + 1 + P_WG - 1) / P_WG;
+    uint32_t gvs[GPT];
+#pragma unroll
+    for (int c = 0; c < GPT; ++c) {
+        const int i = tid + c * P_WG;
         const int64_t p = q0 + i;
         uint32_t gv = G_INVALID;
-        if (p >= 0 && p < N) {
-            if (!s_inv[i]) {
-                const uint64_t m0 = s_m[i], m1 = s_m[i + w - span];
-                const uint64_t H = m0 < m1 ? m0 : m1;
-                gv = uint32_t(H & a.g_mask) | (uint32_t((H >> a.g_bits) & 15u) << 16);
-            }
+        if (i <= P_TILE && p >= 0 && p < N && !((s_inv[i >> 5] >> (i & 31)) & 1u)) {
+            const uint64_t m0 = s_m[i], m1 = s_m[i + w - span];
+            const uint64_t H = m0 < m1 ? m0 : m1;
+            gv = uint32_t(H & a.g_mask) | (uint32_t((H >> a.g_bits) & 15u) << 16);
         }
-        s_g[i] = gv;
-        if (i >= 1 && p < N) gbin[p] = gv;
+        gvs[c] = gv;
+        if (i >= 1 && i <= P_TILE && p < N) gbin[p] = gv;
+    }
+    __syncthreads();  // every minimum read: s_g may overwrite s_m
+#pragma unroll
+    for (int c = 0; c < GPT; ++c) {
+        const int i = tid + c * P_WG;
+        if (i <= P_TILE) s_g[i] = gvs[c];
     }
     __syncthreads();
 
@@ -661,6 +666,8 @@ __global__ __launch_bounds__(256) void k_bucket_features(const uint64_t* __restr
 void launch_minimizer_g(const PartitionArgs& a, uint32_t* gbin, uint64_t* tile_lc, hipStream_t s) {
     const uint64_t tiles = (a.n_bases + P_TILE - 1) / P_TILE;
     if (!tiles) return;
+    const uint32_t fg = uint32_t(std::min<uint64_t>((a.n_frag + 255) / 256, 65536));
+    hipLaunchKernelGGL(k_tile_frag, dim3(fg), dim3(256), 0, s, a.frag_off, a.n_frag, tiles, a.tile_frag);
     hipLaunchKernelGGL(k_minimizer_g, dim3(uint32_t(tiles)), dim3(P_WG), 0, s, a, gbin, tile_lc);
 }
 
