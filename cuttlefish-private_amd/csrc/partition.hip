@@ -582,21 +582,37 @@ __global__ __launch_bounds__(E_WG, 4) void k_pieces_emit(PartitionArgs a, const 
 
 // Level 2: one workgroup per coarse bin moves its records into the final
 // per-g order (the bin's 2^FINE_BITS buckets) and writes their offsets.
-// Records of one bucket are written in increasing slots by one workgroup.
-__global__ __launch_bounds__(1024) void k_bucket_fine(const uint64_t* __restrict__ rec,
-                                                      const uint64_t* __restrict__ coff, uint32_t W,
-                                                      uint64_t* __restrict__ boff, uint32_t* __restrict__ att,
-                                                      uint64_t* __restrict__ labels, uint8_t* __restrict__ mbin) {
+// Records are staged a chunk at a time in LDS and counting-sorted by bucket,
+// so each bucket's records leave as one contiguous run per chunk (coalesced
+// stores instead of 64 interleaved streams).  Order inside a bucket is free.
+constexpr uint32_t FINE_WG = 1024;
+constexpr uint32_t FINE_LDS = 144 * 1024;  // staging bytes
+__host__ __device__ constexpr uint32_t fine_chunk(uint32_t W) {
+    // largest power of two <= FINE_LDS / (record bytes + 2 permutation bytes), at most 4096
+    return (FINE_LDS / (13 + 8 * W)) >= 4096 ? 4096 : (FINE_LDS / (13 + 8 * W)) >= 2048 ? 2048 : 1024;
+}
+
+__global__ __launch_bounds__(FINE_WG) void k_bucket_fine(const uint64_t* __restrict__ rec,
+                                                         const uint64_t* __restrict__ coff, uint32_t W,
+                                                         uint64_t* __restrict__ boff, uint32_t* __restrict__ att,
+                                                         uint64_t* __restrict__ labels, uint8_t* __restrict__ mbin) {
     constexpr uint32_t NF = 1u << FINE_BITS;
-    __shared__ uint32_t s_cnt[NF];
+    __shared__ uint32_t s_cnt[NF], s_ccnt[NF], s_cbase[NF];
     __shared__ unsigned long long s_cur[NF];
+    __shared__ __attribute__((aligned(16))) uint8_t s_stage[FINE_LDS];
     const uint32_t c = blockIdx.x;
-    const int tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x;
     const uint64_t r0 = coff[c], r1 = coff[c + 1];
     const uint64_t RW = l1_words(W);
-    for (uint32_t i = tid; i < NF; i += 1024) s_cnt[i] = 0;
+    const uint32_t CH = fine_chunk(W);
+    uint64_t* s_lab = reinterpret_cast<uint64_t*>(s_stage);                // CH * W
+    uint32_t* s_att = reinterpret_cast<uint32_t*>(s_lab + size_t(CH) * W);  // CH
+    uint16_t* s_perm = reinterpret_cast<uint16_t*>(s_att + CH);             // CH: sorted slot -> staged record
+    uint8_t* s_mb = reinterpret_cast<uint8_t*>(s_perm + CH);                // CH
+
+    for (uint32_t i = tid; i < NF; i += FINE_WG) s_cnt[i] = 0;
     __syncthreads();
-    for (uint64_t r = r0 + tid; r < r1; r += 1024) atomicAdd(&s_cnt[(uint32_t(rec[r * RW]) >> 16) & (NF - 1)], 1u);
+    for (uint64_t r = r0 + tid; r < r1; r += FINE_WG) atomicAdd(&s_cnt[(uint32_t(rec[r * RW]) >> 16) & (NF - 1)], 1u);
     __syncthreads();
     if (tid == 0) {
         uint64_t run = r0;
@@ -607,20 +623,58 @@ __global__ __launch_bounds__(1024) void k_bucket_fine(const uint64_t* __restrict
         }
         if (c + 1 == gridDim.x) boff[(c + 1) * NF] = r1;
     }
-    __syncthreads();
-    for (uint64_t r = r0 + tid; r < r1; r += 1024) {
-        const u64x2* in = reinterpret_cast<const u64x2*>(rec + r * RW);
-        const u64x2 p0 = in[0];
-        const uint32_t at = uint32_t(p0.x);
-        const uint64_t slot = atomicAdd(&s_cur[(at >> 16) & (NF - 1)], 1ull);
-        att[slot] = at;
-        mbin[slot] = uint8_t(p0.x >> 32);
-        labels[slot * W] = p0.y;
-        for (uint32_t q = 1; q < W; q += 2) {
-            const u64x2 pq = in[(q + 1) / 2];
-            labels[slot * W + q] = pq.x;
-            if (q + 1 < W) labels[slot * W + q + 1] = pq.y;
+    for (uint64_t q0 = r0; q0 < r1; q0 += CH) {
+        const uint32_t n = uint32_t(min(uint64_t(CH), r1 - q0));
+        if (tid < NF) s_ccnt[tid] = 0;
+        __syncthreads();
+        // stage the chunk (coalesced 16-byte reads) and rank each record in its bucket
+        uint32_t rank[4], fb[4];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t i = tid + j * FINE_WG;
+            rank[j] = fb[j] = 0;
+            if (i >= n) continue;
+            const u64x2* in = reinterpret_cast<const u64x2*>(rec + (q0 + i) * RW);
+            const u64x2 p0 = in[0];
+            const uint32_t at = uint32_t(p0.x);
+            s_att[i] = at;
+            s_mb[i] = uint8_t(p0.x >> 32);
+            s_lab[size_t(i) * W] = p0.y;
+            for (uint32_t q = 1; q < W; q += 2) {
+                const u64x2 pq = in[(q + 1) / 2];
+                s_lab[size_t(i) * W + q] = pq.x;
+                if (q + 1 < W) s_lab[size_t(i) * W + q + 1] = pq.y;
+            }
+            fb[j] = (at >> 16) & (NF - 1);
+            rank[j] = atomicAdd(&s_ccnt[fb[j]], 1u);
         }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t run = 0;
+            for (uint32_t f = 0; f < NF; ++f) {
+                s_cbase[f] = run;
+                run += s_ccnt[f];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t i = tid + j * FINE_WG;
+            if (i < n) s_perm[s_cbase[fb[j]] + rank[j]] = uint16_t(i);
+        }
+        __syncthreads();
+        // write the sorted chunk: consecutive threads, consecutive slots of a bucket
+        for (uint32_t s2 = tid; s2 < n; s2 += FINE_WG) {
+            const uint32_t i = s_perm[s2];
+            const uint32_t at = s_att[i];
+            const uint32_t f = (at >> 16) & (NF - 1);
+            const uint64_t slot = s_cur[f] + (s2 - s_cbase[f]);
+            att[slot] = at;
+            mbin[slot] = s_mb[i];
+            for (uint32_t q = 0; q < W; ++q) labels[slot * W + q] = s_lab[size_t(i) * W + q];
+        }
+        __syncthreads();
+        if (tid < NF) s_cur[tid] += s_ccnt[tid];
     }
 }
 
@@ -704,7 +758,7 @@ void launch_pieces_emit(const PartitionArgs& a, const uint32_t* gbin, const uint
 
 void launch_bucket_fine(const uint64_t* rec1, const uint64_t* coff, uint32_t W, uint64_t* boff, uint32_t* att,
                         uint64_t* labels, uint8_t* mbin, hipStream_t s) {
-    hipLaunchKernelGGL(k_bucket_fine, dim3(N_COARSE), dim3(1024), 0, s, rec1, coff, W, boff, att, labels, mbin);
+    hipLaunchKernelGGL(k_bucket_fine, dim3(N_COARSE), dim3(FINE_WG), 0, s, rec1, coff, W, boff, att, labels, mbin);
 }
 
 void launch_bucket_features(const uint64_t* off, const uint32_t* att, const uint8_t* mbin,
