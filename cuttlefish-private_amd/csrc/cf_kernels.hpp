@@ -42,9 +42,12 @@ constexpr int E_SPANS_PER_CU = 8;  // emit spans (workgroups) per CU
 constexpr int H_WG = 256;          // count workgroup, one minimizer tile per step
 static_assert((N_COARSE << FINE_BITS) == 16384, "coarse x fine bins cover the 16384 subgraphs");
 constexpr int H_SPLIT = 1;  // count spans per emit span
-// Level-1 record: header word (att | mbin << 32) and W label words, padded to
-// an even number of words so records are 16-byte aligned.
-__host__ __device__ constexpr uint32_t l1_words(uint32_t W) { return (W + 2) & ~1u; }
+// Level-1 records: a header array (att | mbin << 32, one word per record) and
+// a label array of l1_label_words(W) words per record (W padded to even, so
+// labels are 16-byte aligned pairs).
+__host__ __device__ constexpr uint32_t l1_label_words(uint32_t W) { return (W + 1) & ~1u; }
+// Offset of the label array in a level-1 buffer of n records (kept 16-byte aligned).
+__host__ __device__ constexpr uint64_t l1_label_offset(uint64_t n) { return (n + 1) & ~1ull; }
 struct PieceSpans {
     uint64_t per_count, per_emit;  // minimizer tiles (P_TILE positions) per count / emit span
     uint32_t n_count, n_emit;      // spans (workgroups)
@@ -55,12 +58,13 @@ PieceSpans pieces_spans(uint64_t n_bases, uint32_t n_cu);
 // records); ctot[N_COARSE] scratch.
 void launch_pieces_hist(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, const PieceSpans& ps,
                         uint32_t* hist, uint64_t* ctot, uint64_t* coff, hipStream_t s);
-// Pass 3: records into the level-1 array (W + 1 words each), grouped by coarse bin.
+// Pass 3: records into the level-1 arrays (hdr1, lab1), grouped by coarse bin.
 void launch_pieces_emit(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, const PieceSpans& ps,
-                        const uint32_t* hist, const uint64_t* coff, uint32_t W, uint64_t* rec1, hipStream_t s);
+                        const uint32_t* hist, const uint64_t* coff, uint32_t W, uint64_t* hdr1, uint64_t* lab1,
+                        hipStream_t s);
 // Level 2: level-1 records into per-bucket order; boff[G+1] bucket offsets.
-void launch_bucket_fine(const uint64_t* rec1, const uint64_t* coff, uint32_t W, uint64_t* boff, uint32_t* att,
-                        uint64_t* labels, uint8_t* mbin, hipStream_t s);
+void launch_bucket_fine(const uint64_t* hdr1, const uint64_t* lab1, const uint64_t* coff, uint32_t W, uint64_t* boff,
+                        uint32_t* att, uint64_t* labels, uint8_t* mbin, hipStream_t s);
 
 // Per-bucket statistics: [n_rec, n_kmer_instances, n_bases, 0] as u64 and the
 // MLP feature vector (N_FEAT floats).

@@ -915,16 +915,19 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     uint64_t n_rec = 0;
     HIPCK(h, hipMemcpyAsync(&n_rec, h->coff.as<uint64_t>() + N_COARSE, 8, hipMemcpyDeviceToHost, h->stream));
     HIPCK(h, hipStreamSynchronize(h->stream));
-    HIPCK(h, h->rec1.ensure(n_rec * l1_words(h->W) * 8));
+    const uint64_t lab_off = l1_label_offset(n_rec);
+    HIPCK(h, h->rec1.ensure((lab_off + n_rec * l1_label_words(h->W)) * 8));
     HIPCK(h, h->att.ensure(n_rec * 4));
     HIPCK(h, h->labels.ensure(n_rec * h->W * 8));
     HIPCK(h, h->mbin.ensure(n_rec));
     HIPCK(h, hipEventRecord(e3.e, h->stream));
     if (n_frag)
         launch_pieces_emit(a, h->gbin.as<uint32_t>(), h->tile_lc.as<uint64_t>(), ps, h->hist.as<uint32_t>(),
-                           h->coff.as<uint64_t>(), h->W, h->rec1.as<uint64_t>(), h->stream);
-    launch_bucket_fine(h->rec1.as<uint64_t>(), h->coff.as<uint64_t>(), h->W, h->boff.as<uint64_t>(),
-                       h->att.as<uint32_t>(), h->labels.as<uint64_t>(), h->mbin.as<uint8_t>(), h->stream);
+                           h->coff.as<uint64_t>(), h->W, h->rec1.as<uint64_t>(), h->rec1.as<uint64_t>() + lab_off,
+                           h->stream);
+    launch_bucket_fine(h->rec1.as<uint64_t>(), h->rec1.as<uint64_t>() + lab_off, h->coff.as<uint64_t>(), h->W,
+                       h->boff.as<uint64_t>(), h->att.as<uint32_t>(), h->labels.as<uint64_t>(), h->mbin.as<uint8_t>(),
+                       h->stream);
     HIPCK(h, hipEventRecord(e4.e, h->stream));
     launch_bucket_features(h->boff.as<uint64_t>(), h->att.as<uint32_t>(), h->mbin.as<uint8_t>(), G, k,
                            h->stats.as<uint64_t>(), h->feat.as<float>(), h->stream);

@@ -505,14 +505,14 @@ __global__ __launch_bounds__(N_COARSE) void k_coarse_scan(const uint64_t* __rest
 // Pass 3: the same pieces again; emit span s (H_SPLIT count spans) appends
 // its records of coarse bin c to the run starting at coff[c] +
 // hist[c][s * H_SPLIT].  A span is a contiguous part of the input, so each
-// run is written front to back by one workgroup.  Level-1 records are one
-// array of structures, l1_words(W) words each: att | mbin << 32, then the
-// label, padded to whole 16-byte pairs.
+// run is written front to back by one workgroup.  Level-1 records: the
+// header word att | mbin << 32 in hdr, the label in lab as whole 16-byte
+// pairs (l1_label_words(W) words).
 __global__ __launch_bounds__(E_WG, 4) void k_pieces_emit(PartitionArgs a, const uint32_t* __restrict__ gbin,
                                                       const uint64_t* __restrict__ tile_lc, uint64_t per,
                                                       uint32_t n_hist, const uint32_t* __restrict__ hist,
                                                       const uint64_t* __restrict__ coff, uint32_t W,
-                                                      uint64_t* __restrict__ rec) {
+                                                      uint64_t* __restrict__ hdr, uint64_t* __restrict__ lab) {
     __shared__ __attribute__((aligned(16))) uint32_t s_g[E_TILE + WIN_MAX + 2];
     __shared__ unsigned long long s_carry;
     __shared__ uint32_t s_wave[E_WG / 64];
@@ -559,11 +559,11 @@ __global__ __launch_bounds__(E_WG, 4) void k_pieces_emit(PartitionArgs a, const 
                                         [&](uint32_t g, uint32_t mb, uint32_t len, uint32_t l_disc,
                                             uint32_t r_disc, uint64_t start) {
                                             const uint64_t slot = atomicAdd(&s_cur[g >> FINE_BITS], 1ull);
-                                            // record = RW words (16-byte aligned): header, then
-                                            // the label words, written two words at a time
-                                            u64x2* out = reinterpret_cast<u64x2*>(rec + slot * l1_words(W));
-                                            uint64_t prev = uint64_t(len | (l_disc << 8) | (r_disc << 9) | (g << 16)) |
-                                                            (uint64_t(mb) << 32);
+                                            hdr[slot] = uint64_t(len | (l_disc << 8) | (r_disc << 9) | (g << 16)) |
+                                                        (uint64_t(mb) << 32);
+                                            // label words two at a time (16-byte stores)
+                                            u64x2* out = reinterpret_cast<u64x2*>(lab + slot * l1_label_words(W));
+                                            uint64_t prev = 0;
                                             for (uint32_t q = 0; q < W; ++q) {
                                                 const uint32_t wj = W - 1 - q;  // label word q holds bases 32wj ..
                                                 uint64_t x = 0;
@@ -572,10 +572,10 @@ __global__ __launch_bounds__(E_WG, 4) void k_pieces_emit(PartitionArgs a, const 
                                                     const uint32_t nb = len - 32 * wj;
                                                     if (nb < 32) x &= ~0ull << (64 - 2 * nb);
                                                 }
-                                                if (q & 1) prev = x;
+                                                if (!(q & 1)) prev = x;
                                                 else out[q / 2] = u64x2{prev, x};
                                             }
-                                            if (!(W & 1)) out[W / 2] = u64x2{prev, 0ull};
+                                            if (W & 1) out[W / 2] = u64x2{prev, 0ull};
                                         });
     }
 }
@@ -592,7 +592,8 @@ __host__ __device__ constexpr uint32_t fine_chunk(uint32_t W) {
     return (FINE_LDS / (13 + 8 * W)) >= 4096 ? 4096 : (FINE_LDS / (13 + 8 * W)) >= 2048 ? 2048 : 1024;
 }
 
-__global__ __launch_bounds__(FINE_WG) void k_bucket_fine(const uint64_t* __restrict__ rec,
+__global__ __launch_bounds__(FINE_WG) void k_bucket_fine(const uint64_t* __restrict__ hdr,
+                                                         const uint64_t* __restrict__ lab,
                                                          const uint64_t* __restrict__ coff, uint32_t W,
                                                          uint64_t* __restrict__ boff, uint32_t* __restrict__ att,
                                                          uint64_t* __restrict__ labels, uint8_t* __restrict__ mbin) {
@@ -603,7 +604,7 @@ __global__ __launch_bounds__(FINE_WG) void k_bucket_fine(const uint64_t* __restr
     const uint32_t c = blockIdx.x;
     const uint32_t tid = threadIdx.x;
     const uint64_t r0 = coff[c], r1 = coff[c + 1];
-    const uint64_t RW = l1_words(W);
+    const uint64_t LW = l1_label_words(W);
     const uint32_t CH = fine_chunk(W);
     uint64_t* s_lab = reinterpret_cast<uint64_t*>(s_stage);                // CH * W
     uint32_t* s_att = reinterpret_cast<uint32_t*>(s_lab + size_t(CH) * W);  // CH
@@ -612,7 +613,7 @@ __global__ __launch_bounds__(FINE_WG) void k_bucket_fine(const uint64_t* __restr
 
     for (uint32_t i = tid; i < NF; i += FINE_WG) s_cnt[i] = 0;
     __syncthreads();
-    for (uint64_t r = r0 + tid; r < r1; r += FINE_WG) atomicAdd(&s_cnt[(uint32_t(rec[r * RW]) >> 16) & (NF - 1)], 1u);
+    for (uint64_t r = r0 + tid; r < r1; r += FINE_WG) atomicAdd(&s_cnt[(uint32_t(hdr[r]) >> 16) & (NF - 1)], 1u);
     __syncthreads();
     if (tid == 0) {
         uint64_t run = r0;
@@ -634,14 +635,13 @@ __global__ __launch_bounds__(FINE_WG) void k_bucket_fine(const uint64_t* __restr
             const uint32_t i = tid + j * FINE_WG;
             rank[j] = fb[j] = 0;
             if (i >= n) continue;
-            const u64x2* in = reinterpret_cast<const u64x2*>(rec + (q0 + i) * RW);
-            const u64x2 p0 = in[0];
-            const uint32_t at = uint32_t(p0.x);
+            const uint64_t h0 = hdr[q0 + i];
+            const uint32_t at = uint32_t(h0);
             s_att[i] = at;
-            s_mb[i] = uint8_t(p0.x >> 32);
-            s_lab[size_t(i) * W] = p0.y;
-            for (uint32_t q = 1; q < W; q += 2) {
-                const u64x2 pq = in[(q + 1) / 2];
+            s_mb[i] = uint8_t(h0 >> 32);
+            const u64x2* in = reinterpret_cast<const u64x2*>(lab + (q0 + i) * LW);
+            for (uint32_t q = 0; q < W; q += 2) {
+                const u64x2 pq = in[q / 2];
                 s_lab[size_t(i) * W + q] = pq.x;
                 if (q + 1 < W) s_lab[size_t(i) * W + q + 1] = pq.y;
             }
@@ -750,15 +750,17 @@ void launch_pieces_hist(const PartitionArgs& a, const uint32_t* gbin, const uint
 }
 
 void launch_pieces_emit(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, const PieceSpans& ps,
-                        const uint32_t* hist, const uint64_t* coff, uint32_t W, uint64_t* rec1, hipStream_t s) {
+                        const uint32_t* hist, const uint64_t* coff, uint32_t W, uint64_t* hdr1, uint64_t* lab1,
+                        hipStream_t s) {
     if (!ps.n_emit) return;
     hipLaunchKernelGGL(k_pieces_emit, dim3(ps.n_emit), dim3(E_WG), 0, s, a, gbin, tile_lc, ps.per_emit,
-                       ps.n_count, hist, coff, W, rec1);
+                       ps.n_count, hist, coff, W, hdr1, lab1);
 }
 
-void launch_bucket_fine(const uint64_t* rec1, const uint64_t* coff, uint32_t W, uint64_t* boff, uint32_t* att,
+void launch_bucket_fine(const uint64_t* hdr1, const uint64_t* lab1, const uint64_t* coff, uint32_t W, uint64_t* boff,
+                        uint32_t* att,
                         uint64_t* labels, uint8_t* mbin, hipStream_t s) {
-    hipLaunchKernelGGL(k_bucket_fine, dim3(N_COARSE), dim3(FINE_WG), 0, s, rec1, coff, W, boff, att, labels, mbin);
+    hipLaunchKernelGGL(k_bucket_fine, dim3(N_COARSE), dim3(FINE_WG), 0, s, hdr1, lab1, coff, W, boff, att, labels, mbin);
 }
 
 void launch_bucket_features(const uint64_t* off, const uint32_t* att, const uint8_t* mbin,
