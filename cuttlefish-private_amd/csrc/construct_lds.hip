@@ -38,14 +38,38 @@ struct LdsProbe {
     bool inserted;
 };
 
+// 32-bit hash of a k-mer for the pass and group choice of these tables: each
+// key word folded to 32 bits (low half xor odd multiple of the high half) and
+// mixed by the murmur3 32-bit finaliser.  Only the spread of keys over passes
+// and groups depends on it: keys are always compared whole, and nothing
+// looks a key up in a finished table by this hash.
+__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+template <int KW>
+__device__ __forceinline__ uint32_t pass_hash(const Kmer<KW>& x) {
+    uint32_t h = 0;
+#pragma unroll
+    for (int i = 0; i < KW; ++i) {
+        const uint32_t f = uint32_t(x.w[i]) ^ (uint32_t(x.w[i] >> 32) * 0x9e3779b1u);
+        h = i ? fmix32(h) ^ f : f;
+    }
+    return fmix32(h);
+}
+
 // Find or insert `key` in an LDS table of C slots (KW u64 words each), probing
 // groups of 4 slots: the 4 top words are read together, and a group fills
 // from its first slot on (claims take the first empty slot; entries are never
 // removed), so a group with an empty slot ends an unsuccessful search.
 template <int KW, int C>
-__device__ __forceinline__ LdsProbe lds_find_or_insert(uint64_t* s_key, const Kmer<KW>& key, uint64_t h) {
+__device__ __forceinline__ LdsProbe lds_find_or_insert(uint64_t* s_key, const Kmer<KW>& key, uint32_t g) {
     constexpr uint32_t NG = uint32_t(C) / 4;
-    uint32_t g = uint32_t((uint64_t(uint32_t(h)) * NG) >> 32);
     const uint64_t top = key.w[KW - 1];
     for (uint32_t n = 0; n < NG;) {
         uint64_t cur[4];
@@ -136,28 +160,38 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
         __syncthreads();
         if (s_ovf) continue;  // bucket already flagged by another pass
 
-        // Each lane walks a stream of records r, r + LWG, ... and prefetches
-        // the next one while walking the current one, so lanes stay busy
-        // whatever the record lengths.
+        // Each lane walks one record of every block of LWG records, taking
+        // them by lane index in even blocks and in reverse in odd blocks, and
+        // prefetches its next record while walking the current one.  Records
+        // sorted by length (deduplicated buckets are) then give every lane
+        // nearly the same number of k-mers, and the lanes of a wave switch
+        // records together.
         const uint64_t r_end = a.bucket_end ? a.bucket_end[b] : a.bucket_off[b + 1];
-        uint64_t r = a.bucket_off[b] + tid;
+        const uint64_t r0 = a.bucket_off[b];
+        const auto rec_of = [&](uint32_t j) {  // strictly increasing in j
+            return r0 + uint64_t(j) * LWG + ((j & 1u) ? uint32_t(LWG - 1 - tid) : uint32_t(tid));
+        };
+        uint32_t j = 0;
+        uint64_t r = rec_of(0), rn = rec_of(1);
         Walker rw;
         uint32_t at_next = 0, m = 1, m_next = 1;
         uint64_t lab_next[WMAX];
         if (r < r_end) {
             rw.load(a.att, a.labels, r, W, k);
             if (a.mult) m = a.mult[r];
-            if (r + LWG < r_end) {
-                Walker::fetch(a.att, a.labels, r + LWG, W, at_next, lab_next);
-                if (a.mult) m_next = a.mult[r + LWG];
+            if (rn < r_end) {
+                Walker::fetch(a.att, a.labels, rn, W, at_next, lab_next);
+                if (a.mult) m_next = a.mult[rn];
             }
         }
         uint32_t my_ins = 0;  // keys this lane inserted since its last flush into s_cnt
         while (r < r_end) {
             const Inst<KW> in = rw.next();
-            const uint64_t h = km_hash<KW>(in.key);
-            if (uint32_t((uint64_t(uint32_t(h >> 32)) * P) >> 32) == p) {
-                const LdsProbe pr = lds_find_or_insert<KW, C>(s_key, in.key, h);
+            // pass = top of h * P; the group comes from the fraction below it
+            const uint64_t hp = uint64_t(pass_hash<KW>(in.key)) * P;
+            if (uint32_t(hp >> 32) == p) {
+                const uint32_t g0 = uint32_t((uint64_t(uint32_t(hp)) * (uint32_t(C) / 4)) >> 32);
+                const LdsProbe pr = lds_find_or_insert<KW, C>(s_key, in.key, g0);
                 if (pr.slot < 0) {
                     s_ovf = 1;
                 } else {
@@ -180,14 +214,15 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                     if (atomicAdd(&s_cnt, my_ins) + my_ins > LIMIT) s_ovf = 1;
                     my_ins = 0;
                 }
-                r += LWG;
+                r = rn;
+                rn = rec_of(++j + 1);
                 if (r >= r_end) break;
                 if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
                 rw.init(at_next, lab_next, k);
                 m = m_next;
-                if (r + LWG < r_end) {
-                    Walker::fetch(a.att, a.labels, r + LWG, W, at_next, lab_next);
-                    if (a.mult) m_next = a.mult[r + LWG];
+                if (rn < r_end) {
+                    Walker::fetch(a.att, a.labels, rn, W, at_next, lab_next);
+                    if (a.mult) m_next = a.mult[rn];
                 }
             }
         }

@@ -87,6 +87,7 @@ __global__ __launch_bounds__(DD_WG) void k_dedup(DedupArgs a) {
     __shared__ uint64_t s_key[C * KR];
     __shared__ uint32_t s_cnt[C];
     __shared__ uint32_t s_item, s_n, s_ovf, s_base;
+    __shared__ uint32_t s_len[256], s_wsum[4];  // per-len counts, then output positions
     __shared__ unsigned long long s_km;
     const int tid = threadIdx.x;
     constexpr uint32_t LIMIT = (uint32_t(C) * 7u) / 8u;
@@ -147,11 +148,36 @@ __global__ __launch_bounds__(DD_WG) void k_dedup(DedupArgs a) {
             if (tid == 0) atomicOr(&a.overflow[b], 1u);
             continue;
         }
-        // Append the pass's distinct records to the bucket's output region.
+        // Append the pass's distinct records to the bucket's output region,
+        // longest first (a counting sort on len), so that a construct lane
+        // assignment that alternates direction over blocks of records gives
+        // every lane about the same number of k-mers to walk.
         if (tid == 0) {
             s_base = atomicAdd(&a.out_count[b], s_n);
-            s_n = 0;
             s_km = 0;
+        }
+        if (tid < 256) s_len[tid] = 0;
+        __syncthreads();
+        for (int i = tid; i < C; i += DD_WG)
+            if (s_cnt[i]) atomicAdd(&s_len[s_key[i * KR + KR - 1] & 0xFFu], 1u);
+        __syncthreads();
+        // exclusive prefix over len = 255, 254, ..., 0 by the first 4 waves
+        const int lane = tid & 63, wv = tid >> 6;
+        uint32_t v = 0, x = 0;
+        if (tid < 256) {
+            v = x = s_len[255 - tid];
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t u = __shfl_up(x, d);
+                if (lane >= d) x += u;
+            }
+            if (lane == 63) s_wsum[wv] = x;
+        }
+        __syncthreads();
+        if (tid < 256) {
+            uint32_t pre = 0;
+            for (int j = 0; j < wv; ++j) pre += s_wsum[j];
+            s_len[255 - tid] = pre + x - v;  // first output position of this len
         }
         __syncthreads();
         unsigned long long my_km = 0;
@@ -160,8 +186,8 @@ __global__ __launch_bounds__(DD_WG) void k_dedup(DedupArgs a) {
         for (int i = tid; i < C; i += DD_WG) {
             const uint32_t c = s_cnt[i];
             if (!c) continue;
-            const uint64_t o = base + atomicAdd(&s_n, 1u);
             const uint64_t* kp = &s_key[i * KR];
+            const uint64_t o = base + atomicAdd(&s_len[kp[KR - 1] & 0xFFu], 1u);
             my_km += (kp[KR - 1] & 0xFFu) - (a.k - 1);
             a.out_att[o] = uint32_t(kp[KR - 1]) | g16;
             a.out_mult[o] = c;
