@@ -166,6 +166,31 @@ __device__ __forceinline__ uint64_t fmix64(uint64_t h) {
     return h;
 }
 
+// murmur3's 32-bit finaliser.
+__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+// 32-bit hash of N u64 words for choosing a pass and a probe group of an
+// LDS table: each word folded to 32 bits (low half xor an odd multiple of the
+// high half), chained through fmix32.  Pass = top of h * P, group = the
+// fraction below it times the group count, so the two are independent.
+template <int N>
+__device__ __forceinline__ uint32_t pass_hash32(const uint64_t (&w)[N]) {
+    uint32_t h = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const uint32_t f = uint32_t(w[i]) ^ (uint32_t(w[i] >> 32) * 0x9e3779b1u);
+        h = i ? fmix32(h) ^ f : f;
+    }
+    return fmix32(h);
+}
+
 template <int KW>
 __device__ __forceinline__ uint64_t km_hash(const Kmer<KW>& x) {
     uint64_t h = x.w[0];

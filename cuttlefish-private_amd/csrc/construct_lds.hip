@@ -38,29 +38,12 @@ struct LdsProbe {
     bool inserted;
 };
 
-// 32-bit hash of a k-mer for the pass and group choice of these tables: each
-// key word folded to 32 bits (low half xor odd multiple of the high half) and
-// mixed by the murmur3 32-bit finaliser.  Only the spread of keys over passes
-// and groups depends on it: keys are always compared whole, and nothing
-// looks a key up in a finished table by this hash.
-__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
-    h ^= h >> 16;
-    h *= 0x85ebca6bu;
-    h ^= h >> 13;
-    h *= 0xc2b2ae35u;
-    h ^= h >> 16;
-    return h;
-}
-
+// Pass and group choice of these tables: pass_hash32 of the key words.  Only
+// the spread of keys over passes and groups depends on it: keys are always
+// compared whole, and nothing looks a key up in a finished table by it.
 template <int KW>
 __device__ __forceinline__ uint32_t pass_hash(const Kmer<KW>& x) {
-    uint32_t h = 0;
-#pragma unroll
-    for (int i = 0; i < KW; ++i) {
-        const uint32_t f = uint32_t(x.w[i]) ^ (uint32_t(x.w[i] >> 32) * 0x9e3779b1u);
-        h = i ? fmix32(h) ^ f : f;
-    }
-    return fmix32(h);
+    return pass_hash32<KW>(x.w);
 }
 
 // Find or insert `key` in an LDS table of C slots (KW u64 words each), probing

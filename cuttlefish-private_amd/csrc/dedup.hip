@@ -10,7 +10,7 @@
 //
 // One work item = one hash pass of one bucket: records whose key hash selects
 // the pass are counted in an LDS table (key = the label words with bases past
-// `len` cleared, then len | l_disc | r_disc as the top word), and the distinct
+// `len` cleared, then len | l_disc | r_disc | hash << 10 as the top word), and the distinct
 // records are appended to the bucket's region of the output arrays.  A pass
 // whose table fills past 7/8 flags the bucket; the host then copies its
 // records unchanged (multiplicity 1), so results never depend on the sizing.
@@ -27,10 +27,9 @@ constexpr uint64_t BUSY = 1ull << 63;
 // construct tables (top word claimed with BUSY, lower words written, top word
 // published).  Returns the slot or -1 when every group is full.
 template <int KR, int C>
-__device__ __forceinline__ int dd_find_or_insert(uint64_t* s_key, const uint64_t (&key)[KR], uint64_t h,
+__device__ __forceinline__ int dd_find_or_insert(uint64_t* s_key, const uint64_t (&key)[KR], uint32_t g,
                                                  bool& inserted) {
     constexpr uint32_t NG = uint32_t(C) / 4;
-    uint32_t g = uint32_t((uint64_t(uint32_t(h)) * NG) >> 32);
     const uint64_t top = key[KR - 1];
     inserted = false;
     for (uint32_t n = 0; n < NG;) {
@@ -111,30 +110,48 @@ __global__ __launch_bounds__(DD_WG) void k_dedup(DedupArgs a) {
 
         const uint64_t r0 = a.bucket_off[b], r1 = a.bucket_off[b + 1];
         uint32_t my_ins = 0, seen = 0;
+        // The next record is loaded while the current one is probed.
+        uint32_t at_n = 0;
+        uint64_t lab_n[W];
+        if (r0 + tid < r1) {
+            at_n = a.att[r0 + tid];
+#pragma unroll
+            for (int q = 0; q < W; ++q) lab_n[q] = a.labels[(r0 + tid) * W + q];
+        }
         for (uint64_t r = r0 + tid; r < r1; r += DD_WG) {
+            const uint32_t at = at_n;
+            uint64_t lab[W];
+#pragma unroll
+            for (int q = 0; q < W; ++q) lab[q] = lab_n[q];
+            if (r + DD_WG < r1) {
+                at_n = a.att[r + DD_WG];
+#pragma unroll
+                for (int q = 0; q < W; ++q) lab_n[q] = a.labels[(r + DD_WG) * W + q];
+            }
             if ((++seen & 7u) == 0) {  // account inserts every 8 records
                 if (my_ins && atomicAdd(&s_n, my_ins) + my_ins > LIMIT) s_ovf = 1;
                 my_ins = 0;
                 if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
             }
-            const uint32_t at = a.att[r];
             const uint32_t len = at & 0xFFu;
             uint64_t key[KR];
 #pragma unroll
             for (int q = 0; q < W; ++q) {  // word W-1-i holds bases 32i .. 32i+31
                 const int i = W - 1 - q;
                 const int nb = int(len) - 32 * i;
-                const uint64_t x = a.labels[r * W + q];
+                const uint64_t x = lab[q];
                 key[q] = nb >= 32 ? x : (nb <= 0 ? 0ull : x & (~0ull << (64 - 2 * nb)));
             }
             key[KR - 1] = at & 0x3FFu;
-            uint64_t h = key[0];
-#pragma unroll
-            for (int q = 1; q < KR; ++q) h = fmix64(h) ^ key[q];
-            h = fmix64(h);
-            if (uint32_t((uint64_t(uint32_t(h >> 32)) * P) >> 32) != p) continue;
+            const uint32_t h = pass_hash32<KR>(key);
+            // The top word also carries h (a function of the rest of the key),
+            // so that it alone tells almost every pair of distinct keys apart.
+            key[KR - 1] |= uint64_t(h) << 10;
+            const uint64_t hp = uint64_t(h) * P;  // pass = top, group from the fraction
+            if (uint32_t(hp >> 32) != p) continue;
             bool inserted;
-            const int slot = dd_find_or_insert<KR, C>(s_key, key, h, inserted);
+            const int slot = dd_find_or_insert<KR, C>(s_key, key, uint32_t((uint64_t(uint32_t(hp)) * (uint32_t(C) / 4)) >> 32),
+                                                      inserted);
             if (slot < 0) {
                 s_ovf = 1;
                 break;
@@ -189,7 +206,7 @@ __global__ __launch_bounds__(DD_WG) void k_dedup(DedupArgs a) {
             const uint64_t* kp = &s_key[i * KR];
             const uint64_t o = base + atomicAdd(&s_len[kp[KR - 1] & 0xFFu], 1u);
             my_km += (kp[KR - 1] & 0xFFu) - (a.k - 1);
-            a.out_att[o] = uint32_t(kp[KR - 1]) | g16;
+            a.out_att[o] = uint32_t(kp[KR - 1] & 0x3FFu) | g16;
             a.out_mult[o] = c;
 #pragma unroll
             for (int q = 0; q < W; ++q) a.out_labels[o * W + q] = kp[q];
