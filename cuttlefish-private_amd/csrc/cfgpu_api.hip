@@ -101,6 +101,9 @@ struct cfgpu_handle {
         d_qhead, d_round_list;
     std::vector<std::unique_ptr<DevBuf>> pools;  // one per sizing round, reused across calls
     uint32_t round_idx = 0;
+    DevBuf spill_key, spill_meta;  // LDS construct: per-workgroup spill lists of all-pass items
+    uint64_t spill_need = 0;       // largest k-mer instance count of an all-pass item of the round
+    uint32_t n_all_pass = 0;       // all-pass items of the round
     std::vector<uint64_t> h_tbl_ptr;
     std::vector<uint64_t> h_cap;
     std::vector<uint32_t> h_limit, h_nvert, h_flags;
@@ -211,6 +214,21 @@ uint64_t walked_records(const cfgpu_handle* h, uint32_t b) {
     return h->dd_active ? h->h_ddcnt[b] : h->h_stats[4 * b + 0];
 }
 uint64_t bucket_kmers(const cfgpu_handle* h, uint32_t b) { return h->h_stats[4 * b + 1]; }
+// k-mer instances the construct kernels walk (of the distinct records after dedup).
+uint64_t walked_kmers(const cfgpu_handle* h, uint32_t b) {
+    return h->dd_active ? h->h_ddkm[b] : bucket_kmers(h, b);
+}
+
+// An LDS bucket of 2 <= P <= ALL_PASS_MAX passes is one all-pass item when
+// its instances fit a spill list of at most SPILL_BYTES_MAX over all
+// workgroups' two lists; otherwise one item per pass, each walking the
+// records (more passes serialised in one workgroup would outlast the rest of
+// the round).
+constexpr uint64_t SPILL_BYTES_MAX = 2ull << 30;
+constexpr uint32_t ALL_PASS_MAX = 8;
+uint64_t spill_cap_max(const cfgpu_handle* h) {
+    return SPILL_BYTES_MAX / (uint64_t(std::max(h->cus, 1)) * 2 * (8 * h->KW + 4));
+}
 
 // Capacity that holds every k-mer instance of the bucket at the target load:
 // such a table never overflows by load (a probe chain past MAX_PROBES is
@@ -235,14 +253,25 @@ int prepare_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
     HIPCK(h, pool->ensure(n_slots * eb));
     uint64_t off = 0;
     std::vector<uint32_t> ib, ifirst, icnt;
+    h->spill_need = 0;
+    h->n_all_pass = 0;
+    const uint64_t spill_max = spill_cap_max(h);
     for (size_t i = 0; i < buckets.size(); ++i) {
         const uint32_t b = buckets[i];
         h->h_tbl_ptr[b] = reinterpret_cast<uint64_t>(pool->as<uint8_t>() + off * eb);
         h->h_cap[b] = caps[i];
         h->h_limit[b] = limit_for(caps[i]);
         off += caps[i];
-        if (lds_cap) {  // one item per hash pass: (bucket, pass, passes)
+        if (lds_cap) {
             const uint32_t P = uint32_t(caps[i] / lds_cap);
+            const uint64_t inst = walked_kmers(h, b);
+            if (P >= 2 && P <= ALL_PASS_MAX && inst <= spill_max) {  // one item: every pass, records walked once
+                ib.push_back(b), ifirst.push_back(LDS_ALL_PASSES), icnt.push_back(P);
+                h->spill_need = std::max(h->spill_need, inst);
+                ++h->n_all_pass;
+                continue;
+            }
+            // one item per hash pass: (bucket, pass, passes)
             for (uint32_t q = 0; q < P; ++q) ib.push_back(b), ifirst.push_back(q), icnt.push_back(P);
             continue;
         }
@@ -346,6 +375,14 @@ int run_round_batched(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
     a.n_items = n_items;
     const uint32_t per_cu = lds ? 1u : h->tune.wgs_per_cu;  // the LDS kernel fills a CU's LDS
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(n_items, uint32_t(h->cus) * per_cu));
+    if (lds && h->spill_need) {  // two spill lists per workgroup, each for the largest all-pass bucket
+        const uint64_t cap = (h->spill_need + 63) & ~63ull;
+        HIPCK(h, h->spill_key.ensure(uint64_t(grid) * 2 * cap * h->KW * 8));
+        HIPCK(h, h->spill_meta.ensure(uint64_t(grid) * 2 * cap * 4));
+        a.spill_key = h->spill_key.as<uint64_t>();
+        a.spill_meta = h->spill_meta.as<uint32_t>();
+        a.spill_cap = uint32_t(cap);
+    }
     auto reset = [&]() {
         launch_fill_u32(h->d_qhead.as<uint32_t>(), 0u, 1, h->stream);
         launch_zero_buckets(h->d_round_list.as<uint32_t>(), uint32_t(buckets.size()), h->d_nvert.as<uint32_t>(),
@@ -370,11 +407,12 @@ int run_round_batched(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
     }
     // The graph captures kernel arguments (device pointers and sizes); array
     // contents are read when it runs.  Re-capture when any captured value moves.
-    const void* ptrs[] = {pool->p, h->att.p, h->labels.p, h->boff.p, h->d_tbl_ptr.p, h->d_cap.p,
+    const void* ptrs[] = {pool->p, a.att, a.labels, a.mult, a.bucket_end, h->boff.p, h->d_tbl_ptr.p, h->d_cap.p,
                           h->d_limit.p, h->d_nvert.p, h->d_overflow.p, h->d_items_b.p,
-                          h->d_items_f.p, h->d_items_c.p, h->d_qhead.p, h->d_round_list.p};
+                          h->d_items_f.p, h->d_items_c.p, h->d_qhead.p, h->d_round_list.p,
+                          a.spill_key, a.spill_meta};
     std::vector<uint64_t> key = {n_slots, n_items, grid, h->KW, h->W, h->prm.k, h->tune.workgroup,
-                                 buckets.size(), uint64_t(lds)};
+                                 buckets.size(), uint64_t(lds), a.spill_cap};
     for (const void* q : ptrs) key.push_back(reinterpret_cast<uint64_t>(q));
     if (!h->graph_exec || key != h->graph_key) {
         if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec), h->graph_exec = nullptr;
@@ -415,6 +453,7 @@ int run_main(cfgpu_handle* h, const std::vector<uint32_t>& order, const std::vec
     overflowed = o1;
     overflowed.insert(overflowed.end(), o2.begin(), o2.end());
     h->rs.n_lds_buckets = uint32_t(lds_b.size());
+    h->rs.n_lds_all_pass = lds_b.empty() ? 0u : h->n_all_pass;  // the LDS round was the last prepared
     return CFGPU_OK;
 }
 

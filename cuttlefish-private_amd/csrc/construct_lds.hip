@@ -109,6 +109,69 @@ __device__ __forceinline__ LdsProbe lds_find_or_insert(uint64_t* s_key, const Km
     return {-1, false};
 }
 
+// Add one k-mer instance (counters of the front / back bases, m times;
+// discontinuity bits) to its slot of the LDS pass table, inserting the key if
+// new.  Returns false when every group is full.
+template <int KW, int C>
+__device__ __forceinline__ bool lds_add(uint64_t* s_key, uint32_t* s_ef, uint32_t* s_st, const Kmer<KW>& key,
+                                        uint32_t g0, uint32_t front, uint32_t back, uint32_t disc, uint32_t m,
+                                        uint32_t& my_ins) {
+    const LdsProbe pr = lds_find_or_insert<KW, C>(s_key, key, g0);
+    if (pr.slot < 0) return false;
+    // Saturating update (a fresh slot starts at 0; a stale value only costs
+    // a CAS retry).
+    uint32_t old = pr.inserted ? 0u : s_ef[pr.slot];
+    while (true) {
+        const uint32_t nw = ef_add(old, uint8_t(front), uint8_t(back), m);
+        if (nw == old) break;
+        const uint32_t prev = atomicCAS(&s_ef[pr.slot], old, nw);
+        if (prev == old) break;
+        old = prev;
+    }
+    if (disc) atomicOr(&s_st[pr.slot >> 2], disc << (8 * (pr.slot & 3)));
+    my_ins += pr.inserted;
+    return true;
+}
+
+// Spill list of a workgroup: KW key-word arrays then one meta array, `cap`
+// entries each.  meta = front | back << 3 | disc << 6 | min(m, 15) << 9
+// (counters saturate at 15, so adding min(m, 15) equals adding m).
+struct SpillList {
+    uint64_t* key;
+    uint32_t* meta;
+};
+
+template <int KW>
+__device__ __forceinline__ SpillList spill_list(const ConstructArgs& a, uint32_t which) {
+    const uint64_t cap = a.spill_cap;
+    const uint64_t region = uint64_t(blockIdx.x) * 2 + which;
+    return {a.spill_key + region * KW * cap, a.spill_meta + region * cap};
+}
+
+// Append the instances of the lanes with `push` set to a spill list: one LDS
+// atomic per wave, consecutive entries per wave.  A full list sets *ovf (the
+// bucket is then rebuilt on the global path).
+template <int KW>
+__device__ __forceinline__ void spill_push(bool push, const Kmer<KW>& key, uint32_t meta, const SpillList& L,
+                                           uint32_t cap, uint32_t* s_n, uint32_t* s_ovf) {
+    const unsigned long long mask = __ballot(push);
+    if (!mask) return;
+    const int lane = int(__lane_id());
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(s_n, uint32_t(__popcll(mask)));
+    base = __shfl(base, leader);
+    if (!push) return;
+    const uint32_t idx = base + __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+    if (idx >= cap) {
+        *s_ovf = 1;
+        return;
+    }
+#pragma unroll
+    for (int w = 0; w < KW; ++w) L.key[uint64_t(w) * cap + idx] = key.w[w];
+    L.meta[idx] = meta;
+}
+
 template <int KW, int WMAX>
 __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
     constexpr int C = lds_capacity(KW);
@@ -116,115 +179,150 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
     __shared__ uint64_t s_key[C * KW];
     __shared__ uint32_t s_ef[C];
     __shared__ uint32_t s_st[C / 4];  // one Vertex_Status byte per slot
-    __shared__ uint32_t s_item, s_cnt, s_ovf;
+    __shared__ uint32_t s_item, s_cnt, s_ovf, s_sn[2];
 
     const int tid = threadIdx.x;
     const uint32_t W = a.W, k = a.k;
     constexpr uint32_t LIMIT = (uint32_t(C) * 7u) / 8u;
+    constexpr uint32_t NG = uint32_t(C) / 4;
     using Walker = RecordWalker<KW, WMAX>;
 
     while (true) {
         __syncthreads();  // every thread is done with the previous item's flags
         if (tid == 0) {
             s_item = atomicAdd(a.queue_head, 1u);
-            s_cnt = 0;
             s_ovf = 0;
         }
         __syncthreads();
         const uint32_t item = s_item;
         if (item >= a.n_items) break;
         const uint32_t b = a.item_bucket[item];
-        const uint32_t p = a.item_first[item];     // pass index
-        const uint32_t P = a.item_count[item];     // passes of this bucket
+        const uint32_t P = a.item_count[item];  // passes of this bucket
+        // item_first: the pass of this item, or LDS_ALL_PASSES: every pass
+        // here, the first walking the records and leaving the k-mers of later
+        // passes in spill lists that those passes read instead.
+        const bool all = a.item_first[item] == LDS_ALL_PASSES;
+        const uint32_t p_lo = all ? 0u : a.item_first[item], p_hi = all ? P - 1 : p_lo;
         if (tid == 0) s_ovf = __hip_atomic_load(&a.overflow[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        for (int i = tid; i < C * KW; i += LWG) s_key[i] = EMPTY;
-        for (int i = tid; i < C; i += LWG) s_ef[i] = 0u;
-        for (int i = tid; i < C / 4; i += LWG) s_st[i] = 0u;
-        __syncthreads();
-        if (s_ovf) continue;  // bucket already flagged by another pass
 
-        // Each lane walks one record of every block of LWG records, taking
-        // them by lane index in even blocks and in reverse in odd blocks, and
-        // prefetches its next record while walking the current one.  Records
-        // sorted by length (deduplicated buckets are) then give every lane
-        // nearly the same number of k-mers, and the lanes of a wave switch
-        // records together.
-        const uint64_t r_end = a.bucket_end ? a.bucket_end[b] : a.bucket_off[b + 1];
-        const uint64_t r0 = a.bucket_off[b];
-        const auto rec_of = [&](uint32_t j) {  // strictly increasing in j
-            return r0 + uint64_t(j) * LWG + ((j & 1u) ? uint32_t(LWG - 1 - tid) : uint32_t(tid));
-        };
-        uint32_t j = 0;
-        uint64_t r = rec_of(0), rn = rec_of(1);
-        Walker rw;
-        uint32_t at_next = 0, m = 1, m_next = 1;
-        uint64_t lab_next[WMAX];
-        if (r < r_end) {
-            rw.load(a.att, a.labels, r, W, k);
-            if (a.mult) m = a.mult[r];
-            if (rn < r_end) {
-                Walker::fetch(a.att, a.labels, rn, W, at_next, lab_next);
-                if (a.mult) m_next = a.mult[rn];
+        for (uint32_t p = p_lo; p <= p_hi; ++p) {
+            for (int i = tid; i < C * KW; i += LWG) s_key[i] = EMPTY;
+            for (int i = tid; i < C; i += LWG) s_ef[i] = 0u;
+            for (int i = tid; i < C / 4; i += LWG) s_st[i] = 0u;
+            if (tid == 0) {
+                s_cnt = 0;
+                s_sn[(p + 1) & 1u] = 0;  // the list this pass writes
             }
-        }
-        uint32_t my_ins = 0;  // keys this lane inserted since its last flush into s_cnt
-        while (r < r_end) {
-            const Inst<KW> in = rw.next();
-            // pass = top of h * P; the group comes from the fraction below it
-            const uint64_t hp = uint64_t(pass_hash<KW>(in.key)) * P;
-            if (uint32_t(hp >> 32) == p) {
-                const uint32_t g0 = uint32_t((uint64_t(uint32_t(hp)) * (uint32_t(C) / 4)) >> 32);
-                const LdsProbe pr = lds_find_or_insert<KW, C>(s_key, in.key, g0);
-                if (pr.slot < 0) {
-                    s_ovf = 1;
-                } else {
-                    // Saturating update (a fresh slot starts at 0; a stale
-                    // value only costs a CAS retry).
-                    uint32_t old = pr.inserted ? 0u : s_ef[pr.slot];
-                    while (true) {
-                        const uint32_t nw = ef_add(old, in.front, in.back, m);
-                        if (nw == old) break;
-                        const uint32_t prev = atomicCAS(&s_ef[pr.slot], old, nw);
-                        if (prev == old) break;
-                        old = prev;
-                    }
-                    if (in.disc) atomicOr(&s_st[pr.slot >> 2], in.disc << (8 * (pr.slot & 3)));
-                    my_ins += pr.inserted;
-                }
-            }
-            if (rw.i == rw.n) {  // record done: account inserts, start the prefetched record
+            __syncthreads();
+            if (s_ovf) break;  // bucket already flagged
+            const bool spill = all && p < P - 1;
+            const SpillList out = spill_list<KW>(a, (p + 1) & 1u);
+            uint32_t my_ins = 0;  // keys this lane inserted since its last flush into s_cnt
+            const auto flush = [&]() {
                 if (my_ins) {
                     if (atomicAdd(&s_cnt, my_ins) + my_ins > LIMIT) s_ovf = 1;
                     my_ins = 0;
                 }
-                r = rn;
-                rn = rec_of(++j + 1);
-                if (r >= r_end) break;
-                if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
-                rw.init(at_next, lab_next, k);
-                m = m_next;
-                if (rn < r_end) {
-                    Walker::fetch(a.att, a.labels, rn, W, at_next, lab_next);
-                    if (a.mult) m_next = a.mult[rn];
+            };
+
+            if (p == p_lo) {
+                // Each lane walks one record of every block of LWG records,
+                // taking them by lane index in even blocks and in reverse in
+                // odd blocks, and prefetches its next record while walking the
+                // current one.  Records sorted by length (deduplicated buckets
+                // are) then give every lane nearly the same number of k-mers,
+                // and the lanes of a wave switch records together.
+                const uint64_t r_end = a.bucket_end ? a.bucket_end[b] : a.bucket_off[b + 1];
+                const uint64_t r0 = a.bucket_off[b];
+                const auto rec_of = [&](uint32_t j) {  // strictly increasing in j
+                    return r0 + uint64_t(j) * LWG + ((j & 1u) ? uint32_t(LWG - 1 - tid) : uint32_t(tid));
+                };
+                uint32_t j = 0;
+                uint64_t r = rec_of(0), rn = rec_of(1);
+                Walker rw;
+                uint32_t at_next = 0, m = 1, m_next = 1;
+                uint64_t lab_next[WMAX];
+                if (r < r_end) {
+                    rw.load(a.att, a.labels, r, W, k);
+                    if (a.mult) m = a.mult[r];
+                    if (rn < r_end) {
+                        Walker::fetch(a.att, a.labels, rn, W, at_next, lab_next);
+                        if (a.mult) m_next = a.mult[rn];
+                    }
                 }
-            }
-        }
-        __syncthreads();
-        if (s_ovf) {
-            if (tid == 0) atomicOr(&a.overflow[b], 1u);
-            continue;
-        }
-        // Write the finished pass table to slots [p*C, (p+1)*C) of the bucket.
-        typedef __attribute__((address_space(1))) uint64_t g_u64;
-        g_u64* out = reinterpret_cast<g_u64*>(a.tbl_ptr[b]) + uint64_t(p) * C * (KW + 1);
-        for (int i = tid; i < C; i += LWG) {
-            g_u64* e = out + uint64_t(i) * (KW + 1);
+                while (r < r_end) {
+                    const Inst<KW> in = rw.next();
+                    // pass = top of h * P; the group comes from the fraction below it
+                    const uint64_t hp = uint64_t(pass_hash<KW>(in.key)) * P;
+                    const uint32_t q = uint32_t(hp >> 32);
+                    if (q == p) {
+                        const uint32_t g0 = uint32_t((uint64_t(uint32_t(hp)) * NG) >> 32);
+                        if (!lds_add<KW, C>(s_key, s_ef, s_st, in.key, g0, in.front, in.back, in.disc, m, my_ins))
+                            s_ovf = 1;
+                    }
+                    if (spill)
+                        spill_push<KW>(q > p, in.key,
+                                       in.front | (in.back << 3) | (in.disc << 6) | (min(m, 15u) << 9), out,
+                                       a.spill_cap, &s_sn[(p + 1) & 1u], &s_ovf);
+                    if (rw.i == rw.n) {  // record done: account inserts, start the prefetched record
+                        flush();
+                        r = rn;
+                        rn = rec_of(++j + 1);
+                        if (r >= r_end) break;
+                        if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+                        rw.init(at_next, lab_next, k);
+                        m = m_next;
+                        if (rn < r_end) {
+                            Walker::fetch(a.att, a.labels, rn, W, at_next, lab_next);
+                            if (a.mult) m_next = a.mult[rn];
+                        }
+                    }
+                }
+            } else {
+                // A later pass: the instances the previous pass left, one per
+                // lane, in list order.
+                const SpillList in = spill_list<KW>(a, p & 1u);
+                const uint32_t n_in = s_sn[p & 1u];
+                uint32_t seen = 0;
+                for (uint32_t e = tid; e < n_in; e += LWG) {
+                    Kmer<KW> key;
 #pragma unroll
-            for (int w = 0; w < KW; ++w) e[w] = s_key[i * KW + w];
-            const uint32_t st = (s_st[i >> 2] >> (8 * (i & 3))) & 0xFFu;
-            e[KW] = uint64_t(s_ef[i]) | (uint64_t(st) << 32);
+                    for (int w = 0; w < KW; ++w) key.w[w] = in.key[uint64_t(w) * a.spill_cap + e];
+                    const uint32_t meta = in.meta[e];
+                    const uint64_t hp = uint64_t(pass_hash<KW>(key)) * P;
+                    const uint32_t q = uint32_t(hp >> 32);
+                    if (q == p) {
+                        const uint32_t g0 = uint32_t((uint64_t(uint32_t(hp)) * NG) >> 32);
+                        if (!lds_add<KW, C>(s_key, s_ef, s_st, key, g0, meta & 7u, (meta >> 3) & 7u, (meta >> 6) & 7u,
+                                            meta >> 9, my_ins))
+                            s_ovf = 1;
+                    }
+                    if (spill) spill_push<KW>(q > p, key, meta, out, a.spill_cap, &s_sn[(p + 1) & 1u], &s_ovf);
+                    if ((++seen & 7u) == 0) {
+                        flush();
+                        if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+                    }
+                }
+                flush();
+            }
+            __syncthreads();
+            if (s_ovf) {
+                if (tid == 0) atomicOr(&a.overflow[b], 1u);
+                break;
+            }
+            // Write the finished pass table to slots [p*C, (p+1)*C) of the bucket.
+            typedef __attribute__((address_space(1))) uint64_t g_u64;
+            g_u64* tbl = reinterpret_cast<g_u64*>(a.tbl_ptr[b]) + uint64_t(p) * C * (KW + 1);
+            for (int i = tid; i < C; i += LWG) {
+                g_u64* e = tbl + uint64_t(i) * (KW + 1);
+#pragma unroll
+                for (int w = 0; w < KW; ++w) e[w] = s_key[i * KW + w];
+                const uint32_t st = (s_st[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+                e[KW] = uint64_t(s_ef[i]) | (uint64_t(st) << 32);
+            }
+            if (tid == 0) atomicAdd(&a.n_vert[b], s_cnt);
+            __syncthreads();  // table read out and counters used before the next pass clears them
         }
-        if (tid == 0) atomicAdd(&a.n_vert[b], s_cnt);
     }
 }
 

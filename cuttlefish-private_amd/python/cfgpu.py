@@ -57,7 +57,7 @@ class RunStats(ctypes.Structure):
                                                   "mispredict_under_frac", "fallback_rate",
                                                   "holdout_err_mlp", "holdout_err_const")]
                 + [("n_records_walked", ctypes.c_uint64), ("n_dedup_fallback", ctypes.c_uint32),
-                   ("reserved2", ctypes.c_uint32), ("t_dedup_ms", ctypes.c_double),
+                   ("n_lds_all_pass", ctypes.c_uint32), ("t_dedup_ms", ctypes.c_double),
                    ("t_contract_ms", ctypes.c_double), ("n_unitigs", ctypes.c_uint64),
                    ("n_unitig_bytes", ctypes.c_uint64), ("n_contract_global", ctypes.c_uint32),
                    ("n_contract_runs", ctypes.c_uint32), ("t_cdbg_ms", ctypes.c_double),

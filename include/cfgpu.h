@@ -228,7 +228,7 @@ typedef struct cfgpu_run_stats {
     double holdout_err_const;         /*   ... and the constant mean-ratio predictor */
     uint64_t n_records_walked;        /* records the construct kernels walked (distinct after dedup) */
     uint32_t n_dedup_fallback;        /* buckets whose dedup tables overflowed (walked as they are) */
-    uint32_t reserved2;
+    uint32_t n_lds_all_pass;          /* LDS buckets of >= 2 passes built in one workgroup, records walked once */
     double t_dedup_ms;                /* record dedup, incl. its probe round */
     /* last cfgpu_contract */
     double t_contract_ms;             /* both passes, incl. an output re-run when it was short */

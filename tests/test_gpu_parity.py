@@ -308,6 +308,8 @@ def test_zipf_bucket_workload_loaded_records():
                         assert rs["n_lds_buckets"] == 0
                     if mp == 32 and mode == "batched":
                         assert rs["n_lds_buckets"] > 0
+                        # buckets of 2..8 passes walk their records once (spill lists)
+                        assert rs["n_lds_all_pass"] > 0
 
 
 def test_load_buckets_rejects_malformed_records():
