@@ -49,7 +49,8 @@ __device__ __forceinline__ uint32_t pass_hash(const Kmer<KW>& x) {
 // Find or insert `key` in an LDS table of C slots (KW u64 words each), probing
 // groups of 4 slots: the 4 top words are read together, and a group fills
 // from its first slot on (claims take the first empty slot; entries are never
-// removed), so a group with an empty slot ends an unsuccessful search.
+// removed), so a group with an empty slot ends an unsuccessful search.  The
+// group is scanned without branches (select chains, a candidate bit mask).
 template <int KW, int C>
 __device__ __forceinline__ LdsProbe lds_find_or_insert(uint64_t* s_key, const Kmer<KW>& key, uint32_t g) {
     constexpr uint32_t NG = uint32_t(C) / 4;
@@ -59,22 +60,23 @@ __device__ __forceinline__ LdsProbe lds_find_or_insert(uint64_t* s_key, const Km
 #pragma unroll
         for (int j = 0; j < 4; ++j)
             cur[j] = __hip_atomic_load(&s_key[(g * 4 + j) * KW + KW - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        int empty = 4;
-        bool busy = false;
+        const int empty = cur[0] == EMPTY ? 0 : cur[1] == EMPTY ? 1 : cur[2] == EMPTY ? 2 : cur[3] == EMPTY ? 3 : 4;
+        if constexpr (KW == 1) {
+            // an empty slot never equals a key (keys are < 2^62)
+            const int hit = cur[0] == top ? 0 : cur[1] == top ? 1 : cur[2] == top ? 2 : cur[3] == top ? 3 : -1;
+            if (hit >= 0) return {int(g * 4 + uint32_t(hit)), false};
+        } else {
+            uint32_t cand = 0;  // published slots whose top word matches
+            bool busy = false;  // a matching top word is still being published
 #pragma unroll
-        for (int j = 3; j >= 0; --j)
-            if (cur[j] == EMPTY) empty = j;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if (j >= empty) break;
-            if ((cur[j] & ~BUSY) != top) continue;
-            if constexpr (KW == 1) {
-                return {int(g * 4 + j), false};
-            } else {
-                if (cur[j] & BUSY) {
-                    busy = true;
-                    continue;
-                }
+            for (int j = 0; j < 4; ++j) {
+                const bool c = j < empty && (cur[j] & ~BUSY) == top;
+                cand |= uint32_t(c && !(cur[j] & BUSY)) << j;
+                busy |= c && (cur[j] & BUSY);
+            }
+            while (cand) {
+                const uint32_t j = uint32_t(__builtin_ctz(cand));
+                cand &= cand - 1;
                 const uint64_t* kp = &s_key[(g * 4 + j) * KW];
                 bool eq = true;
 #pragma unroll
@@ -82,9 +84,9 @@ __device__ __forceinline__ LdsProbe lds_find_or_insert(uint64_t* s_key, const Km
                     eq &= __hip_atomic_load(kp + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == key.w[w];
                 if (eq) return {int(g * 4 + j), false};
             }
+            if (busy) continue;  // re-read the group once it is published
         }
-        if (busy) continue;  // a matching top word is being published: re-read
-        if (empty == 4) {    // full group without the key: next group
+        if (empty == 4) {  // full group without the key: next group
             g = g + 1 == NG ? 0 : g + 1;
             ++n;
             continue;
@@ -284,11 +286,21 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                 const SpillList in = spill_list<KW>(a, p & 1u);
                 const uint32_t n_in = s_sn[p & 1u];
                 uint32_t seen = 0;
-                for (uint32_t e = tid; e < n_in; e += LWG) {
-                    Kmer<KW> key;
+                Kmer<KW> key_n;  // the lane's next entry, loaded while this one is probed
+                uint32_t meta_n = 0;
+                if (uint32_t(tid) < n_in) {
 #pragma unroll
-                    for (int w = 0; w < KW; ++w) key.w[w] = in.key[uint64_t(w) * a.spill_cap + e];
-                    const uint32_t meta = in.meta[e];
+                    for (int w = 0; w < KW; ++w) key_n.w[w] = in.key[uint64_t(w) * a.spill_cap + tid];
+                    meta_n = in.meta[tid];
+                }
+                for (uint32_t e = tid; e < n_in; e += LWG) {
+                    const Kmer<KW> key = key_n;
+                    const uint32_t meta = meta_n;
+                    if (e + LWG < n_in) {
+#pragma unroll
+                        for (int w = 0; w < KW; ++w) key_n.w[w] = in.key[uint64_t(w) * a.spill_cap + e + LWG];
+                        meta_n = in.meta[e + LWG];
+                    }
                     const uint64_t hp = uint64_t(pass_hash<KW>(key)) * P;
                     const uint32_t q = uint32_t(hp >> 32);
                     if (q == p) {

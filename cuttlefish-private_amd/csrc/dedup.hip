@@ -37,19 +37,18 @@ __device__ __forceinline__ int dd_find_or_insert(uint64_t* s_key, const uint64_t
 #pragma unroll
         for (int j = 0; j < 4; ++j)
             cur[j] = __hip_atomic_load(&s_key[(g * 4 + j) * KR + KR - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        int empty = 4;
-        bool busy = false;
-#pragma unroll
-        for (int j = 3; j >= 0; --j)
-            if (cur[j] == EMPTY) empty = j;
+        const int empty = cur[0] == EMPTY ? 0 : cur[1] == EMPTY ? 1 : cur[2] == EMPTY ? 2 : cur[3] == EMPTY ? 3 : 4;
+        uint32_t cand = 0;  // published slots whose top word matches (scanned without branches)
+        bool busy = false;  // a matching top word is still being published
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            if (j >= empty) break;
-            if ((cur[j] & ~BUSY) != top) continue;
-            if (cur[j] & BUSY) {
-                busy = true;
-                continue;
-            }
+            const bool c = j < empty && (cur[j] & ~BUSY) == top;
+            cand |= uint32_t(c && !(cur[j] & BUSY)) << j;
+            busy |= c && (cur[j] & BUSY);
+        }
+        while (cand) {
+            const uint32_t j = uint32_t(__builtin_ctz(cand));
+            cand &= cand - 1;
             const uint64_t* kp = &s_key[(g * 4 + j) * KR];
             bool eq = true;
 #pragma unroll
