@@ -12,6 +12,7 @@ namespace cfgpu {
 constexpr int P_TILE = 2048;            // (k-1)-mer positions per partition workgroup
 constexpr int P_WG = 256;               // partition workgroup size
 constexpr uint32_t G_INVALID = 0xFFFFFFFFu;
+constexpr uint32_t G16_INVALID = 0xFFFFu;  // per-position g array: no (k-1)-mer starts here
 constexpr int N_FEAT = 19;              // 16-bin minimizer histogram + 3 size features
 
 struct PartitionArgs {
@@ -26,9 +27,10 @@ struct PartitionArgs {
     WyConst wy;
 };
 
-// Pass 1: per-position minimizer hash -> (g | bin << 16), and per-tile last
-// g-change position (+1, 0 = none).
-void launch_minimizer_g(const PartitionArgs& a, uint32_t* gbin, uint64_t* tile_lc, hipStream_t s);
+// Pass 1: per position p the minimizer hash's g (gbin[p], G16_INVALID where
+// no (k-1)-mer starts) and the 4 hash bits above it (nibble p % 8 of
+// mbn[p / 8]), and per tile the last g-change position (+1, 0 = none).
+void launch_minimizer_g(const PartitionArgs& a, uint16_t* gbin, uint32_t* mbn, uint64_t* tile_lc, hipStream_t s);
 
 // Records are bucketed in two levels.  Level 1 splits by coarse bin
 // g >> FINE_BITS: the input is cut into contiguous spans (count spans, and
@@ -56,10 +58,11 @@ PieceSpans pieces_spans(uint64_t n_bases, uint32_t n_cu);
 // Pass 2: record counts per (coarse bin, count span) in hist[N_COARSE][n_count],
 // scanned in place; coff[N_COARSE+1] = coarse bin offsets (coff[N_COARSE] =
 // records); ctot[N_COARSE] scratch.
-void launch_pieces_hist(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, const PieceSpans& ps,
+void launch_pieces_hist(const PartitionArgs& a, const uint16_t* gbin, const uint64_t* tile_lc, const PieceSpans& ps,
                         uint32_t* hist, uint64_t* ctot, uint64_t* coff, hipStream_t s);
 // Pass 3: records into the level-1 arrays (hdr1, lab1), grouped by coarse bin.
-void launch_pieces_emit(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, const PieceSpans& ps,
+void launch_pieces_emit(const PartitionArgs& a, const uint16_t* gbin, const uint32_t* mbn, const uint64_t* tile_lc,
+                        const PieceSpans& ps,
                         const uint32_t* hist, const uint64_t* coff, uint32_t W, uint64_t* hdr1, uint64_t* lab1,
                         hipStream_t s);
 // Level 2: level-1 records into per-bucket order; boff[G+1] bucket offsets.

@@ -90,7 +90,7 @@ struct cfgpu_handle {
     cfgpu_tuning tune{0.6f, 256, 2048, 8, 1.15f, 1, 32, 0, 1};
 
     // partition state
-    DevBuf packed, frag, gbin, tile_lc, tile_frag, boff, att, labels, mbin, stats, feat;
+    DevBuf packed, frag, gbin, mbn, tile_lc, tile_frag, boff, att, labels, mbin, stats, feat;
     DevBuf hist, ctot, coff, rec1;  // level-1 bucketing
     uint64_t n_bases = 0, n_frag = 0, n_records = 0;
     bool partitioned = false;
@@ -918,7 +918,8 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
         d_frag = h->frag.as<uint64_t>();
     }
     const PieceSpans ps = pieces_spans(n_bases, uint32_t(h->cus));
-    HIPCK(h, h->gbin.ensure(n_bases * 4));
+    HIPCK(h, h->gbin.ensure(n_bases * 2));
+    HIPCK(h, h->mbn.ensure(((n_bases + 7) / 8) * 4));
     HIPCK(h, h->tile_lc.ensure(tiles * 8));
     HIPCK(h, h->tile_frag.ensure(tiles * 8));
     HIPCK(h, h->hist.ensure(size_t(N_COARSE) * ps.n_count * 4));
@@ -945,10 +946,10 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     Event e0, e1, e2, e3, e4, e5;
     HIPCK(h, hipMemsetAsync(h->coff.p, 0, size_t(N_COARSE + 1) * 8, h->stream));
     HIPCK(h, hipEventRecord(e0.e, h->stream));
-    if (n_frag) launch_minimizer_g(a, h->gbin.as<uint32_t>(), h->tile_lc.as<uint64_t>(), h->stream);
+    if (n_frag) launch_minimizer_g(a, h->gbin.as<uint16_t>(), h->mbn.as<uint32_t>(), h->tile_lc.as<uint64_t>(), h->stream);
     HIPCK(h, hipEventRecord(e1.e, h->stream));
     if (n_frag)
-        launch_pieces_hist(a, h->gbin.as<uint32_t>(), h->tile_lc.as<uint64_t>(), ps, h->hist.as<uint32_t>(),
+        launch_pieces_hist(a, h->gbin.as<uint16_t>(), h->tile_lc.as<uint64_t>(), ps, h->hist.as<uint32_t>(),
                            h->ctot.as<uint64_t>(), h->coff.as<uint64_t>(), h->stream);
     HIPCK(h, hipEventRecord(e2.e, h->stream));
     uint64_t n_rec = 0;
@@ -961,7 +962,7 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     HIPCK(h, h->mbin.ensure(n_rec));
     HIPCK(h, hipEventRecord(e3.e, h->stream));
     if (n_frag)
-        launch_pieces_emit(a, h->gbin.as<uint32_t>(), h->tile_lc.as<uint64_t>(), ps, h->hist.as<uint32_t>(),
+        launch_pieces_emit(a, h->gbin.as<uint16_t>(), h->mbn.as<uint32_t>(), h->tile_lc.as<uint64_t>(), ps, h->hist.as<uint32_t>(),
                            h->coff.as<uint64_t>(), h->W, h->rec1.as<uint64_t>(), h->rec1.as<uint64_t>() + lab_off,
                            h->stream);
     launch_bucket_fine(h->rec1.as<uint64_t>(), h->rec1.as<uint64_t>() + lab_off, h->coff.as<uint64_t>(), h->W,

@@ -47,8 +47,8 @@ __global__ __launch_bounds__(256) void k_tile_frag(const uint64_t* __restrict__ 
     }
 }
 
-__global__ __launch_bounds__(P_WG) void k_minimizer_g(PartitionArgs a, uint32_t* __restrict__ gbin,
-                                                      uint64_t* __restrict__ tile_lc) {
+__global__ __launch_bounds__(P_WG) void k_minimizer_g(PartitionArgs a, uint16_t* __restrict__ gbin,
+                                                      uint32_t* __restrict__ mbn, uint64_t* __restrict__ tile_lc) {
     __shared__ uint64_t s_words[NWORDS];
     __shared__ uint64_t s_m[S_MAX];
     __shared__ uint32_t s_inv[(P_TILE + 1 + 31) / 32];  // bit i: position q0 + i starts no (k-1)-mer
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(P_WG) void k_minimizer_g(PartitionArgs a, uint32_t*
             gv = uint32_t(H & a.g_mask) | (uint32_t((H >> a.g_bits) & 15u) << 16);
         }
         gvs[c] = gv;
-        if (i >= 1 && i <= P_TILE && p < N) gbin[p] = gv;
+        if (i >= 1 && i <= P_TILE && p < N) gbin[p] = uint16_t(gv);  // G16_INVALID for G_INVALID
     }
     __syncthreads();  // every minimum read: s_g may overwrite s_m
 #pragma unroll
@@ -152,6 +152,13 @@ __global__ __launch_bounds__(P_WG) void k_minimizer_g(PartitionArgs a, uint32_t*
     }
     __syncthreads();
 
+    {  // the 4 hash bits above g of positions t0 + 8 tid .. t0 + 8 tid + 7, a nibble each
+        static_assert(P_TILE == 8 * P_WG, "one nibble word per thread");
+        uint32_t word = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) word |= ((s_g[1 + 8 * tid + j] >> 16) & 15u) << (4 * j);
+        if (t0 + 8 * tid < N) mbn[(t0 >> 3) + tid] = word;
+    }
     unsigned long long my = 0;
     for (int i = tid + 1; i <= P_TILE; i += P_WG) {
         const uint32_t cur = s_g[i];
@@ -236,18 +243,18 @@ __device__ __forceinline__ uint32_t block_excl_suffix_min(uint32_t v, uint32_t* 
 template <int WG, int TILE>
 struct GAhead {
     static constexpr int N = (TILE + WIN_MAX + 2 + WG - 1) / WG;
-    uint32_t v[N];
-    __device__ __forceinline__ void load(const PartitionArgs& a, const uint32_t* __restrict__ gbin, uint64_t t0) {
+    uint16_t v[N];
+    __device__ __forceinline__ void load(const PartitionArgs& a, const uint16_t* __restrict__ gbin, uint64_t t0) {
         const int64_t q0 = int64_t(t0) - 1, n = int64_t(a.n_bases);
         const int span = TILE + int(a.w) + 2;
 #pragma unroll
         for (int c = 0; c < N; ++c) {
             const int i = int(threadIdx.x) + c * WG;
             const int64_t p = q0 + i;
-            v[c] = (i < span && p >= 0 && p < n) ? gbin[p] : G_INVALID;
+            v[c] = (i < span && p >= 0 && p < n) ? gbin[p] : G16_INVALID;
         }
     }
-    __device__ __forceinline__ void store(uint32_t* s_g) const {
+    __device__ __forceinline__ void store(uint16_t* s_g) const {
 #pragma unroll
         for (int c = 0; c < N; ++c) {
             const int i = int(threadIdx.x) + c * WG;
@@ -257,17 +264,19 @@ struct GAhead {
 };
 
 // Weak super k-mers with their first (k-1)-mer in [t0, t0 + TILE) (t0 a
-// multiple of P_TILE), whose g values are in s_g: calls emit(g, mb, len,
-// l_disc, r_disc, start) once per piece, where mb is the 4 hash bits above g
-// and start the first base of the record.  Every thread of the WG-thread
+// multiple of P_TILE), whose g values are in s_g (G16_INVALID: no (k-1)-mer):
+// calls emit(g, mb, len, l_disc, r_disc, start) once per piece, where mb is
+// the 4 hash bits above g (from s_mbw, nibble o = position t0 + o; 0 without
+// it) and start the first base of the record.  Every thread of the WG-thread
 // workgroup must call it (it synchronises).  With COLLECT the pieces are first
 // gathered in s_desc (TILE entries), so emit runs on every lane instead of
 // inside the per-position loop.  `carry` is the last g change (+1) before t0
 // (0: none); the same value for the next tile is returned.
 template <int WG, int TILE, bool COLLECT, class Emit>
 __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a, uint64_t t0,
-                                                          unsigned long long carry, const uint32_t* s_g,
-                                                          uint32_t* s_wave, uint64_t* s_desc, Emit&& emit) {
+                                                          unsigned long long carry, const uint16_t* s_g,
+                                                          const uint32_t* s_mbw, uint32_t* s_wave, uint64_t* s_desc,
+                                                          Emit&& emit) {
     constexpr int PPT = TILE / WG;  // positions per thread
     static_assert(PPT <= 32 && TILE % P_TILE == 0, "tile shape");
     const int tid = threadIdx.x;
@@ -275,25 +284,25 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
     const unsigned k = a.k, w = a.w;
 
     // My PPT contiguous positions' g values (and the one before) in
-    // registers: gv[j + 1] is position base_idx + j, read with 128-bit loads.
+    // registers: gv[j + 1] is position base_idx + j, read with a 128-bit load.
     static_assert(PPT == 8, "register layout below");
     typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
     const int base_idx = 1 + tid * PPT;
     uint32_t gv[PPT + 1];
     {
-        const u32x4* row = reinterpret_cast<const u32x4*>(s_g + tid * PPT);
-        const u32x4 r0 = row[0], r1 = row[1], r2 = row[2];
-        gv[0] = r0.x; gv[1] = r0.y; gv[2] = r0.z; gv[3] = r0.w;
-        gv[4] = r1.x; gv[5] = r1.y; gv[6] = r1.z; gv[7] = r1.w;
-        gv[8] = r2.x;
+        const u32x4 r0 = *reinterpret_cast<const u32x4*>(s_g + tid * PPT);
+        gv[0] = r0.x & 0xFFFFu; gv[1] = r0.x >> 16; gv[2] = r0.y & 0xFFFFu; gv[3] = r0.y >> 16;
+        gv[4] = r0.z & 0xFFFFu; gv[5] = r0.z >> 16; gv[6] = r0.w & 0xFFFFu; gv[7] = r0.w >> 16;
+        gv[8] = s_g[tid * PPT + PPT];
     }
-    // Change flags for my positions.
+    // Change flags for my positions (an invalid previous position differs
+    // from every g).
     uint32_t chg = 0, last_chg = 0;
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
         const uint32_t cur = gv[j + 1], prv = gv[j];
-        if (cur == G_INVALID) continue;
-        if (prv == G_INVALID || ((prv ^ cur) & 0xFFFFu)) {
+        if (cur == G16_INVALID) continue;
+        if (prv != cur) {
             chg |= 1u << j;
             last_chg = uint32_t(base_idx + j);
         }
@@ -317,7 +326,7 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
     // min(next boundary, idx + w).
     uint32_t bnd = chg;
 #pragma unroll
-    for (int j = 0; j < PPT; ++j) bnd |= uint32_t(gv[j + 1] == G_INVALID) << j;
+    for (int j = 0; j < PPT; ++j) bnd |= uint32_t(gv[j + 1] == G16_INVALID) << j;
     constexpr uint32_t NONE = 0xFFFFFFFFu;
     uint32_t tail = NONE;  // first boundary in (TILE, TILE + w + 1]
     if (tid < 64) {
@@ -326,7 +335,7 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
             bool b = false;
             if (i <= TILE + int(w) + 1) {
                 const uint32_t cur = s_g[i], prv = s_g[i - 1];
-                b = cur == G_INVALID || prv == G_INVALID || ((prv ^ cur) & 0xFFFFu);
+                b = cur == G16_INVALID || prv != cur;
             }
             const unsigned long long m = __ballot(b);
             if (m) tail = uint32_t(i0 + __ffsll((long long)m) - 1);
@@ -344,9 +353,9 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
     // valid position after an invalid one always starts a run (it has a change
     // flag), so only the thread's first position can need a division.
     uint32_t ph0 = 0;
-    if (rs0 >= 0 && !(chg & 1u) && gv[1] != G_INVALID)
+    if (rs0 >= 0 && !(chg & 1u) && gv[1] != G16_INVALID)
         ph0 = uint32_t(uint64_t(q0 + base_idx - rs0) % w);
-    uint32_t o = 0, n_tile = 0;
+    uint32_t o_desc = 0, n_tile = 0;
     if constexpr (COLLECT) {
         uint32_t n_mine = 0;
         uint32_t ph = ph0;
@@ -354,9 +363,9 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
         for (int j = 0; j < PPT; ++j) {
             if (j) ph = ph + 1 == w ? 0u : ph + 1;
             if ((chg >> j) & 1u) ph = 0;
-            n_mine += ph == 0 && gv[j + 1] != G_INVALID;
+            n_mine += ph == 0 && gv[j + 1] != G16_INVALID;
         }
-        o = block_excl_sum<WG>(n_mine, s_wave, &n_tile);
+        o_desc = block_excl_sum<WG>(n_mine, s_wave, &n_tile);
     }
     {
         uint32_t ph = ph0;
@@ -366,23 +375,25 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
             if ((chg >> j) & 1u) ph = 0;
             const int idx = base_idx + j;
             const uint32_t cur = gv[j + 1];
-            if (cur == G_INVALID || ph) continue;
+            if (cur == G16_INVALID || ph) continue;
             // Piece [idx, e): (k-1)-mers with this g, at most w of them.
-            const uint32_t g = cur & 0xFFFFu;
+            const uint32_t g = cur;
+            const uint32_t o = uint32_t(idx) - 1u;
+            const uint32_t mb = s_mbw ? (s_mbw[o >> 3] >> (4 * (o & 7u))) & 15u : 0u;
             const uint32_t later = bnd >> (j + 1);  // my boundaries after position j
             const uint32_t nb = later ? uint32_t(idx) + uint32_t(__ffs(int(later))) : next_bnd;
             const int e = int(min(nb, uint32_t(idx) + w));
             const uint32_t nv = s_g[e], pv = gv[j];
-            const uint32_t rj = nv != G_INVALID, lj = pv != G_INVALID;
-            const uint32_t l_disc = lj && ((pv & 0xFFFFu) != g);
-            const uint32_t r_disc = rj && ((nv & 0xFFFFu) != g);
+            const uint32_t rj = nv != G16_INVALID, lj = pv != G16_INVALID;
+            const uint32_t l_disc = lj && pv != g;
+            const uint32_t r_disc = rj && nv != g;
             const uint32_t len = lj + uint32_t(e - idx) + (k - 2) + rj;
             if constexpr (COLLECT)
-                s_desc[o++] = uint64_t(idx) | (uint64_t(g) << 16) | (uint64_t((cur >> 16) & 15u) << 32) |
+                s_desc[o_desc++] = uint64_t(idx) | (uint64_t(g) << 16) | (uint64_t(mb) << 32) |
                               (uint64_t(len) << 36) | (uint64_t(l_disc) << 44) | (uint64_t(r_disc) << 45) |
                               (uint64_t(lj) << 46);
             else
-                emit(g, (cur >> 16) & 15u, len, l_disc, r_disc, uint64_t(q0 + int64_t(idx)) - lj);
+                emit(g, mb, len, l_disc, r_disc, uint64_t(q0 + int64_t(idx)) - lj);
         }
     }
     if constexpr (COLLECT) {
@@ -424,10 +435,10 @@ __device__ __forceinline__ Span span_of(uint64_t n_bases, uint64_t per) {
 
 // Pass 2: records per coarse bin (g >> FINE_BITS) of each count span into
 // hist[c * gridDim.x + span].  No global atomics.
-__global__ __launch_bounds__(H_WG) void k_pieces_hist(PartitionArgs a, const uint32_t* __restrict__ gbin,
+__global__ __launch_bounds__(H_WG) void k_pieces_hist(PartitionArgs a, const uint16_t* __restrict__ gbin,
                                                       const uint64_t* __restrict__ tile_lc, uint64_t per,
                                                       uint32_t* __restrict__ hist) {
-    __shared__ __attribute__((aligned(16))) uint32_t s_g[P_TILE + WIN_MAX + 2];
+    __shared__ __attribute__((aligned(16))) uint16_t s_g[P_TILE + WIN_MAX + 2];
     __shared__ unsigned long long s_carry;
     __shared__ uint32_t s_wave[H_WG / 64];
     __shared__ uint32_t s_hist[N_COARSE];
@@ -448,7 +459,7 @@ __global__ __launch_bounds__(H_WG) void k_pieces_hist(PartitionArgs a, const uin
         ga.store(s_g);
         if (t + 1 < sp.t_end) ga.load(a, gbin, (t + 1) * P_TILE);
         __syncthreads();
-        carry = tile_pieces<H_WG, P_TILE, false>(a, t * P_TILE, carry, s_g, s_wave, nullptr,
+        carry = tile_pieces<H_WG, P_TILE, false>(a, t * P_TILE, carry, s_g, nullptr, s_wave, nullptr,
                                                  [&](uint32_t g, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t) {
                                                      atomicAdd(&s_hist[g >> FINE_BITS], 1u);
                                                  });
@@ -508,12 +519,14 @@ __global__ __launch_bounds__(N_COARSE) void k_coarse_scan(const uint64_t* __rest
 // run is written front to back by one workgroup.  Level-1 records: the
 // header word att | mbin << 32 in hdr, the label in lab as whole 16-byte
 // pairs (l1_label_words(W) words).
-__global__ __launch_bounds__(E_WG, 4) void k_pieces_emit(PartitionArgs a, const uint32_t* __restrict__ gbin,
+__global__ __launch_bounds__(E_WG, 4) void k_pieces_emit(PartitionArgs a, const uint16_t* __restrict__ gbin,
+                                                      const uint32_t* __restrict__ mbn,
                                                       const uint64_t* __restrict__ tile_lc, uint64_t per,
                                                       uint32_t n_hist, const uint32_t* __restrict__ hist,
                                                       const uint64_t* __restrict__ coff, uint32_t W,
                                                       uint64_t* __restrict__ hdr, uint64_t* __restrict__ lab) {
-    __shared__ __attribute__((aligned(16))) uint32_t s_g[E_TILE + WIN_MAX + 2];
+    __shared__ __attribute__((aligned(16))) uint16_t s_g[E_TILE + WIN_MAX + 2];
+    __shared__ uint32_t s_mbw[E_TILE / 8];
     __shared__ unsigned long long s_carry;
     __shared__ uint32_t s_wave[E_WG / 64];
     __shared__ uint64_t s_words[TILE_WORDS(E_TILE)];
@@ -528,10 +541,15 @@ __global__ __launch_bounds__(E_WG, 4) void k_pieces_emit(PartitionArgs a, const 
     // current one is processed
     GAhead<E_WG, E_TILE> ga;
     uint64_t word_next = 0;
+    uint32_t mbw_next = 0;
+    static_assert(E_TILE / 8 <= E_WG, "one nibble word per thread");
+    const uint64_t n_mbw = (a.n_bases + 7) / 8;
     auto load_words = [&](uint64_t t0) {
         const uint64_t wlo = t0 ? t0 / 32 - 1 : 0;
         const uint64_t wi = wlo + threadIdx.x;
         word_next = threadIdx.x < TILE_WORDS(E_TILE) && wi < a.n_words ? a.packed[wi] : 0;
+        const uint64_t mi = t0 / 8 + threadIdx.x;
+        mbw_next = threadIdx.x < E_TILE / 8 && mi < n_mbw ? mbn[mi] : 0u;
     };
     if (sp.t_begin < sp.t_end) {
         ga.load(a, gbin, sp.t_begin * P_TILE);
@@ -548,6 +566,7 @@ __global__ __launch_bounds__(E_WG, 4) void k_pieces_emit(PartitionArgs a, const 
         __syncthreads();  // the previous tile is done with s_g, s_words and s_desc
         ga.store(s_g);
         if (threadIdx.x < TILE_WORDS(E_TILE)) s_words[threadIdx.x] = word_next;
+        if (threadIdx.x < E_TILE / 8) s_mbw[threadIdx.x] = mbw_next;
         if (t + STEP < sp.t_end) {
             ga.load(a, gbin, (t + STEP) * P_TILE);
             load_words((t + STEP) * P_TILE);
@@ -555,7 +574,7 @@ __global__ __launch_bounds__(E_WG, 4) void k_pieces_emit(PartitionArgs a, const 
         __syncthreads();
         const uint64_t wlo = t0 ? t0 / 32 - 1 : 0;  // s_words[0] holds bases wlo*32 ..
         const uint64_t base0 = wlo * 32;
-        carry = tile_pieces<E_WG, E_TILE, true>(a, t0, carry, s_g, s_wave, s_desc,
+        carry = tile_pieces<E_WG, E_TILE, true>(a, t0, carry, s_g, s_mbw, s_wave, s_desc,
                                         [&](uint32_t g, uint32_t mb, uint32_t len, uint32_t l_disc,
                                             uint32_t r_disc, uint64_t start) {
                                             const uint64_t slot = atomicAdd(&s_cur[g >> FINE_BITS], 1ull);
@@ -717,12 +736,12 @@ __global__ __launch_bounds__(256) void k_bucket_features(const uint64_t* __restr
 
 }  // namespace
 
-void launch_minimizer_g(const PartitionArgs& a, uint32_t* gbin, uint64_t* tile_lc, hipStream_t s) {
+void launch_minimizer_g(const PartitionArgs& a, uint16_t* gbin, uint32_t* mbn, uint64_t* tile_lc, hipStream_t s) {
     const uint64_t tiles = (a.n_bases + P_TILE - 1) / P_TILE;
     if (!tiles) return;
     const uint32_t fg = uint32_t(std::min<uint64_t>((a.n_frag + 255) / 256, 65536));
     hipLaunchKernelGGL(k_tile_frag, dim3(fg), dim3(256), 0, s, a.frag_off, a.n_frag, tiles, a.tile_frag);
-    hipLaunchKernelGGL(k_minimizer_g, dim3(uint32_t(tiles)), dim3(P_WG), 0, s, a, gbin, tile_lc);
+    hipLaunchKernelGGL(k_minimizer_g, dim3(uint32_t(tiles)), dim3(P_WG), 0, s, a, gbin, mbn, tile_lc);
 }
 
 PieceSpans pieces_spans(uint64_t n_bases, uint32_t n_cu) {
@@ -741,7 +760,7 @@ PieceSpans pieces_spans(uint64_t n_bases, uint32_t n_cu) {
     return ps;
 }
 
-void launch_pieces_hist(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, const PieceSpans& ps,
+void launch_pieces_hist(const PartitionArgs& a, const uint16_t* gbin, const uint64_t* tile_lc, const PieceSpans& ps,
                         uint32_t* hist, uint64_t* ctot, uint64_t* coff, hipStream_t s) {
     if (!ps.n_count) return;
     hipLaunchKernelGGL(k_pieces_hist, dim3(ps.n_count), dim3(H_WG), 0, s, a, gbin, tile_lc, ps.per_count, hist);
@@ -749,11 +768,12 @@ void launch_pieces_hist(const PartitionArgs& a, const uint32_t* gbin, const uint
     hipLaunchKernelGGL(k_coarse_scan, dim3(1), dim3(N_COARSE), 0, s, ctot, coff);
 }
 
-void launch_pieces_emit(const PartitionArgs& a, const uint32_t* gbin, const uint64_t* tile_lc, const PieceSpans& ps,
+void launch_pieces_emit(const PartitionArgs& a, const uint16_t* gbin, const uint32_t* mbn, const uint64_t* tile_lc,
+                        const PieceSpans& ps,
                         const uint32_t* hist, const uint64_t* coff, uint32_t W, uint64_t* hdr1, uint64_t* lab1,
                         hipStream_t s) {
     if (!ps.n_emit) return;
-    hipLaunchKernelGGL(k_pieces_emit, dim3(ps.n_emit), dim3(E_WG), 0, s, a, gbin, tile_lc, ps.per_emit,
+    hipLaunchKernelGGL(k_pieces_emit, dim3(ps.n_emit), dim3(E_WG), 0, s, a, gbin, mbn, tile_lc, ps.per_emit,
                        ps.n_count, hist, coff, W, hdr1, lab1);
 }
 
