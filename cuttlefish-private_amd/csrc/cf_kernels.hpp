@@ -44,12 +44,14 @@ constexpr int E_SPANS_PER_CU = 8;  // emit spans (workgroups) per CU
 constexpr int H_WG = 256;          // count workgroup, one minimizer tile per step
 static_assert((N_COARSE << FINE_BITS) == 16384, "coarse x fine bins cover the 16384 subgraphs");
 constexpr int H_SPLIT = 1;  // count spans per emit span
-// Level-1 records: a header array (att | mbin << 32, one word per record) and
-// a label array of l1_label_words(W) words per record (W padded to even, so
-// labels are 16-byte aligned pairs).
+// Level-1 records: a header array (one u32 per record: att with the 4 hash
+// bits above g in its free bits 10-13) and a label array of l1_label_words(W)
+// words per record (W padded to even, so labels are 16-byte aligned pairs).
 __host__ __device__ constexpr uint32_t l1_label_words(uint32_t W) { return (W + 1) & ~1u; }
-// Offset of the label array in a level-1 buffer of n records (kept 16-byte aligned).
-__host__ __device__ constexpr uint64_t l1_label_offset(uint64_t n) { return (n + 1) & ~1ull; }
+// Offset in u64 words of the label array in a level-1 buffer of n records
+// (after 4n header bytes, 16-byte aligned).
+__host__ __device__ constexpr uint64_t l1_label_offset(uint64_t n) { return (n + 3) / 4 * 2; }
+constexpr uint32_t L1_MB_SHIFT = 10;
 struct PieceSpans {
     uint64_t per_count, per_emit;  // minimizer tiles (P_TILE positions) per count / emit span
     uint32_t n_count, n_emit;      // spans (workgroups)
@@ -63,10 +65,10 @@ void launch_pieces_hist(const PartitionArgs& a, const uint16_t* gbin, const uint
 // Pass 3: records into the level-1 arrays (hdr1, lab1), grouped by coarse bin.
 void launch_pieces_emit(const PartitionArgs& a, const uint16_t* gbin, const uint32_t* mbn, const uint64_t* tile_lc,
                         const PieceSpans& ps,
-                        const uint32_t* hist, const uint64_t* coff, uint32_t W, uint64_t* hdr1, uint64_t* lab1,
+                        const uint32_t* hist, const uint64_t* coff, uint32_t W, uint32_t* hdr1, uint64_t* lab1,
                         hipStream_t s);
 // Level 2: level-1 records into per-bucket order; boff[G+1] bucket offsets.
-void launch_bucket_fine(const uint64_t* hdr1, const uint64_t* lab1, const uint64_t* coff, uint32_t W, uint64_t* boff,
+void launch_bucket_fine(const uint32_t* hdr1, const uint64_t* lab1, const uint64_t* coff, uint32_t W, uint64_t* boff,
                         uint32_t* att, uint64_t* labels, uint8_t* mbin, hipStream_t s);
 
 // Per-bucket statistics: [n_rec, n_kmer_instances, n_bases, 0] as u64 and the

@@ -699,11 +699,28 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
         }
         rs.n_samples = uint32_t(sample.size());
         if (!sample.empty()) {
-            std::vector<uint64_t> l2(sample.size());
+            // LDS passes sized by the k-mer instances the construct walks (an
+            // upper bound on the distinct vertices), the global kernel for
+            // buckets needing more than max_passes; direct launches (the main
+            // round keeps the captured graph).
             order_largest_first(sample, cost);
-            for (size_t i = 0; i < sample.size(); ++i) l2[i] = upper_cap(h, sample[i]);
-            std::vector<uint32_t> so;
-            CK(run_round_batched(h, sample, l2, so, false));
+            const double C = double(lds_capacity(int(h->KW)));
+            std::vector<uint32_t> lds_s, glob_s, so, so2;
+            std::vector<uint64_t> lds_c, glob_c;
+            for (uint32_t b : sample) {
+                const uint64_t inst = std::max<uint64_t>(1, walked_kmers(h, b));
+                const uint32_t P = uint32_t(std::ceil(double(inst) / (C * LDS_LOAD)));
+                if (P <= h->tune.max_passes) {
+                    lds_s.push_back(b);
+                    lds_c.push_back(uint64_t(P) * uint64_t(C));
+                } else {
+                    glob_s.push_back(b);
+                    glob_c.push_back(upper_cap(h, b));
+                }
+            }
+            if (!glob_s.empty()) CK(run_round_batched(h, glob_s, glob_c, so, false));
+            if (!lds_s.empty()) CK(run_round_batched(h, lds_s, lds_c, so2, false, true));
+            so.insert(so.end(), so2.begin(), so2.end());
             CK(fallback(h, so, false));
             for (uint32_t b : sample) h->h_flags[b] |= CFGPU_BUCKET_SAMPLE;
         }
@@ -963,9 +980,9 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     HIPCK(h, hipEventRecord(e3.e, h->stream));
     if (n_frag)
         launch_pieces_emit(a, h->gbin.as<uint16_t>(), h->mbn.as<uint32_t>(), h->tile_lc.as<uint64_t>(), ps, h->hist.as<uint32_t>(),
-                           h->coff.as<uint64_t>(), h->W, h->rec1.as<uint64_t>(), h->rec1.as<uint64_t>() + lab_off,
+                           h->coff.as<uint64_t>(), h->W, h->rec1.as<uint32_t>(), h->rec1.as<uint64_t>() + lab_off,
                            h->stream);
-    launch_bucket_fine(h->rec1.as<uint64_t>(), h->rec1.as<uint64_t>() + lab_off, h->coff.as<uint64_t>(), h->W,
+    launch_bucket_fine(h->rec1.as<uint32_t>(), h->rec1.as<uint64_t>() + lab_off, h->coff.as<uint64_t>(), h->W,
                        h->boff.as<uint64_t>(), h->att.as<uint32_t>(), h->labels.as<uint64_t>(), h->mbin.as<uint8_t>(),
                        h->stream);
     HIPCK(h, hipEventRecord(e4.e, h->stream));

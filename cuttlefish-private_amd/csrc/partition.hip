@@ -517,14 +517,14 @@ __global__ __launch_bounds__(N_COARSE) void k_coarse_scan(const uint64_t* __rest
 // its records of coarse bin c to the run starting at coff[c] +
 // hist[c][s * H_SPLIT].  A span is a contiguous part of the input, so each
 // run is written front to back by one workgroup.  Level-1 records: the
-// header word att | mbin << 32 in hdr, the label in lab as whole 16-byte
+// header att | mbin << L1_MB_SHIFT (u32) in hdr, the label in lab as whole 16-byte
 // pairs (l1_label_words(W) words).
 __global__ __launch_bounds__(E_WG, 4) void k_pieces_emit(PartitionArgs a, const uint16_t* __restrict__ gbin,
                                                       const uint32_t* __restrict__ mbn,
                                                       const uint64_t* __restrict__ tile_lc, uint64_t per,
                                                       uint32_t n_hist, const uint32_t* __restrict__ hist,
                                                       const uint64_t* __restrict__ coff, uint32_t W,
-                                                      uint64_t* __restrict__ hdr, uint64_t* __restrict__ lab) {
+                                                      uint32_t* __restrict__ hdr, uint64_t* __restrict__ lab) {
     __shared__ __attribute__((aligned(16))) uint16_t s_g[E_TILE + WIN_MAX + 2];
     __shared__ uint32_t s_mbw[E_TILE / 8];
     __shared__ unsigned long long s_carry;
@@ -578,8 +578,7 @@ __global__ __launch_bounds__(E_WG, 4) void k_pieces_emit(PartitionArgs a, const 
                                         [&](uint32_t g, uint32_t mb, uint32_t len, uint32_t l_disc,
                                             uint32_t r_disc, uint64_t start) {
                                             const uint64_t slot = atomicAdd(&s_cur[g >> FINE_BITS], 1ull);
-                                            hdr[slot] = uint64_t(len | (l_disc << 8) | (r_disc << 9) | (g << 16)) |
-                                                        (uint64_t(mb) << 32);
+                                            hdr[slot] = len | (l_disc << 8) | (r_disc << 9) | (mb << L1_MB_SHIFT) | (g << 16);
                                             // label words two at a time (16-byte stores)
                                             u64x2* out = reinterpret_cast<u64x2*>(lab + slot * l1_label_words(W));
                                             uint64_t prev = 0;
@@ -611,7 +610,7 @@ __host__ __device__ constexpr uint32_t fine_chunk(uint32_t W) {
     return (FINE_LDS / (13 + 8 * W)) >= 4096 ? 4096 : (FINE_LDS / (13 + 8 * W)) >= 2048 ? 2048 : 1024;
 }
 
-__global__ __launch_bounds__(FINE_WG) void k_bucket_fine(const uint64_t* __restrict__ hdr,
+__global__ __launch_bounds__(FINE_WG) void k_bucket_fine(const uint32_t* __restrict__ hdr,
                                                          const uint64_t* __restrict__ lab,
                                                          const uint64_t* __restrict__ coff, uint32_t W,
                                                          uint64_t* __restrict__ boff, uint32_t* __restrict__ att,
@@ -632,7 +631,7 @@ __global__ __launch_bounds__(FINE_WG) void k_bucket_fine(const uint64_t* __restr
 
     for (uint32_t i = tid; i < NF; i += FINE_WG) s_cnt[i] = 0;
     __syncthreads();
-    for (uint64_t r = r0 + tid; r < r1; r += FINE_WG) atomicAdd(&s_cnt[(uint32_t(hdr[r]) >> 16) & (NF - 1)], 1u);
+    for (uint64_t r = r0 + tid; r < r1; r += FINE_WG) atomicAdd(&s_cnt[(hdr[r] >> 16) & (NF - 1)], 1u);
     __syncthreads();
     if (tid == 0) {
         uint64_t run = r0;
@@ -654,10 +653,10 @@ __global__ __launch_bounds__(FINE_WG) void k_bucket_fine(const uint64_t* __restr
             const uint32_t i = tid + j * FINE_WG;
             rank[j] = fb[j] = 0;
             if (i >= n) continue;
-            const uint64_t h0 = hdr[q0 + i];
-            const uint32_t at = uint32_t(h0);
+            const uint32_t h0 = hdr[q0 + i];
+            const uint32_t at = h0 & ~(15u << L1_MB_SHIFT);
             s_att[i] = at;
-            s_mb[i] = uint8_t(h0 >> 32);
+            s_mb[i] = uint8_t((h0 >> L1_MB_SHIFT) & 15u);
             const u64x2* in = reinterpret_cast<const u64x2*>(lab + (q0 + i) * LW);
             for (uint32_t q = 0; q < W; q += 2) {
                 const u64x2 pq = in[q / 2];
@@ -770,14 +769,14 @@ void launch_pieces_hist(const PartitionArgs& a, const uint16_t* gbin, const uint
 
 void launch_pieces_emit(const PartitionArgs& a, const uint16_t* gbin, const uint32_t* mbn, const uint64_t* tile_lc,
                         const PieceSpans& ps,
-                        const uint32_t* hist, const uint64_t* coff, uint32_t W, uint64_t* hdr1, uint64_t* lab1,
+                        const uint32_t* hist, const uint64_t* coff, uint32_t W, uint32_t* hdr1, uint64_t* lab1,
                         hipStream_t s) {
     if (!ps.n_emit) return;
     hipLaunchKernelGGL(k_pieces_emit, dim3(ps.n_emit), dim3(E_WG), 0, s, a, gbin, mbn, tile_lc, ps.per_emit,
                        ps.n_count, hist, coff, W, hdr1, lab1);
 }
 
-void launch_bucket_fine(const uint64_t* hdr1, const uint64_t* lab1, const uint64_t* coff, uint32_t W, uint64_t* boff,
+void launch_bucket_fine(const uint32_t* hdr1, const uint64_t* lab1, const uint64_t* coff, uint32_t W, uint64_t* boff,
                         uint32_t* att,
                         uint64_t* labels, uint8_t* mbin, hipStream_t s) {
     hipLaunchKernelGGL(k_bucket_fine, dim3(N_COARSE), dim3(FINE_WG), 0, s, hdr1, lab1, coff, W, boff, att, labels, mbin);
