@@ -38,12 +38,12 @@ void launch_minimizer_g(const PartitionArgs& a, uint16_t* gbin, uint32_t* mbn, u
 // coarse bin into its 2^FINE_BITS buckets.
 constexpr int FINE_BITS = 6;
 constexpr int N_COARSE = 256;
-constexpr int E_WG = 256;          // emit workgroup, E_TILE positions per step
-constexpr int E_TILE = 2048;
-constexpr int E_SPANS_PER_CU = 8;  // emit spans (workgroups) per CU
+constexpr int E_WG = 1024;         // emit workgroup (its LDS staging fills most of a CU)
+constexpr int E_TILE = P_TILE;     // emit spans are whole minimizer tiles
+constexpr int E_SPANS_PER_CU = 1;  // emit spans (workgroups) per CU
 constexpr int H_WG = 256;          // count workgroup, one minimizer tile per step
 static_assert((N_COARSE << FINE_BITS) == 16384, "coarse x fine bins cover the 16384 subgraphs");
-constexpr int H_SPLIT = 1;  // count spans per emit span
+constexpr int H_SPLIT = 8;  // count spans per emit span
 // Level-1 records: a header array (one u32 per record: att with the 4 hash
 // bits above g in its free bits 10-13) and a label array of l1_label_words(W)
 // words per record (W padded to even, so labels are 16-byte aligned pairs).
@@ -57,14 +57,16 @@ struct PieceSpans {
     uint32_t n_count, n_emit;      // spans (workgroups)
 };
 PieceSpans pieces_spans(uint64_t n_bases, uint32_t n_cu);
-// Pass 2: record counts per (coarse bin, count span) in hist[N_COARSE][n_count],
-// scanned in place; coff[N_COARSE+1] = coarse bin offsets (coff[N_COARSE] =
-// records); ctot[N_COARSE] scratch.
-void launch_pieces_hist(const PartitionArgs& a, const uint16_t* gbin, const uint64_t* tile_lc, const PieceSpans& ps,
-                        uint32_t* hist, uint64_t* ctot, uint64_t* coff, hipStream_t s);
-// Pass 3: records into the level-1 arrays (hdr1, lab1), grouped by coarse bin.
-void launch_pieces_emit(const PartitionArgs& a, const uint16_t* gbin, const uint32_t* mbn, const uint64_t* tile_lc,
-                        const PieceSpans& ps,
+// Pass 2: every tile's piece descriptors into desc[t * P_TILE ..] (tile_n[t]
+// of them; desc holds P_TILE per tile), record counts per (coarse bin, count
+// span) in hist[N_COARSE][n_count], scanned in place; coff[N_COARSE+1] =
+// coarse bin offsets (coff[N_COARSE] = records); ctot[N_COARSE] scratch.
+void launch_pieces_hist(const PartitionArgs& a, const uint16_t* gbin, const uint32_t* mbn, const uint64_t* tile_lc,
+                        const PieceSpans& ps, uint32_t* hist, uint64_t* ctot, uint64_t* coff, uint64_t* desc,
+                        uint32_t* tile_n, hipStream_t s);
+// Pass 3: the described pieces as records into the level-1 arrays (hdr1,
+// lab1), grouped by coarse bin.
+void launch_pieces_emit(const PartitionArgs& a, const uint64_t* desc, const uint32_t* tile_n, const PieceSpans& ps,
                         const uint32_t* hist, const uint64_t* coff, uint32_t W, uint32_t* hdr1, uint64_t* lab1,
                         hipStream_t s);
 // Level 2: level-1 records into per-bucket order; boff[G+1] bucket offsets.

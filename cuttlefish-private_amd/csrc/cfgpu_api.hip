@@ -91,6 +91,7 @@ struct cfgpu_handle {
 
     // partition state
     DevBuf packed, frag, gbin, mbn, tile_lc, tile_frag, boff, att, labels, mbin, stats, feat;
+    DevBuf pdesc, tile_n;  // piece descriptors (P_TILE per tile) and per-tile counts
     DevBuf hist, ctot, coff, rec1;  // level-1 bucketing
     uint64_t n_bases = 0, n_frag = 0, n_records = 0;
     bool partitioned = false;
@@ -938,6 +939,8 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     HIPCK(h, h->gbin.ensure(n_bases * 2));
     HIPCK(h, h->mbn.ensure(((n_bases + 7) / 8) * 4));
     HIPCK(h, h->tile_lc.ensure(tiles * 8));
+    HIPCK(h, h->pdesc.ensure(tiles * P_TILE * 8));
+    HIPCK(h, h->tile_n.ensure(tiles * 4));
     HIPCK(h, h->tile_frag.ensure(tiles * 8));
     HIPCK(h, h->hist.ensure(size_t(N_COARSE) * ps.n_count * 4));
     HIPCK(h, h->ctot.ensure(size_t(N_COARSE) * 8));
@@ -966,8 +969,9 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     if (n_frag) launch_minimizer_g(a, h->gbin.as<uint16_t>(), h->mbn.as<uint32_t>(), h->tile_lc.as<uint64_t>(), h->stream);
     HIPCK(h, hipEventRecord(e1.e, h->stream));
     if (n_frag)
-        launch_pieces_hist(a, h->gbin.as<uint16_t>(), h->tile_lc.as<uint64_t>(), ps, h->hist.as<uint32_t>(),
-                           h->ctot.as<uint64_t>(), h->coff.as<uint64_t>(), h->stream);
+        launch_pieces_hist(a, h->gbin.as<uint16_t>(), h->mbn.as<uint32_t>(), h->tile_lc.as<uint64_t>(), ps,
+                           h->hist.as<uint32_t>(), h->ctot.as<uint64_t>(), h->coff.as<uint64_t>(),
+                           h->pdesc.as<uint64_t>(), h->tile_n.as<uint32_t>(), h->stream);
     HIPCK(h, hipEventRecord(e2.e, h->stream));
     uint64_t n_rec = 0;
     HIPCK(h, hipMemcpyAsync(&n_rec, h->coff.as<uint64_t>() + N_COARSE, 8, hipMemcpyDeviceToHost, h->stream));
@@ -979,7 +983,7 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     HIPCK(h, h->mbin.ensure(n_rec));
     HIPCK(h, hipEventRecord(e3.e, h->stream));
     if (n_frag)
-        launch_pieces_emit(a, h->gbin.as<uint16_t>(), h->mbn.as<uint32_t>(), h->tile_lc.as<uint64_t>(), ps, h->hist.as<uint32_t>(),
+        launch_pieces_emit(a, h->pdesc.as<uint64_t>(), h->tile_n.as<uint32_t>(), ps, h->hist.as<uint32_t>(),
                            h->coff.as<uint64_t>(), h->W, h->rec1.as<uint32_t>(), h->rec1.as<uint64_t>() + lab_off,
                            h->stream);
     launch_bucket_fine(h->rec1.as<uint32_t>(), h->rec1.as<uint64_t>() + lab_off, h->coff.as<uint64_t>(), h->W,

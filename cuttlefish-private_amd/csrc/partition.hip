@@ -263,20 +263,28 @@ struct GAhead {
     }
 };
 
+// Piece descriptor (one u64 per weak super k-mer): idx (bits 0-15, its first
+// (k-1)-mer at tile position t0 - 1 + idx), g (16-31), the 4 hash bits above
+// g (32-35), len (36-43), l_disc (44), r_disc (45), l_joined (46); the
+// record starts at base t0 - 1 + idx - l_joined.
+__host__ __device__ constexpr uint64_t piece_desc(uint32_t idx, uint32_t g, uint32_t mb, uint32_t len,
+                                                  uint32_t l_disc, uint32_t r_disc, uint32_t lj) {
+    return uint64_t(idx) | (uint64_t(g) << 16) | (uint64_t(mb) << 32) | (uint64_t(len) << 36) |
+           (uint64_t(l_disc) << 44) | (uint64_t(r_disc) << 45) | (uint64_t(lj) << 46);
+}
+
 // Weak super k-mers with their first (k-1)-mer in [t0, t0 + TILE) (t0 a
-// multiple of P_TILE), whose g values are in s_g (G16_INVALID: no (k-1)-mer):
-// calls emit(g, mb, len, l_disc, r_disc, start) once per piece, where mb is
-// the 4 hash bits above g (from s_mbw, nibble o = position t0 + o; 0 without
-// it) and start the first base of the record.  Every thread of the WG-thread
-// workgroup must call it (it synchronises).  With COLLECT the pieces are first
-// gathered in s_desc (TILE entries), so emit runs on every lane instead of
-// inside the per-position loop.  `carry` is the last g change (+1) before t0
-// (0: none); the same value for the next tile is returned.
-template <int WG, int TILE, bool COLLECT, class Emit>
+// multiple of P_TILE), whose g values are in s_g (G16_INVALID: no (k-1)-mer)
+// and the 4 hash bits above g in s_mbw (nibble o = position t0 + o): calls
+// emit(o, descriptor) once per piece, o = 0 .. n_out - 1 in position order.
+// Every thread of the WG-thread workgroup must call it (it synchronises).
+// `carry` is the last g change (+1) before t0 (0: none); the same value for
+// the next tile is returned.
+template <int WG, int TILE, class Emit>
 __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a, uint64_t t0,
                                                           unsigned long long carry, const uint16_t* s_g,
-                                                          const uint32_t* s_mbw, uint32_t* s_wave, uint64_t* s_desc,
-                                                          Emit&& emit) {
+                                                          const uint32_t* s_mbw, uint32_t* s_wave,
+                                                          uint32_t& n_out, Emit&& emit) {
     constexpr int PPT = TILE / WG;  // positions per thread
     static_assert(PPT <= 32 && TILE % P_TILE == 0, "tile shape");
     const int tid = threadIdx.x;
@@ -356,7 +364,7 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
     if (rs0 >= 0 && !(chg & 1u) && gv[1] != G16_INVALID)
         ph0 = uint32_t(uint64_t(q0 + base_idx - rs0) % w);
     uint32_t o_desc = 0, n_tile = 0;
-    if constexpr (COLLECT) {
+    {
         uint32_t n_mine = 0;
         uint32_t ph = ph0;
 #pragma unroll
@@ -379,7 +387,7 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
             // Piece [idx, e): (k-1)-mers with this g, at most w of them.
             const uint32_t g = cur;
             const uint32_t o = uint32_t(idx) - 1u;
-            const uint32_t mb = s_mbw ? (s_mbw[o >> 3] >> (4 * (o & 7u))) & 15u : 0u;
+            const uint32_t mb = (s_mbw[o >> 3] >> (4 * (o & 7u))) & 15u;
             const uint32_t later = bnd >> (j + 1);  // my boundaries after position j
             const uint32_t nb = later ? uint32_t(idx) + uint32_t(__ffs(int(later))) : next_bnd;
             const int e = int(min(nb, uint32_t(idx) + w));
@@ -388,24 +396,10 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
             const uint32_t l_disc = lj && pv != g;
             const uint32_t r_disc = rj && nv != g;
             const uint32_t len = lj + uint32_t(e - idx) + (k - 2) + rj;
-            if constexpr (COLLECT)
-                s_desc[o_desc++] = uint64_t(idx) | (uint64_t(g) << 16) | (uint64_t(mb) << 32) |
-                              (uint64_t(len) << 36) | (uint64_t(l_disc) << 44) | (uint64_t(r_disc) << 45) |
-                              (uint64_t(lj) << 46);
-            else
-                emit(g, mb, len, l_disc, r_disc, uint64_t(q0 + int64_t(idx)) - lj);
+            emit(o_desc++, piece_desc(uint32_t(idx), g, mb, len, l_disc, r_disc, lj));
         }
     }
-    if constexpr (COLLECT) {
-        __syncthreads();
-        // Every piece of the tile, one lane each.
-        for (uint32_t i = tid; i < n_tile; i += WG) {
-            const uint64_t d = s_desc[i];
-            const uint32_t idx = uint32_t(d & 0xFFFFu), lj = uint32_t(d >> 46) & 1u;
-            emit(uint32_t(d >> 16) & 0xFFFFu, uint32_t(d >> 32) & 15u, uint32_t(d >> 36) & 0xFFu,
-                 uint32_t(d >> 44) & 1u, uint32_t(d >> 45) & 1u, uint64_t(q0 + int64_t(idx)) - lj);
-        }
-    }
+    n_out = n_tile;
     return carry_out;
 }
 
@@ -433,20 +427,35 @@ __device__ __forceinline__ Span span_of(uint64_t n_bases, uint64_t per) {
     return {b, min(n_t, b + per)};
 }
 
-// Pass 2: records per coarse bin (g >> FINE_BITS) of each count span into
-// hist[c * gridDim.x + span].  No global atomics.
+// Pass 2: the pieces of every tile: descriptors to desc[t * P_TILE + o]
+// (o < tile_n[t] <= P_TILE), and records per coarse bin (g >> FINE_BITS) of
+// each count span into hist[c * gridDim.x + span].  No global atomics.
 __global__ __launch_bounds__(H_WG) void k_pieces_hist(PartitionArgs a, const uint16_t* __restrict__ gbin,
+                                                      const uint32_t* __restrict__ mbn,
                                                       const uint64_t* __restrict__ tile_lc, uint64_t per,
-                                                      uint32_t* __restrict__ hist) {
+                                                      uint32_t* __restrict__ hist, uint64_t* __restrict__ desc,
+                                                      uint32_t* __restrict__ tile_n) {
+    static_assert(P_TILE / 8 == H_WG, "one nibble word per thread");
     __shared__ __attribute__((aligned(16))) uint16_t s_g[P_TILE + WIN_MAX + 2];
+    __shared__ uint32_t s_mbw[P_TILE / 8];
     __shared__ unsigned long long s_carry;
     __shared__ uint32_t s_wave[H_WG / 64];
     __shared__ uint32_t s_hist[N_COARSE];
+    __shared__ uint64_t s_desc[P_TILE];  // the tile's descriptors, written out coalesced
     for (int i = threadIdx.x; i < N_COARSE; i += H_WG) s_hist[i] = 0;
     const Span sp = span_of(a.n_bases, per);
+    const uint64_t n_mbw = (a.n_bases + 7) / 8;
+    // g values and hash-bit words of the next tile are loaded while the
+    // current one is processed
     GAhead<H_WG, P_TILE> ga;
+    uint32_t mbw_next = 0;
+    auto load_mbw = [&](uint64_t t) {
+        const uint64_t mi = t * (P_TILE / 8) + threadIdx.x;
+        mbw_next = mi < n_mbw ? mbn[mi] : 0u;
+    };
     if (sp.t_begin < sp.t_end) {
         ga.load(a, gbin, sp.t_begin * P_TILE);
+        load_mbw(sp.t_begin);
         if (threadIdx.x < 64) {
             const unsigned long long c = carry_before(tile_lc, sp.t_begin);
             if (threadIdx.x == 0) s_carry = c;
@@ -455,14 +464,25 @@ __global__ __launch_bounds__(H_WG) void k_pieces_hist(PartitionArgs a, const uin
     __syncthreads();
     unsigned long long carry = s_carry;
     for (uint64_t t = sp.t_begin; t < sp.t_end; ++t) {
-        __syncthreads();  // the previous tile is done with s_g
+        __syncthreads();  // the previous tile is done with s_g, s_mbw and s_desc
         ga.store(s_g);
-        if (t + 1 < sp.t_end) ga.load(a, gbin, (t + 1) * P_TILE);
+        s_mbw[threadIdx.x] = mbw_next;
+        if (t + 1 < sp.t_end) {
+            ga.load(a, gbin, (t + 1) * P_TILE);
+            load_mbw(t + 1);
+        }
         __syncthreads();
-        carry = tile_pieces<H_WG, P_TILE, false>(a, t * P_TILE, carry, s_g, nullptr, s_wave, nullptr,
-                                                 [&](uint32_t g, uint32_t, uint32_t, uint32_t, uint32_t, uint64_t) {
-                                                     atomicAdd(&s_hist[g >> FINE_BITS], 1u);
-                                                 });
+        uint32_t n_t;
+        carry = tile_pieces<H_WG, P_TILE>(a, t * P_TILE, carry, s_g, s_mbw, s_wave, n_t,
+                                          [&](uint32_t o, uint64_t d) { s_desc[o] = d; });
+        __syncthreads();
+        uint64_t* out = desc + t * P_TILE;
+        for (uint32_t i = threadIdx.x; i < n_t; i += H_WG) {
+            const uint64_t d = s_desc[i];
+            out[i] = d;
+            atomicAdd(&s_hist[(uint32_t(d >> 16) & 0xFFFFu) >> FINE_BITS], 1u);
+        }
+        if (threadIdx.x == 0) tile_n[t] = n_t;
     }
     __syncthreads();
     for (int i = threadIdx.x; i < N_COARSE; i += H_WG) hist[uint64_t(i) * gridDim.x + blockIdx.x] = s_hist[i];
@@ -513,88 +533,123 @@ __global__ __launch_bounds__(N_COARSE) void k_coarse_scan(const uint64_t* __rest
     if (tid == 0) coff[0] = 0;
 }
 
-// Pass 3: the same pieces again; emit span s (H_SPLIT count spans) appends
-// its records of coarse bin c to the run starting at coff[c] +
+// Pass 3: the pieces the count pass described; emit span s (H_SPLIT count
+// spans) appends its records of coarse bin c to the run starting at coff[c] +
 // hist[c][s * H_SPLIT].  A span is a contiguous part of the input, so each
-// run is written front to back by one workgroup.  Level-1 records: the
-// header att | mbin << L1_MB_SHIFT (u32) in hdr, the label in lab as whole 16-byte
-// pairs (l1_label_words(W) words).
-__global__ __launch_bounds__(E_WG, 4) void k_pieces_emit(PartitionArgs a, const uint16_t* __restrict__ gbin,
-                                                      const uint32_t* __restrict__ mbn,
-                                                      const uint64_t* __restrict__ tile_lc, uint64_t per,
+// run is written front to back by one workgroup.  The span is taken EB_TILES
+// tiles at a time: their descriptors (at most EB_CAP per round) are staged in
+// LDS and counting-sorted by coarse bin, so consecutive lanes write
+// consecutive slots of a bin and HBM lines leave whole instead of being
+// assembled from single records.  Level-1 records: the header att | mbin <<
+// L1_MB_SHIFT (u32) in hdr, the label in lab as whole 16-byte pairs
+// (l1_label_words(W) words).
+constexpr int EB_TILES = 32;     // tiles per batch (their packed words staged)
+constexpr int EB_CAP = 10240;    // descriptors per sorting round
+constexpr int EB_WORDS = EB_TILES * (P_TILE / 32) + 16;
+__global__ __launch_bounds__(E_WG) void k_pieces_emit(PartitionArgs a, const uint64_t* __restrict__ desc,
+                                                      const uint32_t* __restrict__ tile_n, uint64_t per,
                                                       uint32_t n_hist, const uint32_t* __restrict__ hist,
                                                       const uint64_t* __restrict__ coff, uint32_t W,
                                                       uint32_t* __restrict__ hdr, uint64_t* __restrict__ lab) {
-    __shared__ __attribute__((aligned(16))) uint16_t s_g[E_TILE + WIN_MAX + 2];
-    __shared__ uint32_t s_mbw[E_TILE / 8];
-    __shared__ unsigned long long s_carry;
-    __shared__ uint32_t s_wave[E_WG / 64];
-    __shared__ uint64_t s_words[TILE_WORDS(E_TILE)];
+    static_assert(N_COARSE == 256 && E_WG >= N_COARSE && EB_TILES <= 64, "shapes below");
+    __shared__ uint64_t s_words[EB_WORDS];
+    __shared__ uint64_t s_d[EB_CAP];
+    __shared__ uint16_t s_perm[EB_CAP];
     __shared__ unsigned long long s_cur[N_COARSE];
-    __shared__ uint64_t s_desc[E_TILE];
-    static_assert(TILE_WORDS(E_TILE) <= E_WG, "one packed word per thread");
-    for (int i = threadIdx.x; i < N_COARSE; i += E_WG)
-        s_cur[i] = coff[i] + hist[uint64_t(i) * n_hist + uint64_t(blockIdx.x) * H_SPLIT];
-    const Span sp = span_of(a.n_bases, per);  // per is a multiple of E_TILE / P_TILE
-    constexpr uint64_t STEP = E_TILE / P_TILE;
-    // g values and packed words of the next tile are loaded while the
-    // current one is processed
-    GAhead<E_WG, E_TILE> ga;
-    uint64_t word_next = 0;
-    uint32_t mbw_next = 0;
-    static_assert(E_TILE / 8 <= E_WG, "one nibble word per thread");
-    const uint64_t n_mbw = (a.n_bases + 7) / 8;
-    auto load_words = [&](uint64_t t0) {
-        const uint64_t wlo = t0 ? t0 / 32 - 1 : 0;
-        const uint64_t wi = wlo + threadIdx.x;
-        word_next = threadIdx.x < TILE_WORDS(E_TILE) && wi < a.n_words ? a.packed[wi] : 0;
-        const uint64_t mi = t0 / 8 + threadIdx.x;
-        mbw_next = threadIdx.x < E_TILE / 8 && mi < n_mbw ? mbn[mi] : 0u;
-    };
-    if (sp.t_begin < sp.t_end) {
-        ga.load(a, gbin, sp.t_begin * P_TILE);
-        load_words(sp.t_begin * P_TILE);
-        if (threadIdx.x < 64) {
-            const unsigned long long c = carry_before(tile_lc, sp.t_begin);
-            if (threadIdx.x == 0) s_carry = c;
-        }
-    }
-    __syncthreads();
-    unsigned long long carry = s_carry;
-    for (uint64_t t = sp.t_begin; t < sp.t_end; t += STEP) {
-        const uint64_t t0 = t * P_TILE;
-        __syncthreads();  // the previous tile is done with s_g, s_words and s_desc
-        ga.store(s_g);
-        if (threadIdx.x < TILE_WORDS(E_TILE)) s_words[threadIdx.x] = word_next;
-        if (threadIdx.x < E_TILE / 8) s_mbw[threadIdx.x] = mbw_next;
-        if (t + STEP < sp.t_end) {
-            ga.load(a, gbin, (t + STEP) * P_TILE);
-            load_words((t + STEP) * P_TILE);
-        }
-        __syncthreads();
+    __shared__ uint32_t s_bcnt[N_COARSE], s_bbase[N_COARSE], s_bpos[N_COARSE];
+    __shared__ uint32_t s_toff[EB_TILES + 1];
+    __shared__ uint32_t s_wave[4];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    if (tid < N_COARSE) s_cur[tid] = coff[tid] + hist[uint64_t(tid) * n_hist + uint64_t(blockIdx.x) * H_SPLIT];
+    const Span sp = span_of(a.n_bases, per);
+    for (uint64_t ta = sp.t_begin; ta < sp.t_end; ta += EB_TILES) {
+        const uint32_t nt = uint32_t(min(uint64_t(EB_TILES), sp.t_end - ta));
+        const uint64_t t0 = ta * P_TILE;
         const uint64_t wlo = t0 ? t0 / 32 - 1 : 0;  // s_words[0] holds bases wlo*32 ..
         const uint64_t base0 = wlo * 32;
-        carry = tile_pieces<E_WG, E_TILE, true>(a, t0, carry, s_g, s_mbw, s_wave, s_desc,
-                                        [&](uint32_t g, uint32_t mb, uint32_t len, uint32_t l_disc,
-                                            uint32_t r_disc, uint64_t start) {
-                                            const uint64_t slot = atomicAdd(&s_cur[g >> FINE_BITS], 1ull);
-                                            hdr[slot] = len | (l_disc << 8) | (r_disc << 9) | (mb << L1_MB_SHIFT) | (g << 16);
-                                            // label words two at a time (16-byte stores)
-                                            u64x2* out = reinterpret_cast<u64x2*>(lab + slot * l1_label_words(W));
-                                            uint64_t prev = 0;
-                                            for (uint32_t q = 0; q < W; ++q) {
-                                                const uint32_t wj = W - 1 - q;  // label word q holds bases 32wj ..
-                                                uint64_t x = 0;
-                                                if (32 * wj < len) {
-                                                    x = stream_get64(s_words, start - base0 + 32ull * wj);
-                                                    const uint32_t nb = len - 32 * wj;
-                                                    if (nb < 32) x &= ~0ull << (64 - 2 * nb);
-                                                }
-                                                if (!(q & 1)) prev = x;
-                                                else out[q / 2] = u64x2{prev, x};
-                                            }
-                                            if (W & 1) out[W / 2] = u64x2{prev, 0ull};
-                                        });
+        __syncthreads();  // the previous batch is done with s_words and s_toff
+        for (uint32_t i = tid; i < nt * (P_TILE / 32) + 16; i += E_WG)
+            s_words[i] = wlo + i < a.n_words ? a.packed[wlo + i] : 0ull;
+        if (tid < 64) {  // descriptor offsets of the batch's tiles
+            const uint32_t v = lane < nt ? tile_n[ta + lane] : 0u;
+            uint32_t x = v;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t u = __shfl_up(x, d);
+                if (int(lane) >= d) x += u;
+            }
+            if (lane < nt) s_toff[lane + 1] = x;
+            if (lane == 0) s_toff[0] = 0;
+        }
+        __syncthreads();
+        const uint32_t n_b = s_toff[nt];
+        for (uint32_t x0 = 0; x0 < n_b; x0 += EB_CAP) {
+            const uint32_t m = min(uint32_t(EB_CAP), n_b - x0);
+            if (tid < N_COARSE) s_bcnt[tid] = 0;
+            __syncthreads();
+            for (uint32_t j = tid; j < m; j += E_WG) {  // stage, tagged with the tile (bits 48+)
+                const uint32_t jj = x0 + j;
+                uint32_t lo = 0, hi = nt;  // tile = last tt with s_toff[tt] <= jj
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_toff[mid] <= jj) lo = mid;
+                    else hi = mid;
+                }
+                const uint64_t d = desc[(ta + lo) * P_TILE + (jj - s_toff[lo])];
+                s_d[j] = d | (uint64_t(lo) << 48);
+                atomicAdd(&s_bcnt[(uint32_t(d >> 16) & 0xFFFFu) >> FINE_BITS], 1u);
+            }
+            __syncthreads();
+            uint32_t v = 0, x = 0;  // exclusive scan of the bin counts by the first 4 waves
+            if (tid < N_COARSE) {
+                v = x = s_bcnt[tid];
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const uint32_t u = __shfl_up(x, d);
+                    if (int(lane) >= d) x += u;
+                }
+                if (lane == 63) s_wave[tid >> 6] = x;
+            }
+            __syncthreads();
+            if (tid < N_COARSE) {
+                uint32_t pre = 0;
+                for (uint32_t j = 0; j < (tid >> 6); ++j) pre += s_wave[j];
+                s_bbase[tid] = s_bpos[tid] = pre + x - v;
+            }
+            __syncthreads();
+            for (uint32_t j = tid; j < m; j += E_WG)
+                s_perm[atomicAdd(&s_bpos[(uint32_t(s_d[j] >> 16) & 0xFFFFu) >> FINE_BITS], 1u)] = uint16_t(j);
+            __syncthreads();
+            for (uint32_t q = tid; q < m; q += E_WG) {  // consecutive lanes, consecutive slots of a bin
+                const uint64_t d = s_d[s_perm[q]];
+                const uint32_t idx = uint32_t(d & 0xFFFFu), g = uint32_t(d >> 16) & 0xFFFFu;
+                const uint32_t mb = uint32_t(d >> 32) & 15u, len = uint32_t(d >> 36) & 0xFFu;
+                const uint32_t l_disc = uint32_t(d >> 44) & 1u, r_disc = uint32_t(d >> 45) & 1u;
+                const uint32_t lj = uint32_t(d >> 46) & 1u, tile = uint32_t(d >> 48);
+                const uint32_t c = g >> FINE_BITS;
+                const uint64_t slot = s_cur[c] + (q - s_bbase[c]);
+                // first base: tile position idx - 1 (idx >= 1; lj = 0 at base 0)
+                const uint64_t start = t0 + uint64_t(tile) * P_TILE - 1 + idx - lj;
+                hdr[slot] = len | (l_disc << 8) | (r_disc << 9) | (mb << L1_MB_SHIFT) | (g << 16);
+                // label words two at a time (16-byte stores)
+                u64x2* out = reinterpret_cast<u64x2*>(lab + slot * l1_label_words(W));
+                uint64_t prev = 0;
+                for (uint32_t qq = 0; qq < W; ++qq) {
+                    const uint32_t wj = W - 1 - qq;  // label word qq holds bases 32wj ..
+                    uint64_t xw = 0;
+                    if (32 * wj < len) {
+                        xw = stream_get64(s_words, start - base0 + 32ull * wj);
+                        const uint32_t nb = len - 32 * wj;
+                        if (nb < 32) xw &= ~0ull << (64 - 2 * nb);
+                    }
+                    if (!(qq & 1)) prev = xw;
+                    else out[qq / 2] = u64x2{prev, xw};
+                }
+                if (W & 1) out[W / 2] = u64x2{prev, 0ull};
+            }
+            __syncthreads();
+            if (tid < N_COARSE) s_cur[tid] += s_bcnt[tid];
+        }
     }
 }
 
@@ -759,20 +814,21 @@ PieceSpans pieces_spans(uint64_t n_bases, uint32_t n_cu) {
     return ps;
 }
 
-void launch_pieces_hist(const PartitionArgs& a, const uint16_t* gbin, const uint64_t* tile_lc, const PieceSpans& ps,
-                        uint32_t* hist, uint64_t* ctot, uint64_t* coff, hipStream_t s) {
+void launch_pieces_hist(const PartitionArgs& a, const uint16_t* gbin, const uint32_t* mbn, const uint64_t* tile_lc,
+                        const PieceSpans& ps, uint32_t* hist, uint64_t* ctot, uint64_t* coff, uint64_t* desc,
+                        uint32_t* tile_n, hipStream_t s) {
     if (!ps.n_count) return;
-    hipLaunchKernelGGL(k_pieces_hist, dim3(ps.n_count), dim3(H_WG), 0, s, a, gbin, tile_lc, ps.per_count, hist);
+    hipLaunchKernelGGL(k_pieces_hist, dim3(ps.n_count), dim3(H_WG), 0, s, a, gbin, mbn, tile_lc, ps.per_count, hist,
+                       desc, tile_n);
     hipLaunchKernelGGL(k_hist_scan, dim3(N_COARSE), dim3(1024), 0, s, hist, uint64_t(ps.n_count), ctot);
     hipLaunchKernelGGL(k_coarse_scan, dim3(1), dim3(N_COARSE), 0, s, ctot, coff);
 }
 
-void launch_pieces_emit(const PartitionArgs& a, const uint16_t* gbin, const uint32_t* mbn, const uint64_t* tile_lc,
-                        const PieceSpans& ps,
+void launch_pieces_emit(const PartitionArgs& a, const uint64_t* desc, const uint32_t* tile_n, const PieceSpans& ps,
                         const uint32_t* hist, const uint64_t* coff, uint32_t W, uint32_t* hdr1, uint64_t* lab1,
                         hipStream_t s) {
     if (!ps.n_emit) return;
-    hipLaunchKernelGGL(k_pieces_emit, dim3(ps.n_emit), dim3(E_WG), 0, s, a, gbin, mbn, tile_lc, ps.per_emit,
+    hipLaunchKernelGGL(k_pieces_emit, dim3(ps.n_emit), dim3(E_WG), 0, s, a, desc, tile_n, ps.per_emit,
                        ps.n_count, hist, coff, W, hdr1, lab1);
 }
 
