@@ -30,6 +30,7 @@ constexpr int WIN_MAX = 128;                      // w = k - l <= 126
 constexpr int MAXF = P_TILE / 4 + 8;              // fragments overlapping a tile (len >= 4)
 constexpr int NWORDS = (P_TILE + 2 * WIN_MAX) / 32 + 4;
 constexpr int S_MAX = P_TILE + WIN_MAX;           // l-mer hashes per tile
+constexpr int SPAN_MAX = 64;                      // largest power of two <= w (w <= 126)
 constexpr int PER_T = (S_MAX + P_WG - 1) / P_WG;
 
 // tile_frag[t] = the fragment holding position max(t * P_TILE - 1, 0): the
@@ -50,7 +51,7 @@ __global__ __launch_bounds__(256) void k_tile_frag(const uint64_t* __restrict__ 
 __global__ __launch_bounds__(P_WG) void k_minimizer_g(PartitionArgs a, uint16_t* __restrict__ gbin,
                                                       uint32_t* __restrict__ mbn, uint64_t* __restrict__ tile_lc) {
     __shared__ uint64_t s_words[NWORDS];
-    __shared__ uint64_t s_m[S_MAX];
+    __shared__ uint64_t s_m[S_MAX + SPAN_MAX];  // hashes, then window minima; ~0 past the tile's l-mers
     __shared__ uint32_t s_inv[(P_TILE + 1 + 31) / 32];  // bit i: position q0 + i starts no (k-1)-mer
     __shared__ unsigned long long s_lc;
     uint32_t* s_g = reinterpret_cast<uint32_t*>(s_m);  // reuses s_m once the minima are read
@@ -90,17 +91,21 @@ __global__ __launch_bounds__(P_WG) void k_minimizer_g(PartitionArgs a, uint16_t*
     }
     __syncthreads();
 
-    // Canonical l-mer hashes (include/Minimizer_Iterator.hpp:291-296).
+    // Canonical l-mer hashes (include/Minimizer_Iterator.hpp:291-296).  The
+    // l-mer at q0 + i exists for i in [i_lo, i_hi]; it starts at stream bit
+    // pair i + soff of s_words.  Tile-relative 32-bit indices, no branches.
+    const int i_lo = q0 < 0 ? 1 : 0;
+    const int i_hi = int(min(N - int64_t(l) - q0, int64_t(S)));
+    const int soff = int(q0 - (wlo << 5));  // -1 only in tile 0, where i = 0 has no l-mer
     for (int i = tid; i < S; i += P_WG) {
-        const int64_t q = q0 + i;
-        uint64_t h = ~0ull;
-        if (q >= 0 && q + int64_t(l) <= N) {
-            const uint64_t x = stream_get64(s_words, uint64_t(q - (wlo << 5))) >> (64 - 2 * l);
-            const uint64_t h1 = lmer_hash(x, a.wy), h2 = lmer_hash(lmer_rc(x, l), a.wy);
-            h = h1 < h2 ? h1 : h2;
-        }
-        s_m[i] = h;
+        const bool ok = i >= i_lo && i <= i_hi;
+        const uint32_t pos = ok ? uint32_t(i + soff) : 0u;
+        const uint32_t wi = pos >> 5, sh = (pos & 31u) << 1;
+        const uint64_t x = ((s_words[wi] << sh) | ((s_words[wi + 1] >> 1) >> (63u - sh))) >> (64 - 2 * l);
+        const uint64_t h1 = lmer_hash(x, a.wy), h2 = lmer_hash(lmer_rc(x, l), a.wy);
+        s_m[i] = ok ? (h1 < h2 ? h1 : h2) : ~0ull;
     }
+    if (tid < SPAN_MAX) s_m[S + tid] = ~0ull;  // reads past the last l-mer see the maximum
     __syncthreads();
 
     // Window minimum over w consecutive l-mers by doubling: after the loop
@@ -112,12 +117,8 @@ __global__ __launch_bounds__(P_WG) void k_minimizer_g(PartitionArgs a, uint16_t*
         for (int c = 0; c < PER_T; ++c) {
             const int i = tid + c * P_WG;
             if (i < S) {
-                uint64_t v = s_m[i];
-                if (i + int(span) < S) {
-                    const uint64_t u = s_m[i + span];
-                    v = u < v ? u : v;
-                }
-                tmp[c] = v;
+                const uint64_t v = s_m[i], u = s_m[i + span];
+                tmp[c] = u < v ? u : v;
             }
         }
         __syncthreads();
@@ -130,19 +131,18 @@ __global__ __launch_bounds__(P_WG) void k_minimizer_g(PartitionArgs a, uint16_t*
     }
 
     constexpr int GPT = (P_TILE + 1 + P_WG - 1) / P_WG;
+    const int i_end = int(min(N - q0, int64_t(P_TILE + 1)));  // positions q0 + i < N: i < i_end
     uint32_t gvs[GPT];
 #pragma unroll
     for (int c = 0; c < GPT; ++c) {
         const int i = tid + c * P_WG;
-        const int64_t p = q0 + i;
-        uint32_t gv = G_INVALID;
-        if (i <= P_TILE && p >= 0 && p < N && !((s_inv[i >> 5] >> (i & 31)) & 1u)) {
-            const uint64_t m0 = s_m[i], m1 = s_m[i + w - span];
-            const uint64_t H = m0 < m1 ? m0 : m1;
-            gv = uint32_t(H & a.g_mask) | (uint32_t((H >> a.g_bits) & 15u) << 16);
-        }
+        const int ic = i <= P_TILE ? i : P_TILE;
+        const bool ok = i >= i_lo && i < i_end && !((s_inv[ic >> 5] >> (ic & 31)) & 1u);
+        const uint64_t m0 = s_m[ic], m1 = s_m[ic + w - span];
+        const uint64_t H = m0 < m1 ? m0 : m1;
+        const uint32_t gv = ok ? uint32_t(H & a.g_mask) | (uint32_t((H >> a.g_bits) & 15u) << 16) : G_INVALID;
         gvs[c] = gv;
-        if (i >= 1 && i <= P_TILE && p < N) gbin[p] = uint16_t(gv);  // G16_INVALID for G_INVALID
+        if (i >= 1 && i < i_end) gbin[q0 + i] = uint16_t(gv);  // G16_INVALID for G_INVALID
     }
     __syncthreads();  // every minimum read: s_g may overwrite s_m
 #pragma unroll
