@@ -363,41 +363,38 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
     uint32_t ph0 = 0;
     if (rs0 >= 0 && !(chg & 1u) && gv[1] != G16_INVALID)
         ph0 = uint32_t(uint64_t(q0 + base_idx - rs0) % w);
-    uint32_t o_desc = 0, n_tile = 0;
+    // Piece starts among my positions (bit j: position base_idx + j), then
+    // one descriptor per set bit: the loop runs as often as the wave's
+    // busiest lane has pieces, not once per position.
+    uint32_t starts = 0;
     {
-        uint32_t n_mine = 0;
         uint32_t ph = ph0;
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
             if (j) ph = ph + 1 == w ? 0u : ph + 1;
             if ((chg >> j) & 1u) ph = 0;
-            n_mine += ph == 0 && gv[j + 1] != G16_INVALID;
+            starts |= uint32_t(ph == 0 && gv[j + 1] != G16_INVALID) << j;
         }
-        o_desc = block_excl_sum<WG>(n_mine, s_wave, &n_tile);
     }
-    {
-        uint32_t ph = ph0;
-#pragma unroll
-        for (int j = 0; j < PPT; ++j) {
-            if (j) ph = ph + 1 == w ? 0u : ph + 1;
-            if ((chg >> j) & 1u) ph = 0;
-            const int idx = base_idx + j;
-            const uint32_t cur = gv[j + 1];
-            if (cur == G16_INVALID || ph) continue;
-            // Piece [idx, e): (k-1)-mers with this g, at most w of them.
-            const uint32_t g = cur;
-            const uint32_t o = uint32_t(idx) - 1u;
-            const uint32_t mb = (s_mbw[o >> 3] >> (4 * (o & 7u))) & 15u;
-            const uint32_t later = bnd >> (j + 1);  // my boundaries after position j
-            const uint32_t nb = later ? uint32_t(idx) + uint32_t(__ffs(int(later))) : next_bnd;
-            const int e = int(min(nb, uint32_t(idx) + w));
-            const uint32_t nv = s_g[e], pv = gv[j];
-            const uint32_t rj = nv != G16_INVALID, lj = pv != G16_INVALID;
-            const uint32_t l_disc = lj && pv != g;
-            const uint32_t r_disc = rj && nv != g;
-            const uint32_t len = lj + uint32_t(e - idx) + (k - 2) + rj;
-            emit(o_desc++, piece_desc(uint32_t(idx), g, mb, len, l_disc, r_disc, lj));
-        }
+    uint32_t n_tile = 0;
+    uint32_t o_desc = block_excl_sum<WG>(uint32_t(__popc(starts)), s_wave, &n_tile);
+    for (uint32_t m = starts; m;) {
+        const uint32_t j = uint32_t(__ffs(int(m))) - 1u;
+        m &= m - 1u;
+        const uint32_t idx = uint32_t(base_idx) + j;
+        // Piece [idx, e): (k-1)-mers with this g, at most w of them.
+        const uint32_t g = s_g[idx], pv = s_g[idx - 1];
+        const uint32_t o = idx - 1u;
+        const uint32_t mb = (s_mbw[o >> 3] >> (4 * (o & 7u))) & 15u;
+        const uint32_t later = bnd >> (j + 1);  // my boundaries after position j
+        const uint32_t nb = later ? idx + uint32_t(__ffs(int(later))) : next_bnd;
+        const uint32_t e = min(nb, idx + w);
+        const uint32_t nv = s_g[e];
+        const uint32_t rj = nv != G16_INVALID, lj = pv != G16_INVALID;
+        const uint32_t l_disc = lj && pv != g;
+        const uint32_t r_disc = rj && nv != g;
+        const uint32_t len = lj + (e - idx) + (k - 2) + rj;
+        emit(o_desc++, piece_desc(idx, g, mb, len, l_disc, r_disc, lj));
     }
     n_out = n_tile;
     return carry_out;
@@ -430,7 +427,7 @@ __device__ __forceinline__ Span span_of(uint64_t n_bases, uint64_t per) {
 // Pass 2: the pieces of every tile: descriptors to desc[t * P_TILE + o]
 // (o < tile_n[t] <= P_TILE), and records per coarse bin (g >> FINE_BITS) of
 // each count span into hist[c * gridDim.x + span].  No global atomics.
-__global__ __launch_bounds__(H_WG) void k_pieces_hist(PartitionArgs a, const uint16_t* __restrict__ gbin,
+__global__ __launch_bounds__(H_WG, 5) void k_pieces_hist(PartitionArgs a, const uint16_t* __restrict__ gbin,
                                                       const uint32_t* __restrict__ mbn,
                                                       const uint64_t* __restrict__ tile_lc, uint64_t per,
                                                       uint32_t* __restrict__ hist, uint64_t* __restrict__ desc,
