@@ -174,6 +174,33 @@ __device__ __forceinline__ void spill_push(bool push, const Kmer<KW>& key, uint3
     L.meta[idx] = meta;
 }
 
+// Append every active lane's instance to list 1 (sel) or list 0 (!sel): one
+// LDS atomic per list per wave.  A full list sets *ovf.
+template <int KW>
+__device__ __forceinline__ void spill_push_split(bool sel, const Kmer<KW>& key, uint32_t meta, const SpillList& L0,
+                                                 const SpillList& L1, uint32_t cap, uint32_t* s_n,
+                                                 uint32_t* s_ovf) {
+    const unsigned long long m1 = __ballot(sel), m0 = __ballot(!sel);
+    const int lane = int(__lane_id());
+    const int l0 = m0 ? __ffsll((long long)m0) - 1 : -1, l1 = m1 ? __ffsll((long long)m1) - 1 : -1;
+    uint32_t b = 0;
+    if (lane == l0) b = atomicAdd(&s_n[0], uint32_t(__popcll(m0)));
+    if (lane == l1) b = atomicAdd(&s_n[1], uint32_t(__popcll(m1)));
+    const uint32_t b0 = __shfl(b, l0 < 0 ? lane : l0), b1 = __shfl(b, l1 < 0 ? lane : l1);
+    const unsigned long long m = sel ? m1 : m0;
+    const uint32_t idx = (sel ? b1 : b0) +
+                         __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+    if (idx >= cap) {
+        *s_ovf = 1;
+        return;
+    }
+    uint64_t* kd = sel ? L1.key : L0.key;
+    uint32_t* md = sel ? L1.meta : L0.meta;
+#pragma unroll
+    for (int w = 0; w < KW; ++w) kd[uint64_t(w) * cap + idx] = key.w[w];
+    md[idx] = meta;
+}
+
 template <int KW, int WMAX>
 __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
     constexpr int C = lds_capacity(KW);
@@ -201,8 +228,9 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
         const uint32_t b = a.item_bucket[item];
         const uint32_t P = a.item_count[item];  // passes of this bucket
         // item_first: the pass of this item, or LDS_ALL_PASSES: every pass
-        // here, the first walking the records and leaving the k-mers of later
-        // passes in spill lists that those passes read instead.
+        // here.  The records are then walked once, every instance going to
+        // spill list 0 (pass 0) or 1 (later passes), and each pass probes its
+        // list with every lane busy, passing on what belongs to later passes.
         const bool all = a.item_first[item] == LDS_ALL_PASSES;
         const uint32_t p_lo = all ? 0u : a.item_first[item], p_hi = all ? P - 1 : p_lo;
         if (tid == 0) s_ovf = __hip_atomic_load(&a.overflow[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -213,7 +241,8 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
             for (int i = tid; i < C / 4; i += LWG) s_st[i] = 0u;
             if (tid == 0) {
                 s_cnt = 0;
-                s_sn[(p + 1) & 1u] = 0;  // the list this pass writes
+                if (p == p_lo) s_sn[0] = s_sn[1] = 0;  // the walk fills both lists
+                else s_sn[(p + 1) & 1u] = 0;          // the list this pass writes
             }
             __syncthreads();
             if (s_ovf) break;  // bucket already flagged
@@ -257,15 +286,15 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                     // pass = top of h * P; the group comes from the fraction below it
                     const uint64_t hp = uint64_t(pass_hash<KW>(in.key)) * P;
                     const uint32_t q = uint32_t(hp >> 32);
-                    if (q == p) {
+                    if (all) {
+                        spill_push_split<KW>(q != 0, in.key,
+                                             in.front | (in.back << 3) | (in.disc << 6) | (min(m, 15u) << 9),
+                                             spill_list<KW>(a, 0), spill_list<KW>(a, 1), a.spill_cap, s_sn, &s_ovf);
+                    } else if (q == p) {
                         const uint32_t g0 = uint32_t((uint64_t(uint32_t(hp)) * NG) >> 32);
                         if (!lds_add<KW, C>(s_key, s_ef, s_st, in.key, g0, in.front, in.back, in.disc, m, my_ins))
                             s_ovf = 1;
                     }
-                    if (spill)
-                        spill_push<KW>(q > p, in.key,
-                                       in.front | (in.back << 3) | (in.disc << 6) | (min(m, 15u) << 9), out,
-                                       a.spill_cap, &s_sn[(p + 1) & 1u], &s_ovf);
                     if (rw.i == rw.n) {  // record done: account inserts, start the prefetched record
                         flush();
                         r = rn;
@@ -280,9 +309,11 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                         }
                     }
                 }
-            } else {
-                // A later pass: the instances the previous pass left, one per
-                // lane, in list order.
+            }
+            if (p != p_lo || all) {
+                if (p == p_lo) __syncthreads();  // the walk's lists are complete
+                // A pass over the list the walk or the previous pass left, one
+                // entry per lane, in list order.
                 const SpillList in = spill_list<KW>(a, p & 1u);
                 const uint32_t n_in = s_sn[p & 1u];
                 uint32_t seen = 0;
