@@ -105,6 +105,8 @@ struct cfgpu_handle {
     DevBuf spill_key, spill_meta;  // LDS construct: per-workgroup spill lists of all-pass items
     uint64_t spill_need = 0;       // largest k-mer instance count of an all-pass item of the round
     uint32_t n_all_pass = 0;       // all-pass items of the round
+    uint8_t* stage = nullptr;      // pinned staging of small uploads (h2d); reused after sync_stream
+    size_t stage_cap = 0, stage_off = 0;
     std::vector<uint64_t> h_tbl_ptr;
     std::vector<uint64_t> h_cap;
     std::vector<uint32_t> h_limit, h_nvert, h_flags;
@@ -149,6 +151,36 @@ struct cfgpu_handle {
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
+
+namespace {
+
+// Host-to-device copy of a small host array through a pinned staging buffer:
+// a copy from pageable memory blocks the host for tens of microseconds while
+// the GPU idles between launches.  Staged bytes are reused only after
+// sync_stream (every copy has completed); an upload that does not fit is
+// copied directly.  Not for stream capture (a captured copy would read the
+// staging later).
+constexpr size_t STAGE_BYTES = 8u << 20;
+hipError_t h2d(cfgpu_handle* h, void* dst, const void* src, size_t n) {
+    if (!h->stage && hipHostMalloc(reinterpret_cast<void**>(&h->stage), STAGE_BYTES, hipHostMallocDefault) == hipSuccess)
+        h->stage_cap = STAGE_BYTES;
+    if (h->stage && n <= h->stage_cap - h->stage_off) {
+        uint8_t* p = h->stage + h->stage_off;
+        std::memcpy(p, src, n);
+        h->stage_off += (n + 255) & ~size_t(255);
+        if (h->stage_off > h->stage_cap) h->stage_off = h->stage_cap;
+        return hipMemcpyAsync(dst, p, n, hipMemcpyHostToDevice, h->stream);
+    }
+    return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, h->stream);
+}
+
+hipError_t sync_stream(cfgpu_handle* h) {
+    const hipError_t e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) h->stage_off = 0;
+    return e;
+}
+
+}  // namespace
 
 namespace {
 
@@ -289,18 +321,17 @@ int prepare_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
     HIPCK(h, h->d_items_c.ensure(ib.size() * 4));
     HIPCK(h, h->d_round_list.ensure(buckets.size() * 4));
     const size_t G = h->G;
-    HIPCK(h, hipMemcpyAsync(h->d_tbl_ptr.p, h->h_tbl_ptr.data(), G * 8, hipMemcpyHostToDevice, h->stream));
-    HIPCK(h, hipMemcpyAsync(h->d_cap.p, h->h_cap.data(), G * 8, hipMemcpyHostToDevice, h->stream));
-    HIPCK(h, hipMemcpyAsync(h->d_limit.p, h->h_limit.data(), G * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCK(h, h2d(h, h->d_tbl_ptr.p, h->h_tbl_ptr.data(), G * 8));
+    HIPCK(h, h2d(h, h->d_cap.p, h->h_cap.data(), G * 8));
+    HIPCK(h, h2d(h, h->d_limit.p, h->h_limit.data(), G * 4));
     if (n_items) {
-        HIPCK(h, hipMemcpyAsync(h->d_items_b.p, ib.data(), ib.size() * 4, hipMemcpyHostToDevice, h->stream));
-        HIPCK(h, hipMemcpyAsync(h->d_items_f.p, ifirst.data(), ib.size() * 4, hipMemcpyHostToDevice, h->stream));
-        HIPCK(h, hipMemcpyAsync(h->d_items_c.p, icnt.data(), ib.size() * 4, hipMemcpyHostToDevice, h->stream));
+        HIPCK(h, h2d(h, h->d_items_b.p, ib.data(), ib.size() * 4));
+        HIPCK(h, h2d(h, h->d_items_f.p, ifirst.data(), ib.size() * 4));
+        HIPCK(h, h2d(h, h->d_items_c.p, icnt.data(), ib.size() * 4));
     }
     if (!buckets.empty())
-        HIPCK(h, hipMemcpyAsync(h->d_round_list.p, buckets.data(), buckets.size() * 4,
-                                hipMemcpyHostToDevice, h->stream));
-    HIPCK(h, hipStreamSynchronize(h->stream));  // host vectors above are reused
+        HIPCK(h, h2d(h, h->d_round_list.p, buckets.data(), buckets.size() * 4));
+    HIPCK(h, sync_stream(h));  // host vectors above are reused
     n_slots_out = n_slots;
     pool_out = pool;
     return CFGPU_OK;
@@ -320,7 +351,7 @@ int finish_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
     std::vector<uint32_t> ovf(G);
     HIPCK(h, hipMemcpyAsync(ovf.data(), h->d_overflow.p, G * 4, hipMemcpyDeviceToHost, h->stream));
     HIPCK(h, hipMemcpyAsync(h->h_nvert.data(), h->d_nvert.p, G * 4, hipMemcpyDeviceToHost, h->stream));
-    HIPCK(h, hipStreamSynchronize(h->stream));
+    HIPCK(h, sync_stream(h));
     overflowed.clear();
     for (uint32_t b : buckets) {
         if (ovf[b]) {
@@ -487,8 +518,32 @@ std::vector<uint32_t> nonempty_buckets(const cfgpu_handle* h) {
     return v;
 }
 
+// Largest cost first, ties in the given order: a stable LSD radix sort (3
+// passes of 11 bits) on the inverted bits of the cost as a float (costs are
+// >= 0, so the bit patterns order like the values).  A comparison sort of the
+// 16384 buckets costs about a millisecond of host time between launches.
 void order_largest_first(std::vector<uint32_t>& v, const std::vector<double>& cost) {
-    std::stable_sort(v.begin(), v.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+    const size_t n = v.size();
+    std::vector<uint32_t> key(n), key2(n), v2(n);
+    for (size_t i = 0; i < n; ++i) {
+        const float f = float(std::max(0.0, cost[v[i]]));
+        uint32_t bits;
+        std::memcpy(&bits, &f, 4);
+        key[i] = ~bits;
+    }
+    std::vector<uint32_t> cnt(2049);
+    for (int shift = 0; shift < 33; shift += 11) {
+        std::fill(cnt.begin(), cnt.end(), 0u);
+        for (size_t i = 0; i < n; ++i) ++cnt[((key[i] >> shift) & 2047u) + 1];
+        for (size_t d = 1; d < cnt.size(); ++d) cnt[d] += cnt[d - 1];
+        for (size_t i = 0; i < n; ++i) {
+            const uint32_t pos = cnt[(key[i] >> shift) & 2047u]++;
+            key2[pos] = key[i];
+            v2[pos] = v[i];
+        }
+        key.swap(key2);
+        v.swap(v2);
+    }
 }
 
 // Record dedup before the batched constructs.  A probe set of evenly spread
@@ -513,9 +568,9 @@ int dedup_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets, double di
     HIPCK(h, h->dd_ib.ensure(ib.size() * 4));
     HIPCK(h, h->dd_ip.ensure(ib.size() * 4));
     HIPCK(h, h->dd_iP.ensure(ib.size() * 4));
-    HIPCK(h, hipMemcpyAsync(h->dd_ib.p, ib.data(), ib.size() * 4, hipMemcpyHostToDevice, h->stream));
-    HIPCK(h, hipMemcpyAsync(h->dd_ip.p, ip.data(), ip.size() * 4, hipMemcpyHostToDevice, h->stream));
-    HIPCK(h, hipMemcpyAsync(h->dd_iP.p, iP.data(), iP.size() * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCK(h, h2d(h, h->dd_ib.p, ib.data(), ib.size() * 4));
+    HIPCK(h, h2d(h, h->dd_ip.p, ip.data(), ip.size() * 4));
+    HIPCK(h, h2d(h, h->dd_iP.p, iP.data(), iP.size() * 4));
     launch_fill_u32(h->d_qhead.as<uint32_t>(), 0u, 1, h->stream);
     DedupArgs a{};
     a.att = h->att.as<uint32_t>();
@@ -573,26 +628,26 @@ int dedup_buckets(cfgpu_handle* h, const std::vector<uint32_t>& all) {
     std::vector<uint32_t> cnt(G), ovf(G);
     HIPCK(h, hipMemcpyAsync(cnt.data(), h->dd_cnt.p, G * 4, hipMemcpyDeviceToHost, h->stream));
     HIPCK(h, hipMemcpyAsync(ovf.data(), h->dd_ovf.p, G * 4, hipMemcpyDeviceToHost, h->stream));
-    HIPCK(h, hipStreamSynchronize(h->stream));
+    HIPCK(h, sync_stream(h));
     double d = 0, t = 0;
     for (uint32_t b : probe)
         if (!ovf[b]) d += cnt[b], t += double(bucket_records(h, b));
     const double frac = t > 0 ? d / t : 1.0;
     if (h->tune.dedup == 1 && frac * DEDUP_MIN_GAIN > 1.0) {
         HIPCK(h, hipEventRecord(e1.e, h->stream));
-        HIPCK(h, hipStreamSynchronize(h->stream));
+        HIPCK(h, sync_stream(h));
         rs.t_dedup_ms = elapsed_ms(e0, e1);
         return CFGPU_OK;  // records barely repeat: walk them as they are
     }
     // margin < 1 under-sizes on purpose here too (exercises the copy fallback)
     CK(dedup_round(h, rest, std::min(1.0, frac * 1.1) * std::min(1.0, double(h->tune.margin))));
     HIPCK(h, hipMemcpyAsync(ovf.data(), h->dd_ovf.p, G * 4, hipMemcpyDeviceToHost, h->stream));
-    HIPCK(h, hipStreamSynchronize(h->stream));
+    HIPCK(h, sync_stream(h));
     std::vector<uint32_t> list;
     for (uint32_t b : all)
         if (ovf[b]) list.push_back(b);
     if (!list.empty()) {
-        HIPCK(h, hipMemcpyAsync(h->dd_list.p, list.data(), list.size() * 4, hipMemcpyHostToDevice, h->stream));
+        HIPCK(h, h2d(h, h->dd_list.p, list.data(), list.size() * 4));
         DedupArgs a{};
         a.att = h->att.as<uint32_t>();
         a.labels = h->labels.as<uint64_t>();
@@ -612,7 +667,7 @@ int dedup_buckets(cfgpu_handle* h, const std::vector<uint32_t>& all) {
     HIPCK(h, hipMemcpyAsync(h->h_ddcnt.data(), h->dd_cnt.p, G * 4, hipMemcpyDeviceToHost, h->stream));
     HIPCK(h, hipMemcpyAsync(h->h_ddkm.data(), h->dd_km.p, G * 8, hipMemcpyDeviceToHost, h->stream));
     HIPCK(h, hipEventRecord(e1.e, h->stream));
-    HIPCK(h, hipStreamSynchronize(h->stream));
+    HIPCK(h, sync_stream(h));
     HIPCK(h, hipGetLastError());
     uint64_t walked = 0;
     for (uint32_t b = 0; b < G; ++b) walked += h->h_ddcnt[b];
@@ -760,8 +815,8 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
                 HIPCK(h, h->mlp_target.ensure(idx.size() * 4));
                 HIPCK(h, h->mlp_pred.ensure(G * 4));
                 HIPCK(h, h->mlp_loss.ensure(4));
-                HIPCK(h, hipMemcpyAsync(h->mlp_idx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, h->stream));
-                HIPCK(h, hipMemcpyAsync(h->mlp_target.p, tgt.data(), tgt.size() * 4, hipMemcpyHostToDevice, h->stream));
+                HIPCK(h, h2d(h, h->mlp_idx.p, idx.data(), idx.size() * 4));
+                HIPCK(h, h2d(h, h->mlp_target.p, tgt.data(), tgt.size() * 4));
                 launch_mlp_train(h->mlp_params.as<float>(), h->feat.as<float>(), h->mlp_idx.as<uint32_t>(),
                                  h->mlp_target.as<float>(), uint32_t(idx.size()), 120, 3e-3f, 12345u,
                                  h->mlp_adam.as<float>(), h->mlp_loss.as<float>(), h->stream);
@@ -772,7 +827,7 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
                 HIPCK(h, hipMemcpyAsync(pred.data(), h->mlp_pred.p, G * 4, hipMemcpyDeviceToHost, h->stream));
                 HIPCK(h, hipMemcpyAsync(&loss, h->mlp_loss.p, 4, hipMemcpyDeviceToHost, h->stream));
                 HIPCK(h, hipEventRecord(e5.e, h->stream));
-                HIPCK(h, hipStreamSynchronize(h->stream));
+                HIPCK(h, sync_stream(h));
                 rs.mlp_loss = loss;
                 rs.t_train_ms = elapsed_ms(e1, e4);
                 rs.t_infer_ms = elapsed_ms(e4, e5);
@@ -785,11 +840,12 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
                 use_mlp = force == 1 || hold.empty() || err_mlp <= err_const;
             }
             rs.predictor = use_mlp ? 1u : 2u;
+            const double ratio_const = std::exp(std::min(0.0, mean_t));
             for (uint32_t b : rest) {
                 const double km = double(bucket_kmers(h, b));
-                const double lr = use_mlp ? (std::isfinite(pred[b]) ? std::min(0.0, double(pred[b])) : 0.0)
-                                          : std::min(0.0, mean_t);
-                h->h_pred[b] = float(std::max(1.0, km * std::exp(lr)));
+                const double ratio = use_mlp ? std::exp(std::isfinite(pred[b]) ? std::min(0.0, double(pred[b])) : 0.0)
+                                             : ratio_const;
+                h->h_pred[b] = float(std::max(1.0, km * ratio));
             }
             have_model = true;
         }
@@ -822,7 +878,7 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
     Event ef;
     HIPCK(h, hipEventRecord(ef.e, h->stream));
     HIPCK(h, hipMemcpyAsync(h->h_nvert.data(), h->d_nvert.p, G * 4, hipMemcpyDeviceToHost, h->stream));
-    HIPCK(h, hipStreamSynchronize(h->stream));
+    HIPCK(h, sync_stream(h));
 
     if (mode == CFGPU_MODE_BATCHED_LEARNED) {
         rs.t_sample_ms = elapsed_ms(e0, e1);
@@ -890,6 +946,7 @@ int cfgpu_close(cfgpu_handle* h) {
     if (!h) return CFGPU_E_ARG;
     (void)hipSetDevice(h->prm.device);
     (void)hipStreamSynchronize(h->stream);
+    if (h->stage) (void)hipHostFree(h->stage);
     delete h;
     return CFGPU_OK;
 }
@@ -975,7 +1032,7 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     HIPCK(h, hipEventRecord(e2.e, h->stream));
     uint64_t n_rec = 0;
     HIPCK(h, hipMemcpyAsync(&n_rec, h->coff.as<uint64_t>() + N_COARSE, 8, hipMemcpyDeviceToHost, h->stream));
-    HIPCK(h, hipStreamSynchronize(h->stream));
+    HIPCK(h, sync_stream(h));
     const uint64_t lab_off = l1_label_offset(n_rec);
     HIPCK(h, h->rec1.ensure((lab_off + n_rec * l1_label_words(h->W)) * 8));
     HIPCK(h, h->att.ensure(n_rec * 4));
@@ -998,7 +1055,7 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     h->h_stats.resize(size_t(G) * 4);
     HIPCK(h, hipMemcpyAsync(h->h_boff.data(), h->boff.p, (G + 1) * 8, hipMemcpyDeviceToHost, h->stream));
     HIPCK(h, hipMemcpyAsync(h->h_stats.data(), h->stats.p, size_t(G) * 32, hipMemcpyDeviceToHost, h->stream));
-    HIPCK(h, hipStreamSynchronize(h->stream));
+    HIPCK(h, sync_stream(h));
 
     h->n_bases = n_bases;
     h->n_frag = n_frag;
@@ -1053,7 +1110,7 @@ int cfgpu_records_copy(cfgpu_handle* h, uint32_t* att, uint64_t* labels, uint64_
     if (labels && h->n_records)
         HIPCK(h, hipMemcpyAsync(labels, h->labels.p, h->n_records * h->W * 8, hipMemcpyDeviceToHost, h->stream));
     if (bucket_off) std::memcpy(bucket_off, h->h_boff.data(), (h->G + 1) * 8);
-    HIPCK(h, hipStreamSynchronize(h->stream));
+    HIPCK(h, sync_stream(h));
     return CFGPU_OK;
 }
 
@@ -1094,7 +1151,7 @@ int cfgpu_load_buckets(cfgpu_handle* h, const uint32_t* att, const uint64_t* lab
     h->h_boff.assign(bucket_off, bucket_off + G + 1);
     h->h_stats.resize(size_t(G) * 4);
     HIPCK(h, hipMemcpyAsync(h->h_stats.data(), h->stats.p, size_t(G) * 32, hipMemcpyDeviceToHost, h->stream));
-    HIPCK(h, hipStreamSynchronize(h->stream));
+    HIPCK(h, sync_stream(h));
     h->n_bases = 0;
     h->n_frag = 0;
     h->n_records = n;
@@ -1142,7 +1199,7 @@ int cfgpu_contract(cfgpu_handle* h) {
     HIPCK(h, h->ct_counts.ensure(size_t(h->G) * sizeof(cfgpu_contract_stat)));
     HIPCK(h, h->ct_ctl.ensure(64));
     HIPCK(h, h->ct_scratch.ensure(per_wg * std::max<uint32_t>(grid, 1)));
-    if (n_list) HIPCK(h, hipMemcpyAsync(h->ct_list.p, list.data(), n_list * 4ull, hipMemcpyHostToDevice, h->stream));
+    if (n_list) HIPCK(h, h2d(h, h->ct_list.p, list.data(), n_list * 4ull));
     // output sized for short unitigs; re-run once at the exact size when short
     uint64_t rec_cap = std::max<uint64_t>(4096, sum_nv / 2), lab_cap = std::max<uint64_t>(1 << 20, sum_nv * 8);
     Event e0, e1;
@@ -1180,7 +1237,7 @@ int cfgpu_contract(cfgpu_handle* h) {
         launch_contract(a, KW, grid, grid, h->stream);
         HIPCK(h, hipGetLastError());
         HIPCK(h, hipMemcpyAsync(ctl, h->ct_ctl.p, sizeof ctl, hipMemcpyDeviceToHost, h->stream));
-        HIPCK(h, hipStreamSynchronize(h->stream));
+        HIPCK(h, sync_stream(h));
         if (ctl[4]) return fail(h, CFGPU_E_STATE, "contract: inconsistent subgraph table");
         uint64_t cur[2];
         std::memcpy(cur, ctl + 8, sizeof cur);
@@ -1267,7 +1324,7 @@ int cfgpu_cdbg(cfgpu_handle* h) {
             launch_cdbg_phase(a, KW, grid, 2, 0, c + 5, h->stream);
             uint32_t changed = 0;
             HIPCK(h, hipMemcpyAsync(&changed, c + 5, 4, hipMemcpyDeviceToHost, h->stream));
-            HIPCK(h, hipStreamSynchronize(h->stream));
+            HIPCK(h, sync_stream(h));
             ++rounds;
             if (!changed) break;
         }
@@ -1275,7 +1332,7 @@ int cfgpu_cdbg(cfgpu_handle* h) {
         launch_cdbg_phase(a, KW, grid, 4, 0, nullptr, h->stream);
         HIPCK(h, hipGetLastError());
         HIPCK(h, hipMemcpyAsync(ctl, c, sizeof ctl, hipMemcpyDeviceToHost, h->stream));
-        HIPCK(h, hipStreamSynchronize(h->stream));
+        HIPCK(h, sync_stream(h));
         if (ctl[3]) return fail(h, CFGPU_E_STATE, "cdbg: inconsistent unitig ends (%u)", ctl[3]);
         if (ctl[2]) return fail(h, CFGPU_E_STATE, "cdbg: output longer than the unitig labels");
         if (ctl[4]) {  // cycles through several subgraphs: their pieces to the host
@@ -1284,7 +1341,7 @@ int cfgpu_cdbg(cfgpu_handle* h) {
             launch_cdbg_cycle_list(a, grid, h->cd_cyc.as<uint32_t>(), c + 6, h->stream);
             std::vector<uint32_t> list(ctl[4]);
             HIPCK(h, hipMemcpyAsync(list.data(), h->cd_cyc.p, list.size() * 4, hipMemcpyDeviceToHost, h->stream));
-            HIPCK(h, hipStreamSynchronize(h->stream));
+            HIPCK(h, sync_stream(h));
             std::vector<cfgpu_unitig> recs(list.size());
             std::string labs;
             for (size_t i = 0; i < list.size(); ++i) {
@@ -1340,7 +1397,7 @@ int cfgpu_cdbg_copy(cfgpu_handle* h, char* seq, uint64_t seq_cap, uint64_t* rec,
     if (seq && h->cd_bytes)
         HIPCK(h, hipMemcpyAsync(seq, h->cd_out.p, h->cd_bytes, hipMemcpyDeviceToHost, h->stream));
     if (rec && h->cd_n) HIPCK(h, hipMemcpyAsync(rec, h->cd_rec.p, h->cd_n * 16, hipMemcpyDeviceToHost, h->stream));
-    HIPCK(h, hipStreamSynchronize(h->stream));
+    HIPCK(h, sync_stream(h));
     if (seq && !h->cd_host_seq.empty()) std::memcpy(seq + h->cd_bytes, h->cd_host_seq.data(), h->cd_host_seq.size());
     if (rec)
         for (size_t i = 0; i < h->cd_host_rec.size() / 2; ++i) {
@@ -1357,7 +1414,7 @@ int cfgpu_contract_stats(cfgpu_handle* h, cfgpu_contract_stat* out, uint32_t n) 
     if (n) {
         HIPCK(h, hipMemcpyAsync(out, h->ct_counts.p, size_t(n) * sizeof(cfgpu_contract_stat), hipMemcpyDeviceToHost,
                                 h->stream));
-        HIPCK(h, hipStreamSynchronize(h->stream));
+        HIPCK(h, sync_stream(h));
     }
     return CFGPU_OK;
 }
@@ -1388,7 +1445,7 @@ int cfgpu_unitigs_copy(cfgpu_handle* h, cfgpu_unitig* recs, uint64_t rec_cap, ch
                                 h->stream));
     if (labels && h->n_ubytes)
         HIPCK(h, hipMemcpyAsync(labels, h->ct_labels.p, h->n_ubytes, hipMemcpyDeviceToHost, h->stream));
-    HIPCK(h, hipStreamSynchronize(h->stream));
+    HIPCK(h, sync_stream(h));
     return CFGPU_OK;
 }
 
@@ -1460,7 +1517,7 @@ int cfgpu_table_copy(cfgpu_handle* h, const uint32_t* buckets, uint32_t n_bucket
         if (status) HIPCK(h, hipMemcpyAsync(status, h->ex_st.p, total, hipMemcpyDeviceToHost, h->stream));
         if (dfa) HIPCK(h, hipMemcpyAsync(dfa, h->ex_dfa.p, total, hipMemcpyDeviceToHost, h->stream));
     }
-    HIPCK(h, hipStreamSynchronize(h->stream));
+    HIPCK(h, sync_stream(h));
     size_t j = 0;
     for (uint32_t i = 0; i < n_buckets; ++i) {
         uint32_t c = 0;
