@@ -105,8 +105,14 @@ struct cfgpu_handle {
     DevBuf spill_key, spill_meta;  // LDS construct: per-workgroup spill lists of all-pass items
     uint64_t spill_need = 0;       // largest k-mer instance count of an all-pass item of the round
     uint32_t n_all_pass = 0;       // all-pass items of the round
-    uint8_t* stage = nullptr;      // pinned staging of small uploads (h2d); reused after sync_stream
+    uint8_t* stage = nullptr;      // pinned staging of small copies (h2d, d2h); reused after sync_stream
     size_t stage_cap = 0, stage_off = 0;
+    struct PendingCopy {
+        void* dst;
+        const void* src;
+        size_t n;
+    };
+    std::vector<PendingCopy> stage_pending;  // d2h copies finished on the host by sync_stream
     std::vector<uint64_t> h_tbl_ptr;
     std::vector<uint64_t> h_cap;
     std::vector<uint32_t> h_limit, h_nvert, h_flags;
@@ -174,9 +180,29 @@ hipError_t h2d(cfgpu_handle* h, void* dst, const void* src, size_t n) {
     return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, h->stream);
 }
 
+// Device-to-host copy into a host array through the staging buffer; the
+// array holds the data once sync_stream returns (every caller synchronises
+// with it before reading).
+hipError_t d2h(cfgpu_handle* h, void* dst, const void* src, size_t n) {
+    if (!h->stage && hipHostMalloc(reinterpret_cast<void**>(&h->stage), STAGE_BYTES, hipHostMallocDefault) == hipSuccess)
+        h->stage_cap = STAGE_BYTES;
+    if (h->stage && n <= h->stage_cap - h->stage_off) {
+        uint8_t* p = h->stage + h->stage_off;
+        h->stage_off += (n + 255) & ~size_t(255);
+        if (h->stage_off > h->stage_cap) h->stage_off = h->stage_cap;
+        h->stage_pending.push_back({dst, p, n});
+        return hipMemcpyAsync(p, src, n, hipMemcpyDeviceToHost, h->stream);
+    }
+    return hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, h->stream);
+}
+
 hipError_t sync_stream(cfgpu_handle* h) {
     const hipError_t e = hipStreamSynchronize(h->stream);
-    if (e == hipSuccess) h->stage_off = 0;
+    if (e == hipSuccess) {
+        for (const auto& c : h->stage_pending) std::memcpy(c.dst, c.src, c.n);
+        h->stage_pending.clear();
+        h->stage_off = 0;
+    }
     return e;
 }
 
@@ -186,6 +212,7 @@ namespace {
 
 int fail(cfgpu_handle* h, int code, const char* fmt, ...) {
     if (h) {
+        h->stage_pending.clear();  // staged copies whose host arrays the failing call leaves
         char buf[512];
         va_list ap;
         va_start(ap, fmt);
@@ -349,8 +376,8 @@ int finish_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
                  std::vector<uint32_t>& overflowed) {
     const size_t G = h->G;
     std::vector<uint32_t> ovf(G);
-    HIPCK(h, hipMemcpyAsync(ovf.data(), h->d_overflow.p, G * 4, hipMemcpyDeviceToHost, h->stream));
-    HIPCK(h, hipMemcpyAsync(h->h_nvert.data(), h->d_nvert.p, G * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, d2h(h, ovf.data(), h->d_overflow.p, G * 4));
+    HIPCK(h, d2h(h, h->h_nvert.data(), h->d_nvert.p, G * 4));
     HIPCK(h, sync_stream(h));
     overflowed.clear();
     for (uint32_t b : buckets) {
@@ -626,8 +653,8 @@ int dedup_buckets(cfgpu_handle* h, const std::vector<uint32_t>& all) {
     }
     CK(dedup_round(h, probe, 1.0));
     std::vector<uint32_t> cnt(G), ovf(G);
-    HIPCK(h, hipMemcpyAsync(cnt.data(), h->dd_cnt.p, G * 4, hipMemcpyDeviceToHost, h->stream));
-    HIPCK(h, hipMemcpyAsync(ovf.data(), h->dd_ovf.p, G * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, d2h(h, cnt.data(), h->dd_cnt.p, G * 4));
+    HIPCK(h, d2h(h, ovf.data(), h->dd_ovf.p, G * 4));
     HIPCK(h, sync_stream(h));
     double d = 0, t = 0;
     for (uint32_t b : probe)
@@ -641,7 +668,7 @@ int dedup_buckets(cfgpu_handle* h, const std::vector<uint32_t>& all) {
     }
     // margin < 1 under-sizes on purpose here too (exercises the copy fallback)
     CK(dedup_round(h, rest, std::min(1.0, frac * 1.1) * std::min(1.0, double(h->tune.margin))));
-    HIPCK(h, hipMemcpyAsync(ovf.data(), h->dd_ovf.p, G * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, d2h(h, ovf.data(), h->dd_ovf.p, G * 4));
     HIPCK(h, sync_stream(h));
     std::vector<uint32_t> list;
     for (uint32_t b : all)
@@ -664,8 +691,8 @@ int dedup_buckets(cfgpu_handle* h, const std::vector<uint32_t>& all) {
                      h->stream);
     h->h_ddcnt.resize(G);
     h->h_ddkm.resize(G);
-    HIPCK(h, hipMemcpyAsync(h->h_ddcnt.data(), h->dd_cnt.p, G * 4, hipMemcpyDeviceToHost, h->stream));
-    HIPCK(h, hipMemcpyAsync(h->h_ddkm.data(), h->dd_km.p, G * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, d2h(h, h->h_ddcnt.data(), h->dd_cnt.p, G * 4));
+    HIPCK(h, d2h(h, h->h_ddkm.data(), h->dd_km.p, G * 8));
     HIPCK(h, hipEventRecord(e1.e, h->stream));
     HIPCK(h, sync_stream(h));
     HIPCK(h, hipGetLastError());
@@ -877,7 +904,7 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
     }
     Event ef;
     HIPCK(h, hipEventRecord(ef.e, h->stream));
-    HIPCK(h, hipMemcpyAsync(h->h_nvert.data(), h->d_nvert.p, G * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, d2h(h, h->h_nvert.data(), h->d_nvert.p, G * 4));
     HIPCK(h, sync_stream(h));
 
     if (mode == CFGPU_MODE_BATCHED_LEARNED) {
@@ -1031,7 +1058,7 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
                            h->pdesc.as<uint64_t>(), h->tile_n.as<uint32_t>(), h->stream);
     HIPCK(h, hipEventRecord(e2.e, h->stream));
     uint64_t n_rec = 0;
-    HIPCK(h, hipMemcpyAsync(&n_rec, h->coff.as<uint64_t>() + N_COARSE, 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, d2h(h, &n_rec, h->coff.as<uint64_t>() + N_COARSE, 8));
     HIPCK(h, sync_stream(h));
     const uint64_t lab_off = l1_label_offset(n_rec);
     HIPCK(h, h->rec1.ensure((lab_off + n_rec * l1_label_words(h->W)) * 8));
@@ -1053,8 +1080,8 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     HIPCK(h, hipGetLastError());
     h->h_boff.resize(G + 1);
     h->h_stats.resize(size_t(G) * 4);
-    HIPCK(h, hipMemcpyAsync(h->h_boff.data(), h->boff.p, (G + 1) * 8, hipMemcpyDeviceToHost, h->stream));
-    HIPCK(h, hipMemcpyAsync(h->h_stats.data(), h->stats.p, size_t(G) * 32, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, d2h(h, h->h_boff.data(), h->boff.p, (G + 1) * 8));
+    HIPCK(h, d2h(h, h->h_stats.data(), h->stats.p, size_t(G) * 32));
     HIPCK(h, sync_stream(h));
 
     h->n_bases = n_bases;
@@ -1150,7 +1177,7 @@ int cfgpu_load_buckets(cfgpu_handle* h, const uint32_t* att, const uint64_t* lab
     HIPCK(h, hipGetLastError());
     h->h_boff.assign(bucket_off, bucket_off + G + 1);
     h->h_stats.resize(size_t(G) * 4);
-    HIPCK(h, hipMemcpyAsync(h->h_stats.data(), h->stats.p, size_t(G) * 32, hipMemcpyDeviceToHost, h->stream));
+    HIPCK(h, d2h(h, h->h_stats.data(), h->stats.p, size_t(G) * 32));
     HIPCK(h, sync_stream(h));
     h->n_bases = 0;
     h->n_frag = 0;
