@@ -113,6 +113,7 @@ struct cfgpu_handle {
         size_t n;
     };
     std::vector<PendingCopy> stage_pending;  // d2h copies finished on the host by sync_stream
+    bool stage_direct = false;                // an h2d copy since the last sync read pageable memory
     std::vector<uint64_t> h_tbl_ptr;
     std::vector<uint64_t> h_cap;
     std::vector<uint32_t> h_limit, h_nvert, h_flags;
@@ -177,6 +178,7 @@ hipError_t h2d(cfgpu_handle* h, void* dst, const void* src, size_t n) {
         if (h->stage_off > h->stage_cap) h->stage_off = h->stage_cap;
         return hipMemcpyAsync(dst, p, n, hipMemcpyHostToDevice, h->stream);
     }
+    h->stage_direct = true;
     return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, h->stream);
 }
 
@@ -202,6 +204,7 @@ hipError_t sync_stream(cfgpu_handle* h) {
         for (const auto& c : h->stage_pending) std::memcpy(c.dst, c.src, c.n);
         h->stage_pending.clear();
         h->stage_off = 0;
+        h->stage_direct = false;
     }
     return e;
 }
@@ -313,6 +316,9 @@ int prepare_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
     HIPCK(h, pool->ensure(n_slots * eb));
     uint64_t off = 0;
     std::vector<uint32_t> ib, ifirst, icnt;
+    ib.reserve(buckets.size());
+    ifirst.reserve(buckets.size());
+    icnt.reserve(buckets.size());
     h->spill_need = 0;
     h->n_all_pass = 0;
     const uint64_t spill_max = spill_cap_max(h);
@@ -358,7 +364,9 @@ int prepare_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
     }
     if (!buckets.empty())
         HIPCK(h, h2d(h, h->d_round_list.p, buckets.data(), buckets.size() * 4));
-    HIPCK(h, sync_stream(h));  // host vectors above are reused
+    // the host vectors above are reused: wait only if one was copied directly
+    // (staged copies took their bytes already)
+    if (h->stage_direct) HIPCK(h, sync_stream(h));
     n_slots_out = n_slots;
     pool_out = pool;
     return CFGPU_OK;
@@ -476,6 +484,7 @@ int run_round_batched(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
     if (!h->graph_exec || key != h->graph_key) {
         if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec), h->graph_exec = nullptr;
         if (h->graph) (void)hipGraphDestroy(h->graph), h->graph = nullptr;
+        HIPCK(h, sync_stream(h));  // capture starts on an idle stream
         HIPCK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
         reset();
         kernel();
@@ -495,6 +504,8 @@ int run_main(cfgpu_handle* h, const std::vector<uint32_t>& order, const std::vec
     const double C = double(lds_capacity(int(h->KW)));
     std::vector<uint32_t> lds_b, glob_b;
     std::vector<uint64_t> lds_c, glob_c;
+    lds_b.reserve(order.size());
+    lds_c.reserve(order.size());
     for (uint32_t b : order) {
         const double need = std::max(1.0, est[b] * margin);
         const uint32_t P = uint32_t(std::ceil(need / (C * LDS_LOAD)));
@@ -881,20 +892,8 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
         std::vector<double> pcost(G, 0.0);
         for (uint32_t b : rest) pcost[b] = have_model ? double(h->h_pred[b]) : cost[b];
         order_largest_first(main_set, pcost);
-        main_l2.resize(main_set.size());
-        for (size_t i = 0; i < main_set.size(); ++i) {
-            const uint32_t b = main_set[i];
-            if (!have_model) {
-                main_l2[i] = upper_cap(h, b);
-                continue;
-            }
-            const double need = double(h->h_pred[b]) * h->tune.margin / h->tune.load_factor;
-            main_l2[i] = std::min(upper_cap(h, b), round_cap(need));
-        }
-        std::vector<double> est(G, 0.0);
-        for (uint32_t b : main_set) est[b] = have_model ? double(h->h_pred[b]) : cost[b];
         HIPCK(h, hipEventRecord(e2.e, h->stream));
-        CK(run_main(h, main_set, est, have_model ? double(h->tune.margin) : 1.0, ovf));
+        CK(run_main(h, main_set, pcost, have_model ? double(h->tune.margin) : 1.0, ovf));
         HIPCK(h, hipEventRecord(e3.e, h->stream));
         rs.n_overflowed = uint32_t(ovf.size());
         rs.fallback_rate = main_set.empty() ? 0.0 : double(ovf.size()) / double(main_set.size());
