@@ -6,10 +6,13 @@
 // LDS and writes the finished table to HBM with plain, coalesced stores.
 //
 // A bucket whose distinct k-mers do not fit one LDS table is split by hash into
-// P passes: pass p keeps the k-mers with pass_of(hash) == p, and every pass
-// re-reads the bucket's records (cheap: they are L2 / Infinity-Cache resident
-// across consecutive passes).  Pass p's LDS table becomes slots
-// [p*C, (p+1)*C) of the bucket's HBM table.  One work item = one pass.
+// P passes: pass p keeps the k-mers whose 32-bit pass hash h has
+// (h * P) >> 32 == p.  Pass p's LDS table becomes slots [p*C, (p+1)*C) of
+// the bucket's HBM table.  A bucket of 2..8 passes is one work item: its
+// records are walked once, every instance going to a per-workgroup spill list
+// (pass 0's or the later passes'), and each pass probes its list with every
+// lane busy.  Other buckets are one item per pass (P = 1, or more passes than
+// an item should serialise), each walking the records.
 //
 // Each lane walks a stream of whole records (RecordWalker: the k-mer and its
 // reverse complement are rolled one base per instance), so no staging
