@@ -138,6 +138,12 @@ __global__ __launch_bounds__(256) void k_ends_insert(CdbgArgs a) {
 
 // State 2r + e: piece r entered at end e.  Its successor is the piece across
 // the vertex at r's other end, entered at its matching end.
+// State st has no partner across the end it leaves through: a chain end.
+__device__ __forceinline__ bool rank_end(const CdbgArgs& a, uint64_t st) {
+    const uint32_t es = a.end_slot[st ^ 1];
+    return es == NONE || a.hval[2 * uint64_t(es >> 1) + (1u - (es & 1u))] == NONE;
+}
+
 __global__ __launch_bounds__(256) void k_rank_init(CdbgArgs a) {
     for (uint64_t st = uint64_t(blockIdx.x) * 256 + threadIdx.x; st < 2 * a.n; st += uint64_t(gridDim.x) * 256) {
         const uint32_t es = a.end_slot[st ^ 1];  // the end it leaves through
@@ -180,7 +186,10 @@ __global__ __launch_bounds__(256) void k_place(CdbgArgs a) {
             const UnitigRec u = a.recs[r];
             const uint64_t w0 = jp[2 * r], w1 = jp[2 * r + 1];
             const uint64_t E0 = w0 & JT_MASK, E1 = w1 & JT_MASK;
-            if (jp[E0] != E0 || jp[E1] != E1) {  // not chain ends (word = itself, 0 characters): a cycle
+            // E0 and E1 are where the piece's two readings stop: a chain's ends,
+            // or, on a cycle, wherever the jumping stopped (a cycle state can land
+            // on itself).  Ends are the states with no partner (as in k_rank_init).
+            if (!rank_end(a, E0) || !rank_end(a, E1)) {  // a cycle
                 a.place_head[r] = NONE;
                 atomicAdd(a.n_cycle_pieces, 1u);
             } else {

@@ -66,6 +66,20 @@ def test_cdbg_cycles_within_and_across_subgraphs():
         assert gpu_cdbg.last_stats["n_cdbg_cycle_pieces"] > 0
 
 
+def test_cdbg_two_piece_cycles():
+    # period-16 tandem repeats at k=15, l=7 whose cycle is glued from exactly two
+    # lm-tigs (the shortest cycle through two subgraphs): pointer jumping can
+    # land a state of such a cycle on itself, and it must still read as a cycle
+    units = ["GACCTCTGACGATTGA", "TCTACACTCATTGGAT", "AATCCAAACGCCAGTT",
+             "GCCTAACATCGTTACG", "ACTCTAATCACGAGAG", "CTGGATGGCTCTTTTT"]
+    buf = fasta([(u * 14).encode() for u in units])
+    seq, ro = O.parse(buf, False)
+    say("[case] cdbg-two-piece-cycles k=15")
+    got = gpu_cdbg(buf, False, 15, 7, 1, mode="batched")
+    assert got == U.unitigs(seq, ro, 15, 1)
+    assert gpu_cdbg.last_stats["n_cdbg_cycle_pieces"] >= 2 * len(units)
+
+
 def test_cdbg_reference_with_repeat_copies():
     # 4,000,000 iid bases plus 200 copies of a 1 kb repeat (1 % substitutions), k = 31, cutoff 1
     base = C.synth_genome(4_000_000, 0.0, 300, 6000, 0.0, seed=101)
