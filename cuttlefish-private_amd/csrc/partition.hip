@@ -171,69 +171,6 @@ __global__ __launch_bounds__(P_WG) void k_minimizer_g(PartitionArgs a, uint16_t*
     if (tid == 0) tile_lc[blockIdx.x] = s_lc;
 }
 
-// Inclusive max-scan of one u32 per thread across the workgroup.
-__device__ __forceinline__ uint32_t block_incl_max(uint32_t v, uint32_t* s_wave) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t u = __shfl_up(v, d);
-        if (lane >= d) v = u > v ? u : v;
-    }
-    if (lane == 63) s_wave[wid] = v;
-    __syncthreads();
-    uint32_t pre = 0;
-    for (int j = 0; j < wid; ++j) pre = s_wave[j] > pre ? s_wave[j] : pre;
-    __syncthreads();
-    return v > pre ? v : pre;
-}
-
-// Exclusive sum-scan of one u32 per thread across a WG-thread workgroup;
-// writes the total to *total.
-template <int WG>
-__device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* s_wave, uint32_t* total) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t u = __shfl_up(x, d);
-        if (lane >= d) x += u;
-    }
-    __syncthreads();  // s_wave may still be read by a previous scan
-    if (lane == 63) s_wave[wid] = x;
-    __syncthreads();
-    uint32_t pre = 0, tot = 0;
-#pragma unroll
-    for (int j = 0; j < WG / 64; ++j) {
-        const uint32_t t = s_wave[j];
-        pre += j < wid ? t : 0;
-        tot += t;
-    }
-    *total = tot;
-    return pre + x - v;
-}
-
-// Per thread: min of v over the threads after it in a WG-thread workgroup
-// (tail for the last thread).
-template <int WG>
-__device__ __forceinline__ uint32_t block_excl_suffix_min(uint32_t v, uint32_t* s_wave, uint32_t tail) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t u = __shfl_down(x, d);
-        if (lane + d < 64) x = u < x ? u : x;
-    }
-    __syncthreads();  // s_wave may still be read by a previous scan
-    if (lane == 0) s_wave[wid] = x;
-    __syncthreads();
-    uint32_t after = tail;
-#pragma unroll
-    for (int j = 0; j < WG / 64; ++j)
-        if (j > wid) after = s_wave[j] < after ? s_wave[j] : after;
-    const uint32_t nxt = __shfl_down(x, 1);
-    return lane < 63 ? (nxt < after ? nxt : after) : after;
-}
-
 // Packed words staged per tile: records start at t0 - 1 or later and end
 // before t0 + TILE + 256 (len <= 255), and stream_get64 reads one word ahead.
 #define TILE_WORDS(TILE) ((TILE) / 32 + 16)
@@ -283,8 +220,7 @@ __host__ __device__ constexpr uint64_t piece_desc(uint32_t idx, uint32_t g, uint
 template <int WG, int TILE, class Emit>
 __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a, uint64_t t0,
                                                           unsigned long long carry, const uint16_t* s_g,
-                                                          const uint32_t* s_mbw, uint32_t* s_wave,
-                                                          uint32_t& n_out, Emit&& emit) {
+                                                          const uint32_t* s_mbw, uint32_t& n_out, Emit&& emit) {
     constexpr int PPT = TILE / WG;  // positions per thread
     static_assert(PPT <= 32 && TILE % P_TILE == 0, "tile shape");
     const int tid = threadIdx.x;
@@ -315,20 +251,6 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
             last_chg = uint32_t(base_idx + j);
         }
     }
-    const uint32_t incl = block_incl_max(last_chg, s_wave);
-    const uint32_t excl_t = __shfl_up(incl, 1);
-    uint32_t excl = (threadIdx.x & 63) ? excl_t : 0;
-    if ((threadIdx.x & 63) == 0 && tid > 0) {
-        // first lane of a wave: the exclusive value is the max over earlier waves
-        for (int j = 0; j < (tid >> 6); ++j) excl = s_wave[j] > excl ? s_wave[j] : excl;
-    }
-    // the tile's last change, for the next tile's carry (every wave's
-    // maximum is in s_wave until the next scan)
-    uint32_t tile_last = 0;
-    for (int j = 0; j < WG / 64; ++j) tile_last = s_wave[j] > tile_last ? s_wave[j] : tile_last;
-    const unsigned long long carry_out = tile_last ? (unsigned long long)(q0 + int64_t(tile_last) + 1) : carry;
-    const int64_t rs0 = excl ? q0 + int64_t(excl) : (carry ? int64_t(carry) - 1 : -1);
-
     // Run boundaries (a change flag or an invalid position) and, per thread,
     // the first boundary after its positions: a piece starting at idx ends at
     // min(next boundary, idx + w).
@@ -336,8 +258,24 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
 #pragma unroll
     for (int j = 0; j < PPT; ++j) bnd |= uint32_t(gv[j + 1] == G16_INVALID) << j;
     constexpr uint32_t NONE = 0xFFFFFFFFu;
-    uint32_t tail = NONE;  // first boundary in (TILE, TILE + w + 1]
-    if (tid < 64) {
+    constexpr int NW = WG / 64;
+    __shared__ uint32_t s_wmax[NW], s_wmin[NW], s_wsum[NW], s_tail;
+    const int lane = tid & 63, wid = tid >> 6;
+
+    // One barrier for three things: per wave the running maximum of the last
+    // change (run starts) and the suffix minimum of the first boundary (piece
+    // ends), and the first boundary past the tile found by wave 0.
+    uint32_t vmax = last_chg, vmin = bnd ? uint32_t(base_idx + __ffs(int(bnd)) - 1) : NONE;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(vmax, d), v = __shfl_down(vmin, d);
+        if (lane >= d) vmax = u > vmax ? u : vmax;
+        if (lane + d < 64) vmin = v < vmin ? v : vmin;
+    }
+    if (lane == 63) s_wmax[wid] = vmax;
+    if (lane == 0) s_wmin[wid] = vmin;
+    if (tid < 64) {  // first boundary in (TILE, TILE + w + 1]
+        uint32_t tail = NONE;
         for (int i0 = TILE + 1; i0 <= TILE + int(w) + 1 && tail == NONE; i0 += 64) {
             const int i = i0 + tid;
             bool b = false;
@@ -348,13 +286,23 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
             const unsigned long long m = __ballot(b);
             if (m) tail = uint32_t(i0 + __ffsll((long long)m) - 1);
         }
+        if (tid == 0) s_tail = tail;
     }
-    __shared__ uint32_t s_tail;
-    if (tid == 0) s_tail = tail;
     __syncthreads();
-    tail = s_tail;
-    const uint32_t next_bnd = block_excl_suffix_min<WG>(bnd ? uint32_t(base_idx + __ffs(int(bnd)) - 1) : NONE,
-                                                        s_wave, tail);
+    uint32_t before = 0, tile_last = 0, after = s_tail;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+        const uint32_t m = s_wmax[j], n = s_wmin[j];
+        before = j < wid && m > before ? m : before;
+        tile_last = m > tile_last ? m : tile_last;
+        after = j > wid && n < after ? n : after;
+    }
+    const uint32_t incl = vmax > before ? vmax : before;
+    const uint32_t up = __shfl_up(incl, 1), down = __shfl_down(vmin, 1);
+    const uint32_t excl = lane ? up : before;  // the last change before my positions (0: none)
+    const uint32_t next_bnd = lane < 63 ? (down < after ? down : after) : after;
+    const unsigned long long carry_out = tile_last ? (unsigned long long)(q0 + int64_t(tile_last) + 1) : carry;
+    const int64_t rs0 = excl ? q0 + int64_t(excl) : (carry ? int64_t(carry) - 1 : -1);
 
     // A piece starts at every change and every w (k-1)-mers after one.  The
     // phase (p - run start) mod w is carried from position to position; a
@@ -376,8 +324,24 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
             starts |= uint32_t(ph == 0 && gv[j + 1] != G16_INVALID) << j;
         }
     }
-    uint32_t n_tile = 0;
-    uint32_t o_desc = block_excl_sum<WG>(uint32_t(__popc(starts)), s_wave, &n_tile);
+    // exclusive sum of the piece counts (one barrier; the arrays are rewritten
+    // only after the caller's next barrier)
+    const uint32_t mine = uint32_t(__popc(starts));
+    uint32_t xs = mine;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(xs, d);
+        if (lane >= d) xs += u;
+    }
+    if (lane == 63) s_wsum[wid] = xs;
+    __syncthreads();
+    uint32_t n_tile = 0, o_desc = xs - mine;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+        const uint32_t t = s_wsum[j];
+        n_tile += t;
+        o_desc += j < wid ? t : 0u;
+    }
     for (uint32_t m = starts; m;) {
         const uint32_t j = uint32_t(__ffs(int(m))) - 1u;
         m &= m - 1u;
@@ -436,7 +400,6 @@ __global__ __launch_bounds__(H_WG, 5) void k_pieces_hist(PartitionArgs a, const 
     __shared__ __attribute__((aligned(16))) uint16_t s_g[P_TILE + WIN_MAX + 2];
     __shared__ uint32_t s_mbw[P_TILE / 8];
     __shared__ unsigned long long s_carry;
-    __shared__ uint32_t s_wave[H_WG / 64];
     __shared__ uint32_t s_hist[N_COARSE];
     __shared__ uint64_t s_desc[P_TILE];  // the tile's descriptors, written out coalesced
     for (int i = threadIdx.x; i < N_COARSE; i += H_WG) s_hist[i] = 0;
@@ -470,7 +433,7 @@ __global__ __launch_bounds__(H_WG, 5) void k_pieces_hist(PartitionArgs a, const 
         }
         __syncthreads();
         uint32_t n_t;
-        carry = tile_pieces<H_WG, P_TILE>(a, t * P_TILE, carry, s_g, s_mbw, s_wave, n_t,
+        carry = tile_pieces<H_WG, P_TILE>(a, t * P_TILE, carry, s_g, s_mbw, n_t,
                                           [&](uint32_t o, uint64_t d) { s_desc[o] = d; });
         __syncthreads();
         uint64_t* out = desc + t * P_TILE;
