@@ -318,7 +318,9 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                 // A pass over the list the walk or the previous pass left, one
                 // entry per lane, in list order.
                 const SpillList in = spill_list<KW>(a, p & 1u);
-                const uint32_t n_in = s_sn[p & 1u];
+                // (a list that overflowed has flagged the bucket: nothing past
+                // its capacity is read)
+                const uint32_t n_in = min(s_sn[p & 1u], a.spill_cap);
                 uint32_t seen = 0;
                 Kmer<KW> key_n;  // the lane's next entry, loaded while this one is probed
                 uint32_t meta_n = 0;
