@@ -620,6 +620,7 @@ __global__ __launch_bounds__(E_WG) void k_pieces_emit(PartitionArgs a, const uin
 // stores instead of 64 interleaved streams).  Order inside a bucket is free.
 constexpr uint32_t FINE_WG = 1024;
 constexpr uint32_t FINE_LDS = 144 * 1024;  // staging bytes
+constexpr uint32_t CH_MAX = 4096;
 __host__ __device__ constexpr uint32_t fine_chunk(uint32_t W) {
     // largest power of two <= FINE_LDS / (record bytes + 2 permutation bytes), at most 4096
     return (FINE_LDS / (13 + 8 * W)) >= 4096 ? 4096 : (FINE_LDS / (13 + 8 * W)) >= 2048 ? 2048 : 1024;
@@ -631,6 +632,7 @@ __global__ __launch_bounds__(FINE_WG) void k_bucket_fine(const uint32_t* __restr
                                                          uint64_t* __restrict__ boff, uint32_t* __restrict__ att,
                                                          uint64_t* __restrict__ labels, uint8_t* __restrict__ mbin) {
     constexpr uint32_t NF = 1u << FINE_BITS;
+    static_assert(NF == 64, "one wave scans the bucket counts");
     __shared__ uint32_t s_cnt[NF], s_ccnt[NF], s_cbase[NF];
     __shared__ unsigned long long s_cur[NF];
     __shared__ __attribute__((aligned(16))) uint8_t s_stage[FINE_LDS];
@@ -646,34 +648,69 @@ __global__ __launch_bounds__(FINE_WG) void k_bucket_fine(const uint32_t* __restr
 
     for (uint32_t i = tid; i < NF; i += FINE_WG) s_cnt[i] = 0;
     __syncthreads();
-    for (uint64_t r = r0 + tid; r < r1; r += FINE_WG) atomicAdd(&s_cnt[(hdr[r] >> 16) & (NF - 1)], 1u);
-    __syncthreads();
-    if (tid == 0) {
-        uint64_t run = r0;
-        for (uint32_t f = 0; f < NF; ++f) {
-            boff[c * NF + f] = run;
-            s_cur[f] = run;
-            run += s_cnt[f];
-        }
-        if (c + 1 == gridDim.x) boff[(c + 1) * NF] = r1;
+    // bucket counts of the bin: 8 independent header loads per thread in flight
+    for (uint64_t rb = r0 + tid; rb < r1; rb += 8 * FINE_WG) {
+        uint32_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = hdr[min(rb + uint64_t(j) * FINE_WG, r1 - 1)];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (rb + uint64_t(j) * FINE_WG < r1) atomicAdd(&s_cnt[(v[j] >> 16) & (NF - 1)], 1u);
     }
+    __syncthreads();
+    // exclusive scan of the bucket counts by the first wave (lane f: bucket f)
+    const auto wave_excl = [](uint32_t v) {
+        uint32_t x = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t u = __shfl_up(x, d);
+            if (int(threadIdx.x) >= d) x += u;
+        }
+        return x - v;
+    };
+    if (tid < NF) {
+        const uint64_t run = r0 + wave_excl(s_cnt[tid]);
+        boff[c * NF + tid] = run;
+        s_cur[tid] = run;
+        if (tid == 0 && c + 1 == gridDim.x) boff[(c + 1) * NF] = r1;
+    }
+    // A chunk's headers and first two label words are loaded into registers
+    // while the previous chunk is written out (one workgroup per CU has no
+    // other loads in flight then); later label words are read when staged.
+    static_assert(CH_MAX == 4 * FINE_WG, "4 records per thread");
+    uint32_t h_next[4];
+    u64x2 l_next[4];
+    const auto prefetch = [&](uint64_t q0) {
+        const uint32_t n = uint32_t(min(uint64_t(CH), r1 - q0));
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t i = tid + j * FINE_WG;
+            if (i < n) {
+                h_next[j] = hdr[q0 + i];
+                l_next[j] = *reinterpret_cast<const u64x2*>(lab + (q0 + i) * LW);
+            }
+        }
+    };
+    if (r0 < r1) prefetch(r0);
     for (uint64_t q0 = r0; q0 < r1; q0 += CH) {
         const uint32_t n = uint32_t(min(uint64_t(CH), r1 - q0));
         if (tid < NF) s_ccnt[tid] = 0;
         __syncthreads();
-        // stage the chunk (coalesced 16-byte reads) and rank each record in its bucket
+        // stage the chunk and rank each record in its bucket
         uint32_t rank[4], fb[4];
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
             const uint32_t i = tid + j * FINE_WG;
             rank[j] = fb[j] = 0;
             if (i >= n) continue;
-            const uint32_t h0 = hdr[q0 + i];
+            const uint32_t h0 = h_next[j];
             const uint32_t at = h0 & ~(15u << L1_MB_SHIFT);
             s_att[i] = at;
             s_mb[i] = uint8_t((h0 >> L1_MB_SHIFT) & 15u);
+            s_lab[size_t(i) * W] = l_next[j].x;
+            if (W > 1) s_lab[size_t(i) * W + 1] = l_next[j].y;
             const u64x2* in = reinterpret_cast<const u64x2*>(lab + (q0 + i) * LW);
-            for (uint32_t q = 0; q < W; q += 2) {
+            for (uint32_t q = 2; q < W; q += 2) {
                 const u64x2 pq = in[q / 2];
                 s_lab[size_t(i) * W + q] = pq.x;
                 if (q + 1 < W) s_lab[size_t(i) * W + q + 1] = pq.y;
@@ -682,13 +719,7 @@ __global__ __launch_bounds__(FINE_WG) void k_bucket_fine(const uint32_t* __restr
             rank[j] = atomicAdd(&s_ccnt[fb[j]], 1u);
         }
         __syncthreads();
-        if (tid == 0) {
-            uint32_t run = 0;
-            for (uint32_t f = 0; f < NF; ++f) {
-                s_cbase[f] = run;
-                run += s_ccnt[f];
-            }
-        }
+        if (tid < NF) s_cbase[tid] = wave_excl(s_ccnt[tid]);
         __syncthreads();
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
@@ -696,6 +727,7 @@ __global__ __launch_bounds__(FINE_WG) void k_bucket_fine(const uint32_t* __restr
             if (i < n) s_perm[s_cbase[fb[j]] + rank[j]] = uint16_t(i);
         }
         __syncthreads();
+        if (q0 + CH < r1) prefetch(q0 + CH);
         // write the sorted chunk: consecutive threads, consecutive slots of a bucket
         for (uint32_t s2 = tid; s2 < n; s2 += FINE_WG) {
             const uint32_t i = s_perm[s2];
