@@ -528,11 +528,25 @@ __global__ __launch_bounds__(E_WG) void k_pieces_emit(PartitionArgs a, const uin
         const uint64_t wlo = t0 ? t0 / 32 - 1 : 0;  // s_words[0] holds bases wlo*32 ..
         const uint64_t base0 = wlo * 32;
         __syncthreads();  // the previous batch is done with s_words and s_toff
-        for (uint32_t i = tid; i < nt * (P_TILE / 32) + 16; i += E_WG)
-            s_words[i] = wlo + i < a.n_words ? a.packed[wlo + i] : 0ull;
+        // the batch's tile piece counts, loaded with its packed words
+        const uint32_t tn = tid < 64 && lane < nt ? tile_n[ta + lane] : 0u;
+        {  // every packed-word load of the batch in flight at once
+            constexpr int WU = (EB_WORDS + E_WG - 1) / E_WG;
+            const uint32_t nw = nt * (P_TILE / 32) + 16;
+            uint64_t wv[WU];
+#pragma unroll
+            for (int u = 0; u < WU; ++u) {
+                const uint32_t i = tid + uint32_t(u) * E_WG;
+                wv[u] = i < nw && wlo + i < a.n_words ? a.packed[wlo + i] : 0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < WU; ++u) {
+                const uint32_t i = tid + uint32_t(u) * E_WG;
+                if (i < nw) s_words[i] = wv[u];
+            }
+        }
         if (tid < 64) {  // descriptor offsets of the batch's tiles
-            const uint32_t v = lane < nt ? tile_n[ta + lane] : 0u;
-            uint32_t x = v;
+            uint32_t x = tn;
 #pragma unroll
             for (int d = 1; d < 64; d <<= 1) {
                 const uint32_t u = __shfl_up(x, d);
@@ -547,17 +561,33 @@ __global__ __launch_bounds__(E_WG) void k_pieces_emit(PartitionArgs a, const uin
             const uint32_t m = min(uint32_t(EB_CAP), n_b - x0);
             if (tid < N_COARSE) s_bcnt[tid] = 0;
             __syncthreads();
-            for (uint32_t j = tid; j < m; j += E_WG) {  // stage, tagged with the tile (bits 48+)
-                const uint32_t jj = x0 + j;
-                uint32_t lo = 0, hi = nt;  // tile = last tt with s_toff[tt] <= jj
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (s_toff[mid] <= jj) lo = mid;
-                    else hi = mid;
+            // stage, tagged with the tile (bits 48+): SU descriptors per thread
+            // located (independent searches) and loaded together, indices
+            // clamped to the last one so that no load is conditional
+            constexpr int SU = 4;
+            for (uint32_t j0 = tid; j0 < m; j0 += SU * E_WG) {
+                uint64_t dv[SU];
+                uint32_t tv[SU];
+#pragma unroll
+                for (int u = 0; u < SU; ++u) {
+                    const uint32_t jj = x0 + min(j0 + uint32_t(u) * E_WG, m - 1);
+                    uint32_t lo = 0, hi = nt;  // tile = last tt with s_toff[tt] <= jj
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (s_toff[mid] <= jj) lo = mid;
+                        else hi = mid;
+                    }
+                    tv[u] = lo;
+                    dv[u] = desc[(ta + lo) * P_TILE + (jj - s_toff[lo])];
                 }
-                const uint64_t d = desc[(ta + lo) * P_TILE + (jj - s_toff[lo])];
-                s_d[j] = d | (uint64_t(lo) << 48);
-                atomicAdd(&s_bcnt[(uint32_t(d >> 16) & 0xFFFFu) >> FINE_BITS], 1u);
+#pragma unroll
+                for (int u = 0; u < SU; ++u) {
+                    const uint32_t j = j0 + uint32_t(u) * E_WG;
+                    if (j < m) {
+                        s_d[j] = dv[u] | (uint64_t(tv[u]) << 48);
+                        atomicAdd(&s_bcnt[(uint32_t(dv[u] >> 16) & 0xFFFFu) >> FINE_BITS], 1u);
+                    }
+                }
             }
             __syncthreads();
             uint32_t v = 0, x = 0;  // exclusive scan of the bin counts by the first 4 waves
