@@ -162,10 +162,10 @@ __device__ __forceinline__ void spill_push(bool push, const Kmer<KW>& key, uint3
     const unsigned long long mask = __ballot(push);
     if (!mask) return;
     const int lane = int(__lane_id());
-    const int leader = __ffsll((long long)mask) - 1;
+    const int leader = __ffsll((long long)mask) - 1;  // wave-uniform
     uint32_t base = 0;
     if (lane == leader) base = atomicAdd(s_n, uint32_t(__popcll(mask)));
-    base = __shfl(base, leader);
+    base = __builtin_amdgcn_readlane(base, leader);
     if (!push) return;
     const uint32_t idx = base + __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
     if (idx >= cap) {
@@ -178,18 +178,21 @@ __device__ __forceinline__ void spill_push(bool push, const Kmer<KW>& key, uint3
 }
 
 // Append every active lane's instance to list 1 (sel) or list 0 (!sel): one
-// LDS atomic per list per wave.  A full list sets *ovf.
+// 64-bit LDS atomic per wave reserves the wave's entries in both lists (the
+// two u32 counters s_n[0..1] are one aligned u64).  A full list sets *ovf.
 template <int KW>
 __device__ __forceinline__ void spill_push_split(bool sel, const Kmer<KW>& key, uint32_t meta, const SpillList& L0,
                                                  const SpillList& L1, uint32_t cap, uint32_t* s_n,
                                                  uint32_t* s_ovf) {
     const unsigned long long m1 = __ballot(sel), m0 = __ballot(!sel);
     const int lane = int(__lane_id());
-    const int l0 = m0 ? __ffsll((long long)m0) - 1 : -1, l1 = m1 ? __ffsll((long long)m1) - 1 : -1;
-    uint32_t b = 0;
-    if (lane == l0) b = atomicAdd(&s_n[0], uint32_t(__popcll(m0)));
-    if (lane == l1) b = atomicAdd(&s_n[1], uint32_t(__popcll(m1)));
-    const uint32_t b0 = __shfl(b, l0 < 0 ? lane : l0), b1 = __shfl(b, l1 < 0 ? lane : l1);
+    const int leader = __ffsll((long long)(m0 | m1)) - 1;  // wave-uniform
+    unsigned long long b = 0;
+    if (lane == leader)
+        b = atomicAdd(reinterpret_cast<unsigned long long*>(s_n),
+                      (unsigned long long)__popcll(m0) | ((unsigned long long)__popcll(m1) << 32));
+    const uint32_t b0 = __builtin_amdgcn_readlane(uint32_t(b), leader);
+    const uint32_t b1 = __builtin_amdgcn_readlane(uint32_t(b >> 32), leader);
     const unsigned long long m = sel ? m1 : m0;
     const uint32_t idx = (sel ? b1 : b0) +
                          __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
@@ -211,7 +214,8 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
     __shared__ uint64_t s_key[C * KW];
     __shared__ uint32_t s_ef[C];
     __shared__ uint32_t s_st[C / 4];  // one Vertex_Status byte per slot
-    __shared__ uint32_t s_item, s_cnt, s_ovf, s_sn[2];
+    __shared__ uint32_t s_item, s_cnt, s_ovf;
+    __shared__ __attribute__((aligned(8))) uint32_t s_sn[2];  // spill list lengths (one u64 for the walk's pushes)
 
     const int tid = threadIdx.x;
     const uint32_t W = a.W, k = a.k;
