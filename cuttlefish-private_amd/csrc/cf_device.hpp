@@ -167,6 +167,54 @@ __device__ __forceinline__ uint64_t fmix64(uint64_t h) {
 }
 
 // murmur3's 32-bit finaliser.
+// Wave-wide scans by DPP lane moves (no LDS traffic).  A 64-lane wave is 4
+// rows of 16 lanes: row_shr / row_shl move values within a row, row_bcast
+// 15 / 31 carry a row's last lane into the rows above, wave_shr / wave_shl
+// move every lane by one.  A lane whose source is outside the row (or the
+// wave) receives `old`.
+namespace dpp {
+constexpr int ROW_SHL(int n) { return 0x100 + n; }
+constexpr int ROW_SHR(int n) { return 0x110 + n; }
+constexpr int WAVE_SHL1 = 0x130, WAVE_SHR1 = 0x138, ROW_BCAST15 = 0x142, ROW_BCAST31 = 0x143;
+
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ uint32_t move(uint32_t old, uint32_t src) {
+    return uint32_t(__builtin_amdgcn_update_dpp(int(old), int(src), CTRL, ROW_MASK, 0xF, false));
+}
+
+// Inclusive scan over lanes 0..lane with an associative op whose identity is id.
+template <class Op>
+__device__ __forceinline__ uint32_t incl_scan(uint32_t x, uint32_t id, Op op) {
+    x = op(x, move<ROW_SHR(1)>(id, x));
+    x = op(x, move<ROW_SHR(2)>(id, x));
+    x = op(x, move<ROW_SHR(4)>(id, x));
+    x = op(x, move<ROW_SHR(8)>(id, x));
+    x = op(x, move<ROW_BCAST15, 0xA>(id, x));  // rows 1, 3 take rows 0, 2
+    x = op(x, move<ROW_BCAST31, 0xC>(id, x));  // rows 2, 3 take rows 0..1
+    return x;
+}
+
+// Inclusive scan over lanes lane..63 (a suffix scan) with identity id.
+template <class Op>
+__device__ __forceinline__ uint32_t suffix_scan(uint32_t x, uint32_t id, Op op) {
+    x = op(x, move<ROW_SHL(1)>(id, x));
+    x = op(x, move<ROW_SHL(2)>(id, x));
+    x = op(x, move<ROW_SHL(4)>(id, x));
+    x = op(x, move<ROW_SHL(8)>(id, x));
+    // a row's first lane now holds the whole row; rows above lane's row
+    const uint32_t r1 = __builtin_amdgcn_readlane(x, 16), r2 = __builtin_amdgcn_readlane(x, 32),
+                   r3 = __builtin_amdgcn_readlane(x, 48);
+    const uint32_t row = __lane_id() >> 4;
+    const uint32_t from3 = row < 3 ? r3 : id, from2 = row < 2 ? op(r2, from3) : from3,
+                   from1 = row < 1 ? op(r1, from2) : from2;
+    return op(x, from1);
+}
+
+// The value of lane - 1 (lane 0: old) and of lane + 1 (lane 63: old).
+__device__ __forceinline__ uint32_t from_prev_lane(uint32_t old, uint32_t x) { return move<WAVE_SHR1>(old, x); }
+__device__ __forceinline__ uint32_t from_next_lane(uint32_t old, uint32_t x) { return move<WAVE_SHL1>(old, x); }
+}  // namespace dpp
+
 __device__ __forceinline__ uint32_t fmix32(uint32_t h) {
     h ^= h >> 16;
     h *= 0x85ebca6bu;

@@ -264,14 +264,12 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
 
     // One barrier for three things: per wave the running maximum of the last
     // change (run starts) and the suffix minimum of the first boundary (piece
-    // ends), and the first boundary past the tile found by wave 0.
-    uint32_t vmax = last_chg, vmin = bnd ? uint32_t(base_idx + __ffs(int(bnd)) - 1) : NONE;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t u = __shfl_up(vmax, d), v = __shfl_down(vmin, d);
-        if (lane >= d) vmax = u > vmax ? u : vmax;
-        if (lane + d < 64) vmin = v < vmin ? v : vmin;
-    }
+    // ends), and the first boundary past the tile found by wave 0.  The wave
+    // scans are DPP lane moves (every lane of the workgroup is active here).
+    const auto umax = [](uint32_t x, uint32_t y) { return x > y ? x : y; };
+    const auto umin = [](uint32_t x, uint32_t y) { return x < y ? x : y; };
+    const uint32_t vmax = dpp::incl_scan(last_chg, 0u, umax);
+    const uint32_t vmin = dpp::suffix_scan(bnd ? uint32_t(base_idx + __ffs(int(bnd)) - 1) : NONE, NONE, umin);
     if (lane == 63) s_wmax[wid] = vmax;
     if (lane == 0) s_wmin[wid] = vmin;
     if (tid < 64) {  // first boundary in (TILE, TILE + w + 1]
@@ -298,9 +296,9 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
         after = j > wid && n < after ? n : after;
     }
     const uint32_t incl = vmax > before ? vmax : before;
-    const uint32_t up = __shfl_up(incl, 1), down = __shfl_down(vmin, 1);
-    const uint32_t excl = lane ? up : before;  // the last change before my positions (0: none)
-    const uint32_t next_bnd = lane < 63 ? (down < after ? down : after) : after;
+    // the last change before my positions (0: none), the first boundary after them
+    const uint32_t excl = dpp::from_prev_lane(before, incl);
+    const uint32_t next_bnd = umin(dpp::from_next_lane(NONE, vmin), after);
     const unsigned long long carry_out = tile_last ? (unsigned long long)(q0 + int64_t(tile_last) + 1) : carry;
     const int64_t rs0 = excl ? q0 + int64_t(excl) : (carry ? int64_t(carry) - 1 : -1);
 
@@ -327,12 +325,7 @@ __device__ __forceinline__ unsigned long long tile_pieces(const PartitionArgs& a
     // exclusive sum of the piece counts (one barrier; the arrays are rewritten
     // only after the caller's next barrier)
     const uint32_t mine = uint32_t(__popc(starts));
-    uint32_t xs = mine;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t u = __shfl_up(xs, d);
-        if (lane >= d) xs += u;
-    }
+    const uint32_t xs = dpp::incl_scan(mine, 0u, [](uint32_t x, uint32_t y) { return x + y; });
     if (lane == 63) s_wsum[wid] = xs;
     __syncthreads();
     uint32_t n_tile = 0, o_desc = xs - mine;
