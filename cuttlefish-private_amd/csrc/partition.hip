@@ -538,13 +538,8 @@ __global__ __launch_bounds__(E_WG) void k_pieces_emit(PartitionArgs a, const uin
                 if (i < nw) s_words[i] = wv[u];
             }
         }
-        if (tid < 64) {  // descriptor offsets of the batch's tiles
-            uint32_t x = tn;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const uint32_t u = __shfl_up(x, d);
-                if (int(lane) >= d) x += u;
-            }
+        if (tid < 64) {  // descriptor offsets of the batch's tiles (the whole first wave)
+            const uint32_t x = dpp::incl_scan(tn, 0u, [](uint32_t p, uint32_t q) { return p + q; });
             if (lane < nt) s_toff[lane + 1] = x;
             if (lane == 0) s_toff[0] = 0;
         }
@@ -583,14 +578,10 @@ __global__ __launch_bounds__(E_WG) void k_pieces_emit(PartitionArgs a, const uin
                 }
             }
             __syncthreads();
-            uint32_t v = 0, x = 0;  // exclusive scan of the bin counts by the first 4 waves
+            uint32_t v = 0, x = 0;  // exclusive scan of the bin counts by the whole first 4 waves
             if (tid < N_COARSE) {
-                v = x = s_bcnt[tid];
-#pragma unroll
-                for (int d = 1; d < 64; d <<= 1) {
-                    const uint32_t u = __shfl_up(x, d);
-                    if (int(lane) >= d) x += u;
-                }
+                v = s_bcnt[tid];
+                x = dpp::incl_scan(v, 0u, [](uint32_t p, uint32_t q) { return p + q; });
                 if (lane == 63) s_wave[tid >> 6] = x;
             }
             __syncthreads();
@@ -681,15 +672,9 @@ __global__ __launch_bounds__(FINE_WG) void k_bucket_fine(const uint32_t* __restr
             if (rb + uint64_t(j) * FINE_WG < r1) atomicAdd(&s_cnt[(v[j] >> 16) & (NF - 1)], 1u);
     }
     __syncthreads();
-    // exclusive scan of the bucket counts by the first wave (lane f: bucket f)
+    // exclusive scan of the bucket counts by the whole first wave (lane f: bucket f)
     const auto wave_excl = [](uint32_t v) {
-        uint32_t x = v;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t u = __shfl_up(x, d);
-            if (int(threadIdx.x) >= d) x += u;
-        }
-        return x - v;
+        return dpp::incl_scan(v, 0u, [](uint32_t x, uint32_t y) { return x + y; }) - v;
     };
     if (tid < NF) {
         const uint64_t run = r0 + wave_excl(s_cnt[tid]);

@@ -1,10 +1,10 @@
-"""The independent distinct-k-mer count of the full-size GPU test, checked on
+"""The independent distinct-k-mer count of the full-size GPU test (fullsize_count.py), checked on
 the CPU against a plain numpy restatement on a small read set (both strands
 rolled base by base; min of forward and reverse complement)."""
 import numpy as np
 
 import cfgpu as C
-from test_gpu_fullsize import canonical_kmers_device
+from fullsize_count import canonical_kmers_device
 
 
 def numpy_canonical(reads: bytes, n_reads, read_len, k):

@@ -6,10 +6,16 @@ The oracle cannot run at this size, so the checks are ones the domain fixes.
 (1) A k-mer is a vertex of every subgraph one of its weak super k-mers reaches
 (src/Subgraph.cpp:57-105 inserts each instance into the bucket's table), so
 the union of all subgraphs' vertex keys is exactly the set of distinct
-canonical 31-mers of the reads.  That set is computed independently here with
-torch on the device.  (2) No subgraph holds a key twice.  (3) The learned
-batched path and the plain per-bucket path build the same number of vertices
-in every subgraph."""
+canonical 31-mers of the reads.  (2) No subgraph holds a key twice.  (3) The
+learned batched path and the plain per-bucket path build the same number of
+vertices in every subgraph.  (1) and (2) are counted independently with torch
+on the device, in a child process (fullsize_count.py) so that torch is the
+only GPU user there."""
+import json
+import os
+import subprocess
+import sys
+
 import numpy as np
 import pytest
 
@@ -20,36 +26,10 @@ pytestmark = pytest.mark.gpu
 
 K, L, CUTOFF = 31, 20, 2
 N_READS, READ_LEN = 10_000_000, 150
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def canonical_kmers_device(reads: bytes, n_reads, read_len, k, chunk=2_000_000, device="cuda"):
-    """Sorted distinct canonical k-mers of fixed-length ACGT reads (no k-mer
-    spans two reads), as an int64 tensor on `device`."""
-    import torch
-    lut = torch.full((256,), 255, dtype=torch.uint8)
-    for ch, v in ((b"A", 0), (b"C", 1), (b"G", 2), (b"T", 3)):
-        lut[ch[0]] = v
-    lut = lut.to(device)
-    n = read_len - k + 1
-    parts = []
-    for r0 in range(0, n_reads, chunk):
-        r1 = min(n_reads, r0 + chunk)
-        raw = torch.frombuffer(bytearray(reads[r0 * read_len:r1 * read_len]), dtype=torch.uint8)
-        code = lut[raw.to(device).long()].view(r1 - r0, read_len)
-        assert int(code.max()) <= 3, "reads must be ACGT only"
-        code = code.long()
-        fwd = torch.zeros((r1 - r0, n), dtype=torch.int64, device=device)
-        rev = torch.zeros_like(fwd)
-        for j in range(k):
-            fwd = (fwd << 2) | code[:, j:j + n]
-            rev = rev | ((3 - code[:, j:j + n]) << (2 * j))
-        parts.append(torch.unique(torch.minimum(fwd, rev)))
-        del fwd, rev, code
-    return torch.unique(torch.cat(parts))
-
-
-def test_bench_workload_vertices_are_the_distinct_kmers():
-    import torch
+def test_bench_workload_vertices_are_the_distinct_kmers(tmp_path):
     g = C.synth_genome(50_000_000, 0.0, 300, 6000, 0.0, seed=2026)
     reads = C.synth_reads(g, N_READS, READ_LEN, 0.005, seed=2027)
     rec_off = np.arange(N_READS + 1, dtype=np.uint64) * np.uint64(READ_LEN)
@@ -65,18 +45,17 @@ def test_bench_workload_vertices_are_the_distinct_kmers():
         nv_plain = h.bucket_stats()["n_vertices"].astype(np.int64)
     assert (nv_learned == nv_plain).all()
     assert len(t["g"]) == int(nv_learned.sum())
-    say(f"[case] full-size: {len(t['g'])} vertices; distinct canonical k-mers on the device")
 
-    keys = torch.from_numpy(t["keys"][:, 0].astype(np.int64)).cuda()
-    gs = torch.from_numpy(t["g"].astype(np.int64)).cuda()
-    # (2) no key twice in one subgraph: order by (g, key), no equal neighbours
-    o = torch.sort(keys, stable=True).indices
-    o = o[torch.sort(gs[o], stable=True).indices]
-    kk, gg = keys[o], gs[o]
-    assert not bool(((kk[1:] == kk[:-1]) & (gg[1:] == gg[:-1])).any())
-    # (1) the union of the vertex keys is the set of distinct canonical k-mers
-    got = torch.unique(keys)
-    want = canonical_kmers_device(reads, N_READS, READ_LEN, K)
-    say(f"[case] full-size: union {got.numel()} distinct keys, reads {want.numel()} canonical {K}-mers")
-    assert got.numel() == want.numel()
-    assert torch.equal(got, want)
+    reads_path, vert_path = str(tmp_path / "reads.npy"), str(tmp_path / "vertices.npz")
+    np.save(reads_path, np.frombuffer(reads, np.uint8))
+    np.savez(vert_path, keys=t["keys"][:, 0].astype(np.int64), g=t["g"].astype(np.int32))
+    del t
+    say(f"[case] full-size: {int(nv_learned.sum())} vertices; independent count in a child process")
+    r = subprocess.run([sys.executable, os.path.join(HERE, "fullsize_count.py"), reads_path, vert_path,
+                        str(N_READS), str(READ_LEN), str(K)], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    say(f"[case] full-size: {res}")
+    assert res["duplicate_pairs"] == 0
+    assert res["vertex_keys"] == res["read_kmers"]
+    assert res["equal"]
