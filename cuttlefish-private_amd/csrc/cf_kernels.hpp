@@ -142,9 +142,16 @@ void launch_export(const uint64_t* tbl_ptr, const uint64_t* tbl_cap,
 // Exact record dedup per bucket (dedup.hip): distinct (len, disc bits, label)
 // records with their multiplicities, written to the bucket's own region of
 // the out_* arrays; out_count[b] distinct records.  One work item = one hash
-// pass of one bucket; the LDS table holds dedup_capacity(W) records.
+// pass of one bucket; the LDS table holds dedup_capacity(key words) records.
+// A key is the W label words plus a word for len | disc bits | hash, or, when
+// the label's last word holds at most 15 bases (records are at most max_len =
+// min(255, 2k - l) bases), just the W label words: the last one shifted right
+// by one (bit 63 stays free for the claim flag) with len | disc bits in bits
+// 0-9 and 22 hash bits in bits 10-31.
 constexpr int DD_WG = 1024;
-constexpr int dedup_capacity(int W) { return ((163840 - 2048) / (8 * (W + 1) + 4)) & ~63; }
+constexpr int dedup_capacity(int KR) { return ((163840 - 2048) / (8 * KR + 4)) & ~63; }
+constexpr bool dedup_packed(uint32_t W, uint32_t max_len) { return max_len - 32 * (W - 1) <= 15; }
+constexpr int dedup_key_words(uint32_t W, uint32_t max_len) { return int(W) + (dedup_packed(W, max_len) ? 0 : 1); }
 struct DedupArgs {
     const uint32_t* att;
     const uint64_t* labels;
@@ -162,7 +169,7 @@ struct DedupArgs {
     uint32_t n_items;
     uint32_t* queue_head;
 };
-void launch_dedup(const DedupArgs& a, uint32_t W, uint32_t grid, hipStream_t s);
+void launch_dedup(const DedupArgs& a, uint32_t W, bool packed, uint32_t grid, hipStream_t s);
 // Buckets in list[0..n): records copied unchanged with multiplicity 1.
 void launch_dedup_copy(const DedupArgs& a, const uint32_t* list, uint32_t n, uint32_t W, hipStream_t s);
 void launch_dedup_end(const uint64_t* off, const uint32_t* cnt, uint32_t G, uint64_t* end, hipStream_t s);

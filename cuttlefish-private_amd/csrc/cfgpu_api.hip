@@ -595,7 +595,9 @@ constexpr double DEDUP_MIN_GAIN = 1.33;
 constexpr size_t DEDUP_PROBES = 64;
 
 int dedup_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets, double distinct_frac) {
-    const double C = double(dedup_capacity(int(h->W))) * DEDUP_LOAD;
+    const uint32_t max_len = std::min<uint32_t>(255, 2 * h->prm.k - h->prm.l);  // record length bound
+    const bool packed = dedup_packed(h->W, max_len);
+    const double C = double(dedup_capacity(dedup_key_words(h->W, max_len))) * DEDUP_LOAD;
     std::vector<uint32_t> ib, ip, iP;
     for (uint32_t b : buckets) {
         const double need = std::max(1.0, double(bucket_records(h, b)) * distinct_frac);
@@ -626,7 +628,7 @@ int dedup_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets, double di
     a.item_passes = h->dd_iP.as<uint32_t>();
     a.n_items = uint32_t(ib.size());
     a.queue_head = h->d_qhead.as<uint32_t>();
-    launch_dedup(a, h->W, std::min<uint32_t>(a.n_items, uint32_t(h->cus)), h->stream);
+    launch_dedup(a, h->W, packed, std::min<uint32_t>(a.n_items, uint32_t(h->cus)), h->stream);
     HIPCK(h, hipGetLastError());
     return CFGPU_OK;
 }

@@ -77,10 +77,10 @@ __device__ __forceinline__ int dd_find_or_insert(uint64_t* s_key, const uint64_t
     return -1;
 }
 
-template <int W>
+template <int W, bool PK>
 __global__ __launch_bounds__(DD_WG) void k_dedup(DedupArgs a) {
-    constexpr int KR = W + 1;
-    constexpr int C = dedup_capacity(W);
+    constexpr int KR = PK ? W : W + 1;  // PK: len, disc bits and hash packed into the last label word
+    constexpr int C = dedup_capacity(KR);
     static_assert(C % 64 == 0, "capacity");
     __shared__ uint64_t s_key[C * KR];
     __shared__ uint32_t s_cnt[C];
@@ -133,19 +133,32 @@ __global__ __launch_bounds__(DD_WG) void k_dedup(DedupArgs a) {
                 if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
             }
             const uint32_t len = at & 0xFFu;
-            uint64_t key[KR];
+            uint64_t m[W];
 #pragma unroll
             for (int q = 0; q < W; ++q) {  // word W-1-i holds bases 32i .. 32i+31
                 const int i = W - 1 - q;
                 const int nb = int(len) - 32 * i;
                 const uint64_t x = lab[q];
-                key[q] = nb >= 32 ? x : (nb <= 0 ? 0ull : x & (~0ull << (64 - 2 * nb)));
+                m[q] = nb >= 32 ? x : (nb <= 0 ? 0ull : x & (~0ull << (64 - 2 * nb)));
             }
-            key[KR - 1] = at & 0x3FFu;
-            const uint32_t h = pass_hash32<KR>(key);
+            // key words: PK, label words 1..W-1 then word 0 (the record's last
+            // bases, low bits free) packed as the top word; otherwise the W
+            // label words then a top word of their own
+            uint64_t key[KR];
+#pragma unroll
+            for (int q = 0; q < W; ++q) key[PK ? (q + W - 1) % W : q] = m[q];
             // The top word also carries h (a function of the rest of the key),
             // so that it alone tells almost every pair of distinct keys apart.
-            key[KR - 1] |= uint64_t(h) << 10;
+            uint32_t h;
+            if constexpr (PK) {
+                key[KR - 1] = (key[KR - 1] >> 1) | (at & 0x3FFu);  // bases in bits 33..62
+                h = pass_hash32<KR>(key);
+                key[KR - 1] |= uint64_t(h & ~0x3FFu);
+            } else {
+                key[KR - 1] = at & 0x3FFu;
+                h = pass_hash32<KR>(key);
+                key[KR - 1] |= uint64_t(h) << 10;
+            }
             const uint64_t hp = uint64_t(h) * P;  // pass = top, group from the fraction
             if (uint32_t(hp >> 32) != p) continue;
             bool inserted;
@@ -208,7 +221,8 @@ __global__ __launch_bounds__(DD_WG) void k_dedup(DedupArgs a) {
             a.out_att[o] = uint32_t(kp[KR - 1] & 0x3FFu) | g16;
             a.out_mult[o] = c;
 #pragma unroll
-            for (int q = 0; q < W; ++q) a.out_labels[o * W + q] = kp[q];
+            for (int q = 0; q < W; ++q)
+                a.out_labels[o * W + q] = !PK ? kp[q] : q == 0 ? (kp[KR - 1] >> 32) << 33 : kp[q - 1];
         }
         if (my_km) atomicAdd(&s_km, my_km);
         __syncthreads();
@@ -248,18 +262,18 @@ __global__ __launch_bounds__(256) void k_dedup_end(const uint64_t* __restrict__ 
 
 }  // namespace
 
-void launch_dedup(const DedupArgs& a, uint32_t W, uint32_t grid, hipStream_t s) {
+void launch_dedup(const DedupArgs& a, uint32_t W, bool packed, uint32_t grid, hipStream_t s) {
     if (!grid) return;
+#define CF_DD(w)                                                                                        \
+    case w:                                                                                             \
+        if (packed) hipLaunchKernelGGL((k_dedup<w, true>), dim3(grid), dim3(DD_WG), 0, s, a);           \
+        else hipLaunchKernelGGL((k_dedup<w, false>), dim3(grid), dim3(DD_WG), 0, s, a);                 \
+        break;
     switch (W) {
-        case 1: hipLaunchKernelGGL(k_dedup<1>, dim3(grid), dim3(DD_WG), 0, s, a); break;
-        case 2: hipLaunchKernelGGL(k_dedup<2>, dim3(grid), dim3(DD_WG), 0, s, a); break;
-        case 3: hipLaunchKernelGGL(k_dedup<3>, dim3(grid), dim3(DD_WG), 0, s, a); break;
-        case 4: hipLaunchKernelGGL(k_dedup<4>, dim3(grid), dim3(DD_WG), 0, s, a); break;
-        case 5: hipLaunchKernelGGL(k_dedup<5>, dim3(grid), dim3(DD_WG), 0, s, a); break;
-        case 6: hipLaunchKernelGGL(k_dedup<6>, dim3(grid), dim3(DD_WG), 0, s, a); break;
-        case 7: hipLaunchKernelGGL(k_dedup<7>, dim3(grid), dim3(DD_WG), 0, s, a); break;
-        default: hipLaunchKernelGGL(k_dedup<8>, dim3(grid), dim3(DD_WG), 0, s, a); break;
+        CF_DD(1) CF_DD(2) CF_DD(3) CF_DD(4) CF_DD(5) CF_DD(6) CF_DD(7)
+        default: CF_DD(8)
     }
+#undef CF_DD
 }
 
 void launch_dedup_copy(const DedupArgs& a, const uint32_t* list, uint32_t n, uint32_t W, hipStream_t s) {

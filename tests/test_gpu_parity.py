@@ -211,8 +211,10 @@ def test_lds_overflow_fallback_is_exact():
     # Large Zipf buckets with under-estimated distinct counts get too few LDS
     # passes; the full pass table is detected and the bucket rebuilt on the
     # global path.
+    # (120,000 records: the second and third buckets, outside the dedup probe,
+    # hold more distinct records than a dedup pass table takes)
     k, l = 31, 20
-    att, lab, off = C.synth_bucket_workload(256, 60000, 1.1, k, l, seed=7)
+    att, lab, off = C.synth_bucket_workload(256, 120000, 1.1, k, l, seed=7)
     want = O.construct(k, l, off, (att & 0xFFFF).astype(np.uint16), lab)
     say("[case] lds-overflow")
     with C.Handle(k, l, 1) as h:
