@@ -10,7 +10,10 @@
 //
 // One work item = one hash pass of one bucket: records whose key hash selects
 // the pass are counted in an LDS table (key = the label words with bases past
-// `len` cleared, then len | l_disc | r_disc | hash << 10 as the top word), and the distinct
+// `len` cleared, then len | l_disc | r_disc | hash << 10 as the top word; or,
+// when the label's last word holds at most 15 bases, that word shifted right
+// by one with len | disc bits | 22 hash bits in its free low bits as the top
+// word, one word fewer), and the distinct
 // records are appended to the bucket's region of the output arrays.  A pass
 // whose table fills past 7/8 flags the bucket; the host then copies its
 // records unchanged (multiplicity 1), so results never depend on the sizing.
