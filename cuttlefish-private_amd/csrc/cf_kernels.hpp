@@ -96,9 +96,9 @@ struct ConstructArgs {
     uint32_t* queue_head;       // persistent mode: next item
     const uint32_t* mult;       // per record multiplicity (null: every record counts once)
     const uint64_t* bucket_end; // per bucket end of its records (null: bucket_off[b + 1])
-    uint64_t* spill_key;        // LDS kernel, all-pass items: per workgroup 2 lists of
-    uint32_t* spill_meta;       //   spill_cap entries (KW key words + one meta word each)
-    uint32_t spill_cap;
+    uint64_t* spill_key;        // LDS kernel, all-pass items: per workgroup a region of
+    uint32_t* spill_meta;       //   spill_cap entries (KW key words + one meta word each),
+    uint32_t spill_cap;         //   P sub-lists of spill_cap / P entries for a P-pass item
 };
 
 // Entry stride in bytes for KW key words: KW u64 key words, u32 e_f, u32 status.
@@ -114,14 +114,16 @@ void launch_construct_persistent(const ConstructArgs& a, uint32_t KW, uint32_t g
 // LDS-resident construct: one work item = one hash pass of one bucket
 // (item_first = pass index, item_count = number of passes P; the bucket's
 // table holds P * lds_capacity(KW) slots), or every pass of one bucket
-// (item_first = LDS_ALL_PASSES): the records are walked once and the
-// instances of later passes go through the workgroup's spill lists, which
-// must hold the bucket's k-mer instances.  Persistent over queue_head; grid
-// at most the number of workgroups the spill buffers are sized for.
+// (item_first = LDS_ALL_PASSES, 2 <= P <= LDS_ALL_PASS_MAX): the records are
+// walked once, every instance going to its pass's sub-list of the
+// workgroup's spill region, then each pass probes its own sub-list.
+// Persistent over queue_head; grid at most the number of workgroups the
+// spill regions are sized for.
 // The LDS holds only the pass table: 8*KW key bytes + 4 e_f bytes + 1 status
 // byte per slot, sized to fill the 160 KiB of a gfx950 CU.
 constexpr int LDS_WG = 1024;
 constexpr uint32_t LDS_ALL_PASSES = 0xFFFFFFFFu;
+constexpr uint32_t LDS_ALL_PASS_MAX = 64;  // sub-list counters kept in LDS beside the table
 constexpr int lds_capacity(int KW) { return KW == 1 ? 12544 : KW == 2 ? 7744 : KW == 3 ? 5632 : 4416; }
 void launch_construct_lds(const ConstructArgs& a, uint32_t KW, uint32_t grid, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);

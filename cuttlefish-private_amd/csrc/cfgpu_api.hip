@@ -87,7 +87,7 @@ struct cfgpu_handle {
     int cus = 256;
     hipStream_t stream = nullptr;
     std::string err;
-    cfgpu_tuning tune{0.6f, 256, 2048, 8, 1.15f, 1, 32, 0, 1};
+    cfgpu_tuning tune{0.6f, 256, 2048, 8, 1.15f, 1, LDS_ALL_PASS_MAX, 0, 1};
 
     // partition state
     DevBuf packed, frag, gbin, mbn, tile_lc, tile_frag, boff, att, labels, mbin, stats, feat;
@@ -102,8 +102,8 @@ struct cfgpu_handle {
         d_qhead, d_round_list;
     std::vector<std::unique_ptr<DevBuf>> pools;  // one per sizing round, reused across calls
     uint32_t round_idx = 0;
-    DevBuf spill_key, spill_meta;  // LDS construct: per-workgroup spill lists of all-pass items
-    uint64_t spill_need = 0;       // largest k-mer instance count of an all-pass item of the round
+    DevBuf spill_key, spill_meta;  // LDS construct: per-workgroup spill regions of all-pass items
+    uint64_t spill_need = 0;       // region entries the largest all-pass item of the round needs
     uint32_t n_all_pass = 0;       // all-pass items of the round
     uint8_t* stage = nullptr;      // pinned staging of small copies (h2d, d2h); reused after sync_stream
     size_t stage_cap = 0, stage_off = 0;
@@ -135,6 +135,7 @@ struct cfgpu_handle {
     // contract: per-subgraph counters, unitig records and labels
     DevBuf ct_list, ct_ovf, ct_counts, ct_ctl, ct_scratch, ct_recs, ct_labels;
     uint64_t n_unitigs = 0, n_ubytes = 0;
+    uint64_t ct_need[2] = {0, 0};  // records and label bytes the last contract needed
     bool contracted = false;
 
     // compacted graph: glue table, list-ranking states, output; cycles through
@@ -282,15 +283,23 @@ uint64_t walked_kmers(const cfgpu_handle* h, uint32_t b) {
     return h->dd_active ? h->h_ddkm[b] : bucket_kmers(h, b);
 }
 
-// An LDS bucket of 2 <= P <= ALL_PASS_MAX passes is one all-pass item when
-// its instances fit a spill list of at most SPILL_BYTES_MAX over all
-// workgroups' two lists; otherwise one item per pass, each walking the
-// records (more passes serialised in one workgroup would outlast the rest of
-// the round).
-constexpr uint64_t SPILL_BYTES_MAX = 2ull << 30;
-constexpr uint32_t ALL_PASS_MAX = 8;
+// An LDS bucket of 2 <= P <= LDS_ALL_PASS_MAX passes is one all-pass item
+// (records walked once, every instance spilled to its pass's sub-list) when
+// its sub-lists fit a per-workgroup region of at most SPILL_BYTES_MAX over all
+// workgroups; otherwise one item per pass, each walking the records.
+constexpr uint64_t SPILL_BYTES_MAX = 16ull << 30;
 uint64_t spill_cap_max(const cfgpu_handle* h) {
-    return SPILL_BYTES_MAX / (uint64_t(std::max(h->cus, 1)) * 2 * (8 * h->KW + 4));
+    return SPILL_BYTES_MAX / (uint64_t(std::max(h->cus, 1)) * (8 * h->KW + 4));
+}
+// Region entries an all-pass item of `inst` walked instances over P passes
+// needs: P sub-lists with 50 % headroom over the mean plus 8 standard
+// deviations of a binomial count (instances of one k-mer share a pass, so
+// counts spread more than independent draws; a sub-list that still fills
+// flags the bucket, which is then rebuilt exactly).
+uint64_t spill_need_for(uint64_t inst, uint32_t P) {
+    const double mean = double(inst) / double(P);
+    const uint64_t c = uint64_t(std::ceil(1.5 * mean + 8.0 * std::sqrt(mean) + 64.0));
+    return uint64_t(P) * c;
 }
 
 // Capacity that holds every k-mer instance of the bucket at the target load:
@@ -330,10 +339,10 @@ int prepare_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
         off += caps[i];
         if (lds_cap) {
             const uint32_t P = uint32_t(caps[i] / lds_cap);
-            const uint64_t inst = walked_kmers(h, b);
-            if (P >= 2 && P <= ALL_PASS_MAX && inst <= spill_max) {  // one item: every pass, records walked once
+            const uint64_t need = spill_need_for(walked_kmers(h, b), std::max(P, 1u));
+            if (P >= 2 && P <= LDS_ALL_PASS_MAX && need <= spill_max) {  // one item: every pass, records walked once
                 ib.push_back(b), ifirst.push_back(LDS_ALL_PASSES), icnt.push_back(P);
-                h->spill_need = std::max(h->spill_need, inst);
+                h->spill_need = std::max(h->spill_need, need);
                 ++h->n_all_pass;
                 continue;
             }
@@ -442,10 +451,10 @@ int run_round_batched(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
     a.n_items = n_items;
     const uint32_t per_cu = lds ? 1u : h->tune.wgs_per_cu;  // the LDS kernel fills a CU's LDS
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(n_items, uint32_t(h->cus) * per_cu));
-    if (lds && h->spill_need) {  // two spill lists per workgroup, each for the largest all-pass bucket
+    if (lds && h->spill_need) {  // one region per workgroup, sized for the largest all-pass item
         const uint64_t cap = (h->spill_need + 63) & ~63ull;
-        HIPCK(h, h->spill_key.ensure(uint64_t(grid) * 2 * cap * h->KW * 8));
-        HIPCK(h, h->spill_meta.ensure(uint64_t(grid) * 2 * cap * 4));
+        HIPCK(h, h->spill_key.ensure(uint64_t(grid) * cap * h->KW * 8));
+        HIPCK(h, h->spill_meta.ensure(uint64_t(grid) * cap * 4));
         a.spill_key = h->spill_key.as<uint64_t>();
         a.spill_meta = h->spill_meta.as<uint32_t>();
         a.spill_cap = uint32_t(cap);
@@ -1228,8 +1237,11 @@ int cfgpu_contract(cfgpu_handle* h) {
     HIPCK(h, h->ct_ctl.ensure(64));
     HIPCK(h, h->ct_scratch.ensure(per_wg * std::max<uint32_t>(grid, 1)));
     if (n_list) HIPCK(h, h2d(h, h->ct_list.p, list.data(), n_list * 4ull));
-    // output sized for short unitigs; re-run once at the exact size when short
-    uint64_t rec_cap = std::max<uint64_t>(4096, sum_nv / 2), lab_cap = std::max<uint64_t>(1 << 20, sum_nv * 8);
+    // Output sized from the vertex count (most vertices of read sets are
+    // isolated) or the last call's exact need, whichever is larger; re-run
+    // once at the exact size when short.
+    uint64_t rec_cap = std::max<uint64_t>({4096, sum_nv / 16, h->ct_need[0]}),
+             lab_cap = std::max<uint64_t>({1 << 20, sum_nv * 2, h->ct_need[1]});
     Event e0, e1;
     HIPCK(h, hipEventRecord(e0.e, h->stream));
     uint32_t ctl[16];
@@ -1269,6 +1281,8 @@ int cfgpu_contract(cfgpu_handle* h) {
         if (ctl[4]) return fail(h, CFGPU_E_STATE, "contract: inconsistent subgraph table");
         uint64_t cur[2];
         std::memcpy(cur, ctl + 8, sizeof cur);
+        h->ct_need[0] = cur[0];
+        h->ct_need[1] = cur[1];
         if (!ctl[3]) {
             h->n_unitigs = cur[0];
             h->n_ubytes = cur[1];

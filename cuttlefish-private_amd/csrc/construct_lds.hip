@@ -8,11 +8,12 @@
 // A bucket whose distinct k-mers do not fit one LDS table is split by hash into
 // P passes: pass p keeps the k-mers whose 32-bit pass hash h has
 // (h * P) >> 32 == p.  Pass p's LDS table becomes slots [p*C, (p+1)*C) of
-// the bucket's HBM table.  A bucket of 2..8 passes is one work item: its
-// records are walked once, every instance going to a per-workgroup spill list
-// (pass 0's or the later passes'), and each pass probes its list with every
-// lane busy.  Other buckets are one item per pass (P = 1, or more passes than
-// an item should serialise), each walking the records.
+// the bucket's HBM table.  A bucket of 2..LDS_ALL_PASS_MAX passes is one work
+// item: its records are walked once, every instance going to its pass's
+// sub-list of a per-workgroup spill region, and each pass probes its own
+// sub-list with every lane busy (each instance is written and read once,
+// whatever P is).  Other buckets are one item per pass (P = 1, or buckets too
+// large for the spill regions), each walking the records.
 //
 // Each lane walks a stream of whole records (RecordWalker: the k-mer and its
 // reverse complement are rolled one base per instance), so no staging
@@ -138,73 +139,52 @@ __device__ __forceinline__ bool lds_add(uint64_t* s_key, uint32_t* s_ef, uint32_
     return true;
 }
 
-// Spill list of a workgroup: KW key-word arrays then one meta array, `cap`
-// entries each.  meta = front | back << 3 | disc << 6 | min(m, 15) << 9
-// (counters saturate at 15, so adding min(m, 15) equals adding m).
-struct SpillList {
+// Spill region of a workgroup for one all-pass item: P sub-lists of c =
+// spill_cap / P entries, sub-list q holding the instances of pass q.  Key
+// word w of entry e is at key[w * spill_cap + e]; meta[e] = front | back << 3
+// | disc << 6 | min(m, 15) << 9 (counters saturate at 15, so adding min(m, 15)
+// equals adding m).
+struct SpillRegion {
     uint64_t* key;
     uint32_t* meta;
 };
 
 template <int KW>
-__device__ __forceinline__ SpillList spill_list(const ConstructArgs& a, uint32_t which) {
+__device__ __forceinline__ SpillRegion spill_region(const ConstructArgs& a) {
     const uint64_t cap = a.spill_cap;
-    const uint64_t region = uint64_t(blockIdx.x) * 2 + which;
-    return {a.spill_key + region * KW * cap, a.spill_meta + region * cap};
+    return {a.spill_key + uint64_t(blockIdx.x) * KW * cap, a.spill_meta + uint64_t(blockIdx.x) * cap};
 }
 
-// Append the instances of the lanes with `push` set to a spill list: one LDS
-// atomic per wave, consecutive entries per wave.  A full list sets *ovf (the
-// bucket is then rebuilt on the global path).
+// Append the calling lanes' instances to their passes' sub-lists.  The lanes
+// of a wave are split by pass with nbits ballots (one per bit of q): each
+// group's first lane reserves the group's entries with one LDS atomic, and
+// the lanes take consecutive entries in lane order.  A full sub-list sets
+// *ovf (the bucket is then rebuilt on the global path).
 template <int KW>
-__device__ __forceinline__ void spill_push(bool push, const Kmer<KW>& key, uint32_t meta, const SpillList& L,
-                                           uint32_t cap, uint32_t* s_n, uint32_t* s_ovf) {
-    const unsigned long long mask = __ballot(push);
-    if (!mask) return;
+__device__ __forceinline__ void spill_push_pass(uint32_t q, uint32_t nbits, const Kmer<KW>& key, uint32_t meta,
+                                                const SpillRegion& R, uint32_t cap, uint32_t c, uint32_t* s_n,
+                                                uint32_t* s_ovf) {
+    unsigned long long same = __ballot(true);
+    for (uint32_t bit = 0; bit < nbits; ++bit) {
+        const bool set = (q >> bit) & 1u;
+        const unsigned long long m = __ballot(set);
+        same &= set ? m : ~m;
+    }
     const int lane = int(__lane_id());
-    const int leader = __ffsll((long long)mask) - 1;  // wave-uniform
+    const int leader = __ffsll((long long)same) - 1;
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(same >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(same), 0u));
     uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(s_n, uint32_t(__popcll(mask)));
-    base = __builtin_amdgcn_readlane(base, leader);
-    if (!push) return;
-    const uint32_t idx = base + __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
-    if (idx >= cap) {
+    if (lane == leader) base = atomicAdd(&s_n[q], uint32_t(__popcll(same)));
+    base = __builtin_amdgcn_ds_bpermute(leader << 2, int(base));
+    const uint32_t idx = base + rank;
+    if (idx >= c) {
         *s_ovf = 1;
         return;
     }
+    const uint64_t e = uint64_t(q) * c + idx;
 #pragma unroll
-    for (int w = 0; w < KW; ++w) L.key[uint64_t(w) * cap + idx] = key.w[w];
-    L.meta[idx] = meta;
-}
-
-// Append every active lane's instance to list 1 (sel) or list 0 (!sel): one
-// 64-bit LDS atomic per wave reserves the wave's entries in both lists (the
-// two u32 counters s_n[0..1] are one aligned u64).  A full list sets *ovf.
-template <int KW>
-__device__ __forceinline__ void spill_push_split(bool sel, const Kmer<KW>& key, uint32_t meta, const SpillList& L0,
-                                                 const SpillList& L1, uint32_t cap, uint32_t* s_n,
-                                                 uint32_t* s_ovf) {
-    const unsigned long long m1 = __ballot(sel), m0 = __ballot(!sel);
-    const int lane = int(__lane_id());
-    const int leader = __ffsll((long long)(m0 | m1)) - 1;  // wave-uniform
-    unsigned long long b = 0;
-    if (lane == leader)
-        b = atomicAdd(reinterpret_cast<unsigned long long*>(s_n),
-                      (unsigned long long)__popcll(m0) | ((unsigned long long)__popcll(m1) << 32));
-    const uint32_t b0 = __builtin_amdgcn_readlane(uint32_t(b), leader);
-    const uint32_t b1 = __builtin_amdgcn_readlane(uint32_t(b >> 32), leader);
-    const unsigned long long m = sel ? m1 : m0;
-    const uint32_t idx = (sel ? b1 : b0) +
-                         __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
-    if (idx >= cap) {
-        *s_ovf = 1;
-        return;
-    }
-    uint64_t* kd = sel ? L1.key : L0.key;
-    uint32_t* md = sel ? L1.meta : L0.meta;
-#pragma unroll
-    for (int w = 0; w < KW; ++w) kd[uint64_t(w) * cap + idx] = key.w[w];
-    md[idx] = meta;
+    for (int w = 0; w < KW; ++w) R.key[uint64_t(w) * cap + e] = key.w[w];
+    R.meta[e] = meta;
 }
 
 template <int KW, int WMAX>
@@ -215,7 +195,7 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
     __shared__ uint32_t s_ef[C];
     __shared__ uint32_t s_st[C / 4];  // one Vertex_Status byte per slot
     __shared__ uint32_t s_item, s_cnt, s_ovf;
-    __shared__ __attribute__((aligned(8))) uint32_t s_sn[2];  // spill list lengths (one u64 for the walk's pushes)
+    __shared__ uint32_t s_sn[LDS_ALL_PASS_MAX];  // sub-list lengths of an all-pass item
 
     const int tid = threadIdx.x;
     const uint32_t W = a.W, k = a.k;
@@ -236,25 +216,25 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
         const uint32_t P = a.item_count[item];  // passes of this bucket
         // item_first: the pass of this item, or LDS_ALL_PASSES: every pass
         // here.  The records are then walked once, every instance going to
-        // spill list 0 (pass 0) or 1 (later passes), and each pass probes its
-        // list with every lane busy, passing on what belongs to later passes.
+        // its pass's sub-list, and each pass probes its own sub-list with
+        // every lane busy.
         const bool all = a.item_first[item] == LDS_ALL_PASSES;
         const uint32_t p_lo = all ? 0u : a.item_first[item], p_hi = all ? P - 1 : p_lo;
+        const uint32_t c_sub = all ? a.spill_cap / P : 0u;  // entries per sub-list
+        const SpillRegion R = spill_region<KW>(a);
         if (tid == 0) s_ovf = __hip_atomic_load(&a.overflow[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (all)
+            for (uint32_t i = tid; i < P; i += LWG) s_sn[i] = 0;
 
         for (uint32_t p = p_lo; p <= p_hi; ++p) {
-            for (int i = tid; i < C * KW; i += LWG) s_key[i] = EMPTY;
-            for (int i = tid; i < C; i += LWG) s_ef[i] = 0u;
-            for (int i = tid; i < C / 4; i += LWG) s_st[i] = 0u;
-            if (tid == 0) {
-                s_cnt = 0;
-                if (p == p_lo) s_sn[0] = s_sn[1] = 0;  // the walk fills both lists
-                else s_sn[(p + 1) & 1u] = 0;          // the list this pass writes
+            if (!(all && p == p_lo)) {  // the all-pass walk needs no table
+                for (int i = tid; i < C * KW; i += LWG) s_key[i] = EMPTY;
+                for (int i = tid; i < C; i += LWG) s_ef[i] = 0u;
+                for (int i = tid; i < C / 4; i += LWG) s_st[i] = 0u;
             }
+            if (tid == 0) s_cnt = 0;
             __syncthreads();
             if (s_ovf) break;  // bucket already flagged
-            const bool spill = all && p < P - 1;
-            const SpillList out = spill_list<KW>(a, (p + 1) & 1u);
             uint32_t my_ins = 0;  // keys this lane inserted since its last flush into s_cnt
             const auto flush = [&]() {
                 if (my_ins) {
@@ -275,6 +255,7 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                 const auto rec_of = [&](uint32_t j) {  // strictly increasing in j
                     return r0 + uint64_t(j) * LWG + ((j & 1u) ? uint32_t(LWG - 1 - tid) : uint32_t(tid));
                 };
+                const uint32_t nbits = all ? 32u - uint32_t(__builtin_clz((P - 1) | 1u)) : 0u;  // bits of q < P
                 uint32_t j = 0;
                 uint64_t r = rec_of(0), rn = rec_of(1);
                 Walker rw;
@@ -294,9 +275,9 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                     const uint64_t hp = uint64_t(pass_hash<KW>(in.key)) * P;
                     const uint32_t q = uint32_t(hp >> 32);
                     if (all) {
-                        spill_push_split<KW>(q != 0, in.key,
-                                             in.front | (in.back << 3) | (in.disc << 6) | (min(m, 15u) << 9),
-                                             spill_list<KW>(a, 0), spill_list<KW>(a, 1), a.spill_cap, s_sn, &s_ovf);
+                        spill_push_pass<KW>(q, nbits, in.key,
+                                            in.front | (in.back << 3) | (in.disc << 6) | (min(m, 15u) << 9), R,
+                                            a.spill_cap, c_sub, s_sn, &s_ovf);
                     } else if (q == p) {
                         const uint32_t g0 = uint32_t((uint64_t(uint32_t(hp)) * NG) >> 32);
                         if (!lds_add<KW, C>(s_key, s_ef, s_st, in.key, g0, in.front, in.back, in.disc, m, my_ins))
@@ -316,40 +297,47 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                         }
                     }
                 }
+                if (all) {
+                    // The walk filled every sub-list: pass p_lo starts over
+                    // with a clear table (the next iteration of a loop over
+                    // the same p would need one more flag; the table is
+                    // cleared here instead).
+                    __syncthreads();
+                    if (s_ovf) break;
+                    for (int i = tid; i < C * KW; i += LWG) s_key[i] = EMPTY;
+                    for (int i = tid; i < C; i += LWG) s_ef[i] = 0u;
+                    for (int i = tid; i < C / 4; i += LWG) s_st[i] = 0u;
+                    __syncthreads();
+                }
             }
-            if (p != p_lo || all) {
-                if (p == p_lo) __syncthreads();  // the walk's lists are complete
-                // A pass over the list the walk or the previous pass left, one
-                // entry per lane, in list order.
-                const SpillList in = spill_list<KW>(a, p & 1u);
-                // (a list that overflowed has flagged the bucket: nothing past
-                // its capacity is read)
-                const uint32_t n_in = min(s_sn[p & 1u], a.spill_cap);
-                uint32_t seen = 0;
-                Kmer<KW> key_n;  // the lane's next entry, loaded while this one is probed
+            if (all) {
+                // Pass p over its own sub-list, one entry per lane, the
+                // lane's next entry loaded while this one is probed (a
+                // sub-list that overflowed has flagged the bucket: nothing
+                // past its capacity is read).
+                const uint32_t n_in = min(s_sn[p], c_sub);
+                const uint64_t e0 = uint64_t(p) * c_sub;
+                Kmer<KW> key_n;
                 uint32_t meta_n = 0;
                 if (uint32_t(tid) < n_in) {
 #pragma unroll
-                    for (int w = 0; w < KW; ++w) key_n.w[w] = in.key[uint64_t(w) * a.spill_cap + tid];
-                    meta_n = in.meta[tid];
+                    for (int w = 0; w < KW; ++w) key_n.w[w] = R.key[uint64_t(w) * a.spill_cap + e0 + tid];
+                    meta_n = R.meta[e0 + tid];
                 }
+                uint32_t seen = 0;
                 for (uint32_t e = tid; e < n_in; e += LWG) {
                     const Kmer<KW> key = key_n;
                     const uint32_t meta = meta_n;
                     if (e + LWG < n_in) {
 #pragma unroll
-                        for (int w = 0; w < KW; ++w) key_n.w[w] = in.key[uint64_t(w) * a.spill_cap + e + LWG];
-                        meta_n = in.meta[e + LWG];
+                        for (int w = 0; w < KW; ++w) key_n.w[w] = R.key[uint64_t(w) * a.spill_cap + e0 + e + LWG];
+                        meta_n = R.meta[e0 + e + LWG];
                     }
                     const uint64_t hp = uint64_t(pass_hash<KW>(key)) * P;
-                    const uint32_t q = uint32_t(hp >> 32);
-                    if (q == p) {
-                        const uint32_t g0 = uint32_t((uint64_t(uint32_t(hp)) * NG) >> 32);
-                        if (!lds_add<KW, C>(s_key, s_ef, s_st, key, g0, meta & 7u, (meta >> 3) & 7u, (meta >> 6) & 7u,
-                                            meta >> 9, my_ins))
-                            s_ovf = 1;
-                    }
-                    if (spill) spill_push<KW>(q > p, key, meta, out, a.spill_cap, &s_sn[(p + 1) & 1u], &s_ovf);
+                    const uint32_t g0 = uint32_t((uint64_t(uint32_t(hp)) * NG) >> 32);
+                    if (!lds_add<KW, C>(s_key, s_ef, s_st, key, g0, meta & 7u, (meta >> 3) & 7u, (meta >> 6) & 7u,
+                                        meta >> 9, my_ins))
+                        s_ovf = 1;
                     if ((++seen & 7u) == 0) {
                         flush();
                         if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
@@ -375,6 +363,7 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
             if (tid == 0) atomicAdd(&a.n_vert[b], s_cnt);
             __syncthreads();  // table read out and counters used before the next pass clears them
         }
+        if (s_ovf && tid == 0) atomicOr(&a.overflow[b], 1u);
     }
 }
 

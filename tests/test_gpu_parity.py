@@ -310,8 +310,37 @@ def test_zipf_bucket_workload_loaded_records():
                         assert rs["n_lds_buckets"] == 0
                     if mp == 32 and mode == "batched":
                         assert rs["n_lds_buckets"] > 0
-                        # buckets of 2..8 passes walk their records once (spill lists)
+                        # multi-pass buckets walk their records once (per-pass sub-lists)
                         assert rs["n_lds_all_pass"] > 0
+
+
+LDS_CAP = {1: 12544, 2: 7744, 3: 5632, 4: 4416}  # LDS pass-table slots per key-word count
+
+
+@pytest.mark.parametrize("k", [31, 63, 127])
+def test_many_pass_buckets_walk_once(k):
+    # Buckets of 9..64 LDS passes (the read sets of deep coverage give 20-30 at
+    # k = 63): one item walks the records once and spills every instance to
+    # its pass's sub-list.  Random labels make nearly every instance a
+    # distinct vertex, so the passes are as many as the instances need.
+    l = 20
+    att, lab, off = C.synth_bucket_workload(16, {31: 150000, 63: 60000, 127: 30000}[k], 1.1, k, l, seed=300 + k)
+    want = O.construct(k, l, off, (att & 0xFFFF).astype(np.uint16), lab)
+    cap = LDS_CAP[(k + 31) // 32] * 0.8
+    with C.Handle(k, l, 1) as h:
+        h.load_buckets(att, lab, off)
+        km = h.bucket_stats()["n_kmers"].astype(np.float64)
+        passes = np.ceil(km / cap)
+        say(f"[case] many-pass k={k} passes={sorted(passes[passes > 0].astype(int).tolist())}")
+        assert ((passes > 8) & (passes <= 64)).sum() >= 2
+        for mode in ("batched", "learned"):
+            say(f"[case] many-pass k={k} mode={mode}")
+            h.construct(mode)
+            check_tables(h, None, k, l, 1, want)
+            rs = h.run_stats()
+            assert rs["n_overflowed"] == 0
+            if mode == "batched":
+                assert rs["n_lds_all_pass"] >= int(((passes >= 2) & (passes <= 64)).sum())
 
 
 def test_load_buckets_rejects_malformed_records():
