@@ -186,6 +186,12 @@ constexpr int CT_WG = 1024;
 // LDS table slots: 8*KW key bytes + 4 state bytes + 2 x 4 list-ranking bytes each
 // (a slot's two directed states are numbered below 2^14).
 constexpr int ct_lds_capacity(int KW) { return KW == 1 ? 7360 : KW == 2 ? 5248 : KW == 3 ? 4096 : 3328; }
+// Dense tables (non-isolated vertices beyond an LDS table): at most this many
+// vertices, and per workgroup this many scratch bytes.
+__host__ __device__ constexpr uint32_t ct_big_max_vertices() { return 36864 * 4 / 5; }
+__host__ __device__ constexpr uint64_t ct_big_scratch_bytes(uint32_t KW) {
+    return uint64_t(ct_big_max_vertices()) * (8 * KW + 4 + 16 + 16);
+}
 // Global scratch table slots for a subgraph of nv vertices (load <= 0.7).
 __host__ __device__ constexpr uint64_t ct_global_capacity(uint64_t nv) { return ((nv * 10 / 7 + 64) + 63) & ~63ull; }
 struct UnitigRec {             // layout of cfgpu_unitig
@@ -202,10 +208,13 @@ struct ContractArgs {
     const uint32_t* list;          // buckets for the LDS pass
     uint32_t n_list;
     uint32_t k, cutoff;
-    uint32_t* queue_head;          // [0] LDS pass, [1] global pass
+    uint32_t* queue_head;          // [0] LDS pass, [1] dense pass
     uint32_t* ovf_list;            // buckets the LDS pass could not hold
     uint32_t* ovf_count;
-    uint8_t* scratch;              // global pass: scratch_per_wg bytes per workgroup
+    uint32_t* walk_head;           // global pass
+    uint32_t* ovf2_list;           // buckets the dense pass could not hold
+    uint32_t* ovf2_count;
+    uint8_t* scratch;              // dense and global passes: scratch_per_wg bytes per workgroup
     uint64_t scratch_per_wg;
     uint32_t* counts;              // per bucket 8 counters (CT_*), written by the pass that finishes it
     UnitigRec* recs;
@@ -216,8 +225,9 @@ struct ContractArgs {
     uint32_t* retry;               // set when the output capacity was short
     uint32_t* error;               // set on an inconsistent table
 };
-// Two passes: LDS tables for every listed bucket that fits, then the global
-// scratch tables for the rest (their count is read on the device).
+// Three passes: LDS tables for every listed bucket that fits, dense tables
+// for the rest that fit those, then global scratch tables (the counts of the
+// later passes are read on the device).
 void launch_contract(const ContractArgs& a, uint32_t KW, uint32_t grid_lds, uint32_t grid_global, hipStream_t s);
 
 // ---------------------------------------------------------------------------

@@ -133,7 +133,7 @@ struct cfgpu_handle {
     DevBuf ex_keys, ex_ef, ex_st, ex_dfa, ex_off, ex_cursor, ex_list;
 
     // contract: per-subgraph counters, unitig records and labels
-    DevBuf ct_list, ct_ovf, ct_counts, ct_ctl, ct_scratch, ct_recs, ct_labels;
+    DevBuf ct_list, ct_ovf, ct_ovf2, ct_counts, ct_ctl, ct_scratch, ct_recs, ct_labels;
     uint64_t n_unitigs = 0, n_ubytes = 0;
     uint64_t ct_need[2] = {0, 0};  // records and label bytes the last contract needed
     bool contracted = false;
@@ -1230,9 +1230,13 @@ int cfgpu_contract(cfgpu_handle* h) {
         }
     const uint32_t KW = h->KW, n_list = uint32_t(list.size());
     const uint32_t grid = std::min<uint32_t>(n_list, uint32_t(h->cus));  // one 1024-thread workgroup per CU
-    const uint64_t per_wg = ct_global_capacity(max_nv) * uint64_t(8 * KW + 8);
+    // scratch per workgroup: the dense tables or a global table for the
+    // largest subgraph, whichever is larger (256-byte aligned regions)
+    const uint64_t per_wg =
+        (std::max(ct_global_capacity(max_nv) * uint64_t(8 * KW + 8), ct_big_scratch_bytes(KW)) + 255) & ~255ull;
     HIPCK(h, h->ct_list.ensure(n_list * 4ull));
     HIPCK(h, h->ct_ovf.ensure(n_list * 4ull));
+    HIPCK(h, h->ct_ovf2.ensure(n_list * 4ull));
     HIPCK(h, h->ct_counts.ensure(size_t(h->G) * sizeof(cfgpu_contract_stat)));
     HIPCK(h, h->ct_ctl.ensure(64));
     HIPCK(h, h->ct_scratch.ensure(per_wg * std::max<uint32_t>(grid, 1)));
@@ -1263,6 +1267,9 @@ int cfgpu_contract(cfgpu_handle* h) {
         uint32_t* ctl_d = h->ct_ctl.as<uint32_t>();
         a.queue_head = ctl_d;       // [0], [1]
         a.ovf_count = ctl_d + 2;
+        a.walk_head = ctl_d + 5;
+        a.ovf2_count = ctl_d + 6;
+        a.ovf2_list = h->ct_ovf2.as<uint32_t>();
         a.retry = ctl_d + 3;
         a.error = ctl_d + 4;
         a.cursors = reinterpret_cast<unsigned long long*>(ctl_d + 8);  // [8..11]
@@ -1286,7 +1293,8 @@ int cfgpu_contract(cfgpu_handle* h) {
         if (!ctl[3]) {
             h->n_unitigs = cur[0];
             h->n_ubytes = cur[1];
-            h->rs.n_contract_global = ctl[2];
+            h->rs.n_contract_global = ctl[6];
+            h->rs.n_contract_dense = ctl[2] - ctl[6];
             break;
         }
         if (run == 2) return fail(h, CFGPU_E_OVERFLOW, "contract: output still short after resizing");

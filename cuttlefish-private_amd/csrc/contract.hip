@@ -23,9 +23,12 @@
 //           the path is written in the orientation whose first k-mer is below
 //           its reverse complement's (include/Maximal_Unitig_Scratch.hpp:
 //           136-138, 198-217), each vertex storing its own base.
-//           Global tables (k_contract_walk, subgraphs too large for LDS): a
-//           thread walks from every path end to the other, and the end whose
-//           orientation is canonical writes the path in a second walk;
+//           Dense tables (k_contract_big, subgraphs too large for LDS): the
+//           same ranking over vertex arrays in global scratch, the key ->
+//           vertex index held in LDS.
+//           Global tables (k_contract_walk, larger still): a thread walks
+//           from every path end to the other, and the end whose orientation
+//           is canonical writes the path in a second walk;
 //   cycles  what no path holds lies on cycles; each is walked once and written
 //           from its smallest vertex in canonical form
 //           (Maximal_Unitig_Scratch::finalize, :213-219; FASTA_Record.hpp:142-146).
@@ -123,6 +126,64 @@ __device__ __forceinline__ int tab_find(const CtTab<KW>& t, const Kmer<KW>& x) {
     return -1;
 }
 
+// Dense vertex arrays in global scratch with an LDS index: subgraphs whose
+// non-isolated vertices exceed an LDS table.  Vertex d (0 <= d < cap) has its
+// key words at key[d * KW ..] and its state word at meta[d]; the LDS hash
+// `ht` maps a key to d (entry d | 16 fingerprint bits of the key's hash << 16,
+// all-ones = free), so a lookup reads the one vertex whose fingerprint matches.
+constexpr uint32_t CB_HCAP = 36864;            // LDS index slots (u32 each)
+constexpr uint32_t CB_NMAX = CB_HCAP * 4 / 5;  // vertices an index takes (load <= 0.8)
+static_assert(CB_NMAX == uint32_t(ct_big_max_vertices()), "host and device agree on the dense table size");
+static_assert(2 * CB_NMAX < (1u << 16), "directed states fit 16 bits");
+constexpr uint32_t HT_FREE = 0xFFFFFFFFu;
+
+template <int KW>
+struct BigTab {
+    uint64_t* key;
+    uint32_t* meta;
+    uint32_t* aux;
+    uint32_t cap;  // vertices
+    uint32_t* ht;  // LDS index, CB_HCAP entries
+};
+
+template <int KW>
+__device__ __forceinline__ Kmer<KW> tab_key(const BigTab<KW>& t, uint32_t i) {
+    Kmer<KW> x;
+#pragma unroll
+    for (int w = 0; w < KW; ++w) x.w[w] = t.key[size_t(i) * KW + w];
+    return x;
+}
+
+template <int KW>
+__device__ __forceinline__ int tab_find(const BigTab<KW>& t, const Kmer<KW>& x) {
+    const uint64_t h = km_hash(x);
+    const uint32_t fp = uint32_t(h) & 0xFFFF0000u;
+    uint32_t s = slot_of(h, CB_HCAP);
+    for (uint32_t n = 0; n < CB_HCAP; ++n) {
+        const uint32_t e = t.ht[s];
+        if (e == HT_FREE) return -1;
+        if ((e & 0xFFFF0000u) == fp) {
+            const uint32_t d = e & 0xFFFFu;
+            bool eq = true;
+#pragma unroll
+            for (int w = 0; w < KW; ++w) eq &= t.key[size_t(d) * KW + w] == x.w[w];
+            if (eq) return int(d);
+        }
+        s = s + 1 == CB_HCAP ? 0 : s + 1;
+    }
+    return -1;
+}
+
+// Index vertex d under key x (keys are unique): the first free slot from the
+// key's home slot.
+template <int KW>
+__device__ __forceinline__ void big_index(uint32_t* ht, const Kmer<KW>& x, uint32_t d) {
+    const uint64_t h = km_hash(x);
+    const uint32_t e = d | (uint32_t(h) & 0xFFFF0000u);
+    uint32_t s = slot_of(h, CB_HCAP);
+    while (atomicCAS(&ht[s], HT_FREE, e) != HT_FREE) s = s + 1 == CB_HCAP ? 0 : s + 1;
+}
+
 __device__ __forceinline__ uint32_t ld_meta(const uint32_t* m) {
     return __hip_atomic_load(m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
@@ -130,8 +191,8 @@ __device__ __forceinline__ uint32_t ld_meta(const uint32_t* m) {
 // One step of a walk: from the directed vertex z (rc zb) out of side `side`
 // of slot `cur`, whose state `m` continues there.  Returns the next slot (-1
 // if the table lacks it) and updates z, zb, side (the exit side there) and b.
-template <int KW>
-__device__ __forceinline__ int walk_step(const CtTab<KW>& t, uint32_t m, uint32_t k, Kmer<KW>& z, Kmer<KW>& zb,
+template <int KW, class Tab>
+__device__ __forceinline__ int walk_step(const Tab& t, uint32_t m, uint32_t k, Kmer<KW>& z, Kmer<KW>& zb,
                                          uint32_t& side, uint32_t& b) {
     const uint32_t e = side_edge(m, side);
     b = side == 1 ? e : 3u - e;
@@ -175,9 +236,9 @@ __device__ __forceinline__ void put_kmer(char* out, const Kmer<KW>& z, uint32_t 
 // appended to `list` when given).  Sets *s_ovf when more than `limit` do not
 // fit.  Several slots per thread are loaded before any is used, so each thread
 // keeps several HBM reads in flight.
-template <int KW>
-__device__ __forceinline__ void load_bucket(const ContractArgs& a, uint32_t b, const CtTab<KW>& t, uint32_t limit,
-                                            uint32_t* s_n, uint32_t* s_ovf, uint32_t* s_cnt, uint16_t* list) {
+template <int KW, class Insert>
+__device__ __forceinline__ void load_bucket_with(const ContractArgs& a, uint32_t b, uint32_t limit, uint32_t* s_n,
+                                                 uint32_t* s_ovf, uint32_t* s_cnt, Insert&& insert) {
     constexpr int U = KW <= 2 ? 8 : 4;
     const uint64_t* tbl = reinterpret_cast<const uint64_t*>(a.tbl_ptr[b]);
     const uint64_t tcap = a.tbl_cap[b];
@@ -210,12 +271,9 @@ __device__ __forceinline__ void load_bucket(const ContractArgs& a, uint32_t b, c
             for (int w = 0; w < KW; ++w) x.w[w] = e[u][w];
             const uint32_t m = f | (bk << 3) | ((st & 2u) << 5) | ((st & 4u) << 5);
             const uint32_t d = atomicAdd(s_n, 1u);
-            const int slot = d < limit ? tab_insert(t, x, m) : -1;
-            if (slot < 0) {
+            if (d >= limit || !insert(d, x, m)) {
                 *s_ovf = 1;
                 stop = true;
-            } else if (list) {
-                list[d] = uint16_t(slot);
             }
         }
         if (stop || __hip_atomic_load(s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
@@ -224,16 +282,28 @@ __device__ __forceinline__ void load_bucket(const ContractArgs& a, uint32_t b, c
     if (ph) atomicAdd(&s_cnt[CT_PHANTOM], ph);
 }
 
+// The LDS / global open-addressing tables: vertex d goes to a free slot of t,
+// whose index is appended to `list` when given.
+template <int KW>
+__device__ __forceinline__ void load_bucket(const ContractArgs& a, uint32_t b, const CtTab<KW>& t, uint32_t limit,
+                                            uint32_t* s_n, uint32_t* s_ovf, uint32_t* s_cnt, uint16_t* list) {
+    load_bucket_with<KW>(a, b, limit, s_n, s_ovf, s_cnt, [&](uint32_t d, const Kmer<KW>& x, uint32_t m) {
+        const int slot = tab_insert(t, x, m);
+        if (slot >= 0 && list) list[d] = uint16_t(slot);
+        return slot >= 0;
+    });
+}
+
 // Side s of slot i (key x, rc xr, state m): the directed state it continues to
 // as (slot << 1 | exit side there), or -1 when the unitig ends there.  Sets
 // *err on an inconsistent table.
-template <int KW>
-__device__ __forceinline__ int link_of(const CtTab<KW>& t, uint32_t i, uint32_t s, const Kmer<KW>& x,
+template <int KW, class Tab>
+__device__ __forceinline__ int link_of(const Tab& t, uint32_t i, uint32_t s, const Kmer<KW>& x,
                                        const Kmer<KW>& xr, uint32_t m, uint32_t k, uint32_t* err) {
     if (side_edge(m, s) >= 4) return -1;
     Kmer<KW> z = s ? x : xr, zb = s ? xr : x;
     uint32_t side = s, bb;
-    const int j = walk_step(t, m, k, z, zb, side, bb);
+    const int j = walk_step<KW>(t, m, k, z, zb, side, bb);
     if (j < 0) {
         *err = 1;
         return -1;
@@ -250,8 +320,8 @@ __device__ __forceinline__ int link_of(const CtTab<KW>& t, uint32_t i, uint32_t 
 // when the list overflowed), walked once from a vertex out of its back side;
 // the smallest vertex in canonical form writes it.  Its label offset goes to
 // *off_of(slot).
-template <int KW, class OffOf>
-__device__ void cycles_serial(const CtTab<KW>& t, uint32_t k, const uint32_t* cand, uint32_t ncand, uint32_t* s_bytes,
+template <int KW, class Tab, class OffOf>
+__device__ void cycles_serial(const Tab& t, uint32_t k, const uint32_t* cand, uint32_t ncand, uint32_t* s_bytes,
                               uint32_t* s_nrec, uint32_t* s_cnt, uint32_t* s_err, OffOf off_of) {
     const bool all = ncand > uint32_t(CT_CAND);
     const uint32_t nc = all ? t.cap : ncand;
@@ -259,7 +329,7 @@ __device__ void cycles_serial(const CtTab<KW>& t, uint32_t k, const uint32_t* ca
         const uint32_t c = all ? q : cand[q];
         if (t.key[size_t(c) * KW + KW - 1] == EMPTY || !(t.meta[c] & M_ONCYC) || (t.meta[c] & (M_VIS | M_WRITER)))
             continue;
-        Kmer<KW> z = tab_key(t, c), zb = km_rc(z, k), mn = z;
+        Kmer<KW> z = tab_key<KW>(t, c), zb = km_rc(z, k), mn = z;
         uint32_t side = 1, cur = c, n = 1, ms = c, bb;
         t.meta[c] |= M_VIS;
         for (uint32_t step = 0; step < t.cap; ++step) {
@@ -268,7 +338,7 @@ __device__ void cycles_serial(const CtTab<KW>& t, uint32_t k, const uint32_t* ca
                 *s_err = 1;
                 break;
             }
-            const int j = walk_step(t, m, k, z, zb, side, bb);
+            const int j = walk_step<KW>(t, m, k, z, zb, side, bb);
             if (j < 0) {
                 *s_err = 1;
                 break;
@@ -296,10 +366,10 @@ __device__ void cycles_serial(const CtTab<KW>& t, uint32_t k, const uint32_t* ca
 
 // Writes the cycle whose smallest vertex is slot i (canonical, leaving through
 // its back side) at out[0 .. n + k - 1), never past `end`.
-template <int KW>
-__device__ __forceinline__ uint32_t write_cycle(const CtTab<KW>& t, uint32_t i, uint32_t k, char* out, uint64_t off,
+template <int KW, class Tab>
+__device__ __forceinline__ uint32_t write_cycle(const Tab& t, uint32_t i, uint32_t k, char* out, uint64_t off,
                                                 uint64_t end, uint32_t* err) {
-    Kmer<KW> z = tab_key(t, i), zb = km_rc(z, k);
+    Kmer<KW> z = tab_key<KW>(t, i), zb = km_rc(z, k);
     if (off + k > end) {
         atomicOr(err, 1u);
         return 0;
@@ -309,7 +379,7 @@ __device__ __forceinline__ uint32_t write_cycle(const CtTab<KW>& t, uint32_t i, 
     for (uint32_t step = 0; step < t.cap; ++step) {
         const uint32_t m = t.meta[cur];
         if (!(m & (M_LINK0 << side))) break;
-        const int j = walk_step(t, m, k, z, zb, side, bb);
+        const int j = walk_step<KW>(t, m, k, z, zb, side, bb);
         if (j < 0 || uint32_t(j) == i) break;
         if (off + k + n > end) {
             atomicOr(err, 1u);
@@ -497,7 +567,7 @@ __global__ __launch_bounds__(CT_WG) void k_contract_rank(ContractArgs a) {
         }
         __syncthreads();
         if (tid == 0 && s_ncand)
-            cycles_serial(t, k, s_cand, s_ncand, &s_bytes, &s_nrec, s_cnt, &s_err,
+            cycles_serial<KW>(t, k, s_cand, s_ncand, &s_bytes, &s_nrec, s_cnt, &s_err,
                           [&](uint32_t slot) { return &s_jp[2 * slot]; });
         __syncthreads();
 
@@ -523,7 +593,7 @@ __global__ __launch_bounds__(CT_WG) void k_contract_rank(ContractArgs a) {
             if (!info[q]) {
                 if ((s_meta[i] & (M_WRITER | M_CYCLE)) != (M_WRITER | M_CYCLE)) continue;
                 const uint64_t off = s_bbyte + s_jp[2 * i];
-                const uint32_t n = write_cycle(t, i, k, a.labels + off, off, lab_end, a.error);
+                const uint32_t n = write_cycle<KW>(t, i, k, a.labels + off, off, lab_end, a.error);
                 const uint32_t ri = atomicAdd(&s_rctr, 1u);
                 if (ri < s_nrec) a.recs[s_brec + ri] = UnitigRec{off, n + k - 1, uint16_t(b), 1};
                 continue;
@@ -552,6 +622,221 @@ __global__ __launch_bounds__(CT_WG) void k_contract_rank(ContractArgs a) {
     }
 }
 
+// Subgraphs whose non-isolated vertices exceed an LDS table but not
+// CB_NMAX: the same list ranking as k_contract_rank over dense vertex arrays
+// in this workgroup's global scratch region, with the key -> vertex index in
+// LDS.  Ranking words pack a directed state (16 bits) and a distance (16
+// bits); two buffers alternate between rounds.
+constexpr uint32_t CB_ROUNDS = 17;  // > log2(2 * CB_NMAX): every path converges
+constexpr uint32_t CB_M16 = 0xFFFFu;
+__device__ __forceinline__ uint32_t cb_pack(uint32_t p, uint32_t d) { return p | (min(d, CB_M16) << 16); }
+
+template <int KW>
+__global__ __launch_bounds__(CT_WG) void k_contract_big(ContractArgs a) {
+    __shared__ uint32_t s_ht[CB_HCAP];
+    __shared__ uint32_t s_cand[CT_CAND];
+    __shared__ uint32_t s_cnt[8];
+    __shared__ uint32_t s_chg[CB_ROUNDS];
+    __shared__ uint32_t s_item, s_n, s_ovf, s_ncand, s_bytes, s_nrec, s_rctr, s_err, s_skip;
+    __shared__ unsigned long long s_brec, s_bbyte;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t k = a.k;
+    const uint32_t kt = k - 32 * (KW - 1);
+    // scratch: key words, state, two ranking buffers, path info (first vertex
+    // | forward << 16 | on a path << 17), position, the first vertex's
+    // vertex count | end flags << 29, label offsets
+    constexpr uint64_t N = CB_NMAX;
+    uint8_t* base = a.scratch + uint64_t(blockIdx.x) * a.scratch_per_wg;
+    BigTab<KW> t;
+    t.key = reinterpret_cast<uint64_t*>(base);
+    t.meta = reinterpret_cast<uint32_t*>(base + N * KW * 8);
+    uint32_t* jp0 = t.meta + N;
+    uint32_t* jp1 = jp0 + 2 * N;
+    uint32_t* ia = jp1 + 2 * N;
+    uint32_t* posv = ia + N;
+    uint32_t* wnv = posv + N;
+    uint32_t* offv = wnv + N;
+    t.aux = offv;
+    t.ht = s_ht;
+
+    while (true) {
+        __syncthreads();
+        if (tid == 0) {
+            s_item = atomicAdd(&a.queue_head[1], 1u);
+            s_n = s_ovf = s_ncand = s_bytes = s_nrec = s_rctr = s_err = s_skip = 0;
+        }
+        if (tid < 8) s_cnt[tid] = 0;
+        if (tid < CB_ROUNDS) s_chg[tid] = 0;
+        __syncthreads();
+        if (s_item >= __hip_atomic_load(a.ovf_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        const uint32_t b = a.ovf_list[s_item];
+        for (uint32_t i = tid; i < CB_HCAP; i += CT_WG) s_ht[i] = HT_FREE;
+        __syncthreads();
+        load_bucket_with<KW>(a, b, CB_NMAX, &s_n, &s_ovf, s_cnt, [&](uint32_t d, const Kmer<KW>& x, uint32_t m) {
+#pragma unroll
+            for (int w = 0; w < KW; ++w) t.key[size_t(d) * KW + w] = x.w[w];
+            t.meta[d] = m;
+            big_index(s_ht, x, d);
+            return true;
+        });
+        __syncthreads();
+        if (s_ovf) {
+            if (tid == 0) a.ovf2_list[atomicAdd(a.ovf2_count, 1u)] = b;
+            continue;
+        }
+        const uint32_t n = s_n;
+        t.cap = n;
+
+        // link: each directed state points at its successor (1 step) or at itself (an end)
+        for (uint32_t d = tid; d < n; d += CT_WG) {
+            const Kmer<KW> x = tab_key<KW>(t, d);
+            const Kmer<KW> xr = km_rc(x, k);
+            const uint32_t m = ld_meta(&t.meta[d]);
+            uint32_t add = 0;
+            for (uint32_t sd = 0; sd < 2; ++sd) {
+                const int nx = link_of<KW>(t, d, sd, x, xr, m, k, &s_err);
+                jp0[2 * d + sd] = nx < 0 ? cb_pack(2 * d + sd, 0) : cb_pack(uint32_t(nx), 1);
+                if (nx >= 0) add |= M_LINK0 << sd;
+            }
+            // only this thread writes vertex d's state here; the others read
+            // its DFA bits, which the store keeps
+            if (add) __hip_atomic_store(&t.meta[d], m | add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        __syncthreads();
+
+        // rank: pointer jumping until no path state moves (cycle states stop
+        // counting once past the vertex count)
+        uint32_t* jp = jp0;
+        uint32_t* jq = jp1;
+        for (uint32_t r = 0; r < CB_ROUNDS; ++r) {
+            bool any = false;
+            for (uint32_t x = tid; x < 2 * n; x += CT_WG) {
+                const uint32_t w = jp[x], p = w & CB_M16;
+                uint32_t nw = w;
+                if (p != x) {
+                    const uint32_t w2 = jp[p];
+                    const uint32_t p2 = w2 & CB_M16, nd = (w >> 16) + (w2 >> 16);
+                    nw = cb_pack(p2, nd);
+                    any |= p2 != p && nd < n;
+                }
+                jq[x] = nw;
+            }
+            if (any) s_chg[r] = 1;
+            __syncthreads();
+            uint32_t* tmp = jp;
+            jp = jq;
+            jq = tmp;
+            if (!s_chg[r]) break;
+        }
+
+        // per vertex: the ends of its path, the output orientation and its position
+        for (uint32_t d = tid; d < n; d += CT_WG) {
+            const uint32_t wf = jp[2 * d], wb = jp[2 * d + 1];
+            const uint32_t Ff = wf & CB_M16, Fb = wb & CB_M16;
+            if (jp[Fb] != cb_pack(Fb, 0)) {  // not a path end: the vertex lies on a cycle
+                atomicOr(&t.meta[d], M_ONCYC);
+                const uint32_t c = atomicAdd(&s_ncand, 1u);
+                if (c < uint32_t(CT_CAND)) s_cand[c] = d;
+                ia[d] = 0;
+                continue;
+            }
+            const uint32_t df = wf >> 16, db = wb >> 16;
+            const uint32_t L = Ff >> 1, sL = Ff & 1u, R = Fb >> 1, sR = Fb & 1u;
+            const Kmer<KW> cL = tab_key<KW>(t, L), cR = tab_key<KW>(t, R);
+            const Kmer<KW> first = sL == 0 ? cL : km_rc(cL, k);
+            const Kmer<KW> rlast = sR == 1 ? km_rc(cR, k) : cR;
+            const bool fwd = km_less(first, rlast);
+            const uint32_t pos = fwd ? df : db;
+            const uint32_t A = fwd ? L : R;
+            ia[d] = A | (uint32_t(fwd) << 16) | (1u << 17);
+            posv[d] = pos;
+            if (pos == 0) {
+                const uint32_t sA = fwd ? sL : sR, B = fwd ? R : L, sB = fwd ? sR : sL;
+                const uint32_t fl = (side_exits(ld_meta(&t.meta[A]), sA) ? 2u : 0u) |
+                                    (side_exits(ld_meta(&t.meta[B]), sB) ? 4u : 0u);
+                wnv[d] = (df + db + 1) | (fl << 29);
+            }
+        }
+        __syncthreads();
+
+        // the first vertex of each path reserves its label
+        {
+            uint32_t nrec = 0, verts = 0, lbytes = 0, triv = 0, disc = 0;
+            for (uint32_t d = tid; d < n; d += CT_WG) {
+                if (!ia[d] || posv[d]) continue;
+                const uint32_t wn = wnv[d] & 0x1FFFFFFFu, fl = wnv[d] >> 29;
+                const uint32_t bytes = wn + k - 1;
+                offv[d] = atomicAdd(&s_bytes, bytes);
+                ++nrec;
+                verts += wn;
+                lbytes += bytes;
+                disc += fl != 0;
+                triv += fl == 0;
+            }
+            if (nrec) {
+                atomicAdd(&s_nrec, nrec);
+                atomicAdd(&s_cnt[CT_VERTICES], verts);
+                atomicAdd(&s_cnt[CT_LABEL_BYTES], lbytes);
+                atomicAdd(&s_cnt[CT_TRIVIAL], triv);
+                atomicAdd(&s_cnt[CT_DISC_EDGES], disc);
+            }
+        }
+        __syncthreads();
+        if (tid == 0 && s_ncand)
+            cycles_serial<KW>(t, k, s_cand, s_ncand, &s_bytes, &s_nrec, s_cnt, &s_err,
+                              [&](uint32_t slot) { return &offv[slot]; });
+        __syncthreads();
+
+        // reserve the bucket's output, publish its counters
+        if (tid == 0) {
+            s_brec = atomicAdd(&a.cursors[0], (unsigned long long)s_nrec);
+            s_bbyte = atomicAdd(&a.cursors[1], (unsigned long long)s_bytes);
+            s_skip = s_brec + s_nrec > a.rec_cap || s_bbyte + s_bytes > a.lab_cap;
+            if (s_skip) atomicOr(a.retry, 1u);
+            if (s_err) atomicOr(a.error, 1u);
+            s_cnt[CT_FLAGS] = 8u | (s_err ? 4u : 0u);
+        }
+        __syncthreads();
+        if (tid < 8) a.counts[size_t(b) * 8 + tid] = s_cnt[tid];
+        if (s_skip) continue;
+
+        // write: every path vertex stores its base, the first one its k-mer and the record
+        const uint64_t lab_end = s_bbyte + s_bytes;
+        for (uint32_t d = tid; d < n; d += CT_WG) {
+            const uint32_t info = ia[d];
+            if (!info) {
+                if ((ld_meta(&t.meta[d]) & (M_WRITER | M_CYCLE)) != (M_WRITER | M_CYCLE)) continue;
+                const uint64_t off = s_bbyte + offv[d];
+                const uint32_t nc = write_cycle<KW>(t, d, k, a.labels + off, off, lab_end, a.error);
+                const uint32_t ri = atomicAdd(&s_rctr, 1u);
+                if (ri < s_nrec) a.recs[s_brec + ri] = UnitigRec{off, nc + k - 1, uint16_t(b), 1};
+                continue;
+            }
+            const uint32_t A = info & 0xFFFFu, pos = posv[d];
+            const bool fwd = (info >> 16) & 1u;
+            const uint64_t off = s_bbyte + offv[A];
+            const Kmer<KW> x = tab_key<KW>(t, d);
+            if (pos == 0) {
+                const uint32_t wn = wnv[d] & 0x1FFFFFFFu, fl = wnv[d] >> 29;
+                if (off + wn + k - 1 > lab_end) {
+                    atomicOr(a.error, 1u);
+                    continue;
+                }
+                put_kmer(a.labels + off, fwd ? x : km_rc(x, k), k);
+                const uint32_t ri = atomicAdd(&s_rctr, 1u);
+                if (ri < s_nrec) a.recs[s_brec + ri] = UnitigRec{off, wn + k - 1, uint16_t(b), uint16_t(fl)};
+            } else {
+                const uint32_t c = fwd ? uint32_t(x.w[0] & 3u) : 3u - uint32_t((x.w[KW - 1] >> (2 * (kt - 1))) & 3u);
+                if (off + pos + k - 1 >= lab_end) {
+                    atomicOr(a.error, 1u);
+                    continue;
+                }
+                a.labels[off + pos + k - 1] = char(base_char(c));
+            }
+        }
+    }
+}
+
 // Subgraphs whose vertices do not fit an LDS table: a table in this
 // workgroup's global scratch region, sized by the subgraph's vertex count.
 template <int KW>
@@ -566,13 +851,13 @@ __global__ __launch_bounds__(CT_WG) void k_contract_walk(ContractArgs a) {
     while (true) {
         __syncthreads();
         if (tid == 0) {
-            s_item = atomicAdd(&a.queue_head[1], 1u);
+            s_item = atomicAdd(a.walk_head, 1u);
             s_n = s_ovf = s_ncand = s_bytes = s_nrec = s_rctr = s_err = s_skip = 0;
         }
         if (tid < 8) s_cnt[tid] = 0;
         __syncthreads();
-        if (s_item >= __hip_atomic_load(a.ovf_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-        const uint32_t b = a.ovf_list[s_item];
+        if (s_item >= __hip_atomic_load(a.ovf2_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        const uint32_t b = a.ovf2_list[s_item];
 
         CtTab<KW> t;
         const uint32_t cap = uint32_t(ct_global_capacity(a.n_vert[b]));
@@ -663,7 +948,7 @@ __global__ __launch_bounds__(CT_WG) void k_contract_walk(ContractArgs a) {
         }
         __syncthreads();
         if (tid == 0 && s_ncand)
-            cycles_serial(t, k, s_cand, s_ncand, &s_bytes, &s_nrec, s_cnt, &s_err,
+            cycles_serial<KW>(t, k, s_cand, s_ncand, &s_bytes, &s_nrec, s_cnt, &s_err,
                           [&](uint32_t slot) { return &t.aux[slot]; });
         __syncthreads();
 
@@ -690,7 +975,7 @@ __global__ __launch_bounds__(CT_WG) void k_contract_walk(ContractArgs a) {
             const uint64_t off = s_bbyte + t.aux[i];
             char* out = a.labels + off;
             if (m0 & M_CYCLE) {
-                const uint32_t n = write_cycle(t, i, k, out, off, lab_end, a.error);
+                const uint32_t n = write_cycle<KW>(t, i, k, out, off, lab_end, a.error);
                 const uint32_t ri = atomicAdd(&s_rctr, 1u);
                 if (ri < s_nrec) a.recs[s_brec + ri] = UnitigRec{off, n + k - 1, uint16_t(b), 1};
                 continue;
@@ -728,7 +1013,10 @@ __global__ __launch_bounds__(CT_WG) void k_contract_walk(ContractArgs a) {
 template <int KW>
 void launch_kw(const ContractArgs& a, uint32_t grid_lds, uint32_t grid_global, hipStream_t s) {
     if (grid_lds) hipLaunchKernelGGL(k_contract_rank<KW>, dim3(grid_lds), dim3(CT_WG), 0, s, a);
-    if (grid_global) hipLaunchKernelGGL(k_contract_walk<KW>, dim3(grid_global), dim3(CT_WG), 0, s, a);
+    if (grid_global) {
+        hipLaunchKernelGGL(k_contract_big<KW>, dim3(grid_global), dim3(CT_WG), 0, s, a);
+        hipLaunchKernelGGL(k_contract_walk<KW>, dim3(grid_global), dim3(CT_WG), 0, s, a);
+    }
 }
 
 }  // namespace

@@ -62,7 +62,8 @@ class RunStats(ctypes.Structure):
                    ("n_unitig_bytes", ctypes.c_uint64), ("n_contract_global", ctypes.c_uint32),
                    ("n_contract_runs", ctypes.c_uint32), ("t_cdbg_ms", ctypes.c_double),
                    ("n_cdbg_unitigs", ctypes.c_uint64), ("n_cdbg_bytes", ctypes.c_uint64),
-                   ("n_cdbg_rounds", ctypes.c_uint32), ("n_cdbg_cycle_pieces", ctypes.c_uint32)])
+                   ("n_cdbg_rounds", ctypes.c_uint32), ("n_cdbg_cycle_pieces", ctypes.c_uint32),
+                   ("n_contract_dense", ctypes.c_uint32)])
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -182,7 +182,9 @@ enum { CFGPU_UNITIG_CYCLE = 1, CFGPU_UNITIG_LEFT_EXIT = 2, CFGPU_UNITIG_RIGHT_EX
 int cfgpu_contract(cfgpu_handle* h);
 /* Per-subgraph counters of the last contract (Subgraph's isolated, trivial_mtig_c,
  * icc_count, disc_edge_c, the vertices in unitigs and the label bytes;
- * include/Subgraph.hpp:178-186).  flags: 1 LDS table, 2 global table, 0 not contracted. */
+ * include/Subgraph.hpp:178-186).  flags: 1 LDS table, 8 dense table (vertex arrays in
+ * global memory, index in LDS), 2 global table, 0 not contracted; 4 is added when the
+ * subgraph's table was inconsistent. */
 typedef struct cfgpu_contract_stat {
     uint32_t isolated, phantom, trivial, icc, disc_edges, vertices, label_bytes, flags;
 } cfgpu_contract_stat;
@@ -233,13 +235,14 @@ typedef struct cfgpu_run_stats {
     /* last cfgpu_contract */
     double t_contract_ms;             /* both passes, incl. an output re-run when it was short */
     uint64_t n_unitigs, n_unitig_bytes;
-    uint32_t n_contract_global;       /* subgraphs too large for an LDS table */
+    uint32_t n_contract_global;       /* subgraphs too large for the LDS and dense tables */
     uint32_t n_contract_runs;         /* 1, or 2 after growing the output */
     /* last cfgpu_cdbg */
     double t_cdbg_ms;
     uint64_t n_cdbg_unitigs, n_cdbg_bytes;
     uint32_t n_cdbg_rounds;           /* pointer-jumping rounds */
     uint32_t n_cdbg_cycle_pieces;     /* lm-tigs on cycles through several subgraphs (host) */
+    uint32_t n_contract_dense;        /* subgraphs contracted with dense tables (too large for LDS) */
 } cfgpu_run_stats;
 int cfgpu_run_stats_get(cfgpu_handle* h, cfgpu_run_stats* out);
 

@@ -77,11 +77,12 @@ def test_contract_reference_k127():
     run("k127", buf, False, 127, 20, 1)
 
 
-def test_contract_large_subgraphs_use_global_tables():
-    # Zipf-sized buckets of random records: the largest hold far more vertices
-    # than an LDS table, so the global-scratch pass takes them.
+def test_contract_large_subgraphs_use_dense_and_global_tables():
+    # Zipf-sized buckets of random records: the large ones hold more vertices
+    # than an LDS table (dense tables: vertex arrays in global scratch, index
+    # in LDS), the largest more than a dense table (global-scratch walk).
     for k, l in ((31, 20), (63, 20)):
-        att, lab, off = C.synth_bucket_workload(256, 60000, 1.1, k, l, seed=40 + k)
+        att, lab, off = C.synth_bucket_workload(256, 150000, 1.1, k, l, seed=40 + k)
         st = O.construct(k, l, off, (att & 0xFFFF).astype(np.uint16), lab)
         want = O.contract(k, 1, st, normalise=True)
         say(f"[case] zipf-contract k={k} vertices={len(st['g'])}")
@@ -91,8 +92,11 @@ def test_contract_large_subgraphs_use_global_tables():
             h.contract()
             stats = h.contract_stats()
             compare("zipf", h.unitigs(), stats, want)
-            assert (stats[:, 7] == 2).sum() > 0 and (stats[:, 7] == 1).sum() > 0
-            assert h.run_stats()["n_contract_global"] == (stats[:, 7] == 2).sum()
+            kinds = {f: int((stats[:, 7] == f).sum()) for f in (1, 8, 2)}
+            say(f"[case] zipf-contract k={k} subgraphs by table: {kinds}")
+            assert all(v > 0 for v in kinds.values())
+            rs = h.run_stats()
+            assert rs["n_contract_global"] == kinds[2] and rs["n_contract_dense"] == kinds[8]
 
 
 def test_contract_after_release_skips_subgraph():
