@@ -1,13 +1,21 @@
-"""Benchmark: k-mers/s from device-resident packed reads to per-subgraph vertex
-states (partition + construct) on one MI355X.
+"""Benchmark: k-mers/s from device-resident packed reads to the bit-exact
+compacted de Bruijn graph on one MI355X.
 
-Workload (default): 150 bp reads sampled uniformly from both strands of a 50 Mbp
-iid-ACGT genome at 30x (10,000,000 reads), 0.5% iid substitutions, k=31,
-cutoff 2, l=20 — synthetic, seeded.  A step = cfgpu_partition over the packed
-reads already in HBM + cfgpu_construct (batched persistent launch, learned
-table sizing).  The plain GPU path and the batched path without the learned
-sizing are timed beside it; the CPU baseline is the oracle pipeline on a
-bounded sample of the same reads on the host's cores.
+Workload (default, "c4"): 150 bp reads at 25x (50,000,000 reads) sampled
+uniformly from both strands of a 300 Mbp genome with 20 % repeats (families of
+300 bp - 6 kb, 2 % diverged copies), 1 % iid substitutions, k = 63 (128-bit
+keys), cutoff 2, l = 20 - synthetic, seeded.  ("c2": 10,000,000 reads of a
+50 Mbp iid genome, 0.5 % substitutions, k = 31.)
+
+A step = cfgpu_partition over the packed reads already in HBM
++ cfgpu_construct (batched persistent launch, learned table sizing)
++ cfgpu_contract (Subgraph::contract of every subgraph)
++ cfgpu_cdbg (lm-tigs glued into every maximal unitig of the graph)
++ cfgpu_cdbg_copy (the unitigs copied into pinned host memory).
+The construct alone is also timed on the plain GPU path (one launch per
+bucket, grow-and-retry) and on the batched path with upper-bound sizing; the
+CPU baseline is the oracle pipeline on a bounded sample of the same reads on
+all of the host's cores.
 
 Prints one JSON line.
 """
@@ -26,17 +34,24 @@ sys.path.insert(0, os.path.join(REPO, "cuttlefish-private_amd", "python"))
 import cfgpu as C  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "partition_construct_reads10M_k31_hbm_traffic.txt")
-ROOFLINE_KERNEL = "k_construct_lds<1, 2>"
+
+WORKLOADS = {
+    "c4": dict(genome=300_000_000, repeat_frac=0.2, divergence=0.02, reads=50_000_000, read_len=150,
+               sub_rate=0.01, k=63, l=20, cutoff=2, seed=2026,
+               traffic="construct_reads50M_k63_hbm_traffic.txt", roofline_kernel="k_construct_lds<2, 4>"),
+    "c2": dict(genome=50_000_000, repeat_frac=0.0, divergence=0.0, reads=10_000_000, read_len=150,
+               sub_rate=0.005, k=31, l=20, cutoff=2, seed=2026,
+               traffic="partition_construct_reads10M_k31_hbm_traffic.txt", roofline_kernel="k_construct_lds<1, 2>"),
+}
 
 
-def measured_traffic():
-    """HBM bytes per launch of the roofline kernel from the committed rocprofv3
+def measured_traffic(path, kernel):
+    """HBM bytes per launch of the roofline kernel from a committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (None when absent)."""
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch " + ROOFLINE_KERNEL + ":"):
+                if line.startswith("traffic_bytes_per_launch " + kernel + ":"):
                     return float(line.rsplit(":", 1)[1])
     except OSError:
         pass
@@ -48,14 +63,17 @@ def log(*a):
 
 
 class Hip:
-    """hipMalloc / hipMemcpy through the HIP runtime (device input buffers)."""
+    """hipMalloc / hipHostMalloc / hipMemcpy through the HIP runtime."""
 
     def __init__(self):
         self.L = ctypes.CDLL("libamdhip64.so")
         self.L.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.L.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_uint]
+        self.L.hipHostFree.argtypes = [ctypes.c_void_p]
         self.L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
         self.L.hipFree.argtypes = [ctypes.c_void_p]
         self.L.hipSetDevice.argtypes = [ctypes.c_int]
+        self.L.hipMemGetInfo.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
 
     def upload(self, arr):
         p = ctypes.c_void_p()
@@ -63,11 +81,21 @@ class Hip:
         assert self.L.hipMemcpy(p, arr.ctypes.data, arr.nbytes, 1) == 0, "hipMemcpy failed"
         return p
 
+    def pinned(self, nbytes):
+        p = ctypes.c_void_p()
+        assert self.L.hipHostMalloc(ctypes.byref(p), max(nbytes, 1), 0) == 0, "hipHostMalloc failed"
+        return p
+
     def sync(self):
         assert self.L.hipDeviceSynchronize() == 0
 
     def free(self, p):
         self.L.hipFree(p)
+
+    def used_gib(self):
+        f, t = ctypes.c_size_t(), ctypes.c_size_t()
+        self.L.hipMemGetInfo(ctypes.byref(f), ctypes.byref(t))
+        return (t.value - f.value) / 2**30
 
     def copy_d2d(self, nbytes):
         a, b = ctypes.c_void_p(), ctypes.c_void_p()
@@ -79,6 +107,8 @@ class Hip:
 
 
 def algorithmic_bytes(k, l, n_sk, n_e, n_km):
+    """Hash-insert model: records read, (k+1)-mer edge-count and vertex-state
+    read-modify-writes (n_sk*S + n_e*2*(Ke+4) + n_km*2*(Kv+4))."""
     W = ((2 * (k - 1) - l + 2) + 31) // 32
     S = 4 + 8 * W
     Ke = 8 * ((k + 1 + 31) // 32)
@@ -86,24 +116,55 @@ def algorithmic_bytes(k, l, n_sk, n_e, n_km):
     return n_sk * S + n_e * 2 * (Ke + 4) + n_km * 2 * (Kv + 4)
 
 
-def make_workload(args):
+def make_workload(w):
     t = time.time()
-    g = C.synth_genome(args.genome, 0.0, 300, 6000, 0.0, seed=args.seed)
-    reads = C.synth_reads(g, args.reads, args.read_len, args.sub_rate, seed=args.seed + 1)
-    rec_off = np.arange(args.reads + 1, dtype=np.uint64) * np.uint64(args.read_len)
-    inp = C.Input.from_records(reads, rec_off, args.k)
+    g = C.synth_genome(w["genome"], w["repeat_frac"], 300, 6000, w["divergence"], seed=w["seed"])
+    reads = C.synth_reads(g, w["reads"], w["read_len"], w["sub_rate"], seed=w["seed"] + 1)
+    del g
+    rec_off = np.arange(w["reads"] + 1, dtype=np.uint64) * np.uint64(w["read_len"])
+    inp = C.Input.from_records(reads, rec_off, w["k"])
     log(f"workload ready in {time.time() - t:.1f}s: {inp.n_bases} bases in {inp.n_frag} fragments")
     return reads, rec_off, inp
 
 
-def timed_steps(hip, h, dev, inp, mode, steps):
-    hip.sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        h.partition_device(dev[0].value, inp.n_bases, dev[1].value, inp.n_frag)
-        h.construct(mode)
-    hip.sync()
-    return (time.perf_counter() - t0) * 1000.0 / steps
+class CdbgOut:
+    """Pinned host buffers the unitigs are copied into (grown when short)."""
+
+    def __init__(self, hip):
+        self.hip, self.seq, self.rec, self.seq_cap, self.rec_cap = hip, None, None, 0, 0
+
+    def copy(self, h):
+        n, nb = ctypes.c_uint64(), ctypes.c_uint64()
+        L = C.lib()
+        h._check(L.cfgpu_cdbg_copy(h._h, None, 0, None, 0, ctypes.byref(n), ctypes.byref(nb)), "cdbg_copy")
+        if nb.value > self.seq_cap or n.value > self.rec_cap:
+            if self.seq is not None:
+                self.hip.L.hipHostFree(self.seq)
+                self.hip.L.hipHostFree(self.rec)
+            self.seq_cap, self.rec_cap = int(nb.value * 1.1) + 4096, int(n.value * 1.1) + 16
+            self.seq, self.rec = self.hip.pinned(self.seq_cap), self.hip.pinned(16 * self.rec_cap)
+        h._check(L.cfgpu_cdbg_copy(h._h, self.seq, self.seq_cap, ctypes.cast(self.rec, C.u64p), self.rec_cap,
+                                   ctypes.byref(n), ctypes.byref(nb)), "cdbg_copy")
+        return n.value, nb.value
+
+
+def run_step(h, dev, inp, out, mode="learned"):
+    h.partition_device(dev[0].value, inp.n_bases, dev[1].value, inp.n_frag)
+    h.construct(mode)
+    h.contract()
+    h._check(C.lib().cfgpu_cdbg(h._h), "cdbg")
+    return out.copy(h)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main():
@@ -111,16 +172,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--genome", type=int, default=50_000_000)
-    ap.add_argument("--reads", type=int, default=10_000_000)
-    ap.add_argument("--read-len", type=int, default=150)
-    ap.add_argument("--sub-rate", type=float, default=0.005)
-    ap.add_argument("--k", type=int, default=31)
-    ap.add_argument("--l", type=int, default=20)
-    ap.add_argument("--cutoff", type=int, default=2)
-    ap.add_argument("--seed", type=int, default=2026)
-    ap.add_argument("--compare-steps", type=int, default=2)
-    ap.add_argument("--cpu-sample-reads", type=int, default=3_000_000)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c4")
+    ap.add_argument("--compare-steps", type=int, default=1)
+    ap.add_argument("--cpu-sample-reads", type=int, default=0, help="reads in the CPU baseline sample (0: auto)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -131,29 +185,62 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the multi-GPU design shards buckets by subgraph id, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    w = WORKLOADS[args.workload]
+    k, l, cutoff = w["k"], w["l"], w["cutoff"]
 
-    reads, rec_off, inp = make_workload(args)
+    reads, rec_off, inp = make_workload(w)
     hip = Hip()
     hip.L.hipSetDevice(0)
     dev = (hip.upload(inp.packed), hip.upload(inp.frag_off))
+    out = CdbgOut(hip)
 
-    h = C.Handle(args.k, args.l, args.cutoff)
+    h = C.Handle(k, l, cutoff)
     if args.calib_copy_mib:
         log(f"calibration copy {args.calib_copy_mib} MiB")
         hip.copy_d2d(args.calib_copy_mib << 20)
     log("warmup")
-    for _ in range(args.warmup):
-        h.partition_device(dev[0].value, inp.n_bases, dev[1].value, inp.n_frag)
-        h.construct("learned")
-    log("timed learned steps")
-    ms = timed_steps(hip, h, dev, inp, "learned", args.steps)
+    for _ in range(max(1, args.warmup)):
+        run_step(h, dev, inp, out)
+    log("timed steps")
+    hip.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        n_cdbg, cdbg_bytes = run_step(h, dev, inp, out)
+    hip.sync()
+    ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+    mem_gib = hip.used_gib()
     rs = h.run_stats()
     st = h.bucket_stats()
-    n_vert_learned = st["n_vertices"].copy()
+    cst = h.contract_stats()
     n_km = rs["n_kmers"]
     value = n_km / (ms / 1000.0)
+    n_vert_learned = st["n_vertices"].copy()
 
-    # kernel-only timing of the main construct launch (direct launch, events)
+    # phases of one step, each alone (HIP events inside the library; the
+    # unitig copy timed on the host)
+    log("phases")
+    hip.sync()
+    t0 = time.perf_counter()
+    h.partition_device(dev[0].value, inp.n_bases, dev[1].value, inp.n_frag)
+    hip.sync()
+    t1 = time.perf_counter()
+    h.construct("learned")
+    hip.sync()
+    t2 = time.perf_counter()
+    h.contract()
+    hip.sync()
+    t3 = time.perf_counter()
+    h._check(C.lib().cfgpu_cdbg(h._h), "cdbg")
+    hip.sync()
+    t4 = time.perf_counter()
+    out.copy(h)
+    t5 = time.perf_counter()
+    ph = h.run_stats()
+    wall = {"partition": (t1 - t0) * 1e3, "construct": (t2 - t1) * 1e3, "contract": (t3 - t2) * 1e3,
+            "cdbg": (t4 - t3) * 1e3, "copy_to_host": (t5 - t4) * 1e3}
+
+    # kernel-only timing of the main construct launch (direct launch, events
+    # on the library's stream)
     h.set_tuning(use_graph=0)
     h.partition_device(dev[0].value, inp.n_bases, dev[1].value, inp.n_frag)
     h.construct("learned")
@@ -165,92 +252,79 @@ def main():
     n_sk_m = int(stk["n_records_walked"][main].sum())
     n_km_m = int(stk["n_kmers_walked"][main].sum())
     n_e_m = n_km_m - n_sk_m
-    kbytes = algorithmic_bytes(args.k, args.l, n_sk_m, n_e_m, n_km_m)
+    kbytes = algorithmic_bytes(k, l, n_sk_m, n_e_m, n_km_m)
     kms = rk["t_construct_kernel_ms"]
     achieved = kbytes / (kms / 1000.0) / 1e9 if kms > 0 else 0.0
-    pbytes = inp.n_bases / 4 + rs["n_records"] * (4 + 8 * (((2 * (args.k - 1) - args.l + 2) + 31) // 32))
-    part_ms = rs["t_minimizer_ms"] + rs["t_count_ms"] + rs["t_emit_ms"]
-
-    # Subgraph::contract of every subgraph on the tables just built (the first
-    # call sizes its output buffers)
-    log("contract")
-    h.contract()
-    ct = []
-    for _ in range(3):
-        h.contract()
-        ct.append(h.run_stats()["t_contract_ms"])
-    rc = h.run_stats()
-    cst = h.contract_stats()
-    ct_ms = sorted(ct)[len(ct) // 2]
-    # the compacted graph: lm-tigs glued on the device (the first call sizes its
-    # buffers), then the maximal unitigs copied to the host
-    h.cdbg_device(decode=False)
-    cd = []
-    for _ in range(3):
-        h.cdbg_device(decode=False)
-        cd.append(h.run_stats()["t_cdbg_ms"])
-    rcd = h.run_stats()
-    cdbg_ms = sorted(cd)[len(cd) // 2]
-    hip.sync()
-    t0 = time.perf_counter()
-    n_cdbg, cdbg_bytes = rcd["n_cdbg_unitigs"], rcd["n_cdbg_bytes"]
-    seq_buf = ctypes.create_string_buffer(max(cdbg_bytes, 1))
-    rec_buf = np.zeros(max(2 * n_cdbg, 2), np.uint64)
-    C.lib().cfgpu_cdbg_copy(h._h, seq_buf, cdbg_bytes, rec_buf.ctypes.data_as(C.u64p), n_cdbg, None, None)
-    copy_ms = (time.perf_counter() - t0) * 1000.0
-    del seq_buf, rec_buf
-    contract = {"ms": ct_ms, "unitigs": rc["n_unitigs"], "label_bytes": rc["n_unitig_bytes"],
-                "global_table_subgraphs": rc["n_contract_global"], "runs": rc["n_contract_runs"],
-                "isolated": int(cst[:, 0].sum()), "phantom": int(cst[:, 1].sum()),
-                "trivial": int(cst[:, 2].sum()), "cycles": int(cst[:, 3].sum()),
-                "lmtigs": int(cst[:, 4].sum()), "vertices_in_unitigs": int(cst[:, 5].sum()),
-                "cdbg": {"maximal_unitigs": n_cdbg, "label_bytes": cdbg_bytes, "device_ms": cdbg_ms,
-                         "rounds": rcd["n_cdbg_rounds"], "cycle_pieces_on_host": rcd["n_cdbg_cycle_pieces"],
-                         "copy_to_host_ms": copy_ms}}
-
-    # host buffers in (the packed reads cross PCIe inside the step)
-    hip.sync()
-    t0 = time.perf_counter()
-    h.partition(inp)
-    h.construct("learned")
-    hip.sync()
-    ms_host = (time.perf_counter() - t0) * 1000.0
+    W = ((2 * (k - 1) - l + 2) + 31) // 32
+    pbytes = inp.n_bases / 4 + rs["n_records"] * (4 + 8 * W)
+    part_ms = ph["t_partition_ms"]
 
     ms_plain = ms_batched = float("nan")
-    rs_plain = {"n_overflowed": None, "n_fallback_rounds": None}
+    rs_plain = {"n_overflowed": None, "n_fallback_rounds": None, "t_construct_ms": float("nan")}
+    rs_batched = {"t_construct_ms": float("nan")}
     same_plain = same_batched = None
+    learned_construct = wall["construct"]  # host clock, as the comparison paths below
     if args.compare_steps > 0:
+        # steady state: one partition, then per mode one untimed construct
+        # (its table pools are allocated) and the timed ones
         log("plain and batched comparison")
-        ms_plain = timed_steps(hip, h, dev, inp, "plain", args.compare_steps)
-        rs_plain = h.run_stats()
-        same_plain = bool((h.bucket_stats()["n_vertices"] == n_vert_learned).all())
-        ms_batched = timed_steps(hip, h, dev, inp, "batched", args.compare_steps)
-        same_batched = bool((h.bucket_stats()["n_vertices"] == n_vert_learned).all())
+        h.partition_device(dev[0].value, inp.n_bases, dev[1].value, inp.n_frag)
+        for mode in ("batched", "plain"):
+            log(f"construct {mode}")
+            h.construct(mode)
+            hip.sync()
+            t0 = time.perf_counter()
+            for _ in range(args.compare_steps):
+                h.construct(mode)
+            hip.sync()
+            msm = (time.perf_counter() - t0) * 1e3 / args.compare_steps
+            r = h.run_stats()
+            same = bool((h.bucket_stats()["n_vertices"] == n_vert_learned).all())
+            if mode == "plain":
+                ms_plain, rs_plain, same_plain = msm, r, same
+            else:
+                ms_batched, rs_batched, same_batched = msm, r, same
+        # the learned construct on the same partition, same clock
+        h.construct("learned")
+        hip.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.compare_steps):
+            h.construct("learned")
+        hip.sync()
+        learned_construct = (time.perf_counter() - t0) * 1e3 / args.compare_steps
 
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(REPO, "tests"))
         import oracle_lib as O
+        # the GPU box grants one GPU's share of its host cores: 16
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        nr = min(args.cpu_sample_reads, args.reads)
-        sample = reads[: nr * args.read_len]
-        log(f"cpu baseline: {nr} reads on {threads} threads")
-        t0 = time.perf_counter()
-        r = O.pipeline(sample, rec_off[: nr + 1], args.k, args.l, threads)
-        dt = time.perf_counter() - t0
+        nr = args.cpu_sample_reads or min(w["reads"], 2_000_000)
+        sample = reads[: nr * w["read_len"]]
+        log(f"cpu baseline: {nr} reads on {threads} threads, median of 3")
+        runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            r = O.pipeline(sample, rec_off[: nr + 1], k, l, threads)
+            runs.append((time.perf_counter() - t0, r))
+        runs.sort(key=lambda x: x[0])
+        dt, r = runs[1]
         cpu = {"value": r["n_kmers"] / dt, "unit": "k-mers/s", "cores": threads, "kind": "port",
-               "sample": f"first {nr} of the {args.reads} reads ({r['n_kmers']} k-mer instances): "
+               "cpu_model": cpu_model(),
+               "sample": f"first {nr} of the {w['reads']} reads ({r['n_kmers']} k-mer instances), median of 3: "
                          f"oracle partition + bucketize + per-subgraph construct, "
-                         f"{r['partition_ms']:.0f}/{r['bucketize_ms']:.0f}/{r['construct_ms']:.0f} ms"}
+                         f"{r['partition_ms']:.0f}/{r['bucketize_ms']:.0f}/{r['construct_ms']:.0f} ms "
+                         "(contract and gluing not included)"}
 
     hip.free(dev[0])
     hip.free(dev[1])
     h.close()
-    traffic = measured_traffic()
-    default_workload = (args.genome, args.reads, args.read_len, args.sub_rate, args.k, args.l, args.cutoff,
-                        args.seed) == (50_000_000, 10_000_000, 150, 0.005, 31, 20, 2, 2026)
-    out = {
-        "metric": "k-mers/sec to per-subgraph vertex states (partition + construct), batched+learned",
+    traffic_path = os.path.join(REPO, "profiles", w["traffic"])
+    traffic = measured_traffic(traffic_path, w["roofline_kernel"])
+    other_ms = lambda c: part_ms + c + ph["t_contract_ms"] + ph["t_cdbg_ms"] + wall["copy_to_host"]  # noqa: E731
+    res = {
+        "metric": "k-mers/sec to bit-exact cdBG (partition + construct + contract + gluing + unitigs copied to "
+                  "host), batched+learned",
         "value": value,
         "unit": "k-mers/s",
         "n_gpus": 1,
@@ -262,58 +336,64 @@ def main():
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": f"reads: {args.reads} x {args.read_len} bp from a {args.genome} bp iid genome "
-                               f"(both strands, {args.sub_rate * 100:.1f}% substitutions), k={args.k}, "
-                               f"l={args.l}, cutoff={args.cutoff}, 16384 subgraphs",
-                   "n_kmers": n_km, "n_records": rs["n_records"], "n_vertices": rs["n_vertices"]},
+        "config": {"workload": f"{args.workload}: {w['reads']} x {w['read_len']} bp reads from a {w['genome']} bp "
+                               f"genome ({w['repeat_frac'] * 100:.0f}% repeats, both strands, "
+                               f"{w['sub_rate'] * 100:.1f}% substitutions), k={k}, l={l}, cutoff={cutoff}, "
+                               "16384 subgraphs",
+                   "n_kmers": n_km, "n_records": rs["n_records"], "n_vertices": rs["n_vertices"],
+                   "maximal_unitigs": n_cdbg, "unitig_bytes": cdbg_bytes},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic if default_workload else None,
-                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, REPO) if traffic and default_workload else None,
-                     "kernel": "construct main round (k_construct_lds, plus k_construct for buckets "
-                               "too large for LDS passes)", "kernel_ms": kms,
+                     "traffic": traffic,
+                     "traffic_source": os.path.relpath(traffic_path, REPO) if traffic else None,
+                     "kernel": f"construct main round ({w['roofline_kernel']}: hash-insert of every walked "
+                               "k-mer instance in LDS pass tables)", "kernel_ms": kms,
                      "algorithmic_bytes": kbytes},
         "cpu_baseline": cpu,
+        "device_memory_gib": mem_gib,
+        "phases_ms": {"partition": part_ms, "minimizer": ph["t_minimizer_ms"], "count": ph["t_count_ms"],
+                      "emit": ph["t_emit_ms"], "features": ph["t_features_ms"], "dedup": ph["t_dedup_ms"],
+                      "construct": ph["t_construct_ms"], "sample": ph["t_sample_ms"], "train": ph["t_train_ms"],
+                      "infer": ph["t_infer_ms"], "construct_main": ph["t_construct_main_ms"],
+                      "fallback": ph["t_fallback_ms"], "contract": ph["t_contract_ms"], "cdbg": ph["t_cdbg_ms"],
+                      "copy_to_host": wall["copy_to_host"], "wall": wall},
         "paths": {
-            "batched_learned": {"ms_per_step": ms, "kmers_per_s": value},
-            "plain": {"ms_per_step": ms_plain, "kmers_per_s": n_km / (ms_plain / 1000.0),
-                      "same_vertex_counts": same_plain, "overflowed": rs_plain["n_overflowed"],
-                      "retry_rounds": rs_plain["n_fallback_rounds"]},
-            "batched_upper_bound": {"ms_per_step": ms_batched, "kmers_per_s": n_km / (ms_batched / 1000.0),
-                                    "same_vertex_counts": same_batched},
-            "speedup_vs_plain": ms_plain / ms,
-            "with_contract": {"ms_per_step": ms + ct_ms, "kmers_per_s": n_km / ((ms + ct_ms) / 1000.0),
-                              "note": "the step plus Subgraph::contract of every subgraph (unitig labels on "
-                                      "the device)"},
-            "to_cdbg": {"ms_per_step": ms + ct_ms + cdbg_ms,
-                        "kmers_per_s": n_km / ((ms + ct_ms + cdbg_ms) / 1000.0),
-                        "with_copy_to_host_ms": ms + ct_ms + cdbg_ms + copy_ms,
-                        "note": "the step, contract and the gluing of lm-tigs into every maximal unitig of the "
-                                "graph, all on the device (the copy of the unitigs to the host apart)"},
-            "host_input": {"ms_per_step": ms_host, "kmers_per_s": n_km / (ms_host / 1000.0),
-                           "note": "cfgpu_partition from host buffers: includes the H2D copy of the packed reads"},
+            "vertex_states": {"ms": part_ms + learned_construct,
+                              "kmers_per_s": n_km / ((part_ms + learned_construct) / 1000.0),
+                              "note": "partition + construct only (per-subgraph vertex states)"},
+            "construct_learned_ms": learned_construct,
+            "construct_batched_upper_bound_ms": ms_batched,
+            "construct_plain_ms": ms_plain,
+            "to_cdbg_batched_upper_bound_ms": other_ms(ms_batched),
+            "to_cdbg_plain_ms": other_ms(ms_plain),
+            "speedup_vs_batched_upper_bound": ms_batched / learned_construct,
+            "speedup_vs_plain": ms_plain / learned_construct,
+            "same_vertex_counts": {"plain": same_plain, "batched": same_batched},
+            "plain_overflowed": rs_plain["n_overflowed"], "plain_retry_rounds": rs_plain["n_fallback_rounds"],
+            "note": "construct-only times; the comparison the 1.5x bar reads is learned vs batched with "
+                    "upper-bound sizing (the same persistent LDS kernels); plain is one launch per bucket "
+                    "with global-atomic tables grown and retried",
         },
-        "phases_ms": {"partition": rs["t_partition_ms"], "minimizer": rs["t_minimizer_ms"],
-                      "count": rs["t_count_ms"], "emit": rs["t_emit_ms"], "features": rs["t_features_ms"],
-                      "construct": rs["t_construct_ms"], "sample": rs["t_sample_ms"],
-                      "train": rs["t_train_ms"], "infer": rs["t_infer_ms"],
-                      "construct_main": rs["t_construct_main_ms"], "fallback": rs["t_fallback_ms"],
-                      "dedup": rs["t_dedup_ms"]},
-        "contract": contract,
+        "contract": {"unitigs": ph["n_unitigs"], "label_bytes": ph["n_unitig_bytes"],
+                     "dense_table_subgraphs": ph["n_contract_dense"],
+                     "global_table_subgraphs": ph["n_contract_global"], "runs": ph["n_contract_runs"],
+                     "isolated": int(cst[:, 0].sum()), "phantom": int(cst[:, 1].sum()),
+                     "trivial": int(cst[:, 2].sum()), "cycles": int(cst[:, 3].sum()),
+                     "lmtigs": int(cst[:, 4].sum()), "vertices_in_unitigs": int(cst[:, 5].sum()),
+                     "cdbg_rounds": ph["n_cdbg_rounds"], "cycle_pieces_on_host": ph["n_cdbg_cycle_pieces"]},
         "partition_roofline": {"achieved": pbytes / (part_ms / 1000.0) / 1e9 if part_ms else 0,
                                "unit": "GB/s", "algorithmic_bytes": pbytes},
         "learned": {"samples": rs["n_samples"], "mlp_loss": rs["mlp_loss"],
                     "mispredict_mean_abs_log": rs["mispredict_mean_abs_log"],
                     "mispredict_under_frac": rs["mispredict_under_frac"],
                     "fallback_rate": rs["fallback_rate"], "overflowed": rs["n_overflowed"],
-                    "lds_buckets": rs["n_lds_buckets"],
-                    "records_walked": rs["n_records_walked"], "dedup_ms": rs["t_dedup_ms"],
-                    "dedup_fallback_buckets": rs["n_dedup_fallback"],
+                    "lds_buckets": rs["n_lds_buckets"], "lds_all_pass": rs["n_lds_all_pass"],
+                    "records_walked": rs["n_records_walked"], "dedup_fallback_buckets": rs["n_dedup_fallback"],
                     "predictor": {0: "none", 1: "mlp", 2: "constant ratio"}.get(rs["predictor"]),
                     "holdout_err_mlp": rs["holdout_err_mlp"], "holdout_err_const": rs["holdout_err_const"],
                     "table_bytes": rs["table_bytes"]},
     }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

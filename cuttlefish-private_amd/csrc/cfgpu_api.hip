@@ -214,6 +214,44 @@ hipError_t sync_stream(cfgpu_handle* h) {
 
 namespace {
 
+// Memory pressure: an allocation that runs out of device memory releases
+// the buffers no later step of the current call reads and is tried once
+// more.  Partition scratch is dead once the records are bucketed (the next
+// cfgpu_partition allocates it again), pools past the current construct's
+// rounds hold no live table, and each later step makes the scratch of the
+// earlier ones dead.
+enum class Phase { Construct, Contract, Cdbg };
+
+void trim(cfgpu_handle* h, Phase ph) {
+    for (DevBuf* b : {&h->gbin, &h->mbn, &h->tile_lc, &h->pdesc, &h->tile_n, &h->tile_frag, &h->hist, &h->rec1,
+                      &h->ex_keys, &h->ex_ef, &h->ex_st, &h->ex_dfa})
+        b->release();
+    for (size_t i = h->round_idx; i < h->pools.size(); ++i) h->pools[i]->release();
+    if (ph != Phase::Cdbg)
+        for (DevBuf* b : {&h->cd_hkey, &h->cd_hval, &h->cd_eslot, &h->cd_kend, &h->cd_jp, &h->cd_head, &h->cd_pos,
+                          &h->cd_hoff, &h->cd_out, &h->cd_rec})
+            b->release();
+    if (ph != Phase::Construct)
+        for (DevBuf* b : {&h->spill_key, &h->spill_meta, &h->dd_att, &h->dd_lab, &h->dd_mult}) b->release();
+    if (ph == Phase::Cdbg) h->ct_scratch.release();
+}
+
+hipError_t ensure_mem(cfgpu_handle* h, DevBuf& b, size_t bytes, Phase ph) {
+    hipError_t e = b.ensure(bytes);
+    if (e != hipErrorOutOfMemory) return e;
+    (void)hipGetLastError();  // the failed allocation is handled here
+    e = sync_stream(h);       // nothing queued may still use what is released
+    if (e != hipSuccess) return e;
+    trim(h, ph);
+    e = b.ensure(bytes);
+    if (e == hipErrorOutOfMemory) (void)hipGetLastError();
+    return e;
+}
+
+}  // namespace
+
+namespace {
+
 int fail(cfgpu_handle* h, int code, const char* fmt, ...) {
     if (h) {
         h->stage_pending.clear();  // staged copies whose host arrays the failing call leaves
@@ -322,7 +360,10 @@ int prepare_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
     for (uint64_t c : caps) n_slots += c;
     if (h->pools.size() <= h->round_idx) h->pools.emplace_back(new DevBuf());
     DevBuf* pool = h->pools[h->round_idx++].get();
-    HIPCK(h, pool->ensure(n_slots * eb));
+    // a pool much larger than this round needs (left by another mode) is
+    // given back first, so that the rounds of one call fit together
+    if (pool->n > 2 * n_slots * eb + (size_t(1) << 30)) pool->release();
+    HIPCK(h, ensure_mem(h, *pool, n_slots * eb, Phase::Construct));
     uint64_t off = 0;
     std::vector<uint32_t> ib, ifirst, icnt;
     ib.reserve(buckets.size());
@@ -453,8 +494,8 @@ int run_round_batched(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(n_items, uint32_t(h->cus) * per_cu));
     if (lds && h->spill_need) {  // one region per workgroup, sized for the largest all-pass item
         const uint64_t cap = (h->spill_need + 63) & ~63ull;
-        HIPCK(h, h->spill_key.ensure(uint64_t(grid) * cap * h->KW * 8));
-        HIPCK(h, h->spill_meta.ensure(uint64_t(grid) * cap * 4));
+        HIPCK(h, ensure_mem(h, h->spill_key, uint64_t(grid) * cap * h->KW * 8, Phase::Construct));
+        HIPCK(h, ensure_mem(h, h->spill_meta, uint64_t(grid) * cap * 4, Phase::Construct));
         a.spill_key = h->spill_key.as<uint64_t>();
         a.spill_meta = h->spill_meta.as<uint32_t>();
         a.spill_cap = uint32_t(cap);
@@ -650,9 +691,9 @@ int dedup_buckets(cfgpu_handle* h, const std::vector<uint32_t>& all) {
     if (!h->tune.dedup || all.empty()) return CFGPU_OK;
     const size_t G = h->G;
     const uint64_t n = std::max<uint64_t>(h->n_records, 1);
-    HIPCK(h, h->dd_att.ensure(n * 4));
-    HIPCK(h, h->dd_lab.ensure(n * h->W * 8));
-    HIPCK(h, h->dd_mult.ensure(n * 4));
+    HIPCK(h, ensure_mem(h, h->dd_att, n * 4, Phase::Construct));
+    HIPCK(h, ensure_mem(h, h->dd_lab, n * h->W * 8, Phase::Construct));
+    HIPCK(h, ensure_mem(h, h->dd_mult, n * 4, Phase::Construct));
     HIPCK(h, h->dd_cnt.ensure(G * 4));
     HIPCK(h, h->dd_km.ensure(G * 8));
     HIPCK(h, h->dd_end.ensure(G * 8));
@@ -1239,7 +1280,7 @@ int cfgpu_contract(cfgpu_handle* h) {
     HIPCK(h, h->ct_ovf2.ensure(n_list * 4ull));
     HIPCK(h, h->ct_counts.ensure(size_t(h->G) * sizeof(cfgpu_contract_stat)));
     HIPCK(h, h->ct_ctl.ensure(64));
-    HIPCK(h, h->ct_scratch.ensure(per_wg * std::max<uint32_t>(grid, 1)));
+    HIPCK(h, ensure_mem(h, h->ct_scratch, per_wg * std::max<uint32_t>(grid, 1), Phase::Contract));
     if (n_list) HIPCK(h, h2d(h, h->ct_list.p, list.data(), n_list * 4ull));
     // Output sized from the vertex count (most vertices of read sets are
     // isolated) or the last call's exact need, whichever is larger; re-run
@@ -1252,8 +1293,8 @@ int cfgpu_contract(cfgpu_handle* h) {
     uint32_t run = 0;
     for (;;) {
         ++run;
-        HIPCK(h, h->ct_recs.ensure(rec_cap * sizeof(cfgpu_unitig)));
-        HIPCK(h, h->ct_labels.ensure(lab_cap + 64));  // slack: labels are read as aligned words
+        HIPCK(h, ensure_mem(h, h->ct_recs, rec_cap * sizeof(cfgpu_unitig), Phase::Contract));
+        HIPCK(h, ensure_mem(h, h->ct_labels, lab_cap + 64, Phase::Contract));  // slack: labels are read as aligned words
         launch_fill_u32(h->ct_counts.as<uint32_t>(), 0u, uint64_t(h->G) * 8, h->stream);
         launch_fill_u32(h->ct_ctl.as<uint32_t>(), 0u, 16, h->stream);
         ContractArgs a{};
@@ -1327,15 +1368,15 @@ int cfgpu_cdbg(cfgpu_handle* h) {
     if (n) {
         if (2 * n >= (1ull << 30)) return fail(h, CFGPU_E_ARG, "cdbg: %llu unitigs", (unsigned long long)n);
         const uint64_t hcap = round_cap(4.0 * double(n));
-        HIPCK(h, h->cd_hkey.ensure(hcap * KW * 8));
-        HIPCK(h, h->cd_hval.ensure(hcap * 8));
-        HIPCK(h, h->cd_eslot.ensure(2 * n * 4));
-        HIPCK(h, h->cd_kend.ensure(2 * n * KW * 8));
-        HIPCK(h, h->cd_jp.ensure(2 * n * 8));
+        HIPCK(h, ensure_mem(h, h->cd_hkey, hcap * KW * 8, Phase::Cdbg));
+        HIPCK(h, ensure_mem(h, h->cd_hval, hcap * 8, Phase::Cdbg));
+        HIPCK(h, ensure_mem(h, h->cd_eslot, 2 * n * 4, Phase::Cdbg));
+        HIPCK(h, ensure_mem(h, h->cd_kend, 2 * n * KW * 8, Phase::Cdbg));
+        HIPCK(h, ensure_mem(h, h->cd_jp, 2 * n * 8, Phase::Cdbg));
         HIPCK(h, h->cd_head.ensure(n * 4));
         HIPCK(h, h->cd_pos.ensure(n * 8));
         HIPCK(h, h->cd_hoff.ensure(n * 8));
-        HIPCK(h, h->cd_out.ensure(h->n_ubytes));  // glued labels share k characters: never longer
+        HIPCK(h, ensure_mem(h, h->cd_out, h->n_ubytes, Phase::Cdbg));  // glued labels share k characters: never longer
         HIPCK(h, h->cd_rec.ensure(n * 16));
         HIPCK(h, h->cd_ctl.ensure(64));
         HIPCK(h, hipMemsetAsync(h->cd_hkey.p, 0xFF, hcap * KW * 8, h->stream));

@@ -71,12 +71,16 @@ struct Probe {
 // stale line: a slot's key only ever changes from EMPTY to a key, and counters
 // only grow, so a stale read can only show EMPTY (the CAS then returns the
 // truth) or a smaller counter word (the counter CAS then fails and retries).
+// A long chain polls the bucket's overflow flag: once the table is known to
+// be too small its contents are discarded, so the search gives up.
+constexpr uint32_t POLL_PROBES = 64;
 template <int KW>
-__device__ Probe find_or_insert(g_u8* tbl, uint64_t cap, const Kmer<KW>& key) {
+__device__ Probe find_or_insert(g_u8* tbl, uint64_t cap, const Kmer<KW>& key, const uint32_t* ovf) {
     uint64_t s = home_slot(km_hash(key), cap);
     const uint64_t lim = cap < MAX_PROBES ? cap : MAX_PROBES;
     if constexpr (KW == 1) {
         for (uint64_t n = 0; n < lim; ++n) {
+            if (n % POLL_PROBES == POLL_PROBES - 1 && ld_relaxed32(ovf)) return {-1, 0u, false};
             g_u64* kp = slot_ptr<1>(tbl, s);
             const u32x4 e = *reinterpret_cast<g_u32x4*>(kp);  // key, e_f, status
             const uint64_t cur = uint64_t(e.x) | (uint64_t(e.y) << 32);
@@ -122,6 +126,7 @@ __device__ Probe find_or_insert(g_u8* tbl, uint64_t cap, const Kmer<KW>& key) {
                 if (eq) return {int64_t(s), uint32_t(kp[KW]), false};
             }
             ++n;
+            if (n % POLL_PROBES == 0 && ld_relaxed32(ovf)) return {-1, 0u, false};
             s = s + 1 == cap ? 0 : s + 1;
         }
         return {-1, 0u, false};
@@ -150,8 +155,10 @@ __device__ void construct_item(const ConstructArgs& a, uint32_t item) {
         rw.load(a.att, a.labels, rbase + r, W, k);
         const uint32_t m = a.mult ? a.mult[rbase + r] : 1u;
         for (uint32_t t = 0; t < rw.n; ++t) {
+            // a table flagged too small is discarded: stop adding to it
+            if (t % 16 == 15 && ld_relaxed32(&a.overflow[b])) break;
             const Inst<KW> in = rw.next();
-            const Probe pr = find_or_insert<KW>(tbl, cap, in.key);
+            const Probe pr = find_or_insert<KW>(tbl, cap, in.key, &a.overflow[b]);
             if (pr.slot < 0) {
                 atomicOr(&a.overflow[b], 1u);
             } else {
