@@ -80,6 +80,18 @@ struct Event {
 
 }  // namespace
 
+// Scratch of cfgpu_table_copy_noalloc: the per-bucket output offsets, the
+// export cursors and the bucket list, for at most NOALLOC_MAX_BUCKETS buckets.
+// One array per GPU, held until the process ends and shared by every handle on
+// that GPU; host code uses the device address hipGetSymbolAddress returns
+// (noalloc_scratch in the handle), never the array's name.
+constexpr uint32_t NOALLOC_MAX_BUCKETS = 16384;
+constexpr size_t NOALLOC_OFF_BYTES = (size_t(NOALLOC_MAX_BUCKETS) + 1) * 8;
+constexpr size_t NOALLOC_CURSOR_AT = (NOALLOC_OFF_BYTES + 255) & ~size_t(255);
+constexpr size_t NOALLOC_LIST_AT = NOALLOC_CURSOR_AT + size_t(NOALLOC_MAX_BUCKETS) * 4;
+constexpr size_t NOALLOC_SCRATCH_BYTES = NOALLOC_LIST_AT + size_t(NOALLOC_MAX_BUCKETS) * 4;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
 struct cfgpu_handle {
     cfgpu_params prm{};
     uint32_t KW = 1, W = 1, w = 1, G = G_FIXED, g_bits = 14;
@@ -152,6 +164,12 @@ struct cfgpu_handle {
     std::vector<uint64_t> graph_key;
 
     cfgpu_run_stats rs{};
+
+    // cfgpu_table_copy_noalloc: the device address of this GPU's `scratch`
+    // array and host arrays sized at open (that call allocates nothing)
+    uint8_t* noalloc_scratch = nullptr;
+    std::vector<uint64_t> na_off;
+    std::vector<uint32_t> na_list, na_cnt;
 
     ~cfgpu_handle() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -1016,6 +1034,13 @@ int cfgpu_open(const cfgpu_params* p, cfgpu_handle** out) {
     if (hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess)
         h->cus = 256;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return CFGPU_E_HIP;
+    // this device's copy of the no-alloc scratch array (the device is current)
+    if (hipGetSymbolAddress(reinterpret_cast<void**>(&h->noalloc_scratch), HIP_SYMBOL(scratch)) != hipSuccess ||
+        !h->noalloc_scratch)
+        return CFGPU_E_HIP;
+    h->na_off.resize(NOALLOC_MAX_BUCKETS + 1);
+    h->na_list.resize(NOALLOC_MAX_BUCKETS);
+    h->na_cnt.resize(NOALLOC_MAX_BUCKETS);
     *out = h.release();
     return CFGPU_OK;
 }
@@ -1616,6 +1641,59 @@ int cfgpu_table_copy(cfgpu_handle* h, const uint32_t* buckets, uint32_t n_bucket
         if (c != off[i + 1] - off[i])
             return fail(h, CFGPU_E_STATE, "table copy: bucket %u exported %u of %llu entries", buckets[i], c,
                         (unsigned long long)(off[i + 1] - off[i]));
+        if (bucket_n) bucket_n[i] = c;
+    }
+    return CFGPU_OK;
+}
+
+int cfgpu_table_copy_noalloc(cfgpu_handle* h, const uint32_t* buckets, uint32_t n_buckets, uint64_t* keys,
+                             uint32_t* e_f, uint8_t* status, uint8_t* dfa, uint32_t* bucket_n, uint64_t cap,
+                             uint64_t* n_out) {
+    if (!h || (n_buckets && !buckets)) return CFGPU_E_ARG;
+    if (!h->constructed) return fail(h, CFGPU_E_STATE, "table copy before construct");
+    if (n_buckets > NOALLOC_MAX_BUCKETS)
+        return fail(h, CFGPU_E_ARG, "table copy (no alloc): %u buckets, at most %u", n_buckets, NOALLOC_MAX_BUCKETS);
+    if (!keys || !e_f || !status || !dfa) return fail(h, CFGPU_E_ARG, "table copy (no alloc): null output");
+    HIPCK(h, hipSetDevice(h->prm.device));
+    // offsets of the listed buckets that have a table, back to back
+    uint64_t total = 0;
+    uint32_t m = 0;
+    for (uint32_t i = 0; i < n_buckets; ++i) {
+        const uint32_t b = buckets[i];
+        if (b >= h->G) return fail(h, CFGPU_E_ARG, "table copy: bucket %u out of range", b);
+        if (!h->h_tbl_ptr[b]) continue;
+        h->na_list[m] = b;
+        h->na_off[m] = total;
+        total += h->h_nvert[b];
+        ++m;
+    }
+    h->na_off[m] = total;
+    if (n_out) *n_out = total;
+    if (total > cap) return fail(h, CFGPU_E_ARG, "table copy: capacity %llu < %llu", (unsigned long long)cap,
+                                 (unsigned long long)total);
+    uint64_t* d_off = reinterpret_cast<uint64_t*>(h->noalloc_scratch);
+    uint32_t* d_cursor = reinterpret_cast<uint32_t*>(h->noalloc_scratch + NOALLOC_CURSOR_AT);
+    uint32_t* d_list = reinterpret_cast<uint32_t*>(h->noalloc_scratch + NOALLOC_LIST_AT);
+    if (m) {
+        HIPCK(h, hipMemcpyAsync(d_off, h->na_off.data(), (m + 1) * 8ull, hipMemcpyHostToDevice, h->stream));
+        HIPCK(h, hipMemcpyAsync(d_list, h->na_list.data(), m * 4ull, hipMemcpyHostToDevice, h->stream));
+        launch_fill_u32(d_cursor, 0u, m, h->stream);  // a fill kernel: the cursors start at zero
+        launch_export(h->d_tbl_ptr.as<uint64_t>(), h->d_cap.as<uint64_t>(), h->KW, d_list, m, d_off, d_cursor, keys,
+                      e_f, status, dfa, h->prm.cutoff, h->stream);
+        HIPCK(h, hipGetLastError());
+        HIPCK(h, hipMemcpyAsync(h->na_cnt.data(), d_cursor, m * 4ull, hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCK(h, hipStreamSynchronize(h->stream));
+    for (uint32_t i = 0, j = 0; i < n_buckets; ++i) {
+        const uint32_t b = buckets[i];
+        uint32_t c = 0;
+        if (h->h_tbl_ptr[b]) {
+            c = h->na_cnt[j];
+            if (c != h->na_off[j + 1] - h->na_off[j])
+                return fail(h, CFGPU_E_STATE, "table copy: bucket %u exported %u of %llu entries", b, c,
+                            (unsigned long long)(h->na_off[j + 1] - h->na_off[j]));
+            ++j;
+        }
         if (bucket_n) bucket_n[i] = c;
     }
     return CFGPU_OK;

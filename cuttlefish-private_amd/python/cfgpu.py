@@ -114,6 +114,9 @@ SIGNATURES = {
                                             ctypes.POINTER(TableView)]),
     "cfgpu_table_copy": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_uint32, u64p, u32p, u8p,
                                         u8p, u32p, ctypes.c_uint64, u64p]),
+    "cfgpu_table_copy_noalloc": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_uint32, ctypes.c_void_p,
+                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, u32p,
+                                                ctypes.c_uint64, u64p]),
     "cfgpu_release": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     "cfgpu_contract": (ctypes.c_int, [ctypes.c_void_p]),
     "cfgpu_contract_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ContractStat),

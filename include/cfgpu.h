@@ -151,6 +151,19 @@ int cfgpu_table_copy(cfgpu_handle* h, const uint32_t* buckets, uint32_t n_bucket
                      uint64_t* keys, uint32_t* e_f, uint8_t* status, uint8_t* dfa,
                      uint32_t* bucket_n, uint64_t cap, uint64_t* n_out);
 
+/* As cfgpu_table_copy, but every output (keys, e_f, status, dfa) is a DEVICE array of
+ * the caller's and the call allocates no memory: no hipMalloc, hipFree or allocating
+ * library routine (its bucket offsets, export cursors and bucket list live in a static
+ * device array of 262,400 bytes per GPU, kept until the process ends).  No counterpart in the
+ * reference: for callers that cannot allocate (code under stream capture, serving loops,
+ * harnesses that forbid it).  n_buckets is at most 16384; bucket_n and n_out are host
+ * pointers (may be NULL).  All four outputs are required.  Every handle on one GPU shares
+ * that GPU's scratch array: calls that overlap on one GPU (other host threads or
+ * handles) race. */
+int cfgpu_table_copy_noalloc(cfgpu_handle* h, const uint32_t* buckets, uint32_t n_buckets, uint64_t* keys,
+                             uint32_t* e_f, uint8_t* status, uint8_t* dfa, uint32_t* bucket_n, uint64_t cap,
+                             uint64_t* n_out);
+
 /* Releases subgraph g's table view (Super_Kmer_Bucket::remove,
  * src/Subgraphs_Manager.cpp:158, frees bucket files; here the view becomes invalid). */
 int cfgpu_release(cfgpu_handle* h, uint32_t g);
