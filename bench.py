@@ -177,6 +177,8 @@ def main():
     ap.add_argument("--cpu-sample-reads", type=int, default=0, help="reads in the CPU baseline sample (0: auto)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--max-passes", type=int, default=-1,
+                    help="tuning: LDS passes above which a bucket is built with global atomics (0: always)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="before the steps, copy this many MiB device-to-device (a known-size "
                          "copy for scaling profiler byte counters)")
@@ -195,6 +197,8 @@ def main():
     out = CdbgOut(hip)
 
     h = C.Handle(k, l, cutoff)
+    if args.max_passes >= 0:
+        h.set_tuning(max_passes=args.max_passes)
     if args.calib_copy_mib:
         log(f"calibration copy {args.calib_copy_mib} MiB")
         hip.copy_d2d(args.calib_copy_mib << 20)

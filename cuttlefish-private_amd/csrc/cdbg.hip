@@ -11,9 +11,11 @@
 // 131-190).  Every unitig record is a piece of exactly one chain (trivial
 // unitigs are chains of one piece):
 //
-//   ends    each exiting end is entered in a hash table keyed by its vertex's
-//           canonical k-mer, in the slot half of its side; the other half holds
-//           the partner end;
+//   ends    each exiting end is keyed by its vertex's canonical k-mer and
+//           moved into one of up to 8192 hash bins (a block histogram, scans,
+//           a scatter: no global atomics); one workgroup per bin pairs the
+//           ends of each vertex in an LDS table (the end on the other side
+//           of the vertex is an end's partner);
 //   rank    directed states (piece, end it is entered at) point at the next
 //           piece of the chain; pointer jumping gives every state the last state
 //           of its chain and the label characters of the pieces up to it;
@@ -40,97 +42,247 @@ constexpr uint32_t JT_BITS = 30;
 constexpr uint64_t JT_MASK = (1ull << JT_BITS) - 1, JC_MAX = (1ull << (64 - JT_BITS)) - 1;
 __device__ __forceinline__ uint64_t jword(uint64_t t, uint64_t c) { return t | (min(c, JC_MAX) << JT_BITS); }
 
-__device__ __forceinline__ uint32_t ccode(char c) { return c == 'A' ? 0u : c == 'C' ? 1u : c == 'G' ? 2u : 3u; }
-
-// k characters at s as a k-mer (Kmer layout: word 0 holds the last 32 bases),
-// f, and its reverse complement, r, in one pass.
-// The characters are read as aligned 4-byte words (the label buffer has at
-// least 4 bytes of slack after its last label).
+// k characters at s as a k-mer (Kmer layout: word 0 holds the last 32
+// bases).  The characters are read as aligned 4-byte words (the label buffer
+// has at least 4 bytes of slack after its last label) and decoded four at a
+// time: ((c >> 1) ^ (c >> 2)) & 3 maps A C G T to 0 1 2 3 in every byte.
 template <int KW>
-__device__ __forceinline__ void enc2(const char* s, uint32_t k, Kmer<KW>& f, Kmer<KW>& r) {
-#pragma unroll
-    for (int w = 0; w < KW; ++w) f.w[w] = r.w[w] = 0;
-    const uint32_t kt = k - 32 * (KW - 1);
+__device__ __forceinline__ Kmer<KW> enc_kmer(const char* s, uint32_t k) {
     const uint32_t* p = reinterpret_cast<const uint32_t*>(reinterpret_cast<uintptr_t>(s) & ~uintptr_t(3));
     const uint32_t sh = uint32_t(reinterpret_cast<uintptr_t>(s) & 3u);
     const uint32_t nw = (sh + k + 3) / 4;
-#pragma unroll 4
+    uint64_t acc[KW + 1];
+#pragma unroll
+    for (int w = 0; w <= KW; ++w) acc[w] = 0;
     for (uint32_t j = 0; j < nw; ++j) {
         const uint32_t v = p[j];
+        const uint32_t c = ((v >> 1) ^ (v >> 2)) & 0x03030303u;  // byte q: code of character 4j + q
+        const uint32_t b8 = ((c & 3u) << 6) | (((c >> 8) & 3u) << 4) | (((c >> 16) & 3u) << 2) | (c >> 24);
 #pragma unroll
-        for (uint32_t q = 0; q < 4; ++q) {
-            const uint32_t i = 4 * j + q;  // character i - sh of the k-mer
-            if (i < sh || i >= sh + k) continue;
-            const uint32_t b = ccode(char((v >> (8 * q)) & 0xFFu));
-#pragma unroll
-            for (int w = KW - 1; w > 0; --w) f.w[w] = (f.w[w] << 2) | (f.w[w - 1] >> 62);
-            f.w[0] = (f.w[0] << 2) | b;
-            r = km_prepend(r, uint8_t(3u - b), k);
-        }
+        for (int w = KW; w > 0; --w) acc[w] = (acc[w] << 8) | (acc[w - 1] >> 56);
+        acc[0] = (acc[0] << 8) | b8;
     }
+    // drop the characters read past the k-mer (fewer than 4), then the ones
+    // before it (the mask)
+    const uint32_t t = 2 * (4 * nw - sh - k);
+    Kmer<KW> f;
+#pragma unroll
+    for (int w = 0; w < KW; ++w) f.w[w] = t ? (acc[w] >> t) | (acc[w + 1] << (64 - t)) : acc[w];
+    const uint32_t kt = k - 32 * (KW - 1);
     f.w[KW - 1] &= (1ull << (2 * kt)) - 1;
+    return f;
 }
 
-__device__ __forceinline__ uint32_t slot_of(uint64_t h, uint64_t cap) {
-    return uint32_t((uint64_t(uint32_t(h >> 32)) * cap) >> 32);
+// The vertex and side of piece r's end `at` (0 first k-mer, 1 last), from
+// kend: the first k characters (at 0) or the reverse complement of the last
+// k (at 1).  The side is the one facing the label: first k-mer -> back side
+// if canonical; last k-mer -> front side if canonical.
+template <int KW>
+__device__ __forceinline__ void end_vertex(const Kmer<KW>& e, uint32_t at, uint32_t k, Kmer<KW>& key,
+                                           uint32_t& side) {
+    const Kmer<KW> er = km_rc(e, k);  // at 1: the last k characters themselves
+    if (at == 0) {
+        const bool can = km_less(e, er);
+        key = can ? e : er;
+        side = can ? 1u : 0u;
+    } else {
+        const bool can = km_less(er, e);
+        key = can ? er : e;
+        side = can ? 0u : 1u;
+    }
+}
+
+__device__ __forceinline__ uint32_t bin_of(uint64_t h, uint32_t nb_bits) {
+    return nb_bits ? uint32_t(h >> (64 - nb_bits)) : 0u;
+}
+
+// Pass 1, block b owns pieces [b * per_block, (b + 1) * per_block): both ends'
+// k-mers into kend, partner[] cleared, and the block's count of exiting ends
+// per hash bin into hist[bin * n_blocks + b].
+template <int KW>
+__global__ __launch_bounds__(256) void k_ends_encode(CdbgArgs a) {
+    __shared__ uint32_t s_hist[CD_MAX_BINS];
+    Kmer<KW>* kend = reinterpret_cast<Kmer<KW>*>(a.kend);
+    const uint32_t nb = 1u << a.nb_bits;
+    for (uint32_t i = threadIdx.x; i < nb; i += 256) s_hist[i] = 0;
+    __syncthreads();
+    const uint64_t r0 = uint64_t(blockIdx.x) * a.per_block, r1 = min(a.n, r0 + a.per_block);
+    for (uint64_t r = r0 + threadIdx.x; r < r1; r += 256) {
+        const UnitigRec u = a.recs[r];
+        const char* lab = a.labels + u.label_off;
+        const Kmer<KW> A = enc_kmer<KW>(lab, a.k);
+        const Kmer<KW> B = km_rc(enc_kmer<KW>(lab + u.label_len - a.k, a.k), a.k);
+        kend[2 * r] = A;
+        kend[2 * r + 1] = B;
+        a.partner[2 * r] = a.partner[2 * r + 1] = NONE;
+        for (uint32_t at = 0; at < 2; ++at) {
+            if (!(u.flags & (at ? 4u : 2u))) continue;
+            Kmer<KW> key;
+            uint32_t side;
+            end_vertex(at ? B : A, at, a.k, key, side);
+            atomicAdd(&s_hist[bin_of(km_hash(key), a.nb_bits)], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nb; i += 256) a.bin_hist[uint64_t(i) * a.n_blocks + blockIdx.x] = s_hist[i];
+}
+
+// Per bin (one workgroup each): exclusive scan of hist[bin][*] over the
+// blocks, in place, and the bin's total.
+__global__ __launch_bounds__(1024) void k_bin_row_scan(uint32_t* hist, uint32_t n_blocks, uint32_t* tot) {
+    __shared__ uint32_t s[1024];
+    uint32_t* row = hist + uint64_t(blockIdx.x) * n_blocks;
+    const uint32_t tid = threadIdx.x;
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < n_blocks; c0 += 1024) {
+        const uint32_t i = c0 + tid;
+        const uint32_t v = i < n_blocks ? row[i] : 0u;
+        s[tid] = v;
+        __syncthreads();
+        for (uint32_t d = 1; d < 1024; d <<= 1) {
+            const uint32_t x = tid >= d ? s[tid - d] : 0u;
+            __syncthreads();
+            s[tid] += x;
+            __syncthreads();
+        }
+        if (i < n_blocks) row[i] = carry + s[tid] - v;
+        carry += s[1023];
+        __syncthreads();
+    }
+    if (tid == 0) tot[blockIdx.x] = carry;
+}
+
+// Exclusive scan of the bin totals -> off[0..nb] (one workgroup).
+__global__ __launch_bounds__(1024) void k_bin_off_scan(const uint32_t* tot, uint32_t nb, uint32_t* off) {
+    __shared__ uint32_t s[1024];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t per = (nb + 1023) / 1024;
+    uint32_t sum = 0;
+    for (uint32_t i = tid * per; i < min(nb, (tid + 1) * per); ++i) sum += tot[i];
+    s[tid] = sum;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {
+        const uint32_t x = tid >= d ? s[tid - d] : 0u;
+        __syncthreads();
+        s[tid] += x;
+        __syncthreads();
+    }
+    uint32_t run = s[tid] - sum;
+    for (uint32_t i = tid * per; i < min(nb, (tid + 1) * per); ++i) {
+        off[i] = run;
+        run += tot[i];
+    }
+    if (tid == 1023) off[nb] = s[1023];
+}
+
+// Pass 2: every exiting end (vertex key, end id << 1 | side) into its bin's
+// run, block after block (the same block ranges as pass 1).
+template <int KW>
+__global__ __launch_bounds__(256) void k_ends_scatter(CdbgArgs a) {
+    __shared__ uint32_t s_cur[CD_MAX_BINS];
+    const Kmer<KW>* kend = reinterpret_cast<const Kmer<KW>*>(a.kend);
+    Kmer<KW>* bkey = reinterpret_cast<Kmer<KW>*>(a.bin_key);
+    const uint32_t nb = 1u << a.nb_bits;
+    for (uint32_t i = threadIdx.x; i < nb; i += 256)
+        s_cur[i] = a.bin_off[i] + a.bin_hist[uint64_t(i) * a.n_blocks + blockIdx.x];
+    __syncthreads();
+    const uint64_t r0 = uint64_t(blockIdx.x) * a.per_block, r1 = min(a.n, r0 + a.per_block);
+    for (uint64_t r = r0 + threadIdx.x; r < r1; r += 256) {
+        const uint32_t fl = a.recs[r].flags;
+        for (uint32_t at = 0; at < 2; ++at) {
+            if (!(fl & (at ? 4u : 2u))) continue;
+            Kmer<KW> key;
+            uint32_t side;
+            end_vertex(kend[2 * r + at], at, a.k, key, side);
+            const uint32_t pos = atomicAdd(&s_cur[bin_of(km_hash(key), a.nb_bits)], 1u);
+            bkey[pos] = key;
+            a.bin_id[pos] = uint32_t((2 * r + at) << 1) | side;
+        }
+    }
+}
+
+// Pass 3, one workgroup per bin: the bin's ends are entered in an LDS table
+// keyed by vertex (P hash passes when the bin is large), one end id per side
+// of each vertex; then every end reads the id on the other side of its
+// vertex, its partner.  A vertex side claimed twice is an inconsistent input.
+template <int KW>
+struct PairTab {
+    static constexpr uint32_t CAP = KW <= 3 ? 4096u : 3584u;
+};
+
+template <int KW>
+__device__ __forceinline__ int pair_slot(uint64_t* s_key, const Kmer<KW>& key, uint32_t h, bool insert) {
+    constexpr uint32_t CAP = PairTab<KW>::CAP;
+    const uint64_t top = key.w[KW - 1];
+    uint32_t s = uint32_t((uint64_t(h) * CAP) >> 32);
+    for (uint32_t n = 0; n < CAP;) {
+        uint64_t* kp = &s_key[s * KW];
+        uint64_t cur = __hip_atomic_load(kp + KW - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (cur == EMPTY) {
+            if (!insert) return -1;
+            const uint64_t claim = KW == 1 ? top : (top | BUSY);
+            cur = atomicCAS(reinterpret_cast<unsigned long long*>(kp + KW - 1), EMPTY, claim);
+            if (cur == EMPTY) {
+                if constexpr (KW > 1) {
+#pragma unroll
+                    for (int w = 0; w < KW - 1; ++w)
+                        __hip_atomic_store(kp + w, key.w[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __hip_atomic_store(kp + KW - 1, top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                return int(s);
+            }
+        }
+        if (KW > 1 && cur == (top | BUSY)) continue;  // being written: the claimer publishes in its own iteration
+        if (cur == top) {
+            bool eq = true;
+#pragma unroll
+            for (int w = 0; w < KW - 1; ++w)
+                eq &= __hip_atomic_load(kp + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == key.w[w];
+            if (eq) return int(s);
+        }
+        s = s + 1 == CAP ? 0 : s + 1;
+        ++n;
+    }
+    return -1;
 }
 
 template <int KW>
-__global__ __launch_bounds__(256) void k_ends_insert(CdbgArgs a) {
-    Kmer<KW>* kend = reinterpret_cast<Kmer<KW>*>(a.kend);
-    for (uint64_t r = uint64_t(blockIdx.x) * 256 + threadIdx.x; r < a.n; r += uint64_t(gridDim.x) * 256) {
-        const UnitigRec u = a.recs[r];
-        const char* lab = a.labels + u.label_off;
-        Kmer<KW> A0, A1, B0, B1;  // first k characters and their rc; last k characters and their rc
-        enc2<KW>(lab, a.k, A0, A1);
-        enc2<KW>(lab + u.label_len - a.k, a.k, B0, B1);
-        kend[2 * r] = A0;
-        kend[2 * r + 1] = B1;
-        for (uint32_t at = 0; at < 2; ++at) {
-            a.end_slot[2 * r + at] = NONE;
-            if (!(u.flags & (at ? 4u : 2u))) continue;
-            // the end's vertex (canonical k-mer) and the side facing the label:
-            // first k-mer -> back side if canonical; last k-mer -> front side if canonical
-            const bool can = at ? km_less(B0, B1) : km_less(A0, A1);
-            const uint32_t side = at ? (can ? 0u : 1u) : (can ? 1u : 0u);
-            Kmer<KW> x;
-#pragma unroll
-            for (int w = 0; w < KW; ++w) x.w[w] = at ? (can ? B0.w[w] : B1.w[w]) : (can ? A0.w[w] : A1.w[w]);
-            uint32_t s = slot_of(km_hash(x), a.hcap);
-            // claim a slot: the top key word with BUSY set while the lower words
-            // are written (a lane that finds BUSY retries the slot; it never
-            // waits inside this iteration)
-            const uint64_t top = KW == 1 ? x.w[0] : (x.w[KW - 1] | BUSY);
-            for (uint64_t n = 0; n < 2 * a.hcap;) {
-                uint64_t* kp = &a.hkey[uint64_t(s) * KW];
-                uint64_t cur = __hip_atomic_load(kp + KW - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (cur == EMPTY) {
-                    cur = atomicCAS(reinterpret_cast<unsigned long long*>(kp + KW - 1), EMPTY, top);
-                    if (cur == EMPTY) {
-#pragma unroll
-                        for (int w = 0; w < KW - 1; ++w)
-                            __hip_atomic_store(kp + w, x.w[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        if (KW > 1)
-                            __hip_atomic_store(kp + KW - 1, x.w[KW - 1], __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                        cur = x.w[KW - 1];
-                    } else if (KW > 1) {
-                        continue;  // lost the claim: look again
-                    }
-                }
-                if (KW > 1 && cur == (x.w[KW - 1] | BUSY)) continue;  // being written: look again
-                bool eq = cur == x.w[KW - 1];
-                if (KW > 1 && eq) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // published: its lower words
-#pragma unroll
-                for (int w = 0; w < KW - 1; ++w)
-                    eq &= eq && __hip_atomic_load(kp + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == x.w[w];
-                if (eq) {
-                    if (atomicCAS(&a.hval[2 * uint64_t(s) + side], NONE, uint32_t(2 * r + at)) != NONE)
-                        atomicOr(a.error, 2u);  // two lm-tigs on one side of a vertex
-                    a.end_slot[2 * r + at] = (s << 1) | side;
-                    break;
-                }
-                s = s + 1 == a.hcap ? 0 : s + 1;
-                if (++n == a.hcap) atomicOr(a.error, 1u);
+__global__ __launch_bounds__(1024) void k_ends_pair(CdbgArgs a) {
+    constexpr uint32_t CAP = PairTab<KW>::CAP;
+    constexpr uint32_t LOAD = CAP * 3 / 5;  // ends per pass (each vertex holds one or two)
+    __shared__ uint64_t s_key[CAP * KW];
+    __shared__ uint32_t s_id[2 * CAP];
+    const Kmer<KW>* bkey = reinterpret_cast<const Kmer<KW>*>(a.bin_key);
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nb = 1u << a.nb_bits;
+    for (uint32_t bin = blockIdx.x; bin < nb; bin += gridDim.x) {
+        const uint32_t e0 = a.bin_off[bin], e1 = a.bin_off[bin + 1];
+        const uint32_t P = (e1 - e0 + LOAD - 1) / LOAD;
+        for (uint32_t p = 0; p < P; ++p) {
+            __syncthreads();  // the previous pass is done with the table
+            for (uint32_t i = tid; i < CAP; i += 1024) {
+                s_key[i * KW + KW - 1] = EMPTY;
+                s_id[2 * i] = s_id[2 * i + 1] = NONE;
+            }
+            __syncthreads();
+            for (uint32_t e = e0 + tid; e < e1; e += 1024) {
+                const Kmer<KW> key = bkey[e];
+                const uint32_t h = uint32_t(km_hash(key));
+                if (P > 1 && uint32_t((uint64_t(fmix32(h)) * P) >> 32) != p) continue;
+                const int sl = pair_slot<KW>(s_key, key, h, true);
+                const uint32_t id = a.bin_id[e];
+                if (sl < 0 || atomicCAS(&s_id[2 * sl + (id & 1u)], NONE, id >> 1) != NONE)
+                    atomicOr(a.error, sl < 0 ? 1u : 2u);  // full table / two lm-tigs on one side of a vertex
+            }
+            __syncthreads();
+            for (uint32_t e = e0 + tid; e < e1; e += 1024) {
+                const Kmer<KW> key = bkey[e];
+                const uint32_t h = uint32_t(km_hash(key));
+                if (P > 1 && uint32_t((uint64_t(fmix32(h)) * P) >> 32) != p) continue;
+                const int sl = pair_slot<KW>(s_key, key, h, false);
+                const uint32_t id = a.bin_id[e];
+                if (sl >= 0) a.partner[id >> 1] = s_id[2 * sl + 1 - (id & 1u)];
             }
         }
     }
@@ -139,15 +291,11 @@ __global__ __launch_bounds__(256) void k_ends_insert(CdbgArgs a) {
 // State 2r + e: piece r entered at end e.  Its successor is the piece across
 // the vertex at r's other end, entered at its matching end.
 // State st has no partner across the end it leaves through: a chain end.
-__device__ __forceinline__ bool rank_end(const CdbgArgs& a, uint64_t st) {
-    const uint32_t es = a.end_slot[st ^ 1];
-    return es == NONE || a.hval[2 * uint64_t(es >> 1) + (1u - (es & 1u))] == NONE;
-}
+__device__ __forceinline__ bool rank_end(const CdbgArgs& a, uint64_t st) { return a.partner[st ^ 1] == NONE; }
 
 __global__ __launch_bounds__(256) void k_rank_init(CdbgArgs a) {
     for (uint64_t st = uint64_t(blockIdx.x) * 256 + threadIdx.x; st < 2 * a.n; st += uint64_t(gridDim.x) * 256) {
-        const uint32_t es = a.end_slot[st ^ 1];  // the end it leaves through
-        const uint32_t next = es == NONE ? NONE : a.hval[2 * uint64_t(es >> 1) + (1u - (es & 1u))];
+        const uint32_t next = a.partner[st ^ 1];  // the end across the vertex it leaves through
         a.jp[st] = next == NONE ? jword(st, 0) : jword(next, a.recs[next >> 1].label_len - a.k);
     }
 }
@@ -277,7 +425,15 @@ __global__ __launch_bounds__(256) void k_cycle_list(CdbgArgs a, uint32_t* list, 
 template <int KW>
 void launch_kw(const CdbgArgs& a, uint32_t grid, int phase, uint32_t src, uint32_t* changed, hipStream_t s) {
     switch (phase) {
-        case 0: hipLaunchKernelGGL(k_ends_insert<KW>, dim3(grid), dim3(256), 0, s, a); break;
+        case 0: {
+            const uint32_t nb = 1u << a.nb_bits;
+            hipLaunchKernelGGL(k_ends_encode<KW>, dim3(a.n_blocks), dim3(256), 0, s, a);
+            hipLaunchKernelGGL(k_bin_row_scan, dim3(nb), dim3(1024), 0, s, a.bin_hist, a.n_blocks, a.bin_tot);
+            hipLaunchKernelGGL(k_bin_off_scan, dim3(1), dim3(1024), 0, s, a.bin_tot, nb, a.bin_off);
+            hipLaunchKernelGGL(k_ends_scatter<KW>, dim3(a.n_blocks), dim3(256), 0, s, a);
+            hipLaunchKernelGGL(k_ends_pair<KW>, dim3(grid < nb ? grid : nb), dim3(1024), 0, s, a);
+            break;
+        }
         case 1: hipLaunchKernelGGL(k_rank_init, dim3(grid), dim3(256), 0, s, a); break;
         case 2: hipLaunchKernelGGL(k_rank_round, dim3(grid), dim3(256), 0, s, a, changed); break;
         case 3: hipLaunchKernelGGL(k_place<KW>, dim3(grid), dim3(256), 0, s, a); break;

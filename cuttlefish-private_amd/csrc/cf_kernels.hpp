@@ -232,18 +232,24 @@ void launch_contract(const ContractArgs& a, uint32_t KW, uint32_t grid_lds, uint
 
 // ---------------------------------------------------------------------------
 // The compacted graph (cdbg.hip): contract's unitigs glued into the maximal
-// unitigs of the whole graph.  Phases 0 ends into the hash table, 1 chain
+// unitigs of the whole graph.  Phases 0 ends binned and paired, 1 chain
 // successors, 2 one pointer-jumping pass (sets *changed), 3 placement, 4 label
 // copies.
+constexpr uint32_t CD_MAX_BINS = 8192;  // pairing bins (LDS counters in the binning passes)
 struct CdbgArgs {
     const UnitigRec* recs;
     uint64_t n;
     const char* labels;
     uint32_t k;
-    uint64_t* hkey;             // hcap * KW words, all-ones = free
-    uint32_t* hval;             // 2 per slot: the end on the front / back side (all-ones = none)
-    uint64_t hcap;
-    uint32_t* end_slot;         // per piece end: hash slot << 1 | side, all-ones if it does not exit
+    uint32_t* partner;          // per piece end: the end across its vertex (all-ones: none, or not exiting)
+    uint32_t* bin_hist;         // pairing bins: exiting ends per (bin, block), scanned in place
+    uint32_t* bin_tot;          //   per bin total
+    uint32_t* bin_off;          //   bin offsets (1 << nb_bits) + 1
+    uint64_t* bin_key;          //   vertex keys of the exiting ends, bin after bin (KW words each)
+    uint32_t* bin_id;           //   their end ids << 1 | side
+    uint32_t nb_bits;           // log2 bins (<= log2 CD_MAX_BINS)
+    uint32_t n_blocks;          // pass 1 / pass 2 blocks, per_block pieces each
+    uint32_t per_block;
     uint64_t* kend;             // per piece: its first k characters and the rc of its last k (KW words each)
     uint64_t* jp;               // per state: the state it points at | label characters up to it << 30
     uint64_t total_chars;       // label bytes of all pieces

@@ -152,7 +152,8 @@ struct cfgpu_handle {
 
     // compacted graph: glue table, list-ranking states, output; cycles through
     // several subgraphs finished on the host (appended after the device part)
-    DevBuf cd_hkey, cd_hval, cd_eslot, cd_kend, cd_jp, cd_head, cd_pos, cd_hoff, cd_out, cd_rec, cd_ctl, cd_cyc;
+    DevBuf cd_partner, cd_bhist, cd_btot, cd_boff, cd_bkey, cd_bid, cd_kend, cd_jp, cd_head, cd_pos, cd_hoff, cd_out,
+        cd_rec, cd_ctl, cd_cyc;
     uint64_t cd_n = 0, cd_bytes = 0;
     std::string cd_host_seq;
     std::vector<uint64_t> cd_host_rec;
@@ -246,8 +247,8 @@ void trim(cfgpu_handle* h, Phase ph) {
         b->release();
     for (size_t i = h->round_idx; i < h->pools.size(); ++i) h->pools[i]->release();
     if (ph != Phase::Cdbg)
-        for (DevBuf* b : {&h->cd_hkey, &h->cd_hval, &h->cd_eslot, &h->cd_kend, &h->cd_jp, &h->cd_head, &h->cd_pos,
-                          &h->cd_hoff, &h->cd_out, &h->cd_rec})
+        for (DevBuf* b : {&h->cd_partner, &h->cd_bhist, &h->cd_bkey, &h->cd_bid, &h->cd_kend, &h->cd_jp, &h->cd_head,
+                          &h->cd_pos, &h->cd_hoff, &h->cd_out, &h->cd_rec})
             b->release();
     if (ph != Phase::Construct)
         for (DevBuf* b : {&h->spill_key, &h->spill_meta, &h->dd_att, &h->dd_lab, &h->dd_mult}) b->release();
@@ -1285,28 +1286,34 @@ int cfgpu_contract(cfgpu_handle* h) {
     if (!h->constructed) return fail(h, CFGPU_E_STATE, "contract before construct");
     HIPCK(h, hipSetDevice(h->prm.device));
     h->contracted = h->cdbg_done = false;
-    std::vector<uint32_t> list;
+    // Subgraphs whose vertex count passes 16 LDS tables go straight to the
+    // dense tables (their non-isolated vertices would not fit an LDS table
+    // unless nearly every vertex were isolated; the dense pass takes any
+    // subgraph, so the choice changes only where the work runs).
+    std::vector<uint32_t> list, big;
     uint64_t sum_nv = 0;
     uint32_t max_nv = 0;
+    const uint32_t direct = 16u * uint32_t(ct_lds_capacity(int(h->KW))) * 4u / 5u;
     for (uint32_t g = 0; g < h->G; ++g)
         if (h->h_tbl_ptr[g] && h->h_nvert[g]) {
-            list.push_back(g);
+            (h->h_nvert[g] > direct ? big : list).push_back(g);
             sum_nv += h->h_nvert[g];
             max_nv = std::max(max_nv, h->h_nvert[g]);
         }
-    const uint32_t KW = h->KW, n_list = uint32_t(list.size());
-    const uint32_t grid = std::min<uint32_t>(n_list, uint32_t(h->cus));  // one 1024-thread workgroup per CU
+    const uint32_t KW = h->KW, n_list = uint32_t(list.size()), n_big = uint32_t(big.size());
+    const uint32_t grid = std::min<uint32_t>(n_list + n_big, uint32_t(h->cus));  // one 1024-thread workgroup per CU
     // scratch per workgroup: the dense tables or a global table for the
     // largest subgraph, whichever is larger (256-byte aligned regions)
     const uint64_t per_wg =
         (std::max(ct_global_capacity(max_nv) * uint64_t(8 * KW + 8), ct_big_scratch_bytes(KW)) + 255) & ~255ull;
     HIPCK(h, h->ct_list.ensure(n_list * 4ull));
-    HIPCK(h, h->ct_ovf.ensure(n_list * 4ull));
-    HIPCK(h, h->ct_ovf2.ensure(n_list * 4ull));
+    HIPCK(h, h->ct_ovf.ensure((n_list + n_big) * 4ull));
+    HIPCK(h, h->ct_ovf2.ensure((n_list + n_big) * 4ull));
     HIPCK(h, h->ct_counts.ensure(size_t(h->G) * sizeof(cfgpu_contract_stat)));
     HIPCK(h, h->ct_ctl.ensure(64));
     HIPCK(h, ensure_mem(h, h->ct_scratch, per_wg * std::max<uint32_t>(grid, 1), Phase::Contract));
     if (n_list) HIPCK(h, h2d(h, h->ct_list.p, list.data(), n_list * 4ull));
+    if (n_big) HIPCK(h, h2d(h, h->ct_ovf.p, big.data(), n_big * 4ull));  // the dense pass's list starts with them
     // Output sized from the vertex count (most vertices of read sets are
     // isolated) or the last call's exact need, whichever is larger; re-run
     // once at the exact size when short.
@@ -1322,6 +1329,7 @@ int cfgpu_contract(cfgpu_handle* h) {
         HIPCK(h, ensure_mem(h, h->ct_labels, lab_cap + 64, Phase::Contract));  // slack: labels are read as aligned words
         launch_fill_u32(h->ct_counts.as<uint32_t>(), 0u, uint64_t(h->G) * 8, h->stream);
         launch_fill_u32(h->ct_ctl.as<uint32_t>(), 0u, 16, h->stream);
+        launch_fill_u32(h->ct_ctl.as<uint32_t>() + 2, n_big, 1, h->stream);  // ovf_count
         ContractArgs a{};
         a.tbl_ptr = h->d_tbl_ptr.as<uint64_t>();
         a.tbl_cap = h->d_cap.as<uint64_t>();
@@ -1392,10 +1400,18 @@ int cfgpu_cdbg(cfgpu_handle* h) {
     uint32_t rounds = 0;
     if (n) {
         if (2 * n >= (1ull << 30)) return fail(h, CFGPU_E_ARG, "cdbg: %llu unitigs", (unsigned long long)n);
-        const uint64_t hcap = round_cap(4.0 * double(n));
-        HIPCK(h, ensure_mem(h, h->cd_hkey, hcap * KW * 8, Phase::Cdbg));
-        HIPCK(h, ensure_mem(h, h->cd_hval, hcap * 8, Phase::Cdbg));
-        HIPCK(h, ensure_mem(h, h->cd_eslot, 2 * n * 4, Phase::Cdbg));
+        // pairing bins: about 4096 ends each, at most CD_MAX_BINS
+        uint32_t nb_bits = 0;
+        while (nb_bits < 13 && (uint64_t(4096) << nb_bits) < 2 * n) ++nb_bits;
+        static_assert((1u << 13) == CD_MAX_BINS, "bin bits");
+        const uint32_t nb = 1u << nb_bits, per_block = 8192;
+        const uint32_t n_blocks = uint32_t((n + per_block - 1) / per_block);
+        HIPCK(h, ensure_mem(h, h->cd_partner, 2 * n * 4, Phase::Cdbg));
+        HIPCK(h, ensure_mem(h, h->cd_bhist, uint64_t(nb) * n_blocks * 4, Phase::Cdbg));
+        HIPCK(h, h->cd_btot.ensure(uint64_t(nb) * 4));
+        HIPCK(h, h->cd_boff.ensure((uint64_t(nb) + 1) * 4));
+        HIPCK(h, ensure_mem(h, h->cd_bkey, 2 * n * KW * 8, Phase::Cdbg));
+        HIPCK(h, ensure_mem(h, h->cd_bid, 2 * n * 4, Phase::Cdbg));
         HIPCK(h, ensure_mem(h, h->cd_kend, 2 * n * KW * 8, Phase::Cdbg));
         HIPCK(h, ensure_mem(h, h->cd_jp, 2 * n * 8, Phase::Cdbg));
         HIPCK(h, h->cd_head.ensure(n * 4));
@@ -1404,20 +1420,23 @@ int cfgpu_cdbg(cfgpu_handle* h) {
         HIPCK(h, ensure_mem(h, h->cd_out, h->n_ubytes, Phase::Cdbg));  // glued labels share k characters: never longer
         HIPCK(h, h->cd_rec.ensure(n * 16));
         HIPCK(h, h->cd_ctl.ensure(64));
-        HIPCK(h, hipMemsetAsync(h->cd_hkey.p, 0xFF, hcap * KW * 8, h->stream));
-        HIPCK(h, hipMemsetAsync(h->cd_hval.p, 0xFF, hcap * 8, h->stream));
-        HIPCK(h, hipMemsetAsync(h->cd_hoff.p, 0xFF, n * 8, h->stream));
-        HIPCK(h, hipMemsetAsync(h->cd_ctl.p, 0, 64, h->stream));
+        launch_fill_u32(reinterpret_cast<uint32_t*>(h->cd_hoff.p), 0xFFFFFFFFu, 2 * n, h->stream);
+        launch_fill_u32(h->cd_ctl.as<uint32_t>(), 0u, 16, h->stream);
         uint32_t* c = h->cd_ctl.as<uint32_t>();
         CdbgArgs a{};
         a.recs = h->ct_recs.as<const UnitigRec>();
         a.n = n;
         a.labels = h->ct_labels.as<const char>();
         a.k = h->prm.k;
-        a.hkey = h->cd_hkey.as<uint64_t>();
-        a.hval = h->cd_hval.as<uint32_t>();
-        a.hcap = hcap;
-        a.end_slot = h->cd_eslot.as<uint32_t>();
+        a.partner = h->cd_partner.as<uint32_t>();
+        a.bin_hist = h->cd_bhist.as<uint32_t>();
+        a.bin_tot = h->cd_btot.as<uint32_t>();
+        a.bin_off = h->cd_boff.as<uint32_t>();
+        a.bin_key = h->cd_bkey.as<uint64_t>();
+        a.bin_id = h->cd_bid.as<uint32_t>();
+        a.nb_bits = nb_bits;
+        a.n_blocks = n_blocks;
+        a.per_block = per_block;
         a.kend = h->cd_kend.as<uint64_t>();
         a.jp = h->cd_jp.as<uint64_t>();
         a.total_chars = h->n_ubytes;

@@ -142,8 +142,9 @@ __device__ __forceinline__ bool lds_add(uint64_t* s_key, uint32_t* s_ef, uint32_
 // Spill region of a workgroup for one all-pass item: P sub-lists of c =
 // spill_cap / P entries, sub-list q holding the instances of pass q.  Key
 // word w of entry e is at key[w * spill_cap + e]; meta[e] = front | back << 3
-// | disc << 6 | min(m, 15) << 9 (counters saturate at 15, so adding min(m, 15)
-// equals adding m).
+// | disc << 6 | min(m, 15) << 9 | group << 13 (counters saturate at 15, so
+// adding min(m, 15) equals adding m; the group of the pass table, below
+// 2^12, is computed once by the walk).
 struct SpillRegion {
     uint64_t* key;
     uint32_t* meta;
@@ -155,28 +156,38 @@ __device__ __forceinline__ SpillRegion spill_region(const ConstructArgs& a) {
     return {a.spill_key + uint64_t(blockIdx.x) * KW * cap, a.spill_meta + uint64_t(blockIdx.x) * cap};
 }
 
-// Append the calling lanes' instances to their passes' sub-lists.  The lanes
-// of a wave are split by pass with nbits ballots (one per bit of q): each
-// group's first lane reserves the group's entries with one LDS atomic, and
-// the lanes take consecutive entries in lane order.  A full sub-list sets
-// *ovf (the bucket is then rebuilt on the global path).
+// Append the calling lanes' instances to their passes' sub-lists.  With more
+// than 4 passes each lane reserves its entry with its own LDS atomic (lanes
+// of a wave spread over many counters; the LDS unit resolves the few that
+// share one, and the VALU, the kernel's bottleneck, issues one instruction).
+// With at most 4 passes, where up to 32 lanes would share a counter, the
+// lanes are split by pass with nbits ballots (one per bit of q): each group's
+// first lane reserves the group's entries with one LDS atomic, and the lanes
+// take consecutive entries in lane order.  A full sub-list sets *ovf (the
+// bucket is then rebuilt on the global path).
 template <int KW>
 __device__ __forceinline__ void spill_push_pass(uint32_t q, uint32_t nbits, const Kmer<KW>& key, uint32_t meta,
                                                 const SpillRegion& R, uint32_t cap, uint32_t c, uint32_t* s_n,
                                                 uint32_t* s_ovf) {
-    unsigned long long same = __ballot(true);
-    for (uint32_t bit = 0; bit < nbits; ++bit) {
-        const bool set = (q >> bit) & 1u;
-        const unsigned long long m = __ballot(set);
-        same &= set ? m : ~m;
+    uint32_t idx;
+    if (nbits > 2) {
+        idx = atomicAdd(&s_n[q], 1u);
+    } else {
+        unsigned long long same = __ballot(true);
+        for (uint32_t bit = 0; bit < nbits; ++bit) {
+            const bool set = (q >> bit) & 1u;
+            const unsigned long long m = __ballot(set);
+            same &= set ? m : ~m;
+        }
+        const int lane = int(__lane_id());
+        const int leader = __ffsll((long long)same) - 1;
+        const uint32_t rank =
+            __builtin_amdgcn_mbcnt_hi(uint32_t(same >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(same), 0u));
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&s_n[q], uint32_t(__popcll(same)));
+        base = __builtin_amdgcn_ds_bpermute(leader << 2, int(base));
+        idx = base + rank;
     }
-    const int lane = int(__lane_id());
-    const int leader = __ffsll((long long)same) - 1;
-    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(same >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(same), 0u));
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&s_n[q], uint32_t(__popcll(same)));
-    base = __builtin_amdgcn_ds_bpermute(leader << 2, int(base));
-    const uint32_t idx = base + rank;
     if (idx >= c) {
         *s_ovf = 1;
         return;
@@ -191,6 +202,7 @@ template <int KW, int WMAX>
 __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
     constexpr int C = lds_capacity(KW);
     static_assert(C % 64 == 0, "capacity");
+    static_assert(C / 4 <= (1 << 12), "a group index fits the spill meta word");
     __shared__ uint64_t s_key[C * KW];
     __shared__ uint32_t s_ef[C];
     __shared__ uint32_t s_st[C / 4];  // one Vertex_Status byte per slot
@@ -274,12 +286,13 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                     // pass = top of h * P; the group comes from the fraction below it
                     const uint64_t hp = uint64_t(pass_hash<KW>(in.key)) * P;
                     const uint32_t q = uint32_t(hp >> 32);
+                    const uint32_t g0 = uint32_t((uint64_t(uint32_t(hp)) * NG) >> 32);
                     if (all) {
                         spill_push_pass<KW>(q, nbits, in.key,
-                                            in.front | (in.back << 3) | (in.disc << 6) | (min(m, 15u) << 9), R,
-                                            a.spill_cap, c_sub, s_sn, &s_ovf);
+                                            in.front | (in.back << 3) | (in.disc << 6) | (min(m, 15u) << 9) |
+                                                (g0 << 13),
+                                            R, a.spill_cap, c_sub, s_sn, &s_ovf);
                     } else if (q == p) {
-                        const uint32_t g0 = uint32_t((uint64_t(uint32_t(hp)) * NG) >> 32);
                         if (!lds_add<KW, C>(s_key, s_ef, s_st, in.key, g0, in.front, in.back, in.disc, m, my_ins))
                             s_ovf = 1;
                     }
@@ -333,10 +346,8 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                         for (int w = 0; w < KW; ++w) key_n.w[w] = R.key[uint64_t(w) * a.spill_cap + e0 + e + LWG];
                         meta_n = R.meta[e0 + e + LWG];
                     }
-                    const uint64_t hp = uint64_t(pass_hash<KW>(key)) * P;
-                    const uint32_t g0 = uint32_t((uint64_t(uint32_t(hp)) * NG) >> 32);
-                    if (!lds_add<KW, C>(s_key, s_ef, s_st, key, g0, meta & 7u, (meta >> 3) & 7u, (meta >> 6) & 7u,
-                                        meta >> 9, my_ins))
+                    if (!lds_add<KW, C>(s_key, s_ef, s_st, key, meta >> 13, meta & 7u, (meta >> 3) & 7u,
+                                        (meta >> 6) & 7u, (meta >> 9) & 15u, my_ins))
                         s_ovf = 1;
                     if ((++seen & 7u) == 0) {
                         flush();

@@ -27,7 +27,6 @@ typedef __attribute__((ext_vector_type(2))) uint64_t u64x2;
 
 
 constexpr int WIN_MAX = 128;                      // w = k - l <= 126
-constexpr int MAXF = P_TILE / 4 + 8;              // fragments overlapping a tile (len >= 4)
 constexpr int NWORDS = (P_TILE + 2 * WIN_MAX) / 32 + 4;
 constexpr int S_MAX = P_TILE + WIN_MAX;           // l-mer hashes per tile
 constexpr int SPAN_MAX = 64;                      // largest power of two <= w (w <= 126)
