@@ -142,8 +142,9 @@ struct BigTab {
     uint64_t* key;
     uint32_t* meta;
     uint32_t* aux;
-    uint32_t cap;  // vertices
-    uint32_t* ht;  // LDS index, CB_HCAP entries
+    uint32_t cap;         // vertices
+    uint32_t* ht;         // LDS index, CB_HCAP entries (until the links are made)
+    const uint32_t* succ; // per directed state its successor state | 1 << 16, or itself (global scratch)
 };
 
 template <int KW>
@@ -174,6 +175,19 @@ __device__ __forceinline__ int tab_find(const BigTab<KW>& t, const Kmer<KW>& x) 
     return -1;
 }
 
+template <int KW>
+__device__ __forceinline__ int walk_next(const BigTab<KW>& t, uint32_t cur, uint32_t m, uint32_t k, Kmer<KW>& z,
+                                         Kmer<KW>& zb, uint32_t& side, uint32_t& b) {
+    const uint32_t e = (m >> (3 * side)) & 7u;
+    b = side == 1 ? e : 3u - e;
+    z = km_append(z, b, k);
+    zb = km_prepend(zb, uint8_t(3u - b), k);
+    const uint32_t x = 2 * cur + side, st = t.succ[x] & 0xFFFFu;
+    if (st == x) return -1;
+    side = st & 1u;
+    return int(st >> 1);
+}
+
 // Index vertex d under key x (keys are unique): the first free slot from the
 // key's home slot.
 template <int KW>
@@ -201,6 +215,14 @@ __device__ __forceinline__ int walk_step(const Tab& t, uint32_t m, uint32_t k, K
     const bool can = km_less(z, zb);
     side = can ? 1u : 0u;  // entrance side front (canonical) -> leave through the back
     return tab_find(t, can ? z : zb);
+}
+
+// One step of a walk that knows its current slot: the lookup of walk_step,
+// or, for dense tables once linked, the successor word of the state.
+template <int KW, class Tab>
+__device__ __forceinline__ int walk_next(const Tab& t, uint32_t cur, uint32_t m, uint32_t k, Kmer<KW>& z, Kmer<KW>& zb,
+                                         uint32_t& side, uint32_t& b) {
+    return walk_step<KW>(t, m, k, z, zb, side, b);
 }
 
 // ASCII of base code b (A C G T).
@@ -338,7 +360,7 @@ __device__ void cycles_serial(const Tab& t, uint32_t k, const uint32_t* cand, ui
                 *s_err = 1;
                 break;
             }
-            const int j = walk_step<KW>(t, m, k, z, zb, side, bb);
+            const int j = walk_next<KW>(t, cur, m, k, z, zb, side, bb);
             if (j < 0) {
                 *s_err = 1;
                 break;
@@ -379,7 +401,7 @@ __device__ __forceinline__ uint32_t write_cycle(const Tab& t, uint32_t i, uint32
     for (uint32_t step = 0; step < t.cap; ++step) {
         const uint32_t m = t.meta[cur];
         if (!(m & (M_LINK0 << side))) break;
-        const int j = walk_step<KW>(t, m, k, z, zb, side, bb);
+        const int j = walk_next<KW>(t, cur, m, k, z, zb, side, bb);
         if (j < 0 || uint32_t(j) == i) break;
         if (off + k + n > end) {
             atomicOr(err, 1u);
@@ -631,9 +653,16 @@ constexpr uint32_t CB_ROUNDS = 17;  // > log2(2 * CB_NMAX): every path converges
 constexpr uint32_t CB_M16 = 0xFFFFu;
 __device__ __forceinline__ uint32_t cb_pack(uint32_t p, uint32_t d) { return p | (min(d, CB_M16) << 16); }
 
+// Ranking words kept in LDS (over the index, no longer needed once linked)
+// when the directed states fit: 2n <= CB_JP_LDS.
+constexpr uint32_t CB_JP_LDS = 39936;
+constexpr uint32_t CB_JQ = CB_JP_LDS / CT_WG;  // states per thread
+static_assert(CB_JP_LDS >= CB_HCAP && CB_JP_LDS % CT_WG == 0, "LDS ranking buffer");
+
 template <int KW>
 __global__ __launch_bounds__(CT_WG) void k_contract_big(ContractArgs a) {
-    __shared__ uint32_t s_ht[CB_HCAP];
+    __shared__ uint32_t s_buf[CB_JP_LDS];  // the key index, then the ranking words
+    uint32_t* s_ht = s_buf;
     __shared__ uint32_t s_cand[CT_CAND];
     __shared__ uint32_t s_cnt[8];
     __shared__ uint32_t s_chg[CB_ROUNDS];
@@ -658,6 +687,7 @@ __global__ __launch_bounds__(CT_WG) void k_contract_big(ContractArgs a) {
     uint32_t* offv = wnv + N;
     t.aux = offv;
     t.ht = s_ht;
+    t.succ = jp0;
 
     while (true) {
         __syncthreads();
@@ -705,28 +735,69 @@ __global__ __launch_bounds__(CT_WG) void k_contract_big(ContractArgs a) {
         __syncthreads();
 
         // rank: pointer jumping until no path state moves (cycle states stop
-        // counting once past the vertex count)
-        uint32_t* jp = jp0;
-        uint32_t* jq = jp1;
-        for (uint32_t r = 0; r < CB_ROUNDS; ++r) {
-            bool any = false;
-            for (uint32_t x = tid; x < 2 * n; x += CT_WG) {
-                const uint32_t w = jp[x], p = w & CB_M16;
-                uint32_t nw = w;
-                if (p != x) {
-                    const uint32_t w2 = jp[p];
+        // counting once past the vertex count).  The successor words in jp0
+        // stay as they are (cycle walks follow them).
+        const uint32_t* jp;
+        if (2 * n <= CB_JP_LDS) {
+            // in LDS, in place: new words in registers between two barriers
+            for (uint32_t x = tid; x < 2 * n; x += CT_WG) s_buf[x] = jp0[x];
+            __syncthreads();
+            for (uint32_t r = 0; r < CB_ROUNDS; ++r) {
+                uint32_t nw[CB_JQ];
+                bool any = false;
+#pragma unroll
+                for (uint32_t q = 0; q < CB_JQ; ++q) {
+                    const uint32_t x = tid + q * CT_WG;
+                    nw[q] = 0;
+                    if (x >= 2 * n) continue;
+                    const uint32_t w = s_buf[x], p = w & CB_M16;
+                    nw[q] = w;
+                    if (p == x) continue;
+                    const uint32_t w2 = s_buf[p];
                     const uint32_t p2 = w2 & CB_M16, nd = (w >> 16) + (w2 >> 16);
-                    nw = cb_pack(p2, nd);
+                    nw[q] = cb_pack(p2, nd);
                     any |= p2 != p && nd < n;
                 }
-                jq[x] = nw;
+                __syncthreads();
+#pragma unroll
+                for (uint32_t q = 0; q < CB_JQ; ++q) {
+                    const uint32_t x = tid + q * CT_WG;
+                    if (x < 2 * n) s_buf[x] = nw[q];
+                }
+                if (any) s_chg[r] = 1;
+                __syncthreads();
+                if (!s_chg[r]) break;
             }
-            if (any) s_chg[r] = 1;
-            __syncthreads();
-            uint32_t* tmp = jp;
-            jp = jq;
-            jq = tmp;
-            if (!s_chg[r]) break;
+            jp = s_buf;
+        } else {
+            // in global scratch, two buffers
+            const uint32_t* src = jp0;
+            uint32_t* dst = jp1;
+            for (uint32_t r = 0; r < CB_ROUNDS; ++r) {
+                bool any = false;
+                for (uint32_t x = tid; x < 2 * n; x += CT_WG) {
+                    const uint32_t w = src[x], p = w & CB_M16;
+                    uint32_t nw = w;
+                    if (p != x) {
+                        const uint32_t w2 = src[p];
+                        const uint32_t p2 = w2 & CB_M16, nd = (w >> 16) + (w2 >> 16);
+                        nw = cb_pack(p2, nd);
+                        any |= p2 != p && nd < n;
+                    }
+                    dst[x] = nw;
+                }
+                if (any) s_chg[r] = 1;
+                __syncthreads();
+                src = dst;
+                dst = dst == jp1 ? ia : jp1;  // ia (with posv) is free until the info pass: 2N words
+                if (!s_chg[r]) break;
+            }
+            if (src == ia) {  // the info pass below writes ia: move the words to jp1
+                for (uint32_t x = tid; x < 2 * n; x += CT_WG) jp1[x] = ia[x];
+                __syncthreads();
+                src = jp1;
+            }
+            jp = src;
         }
 
         // per vertex: the ends of its path, the output orientation and its position

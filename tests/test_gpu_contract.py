@@ -119,3 +119,53 @@ def test_contract_after_release_skips_subgraph():
         np.testing.assert_array_equal(after[keep], before[keep])
         got = h.unitigs()
         assert g not in set(got["g"].tolist())
+
+
+def _records_one_bucket(seqs, k, l, rng, disc_rate=0.0):
+    """Overlapping records (length k+1 .. 2k-l, consecutive ones sharing k
+    bases, so every (k+1)-mer of a sequence lies in one record) of every
+    sequence, all in bucket 0, in the reference's record layout."""
+    W = O.label_words(k, l)
+    code = {"A": 0, "C": 1, "G": 2, "T": 3}
+    att, lab = [], []
+    for s in seqs:
+        pos = 0
+        while pos + k < len(s):
+            L = min(int(rng.integers(k + 1, 2 * k - l + 1)), len(s) - pos)
+            rec = s[pos:pos + L]
+            words = [0] * W
+            for i, ch in enumerate(rec):
+                words[W - 1 - i // 32] |= code[ch] << (62 - 2 * (i % 32))
+            ld = int(rng.random() < disc_rate) if pos else 0
+            rd = int(rng.random() < disc_rate)
+            att.append(L | (ld << 8) | (rd << 9))
+            lab.append(words)
+            pos += L - k
+    off = np.zeros(C.G + 1, np.uint64)
+    off[1:] = len(att)
+    return np.array(att, np.uint32), np.array(lab, np.uint64).reshape(len(att), W), off
+
+
+@pytest.mark.parametrize("n_random", [8000, 24000])
+def test_contract_dense_tables_with_cycles(n_random):
+    # One subgraph past an LDS table, holding a long random path and tandem
+    # repeats (each a cycle of its period's k-mers): ranked in LDS (8000
+    # vertices) or in global scratch (24000), cycles walked by successor words.
+    k, l = 31, 20
+    rng = np.random.default_rng(n_random)
+    seqs = [rand_seq(rng, n_random)]
+    for p in rng.integers(40, 200, size=12):
+        u = rand_seq(rng, int(p))
+        seqs.append(u * (2 + (k + 1) // int(p) + 1))
+    att, lab, off = _records_one_bucket(seqs, k, l, rng, disc_rate=0.05)
+    st = O.construct(k, l, off, (att & 0xFFFF).astype(np.uint16), lab)
+    want = O.contract(k, 1, st, normalise=True)
+    assert (want["flags"] & 1).sum() >= 10
+    say(f"[case] dense-cycles n={n_random} vertices={len(st['g'])} cycles={int((want['flags'] & 1).sum())}")
+    with C.Handle(k, l, 1) as h:
+        h.load_buckets(att, lab, off)
+        h.construct("batched")
+        h.contract()
+        stats = h.contract_stats()
+        assert stats[0, 7] == 8  # the dense tables took it
+        compare("dense-cycles", h.unitigs(), stats, want)
