@@ -12,6 +12,45 @@
 
 using namespace cforacle;
 
+// ---------------------------------------------------------------------------
+// Order-independent checksums for comparisons at sizes where rows cannot be
+// compared one by one: per bucket g, the number of items and the wrapping sum
+// of a 64-bit mix of each item.  Equal multisets give equal sums; an item
+// that differs changes its bucket's sum except with probability about 2^-64.
+namespace {
+uint64_t mix64(uint64_t z) {  // the splitmix64 finaliser
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+uint64_t record_mix(uint16_t att, const uint64_t* lab, unsigned W) {
+    uint64_t h = mix64(att);
+    for (unsigned w = 0; w < W; ++w) h = mix64(h ^ lab[w] ^ (uint64_t(w + 1) << 56));
+    return h;
+}
+uint64_t state_mix(const uint64_t* key, unsigned kw, uint32_t e_f, uint8_t status) {
+    uint64_t h = mix64(uint64_t(e_f) | (uint64_t(status) << 32));
+    for (unsigned w = 0; w < kw; ++w) h = mix64(h ^ key[w] ^ (uint64_t(w + 1) << 56));
+    return h;
+}
+template <class F>
+void per_thread_sums(uint64_t n, int threads, uint32_t G, uint64_t* sum_out, uint64_t* cnt_out, F&& item) {
+    if (threads < 1) threads = 1;
+    std::vector<std::vector<uint64_t>> S(threads, std::vector<uint64_t>(G)), N(threads, std::vector<uint64_t>(G));
+    std::vector<std::thread> th;
+    for (int t = 0; t < threads; ++t)
+        th.emplace_back([&, t]() {
+            for (uint64_t i = n * t / threads; i < n * (t + 1) / threads; ++i) item(i, S[t], N[t]);
+        });
+    for (auto& x : th) x.join();
+    for (uint32_t g = 0; g < G; ++g) {
+        sum_out[g] = cnt_out[g] = 0;
+        for (int t = 0; t < threads; ++t) sum_out[g] += S[t][g], cnt_out[g] += N[t][g];
+    }
+}
+}  // namespace
+
 extern "C" {
 
 uint64_t cfo_wyhash(const void* key, size_t len, uint64_t seed) { return wyhash(key, len, seed, kWySalt); }
@@ -119,6 +158,56 @@ int64_t cfo_construct(int k, int l, uint32_t G, int mode, const uint64_t* bucket
         st_out[i] = all[i].st.status;
     }
     return int64_t(all.size());
+}
+
+
+// Partition every record (the oracle's sequential split, records spread over
+// `threads` workers) and checksum the weak super k-mers of each bucket:
+// mix of (attribute pack, label words, bits past the length zero).
+int cfo_partition_hash(const char* seq, const uint64_t* rec_off, size_t n_rec, int k, int l, int threads,
+                       uint64_t* sum_out, uint64_t* cnt_out) {
+    const unsigned W = label_words(k, l);
+    const uint32_t G = 16384;
+    per_thread_sums(n_rec, threads, G, sum_out, cnt_out,
+                    [&](uint64_t r, std::vector<uint64_t>& S, std::vector<uint64_t>& N) {
+                        thread_local std::vector<SuperKmer> sk;
+                        uint64_t lab[8];
+                        const char* rs = seq + rec_off[r];
+                        for (const auto& f : fragments(rs, rec_off[r + 1] - rec_off[r], k)) {
+                            sk.clear();
+                            partition_fragment(rs + f.first, f.second, k, l, G, sk);
+                            for (const auto& x : sk) {
+                                encode_label(rs + f.first + x.off, x.len, W, lab);
+                                S[x.g] += record_mix(att_pack(x.len, x.l_disc, x.r_disc), lab, W);
+                                ++N[x.g];
+                            }
+                        }
+                    });
+    return 0;
+}
+
+// The same checksums of records given as the device returns them: att with
+// the bucket in its high half, W label words each.
+int cfo_records_hash(const uint32_t* att, const uint64_t* labels, uint64_t n, int W, int threads,
+                     uint64_t* sum_out, uint64_t* cnt_out) {
+    per_thread_sums(n, threads, 16384, sum_out, cnt_out,
+                    [&](uint64_t i, std::vector<uint64_t>& S, std::vector<uint64_t>& N) {
+                        const uint32_t g = att[i] >> 16;
+                        S[g] += record_mix(uint16_t(att[i] & 0xFFFF), labels + i * uint64_t(W), unsigned(W));
+                        ++N[g];
+                    });
+    return 0;
+}
+
+// Checksums of vertex states per bucket: mix of (key words, e_f, status).
+int cfo_states_hash(const uint32_t* g, const uint64_t* keys, int kw, const uint32_t* e_f, const uint8_t* status,
+                    uint64_t n, int threads, uint64_t* sum_out, uint64_t* cnt_out) {
+    per_thread_sums(n, threads, 16384, sum_out, cnt_out,
+                    [&](uint64_t i, std::vector<uint64_t>& S, std::vector<uint64_t>& N) {
+                        S[g[i]] += state_mix(keys + i * uint64_t(kw), unsigned(kw), e_f[i], status[i]);
+                        ++N[g[i]];
+                    });
+    return 0;
 }
 
 uint32_t cfo_edge_at(uint32_t e_f, int side, int f_th) { return edge_at(e_f, side, f_th); }

@@ -52,6 +52,14 @@ def lib():
         L.cfo_pipeline.restype = ctypes.c_int
         L.cfo_pipeline.argtypes = [ctypes.c_char_p, u64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.cfo_partition_hash.restype = ctypes.c_int
+        L.cfo_partition_hash.argtypes = [ctypes.c_char_p, u64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, u64p, u64p]
+        L.cfo_records_hash.restype = ctypes.c_int
+        L.cfo_records_hash.argtypes = [u32p, u64p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, u64p, u64p]
+        L.cfo_states_hash.restype = ctypes.c_int
+        L.cfo_states_hash.argtypes = [u32p, u64p, ctypes.c_int, u32p, u8p, ctypes.c_uint64, ctypes.c_int, u64p,
+                                      u64p]
         L.cfo_edge_at.restype = ctypes.c_uint32
         L.cfo_edge_at.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
         L.cfo_contract.restype = ctypes.c_int64
@@ -189,3 +197,33 @@ def pipeline(seq: bytes, rec_off, k, l, threads):
     assert r == 0
     return {"partition_ms": out[0], "bucketize_ms": out[1], "construct_ms": out[2],
             "n_kmers": int(out[3]), "n_records": int(out[4]), "n_states": int(out[5])}
+
+
+def partition_hash(seq: bytes, rec_off, k, l, threads=16):
+    """Per-bucket (checksum, count) of the oracle's weak super k-mers."""
+    ro = np.ascontiguousarray(rec_off, dtype=np.uint64)
+    s, c = np.zeros(16384, np.uint64), np.zeros(16384, np.uint64)
+    lib().cfo_partition_hash(seq, _ptr(ro, u64p), len(ro) - 1, k, l, threads, _ptr(s, u64p), _ptr(c, u64p))
+    return s, c
+
+
+def records_hash(att, labels, W, threads=16):
+    """Per-bucket (checksum, count) of records as the device returns them."""
+    a = np.ascontiguousarray(att, dtype=np.uint32)
+    lb = np.ascontiguousarray(labels, dtype=np.uint64).reshape(-1)
+    s, c = np.zeros(16384, np.uint64), np.zeros(16384, np.uint64)
+    lib().cfo_records_hash(_ptr(a, u32p), _ptr(lb, u64p), len(a), W, threads, _ptr(s, u64p), _ptr(c, u64p))
+    return s, c
+
+
+def states_hash(t, threads=16):
+    """Per-bucket (checksum, count) of vertex states (dict g, keys, e_f, status)."""
+    g = np.ascontiguousarray(t["g"], dtype=np.uint32)
+    keys = np.ascontiguousarray(t["keys"], dtype=np.uint64)
+    ef = np.ascontiguousarray(t["e_f"], dtype=np.uint32)
+    st = np.ascontiguousarray(t["status"], dtype=np.uint8)
+    s, c = np.zeros(16384, np.uint64), np.zeros(16384, np.uint64)
+    lib().cfo_states_hash(_ptr(g, u32p), _ptr(keys, u64p), keys.shape[1] if keys.ndim == 2 else 1, _ptr(ef, u32p),
+                          _ptr(st, u8p), len(g), threads, _ptr(s, u64p), _ptr(c, u64p))
+    return s, c
+

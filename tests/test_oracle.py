@@ -299,3 +299,33 @@ def test_edge_at_cutoff():
     assert O.edge_at(ef, 0, 3) == 5          # none -> E
     assert O.edge_at(ef, 1, 15) == 4
     assert O.edge_at(0, 1, 1) == 5
+
+
+@pytest.mark.parametrize("k", [31, 63])
+def test_bucket_checksums_pin_record_multisets(k):
+    # The full-size tests compare per-bucket checksums of records: the oracle's
+    # streaming partition checksum must equal the checksum of its own records,
+    # ignore their order, and change when one label bit changes.
+    import cfgpu as C
+    g = C.synth_genome(100000, 0.1, 300, 2000, 0.02, seed=5)
+    reads = C.synth_reads(g, 6000, 150, 0.01, seed=6)
+    ro = np.arange(6001, dtype=np.uint64) * np.uint64(150)
+    part = O.partition(reads, ro, k, 20)
+    W = O.label_words(k, 20)
+    att = part["att"].astype(np.uint32) | (part["g"].astype(np.uint32) << 16)
+    s1, c1 = O.partition_hash(reads, ro, k, 20, threads=3)
+    s2, c2 = O.records_hash(att, part["labels"], W, threads=2)
+    np.testing.assert_array_equal(s1, s2)
+    np.testing.assert_array_equal(c1, c2)
+    assert int(c1.sum()) == len(att)
+    p = np.random.default_rng(1).permutation(len(att))
+    s3, _ = O.records_hash(att[p], part["labels"][p], W)
+    np.testing.assert_array_equal(s2, s3)
+    bad = part["labels"].copy()
+    bad[7, 0] ^= np.uint64(4)
+    s4, _ = O.records_hash(att, bad, W)
+    assert int((s4 != s2).sum()) == 1
+    badatt = att.copy()
+    badatt[9] ^= np.uint32(1 << 9)  # a flipped r_disc bit
+    s5, _ = O.records_hash(badatt, part["labels"], W)
+    assert int((s5 != s2).sum()) == 1
