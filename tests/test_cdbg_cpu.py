@@ -18,7 +18,8 @@ def oracle_unitigs(seq, ro, k, l, f_th):
     return O.contract(k, f_th, O.construct(k, l, off, att, lab), normalise=True)
 
 
-@pytest.mark.parametrize("k,l,f_th", [(31, 20, 1), (31, 20, 2), (15, 7, 1), (15, 7, 3), (21, 11, 1), (63, 20, 1)])
+@pytest.mark.parametrize("k,l,f_th", [(31, 20, 1), (31, 20, 2), (15, 7, 1), (15, 7, 3), (21, 11, 1), (63, 20, 1),
+                                      (95, 20, 1), (95, 20, 2), (127, 20, 1), (127, 20, 2)])
 def test_cdbg_equals_bruteforce_unitigs(k, l, f_th):
     rng = np.random.default_rng(70 + k + f_th)
     seq, ro = O.parse(mixed_input(rng), fastq=False)
@@ -43,7 +44,7 @@ def test_cdbg_reads_and_cycles():
     reads += [rand_seq(rng, p) * (1 + 120 // p) for p in (3, 5, 7, 40, 45, 60)]
     buf = "".join(f">r{i}\n{r}\n" for i, r in enumerate(reads)).encode()
     seq, ro = O.parse(buf, fastq=False)
-    for k, l, f_th in ((21, 11, 1), (21, 11, 2), (31, 20, 1)):
+    for k, l, f_th in ((21, 11, 1), (21, 11, 2), (31, 20, 1), (63, 20, 1), (95, 20, 2)):
         res = oracle_unitigs(seq, ro, k, l, f_th)
         got = C.cdbg_from_unitigs(k, res["g"], res["flags"], res["labels"])
         want = U.unitigs(seq, ro, k, f_th)

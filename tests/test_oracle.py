@@ -230,7 +230,7 @@ def _stitch_check(seq, ro, k, l, states):
     assert seen == verts
 
 
-@pytest.mark.parametrize("k,l", [(31, 20), (15, 7)])
+@pytest.mark.parametrize("k,l", [(31, 20), (15, 7), (63, 20), (95, 20), (127, 20)])
 def test_construct_three_ways(k, l):
     rng = np.random.default_rng(100 + k)
     base = rand_seq(rng, 800)

@@ -157,7 +157,8 @@ def inputs(rng):
     return "".join(f">r{i}\n{r}\n" for i, r in enumerate(recs)).encode()
 
 
-@pytest.mark.parametrize("k,l,f_th", [(31, 20, 1), (31, 20, 2), (15, 7, 1), (15, 7, 3), (21, 11, 1)])
+@pytest.mark.parametrize("k,l,f_th", [(31, 20, 1), (31, 20, 2), (15, 7, 1), (15, 7, 3), (21, 11, 1), (63, 20, 1),
+                                      (95, 20, 1), (95, 20, 2), (127, 20, 1), (127, 20, 2)])
 def test_contract_glued_equals_global_unitigs(k, l, f_th):
     rng = np.random.default_rng(7 + k + f_th)
     seq, ro = O.parse(inputs(rng), fastq=False)
