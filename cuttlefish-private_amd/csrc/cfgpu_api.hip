@@ -30,6 +30,8 @@ constexpr uint64_t SAMPLE_TARGET = MLP_TRAIN_MAX + MLP_TRAIN_MAX / 3;  // 3/4 tr
 // buckets by more than this (mean |log error|).
 constexpr double CONST_RATIO_GOOD = 0.05;
 constexpr uint32_t MAX_ROUNDS = 8;
+// MLP weight initialisation seed (epochs and learning rate are tuning knobs)
+constexpr uint32_t MLP_SEED = 12345u;
 constexpr double LDS_LOAD = 0.8;  // target fill of an LDS pass table (overflow at 7/8)
 
 struct DevBuf {
@@ -99,7 +101,7 @@ struct cfgpu_handle {
     int cus = 256;
     hipStream_t stream = nullptr;
     std::string err;
-    cfgpu_tuning tune{0.6f, 256, 2048, 8, 1.15f, 1, LDS_ALL_PASS_MAX, 0, 1};
+    cfgpu_tuning tune{0.6f, 256, 2048, 8, 1.15f, 1, LDS_ALL_PASS_MAX, 0, 1, 120, 3e-3f};
 
     // partition state
     DevBuf packed, frag, gbin, mbn, tile_lc, tile_frag, boff, att, labels, mbin, stats, feat;
@@ -138,8 +140,13 @@ struct cfgpu_handle {
     std::vector<uint32_t> h_ddcnt;
     std::vector<uint64_t> h_ddkm;
 
-    // MLP
+    // MLP: device state, and the training set and settings of the last run
     DevBuf mlp_params, mlp_adam, mlp_idx, mlp_target, mlp_pred, mlp_loss;
+    std::vector<uint32_t> h_mlp_idx;
+    std::vector<float> h_mlp_target;
+    bool mlp_trained = false;
+    float mlp_loss_last = 0.f, mlp_lr_last = 0.f;
+    uint32_t mlp_epochs_last = 0;
 
     // export scratch
     DevBuf ex_keys, ex_ef, ex_st, ex_dfa, ex_off, ex_cursor, ex_list;
@@ -927,8 +934,12 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
                 HIPCK(h, h2d(h, h->mlp_idx.p, idx.data(), idx.size() * 4));
                 HIPCK(h, h2d(h, h->mlp_target.p, tgt.data(), tgt.size() * 4));
                 launch_mlp_train(h->mlp_params.as<float>(), h->feat.as<float>(), h->mlp_idx.as<uint32_t>(),
-                                 h->mlp_target.as<float>(), uint32_t(idx.size()), 120, 3e-3f, 12345u,
+                                 h->mlp_target.as<float>(), uint32_t(idx.size()), h->tune.mlp_epochs, h->tune.mlp_lr, MLP_SEED,
                                  h->mlp_adam.as<float>(), h->mlp_loss.as<float>(), h->stream);
+                h->h_mlp_idx = idx;
+                h->h_mlp_target = tgt;
+                h->mlp_epochs_last = h->tune.mlp_epochs;
+                h->mlp_lr_last = h->tune.mlp_lr;
                 HIPCK(h, hipEventRecord(e4.e, h->stream));
                 launch_mlp_infer(h->mlp_params.as<float>(), h->feat.as<float>(), uint32_t(G),
                                  h->mlp_pred.as<float>(), h->stream);
@@ -938,6 +949,8 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
                 HIPCK(h, hipEventRecord(e5.e, h->stream));
                 HIPCK(h, sync_stream(h));
                 rs.mlp_loss = loss;
+                h->mlp_trained = true;
+                h->mlp_loss_last = loss;
                 rs.t_train_ms = elapsed_ms(e1, e4);
                 rs.t_infer_ms = elapsed_ms(e4, e5);
                 auto mlp_log = [&](uint32_t b) {
@@ -1081,6 +1094,7 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
         }
     }
     h->partitioned = h->constructed = h->contracted = h->cdbg_done = false;
+    h->mlp_trained = false;
     const uint32_t G = h->G;
     const uint64_t n_words = (n_bases + 31) / 32 + 2;
     const uint64_t tiles = (n_bases + P_TILE - 1) / P_TILE;
@@ -1728,6 +1742,34 @@ int cfgpu_release(cfgpu_handle* h, uint32_t g) {
     return CFGPU_OK;
 }
 
+int cfgpu_sizing_model_get(cfgpu_handle* h, cfgpu_sizing_model* info, float* params, float* features,
+                           uint32_t* train_buckets, float* train_targets, float* predictions) {
+    if (!h) return CFGPU_E_ARG;
+    if (!h->mlp_trained) return fail(h, CFGPU_E_STATE, "sizing model: no MLP trained since the last partition");
+    HIPCK(h, hipSetDevice(h->prm.device));
+    const uint32_t n_train = uint32_t(h->h_mlp_idx.size());
+    if (info) {
+        info->n_features = N_FEAT;
+        info->hidden = MLP_H;
+        info->n_params = uint32_t(mlp_param_floats());
+        info->n_train = n_train;
+        info->epochs = h->mlp_epochs_last;
+        info->seed = MLP_SEED;
+        info->lr = h->mlp_lr_last;
+        info->loss = h->mlp_loss_last;
+    }
+    if (params) HIPCK(h, hipMemcpyAsync(params, h->mlp_params.p, size_t(mlp_param_floats()) * 4, hipMemcpyDeviceToHost,
+                                        h->stream));
+    if (features) HIPCK(h, hipMemcpyAsync(features, h->feat.p, size_t(h->G) * N_FEAT * 4, hipMemcpyDeviceToHost,
+                                          h->stream));
+    if (predictions) HIPCK(h, hipMemcpyAsync(predictions, h->mlp_pred.p, size_t(h->G) * 4, hipMemcpyDeviceToHost,
+                                             h->stream));
+    HIPCK(h, sync_stream(h));
+    if (train_buckets) std::memcpy(train_buckets, h->h_mlp_idx.data(), n_train * 4ull);
+    if (train_targets) std::memcpy(train_targets, h->h_mlp_target.data(), n_train * 4ull);
+    return CFGPU_OK;
+}
+
 int cfgpu_run_stats_get(cfgpu_handle* h, cfgpu_run_stats* out) {
     if (!h || !out) return CFGPU_E_ARG;
     *out = h->rs;
@@ -1746,7 +1788,8 @@ int cfgpu_tuning_set(cfgpu_handle* h, const cfgpu_tuning* in) {
         (in->workgroup != 256 && in->workgroup != 512) || in->chunk_records < 64 ||
         in->chunk_records > (1u << 24) || in->wgs_per_cu < 1 || in->wgs_per_cu > 32 ||
         !(in->margin >= 0.01f && in->margin <= 4.0f) || in->use_graph > 1 || in->max_passes > 4096 ||
-        in->predictor > 2 || in->dedup > 2)
+        in->predictor > 2 || in->dedup > 2 || in->mlp_epochs < 1 || in->mlp_epochs > 10000 ||
+        !(in->mlp_lr > 0.f && in->mlp_lr <= 1.f))
         return fail(h, CFGPU_E_ARG, "tuning out of range");
     h->tune = *in;
     return CFGPU_OK;
@@ -1761,7 +1804,11 @@ int cfgpu_autotune(cfgpu_handle* h, uint32_t mode, cfgpu_tune_trial* trials_out,
     for (float lf : {0.5f, 0.65f})
         for (uint32_t wg : {256u, 512u})
             for (uint32_t ch : {1024u, 4096u})
-                for (uint32_t wpc : {4u, 8u}) grid.push_back({lf, wg, ch, wpc, base.margin, base.use_graph, base.max_passes, base.predictor, base.dedup});
+                for (uint32_t wpc : {4u, 8u}) {
+                    cfgpu_tuning t = base;
+                    t.load_factor = lf, t.workgroup = wg, t.chunk_records = ch, t.wgs_per_cu = wpc;
+                    grid.push_back(t);
+                }
     cfgpu_tuning best = base;
     double best_ms = 1e300;
     uint32_t w = 0;

@@ -87,10 +87,15 @@ class Tuning(ctypes.Structure):
                 ("chunk_records", ctypes.c_uint32), ("wgs_per_cu", ctypes.c_uint32),
                 ("margin", ctypes.c_float), ("use_graph", ctypes.c_uint32),
                 ("max_passes", ctypes.c_uint32), ("predictor", ctypes.c_uint32),
-                ("dedup", ctypes.c_uint32)]
+                ("dedup", ctypes.c_uint32), ("mlp_epochs", ctypes.c_uint32), ("mlp_lr", ctypes.c_float)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class SizingModel(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("n_features", "hidden", "n_params", "n_train", "epochs", "seed")] + \
+               [("lr", ctypes.c_float), ("loss", ctypes.c_float)]
 
 
 class TuneTrial(ctypes.Structure):
@@ -126,6 +131,8 @@ SIGNATURES = {
     "cfgpu_unitigs_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Unitig), ctypes.c_uint64,
                                           ctypes.c_char_p, ctypes.c_uint64, u64p, u64p]),
     "cfgpu_run_stats_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(RunStats)]),
+    "cfgpu_sizing_model_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SizingModel), ctypes.c_void_p,
+                                              ctypes.c_void_p, u32p, ctypes.c_void_p, ctypes.c_void_p]),
     "cfgpu_tuning_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Tuning)]),
     "cfgpu_tuning_set": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Tuning)]),
     "cfgpu_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32,
@@ -492,6 +499,24 @@ class Handle:
         s = RunStats()
         self._check(lib().cfgpu_run_stats_get(self._h, ctypes.byref(s)), "run_stats")
         return s.as_dict()
+
+    def sizing_model(self):
+        """The trained MLP predictor: dict of its settings and arrays."""
+        info = SizingModel()
+        self._check(lib().cfgpu_sizing_model_get(self._h, ctypes.byref(info), None, None, None, None, None),
+                    "sizing_model_get")
+        params = np.zeros(info.n_params, np.float32)
+        feat = np.zeros((G, info.n_features), np.float32)
+        idx = np.zeros(max(info.n_train, 1), np.uint32)
+        tgt = np.zeros(max(info.n_train, 1), np.float32)
+        pred = np.zeros(G, np.float32)
+        self._check(lib().cfgpu_sizing_model_get(self._h, ctypes.byref(info), params.ctypes.data, feat.ctypes.data,
+                                                 _p(idx, u32p), tgt.ctypes.data, pred.ctypes.data),
+                    "sizing_model_get")
+        out = {n: getattr(info, n) for n, _ in SizingModel._fields_}
+        out.update(params=params, features=feat, train_buckets=idx[:info.n_train], train_targets=tgt[:info.n_train],
+                   predictions=pred)
+        return out
 
     def tuning(self):
         t = Tuning()

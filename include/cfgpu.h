@@ -259,6 +259,24 @@ typedef struct cfgpu_run_stats {
 } cfgpu_run_stats;
 int cfgpu_run_stats_get(cfgpu_handle* h, cfgpu_run_stats* out);
 
+/* The FP32 MLP bucket-size predictor of the last learned construct that trained one
+ * (tuning.predictor 1, or 0 when the constant ratio missed the held-out buckets): the
+ * slot of the reference's HyperLogLog estimate (include/HyperLogLog.hpp:29-58;
+ * Subgraphs_Manager::estimate_size, disabled at src/Subgraphs_Manager.cpp:98-115).
+ * Network n_features -> hidden -> hidden -> 1 with ReLU, predicting
+ * log(distinct vertices / k-mer instances) of a bucket from its features, trained
+ * full-batch with Adam (lr, epochs, weight init from seed) on the listed buckets.
+ * params: W1[hidden][n_features] b1 W2[hidden][hidden] b2 w3 b3, then the training
+ * set's feature mean[n_features] and 1/std[n_features]; features: 16384 x n_features
+ * (16-bin minimizer-hash histogram, records, k-mer instances, mean length);
+ * predictions: 16384 (every bucket).  Any output pointer may be NULL. */
+typedef struct cfgpu_sizing_model {
+    uint32_t n_features, hidden, n_params, n_train, epochs, seed;
+    float lr, loss;
+} cfgpu_sizing_model;
+int cfgpu_sizing_model_get(cfgpu_handle* h, cfgpu_sizing_model* info, float* params, float* features,
+                           uint32_t* train_buckets, float* train_targets, float* predictions);
+
 /* Launch-shape knobs (defaults chosen by cfgpu_autotune or built-in). */
 typedef struct cfgpu_tuning {
     float load_factor;        /* target table load for sized tables (0.3 .. 0.8) */
@@ -278,6 +296,8 @@ typedef struct cfgpu_tuning {
     uint32_t dedup;           /* batched paths: walk each bucket's distinct records once with
                                  their multiplicity.  0 off, 1 when sampled buckets repeat
                                  records at least 1.33x, 2 always (results never change) */
+    uint32_t mlp_epochs;      /* learned sizing: MLP training epochs (1 .. 10000) */
+    float mlp_lr;             /* learned sizing: Adam learning rate (0 < lr <= 1) */
 } cfgpu_tuning;
 int cfgpu_tuning_get(cfgpu_handle* h, cfgpu_tuning* out);
 int cfgpu_tuning_set(cfgpu_handle* h, const cfgpu_tuning* in);

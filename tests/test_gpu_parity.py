@@ -257,13 +257,17 @@ def test_repeat_calls_are_idempotent_and_release():
         assert len(h.tables(buckets=[int(nz[1])])["g"]) > 0
 
 
-def test_record_dedup_is_exact():
+@pytest.mark.parametrize("k,l", [(31, 20), (37, 20), (21, 11)])
+def test_record_dedup_is_exact(k, l):
     # High-coverage reads repeat most weak super k-mers: the deduplicated walk
     # (always / auto) must give the same tables as walking every record.
+    # k=31, l=20: records of at most 42 bases leave at most 10 in the label's
+    # last word, so the dedup key packs len, disc bits and hash bits into it;
+    # k=37 (22 bases there, 2 words) and k=21, l=11 (31 bases, 1 word) use the
+    # key with a separate length / hash word.
     g = C.synth_genome(40000, 0.1, 300, 2000, 0.02, seed=71)
     reads = C.synth_reads(g, 20000, 150, 0.002, seed=72)
     buf = fastq(reads, 150)
-    k, l = 31, 20
     seq, ro, part = oracle_records(buf, True, k, l)
     inp = C.Input.from_buffer(buf, True, k)
     say("[case] dedup")
@@ -277,6 +281,7 @@ def test_record_dedup_is_exact():
                 h.construct(mode)
                 want = check_tables(h, part, k, l, 2, want)
                 rs = h.run_stats()
+                say(f"[case] dedup k={k} walked {rs['n_records_walked']} of {rs['n_records']}")
                 if dd == 0:
                     assert rs["n_records_walked"] == rs["n_records"]
                 else:
