@@ -32,7 +32,6 @@ constexpr double CONST_RATIO_GOOD = 0.05;
 constexpr uint32_t MAX_ROUNDS = 8;
 // MLP weight initialisation seed (epochs and learning rate are tuning knobs)
 constexpr uint32_t MLP_SEED = 12345u;
-constexpr double LDS_LOAD = 0.8;  // target fill of an LDS pass table (overflow at 7/8)
 
 struct DevBuf {
     void* p = nullptr;
@@ -101,7 +100,8 @@ struct cfgpu_handle {
     int cus = 256;
     hipStream_t stream = nullptr;
     std::string err;
-    cfgpu_tuning tune{0.6f, 256, 2048, 8, 1.15f, 1, LDS_ALL_PASS_MAX, 0, 1, 120, 3e-3f};
+    // target fill of an LDS pass table 0.8 (a table overflows at 7/8)
+    cfgpu_tuning tune{0.6f, 256, 2048, 8, 1.15f, 1, LDS_ALL_PASS_MAX, 0, 1, 120, 3e-3f, 0.8f};
 
     // partition state
     DevBuf packed, frag, gbin, mbn, tile_lc, tile_frag, boff, att, labels, mbin, stats, feat;
@@ -584,7 +584,7 @@ int run_main(cfgpu_handle* h, const std::vector<uint32_t>& order, const std::vec
     lds_c.reserve(order.size());
     for (uint32_t b : order) {
         const double need = std::max(1.0, est[b] * margin);
-        const uint32_t P = uint32_t(std::ceil(need / (C * LDS_LOAD)));
+        const uint32_t P = uint32_t(std::ceil(need / (C * double(h->tune.lds_load))));
         if (P <= h->tune.max_passes) {
             lds_b.push_back(b);
             lds_c.push_back(uint64_t(P) * uint64_t(C));
@@ -881,7 +881,7 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
             std::vector<uint64_t> lds_c, glob_c;
             for (uint32_t b : sample) {
                 const uint64_t inst = std::max<uint64_t>(1, walked_kmers(h, b));
-                const uint32_t P = uint32_t(std::ceil(double(inst) / (C * LDS_LOAD)));
+                const uint32_t P = uint32_t(std::ceil(double(inst) / (C * double(h->tune.lds_load))));
                 if (P <= h->tune.max_passes) {
                     lds_s.push_back(b);
                     lds_c.push_back(uint64_t(P) * uint64_t(C));
@@ -1789,7 +1789,7 @@ int cfgpu_tuning_set(cfgpu_handle* h, const cfgpu_tuning* in) {
         in->chunk_records > (1u << 24) || in->wgs_per_cu < 1 || in->wgs_per_cu > 32 ||
         !(in->margin >= 0.01f && in->margin <= 4.0f) || in->use_graph > 1 || in->max_passes > 4096 ||
         in->predictor > 2 || in->dedup > 2 || in->mlp_epochs < 1 || in->mlp_epochs > 10000 ||
-        !(in->mlp_lr > 0.f && in->mlp_lr <= 1.f))
+        !(in->mlp_lr > 0.f && in->mlp_lr <= 1.f) || !(in->lds_load >= 0.5f && in->lds_load <= 0.85f))
         return fail(h, CFGPU_E_ARG, "tuning out of range");
     h->tune = *in;
     return CFGPU_OK;
@@ -1800,27 +1800,43 @@ int cfgpu_autotune(cfgpu_handle* h, uint32_t mode, cfgpu_tune_trial* trials_out,
     if (!h) return CFGPU_E_ARG;
     if (!h->partitioned) return fail(h, CFGPU_E_STATE, "autotune before partition");
     const cfgpu_tuning base = h->tune;
-    std::vector<cfgpu_tuning> grid;
-    for (float lf : {0.5f, 0.65f})
-        for (uint32_t wg : {256u, 512u})
-            for (uint32_t ch : {1024u, 4096u})
-                for (uint32_t wpc : {4u, 8u}) {
-                    cfgpu_tuning t = base;
-                    t.load_factor = lf, t.workgroup = wg, t.chunk_records = ch, t.wgs_per_cu = wpc;
-                    grid.push_back(t);
-                }
+    // the base tuning first: its vertex counts are the ones every trial must give
+    std::vector<cfgpu_tuning> grid = {base};
+    for (float ll : {0.7f, 0.8f, 0.85f})
+        for (float mg : {1.05f, 1.15f, 1.3f}) {
+            cfgpu_tuning t = base;
+            t.lds_load = ll, t.margin = mg;
+            grid.push_back(t);
+        }
+    int r = construct_impl(h, mode);
+    if (r != CFGPU_OK) return r;
+    if (h->rs.n_lds_buckets < h->rs.n_buckets_nonempty)  // some buckets take the global-atomic kernel
+        for (float lf : {0.5f, 0.65f})
+            for (uint32_t wg : {256u, 512u})
+                for (uint32_t ch : {1024u, 4096u})
+                    for (uint32_t wpc : {4u, 8u}) {
+                        cfgpu_tuning t = base;
+                        t.load_factor = lf, t.workgroup = wg, t.chunk_records = ch, t.wgs_per_cu = wpc;
+                        grid.push_back(t);
+                    }
+    const std::vector<uint32_t> nv0 = h->h_nvert;
     cfgpu_tuning best = base;
     double best_ms = 1e300;
     uint32_t w = 0;
     for (const cfgpu_tuning& t : grid) {
         h->tune = t;
-        const int r = construct_impl(h, mode);
+        r = construct_impl(h, mode);
         if (r != CFGPU_OK) {
             h->tune = base;
             return r;
         }
-        const double ms = h->rs.t_construct_main_ms + h->rs.t_fallback_ms;
-        if (trials_out && w < n_trials) trials_out[w++] = {t, ms};
+        const double ms = h->rs.t_construct_ms + h->rs.t_dedup_ms;
+        const uint32_t same = h->h_nvert == nv0 ? 1u : 0u;
+        if (trials_out && w < n_trials) trials_out[w++] = {t, ms, same, 0};
+        if (!same) {
+            h->tune = base;
+            return fail(h, CFGPU_E_STATE, "autotune: a tuning changed the vertex counts");
+        }
         if (ms < best_ms) best_ms = ms, best = t;
     }
     if (n_written) *n_written = w;

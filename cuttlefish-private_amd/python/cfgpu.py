@@ -87,7 +87,8 @@ class Tuning(ctypes.Structure):
                 ("chunk_records", ctypes.c_uint32), ("wgs_per_cu", ctypes.c_uint32),
                 ("margin", ctypes.c_float), ("use_graph", ctypes.c_uint32),
                 ("max_passes", ctypes.c_uint32), ("predictor", ctypes.c_uint32),
-                ("dedup", ctypes.c_uint32), ("mlp_epochs", ctypes.c_uint32), ("mlp_lr", ctypes.c_float)]
+                ("dedup", ctypes.c_uint32), ("mlp_epochs", ctypes.c_uint32), ("mlp_lr", ctypes.c_float),
+                ("lds_load", ctypes.c_float)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -99,7 +100,8 @@ class SizingModel(ctypes.Structure):
 
 
 class TuneTrial(ctypes.Structure):
-    _fields_ = [("tuning", Tuning), ("ms", ctypes.c_double)]
+    _fields_ = [("tuning", Tuning), ("ms", ctypes.c_double), ("identical", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 # name -> (restype, argtypes); the set of names must equal the header's.
@@ -535,7 +537,7 @@ class Handle:
         w = ctypes.c_uint32()
         self._check(lib().cfgpu_autotune(self._h, MODES.get(mode, mode), arr, n, ctypes.byref(w)),
                     "autotune")
-        return [dict(arr[i].tuning.as_dict(), ms=arr[i].ms) for i in range(w.value)]
+        return [dict(arr[i].tuning.as_dict(), ms=arr[i].ms, identical=bool(arr[i].identical)) for i in range(w.value)]
 
     def __del__(self):
         try:

@@ -298,15 +298,23 @@ typedef struct cfgpu_tuning {
                                  records at least 1.33x, 2 always (results never change) */
     uint32_t mlp_epochs;      /* learned sizing: MLP training epochs (1 .. 10000) */
     float mlp_lr;             /* learned sizing: Adam learning rate (0 < lr <= 1) */
+    float lds_load;           /* batched paths: target fill of an LDS pass table (0.5 .. 0.85);
+                                 fewer, fuller passes or more, emptier ones */
 } cfgpu_tuning;
 int cfgpu_tuning_get(cfgpu_handle* h, cfgpu_tuning* out);
 int cfgpu_tuning_set(cfgpu_handle* h, const cfgpu_tuning* in);
-/* Times the batched construct on the current partition for a small grid of tunings
- * (results never change) and keeps the fastest.  `trials_out` (may be NULL) receives
- * up to n_trials (tuning, ms) pairs. */
+/* Times the construct in `mode` on the current partition for a grid of tunings and
+ * keeps the fastest (whole construct: dedup, samples, main round, fallbacks).  The
+ * grid is the LDS pass-table load x the learned sizing margin, plus the global-atomic
+ * kernel's load factor x workgroup x chunk x workgroups per CU when any bucket takes
+ * that kernel.  Results never change with the tuning: each trial's per-bucket vertex
+ * counts are compared with the first trial's (`identical`).  `trials_out` (may be
+ * NULL) receives up to n_trials trials. */
 typedef struct cfgpu_tune_trial {
     cfgpu_tuning tuning;
     double ms;
+    uint32_t identical;   /* 1: every bucket's vertex count equals the first trial's */
+    uint32_t reserved;
 } cfgpu_tune_trial;
 int cfgpu_autotune(cfgpu_handle* h, uint32_t mode, cfgpu_tune_trial* trials_out,
                    uint32_t n_trials, uint32_t* n_written);

@@ -348,6 +348,29 @@ def test_many_pass_buckets_walk_once(k):
                 assert rs["n_lds_all_pass"] >= int(((passes >= 2) & (passes <= 64)).sum())
 
 
+def test_autotune_keeps_results_and_keeps_the_fastest():
+    # Zipf buckets with max_passes 4: the largest take the global-atomic kernel,
+    # so both parts of the grid run (LDS load x margin, global-kernel shapes).
+    # Every trial must build the same vertex counts, the kept tuning must be the
+    # fastest trial's, and the final tables must equal the oracle's.
+    k, l = 31, 20
+    att, lab, off = C.synth_bucket_workload(512, 80000, 1.1, k, l, seed=17)
+    want = O.construct(k, l, off, (att & 0xFFFF).astype(np.uint16), lab)
+    with C.Handle(k, l, 1) as h:
+        h.load_buckets(att, lab, off)
+        h.set_tuning(max_passes=4)
+        trials = h.autotune("learned")
+        say(f"[case] autotune: {len(trials)} trials, fastest {min(t['ms'] for t in trials):.3f} ms")
+        assert len(trials) == 1 + 9 + 16
+        assert all(t["identical"] for t in trials)
+        best = min(trials, key=lambda t: t["ms"])
+        tun = h.tuning()
+        for key in ("lds_load", "margin", "load_factor", "workgroup", "chunk_records", "wgs_per_cu"):
+            assert tun[key] == pytest.approx(best[key]), key
+        assert h.run_stats()["n_lds_buckets"] < h.run_stats()["n_buckets_nonempty"]
+        check_tables(h, None, k, l, 1, want)
+
+
 def test_load_buckets_rejects_malformed_records():
     att, lab, off = C.synth_bucket_workload(8, 100, 1.1, 31, 20, seed=1)
     with C.Handle(31, 20, 1) as h:
