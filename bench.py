@@ -58,6 +58,27 @@ def measured_traffic(path, kernel):
     return None
 
 
+ATOMICS_PROFILE = "atomics_reads50M_k63_and_global_path.txt"
+
+
+def measured_atomics(path):
+    """Atomics per launch of the LDS construct and the global-atomic kernel's
+    rate, from the committed rocprofv3 summary (None when absent)."""
+    out = {}
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("atomics_per_launch k_construct_lds<2, 4>:"):
+                    for kv in line.split(":", 1)[1].split():
+                        k_, v = kv.split("=")
+                        out[k_] = float(v)
+                elif line.startswith("global_atomic_rate"):
+                    out["global_rate"] = float(line.rsplit(":", 1)[1])
+    except OSError:
+        return None
+    return out or None
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -332,6 +353,8 @@ def main():
     h.close()
     traffic_path = os.path.join(REPO, "profiles", w["traffic"])
     traffic = measured_traffic(traffic_path, w["roofline_kernel"])
+    atom_path = os.path.join(REPO, "profiles", ATOMICS_PROFILE)
+    atom = measured_atomics(atom_path) if args.workload == "c4" else None
     other_ms = lambda c: part_ms + c + ph["t_contract_ms"] + ph["t_cdbg_ms"] + wall["copy_to_host"]  # noqa: E731
     res = {
         "metric": "k-mers/sec to bit-exact cdBG (partition + construct + contract + gluing + unitigs copied to "
@@ -359,7 +382,17 @@ def main():
                      "traffic_source": os.path.relpath(traffic_path, REPO) if traffic else None,
                      "kernel": f"construct main round ({w['roofline_kernel']}: hash-insert of every walked "
                                "k-mer instance in LDS pass tables)", "kernel_ms": kms,
-                     "algorithmic_bytes": kbytes},
+                     "algorithmic_bytes": kbytes,
+                     "atomics": None if not atom else {
+                         "device_scope_per_launch": atom.get("tcc"),
+                         "lds_wave_instructions_per_launch": atom.get("lds_wave_instructions"),
+                         "lds_atomic_wave_instructions_per_s": atom.get("lds_wave_instructions", 0) / (kms / 1e3)
+                         if kms else None,
+                         "global_atomic_kernel_rate_per_s": atom.get("global_rate"),
+                         "source": os.path.relpath(atom_path, REPO),
+                         "note": "the hash-insert kernel's counters and states live in LDS (no device-scope "
+                                 "atomics); the global-atomic construct (max_passes 0, 10M reads k=31) sustains "
+                                 "the device-scope rate given"}},
         "cpu_baseline": cpu,
         "device_memory_gib": mem_gib,
         "phases_ms": {"partition": part_ms, "minimizer": ph["t_minimizer_ms"], "count": ph["t_count_ms"],
