@@ -447,6 +447,109 @@ void construct_literal(unsigned k, unsigned W, uint32_t g, size_t n, const uint1
     emit(M, g, out);
 }
 
+// ---------------------------------------------------------------------------
+// The same construct with an open-addressing map in the class of the
+// reference's ankerl::unordered_dense (include/Subgraph.hpp:53): linear
+// probing over a power-of-two slot array, grown by doubling past 3/4 load,
+// kept by a worker across subgraphs and cleared, not freed, for the next one
+// (Subgraph's constructor clears the worker's map, src/Subgraph.cpp:20-40).
+// Only the speed differs from construct_literal: the states are the same.
+struct FlatMap::Impl {
+    std::vector<Km> key;
+    std::vector<State> val;
+    std::vector<uint8_t> used;
+    size_t n = 0, mask = 0;
+};
+FlatMap::FlatMap() : p(new Impl) {}
+FlatMap::~FlatMap() { delete p; }
+namespace {
+inline size_t km_hash_cpu(const Km& x) {
+    uint64_t h = x.w[0] * 0x9E3779B97F4A7C15ull;
+    for (int i = 1; i < 4; ++i) h = (h ^ (h >> 29) ^ x.w[i]) * 0xBF58476D1CE4E5B9ull;
+    return size_t(h ^ (h >> 32));
+}
+}  // namespace
+void FlatMap::clear(size_t expect) {
+    size_t cap = 64;
+    while (cap * 3 < expect * 4) cap <<= 1;
+    if (cap > p->key.size()) {
+        p->key.assign(cap, Km{});
+        p->val.assign(cap, State{});
+        p->used.assign(cap, 0);
+    } else {
+        cap = p->key.size();
+        std::fill(p->used.begin(), p->used.end(), 0);
+    }
+    p->mask = cap - 1;
+    p->n = 0;
+}
+State& FlatMap::at(const Km& x) {
+    if ((p->n + 1) * 4 > (p->mask + 1) * 3) {  // grow by doubling, reinserting
+        std::vector<Km> k2;
+        std::vector<State> v2;
+        for (size_t i = 0; i <= p->mask; ++i)
+            if (p->used[i]) k2.push_back(p->key[i]), v2.push_back(p->val[i]);
+        const size_t cap = (p->mask + 1) * 2;
+        p->key.assign(cap, Km{});
+        p->val.assign(cap, State{});
+        p->used.assign(cap, 0);
+        p->mask = cap - 1;
+        p->n = 0;
+        for (size_t i = 0; i < k2.size(); ++i) at(k2[i]) = v2[i];
+    }
+    size_t s = km_hash_cpu(x) & p->mask;
+    while (p->used[s]) {
+        if (p->key[s] == x) return p->val[s];
+        s = (s + 1) & p->mask;
+    }
+    p->used[s] = 1;
+    p->key[s] = x;
+    p->val[s] = State{};
+    ++p->n;
+    return p->val[s];
+}
+void FlatMap::emit(uint32_t g, std::vector<VertexState>& out) const {
+    for (size_t i = 0; i <= p->mask; ++i)
+        if (p->used[i]) out.push_back({g, p->key[i], p->val[i]});
+}
+
+void construct_flat(unsigned k, unsigned W, uint32_t g, size_t n, const uint16_t* att, const uint64_t* labels,
+                    FlatMap& M, std::vector<VertexState>& out) {
+    const KmOps ops(k);
+    size_t inst = 0;
+    for (size_t r = 0; r < n; ++r) inst += (att[r] & 0xFF) - k + 1;
+    M.clear(inst / 2);
+    for (size_t r = 0; r < n; ++r) {
+        const uint64_t* label = labels + r * W;
+        const unsigned len = att[r] & 0xFF;
+        const bool l_disc = (att[r] >> 8) & 1, r_disc = (att[r] >> 9) & 1;
+        Km v, vbar;
+        for (unsigned i = 0; i < k; ++i) ops.roll_fwd(v, label_base(label, W, i));
+        vbar = ops.rc(v);
+        Km pred_v;
+        for (size_t idx = 0;; ++idx) {
+            const bool is_can = ops.less(v, vbar);
+            const Km& can = is_can ? v : vbar;
+            const uint8_t pred = idx == 0 ? BASE_E : label_base(label, W, idx - 1);
+            const uint8_t succ = idx + k == len ? BASE_E : label_base(label, W, idx + k);
+            uint8_t front = is_can ? pred : complement(succ);
+            uint8_t back = is_can ? succ : complement(pred);
+            if (idx > 0 && can == pred_v) (is_can ? front : back) = BASE_E;
+            State& st = M.at(can);
+            if (front != BASE_E) add_edge(st.e_f, FRONT, front);
+            if (back != BASE_E) add_edge(st.e_f, BACK, back);
+            const unsigned entrance = is_can ? FRONT : BACK, exit_ = is_can ? BACK : FRONT;
+            mark_disc(st, (idx == 0 && l_disc) ? entrance : UNSPEC);
+            mark_disc(st, (idx + k == len && r_disc) ? exit_ : UNSPEC);
+            if (idx + k == len) break;
+            pred_v = can;
+            ops.roll_fwd(v, succ);
+            ops.roll_rc(vbar, succ);
+        }
+    }
+    M.emit(g, out);
+}
+
 void construct_closed_form(unsigned k, unsigned W, uint32_t g, size_t n, const uint16_t* att,
                            const uint64_t* labels, std::vector<VertexState>& out) {
     const KmOps ops(k), eops(k + 1);

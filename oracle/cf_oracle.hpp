@@ -148,6 +148,23 @@ struct VertexState {
 // `att` are packed attributes, `labels` W words per record, all of subgraph g.
 void construct_literal(unsigned k, unsigned W, uint32_t g, size_t n, const uint16_t* att,
                        const uint64_t* labels, std::vector<VertexState>& out);
+// The same construct over a worker's reusable open-addressing map (the CPU
+// baseline's shape: the reference's ankerl::unordered_dense class).
+class FlatMap {
+  public:
+    FlatMap();
+    ~FlatMap();
+    FlatMap(const FlatMap&) = delete;
+    FlatMap& operator=(const FlatMap&) = delete;
+    void clear(size_t expect);
+    State& at(const Km& x);
+    void emit(uint32_t g, std::vector<VertexState>& out) const;
+  private:
+    struct Impl;
+    Impl* p;
+};
+void construct_flat(unsigned k, unsigned W, uint32_t g, size_t n, const uint16_t* att, const uint64_t* labels,
+                    FlatMap& M, std::vector<VertexState>& out);
 // Closed form: canonical (k+1)-mer counts (both orientations) with the
 // palindrome and homopolymer exceptions, then e_f = min(15,count) per slot.
 void construct_closed_form(unsigned k, unsigned W, uint32_t g, size_t n, const uint16_t* att,

@@ -126,7 +126,7 @@ int64_t cfo_partition(const char* seq, const uint64_t* rec_off, size_t n_rec, in
 
 // Construct all subgraphs from bucketed records (bucket_off[G+1], records of
 // bucket g at [bucket_off[g], bucket_off[g+1])).  mode 0 = literal, 1 = closed
-// form.  Output sorted by (g, key); keys as 4 words (Kmer layout, w[0] low).
+// form, 2 = literal over the reusable open-addressing map (the CPU baseline's).  Output sorted by (g, key); keys as 4 words (Kmer layout, w[0] low).
 int64_t cfo_construct(int k, int l, uint32_t G, int mode, const uint64_t* bucket_off,
                       const uint16_t* att, const uint64_t* labels, int threads, uint32_t* g_out,
                       uint64_t* key_out, uint32_t* ef_out, uint8_t* st_out, size_t cap) {
@@ -135,13 +135,15 @@ int64_t cfo_construct(int k, int l, uint32_t G, int mode, const uint64_t* bucket
     std::vector<std::vector<VertexState>> per_t(threads);
     std::atomic<uint32_t> next{0};
     auto work = [&](int t) {
+        FlatMap M;
         while (true) {
             const uint32_t g = next.fetch_add(1);
             if (g >= G) break;
             const uint64_t b = bucket_off[g], e = bucket_off[g + 1];
             if (b == e) continue;
             if (mode == 0) construct_literal(k, W, g, e - b, att + b, labels + b * W, per_t[t]);
-            else construct_closed_form(k, W, g, e - b, att + b, labels + b * W, per_t[t]);
+            else if (mode == 1) construct_closed_form(k, W, g, e - b, att + b, labels + b * W, per_t[t]);
+            else construct_flat(k, W, g, e - b, att + b, labels + b * W, M, per_t[t]);
         }
     };
     std::vector<std::thread> th;
@@ -207,6 +209,126 @@ int cfo_states_hash(const uint32_t* g, const uint64_t* keys, int kw, const uint3
                         S[g[i]] += state_mix(keys + i * uint64_t(kw), unsigned(kw), e_f[i], status[i]);
                         ++N[g[i]];
                     });
+    return 0;
+}
+
+// CPU baseline in the reference's shape, per phase:
+//  1. partition on part_threads workers over contiguous record ranges (the
+//     reference partitions one input file per worker, src/Graph_Partitioner.cpp:
+//     121-184: part_threads = 1 for a single read file; = threads is the
+//     parallel variant);
+//  2. bucketize by g (Atlas / Super_Kmer_Bucket);
+//  3. construct every subgraph with each worker's reusable open-addressing
+//     map, buckets largest first (src/Subgraphs_Manager.cpp:222-255);
+//  4. contract every subgraph's table (Subgraph::contract restated).
+// out[0..8] = partition ms, bucketize ms, construct ms, contract ms, k-mer
+// instances, weak super k-mers, vertex states, unitigs, construct + contract
+// wall ms when run together per subgraph.
+int cfo_baseline(const char* seq, const uint64_t* rec_off, size_t n_rec, int k, int l, int f_th, int part_threads,
+                 int threads, double* out) {
+    using clk = std::chrono::steady_clock;
+    const unsigned W = label_words(k, l);
+    const uint32_t G = 16384;
+    if (threads < 1) threads = 1;
+    if (part_threads < 1) part_threads = 1;
+    struct Part { std::vector<uint32_t> g; std::vector<uint16_t> att; std::vector<uint64_t> lab; };
+    std::vector<Part> parts(part_threads);
+    const auto t0 = clk::now();
+    {
+        std::vector<std::thread> th;
+        for (int t = 0; t < part_threads; ++t)
+            th.emplace_back([&, t]() {
+                Part& P = parts[t];
+                std::vector<SuperKmer> sk;
+                const size_t r0 = n_rec * t / part_threads, r1 = n_rec * (t + 1) / part_threads;
+                for (size_t r = r0; r < r1; ++r) {
+                    const char* rs = seq + rec_off[r];
+                    for (const auto& f : fragments(rs, rec_off[r + 1] - rec_off[r], k)) {
+                        sk.clear();
+                        partition_fragment(rs + f.first, f.second, k, l, G, sk);
+                        for (const auto& x : sk) {
+                            P.g.push_back(x.g);
+                            P.att.push_back(att_pack(x.len, x.l_disc, x.r_disc));
+                            const size_t o = P.lab.size();
+                            P.lab.resize(o + W);
+                            encode_label(rs + f.first + x.off, x.len, W, &P.lab[o]);
+                        }
+                    }
+                }
+            });
+        for (auto& x : th) x.join();
+    }
+    const auto t1 = clk::now();
+    std::vector<uint64_t> off(G + 1, 0);
+    for (const auto& P : parts) for (uint32_t g : P.g) ++off[g + 1];
+    for (uint32_t g = 0; g < G; ++g) off[g + 1] += off[g];
+    const uint64_t n_sk = off[G];
+    std::vector<uint16_t> att(n_sk);
+    std::vector<uint64_t> lab(n_sk * W);
+    {
+        std::vector<uint64_t> cur(off.begin(), off.end() - 1);
+        for (const auto& P : parts)
+            for (size_t i = 0; i < P.g.size(); ++i) {
+                const uint64_t d = cur[P.g[i]]++;
+                att[d] = P.att[i];
+                std::memcpy(&lab[d * W], &P.lab[i * W], W * 8);
+            }
+    }
+    parts.clear();
+    const auto t2 = clk::now();
+    std::vector<uint32_t> order(G);
+    for (uint32_t g = 0; g < G; ++g) order[g] = g;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return off[a + 1] - off[a] > off[b + 1] - off[b];
+    });
+    std::atomic<uint32_t> next{0};
+    std::atomic<uint64_t> states{0}, unitigs{0}, ns_con{0}, ns_ctr{0};
+    {
+        std::vector<std::thread> th;
+        for (int t = 0; t < threads; ++t)
+            th.emplace_back([&]() {
+                FlatMap M;
+                std::vector<VertexState> vs;
+                ContractResult res;
+                uint64_t s = 0, u = 0, c0 = 0, c1 = 0;
+                while (true) {
+                    const uint32_t i = next.fetch_add(1);
+                    if (i >= G) break;
+                    const uint32_t g = order[i];
+                    if (off[g + 1] == off[g]) continue;
+                    const auto a = clk::now();
+                    vs.clear();
+                    construct_flat(k, W, g, off[g + 1] - off[g], &att[off[g]], &lab[off[g] * W], M, vs);
+                    const auto b = clk::now();
+                    res = ContractResult{};
+                    contract_subgraph(k, f_th, vs.data(), vs.size(), res);
+                    const auto c = clk::now();
+                    s += vs.size();
+                    u += res.unitigs.size();
+                    c0 += uint64_t(std::chrono::duration_cast<std::chrono::nanoseconds>(b - a).count());
+                    c1 += uint64_t(std::chrono::duration_cast<std::chrono::nanoseconds>(c - b).count());
+                }
+                states += s;
+                unitigs += u;
+                ns_con += c0;
+                ns_ctr += c1;
+            });
+        for (auto& x : th) x.join();
+    }
+    const auto t3 = clk::now();
+    uint64_t n_km = 0;
+    for (uint64_t i = 0; i < n_sk; ++i) n_km += (att[i] & 0xFF) - (k - 1);
+    auto ms = [](clk::duration d) { return std::chrono::duration<double, std::milli>(d).count(); };
+    const double wall = ms(t3 - t2), share = double(ns_con) / double(std::max<uint64_t>(1, ns_con + ns_ctr));
+    out[0] = ms(t1 - t0);
+    out[1] = ms(t2 - t1);
+    out[2] = wall * share;          // the workers' construct share of the joint phase
+    out[3] = wall * (1.0 - share);  // ... and their contract share
+    out[4] = double(n_km);
+    out[5] = double(n_sk);
+    out[6] = double(states.load());
+    out[7] = double(unitigs.load());
+    out[8] = wall;
     return 0;
 }
 

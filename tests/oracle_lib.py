@@ -60,6 +60,9 @@ def lib():
         L.cfo_states_hash.restype = ctypes.c_int
         L.cfo_states_hash.argtypes = [u32p, u64p, ctypes.c_int, u32p, u8p, ctypes.c_uint64, ctypes.c_int, u64p,
                                       u64p]
+        L.cfo_baseline.restype = ctypes.c_int
+        L.cfo_baseline.argtypes = [ctypes.c_char_p, u64p, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.cfo_edge_at.restype = ctypes.c_uint32
         L.cfo_edge_at.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
         L.cfo_contract.restype = ctypes.c_int64
@@ -226,4 +229,21 @@ def states_hash(t, threads=16):
     lib().cfo_states_hash(_ptr(g, u32p), _ptr(keys, u64p), keys.shape[1] if keys.ndim == 2 else 1, _ptr(ef, u32p),
                           _ptr(st, u8p), len(g), threads, _ptr(s, u64p), _ptr(c, u64p))
     return s, c
+
+
+def baseline(seq: bytes, rec_off, k, l, cutoff, part_threads, threads):
+    """CPU baseline in the reference's shape (partition on part_threads workers,
+    bucketize, per-subgraph construct with a reusable open-addressing map and
+    contract, largest buckets first, on `threads` workers); times in ms."""
+    ro = np.ascontiguousarray(rec_off, dtype=np.uint64)
+    out = (ctypes.c_double * 9)()
+    r = lib().cfo_baseline(seq, _ptr(ro, u64p), len(ro) - 1, k, l, cutoff, part_threads, threads, out)
+    if r != 0:
+        raise RuntimeError("cfo_baseline failed")
+    keys = ("partition_ms", "bucketize_ms", "construct_ms", "contract_ms", "n_kmers", "n_superkmers",
+            "n_states", "n_unitigs", "construct_contract_ms")
+    d = dict(zip(keys, list(out)))
+    for kk in ("n_kmers", "n_superkmers", "n_states", "n_unitigs"):
+        d[kk] = int(d[kk])
+    return d
 

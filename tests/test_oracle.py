@@ -245,8 +245,10 @@ def test_construct_three_ways(k, l):
     off, att, lab = O.bucketize(part)
     lit = O.construct(k, l, off, att, lab, mode=0)
     cf = O.construct(k, l, off, att, lab, mode=1)
+    flat = O.construct(k, l, off, att, lab, mode=2)  # the CPU baseline's open-addressing map
     for key in ("g", "keys", "e_f", "status"):
         np.testing.assert_array_equal(lit[key], cf[key])
+        np.testing.assert_array_equal(lit[key], flat[key])
     _stitch_check(seq, ro, k, l, lit)
 
 
