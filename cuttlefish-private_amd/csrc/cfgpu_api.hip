@@ -794,7 +794,9 @@ int dedup_buckets(cfgpu_handle* h, const std::vector<uint32_t>& all) {
     return CFGPU_OK;
 }
 
-int construct_impl(cfgpu_handle* h, uint32_t mode) {
+// Construct the listed buckets (all non-empty ones when subset is null);
+// every other bucket is left without a table.
+int construct_impl(cfgpu_handle* h, uint32_t mode, const uint32_t* subset = nullptr, uint32_t n_subset = 0) {
     const size_t G = h->G;
     h->round_idx = 0;
     h->h_tbl_ptr.assign(G, 0);
@@ -821,7 +823,15 @@ int construct_impl(cfgpu_handle* h, uint32_t mode) {
     rs.t_construct_kernel_ms = 0;
     rs.n_lds_buckets = 0;
 
-    std::vector<uint32_t> all = nonempty_buckets(h);
+    std::vector<uint32_t> all;
+    if (subset) {
+        std::vector<char> in(G, 0);
+        for (uint32_t i = 0; i < n_subset; ++i) in[subset[i]] = 1;
+        for (uint32_t b = 0; b < G; ++b)
+            if (in[b] && bucket_records(h, b)) all.push_back(b);
+    } else {
+        all = nonempty_buckets(h);
+    }
     rs.n_buckets_nonempty = uint32_t(all.size());
     h->dd_active = false;
     rs.n_records_walked = h->n_records;
@@ -1290,6 +1300,16 @@ int cfgpu_construct(cfgpu_handle* h, uint32_t mode) {
     HIPCK(h, hipSetDevice(h->prm.device));
     h->constructed = h->contracted = h->cdbg_done = false;
     return construct_impl(h, mode);
+}
+
+int cfgpu_construct_buckets(cfgpu_handle* h, uint32_t mode, const uint32_t* buckets, uint32_t n) {
+    if (!h || (n && !buckets)) return CFGPU_E_ARG;
+    if (!h->partitioned) return fail(h, CFGPU_E_STATE, "construct before partition");
+    for (uint32_t i = 0; i < n; ++i)
+        if (buckets[i] >= h->G) return fail(h, CFGPU_E_ARG, "construct: bucket %u out of range", buckets[i]);
+    HIPCK(h, hipSetDevice(h->prm.device));
+    h->constructed = h->contracted = h->cdbg_done = false;
+    return construct_impl(h, mode, buckets, n);
 }
 
 static_assert(sizeof(cfgpu_unitig) == sizeof(UnitigRec) && sizeof(cfgpu_unitig) == 16, "cfgpu_unitig layout");

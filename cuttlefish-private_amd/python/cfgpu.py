@@ -117,6 +117,7 @@ SIGNATURES = {
                                           u64p]),
     "cfgpu_load_buckets": (ctypes.c_int, [ctypes.c_void_p, u32p, u64p, u64p]),
     "cfgpu_construct": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "cfgpu_construct_buckets": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, u32p, ctypes.c_uint32]),
     "cfgpu_table_view_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32,
                                             ctypes.POINTER(TableView)]),
     "cfgpu_table_copy": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_uint32, u64p, u32p, u8p,
@@ -392,6 +393,13 @@ class Handle:
 
     def construct(self, mode="learned"):
         self._check(lib().cfgpu_construct(self._h, MODES.get(mode, mode)), "construct")
+
+    def construct_buckets(self, buckets, mode="learned"):
+        b = np.ascontiguousarray(buckets, dtype=np.uint32)
+        if b.size == 0:
+            b = np.zeros(1, np.uint32)[:0]
+        self._check(lib().cfgpu_construct_buckets(self._h, MODES.get(mode, mode), _p(b, u32p) if b.size else None,
+                                                  len(b)), "construct_buckets")
 
     def bucket_stats(self):
         arr = (BucketStat * G)()

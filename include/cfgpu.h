@@ -125,6 +125,13 @@ int cfgpu_load_buckets(cfgpu_handle* h, const uint32_t* att, const uint64_t* lab
  *                   undersized tables detected exactly and rebuilt. */
 enum { CFGPU_MODE_PLAIN = 0, CFGPU_MODE_BATCHED = 1, CFGPU_MODE_BATCHED_LEARNED = 2 };
 int cfgpu_construct(cfgpu_handle* h, uint32_t mode);
+/* cfgpu_construct of the listed buckets only (duplicates and empty buckets ignored):
+ * the reference constructs subgraph by subgraph (Subgraphs_Manager::process_graph,
+ * src/Subgraphs_Manager.cpp:147-160), and a caller bounds device memory by building
+ * the buckets in batches - construct a batch, contract it, copy its unitigs out, go
+ * on (cfhost_cdbg_build glues the batches' unitigs on the host).  Every bucket not
+ * listed is left without a table, as after cfgpu_release. */
+int cfgpu_construct_buckets(cfgpu_handle* h, uint32_t mode, const uint32_t* buckets, uint32_t n);
 
 /* Device view of subgraph g's table: the contract() side of HT_Router
  * (include/Subgraph.hpp:336-358).  Each entry is key_words u64 key words (Kmer<k>

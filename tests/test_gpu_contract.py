@@ -12,7 +12,7 @@ import pytest
 
 import cfgpu as C
 import oracle_lib as O
-from test_gpu_parity import fastq, oracle_records, say
+from test_gpu_parity import fastq, oracle_records, rows_sorted, say
 from test_oracle import rand_seq, rc
 from test_oracle_contract import inputs as mixed_input
 
@@ -169,3 +169,45 @@ def test_contract_dense_tables_with_cycles(n_random):
         stats = h.contract_stats()
         assert stats[0, 7] == 8  # the dense tables took it
         compare("dense-cycles", h.unitigs(), stats, want)
+
+
+def test_construct_in_bucket_batches_glues_to_the_whole_graph():
+    # cfgpu_construct_buckets: two batches of buckets, each contracted on its
+    # own; each batch's tables equal the oracle's for its buckets, the other
+    # buckets have none, and the two batches' unitigs glued on the host give
+    # the compacted graph of a single construct.
+    g = C.synth_genome(300_000, 0.2, 300, 3000, 0.02, seed=51)
+    reads = C.synth_reads(g, 40_000, 150, 0.005, seed=52)
+    k, l, cutoff = 31, 20, 2
+    seq, ro, part = oracle_records(fastq(reads, 150), True, k, l)
+    off, att, lab = O.bucketize(part)
+    want = O.construct(k, l, off, att, lab)
+    inp = C.Input.from_records(reads, np.arange(40_001, dtype=np.uint64) * np.uint64(150), k)
+    rng = np.random.default_rng(5)
+    with C.Handle(k, l, cutoff) as h:
+        h.partition(inp)
+        h.construct("learned")
+        h.contract()
+        whole = h.cdbg_device()
+        nz = np.nonzero(h.bucket_stats()["n_records"])[0].astype(np.uint32)
+        first = np.sort(rng.choice(nz, len(nz) // 2, replace=False))
+        second = np.setdiff1d(nz, first)
+        recs_all, labels_all = [], []
+        for part_b in (first, second):
+            for mode in ("batched", "learned"):
+                h.construct_buckets(part_b, mode)
+                t = h.tables()
+                keep = np.isin(want["g"], part_b)
+                assert set(np.unique(t["g"]).tolist()) <= set(part_b.tolist())
+                np.testing.assert_array_equal(
+                    rows_sorted([t["g"], t["keys"][:, 0], t["e_f"], t["status"]]),
+                    rows_sorted([want["g"][keep], want["keys"][keep, 0], want["e_f"][keep], want["status"][keep]]))
+            other = np.setdiff1d(nz, part_b)
+            assert h.table_view(int(other[0])).capacity == 0
+            h.contract()
+            u = h.unitigs()
+            recs_all += list(zip(u["g"].tolist(), u["flags"].tolist()))
+            labels_all += u["labels"]
+        got = C.cdbg_from_unitigs(k, [r[0] for r in recs_all], [r[1] for r in recs_all], labels_all)
+        say(f"[case] batches: {len(got)} maximal unitigs from two batches, {len(whole)} from one construct")
+        assert got == whole
