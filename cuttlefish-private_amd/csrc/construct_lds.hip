@@ -7,8 +7,9 @@
 //
 // A bucket whose distinct k-mers do not fit one LDS table is split by hash into
 // P passes: pass p keeps the k-mers whose 32-bit pass hash h has
-// (h * P) >> 32 == p.  Pass p's LDS table becomes slots [p*C, (p+1)*C) of
-// the bucket's HBM table.  A bucket of 2..LDS_ALL_PASS_MAX passes is one work
+// (h * P) >> 32 == p.  Each finished pass appends its occupied slots to the
+// bucket's HBM table, which ends up holding exactly the bucket's vertices
+// (its allocation is P * C entries; the rest is never written or read).  A bucket of 2..LDS_ALL_PASS_MAX passes is one work
 // item: its records are walked once, every instance going to its pass's
 // sub-list of a per-workgroup spill region, and each pass probes its own
 // sub-list with every lane busy (each instance is written and read once,
@@ -206,10 +207,11 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
     __shared__ uint64_t s_key[C * KW];
     __shared__ uint32_t s_ef[C];
     __shared__ uint32_t s_st[C / 4];  // one Vertex_Status byte per slot
-    __shared__ uint32_t s_item, s_cnt, s_ovf;
+    __shared__ uint32_t s_item, s_cnt, s_ovf, s_base, s_wr;
     __shared__ uint32_t s_sn[LDS_ALL_PASS_MAX];  // sub-list lengths of an all-pass item
 
     const int tid = threadIdx.x;
+    const int lane = tid & 63;
     const uint32_t W = a.W, k = a.k;
     constexpr uint32_t LIMIT = (uint32_t(C) * 7u) / 8u;
     constexpr uint32_t NG = uint32_t(C) / 4;
@@ -361,17 +363,32 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                 if (tid == 0) atomicOr(&a.overflow[b], 1u);
                 break;
             }
-            // Write the finished pass table to slots [p*C, (p+1)*C) of the bucket.
+            // Append the pass's vertices (its occupied slots, s_cnt of them)
+            // after those of the bucket's other passes: the bucket's table is
+            // its n_vert[b] vertices, dense, in no particular order.  Each wave
+            // reserves a run for its occupied slots with one LDS atomic.
+            if (tid == 0) {
+                s_base = atomicAdd(&a.n_vert[b], s_cnt);
+                s_wr = 0;
+            }
+            __syncthreads();
             typedef __attribute__((address_space(1))) uint64_t g_u64;
-            g_u64* tbl = reinterpret_cast<g_u64*>(a.tbl_ptr[b]) + uint64_t(p) * C * (KW + 1);
-            for (int i = tid; i < C; i += LWG) {
-                g_u64* e = tbl + uint64_t(i) * (KW + 1);
+            g_u64* tbl = reinterpret_cast<g_u64*>(a.tbl_ptr[b]) + uint64_t(s_base) * (KW + 1);
+            for (int i = tid; i < C; i += LWG) {  // C % 64 == 0: a wave's slots are all below C or none
+                const bool occ = s_key[i * KW + KW - 1] != EMPTY;
+                const unsigned long long mk = __ballot(occ);
+                uint32_t run = 0;
+                if (lane == 0 && mk) run = atomicAdd(&s_wr, uint32_t(__popcll(mk)));
+                run = uint32_t(__shfl(int(run), 0));
+                if (!occ) continue;
+                const uint32_t pos =
+                    run + __builtin_amdgcn_mbcnt_hi(uint32_t(mk >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mk), 0u));
+                g_u64* e = tbl + uint64_t(pos) * (KW + 1);
 #pragma unroll
                 for (int w = 0; w < KW; ++w) e[w] = s_key[i * KW + w];
                 const uint32_t st = (s_st[i >> 2] >> (8 * (i & 3))) & 0xFFu;
                 e[KW] = uint64_t(s_ef[i]) | (uint64_t(st) << 32);
             }
-            if (tid == 0) atomicAdd(&a.n_vert[b], s_cnt);
             __syncthreads();  // table read out and counters used before the next pass clears them
         }
         if (s_ovf && tid == 0) atomicOr(&a.overflow[b], 1u);

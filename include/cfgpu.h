@@ -136,9 +136,11 @@ int cfgpu_construct_buckets(cfgpu_handle* h, uint32_t mode, const uint32_t* buck
 /* Device view of subgraph g's table: the contract() side of HT_Router
  * (include/Subgraph.hpp:336-358).  Each entry is key_words u64 key words (Kmer<k>
  * layout, include/Kmer.hpp:45-54: word 0 least significant), a u32 Edge_Frequency word
- * and a u32 status word (State_Config<false>, include/State_Config.hpp:24-163).  An
- * empty entry has all-ones in key word key_words-1.  Valid until cfgpu_release(g),
- * the next cfgpu_partition/cfgpu_construct, or cfgpu_close. */
+ * and a u32 status word (State_Config<false>, include/State_Config.hpp:24-163).  Each
+ * of the `capacity` entries is a vertex or empty (all-ones in key word key_words-1);
+ * a table built in LDS holds exactly the vertices (capacity == n_vertices), one built
+ * with global atomics is open addressing.  Entry order is unspecified.  Valid until
+ * cfgpu_release(g), the next cfgpu_partition/cfgpu_construct, or cfgpu_close. */
 typedef struct cfgpu_table_view {
     const void* entries;
     uint64_t capacity;
@@ -302,7 +304,7 @@ typedef struct cfgpu_tuning {
                                  1 always the MLP, 2 always the constant mean ratio */
     uint32_t dedup;           /* batched paths: walk each bucket's distinct records once with
                                  their multiplicity.  0 off, 1 when sampled buckets repeat
-                                 records at least 1.33x, 2 always (results never change) */
+                                 records at least 1.1x, 2 always (results never change) */
     uint32_t mlp_epochs;      /* learned sizing: MLP training epochs (1 .. 10000) */
     float mlp_lr;             /* learned sizing: Adam learning rate (0 < lr <= 1) */
     float lds_load;           /* batched paths: target fill of an LDS pass table (0.5 .. 0.85);
@@ -311,10 +313,13 @@ typedef struct cfgpu_tuning {
 int cfgpu_tuning_get(cfgpu_handle* h, cfgpu_tuning* out);
 int cfgpu_tuning_set(cfgpu_handle* h, const cfgpu_tuning* in);
 /* Times the construct in `mode` on the current partition for a grid of tunings and
- * keeps the fastest (whole construct: dedup, samples, main round, fallbacks).  The
- * grid is the LDS pass-table load x the learned sizing margin, plus the global-atomic
- * kernel's load factor x workgroup x chunk x workgroups per CU when any bucket takes
- * that kernel.  Results never change with the tuning: each trial's per-bucket vertex
+ * keeps the fastest (whole construct: dedup, samples, main round, fallbacks; each
+ * trial timed on its second run, after its pools are allocated).  Stage
+ * one times the current tuning and, when its dedup is automatic, the same with dedup
+ * always on; stage two, around the faster, is the LDS pass-table load x the learned
+ * sizing margin, plus the global-atomic kernel's load factor x workgroup x chunk x
+ * workgroups per CU when buckets other than the learned mode's samples take that
+ * kernel.  Results never change with the tuning: each trial's per-bucket vertex
  * counts are compared with the first trial's (`identical`).  `trials_out` (may be
  * NULL) receives up to n_trials trials. */
 typedef struct cfgpu_tune_trial {

@@ -6,6 +6,7 @@ and label) equals the oracle's.  Construct: the sorted (g, canonical k-mer,
 Edge_Frequency word, status) rows equal the oracle's literal restatement of
 Subgraph::construct, and the exported DFA byte equals edge_at/disc under the
 cutoff.  All three construct modes must agree."""
+import ctypes
 import sys
 
 import numpy as np
@@ -251,6 +252,16 @@ def test_repeat_calls_are_idempotent_and_release():
         nz = np.nonzero(h.bucket_stats()["n_records"])[0]
         v = h.table_view(int(nz[0]))
         assert v.capacity >= v.n_vertices > 0 and v.key_words == 1 and v.entry_bytes == 16
+        # the view's device entries: vertices or empty, the vertices those of table copy
+        ent = np.zeros(2 * v.capacity, np.uint64)
+        hip = ctypes.CDLL("libamdhip64.so")
+        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        assert hip.hipMemcpy(ent.ctypes.data, v.entries, ent.nbytes, 2) == 0
+        keys = ent[0::2]
+        occ = keys != np.uint64(0xFFFFFFFFFFFFFFFF)
+        assert int(occ.sum()) == v.n_vertices
+        one = h.tables(buckets=[int(nz[0])])
+        np.testing.assert_array_equal(np.sort(keys[occ]), np.sort(one["keys"][:, 0]))
         h.release(int(nz[0]))
         assert h.table_view(int(nz[0])).capacity == 0
         assert len(h.tables(buckets=[int(nz[0])])["g"]) == 0
@@ -349,8 +360,9 @@ def test_many_pass_buckets_walk_once(k):
 
 
 def test_autotune_keeps_results_and_keeps_the_fastest():
-    # Zipf buckets with max_passes 4: the largest take the global-atomic kernel,
-    # so both parts of the grid run (LDS load x margin, global-kernel shapes).
+    # Zipf buckets with max_passes 1: the largest take the global-atomic kernel,
+    # so every part of the search runs (record dedup auto / on, then LDS load x
+    # margin and global-kernel shapes around the faster of the two).
     # Every trial must build the same vertex counts, the kept tuning must be the
     # fastest trial's, and the final tables must equal the oracle's.
     k, l = 31, 20
@@ -358,16 +370,18 @@ def test_autotune_keeps_results_and_keeps_the_fastest():
     want = O.construct(k, l, off, (att & 0xFFFF).astype(np.uint16), lab)
     with C.Handle(k, l, 1) as h:
         h.load_buckets(att, lab, off)
-        h.set_tuning(max_passes=4)
+        h.set_tuning(max_passes=1)
         trials = h.autotune("learned")
         say(f"[case] autotune: {len(trials)} trials, fastest {min(t['ms'] for t in trials):.3f} ms")
-        assert len(trials) == 1 + 9 + 16
+        assert len(trials) == 2 + 9 + 16                 # dedup auto / on, then the grids
+        assert [t["dedup"] for t in trials[:2]] == [1, 2]
         assert all(t["identical"] for t in trials)
         best = min(trials, key=lambda t: t["ms"])
         tun = h.tuning()
-        for key in ("lds_load", "margin", "load_factor", "workgroup", "chunk_records", "wgs_per_cu"):
+        for key in ("dedup", "lds_load", "margin", "load_factor", "workgroup", "chunk_records", "wgs_per_cu"):
             assert tun[key] == pytest.approx(best[key]), key
-        assert h.run_stats()["n_lds_buckets"] < h.run_stats()["n_buckets_nonempty"]
+        rs = h.run_stats()
+        assert rs["n_lds_buckets"] + rs["n_samples"] < rs["n_buckets_nonempty"]  # main-round buckets went global
         check_tables(h, None, k, l, 1, want)
 
 
