@@ -658,10 +658,13 @@ __device__ __forceinline__ uint32_t cb_pack(uint32_t p, uint32_t d) { return p |
 constexpr uint32_t CB_JP_LDS = 39936;
 constexpr uint32_t CB_JQ = CB_JP_LDS / CT_WG;  // states per thread
 static_assert(CB_JP_LDS >= CB_HCAP && CB_JP_LDS % CT_WG == 0, "LDS ranking buffer");
+constexpr uint32_t CB_RU = 8;                        // states per thread in flight, global ranking
+constexpr uint32_t CB_WU = 4;                        // vertices per thread in flight, label writes
+constexpr uint32_t CB_STAGE = CB_JP_LDS * 4;         // label bytes staged in LDS (the same buffer)
 
 template <int KW>
 __global__ __launch_bounds__(CT_WG) void k_contract_big(ContractArgs a) {
-    __shared__ uint32_t s_buf[CB_JP_LDS];  // the key index, then the ranking words
+    __shared__ __align__(16) uint32_t s_buf[CB_JP_LDS];  // the key index, the ranking words, then labels
     uint32_t* s_ht = s_buf;
     __shared__ uint32_t s_cand[CT_CAND];
     __shared__ uint32_t s_cnt[8];
@@ -770,21 +773,38 @@ __global__ __launch_bounds__(CT_WG) void k_contract_big(ContractArgs a) {
             }
             jp = s_buf;
         } else {
-            // in global scratch, two buffers
+            // in global scratch, two buffers; CB_RU states per thread at a
+            // time, their loads issued before any store (the stores could
+            // alias them as far as the compiler knows)
             const uint32_t* src = jp0;
             uint32_t* dst = jp1;
             for (uint32_t r = 0; r < CB_ROUNDS; ++r) {
                 bool any = false;
-                for (uint32_t x = tid; x < 2 * n; x += CT_WG) {
-                    const uint32_t w = src[x], p = w & CB_M16;
-                    uint32_t nw = w;
-                    if (p != x) {
-                        const uint32_t w2 = src[p];
-                        const uint32_t p2 = w2 & CB_M16, nd = (w >> 16) + (w2 >> 16);
-                        nw = cb_pack(p2, nd);
-                        any |= p2 != p && nd < n;
+                for (uint32_t x0 = tid; x0 < 2 * n; x0 += CT_WG * CB_RU) {
+                    uint32_t w[CB_RU], w2[CB_RU];
+#pragma unroll
+                    for (uint32_t u = 0; u < CB_RU; ++u) {
+                        const uint32_t x = x0 + u * CT_WG;
+                        w[u] = x < 2 * n ? src[x] : x;
                     }
-                    dst[x] = nw;
+#pragma unroll
+                    for (uint32_t u = 0; u < CB_RU; ++u) {
+                        const uint32_t x = x0 + u * CT_WG, p = w[u] & CB_M16;
+                        w2[u] = x < 2 * n && p != x ? src[p] : 0u;
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < CB_RU; ++u) {
+                        const uint32_t x = x0 + u * CT_WG;
+                        if (x >= 2 * n) continue;
+                        const uint32_t p = w[u] & CB_M16;
+                        uint32_t nw = w[u];
+                        if (p != x) {
+                            const uint32_t p2 = w2[u] & CB_M16, nd = (w[u] >> 16) + (w2[u] >> 16);
+                            nw = cb_pack(p2, nd);
+                            any |= p2 != p && nd < n;
+                        }
+                        dst[x] = nw;
+                    }
                 }
                 if (any) s_chg[r] = 1;
                 __syncthreads();
@@ -871,38 +891,81 @@ __global__ __launch_bounds__(CT_WG) void k_contract_big(ContractArgs a) {
         if (tid < 8) a.counts[size_t(b) * 8 + tid] = s_cnt[tid];
         if (s_skip) continue;
 
-        // write: every path vertex stores its base, the first one its k-mer and the record
+        // write: every path vertex stores its base, the first one its k-mer
+        // and the record.  The ranking words are no longer needed: when the
+        // subgraph's labels fit, they are assembled in that LDS buffer (at the
+        // offset that keeps the output's 16-byte alignment) and copied out
+        // with whole 16-byte stores.  Each thread loads CB_WU vertices'
+        // words before it writes any byte.
         const uint64_t lab_end = s_bbyte + s_bytes;
-        for (uint32_t d = tid; d < n; d += CT_WG) {
-            const uint32_t info = ia[d];
-            if (!info) {
-                if ((ld_meta(&t.meta[d]) & (M_WRITER | M_CYCLE)) != (M_WRITER | M_CYCLE)) continue;
-                const uint64_t off = s_bbyte + offv[d];
-                const uint32_t nc = write_cycle<KW>(t, d, k, a.labels + off, off, lab_end, a.error);
-                const uint32_t ri = atomicAdd(&s_rctr, 1u);
-                if (ri < s_nrec) a.recs[s_brec + ri] = UnitigRec{off, nc + k - 1, uint16_t(b), 1};
-                continue;
+        const uint32_t sh = uint32_t(s_bbyte & 15u);
+        const bool staged = sh + s_bytes <= CB_STAGE;
+        char* s_lab = reinterpret_cast<char*>(s_buf);
+        char* out0 = staged ? s_lab + sh : a.labels + s_bbyte;  // label byte s_bbyte + j is out0[j]
+        for (uint32_t d0 = tid; d0 < n; d0 += CT_WG * CB_WU) {
+            uint32_t info[CB_WU], pos[CB_WU], offa[CB_WU];
+            Kmer<KW> x[CB_WU];
+#pragma unroll
+            for (uint32_t u = 0; u < CB_WU; ++u) {
+                const uint32_t d = d0 + u * CT_WG;
+                info[u] = d < n ? ia[d] : 0u;
+                pos[u] = d < n ? posv[d] : 0u;
             }
-            const uint32_t A = info & 0xFFFFu, pos = posv[d];
-            const bool fwd = (info >> 16) & 1u;
-            const uint64_t off = s_bbyte + offv[A];
-            const Kmer<KW> x = tab_key<KW>(t, d);
-            if (pos == 0) {
-                const uint32_t wn = wnv[d] & 0x1FFFFFFFu, fl = wnv[d] >> 29;
-                if (off + wn + k - 1 > lab_end) {
-                    atomicOr(a.error, 1u);
+#pragma unroll
+            for (uint32_t u = 0; u < CB_WU; ++u) {
+                const uint32_t d = d0 + u * CT_WG;
+                offa[u] = info[u] ? offv[info[u] & 0xFFFFu] : 0u;
+                if (info[u]) x[u] = tab_key<KW>(t, d);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < CB_WU; ++u) {
+                const uint32_t d = d0 + u * CT_WG;
+                if (d >= n) continue;
+                if (!info[u]) {
+                    if ((ld_meta(&t.meta[d]) & (M_WRITER | M_CYCLE)) != (M_WRITER | M_CYCLE)) continue;
+                    const uint64_t off = s_bbyte + offv[d];
+                    const uint32_t nc = write_cycle<KW>(t, d, k, out0 + offv[d], off, lab_end, a.error);
+                    const uint32_t ri = atomicAdd(&s_rctr, 1u);
+                    if (ri < s_nrec) a.recs[s_brec + ri] = UnitigRec{off, nc + k - 1, uint16_t(b), 1};
                     continue;
                 }
-                put_kmer(a.labels + off, fwd ? x : km_rc(x, k), k);
-                const uint32_t ri = atomicAdd(&s_rctr, 1u);
-                if (ri < s_nrec) a.recs[s_brec + ri] = UnitigRec{off, wn + k - 1, uint16_t(b), uint16_t(fl)};
-            } else {
-                const uint32_t c = fwd ? uint32_t(x.w[0] & 3u) : 3u - uint32_t((x.w[KW - 1] >> (2 * (kt - 1))) & 3u);
-                if (off + pos + k - 1 >= lab_end) {
-                    atomicOr(a.error, 1u);
-                    continue;
+                const bool fwd = (info[u] >> 16) & 1u;
+                const uint64_t off = s_bbyte + offa[u];
+                if (pos[u] == 0) {
+                    const uint32_t wn = wnv[d] & 0x1FFFFFFFu, fl = wnv[d] >> 29;
+                    if (off + wn + k - 1 > lab_end) {
+                        atomicOr(a.error, 1u);
+                        continue;
+                    }
+                    put_kmer(out0 + offa[u], fwd ? x[u] : km_rc(x[u], k), k);
+                    const uint32_t ri = atomicAdd(&s_rctr, 1u);
+                    if (ri < s_nrec) a.recs[s_brec + ri] = UnitigRec{off, wn + k - 1, uint16_t(b), uint16_t(fl)};
+                } else {
+                    const uint32_t c =
+                        fwd ? uint32_t(x[u].w[0] & 3u) : 3u - uint32_t((x[u].w[KW - 1] >> (2 * (kt - 1))) & 3u);
+                    if (off + pos[u] + k - 1 >= lab_end) {
+                        atomicOr(a.error, 1u);
+                        continue;
+                    }
+                    out0[offa[u] + pos[u] + k - 1] = char(base_char(c));
                 }
-                a.labels[off + pos + k - 1] = char(base_char(c));
+            }
+        }
+        if (staged) {
+            // LDS bytes [sh, sh + s_bytes) go to labels [s_bbyte, lab_end);
+            // 16-byte chunk c covers LDS [16c, 16c + 16) and the same offsets
+            // from s_bbyte - sh (16-byte aligned)
+            __syncthreads();
+            char* g0 = a.labels + (s_bbyte - sh);
+            const uint32_t end = sh + s_bytes;
+            for (uint32_t c = tid; c < (end + 15) / 16; c += CT_WG) {
+                const uint32_t lo = 16 * c;
+                if (lo >= sh && lo + 16 <= end) {
+                    *reinterpret_cast<uint4*>(g0 + lo) = *reinterpret_cast<const uint4*>(s_lab + lo);
+                } else {
+                    for (uint32_t j = lo; j < lo + 16; ++j)
+                        if (j >= sh && j < end) g0[j] = s_lab[j];
+                }
             }
         }
     }
