@@ -37,9 +37,12 @@ namespace {
 constexpr uint64_t EMPTY = ~0ull;
 constexpr uint64_t BUSY = 1ull << 63;
 constexpr int LWG = LDS_WG;
+// Group reads after which a pass-phase search is set aside (see the pass
+// phase): 1 measured faster than 2 at 50M reads, k = 63 (92.5 vs 95.1 ms).
+constexpr uint32_t DEFER_TRIPS = 1;
 
 struct LdsProbe {
-    int slot;  // -1: every group is full
+    int slot;  // -1: every group is full; -2: max_trips group reads without a result
     bool inserted;
 };
 
@@ -56,11 +59,17 @@ __device__ __forceinline__ uint32_t pass_hash(const Kmer<KW>& x) {
 // from its first slot on (claims take the first empty slot; entries are never
 // removed), so a group with an empty slot ends an unsuccessful search.  The
 // group is scanned without branches (select chains, a candidate bit mask).
+// After max_trips group reads (re-reads included) without a result it returns
+// slot -2 with g the group to go on from: every group before g in the key's
+// probe order was full without the key, and stays so, so a later search from
+// g finds or inserts the key where a search from its home group would.
 template <int KW, int C>
-__device__ __forceinline__ LdsProbe lds_find_or_insert(uint64_t* s_key, const Kmer<KW>& key, uint32_t g) {
+__device__ __forceinline__ LdsProbe lds_find_or_insert(uint64_t* s_key, const Kmer<KW>& key, uint32_t& g,
+                                                       uint32_t max_trips = ~0u) {
     constexpr uint32_t NG = uint32_t(C) / 4;
     const uint64_t top = key.w[KW - 1];
-    for (uint32_t n = 0; n < NG;) {
+    for (uint32_t n = 0, trips = 0; n < NG;) {
+        if (trips++ == max_trips) return {-2, false};
         uint64_t cur[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j)
@@ -117,14 +126,9 @@ __device__ __forceinline__ LdsProbe lds_find_or_insert(uint64_t* s_key, const Km
 }
 
 // Add one k-mer instance (counters of the front / back bases, m times;
-// discontinuity bits) to its slot of the LDS pass table, inserting the key if
-// new.  Returns false when every group is full.
-template <int KW, int C>
-__device__ __forceinline__ bool lds_add(uint64_t* s_key, uint32_t* s_ef, uint32_t* s_st, const Kmer<KW>& key,
-                                        uint32_t g0, uint32_t front, uint32_t back, uint32_t disc, uint32_t m,
-                                        uint32_t& my_ins) {
-    const LdsProbe pr = lds_find_or_insert<KW, C>(s_key, key, g0);
-    if (pr.slot < 0) return false;
+// discontinuity bits) to the slot pr found or inserted.
+__device__ __forceinline__ void lds_count(uint32_t* s_ef, uint32_t* s_st, const LdsProbe& pr, uint32_t front,
+                                          uint32_t back, uint32_t disc, uint32_t m, uint32_t& my_ins) {
     // Saturating update (a fresh slot starts at 0; a stale value only costs
     // a CAS retry).
     uint32_t old = pr.inserted ? 0u : s_ef[pr.slot];
@@ -137,6 +141,16 @@ __device__ __forceinline__ bool lds_add(uint64_t* s_key, uint32_t* s_ef, uint32_
     }
     if (disc) atomicOr(&s_st[pr.slot >> 2], disc << (8 * (pr.slot & 3)));
     my_ins += pr.inserted;
+}
+
+// lds_find_or_insert + lds_count for one instance; false when every group is full.
+template <int KW, int C>
+__device__ __forceinline__ bool lds_add(uint64_t* s_key, uint32_t* s_ef, uint32_t* s_st, const Kmer<KW>& key,
+                                        uint32_t g0, uint32_t front, uint32_t back, uint32_t disc, uint32_t m,
+                                        uint32_t& my_ins) {
+    const LdsProbe pr = lds_find_or_insert<KW, C>(s_key, key, g0);
+    if (pr.slot < 0) return false;
+    lds_count(s_ef, s_st, pr, front, back, disc, m, my_ins);
     return true;
 }
 
@@ -207,7 +221,7 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
     __shared__ uint64_t s_key[C * KW];
     __shared__ uint32_t s_ef[C];
     __shared__ uint32_t s_st[C / 4];  // one Vertex_Status byte per slot
-    __shared__ uint32_t s_item, s_cnt, s_ovf, s_base, s_wr;
+    __shared__ uint32_t s_item, s_cnt, s_ovf, s_base, s_wr, s_ndef;
     __shared__ uint32_t s_sn[LDS_ALL_PASS_MAX];  // sub-list lengths of an all-pass item
 
     const int tid = threadIdx.x;
@@ -246,7 +260,7 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                 for (int i = tid; i < C; i += LWG) s_ef[i] = 0u;
                 for (int i = tid; i < C / 4; i += LWG) s_st[i] = 0u;
             }
-            if (tid == 0) s_cnt = 0;
+            if (tid == 0) s_cnt = s_ndef = 0;
             __syncthreads();
             if (s_ovf) break;  // bucket already flagged
             uint32_t my_ins = 0;  // keys this lane inserted since its last flush into s_cnt
@@ -329,34 +343,67 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                 // Pass p over its own sub-list, one entry per lane, the
                 // lane's next entry loaded while this one is probed (a
                 // sub-list that overflowed has flagged the bucket: nothing
-                // past its capacity is read).
+                // past its capacity is read).  A wave's loop runs as long as
+                // its longest search, so a search that has not ended after
+                // DEFER_TRIPS group reads is set aside: the instance, with
+                // the group to go on from, is appended after the sub-list
+                // (its unused capacity), and the set-aside instances are
+                // searched after a barrier.  Counter adds commute, so the
+                // order of the adds never shows.  With no room left a search
+                // goes on in place.
                 const uint32_t n_in = min(s_sn[p], c_sub);
+                const uint32_t room = c_sub - n_in;
                 const uint64_t e0 = uint64_t(p) * c_sub;
-                Kmer<KW> key_n;
-                uint32_t meta_n = 0;
-                if (uint32_t(tid) < n_in) {
+                const auto sweep = [&](uint32_t lo, uint32_t hi, uint32_t max_trips) {
+                    Kmer<KW> key_n;
+                    uint32_t meta_n = 0;
+                    if (lo + uint32_t(tid) < hi) {
 #pragma unroll
-                    for (int w = 0; w < KW; ++w) key_n.w[w] = R.key[uint64_t(w) * a.spill_cap + e0 + tid];
-                    meta_n = R.meta[e0 + tid];
-                }
-                uint32_t seen = 0;
-                for (uint32_t e = tid; e < n_in; e += LWG) {
-                    const Kmer<KW> key = key_n;
-                    const uint32_t meta = meta_n;
-                    if (e + LWG < n_in) {
+                        for (int w = 0; w < KW; ++w) key_n.w[w] = R.key[uint64_t(w) * a.spill_cap + e0 + lo + tid];
+                        meta_n = R.meta[e0 + lo + tid];
+                    }
+                    uint32_t seen = 0;
+                    for (uint32_t e = lo + tid; e < hi; e += LWG) {
+                        const Kmer<KW> key = key_n;
+                        const uint32_t meta = meta_n;
+                        if (e + LWG < hi) {
 #pragma unroll
-                        for (int w = 0; w < KW; ++w) key_n.w[w] = R.key[uint64_t(w) * a.spill_cap + e0 + e + LWG];
-                        meta_n = R.meta[e0 + e + LWG];
+                            for (int w = 0; w < KW; ++w)
+                                key_n.w[w] = R.key[uint64_t(w) * a.spill_cap + e0 + e + LWG];
+                            meta_n = R.meta[e0 + e + LWG];
+                        }
+                        uint32_t g = meta >> 13;
+                        LdsProbe pr = lds_find_or_insert<KW, C>(s_key, key, g, max_trips);
+                        bool counted = true;
+                        if (pr.slot == -2) {
+                            const uint32_t d = atomicAdd(&s_ndef, 1u);
+                            if (d < room) {
+                                const uint64_t at = e0 + n_in + d;
+#pragma unroll
+                                for (int w = 0; w < KW; ++w) R.key[uint64_t(w) * a.spill_cap + at] = key.w[w];
+                                R.meta[at] = (meta & 0x1FFFu) | (g << 13);
+                                counted = false;
+                            } else {
+                                pr = lds_find_or_insert<KW, C>(s_key, key, g);
+                            }
+                        }
+                        if (counted) {
+                            if (pr.slot < 0)
+                                s_ovf = 1;
+                            else
+                                lds_count(s_ef, s_st, pr, meta & 7u, (meta >> 3) & 7u, (meta >> 6) & 7u,
+                                          (meta >> 9) & 15u, my_ins);
+                        }
+                        if ((++seen & 7u) == 0) {
+                            flush();
+                            if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+                        }
                     }
-                    if (!lds_add<KW, C>(s_key, s_ef, s_st, key, meta >> 13, meta & 7u, (meta >> 3) & 7u,
-                                        (meta >> 6) & 7u, (meta >> 9) & 15u, my_ins))
-                        s_ovf = 1;
-                    if ((++seen & 7u) == 0) {
-                        flush();
-                        if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
-                    }
-                }
-                flush();
+                    flush();
+                };
+                sweep(0, n_in, DEFER_TRIPS);
+                __syncthreads();  // every set-aside instance written
+                if (!s_ovf) sweep(n_in, n_in + min(s_ndef, room), ~0u);
             }
             __syncthreads();
             if (s_ovf) {
