@@ -398,13 +398,15 @@ def main():
                      "atomics": None if not atom else {
                          "device_scope_per_launch": atom.get("tcc"),
                          "lds_wave_instructions_per_launch": atom.get("lds_wave_instructions"),
-                         "lds_atomic_wave_instructions_per_s": atom.get("lds_wave_instructions", 0) / (kms / 1e3)
-                         if kms else None,
+                         "lds_atomic_wave_instructions_per_launch": atom.get("lds_atomic_wave_instructions"),
+                         "lds_atomic_wave_instructions_per_s": atom.get("lds_atomic_wave_instructions", 0) / (kms / 1e3)
+                         if kms and atom.get("lds_atomic_wave_instructions") else None,
                          "global_atomic_kernel_rate_per_s": atom.get("global_rate"),
                          "source": os.path.relpath(atom_path, REPO),
-                         "note": "the hash-insert kernel's counters and states live in LDS (no device-scope "
-                                 "atomics); the global-atomic construct (max_passes 0, 10M reads k=31) sustains "
-                                 "the device-scope rate given"}},
+                         "note": "the hash-insert kernel's tables, counters and states live in LDS (its "
+                                 "device-scope atomics are the work queue's and the per-bucket counts'); the "
+                                 "global-atomic construct (max_passes 0, 10M reads k=31) sustains the "
+                                 "device-scope rate given"}},
         "cpu_baseline": cpu,
         "device_memory_gib": mem_gib,
         "phases_ms": {"partition": part_ms, "minimizer": ph["t_minimizer_ms"], "count": ph["t_count_ms"],
