@@ -73,6 +73,18 @@ void launch_pieces_emit(const PartitionArgs& a, const uint64_t* desc, const uint
 void launch_bucket_fine(const uint32_t* hdr1, const uint64_t* lab1, const uint64_t* coff, uint32_t W, uint64_t* boff,
                         uint32_t* att, uint64_t* labels, uint8_t* mbin, hipStream_t s);
 
+// Buckets of separately partitioned input chunks gathered into one bucket
+// array: bucket b receives chunk 0's records of b, then chunk 1's, and so on;
+// chunk c's records of bucket b start at dst_off[c * G + b] of the output.
+struct ChunkSrc {
+    const uint32_t* att;
+    const uint64_t* labels;
+    const uint8_t* mbin;
+    const uint64_t* boff;  // G + 1 offsets
+};
+void launch_merge_chunks(const ChunkSrc* src, uint32_t n_chunks, uint32_t G, uint32_t W, const uint64_t* dst_off,
+                         uint32_t* att, uint64_t* labels, uint8_t* mbin, hipStream_t s);
+
 // Per-bucket statistics: [n_rec, n_kmer_instances, n_bases, 0] as u64 and the
 // MLP feature vector (N_FEAT floats).
 void launch_bucket_features(const uint64_t* off, const uint32_t* att, const uint8_t* mbin,

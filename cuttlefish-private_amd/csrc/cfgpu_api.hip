@@ -61,6 +61,11 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+void swap_buf(DevBuf& a, DevBuf& b) {
+    std::swap(a.p, b.p);
+    std::swap(a.n, b.n);
+}
+
 uint64_t wymix_host(uint64_t a, uint64_t b) {
     const unsigned __int128 r = static_cast<unsigned __int128>(a) * b;
     return uint64_t(r) ^ uint64_t(r >> 64);
@@ -107,6 +112,16 @@ struct cfgpu_handle {
     DevBuf packed, frag, gbin, mbn, tile_lc, tile_frag, boff, att, labels, mbin, stats, feat;
     DevBuf pdesc, tile_n;  // piece descriptors (P_TILE per tile) and per-tile counts
     DevBuf hist, ctot, coff, rec1;  // level-1 bucketing
+    // Chunks partitioned with CFGPU_PARTITION_APPEND, waiting for cfgpu_partition_finish:
+    // each chunk's buckets (records, minimizer bins, offsets) and its counters.
+    struct PartChunk {
+        DevBuf att, labels, mbin, boff;
+        std::vector<uint64_t> h_boff;
+        uint64_t n_rec = 0, n_bases = 0, n_frag = 0;
+        double t_minimizer = 0, t_count = 0, t_emit = 0, t_features = 0, t_partition = 0;
+    };
+    std::vector<std::unique_ptr<PartChunk>> chunks;
+    DevBuf ch_src, ch_dst;  // gather arguments
     uint64_t n_bases = 0, n_frag = 0, n_records = 0;
     bool partitioned = false;
     std::vector<uint64_t> h_boff, h_stats;
@@ -247,13 +262,16 @@ namespace {
 // cfgpu_partition allocates it again), pools past the current construct's
 // rounds hold no live table, and each later step makes the scratch of the
 // earlier ones dead.
-enum class Phase { Construct, Contract, Cdbg };
+// Phase::Partition is the gather of appended chunks, after which every table,
+// and every later step's scratch, is dead too.
+enum class Phase { Partition, Construct, Contract, Cdbg };
 
 void trim(cfgpu_handle* h, Phase ph) {
     for (DevBuf* b : {&h->gbin, &h->mbn, &h->tile_lc, &h->pdesc, &h->tile_n, &h->tile_frag, &h->hist, &h->rec1,
                       &h->ex_keys, &h->ex_ef, &h->ex_st, &h->ex_dfa})
         b->release();
-    for (size_t i = h->round_idx; i < h->pools.size(); ++i) h->pools[i]->release();
+    for (size_t i = ph == Phase::Partition ? 0 : h->round_idx; i < h->pools.size(); ++i) h->pools[i]->release();
+    if (ph == Phase::Partition) h->ct_scratch.release();
     if (ph != Phase::Cdbg)
         for (DevBuf* b : {&h->cd_partner, &h->cd_bhist, &h->cd_bkey, &h->cd_bid, &h->cd_kend, &h->cd_jp, &h->cd_head,
                           &h->cd_pos, &h->cd_hoff, &h->cd_out, &h->cd_rec})
@@ -1099,14 +1117,22 @@ int cfgpu_last_error(const cfgpu_handle* h, char* buf, size_t n) {
     return CFGPU_OK;
 }
 
+int cfgpu_params_get(const cfgpu_handle* h, cfgpu_params* out) {
+    if (!h || !out) return CFGPU_E_ARG;
+    *out = h->prm;
+    return CFGPU_OK;
+}
+
 int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
                     const uint64_t* frag_off, uint64_t n_frag, uint32_t flags) {
     if (!h) return CFGPU_E_ARG;
     HIPCK(h, hipSetDevice(h->prm.device));
     const uint32_t k = h->prm.k;
-    if ((n_frag && !packed) || (!frag_off && n_frag) || (flags & ~uint32_t(CFGPU_INPUT_DEVICE)))
+    if ((n_frag && !packed) || (!frag_off && n_frag) ||
+        (flags & ~uint32_t(CFGPU_INPUT_DEVICE | CFGPU_PARTITION_APPEND)))
         return fail(h, CFGPU_E_ARG, "partition: bad pointers or flags");
     const bool dev = flags & CFGPU_INPUT_DEVICE;
+    const bool append = flags & CFGPU_PARTITION_APPEND;
     if (!dev) {
         if (n_frag == 0 && n_bases != 0) return fail(h, CFGPU_E_ARG, "partition: bases without fragments");
         if (n_frag) {
@@ -1120,6 +1146,7 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     }
     h->partitioned = h->constructed = h->contracted = h->cdbg_done = false;
     h->mlp_trained = false;
+    if (!append) h->chunks.clear();
     const uint32_t G = h->G;
     const uint64_t n_words = (n_bases + 31) / 32 + 2;
     const uint64_t tiles = (n_bases + P_TILE - 1) / P_TILE;
@@ -1217,6 +1244,101 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     rs.t_emit_ms = elapsed_ms(e3, e4);
     rs.t_features_ms = elapsed_ms(e4, e5);
     rs.t_partition_ms = elapsed_ms(e0, e5);
+    rs.n_partition_chunks = 1;
+    if (append) {
+        // the chunk's buckets move aside; the next chunk allocates its own
+        auto c = std::make_unique<cfgpu_handle::PartChunk>();
+        swap_buf(c->att, h->att);
+        swap_buf(c->labels, h->labels);
+        swap_buf(c->mbin, h->mbin);
+        swap_buf(c->boff, h->boff);
+        c->h_boff = h->h_boff;
+        c->n_rec = n_rec;
+        c->n_bases = n_bases;
+        c->n_frag = n_frag;
+        c->t_minimizer = rs.t_minimizer_ms;
+        c->t_count = rs.t_count_ms;
+        c->t_emit = rs.t_emit_ms;
+        c->t_features = rs.t_features_ms;
+        c->t_partition = rs.t_partition_ms;
+        h->chunks.push_back(std::move(c));
+        rs.n_partition_chunks = uint32_t(h->chunks.size());
+        return CFGPU_OK;
+    }
+    h->partitioned = true;
+    return CFGPU_OK;
+}
+
+int cfgpu_partition_finish(cfgpu_handle* h) {
+    if (!h) return CFGPU_E_ARG;
+    if (h->chunks.empty()) return fail(h, CFGPU_E_STATE, "partition_finish without appended chunks");
+    HIPCK(h, hipSetDevice(h->prm.device));
+    const uint32_t G = h->G, nc = uint32_t(h->chunks.size());
+    // bucket b: chunk 0's records, then chunk 1's, ...
+    std::vector<uint64_t> boff(G + 1, 0), dst(size_t(nc) * G);
+    for (uint32_t b = 0; b < G; ++b) {
+        uint64_t at = boff[b];
+        for (uint32_t c = 0; c < nc; ++c) {
+            const auto& hb = h->chunks[c]->h_boff;
+            dst[size_t(c) * G + b] = at;
+            at += hb[b + 1] - hb[b];
+        }
+        boff[b + 1] = at;
+    }
+    const uint64_t n = boff[G];
+    std::vector<ChunkSrc> src(nc);
+    uint64_t n_bases = 0, n_frag = 0;
+    cfgpu_run_stats sum{};
+    for (uint32_t c = 0; c < nc; ++c) {
+        const auto& ch = *h->chunks[c];
+        src[c] = {ch.att.as<uint32_t>(), ch.labels.as<uint64_t>(), ch.mbin.as<uint8_t>(), ch.boff.as<uint64_t>()};
+        n_bases += ch.n_bases;
+        n_frag += ch.n_frag;
+        sum.t_minimizer_ms += ch.t_minimizer;
+        sum.t_count_ms += ch.t_count;
+        sum.t_emit_ms += ch.t_emit;
+        sum.t_features_ms += ch.t_features;
+        sum.t_partition_ms += ch.t_partition;
+    }
+    HIPCK(h, ensure_mem(h, h->att, n * 4, Phase::Partition));
+    HIPCK(h, ensure_mem(h, h->labels, n * h->W * 8, Phase::Partition));
+    HIPCK(h, ensure_mem(h, h->mbin, n, Phase::Partition));
+    HIPCK(h, h->boff.ensure((G + 1) * 8));
+    HIPCK(h, h->stats.ensure(size_t(G) * 4 * 8));
+    HIPCK(h, h->feat.ensure(size_t(G) * N_FEAT * 4));
+    HIPCK(h, h->ch_src.ensure(nc * sizeof(ChunkSrc)));
+    HIPCK(h, h->ch_dst.ensure(dst.size() * 8));
+    HIPCK(h, h2d(h, h->ch_src.p, src.data(), nc * sizeof(ChunkSrc)));
+    HIPCK(h, h2d(h, h->ch_dst.p, dst.data(), dst.size() * 8));
+    HIPCK(h, h2d(h, h->boff.p, boff.data(), (G + 1) * 8));
+    Event e0, e1;
+    HIPCK(h, hipEventRecord(e0.e, h->stream));
+    launch_merge_chunks(h->ch_src.as<ChunkSrc>(), nc, G, h->W, h->ch_dst.as<uint64_t>(), h->att.as<uint32_t>(),
+                        h->labels.as<uint64_t>(), h->mbin.as<uint8_t>(), h->stream);
+    HIPCK(h, hipEventRecord(e1.e, h->stream));
+    launch_bucket_features(h->boff.as<uint64_t>(), h->att.as<uint32_t>(), h->mbin.as<uint8_t>(), G, h->prm.k,
+                           h->stats.as<uint64_t>(), h->feat.as<float>(), h->stream);
+    HIPCK(h, hipGetLastError());
+    h->h_boff = boff;
+    h->h_stats.resize(size_t(G) * 4);
+    HIPCK(h, d2h(h, h->h_stats.data(), h->stats.p, size_t(G) * 32));
+    HIPCK(h, sync_stream(h));
+    h->chunks.clear();
+
+    h->n_bases = n_bases;
+    h->n_frag = n_frag;
+    h->n_records = n;
+    cfgpu_run_stats& rs = h->rs;
+    rs = sum;
+    rs.n_bases = n_bases;
+    rs.n_fragments = n_frag;
+    rs.n_records = n;
+    for (uint32_t b = 0; b < G; ++b) {
+        rs.n_kmers += h->h_stats[4 * b + 1];
+        rs.n_edges += h->h_stats[4 * b + 1] - h->h_stats[4 * b + 0];
+    }
+    rs.n_partition_chunks = nc;
+    rs.t_partition_merge_ms = elapsed_ms(e0, e1);
     h->partitioned = true;
     return CFGPU_OK;
 }
@@ -1275,6 +1397,7 @@ int cfgpu_load_buckets(cfgpu_handle* h, const uint32_t* att, const uint64_t* lab
         }
     }
     h->partitioned = h->constructed = h->contracted = h->cdbg_done = false;
+    h->chunks.clear();
     const uint64_t n = bucket_off[G];
     HIPCK(h, h->att.ensure(n * 4));
     HIPCK(h, h->labels.ensure(n * h->W * 8));

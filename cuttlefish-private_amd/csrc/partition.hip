@@ -842,4 +842,30 @@ void launch_bucket_features(const uint64_t* off, const uint32_t* att, const uint
     hipLaunchKernelGGL(k_bucket_features, dim3(G), dim3(256), 0, s, off, att, mbin, k, stats, feat);
 }
 
+namespace {
+// One workgroup per bucket: each chunk's run of the bucket's records (headers,
+// minimizer bins, W label words each) copied to its place, coalesced.
+__global__ __launch_bounds__(256) void k_merge_chunks(const ChunkSrc* __restrict__ src, uint32_t n_chunks,
+                                                      uint32_t G, uint32_t W, const uint64_t* __restrict__ dst_off,
+                                                      uint32_t* __restrict__ att, uint64_t* __restrict__ labels,
+                                                      uint8_t* __restrict__ mbin) {
+    const uint32_t b = blockIdx.x;
+    for (uint32_t c = 0; c < n_chunks; ++c) {
+        const ChunkSrc sc = src[c];
+        const uint64_t s0 = sc.boff[b], n = sc.boff[b + 1] - s0, d0 = dst_off[uint64_t(c) * G + b];
+        for (uint64_t r = threadIdx.x; r < n; r += 256) {
+            att[d0 + r] = sc.att[s0 + r];
+            mbin[d0 + r] = sc.mbin[s0 + r];
+        }
+        for (uint64_t i = threadIdx.x; i < n * W; i += 256) labels[d0 * W + i] = sc.labels[s0 * W + i];
+    }
+}
+}  // namespace
+
+void launch_merge_chunks(const ChunkSrc* src, uint32_t n_chunks, uint32_t G, uint32_t W, const uint64_t* dst_off,
+                         uint32_t* att, uint64_t* labels, uint8_t* mbin, hipStream_t s) {
+    if (!n_chunks) return;
+    hipLaunchKernelGGL(k_merge_chunks, dim3(G), dim3(256), 0, s, src, n_chunks, G, W, dst_off, att, labels, mbin);
+}
+
 }  // namespace cfgpu

@@ -1,7 +1,7 @@
 // Host ingest: FASTA/FASTQ records -> fragments -> 2-bit MSB-first packed stream
 // (the input format of cfgpu_partition).  See include/cfhost.h for the rules
 // and the reference lines they follow.
-#include "../../include/cfhost.h"
+#include "ingest_core.hpp"
 
 #include <algorithm>
 #include <cstdio>
@@ -10,12 +10,15 @@
 #include <thread>
 #include <vector>
 
+#include "../../include/cfhost.h"
+
 struct cfhost_input {
     std::vector<uint64_t> packed;   // n_words = ceil(n_bases/32) + 2
     std::vector<uint64_t> frag_off; // n_frag + 1
     uint64_t n_bases = 0, n_records = 0;
 };
 
+namespace cfhost_detail {
 namespace {
 
 // DNA_Utility::is_DNA_base (include/DNA_Utility.hpp:167-172) and the 2-bit code
@@ -36,27 +39,6 @@ struct CodeLut {
 };
 const CodeLut LUT;
 
-struct Frag {
-    uint64_t beg, len;  // in the record byte buffer
-};
-
-// Maximal DNA runs of length >= k+1 in [b, e) (src/Graph_Partitioner.cpp:476-501).
-void find_fragments(const char* s, uint64_t b, uint64_t e, uint32_t k, std::vector<Frag>& out) {
-    uint64_t i = b;
-    while (i < e) {
-        while (i < e && LUT.c[uint8_t(s[i])] == 0xFF) ++i;
-        const uint64_t st = i;
-        while (i < e && LUT.c[uint8_t(s[i])] != 0xFF) ++i;
-        if (i - st >= uint64_t(k) + 1) out.push_back({st, i - st});
-    }
-}
-
-uint32_t pick_threads(uint32_t t) {
-    if (t) return t;
-    const unsigned hw = std::thread::hardware_concurrency();
-    return hw ? std::min(hw, 16u) : 1u;
-}
-
 template <class F>
 void parallel(uint32_t T, F&& f) {
     if (T <= 1) {
@@ -68,13 +50,109 @@ void parallel(uint32_t T, F&& f) {
     for (auto& x : th) x.join();
 }
 
-int build(const char* seq, const uint64_t* rec_off, uint64_t n_rec, uint32_t k, uint32_t threads,
-          cfhost_input** out) {
-    if (!out || (n_rec && (!seq || !rec_off)) || k < 3 || k > 127) return -1;
-    const uint32_t T = std::max<uint32_t>(1, std::min<uint64_t>(pick_threads(threads), std::max<uint64_t>(1, n_rec)));
-    std::unique_ptr<cfhost_input> in(new cfhost_input());
-    in->n_records = n_rec;
-    // record ranges per thread, balanced by bytes
+// End of the line starting at p ('\n' position, or n).
+uint64_t line_end(const char* buf, uint64_t n, uint64_t p) {
+    const void* q = std::memchr(buf + p, '\n', n - p);
+    return q ? uint64_t(static_cast<const char*>(q) - buf) : n;
+}
+
+// First line start at or after p (n when none).
+uint64_t line_start_from(const char* buf, uint64_t n, uint64_t p) {
+    if (p == 0) return 0;
+    if (p >= n) return n;
+    if (buf[p - 1] == '\n') return p;
+    const uint64_t e = line_end(buf, n, p);
+    return e < n ? e + 1 : n;
+}
+
+// First '>' line starting at or after p (n when none).
+uint64_t header_from(const char* buf, uint64_t n, uint64_t p) {
+    p = line_start_from(buf, n, p);
+    if (p < n && buf[p] == '>') return p;
+    while (p < n) {  // p is a line start holding no '>', or just past a '>' that starts no line
+        const void* q = std::memchr(buf + p, '>', n - p);
+        if (!q) return n;
+        const uint64_t x = uint64_t(static_cast<const char*>(q) - buf);  // x > 0
+        if (buf[x - 1] == '\n') return x;
+        p = x + 1;
+    }
+    return n;
+}
+
+// FASTQ records whose lines start in [b, e); `li` is the number of the first
+// line starting at or after b (lines counted from the text's start).
+void fastq_range(const char* buf, uint64_t n, uint64_t b, uint64_t e, uint64_t nl_before, Seg& sg) {
+    uint64_t p = line_start_from(buf, n, b);
+    uint64_t li = nl_before + (p > b ? 1 : 0);  // one '\n' in [b, p) when p moved
+    sg.seq = buf;
+    while (p < e && p < n) {
+        const uint64_t end = line_end(buf, n, p);
+        if (li % 4 == 1) sg.rec.emplace_back(p, end);
+        ++li;
+        p = end + 1;
+    }
+}
+
+// FASTA records whose '>' line starts in [b, e), each running to the next
+// '>' line (wherever it starts) or the end of the text.
+void fasta_range(const char* buf, uint64_t n, uint64_t b, uint64_t e, Seg& sg) {
+    uint64_t p = header_from(buf, n, b);
+    bool have = false;
+    uint64_t start = 0;
+    while (p < n) {
+        const uint64_t end = line_end(buf, n, p);
+        if (end > p && buf[p] == '>') {
+            if (p >= e) break;  // a later segment's record
+            if (have) sg.rec.emplace_back(start, sg.own.size());
+            have = true;
+            start = sg.own.size();
+        } else if (have) {
+            sg.own.insert(sg.own.end(), buf + p, buf + end);
+        }
+        p = end + 1;
+    }
+    if (have) sg.rec.emplace_back(start, sg.own.size());
+    sg.seq = sg.own.data();
+}
+
+// Maximal DNA runs of length >= k+1 in s[b, e) (src/Graph_Partitioner.cpp:476-501).
+template <class V>
+void find_fragments(const char* s, uint64_t b, uint64_t e, uint32_t k, V& out) {
+    uint64_t i = b;
+    while (i < e) {
+        while (i < e && LUT.c[uint8_t(s[i])] == 0xFF) ++i;
+        const uint64_t st = i;
+        while (i < e && LUT.c[uint8_t(s[i])] != 0xFF) ++i;
+        if (i - st >= uint64_t(k) + 1) out.push_back({st, i - st});
+    }
+}
+
+}  // namespace
+
+uint32_t pick_threads(uint32_t t) {
+    if (t) return t;
+    const unsigned hw = std::thread::hardware_concurrency();
+    return hw ? std::min(hw, 16u) : 1u;
+}
+
+std::vector<Seg> parse_text(const char* buf, uint64_t n, int is_fastq, uint32_t T) {
+    T = std::max<uint32_t>(1, std::min<uint64_t>(T, std::max<uint64_t>(1, n >> 16)));  // >= 64 KiB each
+    std::vector<uint64_t> r(T + 1);
+    for (uint32_t t = 0; t <= T; ++t) r[t] = n * t / T;
+    std::vector<Seg> segs(T);
+    if (is_fastq) {
+        std::vector<uint64_t> nl(T + 1, 0);  // '\n' in [0, r[t])
+        parallel(T, [&](uint32_t t) { nl[t + 1] = uint64_t(std::count(buf + r[t], buf + r[t + 1], '\n')); });
+        for (uint32_t t = 0; t < T; ++t) nl[t + 1] += nl[t];
+        parallel(T, [&](uint32_t t) { fastq_range(buf, n, r[t], r[t + 1], nl[t], segs[t]); });
+    } else {
+        parallel(T, [&](uint32_t t) { fasta_range(buf, n, r[t], r[t + 1], segs[t]); });
+    }
+    return segs;
+}
+
+std::vector<Seg> segs_of_records(const char* seq, const uint64_t* rec_off, uint64_t n_rec, uint32_t T) {
+    T = std::max<uint32_t>(1, std::min<uint64_t>(T, std::max<uint64_t>(1, n_rec)));
     const uint64_t total = n_rec ? rec_off[n_rec] - rec_off[0] : 0;
     std::vector<uint64_t> rb(T + 1, n_rec);
     rb[0] = 0;
@@ -83,34 +161,52 @@ int build(const char* seq, const uint64_t* rec_off, uint64_t n_rec, uint32_t k, 
         rb[t] = uint64_t(std::lower_bound(rec_off, rec_off + n_rec, target) - rec_off);
         rb[t] = std::max(rb[t], rb[t - 1]);
     }
-    std::vector<std::vector<Frag>> fr(T);
+    std::vector<Seg> segs(T);
+    for (uint32_t t = 0; t < T; ++t) {
+        segs[t].seq = seq;
+        for (uint64_t i = rb[t]; i < rb[t + 1]; ++i) segs[t].rec.emplace_back(rec_off[i], rec_off[i + 1]);
+    }
+    return segs;
+}
+
+void Packer::plan(const std::vector<Seg>& segs, uint32_t k) {
+    segs_ = &segs;
+    const uint32_t T = uint32_t(segs.size());
+    fr_.assign(T, {});
     parallel(T, [&](uint32_t t) {
-        for (uint64_t r = rb[t]; r < rb[t + 1]; ++r) find_fragments(seq, rec_off[r], rec_off[r + 1], k, fr[t]);
+        for (const auto& r : segs[t].rec) find_fragments(segs[t].seq, r.first, r.second, k, fr_[t]);
     });
-    std::vector<uint64_t> tb(T + 1, 0), tf(T + 1, 0);
+    tb_.assign(T + 1, 0);
+    tf_.assign(T + 1, 0);
     for (uint32_t t = 0; t < T; ++t) {
         uint64_t s = 0;
-        for (const Frag& f : fr[t]) s += f.len;
-        tb[t + 1] = tb[t] + s;
-        tf[t + 1] = tf[t] + fr[t].size();
+        for (const Frag& f : fr_[t]) s += f.len;
+        tb_[t + 1] = tb_[t] + s;
+        tf_[t + 1] = tf_[t] + fr_[t].size();
     }
-    in->n_bases = tb[T];
-    in->packed.assign((in->n_bases + 31) / 32 + 2, 0);
-    in->frag_off.resize(tf[T] + 1);
-    in->frag_off[tf[T]] = in->n_bases;
-    uint64_t* P = in->packed.data();
+}
+
+void Packer::write(uint64_t* P, uint64_t* frag_off) const {
+    const std::vector<Seg>& segs = *segs_;
+    const uint32_t T = uint32_t(segs.size());
+    const uint64_t nb = n_bases(), nw = n_words();
+    // words shared by two segments are ORed into: zero them (and the two pad words) first
+    P[nw - 1] = P[nw - 2] = 0;
+    for (uint32_t t = 0; t < T; ++t)
+        if (tb_[t + 1] > tb_[t]) P[tb_[t] / 32] = P[(tb_[t + 1] - 1) / 32] = 0;
+    frag_off[tf_[T]] = nb;
     parallel(T, [&](uint32_t t) {
-        uint64_t pos = tb[t], fi = tf[t];
-        const uint64_t first_w = tb[t] / 32, last_w = tb[t + 1] ? (tb[t + 1] - 1) / 32 : 0;
+        if (tb_[t + 1] == tb_[t]) return;
+        uint64_t pos = tb_[t], fi = tf_[t];
+        const uint64_t first_w = tb_[t] / 32, last_w = (tb_[t + 1] - 1) / 32;
         uint64_t cur = 0, cur_w = first_w;
         auto flush = [&](uint64_t w, uint64_t v) {
-            // words shared with a neighbouring thread's range are merged with OR
             if (w == first_w || w == last_w) __atomic_fetch_or(&P[w], v, __ATOMIC_RELAXED);
             else P[w] = v;
         };
-        for (const Frag& f : fr[t]) {
-            in->frag_off[fi++] = pos;
-            const char* s = seq + f.beg;
+        for (const Frag& f : fr_[t]) {
+            frag_off[fi++] = pos;
+            const char* s = segs[t].seq + f.beg;
             for (uint64_t i = 0; i < f.len; ++i, ++pos) {
                 const uint64_t w = pos >> 5;
                 if (w != cur_w) {
@@ -121,40 +217,66 @@ int build(const char* seq, const uint64_t* rec_off, uint64_t n_rec, uint32_t k, 
                 cur |= uint64_t(LUT.c[uint8_t(s[i])]) << (62 - 2 * (pos & 31));
             }
         }
-        if (tb[t + 1] > tb[t]) flush(cur_w, cur);
+        flush(cur_w, cur);
     });
-    *out = in.release();
-    return 0;
 }
 
-// FASTA / FASTQ -> concatenated record bytes + offsets.
-void parse(const char* buf, uint64_t n, int is_fastq, std::vector<char>& seq,
-           std::vector<uint64_t>& off) {
-    seq.clear();
-    off.assign(1, 0);
-    seq.reserve(n);
-    uint64_t pos = 0, line = 0;
-    bool have = false;
-    while (pos < n) {
-        const void* nl = std::memchr(buf + pos, '\n', n - pos);
-        const uint64_t end = nl ? uint64_t(static_cast<const char*>(nl) - buf) : n;
-        const char* s = buf + pos;
-        const uint64_t len = end - pos;
-        if (is_fastq) {
-            if (line % 4 == 1) {
-                seq.insert(seq.end(), s, s + len);
-                off.push_back(seq.size());
-            }
-        } else if (len > 0 && s[0] == '>') {
-            if (have) off.push_back(seq.size());
-            have = true;
-        } else if (have) {
-            seq.insert(seq.end(), s, s + len);
-        }
-        ++line;
-        pos = end + 1;
+std::vector<uint64_t> split_records(const char* buf, uint64_t n, int is_fastq, uint64_t chunk_bytes, uint32_t T) {
+    std::vector<uint64_t> off{0};
+    if (n == 0) {
+        off.push_back(0);
+        return off;
     }
-    if (!is_fastq && have) off.push_back(seq.size());
+    chunk_bytes = std::max<uint64_t>(chunk_bytes, 1);
+    const uint64_t m = (n + chunk_bytes - 1) / chunk_bytes;
+    std::vector<uint64_t> cand(m + 1, n);
+    cand[0] = 0;
+    if (is_fastq) {
+        // line numbers at the targets: '\n' counts per target interval
+        std::vector<uint64_t> nl(m + 1, 0);
+        T = std::max<uint32_t>(1, std::min<uint64_t>(T, m));
+        parallel(T, [&](uint32_t t) {
+            for (uint64_t j = t; j < m; j += T) {
+                const uint64_t a = j * chunk_bytes, b = std::min(n, a + chunk_bytes);
+                nl[j + 1] = uint64_t(std::count(buf + a, buf + b, '\n'));
+            }
+        });
+        for (uint64_t j = 0; j < m; ++j) nl[j + 1] += nl[j];
+        for (uint64_t j = 1; j < m; ++j) {
+            const uint64_t x = j * chunk_bytes;
+            uint64_t p = line_start_from(buf, n, x), li = nl[j] + (p > x ? 1 : 0);
+            while (p < n && li % 4 != 0) {
+                p = line_end(buf, n, p) + 1;
+                ++li;
+            }
+            cand[j] = std::min(p, n);
+        }
+    } else {
+        for (uint64_t j = 1; j < m; ++j) cand[j] = header_from(buf, n, j * chunk_bytes);
+    }
+    for (uint64_t j = 1; j <= m; ++j)
+        if (cand[j] > off.back()) off.push_back(cand[j]);
+    if (off.back() != n) off.push_back(n);
+    return off;
+}
+
+}  // namespace cfhost_detail
+
+using namespace cfhost_detail;
+
+namespace {
+
+int build(std::vector<Seg> segs, uint64_t n_rec, uint32_t k, cfhost_input** out) {
+    std::unique_ptr<cfhost_input> in(new cfhost_input());
+    Packer pk;
+    pk.plan(segs, k);
+    in->n_records = n_rec;
+    in->n_bases = pk.n_bases();
+    in->packed.assign(pk.n_words(), 0);
+    in->frag_off.resize(pk.n_frag() + 1);
+    pk.write(in->packed.data(), in->frag_off.data());
+    *out = in.release();
+    return 0;
 }
 
 }  // namespace
@@ -163,16 +285,17 @@ extern "C" {
 
 int cfhost_input_from_records(const char* seq, const uint64_t* rec_off, uint64_t n_rec, uint32_t k,
                               uint32_t threads, cfhost_input** out) {
-    return build(seq, rec_off, n_rec, k, threads, out);
+    if (!out || (n_rec && (!seq || !rec_off)) || k < 3 || k > 127) return -1;
+    return build(segs_of_records(seq, rec_off, n_rec, pick_threads(threads)), n_rec, k, out);
 }
 
 int cfhost_input_from_buffer(const char* buf, uint64_t n, int is_fastq, uint32_t k, uint32_t threads,
                              cfhost_input** out) {
-    if (!buf && n) return -1;
-    std::vector<char> seq;
-    std::vector<uint64_t> off;
-    parse(buf, n, is_fastq, seq, off);
-    return build(seq.data(), off.data(), off.size() - 1, k, threads, out);
+    if (!out || (!buf && n) || k < 3 || k > 127) return -1;
+    std::vector<Seg> segs = parse_text(buf, n, is_fastq, pick_threads(threads));
+    uint64_t n_rec = 0;
+    for (const Seg& s : segs) n_rec += s.rec.size();
+    return build(std::move(segs), n_rec, k, out);
 }
 
 int cfhost_input_from_file(const char* path, int is_fastq, uint32_t k, uint32_t threads,
@@ -204,5 +327,16 @@ int cfhost_input_get(const cfhost_input* in, const uint64_t** packed, uint64_t* 
 }
 
 void cfhost_input_free(cfhost_input* in) { delete in; }
+
+int cfhost_split_records(const char* buf, uint64_t n, int is_fastq, uint64_t chunk_bytes, uint32_t threads,
+                         uint64_t* offsets, uint64_t cap, uint64_t* n_chunks) {
+    if ((!buf && n) || !n_chunks) return -1;
+    const std::vector<uint64_t> off = split_records(buf, n, is_fastq, chunk_bytes, pick_threads(threads));
+    *n_chunks = off.size() - 1;
+    if (!offsets) return 0;
+    if (cap < off.size()) return -1;
+    std::copy(off.begin(), off.end(), offsets);
+    return 0;
+}
 
 }  // extern "C"

@@ -63,7 +63,8 @@ class RunStats(ctypes.Structure):
                    ("n_contract_runs", ctypes.c_uint32), ("t_cdbg_ms", ctypes.c_double),
                    ("n_cdbg_unitigs", ctypes.c_uint64), ("n_cdbg_bytes", ctypes.c_uint64),
                    ("n_cdbg_rounds", ctypes.c_uint32), ("n_cdbg_cycle_pieces", ctypes.c_uint32),
-                   ("n_contract_dense", ctypes.c_uint32)])
+                   ("n_contract_dense", ctypes.c_uint32), ("n_partition_chunks", ctypes.c_uint32),
+                   ("t_partition_merge_ms", ctypes.c_double)])
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -99,6 +100,16 @@ class SizingModel(ctypes.Structure):
                [("lr", ctypes.c_float), ("loss", ctypes.c_float)]
 
 
+class IngestStats(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_uint64) for n in ("n_bytes", "n_records", "n_bases", "n_fragments")]
+                + [("n_chunks", ctypes.c_uint32), ("threads", ctypes.c_uint32)]
+                + [(n, ctypes.c_double) for n in ("t_total_ms", "t_split_ms", "t_host_ms", "t_device_ms",
+                                                  "t_finish_ms")])
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class TuneTrial(ctypes.Structure):
     _fields_ = [("tuning", Tuning), ("ms", ctypes.c_double), ("identical", ctypes.c_uint32),
                 ("reserved", ctypes.c_uint32)]
@@ -109,8 +120,10 @@ SIGNATURES = {
     "cfgpu_open": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_void_p)]),
     "cfgpu_close": (ctypes.c_int, [ctypes.c_void_p]),
     "cfgpu_last_error": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]),
+    "cfgpu_params_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Params)]),
     "cfgpu_partition": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
                                        ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32]),
+    "cfgpu_partition_finish": (ctypes.c_int, [ctypes.c_void_p]),
     "cfgpu_bucket_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BucketStat),
                                           ctypes.c_uint32]),
     "cfgpu_records_copy": (ctypes.c_int, [ctypes.c_void_p, u32p, u64p, u64p, ctypes.c_uint64,
@@ -161,6 +174,12 @@ SIGNATURES = {
     "cfhost_input_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(u64p), u64p, u64p,
                                         ctypes.POINTER(u64p), u64p, u64p]),
     "cfhost_input_free": (None, [ctypes.c_void_p]),
+    "cfhost_split_records": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64,
+                                            ctypes.c_uint32, u64p, ctypes.c_uint64, u64p]),
+    "cfhost_partition_buffer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int,
+                                               ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(IngestStats)]),
+    "cfhost_partition_file": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_uint32,
+                                             ctypes.c_uint64, ctypes.POINTER(IngestStats)]),
     "cfhost_synth_genome": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_double, ctypes.c_uint32,
                                            ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64,
                                            ctypes.c_void_p]),
@@ -200,6 +219,18 @@ def lib():
 
 def _p(a, t):
     return a.ctypes.data_as(t)
+
+
+def split_records(buf: bytes, fastq: bool, chunk_bytes: int, threads: int = 0):
+    """cfhost_split_records: chunk boundaries (byte offsets) at record starts."""
+    n = ctypes.c_uint64()
+    if lib().cfhost_split_records(buf, len(buf), int(fastq), chunk_bytes, threads, None, 0, ctypes.byref(n)) != 0:
+        raise RuntimeError("cfhost_split_records failed")
+    off = np.zeros(n.value + 1, np.uint64)
+    if lib().cfhost_split_records(buf, len(buf), int(fastq), chunk_bytes, threads, _p(off, u64p), off.size,
+                                  ctypes.byref(n)) != 0:
+        raise RuntimeError("cfhost_split_records failed")
+    return off
 
 
 # ---------------------------------------------------------------------------
@@ -372,9 +403,25 @@ class Handle:
     def __exit__(self, *a):
         self.close()
 
-    def partition(self, inp: Input):
+    def partition(self, inp: Input, append: bool = False):
         self._check(lib().cfgpu_partition(self._h, inp.packed.ctypes.data, inp.n_bases,
-                                          inp.frag_off.ctypes.data, inp.n_frag, 0), "partition")
+                                          inp.frag_off.ctypes.data, inp.n_frag, 2 if append else 0), "partition")
+
+    def partition_finish(self):
+        self._check(lib().cfgpu_partition_finish(self._h), "partition_finish")
+
+    def ingest_buffer(self, buf: bytes, fastq: bool, threads: int = 0, chunk_bytes: int = 0):
+        """cfhost_partition_buffer: chunked parse + pack overlapped with the device partition."""
+        st = IngestStats()
+        r = lib().cfhost_partition_buffer(self._h, buf, len(buf), int(fastq), threads, chunk_bytes, ctypes.byref(st))
+        self._check(r, "ingest_buffer")
+        return st.as_dict()
+
+    def ingest_file(self, path: str, fastq: bool, threads: int = 0, chunk_bytes: int = 0):
+        st = IngestStats()
+        r = lib().cfhost_partition_file(self._h, path.encode(), int(fastq), threads, chunk_bytes, ctypes.byref(st))
+        self._check(r, "ingest_file")
+        return st.as_dict()
 
     def partition_device(self, packed_ptr, n_bases, frag_ptr, n_frag):
         self._check(lib().cfgpu_partition(self._h, packed_ptr, n_bases, frag_ptr, n_frag, 1),

@@ -58,6 +58,8 @@ typedef struct cfgpu_handle cfgpu_handle;
  * (src/Subgraphs_Manager.cpp:26-52) and Edge_Frequency::set_edge_threshold
  * (src/State_Config.cpp:10-14, called at src/dBG_Contractor.cpp:24). */
 int cfgpu_open(const cfgpu_params* params, cfgpu_handle** out);
+/* The parameters the handle was opened with. */
+int cfgpu_params_get(const cfgpu_handle* h, cfgpu_params* out);
 int cfgpu_close(cfgpu_handle* h);
 /* Copies the last error message (NUL-terminated, truncated to n bytes). */
 int cfgpu_last_error(const cfgpu_handle* h, char* buf, size_t n);
@@ -74,9 +76,18 @@ int cfgpu_last_error(const cfgpu_handle* h, char* buf, size_t n);
  * (include/Subgraphs_Manager.hpp:93, 113; src/Subgraphs_Manager.cpp:84-95): the weak
  * super k-mers are left in 16384 device-resident buckets in the reference record
  * format (Super_Kmer_Attributes<false> + MSB-aligned label). */
-enum { CFGPU_INPUT_DEVICE = 1 };
+enum { CFGPU_INPUT_DEVICE = 1, CFGPU_PARTITION_APPEND = 2 };
 int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
                     const uint64_t* frag_off, uint64_t n_frag, uint32_t flags);
+/* Input in chunks: with CFGPU_PARTITION_APPEND, cfgpu_partition partitions one chunk of
+ * whole fragments and keeps its records aside; cfgpu_partition_finish gathers every
+ * appended chunk's records into the buckets, which are then what one cfgpu_partition of
+ * all the chunks' fragments gives (record order inside a bucket aside).  This is the
+ * reference's Subgraphs_Manager::finalize after the partitioner's workers have added
+ * their chunks' super k-mers (include/Subgraphs_Manager.hpp:93, 113; src/Graph_Partitioner.cpp:
+ * 121-184).  Until the finish, the handle holds no buckets; a partition without APPEND
+ * or a cfgpu_load_buckets drops the appended chunks. */
+int cfgpu_partition_finish(cfgpu_handle* h);
 
 typedef struct cfgpu_bucket_stat {
     uint64_t n_records;   /* weak super k-mers in the bucket */
@@ -265,6 +276,8 @@ typedef struct cfgpu_run_stats {
     uint32_t n_cdbg_rounds;           /* pointer-jumping rounds */
     uint32_t n_cdbg_cycle_pieces;     /* lm-tigs on cycles through several subgraphs (host) */
     uint32_t n_contract_dense;        /* subgraphs contracted with dense tables (too large for LDS) */
+    uint32_t n_partition_chunks;      /* input chunks the last partition gathered (1 without APPEND) */
+    double t_partition_merge_ms;      /* cfgpu_partition_finish's gather of the chunks' buckets */
 } cfgpu_run_stats;
 int cfgpu_run_stats_get(cfgpu_handle* h, cfgpu_run_stats* out);
 

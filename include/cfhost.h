@@ -38,6 +38,39 @@ int cfhost_input_get(const cfhost_input* in, const uint64_t** packed, uint64_t* 
                      uint64_t* n_records);
 void cfhost_input_free(cfhost_input* in);
 
+/* Chunk boundaries at record starts about chunk_bytes apart: offsets[0] = 0 < ... <
+ * offsets[*n_chunks] = n (offsets may be NULL to query *n_chunks; it then holds
+ * *n_chunks + 1 entries, cap of them at most).  Each chunk parses on its own to the
+ * records the whole text gives there: a FASTQ boundary starts a line whose number is a
+ * multiple of 4, a FASTA boundary a '>' line (a record longer than chunk_bytes is one
+ * chunk). */
+int cfhost_split_records(const char* buf, uint64_t n, int is_fastq, uint64_t chunk_bytes, uint32_t threads,
+                         uint64_t* offsets, uint64_t cap, uint64_t* n_chunks);
+
+/* The reference's partition of an input file, end to end, with host and device
+ * overlapped (src/Graph_Partitioner.cpp:121-184, 424-614; Subgraphs_Manager::finalize,
+ * include/Subgraphs_Manager.hpp:113): the text is cut into chunks of whole records
+ * (cfhost_split_records; chunk_bytes 0 means 256 MiB); while the device copies chunk i
+ * from pinned host memory and partitions it (cfgpu_partition with
+ * CFGPU_PARTITION_APPEND), `threads` host threads parse and pack chunk i+1; then
+ * cfgpu_partition_finish.  The handle then holds the buckets one cfgpu_partition of the
+ * whole input gives.  The file is memory-mapped and read as the parser reaches it.
+ * Returns 0, a CFGPU_E_* code from the device calls, -1 on bad arguments, -2 when the
+ * file cannot be read, -3 when pinned host memory runs out. */
+typedef struct cfhost_ingest_stats {
+    uint64_t n_bytes, n_records, n_bases, n_fragments;
+    uint32_t n_chunks, threads;
+    double t_total_ms;   /* text -> buckets finished */
+    double t_split_ms;   /* chunk boundaries */
+    double t_host_ms;    /* parse + pack, summed over chunks (all but the first overlap the device) */
+    double t_device_ms;  /* the cfgpu_partition calls (copy from pinned memory + kernels), summed */
+    double t_finish_ms;  /* cfgpu_partition_finish */
+} cfhost_ingest_stats;
+int cfhost_partition_buffer(cfgpu_handle* h, const char* buf, uint64_t n, int is_fastq, uint32_t threads,
+                            uint64_t chunk_bytes, cfhost_ingest_stats* stats);
+int cfhost_partition_file(cfgpu_handle* h, const char* path, int is_fastq, uint32_t threads, uint64_t chunk_bytes,
+                          cfhost_ingest_stats* stats);
+
 /* Seeded synthetic sequences (ACGT only).  All are deterministic in `seed`
  * regardless of `threads`. */
 /* iid genome of length len; then repeat families (lengths log-uniform in
