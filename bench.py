@@ -198,6 +198,9 @@ def main():
     ap.add_argument("--cpu-sample-reads", type=int, default=0, help="reads in the CPU baseline sample (0: auto)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ingest", action="store_true")
+    ap.add_argument("--ingest-chunk-mib", type=int, default=1024,
+                    help="chunk size of the pipelined FASTQ ingest (parse + pack overlapped with the partition)")
     ap.add_argument("--max-passes", type=int, default=-1,
                     help="tuning: LDS passes above which a bucket is built with global atomics (0: always)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -325,6 +328,55 @@ def main():
         hip.sync()
         learned_construct = (time.perf_counter() - t0) * 1e3 / args.compare_steps
 
+    ingest = None
+    if not args.no_ingest:
+        # the workload's reads as FASTQ text in host memory (4-line records),
+        # to buckets: parse + pack everything, then one partition from pageable
+        # memory; and the pipeline (chunks of whole records, parse + pack of the
+        # next chunk into pinned memory overlapped with the current one's copy
+        # and partition, then the gather)
+        rl, nrd = w["read_len"], w["reads"]
+        txt = np.empty((nrd, 2 * rl + 7), np.uint8)
+        txt[:, 0:3] = np.frombuffer(b"@r\n", np.uint8)
+        txt[:, 3:3 + rl] = np.frombuffer(reads, np.uint8).reshape(nrd, rl)
+        txt[:, 3 + rl:6 + rl] = np.frombuffer(b"\n+\n", np.uint8)
+        txt[:, 6 + rl:6 + 2 * rl] = ord("I")
+        txt[:, 6 + 2 * rl] = ord("\n")
+        text = txt.tobytes()
+        del txt
+        log(f"ingest: {len(text) / 1e9:.2f} GB of FASTQ, sequential")
+        hip.sync()
+        t0 = time.perf_counter()
+        hin = C.Input.from_buffer(text, True, k, threads=16)
+        t1 = time.perf_counter()
+        h.partition(hin)
+        hip.sync()
+        t2 = time.perf_counter()
+        n_rec_seq = h.run_stats()["n_records"]
+        del hin
+        log("ingest: pipelined")
+        runs = []
+        for _ in range(2):
+            t3 = time.perf_counter()
+            sti = h.ingest_buffer(text, True, threads=16, chunk_bytes=args.ingest_chunk_mib << 20)
+            runs.append(((time.perf_counter() - t3) * 1e3, sti))
+        t_pipe, sti = min(runs, key=lambda x: x[0])
+        rsi = h.run_stats()
+        assert rsi["n_records"] == n_rec_seq == rs["n_records"], "ingest paths disagree on the records"
+        rest = learned_construct + ph["t_contract_ms"] + ph["t_cdbg_ms"] + wall["copy_to_host"]
+        ingest = {"fastq_bytes": len(text), "threads": sti["threads"],
+                  "sequential_ms": {"parse_pack": (t1 - t0) * 1e3, "partition_from_host": (t2 - t1) * 1e3,
+                                    "total": (t2 - t0) * 1e3},
+                  "pipelined_ms": {"total": t_pipe, "chunks": sti["n_chunks"], "chunk_mib": args.ingest_chunk_mib,
+                                   "host_parse_pack_summed": sti["t_host_ms"],
+                                   "device_partition_summed": sti["t_device_ms"],
+                                   "gather": sti["t_finish_ms"]},
+                  "text_gb_per_s": len(text) / (t_pipe / 1e3) / 1e9,
+                  "fastq_to_unitigs_kmers_per_s": n_km / ((t_pipe + rest) / 1e3),
+                  "note": "FASTQ text already in host memory (file reading not included); the step after the "
+                          "partition as in `value`"}
+        del text
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -415,6 +467,7 @@ def main():
                       "infer": ph["t_infer_ms"], "construct_main": ph["t_construct_main_ms"],
                       "fallback": ph["t_fallback_ms"], "contract": ph["t_contract_ms"], "cdbg": ph["t_cdbg_ms"],
                       "copy_to_host": wall["copy_to_host"], "wall": wall},
+        "ingest": ingest,
         "paths": {
             "vertex_states": {"ms": part_ms + learned_construct,
                               "kmers_per_s": n_km / ((part_ms + learned_construct) / 1000.0),

@@ -116,15 +116,35 @@ void fasta_range(const char* buf, uint64_t n, uint64_t b, uint64_t e, Seg& sg) {
 }
 
 // Maximal DNA runs of length >= k+1 in s[b, e) (src/Graph_Partitioner.cpp:476-501).
+// Runs are crossed 8 bytes per branch: the OR of 8 codes has bit 7 set when
+// any of the 8 bytes is not a base.
 template <class V>
 void find_fragments(const char* s, uint64_t b, uint64_t e, uint32_t k, V& out) {
+    const uint8_t* c = LUT.c;
     uint64_t i = b;
     while (i < e) {
-        while (i < e && LUT.c[uint8_t(s[i])] == 0xFF) ++i;
+        while (i < e && c[uint8_t(s[i])] == 0xFF) ++i;
         const uint64_t st = i;
-        while (i < e && LUT.c[uint8_t(s[i])] != 0xFF) ++i;
+        while (i + 8 <= e &&
+               !((c[uint8_t(s[i])] | c[uint8_t(s[i + 1])] | c[uint8_t(s[i + 2])] | c[uint8_t(s[i + 3])] |
+                  c[uint8_t(s[i + 4])] | c[uint8_t(s[i + 5])] | c[uint8_t(s[i + 6])] | c[uint8_t(s[i + 7])]) &
+                 0x80))
+            i += 8;
+        while (i < e && c[uint8_t(s[i])] != 0xFF) ++i;
         if (i - st >= uint64_t(k) + 1) out.push_back({st, i - st});
     }
+}
+
+// 2-bit codes of 8 bases (A/C/G/T in either case) at s, the first in the top
+// two bits: ((c >> 1) ^ (c >> 2)) & 3 maps A C G T (and a c g t) to 0 1 2 3.
+inline uint32_t codes8(const char* s) {
+    uint64_t v;
+    std::memcpy(&v, s, 8);
+    v = __builtin_bswap64(v);  // the first base in the top byte
+    uint64_t c = ((v >> 1) ^ (v >> 2)) & 0x0303030303030303ull;
+    c = (c | (c >> 6)) & 0x000F000F000F000Full;
+    c = (c | (c >> 12)) & 0x000000FF000000FFull;
+    return uint32_t((c | (c >> 24)) & 0xFFFFu);
 }
 
 }  // namespace
@@ -135,20 +155,35 @@ uint32_t pick_threads(uint32_t t) {
     return hw ? std::min(hw, 16u) : 1u;
 }
 
-std::vector<Seg> parse_text(const char* buf, uint64_t n, int is_fastq, uint32_t T) {
-    T = std::max<uint32_t>(1, std::min<uint64_t>(T, std::max<uint64_t>(1, n >> 16)));  // >= 64 KiB each
+std::vector<Seg> parse_chunk(const char* buf, uint64_t n, int is_fastq, uint64_t b, uint64_t x, uint32_t T,
+                             uint64_t* end) {
+    x = std::min(std::max(x, b), n);
+    T = std::max<uint32_t>(1, std::min<uint64_t>(T, std::max<uint64_t>(1, (x - b) >> 16)));  // >= 64 KiB each
     std::vector<uint64_t> r(T + 1);
-    for (uint32_t t = 0; t <= T; ++t) r[t] = n * t / T;
+    for (uint32_t t = 0; t <= T; ++t) r[t] = b + (x - b) * t / T;
     std::vector<Seg> segs(T);
+    uint64_t e = n;
     if (is_fastq) {
-        std::vector<uint64_t> nl(T + 1, 0);  // '\n' in [0, r[t])
+        // line numbers from b (a record start: its line number is a multiple of 4)
+        std::vector<uint64_t> nl(T + 1, 0);
         parallel(T, [&](uint32_t t) { nl[t + 1] = uint64_t(std::count(buf + r[t], buf + r[t + 1], '\n')); });
         for (uint32_t t = 0; t < T; ++t) nl[t + 1] += nl[t];
+        // the chunk ends at the first record start at or after x
+        e = line_start_from(buf, n, x);
+        for (uint64_t li = nl[T] + (e > x ? 1 : 0); e < n && li % 4 != 0; ++li) e = line_end(buf, n, e) + 1;
+        e = std::min(e, n);
+        r[T] = e;  // the last segment takes the lines from x to the boundary
         parallel(T, [&](uint32_t t) { fastq_range(buf, n, r[t], r[t + 1], nl[t], segs[t]); });
     } else {
+        e = header_from(buf, n, x);  // records whose '>' line starts before x run up to it
         parallel(T, [&](uint32_t t) { fasta_range(buf, n, r[t], r[t + 1], segs[t]); });
     }
+    if (end) *end = e;
     return segs;
+}
+
+std::vector<Seg> parse_text(const char* buf, uint64_t n, int is_fastq, uint32_t T) {
+    return parse_chunk(buf, n, is_fastq, 0, n, T, nullptr);
 }
 
 std::vector<Seg> segs_of_records(const char* seq, const uint64_t* rec_off, uint64_t n_rec, uint32_t T) {
@@ -207,7 +242,27 @@ void Packer::write(uint64_t* P, uint64_t* frag_off) const {
         for (const Frag& f : fr_[t]) {
             frag_off[fi++] = pos;
             const char* s = segs[t].seq + f.beg;
-            for (uint64_t i = 0; i < f.len; ++i, ++pos) {
+            uint64_t i = 0;
+            // one base at a time up to a multiple of 8 in the word, then 8 at a time
+            for (; i < f.len && ((pos & 7) || i + 8 > f.len); ++i, ++pos) {
+                const uint64_t w = pos >> 5;
+                if (w != cur_w) {
+                    flush(cur_w, cur);
+                    cur = 0;
+                    cur_w = w;
+                }
+                cur |= uint64_t(LUT.c[uint8_t(s[i])]) << (62 - 2 * (pos & 31));
+            }
+            for (; i + 8 <= f.len; i += 8, pos += 8) {
+                const uint64_t w = pos >> 5;
+                if (w != cur_w) {
+                    flush(cur_w, cur);
+                    cur = 0;
+                    cur_w = w;
+                }
+                cur |= uint64_t(codes8(s + i)) << (48 - 2 * (pos & 31));
+            }
+            for (; i < f.len; ++i, ++pos) {
                 const uint64_t w = pos >> 5;
                 if (w != cur_w) {
                     flush(cur_w, cur);
@@ -232,19 +287,29 @@ std::vector<uint64_t> split_records(const char* buf, uint64_t n, int is_fastq, u
     std::vector<uint64_t> cand(m + 1, n);
     cand[0] = 0;
     if (is_fastq) {
-        // line numbers at the targets: '\n' counts per target interval
-        std::vector<uint64_t> nl(m + 1, 0);
-        T = std::max<uint32_t>(1, std::min<uint64_t>(T, m));
-        parallel(T, [&](uint32_t t) {
-            for (uint64_t j = t; j < m; j += T) {
-                const uint64_t a = j * chunk_bytes, b = std::min(n, a + chunk_bytes);
-                nl[j + 1] = uint64_t(std::count(buf + a, buf + b, '\n'));
-            }
+        // line numbers at the targets: '\n' counts per 64 MiB block on every
+        // thread, then per target the rest of its block
+        constexpr uint64_t BLK = uint64_t(64) << 20;
+        const uint64_t nblk = (n + BLK - 1) / BLK;
+        std::vector<uint64_t> nl(nblk + 1, 0);
+        const uint32_t Tb = std::max<uint32_t>(1, std::min<uint64_t>(T, nblk));
+        parallel(Tb, [&](uint32_t t) {
+            for (uint64_t j = t; j < nblk; j += Tb)
+                nl[j + 1] = uint64_t(std::count(buf + j * BLK, buf + std::min(n, (j + 1) * BLK), '\n'));
         });
-        for (uint64_t j = 0; j < m; ++j) nl[j + 1] += nl[j];
+        for (uint64_t j = 0; j < nblk; ++j) nl[j + 1] += nl[j];
+        uint64_t at = 0, nl_at = 0;  // '\n' in [0, at), carried from target to target
         for (uint64_t j = 1; j < m; ++j) {
-            const uint64_t x = j * chunk_bytes;
-            uint64_t p = line_start_from(buf, n, x), li = nl[j] + (p > x ? 1 : 0);
+            const uint64_t x = j * chunk_bytes, xb = x / BLK;
+            if (cand[j - 1] >= x) {  // no record starts in [x_{j-1}, x): the same boundary
+                cand[j] = cand[j - 1];
+                continue;
+            }
+            if (xb * BLK > at) at = xb * BLK, nl_at = nl[xb];
+            const uint64_t nl_x = nl_at + uint64_t(std::count(buf + at, buf + x, '\n'));
+            at = x;
+            nl_at = nl_x;
+            uint64_t p = line_start_from(buf, n, x), li = nl_x + (p > x ? 1 : 0);
             while (p < n && li % 4 != 0) {
                 p = line_end(buf, n, p) + 1;
                 ++li;
@@ -252,7 +317,8 @@ std::vector<uint64_t> split_records(const char* buf, uint64_t n, int is_fastq, u
             cand[j] = std::min(p, n);
         }
     } else {
-        for (uint64_t j = 1; j < m; ++j) cand[j] = header_from(buf, n, j * chunk_bytes);
+        for (uint64_t j = 1; j < m; ++j)  // a boundary past the next target is that target's too
+            cand[j] = cand[j - 1] >= j * chunk_bytes ? cand[j - 1] : header_from(buf, n, j * chunk_bytes);
     }
     for (uint64_t j = 1; j <= m; ++j)
         if (cand[j] > off.back()) off.push_back(cand[j]);

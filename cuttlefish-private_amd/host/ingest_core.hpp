@@ -28,6 +28,12 @@ uint32_t pick_threads(uint32_t t);
 // (RabbitFX include/RabbitFX/io/Formater.cpp:87-130, 238-263).
 std::vector<Seg> parse_text(const char* buf, uint64_t n, int is_fastq, uint32_t T);
 
+// One chunk of that text: the records from b (0 or a record start) up to the
+// first record start at or after x (*end; n when none), with the same rules
+// (FASTQ line numbers counted from b, whose line number is a multiple of 4).
+std::vector<Seg> parse_chunk(const char* buf, uint64_t n, int is_fastq, uint64_t b, uint64_t x, uint32_t T,
+                             uint64_t* end);
+
 // Records given directly, split into up to T segments balanced by bytes.
 std::vector<Seg> segs_of_records(const char* seq, const uint64_t* rec_off, uint64_t n_rec, uint32_t T);
 

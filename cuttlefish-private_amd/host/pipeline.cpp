@@ -1,5 +1,6 @@
 // Chunked ingest overlapped with the device partition: the text is cut into
-// chunks of whole records; while the device copies chunk i from pinned host
+// chunks of whole records (each chunk's end found while it is parsed: the
+// first record start past chunk_bytes); while the device copies chunk i from pinned host
 // memory and partitions it (cfgpu_partition with CFGPU_PARTITION_APPEND), the
 // host threads parse and pack chunk i+1 into the other pinned buffer pair.
 // cfgpu_partition_finish then gathers the chunks' buckets.  The reference reads
@@ -52,17 +53,20 @@ struct Pinned {
     }
 };
 
-// One chunk parsed and packed.
+// One chunk parsed and packed: the records of [b, end).
 struct Packed {
     Pinned packed, frag;
-    uint64_t n_bases = 0, n_frag = 0, n_rec = 0;
+    uint64_t n_bases = 0, n_frag = 0, n_rec = 0, end = 0;
     double ms = 0;
     int err = 0;
 };
 
-void pack_chunk(const char* buf, uint64_t n, int is_fastq, uint32_t k, uint32_t T, Packed& c) {
+// The chunk from b (a record start) to the first record start at or after
+// b + chunk_bytes, found while it is parsed.
+void pack_chunk(const char* buf, uint64_t n, int is_fastq, uint64_t b, uint64_t chunk_bytes, uint32_t k, uint32_t T,
+                Packed& c) {
     const double t0 = now_ms();
-    const std::vector<Seg> segs = parse_text(buf, n, is_fastq, T);
+    const std::vector<Seg> segs = parse_chunk(buf, n, is_fastq, b, b + chunk_bytes, T, &c.end);
     Packer pk;
     pk.plan(segs, k);
     c.err = 0;
@@ -88,24 +92,19 @@ int cfhost_partition_buffer(cfgpu_handle* h, const char* buf, uint64_t n, int is
     cfgpu_params prm{};
     if (cfgpu_params_get(h, &prm) != CFGPU_OK) return -1;
     const uint32_t T = pick_threads(threads);
-    if (!chunk_bytes) chunk_bytes = uint64_t(256) << 20;
+    if (!chunk_bytes) chunk_bytes = uint64_t(1) << 30;
     const double t0 = now_ms();
-    const std::vector<uint64_t> off = split_records(buf, n, is_fastq, chunk_bytes, T);
-    const uint64_t m = off.size() - 1;
-    const double t1 = now_ms();
     Packed c[2];
-    pack_chunk(buf + off[0], off[1] - off[0], is_fastq, prm.k, T, c[0]);
+    pack_chunk(buf, n, is_fastq, 0, chunk_bytes, prm.k, T, c[0]);
     double t_host = c[0].ms, t_dev = 0;
-    uint64_t n_rec = 0, n_bases = 0, n_frag = 0;
-    for (uint64_t i = 0; i < m; ++i) {
+    uint64_t n_rec = 0, n_bases = 0, n_frag = 0, m = 0;
+    for (uint64_t i = 0;; ++i) {
         Packed& cur = c[i & 1];
         Packed& nxt = c[(i + 1) & 1];
         if (cur.err) return cur.err;
+        const bool more = cur.end < n;
         std::thread th;
-        if (i + 1 < m)
-            th = std::thread([&, i]() {
-                pack_chunk(buf + off[i + 1], off[i + 2] - off[i + 1], is_fastq, prm.k, T, nxt);
-            });
+        if (more) th = std::thread([&]() { pack_chunk(buf, n, is_fastq, cur.end, chunk_bytes, prm.k, T, nxt); });
         const double d0 = now_ms();
         const int rc = cfgpu_partition(h, cur.packed.p, cur.n_bases, cur.frag.p, cur.n_frag, CFGPU_PARTITION_APPEND);
         t_dev += now_ms() - d0;
@@ -114,7 +113,9 @@ int cfhost_partition_buffer(cfgpu_handle* h, const char* buf, uint64_t n, int is
         n_rec += cur.n_rec;
         n_bases += cur.n_bases;
         n_frag += cur.n_frag;
-        if (i + 1 < m) t_host += nxt.ms;
+        ++m;
+        if (!more) break;
+        t_host += nxt.ms;
     }
     const double f0 = now_ms();
     const int rc = cfgpu_partition_finish(h);
@@ -128,7 +129,6 @@ int cfhost_partition_buffer(cfgpu_handle* h, const char* buf, uint64_t n, int is
         st->n_chunks = uint32_t(m);
         st->threads = T;
         st->t_total_ms = f1 - t0;
-        st->t_split_ms = t1 - t0;
         st->t_host_ms = t_host;
         st->t_device_ms = t_dev;
         st->t_finish_ms = f1 - f0;

@@ -103,7 +103,7 @@ class SizingModel(ctypes.Structure):
 class IngestStats(ctypes.Structure):
     _fields_ = ([(n, ctypes.c_uint64) for n in ("n_bytes", "n_records", "n_bases", "n_fragments")]
                 + [("n_chunks", ctypes.c_uint32), ("threads", ctypes.c_uint32)]
-                + [(n, ctypes.c_double) for n in ("t_total_ms", "t_split_ms", "t_host_ms", "t_device_ms",
+                + [(n, ctypes.c_double) for n in ("t_total_ms", "t_host_ms", "t_device_ms",
                                                   "t_finish_ms")])
 
     def as_dict(self):

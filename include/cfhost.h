@@ -49,8 +49,9 @@ int cfhost_split_records(const char* buf, uint64_t n, int is_fastq, uint64_t chu
 
 /* The reference's partition of an input file, end to end, with host and device
  * overlapped (src/Graph_Partitioner.cpp:121-184, 424-614; Subgraphs_Manager::finalize,
- * include/Subgraphs_Manager.hpp:113): the text is cut into chunks of whole records
- * (cfhost_split_records; chunk_bytes 0 means 256 MiB); while the device copies chunk i
+ * include/Subgraphs_Manager.hpp:113): the text is cut into chunks of whole records (the
+ * boundaries cfhost_split_records gives, each found while its chunk is parsed;
+ * chunk_bytes 0 means 1 GiB); while the device copies chunk i
  * from pinned host memory and partitions it (cfgpu_partition with
  * CFGPU_PARTITION_APPEND), `threads` host threads parse and pack chunk i+1; then
  * cfgpu_partition_finish.  The handle then holds the buckets one cfgpu_partition of the
@@ -61,7 +62,6 @@ typedef struct cfhost_ingest_stats {
     uint64_t n_bytes, n_records, n_bases, n_fragments;
     uint32_t n_chunks, threads;
     double t_total_ms;   /* text -> buckets finished */
-    double t_split_ms;   /* chunk boundaries */
     double t_host_ms;    /* parse + pack, summed over chunks (all but the first overlap the device) */
     double t_device_ms;  /* the cfgpu_partition calls (copy from pinned memory + kernels), summed */
     double t_finish_ms;  /* cfgpu_partition_finish */
