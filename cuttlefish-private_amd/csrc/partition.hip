@@ -96,13 +96,36 @@ __global__ __launch_bounds__(P_WG) void k_minimizer_g(PartitionArgs a, uint16_t*
     const int i_lo = q0 < 0 ? 1 : 0;
     const int i_hi = int(min(N - int64_t(l) - q0, int64_t(S)));
     const int soff = int(q0 - (wlo << 5));  // -1 only in tile 0, where i = 0 has no l-mer
-    for (int i = tid; i < S; i += P_WG) {
-        const bool ok = i >= i_lo && i <= i_hi;
-        const uint32_t pos = ok ? uint32_t(i + soff) : 0u;
-        const uint32_t wi = pos >> 5, sh = (pos & 31u) << 1;
-        const uint64_t x = ((s_words[wi] << sh) | ((s_words[wi + 1] >> 1) >> (63u - sh))) >> (64 - 2 * l);
-        const uint64_t h1 = lmer_hash(x, a.wy), h2 = lmer_hash(lmer_rc(x, l), a.wy);
-        s_m[i] = ok ? (h1 < h2 ? h1 : h2) : ~0ull;
+    const uint64_t lmask = l == 32 ? ~0ull : ((1ull << (2 * l)) - 1);
+    // Eight consecutive l-mers per thread: their 32 bases as one MSB-first
+    // word `seg` and its reverse complement `rseg` (base q of rseg = the
+    // complement of base 31 - q of seg); l-mer j is bits [64 - 2(j+l), 64 - 2j)
+    // of seg and its reverse complement bits [2j, 2j + 2l) of rseg (l <= 25).
+    for (int g0 = tid * 8; g0 < S; g0 += P_WG * 8) {
+        const int p0 = g0 + soff;
+        if (l <= 25 && p0 >= 0) {
+            const uint32_t wi = uint32_t(p0) >> 5, sh = (uint32_t(p0) & 31u) << 1;
+            const uint64_t seg = (s_words[wi] << sh) | ((s_words[wi + 1] >> 1) >> (63u - sh));
+            const uint64_t rseg = ~rev2(seg);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int i = g0 + j;
+                const bool ok = i < S && i >= i_lo && i <= i_hi;
+                const uint64_t x = (seg >> (64 - 2 * l - 2 * j)) & lmask;
+                const uint64_t xr = (rseg >> (2 * j)) & lmask;
+                const uint64_t h1 = lmer_hash(x, a.wy), h2 = lmer_hash(xr, a.wy);
+                if (i < S) s_m[i] = ok ? (h1 < h2 ? h1 : h2) : ~0ull;
+            }
+        } else {
+            for (int i = g0; i < g0 + 8 && i < S; ++i) {
+                const bool ok = i >= i_lo && i <= i_hi;
+                const uint32_t pos = ok ? uint32_t(i + soff) : 0u;
+                const uint32_t wi = pos >> 5, sh = (pos & 31u) << 1;
+                const uint64_t x = ((s_words[wi] << sh) | ((s_words[wi + 1] >> 1) >> (63u - sh))) >> (64 - 2 * l);
+                const uint64_t h1 = lmer_hash(x, a.wy), h2 = lmer_hash(lmer_rc(x, l), a.wy);
+                s_m[i] = ok ? (h1 < h2 ? h1 : h2) : ~0ull;
+            }
+        }
     }
     if (tid < SPAN_MAX) s_m[S + tid] = ~0ull;  // reads past the last l-mer see the maximum
     __syncthreads();
