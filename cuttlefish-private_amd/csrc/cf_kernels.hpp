@@ -182,7 +182,14 @@ struct DedupArgs {
     const uint32_t* item_passes;
     uint32_t n_items;
     uint32_t* queue_head;
+    uint8_t* region;  // per workgroup DD_REGION_RECORDS * 5 bytes (all-pass items)
 };
+// An all-pass dedup item (item_pass == DD_ALL_PASSES): a bucket of 2 ..
+// DD_ALL_PASS_MAX passes and at most DD_REGION_RECORDS records, its passes in
+// one workgroup.
+constexpr uint32_t DD_ALL_PASSES = 0xFFFFFFFFu;
+constexpr uint32_t DD_ALL_PASS_MAX = 64;
+constexpr uint32_t DD_REGION_RECORDS = 1u << 16;
 void launch_dedup(const DedupArgs& a, uint32_t W, bool packed, uint32_t grid, hipStream_t s);
 // Buckets in list[0..n): records copied unchanged with multiplicity 1.
 void launch_dedup_copy(const DedupArgs& a, const uint32_t* list, uint32_t n, uint32_t W, hipStream_t s);

@@ -151,7 +151,7 @@ struct cfgpu_handle {
     bool constructed = false;
 
     // record dedup: distinct records and multiplicities in each bucket's region
-    DevBuf dd_att, dd_lab, dd_mult, dd_cnt, dd_km, dd_end, dd_ovf, dd_ib, dd_ip, dd_iP, dd_list;
+    DevBuf dd_att, dd_lab, dd_mult, dd_cnt, dd_km, dd_end, dd_ovf, dd_ib, dd_ip, dd_iP, dd_list, dd_region;
     bool dd_active = false;
     std::vector<uint32_t> h_ddcnt;
     std::vector<uint64_t> h_ddkm;
@@ -277,7 +277,7 @@ void trim(cfgpu_handle* h, Phase ph) {
                           &h->cd_pos, &h->cd_hoff, &h->cd_out, &h->cd_rec})
             b->release();
     if (ph != Phase::Construct)
-        for (DevBuf* b : {&h->spill_key, &h->spill_meta, &h->dd_att, &h->dd_lab, &h->dd_mult}) b->release();
+        for (DevBuf* b : {&h->spill_key, &h->spill_meta, &h->dd_att, &h->dd_lab, &h->dd_mult, &h->dd_region}) b->release();
     if (ph == Phase::Cdbg) h->ct_scratch.release();
 }
 
@@ -693,14 +693,23 @@ constexpr double DEDUP_LOAD = 0.85;
 constexpr double DEDUP_MIN_GAIN = 1.1;
 constexpr size_t DEDUP_PROBES = 64;
 
-int dedup_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets, double distinct_frac) {
+// all_pass: buckets of 2 .. DD_ALL_PASS_MAX passes are one work item each (the
+// probe round, a few buckets, keeps one item per pass: more workgroups busy).
+int dedup_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets, double distinct_frac, bool all_pass) {
     const uint32_t max_len = std::min<uint32_t>(255, 2 * h->prm.k - h->prm.l);  // record length bound
     const bool packed = dedup_packed(h->W, max_len);
     const double C = double(dedup_capacity(dedup_key_words(h->W, max_len))) * DEDUP_LOAD;
     std::vector<uint32_t> ib, ip, iP;
+    bool any_all = false;
     for (uint32_t b : buckets) {
-        const double need = std::max(1.0, double(bucket_records(h, b)) * distinct_frac);
+        const uint64_t nr = bucket_records(h, b);
+        const double need = std::max(1.0, double(nr) * distinct_frac);
         const uint32_t P = uint32_t(std::min(4096.0, std::ceil(need / C)));
+        if (all_pass && P >= 2 && P <= DD_ALL_PASS_MAX && nr <= DD_REGION_RECORDS) {  // every pass in one workgroup
+            ib.push_back(b), ip.push_back(DD_ALL_PASSES), iP.push_back(P);
+            any_all = true;
+            continue;
+        }
         for (uint32_t q = 0; q < P; ++q) ib.push_back(b), ip.push_back(q), iP.push_back(P);
     }
     if (ib.empty()) return CFGPU_OK;
@@ -727,7 +736,12 @@ int dedup_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets, double di
     a.item_passes = h->dd_iP.as<uint32_t>();
     a.n_items = uint32_t(ib.size());
     a.queue_head = h->d_qhead.as<uint32_t>();
-    launch_dedup(a, h->W, packed, std::min<uint32_t>(a.n_items, uint32_t(h->cus)), h->stream);
+    const uint32_t grid = std::min<uint32_t>(a.n_items, uint32_t(h->cus));
+    if (any_all) {
+        HIPCK(h, ensure_mem(h, h->dd_region, size_t(grid) * DD_REGION_RECORDS * 5, Phase::Construct));
+        a.region = h->dd_region.as<uint8_t>();
+    }
+    launch_dedup(a, h->W, packed, grid, h->stream);
     HIPCK(h, hipGetLastError());
     return CFGPU_OK;
 }
@@ -763,7 +777,7 @@ int dedup_buckets(cfgpu_handle* h, const std::vector<uint32_t>& all) {
     } else {
         probe = all;
     }
-    CK(dedup_round(h, probe, 1.0));
+    CK(dedup_round(h, probe, 1.0, false));
     std::vector<uint32_t> cnt(G), ovf(G);
     HIPCK(h, d2h(h, cnt.data(), h->dd_cnt.p, G * 4));
     HIPCK(h, d2h(h, ovf.data(), h->dd_ovf.p, G * 4));
@@ -779,7 +793,7 @@ int dedup_buckets(cfgpu_handle* h, const std::vector<uint32_t>& all) {
         return CFGPU_OK;  // records barely repeat: walk them as they are
     }
     // margin < 1 under-sizes on purpose here too (exercises the copy fallback)
-    CK(dedup_round(h, rest, std::min(1.0, frac * 1.1) * std::min(1.0, double(h->tune.margin))));
+    CK(dedup_round(h, rest, std::min(1.0, frac * 1.1) * std::min(1.0, double(h->tune.margin)), true));
     HIPCK(h, d2h(h, ovf.data(), h->dd_ovf.p, G * 4));
     HIPCK(h, sync_stream(h));
     std::vector<uint32_t> list;

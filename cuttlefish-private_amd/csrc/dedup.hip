@@ -80,6 +80,48 @@ __device__ __forceinline__ int dd_find_or_insert(uint64_t* s_key, const uint64_t
     return -1;
 }
 
+// Key of a record (att, W label words): bases past len cleared; PK: label
+// words 1..W-1 then word 0 (the record's last bases, low bits free) packed as
+// the top word, else the W label words then a top word of their own.  The top
+// word also carries h (a function of the rest of the key), so that it alone
+// tells almost every pair of distinct keys apart; h is returned (its top
+// selects the pass, the fraction below it the group).
+template <int W, bool PK, int KR>
+__device__ __forceinline__ uint32_t dd_key(uint32_t at, const uint64_t (&lab)[W], uint64_t (&key)[KR]) {
+    const uint32_t len = at & 0xFFu;
+#pragma unroll
+    for (int q = 0; q < W; ++q) {  // word W-1-i holds bases 32i .. 32i+31
+        const int i = W - 1 - q;
+        const int nb = int(len) - 32 * i;
+        const uint64_t x = lab[q];
+        key[PK ? (q + W - 1) % W : q] = nb >= 32 ? x : (nb <= 0 ? 0ull : x & (~0ull << (64 - 2 * nb)));
+    }
+    uint32_t h;
+    if constexpr (PK) {
+        key[KR - 1] = (key[KR - 1] >> 1) | (at & 0x3FFu);  // bases in bits 33..62
+        h = pass_hash32<KR>(key);
+        key[KR - 1] |= uint64_t(h & ~0x3FFu);
+    } else {
+        key[KR - 1] = at & 0x3FFu;
+        h = pass_hash32<KR>(key);
+        key[KR - 1] |= uint64_t(h) << 10;
+    }
+    return h;
+}
+
+template <int W>
+__device__ __forceinline__ void dd_load(const DedupArgs& a, uint64_t r, uint32_t& at, uint64_t (&lab)[W]) {
+    at = a.att[r];
+#pragma unroll
+    for (int q = 0; q < W; ++q) lab[q] = a.labels[r * W + q];
+}
+
+// One work item is one hash pass of one bucket, or (item_pass ==
+// DD_ALL_PASSES) every pass of a bucket of at most DD_REGION_RECORDS records:
+// the workgroup then hashes each record once, lists the records of each pass
+// in its region of `region` (a pass byte and a u32 index per record, the
+// indices grouped by pass at exact offsets), and probes pass after pass over
+// its list with every lane busy.
 template <int W, bool PK>
 __global__ __launch_bounds__(DD_WG) void k_dedup(DedupArgs a) {
     constexpr int KR = PK ? W : W + 1;  // PK: len, disc bits and hash packed into the last label word
@@ -89,147 +131,154 @@ __global__ __launch_bounds__(DD_WG) void k_dedup(DedupArgs a) {
     __shared__ uint32_t s_cnt[C];
     __shared__ uint32_t s_item, s_n, s_ovf, s_base;
     __shared__ uint32_t s_len[256], s_wsum[4];  // per-len counts, then output positions
+    __shared__ uint32_t s_pc[DD_ALL_PASS_MAX], s_po[DD_ALL_PASS_MAX + 1];  // records per pass, their offsets
     __shared__ unsigned long long s_km;
     const int tid = threadIdx.x;
     constexpr uint32_t LIMIT = (uint32_t(C) * 7u) / 8u;
+    uint8_t* rq = a.region + uint64_t(blockIdx.x) * DD_REGION_RECORDS * 5;  // pass of each record
+    uint32_t* ridx = reinterpret_cast<uint32_t*>(rq + DD_REGION_RECORDS);   // records grouped by pass
 
     while (true) {
         __syncthreads();
         if (tid == 0) {
             s_item = atomicAdd(a.queue_head, 1u);
-            s_n = 0;
             s_ovf = 0;
         }
         __syncthreads();
         const uint32_t item = s_item;
         if (item >= a.n_items) break;
-        const uint32_t b = a.item_bucket[item], p = a.item_pass[item], P = a.item_passes[item];
+        const uint32_t b = a.item_bucket[item], p_it = a.item_pass[item], P = a.item_passes[item];
+        const bool all = p_it == DD_ALL_PASSES;
         if (tid == 0) s_ovf = __hip_atomic_load(&a.overflow[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        for (int i = tid; i < C * KR; i += DD_WG) s_key[i] = EMPTY;
-        for (int i = tid; i < C; i += DD_WG) s_cnt[i] = 0;
-        __syncthreads();
-        if (s_ovf) continue;
-
         const uint64_t r0 = a.bucket_off[b], r1 = a.bucket_off[b + 1];
-        uint32_t my_ins = 0, seen = 0;
-        // The next record is loaded while the current one is probed.
-        uint32_t at_n = 0;
-        uint64_t lab_n[W];
-        if (r0 + tid < r1) {
-            at_n = a.att[r0 + tid];
-#pragma unroll
-            for (int q = 0; q < W; ++q) lab_n[q] = a.labels[(r0 + tid) * W + q];
-        }
-        for (uint64_t r = r0 + tid; r < r1; r += DD_WG) {
-            const uint32_t at = at_n;
-            uint64_t lab[W];
-#pragma unroll
-            for (int q = 0; q < W; ++q) lab[q] = lab_n[q];
-            if (r + DD_WG < r1) {
-                at_n = a.att[r + DD_WG];
-#pragma unroll
-                for (int q = 0; q < W; ++q) lab_n[q] = a.labels[(r + DD_WG) * W + q];
+        const uint32_t n = uint32_t(r1 - r0);
+        if (all) {
+            // each record's pass (one hash per record), then its index at its
+            // pass's exact place
+            for (int i = tid; i < DD_ALL_PASS_MAX; i += DD_WG) s_pc[i] = 0;
+            __syncthreads();
+            for (uint32_t r = tid; r < n; r += DD_WG) {
+                uint32_t at;
+                uint64_t lab[W], key[KR];
+                dd_load<W>(a, r0 + r, at, lab);
+                const uint32_t q = uint32_t((uint64_t(dd_key<W, PK, KR>(at, lab, key)) * P) >> 32);
+                rq[r] = uint8_t(q);
+                atomicAdd(&s_pc[q], 1u);
             }
-            if ((++seen & 7u) == 0) {  // account inserts every 8 records
-                if (my_ins && atomicAdd(&s_n, my_ins) + my_ins > LIMIT) s_ovf = 1;
-                my_ins = 0;
-                if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+            __syncthreads();
+            if (tid == 0) {
+                uint32_t o = 0;
+                for (uint32_t q = 0; q < P; ++q) {
+                    s_po[q] = o;
+                    o += s_pc[q];
+                    s_pc[q] = s_po[q];  // now the cursor of pass q
+                }
+                s_po[P] = o;
             }
-            const uint32_t len = at & 0xFFu;
-            uint64_t m[W];
+            __syncthreads();
+            for (uint32_t r = tid; r < n; r += DD_WG) ridx[atomicAdd(&s_pc[rq[r]], 1u)] = r;
+        }
+        for (uint32_t p = all ? 0u : p_it; p <= (all ? P - 1 : p_it); ++p) {
+            if (tid == 0) s_n = 0;
+            for (int i = tid; i < C * KR; i += DD_WG) s_key[i] = EMPTY;
+            for (int i = tid; i < C; i += DD_WG) s_cnt[i] = 0;
+            __syncthreads();
+            if (s_ovf) break;
+            // one pass: the records of the pass's list (all), or every record
+            // of the bucket, keeping those of pass p; the next record is
+            // loaded while the current one is probed
+            const uint32_t e0 = all ? s_po[p] : 0u, e1 = all ? s_po[p + 1] : n;
+            uint32_t my_ins = 0, seen = 0;
+            uint32_t at_n = 0, ix_n = 0;
+            uint64_t lab_n[W];
+            if (e0 + tid < e1) {
+                const uint32_t r = all ? ridx[e0 + tid] : e0 + tid;
+                dd_load<W>(a, r0 + r, at_n, lab_n);
+                if (all && e0 + tid + DD_WG < e1) ix_n = ridx[e0 + tid + DD_WG];
+            }
+            for (uint32_t e = e0 + tid; e < e1; e += DD_WG) {
+                const uint32_t at = at_n;
+                uint64_t lab[W];
 #pragma unroll
-            for (int q = 0; q < W; ++q) {  // word W-1-i holds bases 32i .. 32i+31
-                const int i = W - 1 - q;
-                const int nb = int(len) - 32 * i;
-                const uint64_t x = lab[q];
-                m[q] = nb >= 32 ? x : (nb <= 0 ? 0ull : x & (~0ull << (64 - 2 * nb)));
+                for (int q = 0; q < W; ++q) lab[q] = lab_n[q];
+                if (e + DD_WG < e1) {
+                    const uint32_t r = all ? ix_n : e + DD_WG;
+                    dd_load<W>(a, r0 + r, at_n, lab_n);
+                    if (all && e + 2 * DD_WG < e1) ix_n = ridx[e + 2 * DD_WG];
+                }
+                if ((++seen & 7u) == 0) {  // account inserts every 8 records
+                    if (my_ins && atomicAdd(&s_n, my_ins) + my_ins > LIMIT) s_ovf = 1;
+                    my_ins = 0;
+                    if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+                }
+                uint64_t key[KR];
+                const uint64_t hp = uint64_t(dd_key<W, PK, KR>(at, lab, key)) * P;  // pass = top, group from the fraction
+                if (!all && uint32_t(hp >> 32) != p) continue;
+                bool inserted;
+                const int slot = dd_find_or_insert<KR, C>(
+                    s_key, key, uint32_t((uint64_t(uint32_t(hp)) * (uint32_t(C) / 4)) >> 32), inserted);
+                if (slot < 0) {
+                    s_ovf = 1;
+                    break;
+                }
+                atomicAdd(&s_cnt[slot], 1u);
+                my_ins += inserted;
             }
-            // key words: PK, label words 1..W-1 then word 0 (the record's last
-            // bases, low bits free) packed as the top word; otherwise the W
-            // label words then a top word of their own
-            uint64_t key[KR];
+            if (my_ins && atomicAdd(&s_n, my_ins) + my_ins > LIMIT) s_ovf = 1;
+            __syncthreads();
+            if (s_ovf) break;
+            // Append the pass's distinct records to the bucket's output region,
+            // longest first (a counting sort on len), so that a construct lane
+            // assignment that alternates direction over blocks of records gives
+            // every lane about the same number of k-mers to walk.
+            if (tid == 0) {
+                s_base = atomicAdd(&a.out_count[b], s_n);
+                s_km = 0;
+            }
+            if (tid < 256) s_len[tid] = 0;
+            __syncthreads();
+            for (int i = tid; i < C; i += DD_WG)
+                if (s_cnt[i]) atomicAdd(&s_len[s_key[i * KR + KR - 1] & 0xFFu], 1u);
+            __syncthreads();
+            // exclusive prefix over len = 255, 254, ..., 0 by the first 4 waves
+            const int lane = tid & 63, wv = tid >> 6;
+            uint32_t v = 0, x = 0;
+            if (tid < 256) {
+                v = x = s_len[255 - tid];
 #pragma unroll
-            for (int q = 0; q < W; ++q) key[PK ? (q + W - 1) % W : q] = m[q];
-            // The top word also carries h (a function of the rest of the key),
-            // so that it alone tells almost every pair of distinct keys apart.
-            uint32_t h;
-            if constexpr (PK) {
-                key[KR - 1] = (key[KR - 1] >> 1) | (at & 0x3FFu);  // bases in bits 33..62
-                h = pass_hash32<KR>(key);
-                key[KR - 1] |= uint64_t(h & ~0x3FFu);
-            } else {
-                key[KR - 1] = at & 0x3FFu;
-                h = pass_hash32<KR>(key);
-                key[KR - 1] |= uint64_t(h) << 10;
+                for (int d = 1; d < 64; d <<= 1) {
+                    const uint32_t u = __shfl_up(x, d);
+                    if (lane >= d) x += u;
+                }
+                if (lane == 63) s_wsum[wv] = x;
             }
-            const uint64_t hp = uint64_t(h) * P;  // pass = top, group from the fraction
-            if (uint32_t(hp >> 32) != p) continue;
-            bool inserted;
-            const int slot = dd_find_or_insert<KR, C>(s_key, key, uint32_t((uint64_t(uint32_t(hp)) * (uint32_t(C) / 4)) >> 32),
-                                                      inserted);
-            if (slot < 0) {
-                s_ovf = 1;
-                break;
+            __syncthreads();
+            if (tid < 256) {
+                uint32_t pre = 0;
+                for (int j = 0; j < wv; ++j) pre += s_wsum[j];
+                s_len[255 - tid] = pre + x - v;  // first output position of this len
             }
-            atomicAdd(&s_cnt[slot], 1u);
-            my_ins += inserted;
-        }
-        if (my_ins && atomicAdd(&s_n, my_ins) + my_ins > LIMIT) s_ovf = 1;
-        __syncthreads();
-        if (s_ovf) {
-            if (tid == 0) atomicOr(&a.overflow[b], 1u);
-            continue;
-        }
-        // Append the pass's distinct records to the bucket's output region,
-        // longest first (a counting sort on len), so that a construct lane
-        // assignment that alternates direction over blocks of records gives
-        // every lane about the same number of k-mers to walk.
-        if (tid == 0) {
-            s_base = atomicAdd(&a.out_count[b], s_n);
-            s_km = 0;
-        }
-        if (tid < 256) s_len[tid] = 0;
-        __syncthreads();
-        for (int i = tid; i < C; i += DD_WG)
-            if (s_cnt[i]) atomicAdd(&s_len[s_key[i * KR + KR - 1] & 0xFFu], 1u);
-        __syncthreads();
-        // exclusive prefix over len = 255, 254, ..., 0 by the first 4 waves
-        const int lane = tid & 63, wv = tid >> 6;
-        uint32_t v = 0, x = 0;
-        if (tid < 256) {
-            v = x = s_len[255 - tid];
+            __syncthreads();
+            unsigned long long my_km = 0;
+            const uint64_t base = r0 + s_base;
+            const uint32_t g16 = uint32_t(b) << 16;
+            for (int i = tid; i < C; i += DD_WG) {
+                const uint32_t c = s_cnt[i];
+                if (!c) continue;
+                const uint64_t* kp = &s_key[i * KR];
+                const uint64_t o = base + atomicAdd(&s_len[kp[KR - 1] & 0xFFu], 1u);
+                my_km += (kp[KR - 1] & 0xFFu) - (a.k - 1);
+                a.out_att[o] = uint32_t(kp[KR - 1] & 0x3FFu) | g16;
+                a.out_mult[o] = c;
 #pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const uint32_t u = __shfl_up(x, d);
-                if (lane >= d) x += u;
+                for (int q = 0; q < W; ++q)
+                    a.out_labels[o * W + q] = !PK ? kp[q] : q == 0 ? (kp[KR - 1] >> 32) << 33 : kp[q - 1];
             }
-            if (lane == 63) s_wsum[wv] = x;
+            if (my_km) atomicAdd(&s_km, my_km);
+            __syncthreads();
+            if (tid == 0 && s_km) atomicAdd(&a.out_kmers[b], s_km);
         }
-        __syncthreads();
-        if (tid < 256) {
-            uint32_t pre = 0;
-            for (int j = 0; j < wv; ++j) pre += s_wsum[j];
-            s_len[255 - tid] = pre + x - v;  // first output position of this len
-        }
-        __syncthreads();
-        unsigned long long my_km = 0;
-        const uint64_t base = r0 + s_base;
-        const uint32_t g16 = uint32_t(b) << 16;
-        for (int i = tid; i < C; i += DD_WG) {
-            const uint32_t c = s_cnt[i];
-            if (!c) continue;
-            const uint64_t* kp = &s_key[i * KR];
-            const uint64_t o = base + atomicAdd(&s_len[kp[KR - 1] & 0xFFu], 1u);
-            my_km += (kp[KR - 1] & 0xFFu) - (a.k - 1);
-            a.out_att[o] = uint32_t(kp[KR - 1] & 0x3FFu) | g16;
-            a.out_mult[o] = c;
-#pragma unroll
-            for (int q = 0; q < W; ++q)
-                a.out_labels[o * W + q] = !PK ? kp[q] : q == 0 ? (kp[KR - 1] >> 32) << 33 : kp[q - 1];
-        }
-        if (my_km) atomicAdd(&s_km, my_km);
-        __syncthreads();
-        if (tid == 0 && s_km) atomicAdd(&a.out_kmers[b], s_km);
+        if (s_ovf && tid == 0) atomicOr(&a.overflow[b], 1u);
     }
 }
 
