@@ -304,18 +304,35 @@ __global__ __launch_bounds__(256) void k_rank_init(CdbgArgs a) {
 // combination with the word of the state it points at.  Words are read and
 // written whole, so every word always names a state further along the chain
 // with the characters up to it, whatever other states have already jumped.
+// A state that is a chain end, or points at one, never changes again: its bit
+// in `settled` is set (one atomic per 32 states: a wave's lanes cover 64
+// consecutive states) and later passes skip it without reading the word it
+// points at.
 __global__ __launch_bounds__(256) void k_rank_round(CdbgArgs a, uint32_t* changed) {
     bool any = false;
+    const int lane = int(threadIdx.x & 63u);
     for (uint64_t st = uint64_t(blockIdx.x) * 256 + threadIdx.x; st < 2 * a.n; st += uint64_t(gridDim.x) * 256) {
-        const uint64_t w = __hip_atomic_load(&a.jp[st], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint64_t t = w & JT_MASK;
-        if (t == st) continue;  // chain end
-        const uint64_t w2 = __hip_atomic_load(&a.jp[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint64_t t2 = w2 & JT_MASK;
-        if (t2 == t) continue;  // points at a chain end already
-        const uint64_t c = (w >> JT_BITS) + (w2 >> JT_BITS);
-        __hip_atomic_store(&a.jp[st], jword(t2, c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        any |= c < a.total_chars;  // chains settle; cycles stop counting past every character
+        bool done = (a.settled[st >> 5] >> (st & 31)) & 1u;
+        bool now = false;
+        if (!done) {
+            const uint64_t w = __hip_atomic_load(&a.jp[st], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t t = w & JT_MASK;
+            if (t == st) {  // chain end
+                now = true;
+            } else {
+                const uint64_t w2 = __hip_atomic_load(&a.jp[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t t2 = w2 & JT_MASK;
+                if (t2 == t) {  // points at a chain end already
+                    now = true;
+                } else {
+                    const uint64_t c = (w >> JT_BITS) + (w2 >> JT_BITS);
+                    __hip_atomic_store(&a.jp[st], jword(t2, c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    any |= c < a.total_chars;  // chains settle; cycles stop counting past every character
+                }
+            }
+        }
+        const unsigned long long m = __ballot(now);
+        if ((lane == 0 || lane == 32) && uint32_t(m >> lane)) atomicOr(&a.settled[st >> 5], uint32_t(m >> lane));
     }
     if (any) atomicOr(changed, 1u);
 }

@@ -175,6 +175,7 @@ struct cfgpu_handle {
 
     // compacted graph: glue table, list-ranking states, output; cycles through
     // several subgraphs finished on the host (appended after the device part)
+    DevBuf cd_settled;
     DevBuf cd_partner, cd_bhist, cd_btot, cd_boff, cd_bkey, cd_bid, cd_kend, cd_jp, cd_head, cd_pos, cd_hoff, cd_out,
         cd_rec, cd_ctl, cd_cyc;
     uint64_t cd_n = 0, cd_bytes = 0;
@@ -273,7 +274,7 @@ void trim(cfgpu_handle* h, Phase ph) {
     for (size_t i = ph == Phase::Partition ? 0 : h->round_idx; i < h->pools.size(); ++i) h->pools[i]->release();
     if (ph == Phase::Partition) h->ct_scratch.release();
     if (ph != Phase::Cdbg)
-        for (DevBuf* b : {&h->cd_partner, &h->cd_bhist, &h->cd_bkey, &h->cd_bid, &h->cd_kend, &h->cd_jp, &h->cd_head,
+        for (DevBuf* b : {&h->cd_settled, &h->cd_partner, &h->cd_bhist, &h->cd_bkey, &h->cd_bid, &h->cd_kend, &h->cd_jp, &h->cd_head,
                           &h->cd_pos, &h->cd_hoff, &h->cd_out, &h->cd_rec})
             b->release();
     if (ph != Phase::Construct)
@@ -1638,6 +1639,9 @@ int cfgpu_cdbg(cfgpu_handle* h) {
         a.error = c + 3;
         a.n_cycle_pieces = c + 4;
         const uint32_t grid = uint32_t(std::min<uint64_t>((2 * n + 255) / 256, uint64_t(h->cus) * 8));
+        HIPCK(h, ensure_mem(h, h->cd_settled, ((2 * n + 63) / 64) * 8, Phase::Cdbg));  // whole words of 64 states
+        a.settled = h->cd_settled.as<uint32_t>();
+        launch_fill_u32(a.settled, 0u, ((2 * n + 63) / 64) * 2, h->stream);
         launch_cdbg_phase(a, KW, grid, 0, 0, nullptr, h->stream);
         launch_cdbg_phase(a, KW, grid, 1, 0, nullptr, h->stream);
         for (rounds = 0; rounds < 64;) {
