@@ -228,29 +228,37 @@ __device__ __forceinline__ int walk_next(const Tab& t, uint32_t cur, uint32_t m,
 // ASCII of base code b (A C G T).
 __device__ __forceinline__ uint32_t base_char(uint32_t b) { return (0x54474341u >> (8 * b)) & 0xFFu; }
 
-// z's k bases as ASCII at out: bytes up to the first 4-byte boundary, then
-// 4-byte stores, then the remaining bytes.
+// The 2-bit codes of bases i .. i+3 of z (k bases, Kmer layout: the k-mer
+// is the 2k-bit number w[KW-1] .. w[0], its first base highest), base i in
+// bits 7-6.  Words are chosen by selects (no indexed register array).
+template <int KW>
+__device__ __forceinline__ uint32_t kmer_codes4(const Kmer<KW>& z, uint32_t k, uint32_t i) {
+    const uint32_t p = 2 * (k - 4 - i);  // low bit of base i + 3
+    const uint32_t wi = p >> 6, sh = p & 63u;
+    uint64_t lo = z.w[0], hi = KW > 1 ? z.w[KW > 1 ? 1 : 0] : 0ull;
+#pragma unroll
+    for (int w = 1; w < KW; ++w)
+        if (uint32_t(w) == wi) lo = z.w[w], hi = w + 1 < KW ? z.w[w + 1 < KW ? w + 1 : w] : 0ull;
+    const uint64_t v = sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
+    return uint32_t(v) & 0xFFu;
+}
+
+// Four bases (codes in bits 7-6, 5-4, 3-2, 1-0) as four ASCII bytes, the
+// first base's in the lowest byte.
+__device__ __forceinline__ uint32_t chars4(uint32_t c) {
+    return base_char(c >> 6) | (base_char((c >> 4) & 3u) << 8) | (base_char((c >> 2) & 3u) << 16) |
+           (base_char(c & 3u) << 24);
+}
+
+// z's k bases as ASCII at out: single bytes up to the first 4-byte boundary,
+// then four bases per aligned 4-byte store, then the remaining bytes.
 template <int KW>
 __device__ __forceinline__ void put_kmer(char* out, const Kmer<KW>& z, uint32_t k) {
-    const uint32_t head = (4u - uint32_t(reinterpret_cast<uintptr_t>(out) & 3u)) & 3u;
-    uint32_t p = 0, acc = 0;
-#pragma unroll
-    for (int w = KW - 1; w >= 0; --w) {  // word KW-1 holds the first k - 32(KW-1) bases
-        const uint32_t nb = w == KW - 1 ? k - 32 * (KW - 1) : 32u;
-        const uint64_t v = z.w[w];
-        for (uint32_t q = 0; q < nb; ++q, ++p) {
-            const uint32_t c = base_char(uint32_t(v >> (2 * (nb - 1 - q))) & 3u);
-            if (p < head) {
-                out[p] = char(c);
-                continue;
-            }
-            const uint32_t at = (p - head) & 3u;
-            acc = at ? acc | (c << (8 * at)) : c;
-            if (at == 3) *reinterpret_cast<uint32_t*>(out + p - 3) = acc;
-        }
-    }
-    const uint32_t tail = (k - head) & 3u;  // k > head: k >= 15
-    for (uint32_t j = 0; j < tail; ++j) out[k - tail + j] = char(acc >> (8 * j));
+    const uint32_t head = (4u - uint32_t(reinterpret_cast<uintptr_t>(out) & 3u)) & 3u;  // k >= 15 > head
+    for (uint32_t i = 0; i < head; ++i) out[i] = char(base_char(kmer_codes4(z, k, i) >> 6));
+    uint32_t i = head;
+    for (; i + 4 <= k; i += 4) *reinterpret_cast<uint32_t*>(out + i) = chars4(kmer_codes4(z, k, i));
+    for (; i < k; ++i) out[i] = char(base_char((kmer_codes4(z, k, i - 3) & 3u)));
 }
 
 // Reads bucket b's construct table: vertices in no edge at the cutoff are
