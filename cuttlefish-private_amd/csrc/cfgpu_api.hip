@@ -264,10 +264,22 @@ namespace {
 // rounds hold no live table, and each later step makes the scratch of the
 // earlier ones dead.
 // Phase::Partition is the gather of appended chunks, after which every table,
-// and every later step's scratch, is dead too.
-enum class Phase { Partition, Construct, Contract, Cdbg };
+// and every later step's scratch, is dead too.  Phase::Input is a partition
+// of new input: every table, every later step's scratch and the unitigs of
+// the previous input are dead, and only the partition's own scratch and the
+// appended chunks' buckets are live.
+enum class Phase { Input, Partition, Construct, Contract, Cdbg };
 
 void trim(cfgpu_handle* h, Phase ph) {
+    if (ph == Phase::Input) {
+        for (auto& pool : h->pools) pool->release();
+        for (DevBuf* b : {&h->ex_keys, &h->ex_ef, &h->ex_st, &h->ex_dfa, &h->spill_key, &h->spill_meta, &h->dd_att,
+                          &h->dd_lab, &h->dd_mult, &h->dd_region, &h->ct_scratch, &h->ct_recs, &h->ct_labels,
+                          &h->cd_settled, &h->cd_partner, &h->cd_bhist, &h->cd_bkey, &h->cd_bid, &h->cd_kend,
+                          &h->cd_jp, &h->cd_head, &h->cd_pos, &h->cd_hoff, &h->cd_out, &h->cd_rec})
+            b->release();
+        return;
+    }
     for (DevBuf* b : {&h->gbin, &h->mbn, &h->tile_lc, &h->pdesc, &h->tile_n, &h->tile_frag, &h->hist, &h->rec1,
                       &h->ex_keys, &h->ex_ef, &h->ex_st, &h->ex_dfa})
         b->release();
@@ -1168,8 +1180,8 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     const uint64_t* d_packed = packed;
     const uint64_t* d_frag = frag_off;
     if (!dev) {
-        HIPCK(h, h->packed.ensure(n_words * 8));
-        HIPCK(h, h->frag.ensure((n_frag + 1) * 8));
+        HIPCK(h, ensure_mem(h, h->packed, n_words * 8, Phase::Input));
+        HIPCK(h, ensure_mem(h, h->frag, (n_frag + 1) * 8, Phase::Input));
         if (n_words > 2)
             HIPCK(h, hipMemcpyAsync(h->packed.p, packed, (n_words - 2) * 8, hipMemcpyHostToDevice, h->stream));
         HIPCK(h, hipMemsetAsync(h->packed.as<uint64_t>() + (n_words - 2), 0, 16, h->stream));
@@ -1178,13 +1190,13 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
         d_frag = h->frag.as<uint64_t>();
     }
     const PieceSpans ps = pieces_spans(n_bases, uint32_t(h->cus));
-    HIPCK(h, h->gbin.ensure(n_bases * 2));
-    HIPCK(h, h->mbn.ensure(((n_bases + 7) / 8) * 4));
-    HIPCK(h, h->tile_lc.ensure(tiles * 8));
-    HIPCK(h, h->pdesc.ensure(tiles * P_TILE * 8));
-    HIPCK(h, h->tile_n.ensure(tiles * 4));
-    HIPCK(h, h->tile_frag.ensure(tiles * 8));
-    HIPCK(h, h->hist.ensure(size_t(N_COARSE) * ps.n_count * 4));
+    HIPCK(h, ensure_mem(h, h->gbin, n_bases * 2, Phase::Input));
+    HIPCK(h, ensure_mem(h, h->mbn, ((n_bases + 7) / 8) * 4, Phase::Input));
+    HIPCK(h, ensure_mem(h, h->tile_lc, tiles * 8, Phase::Input));
+    HIPCK(h, ensure_mem(h, h->pdesc, tiles * P_TILE * 8, Phase::Input));
+    HIPCK(h, ensure_mem(h, h->tile_n, tiles * 4, Phase::Input));
+    HIPCK(h, ensure_mem(h, h->tile_frag, tiles * 8, Phase::Input));
+    HIPCK(h, ensure_mem(h, h->hist, size_t(N_COARSE) * ps.n_count * 4, Phase::Input));
     HIPCK(h, h->ctot.ensure(size_t(N_COARSE) * 8));
     HIPCK(h, h->coff.ensure(size_t(N_COARSE + 1) * 8));
     HIPCK(h, h->boff.ensure((G + 1) * 8));
@@ -1219,10 +1231,10 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     HIPCK(h, d2h(h, &n_rec, h->coff.as<uint64_t>() + N_COARSE, 8));
     HIPCK(h, sync_stream(h));
     const uint64_t lab_off = l1_label_offset(n_rec);
-    HIPCK(h, h->rec1.ensure((lab_off + n_rec * l1_label_words(h->W)) * 8));
-    HIPCK(h, h->att.ensure(n_rec * 4));
-    HIPCK(h, h->labels.ensure(n_rec * h->W * 8));
-    HIPCK(h, h->mbin.ensure(n_rec));
+    HIPCK(h, ensure_mem(h, h->rec1, (lab_off + n_rec * l1_label_words(h->W)) * 8, Phase::Input));
+    HIPCK(h, ensure_mem(h, h->att, n_rec * 4, Phase::Input));
+    HIPCK(h, ensure_mem(h, h->labels, n_rec * h->W * 8, Phase::Input));
+    HIPCK(h, ensure_mem(h, h->mbin, n_rec, Phase::Input));
     HIPCK(h, hipEventRecord(e3.e, h->stream));
     if (n_frag)
         launch_pieces_emit(a, h->pdesc.as<uint64_t>(), h->tile_n.as<uint32_t>(), ps, h->hist.as<uint32_t>(),
