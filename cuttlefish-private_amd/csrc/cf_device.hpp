@@ -276,4 +276,13 @@ __device__ __forceinline__ uint32_t edge_at(uint32_t e_f, unsigned off, uint32_t
     return c == 0 ? BASE_E : (c == 1 ? which : BASE_N);
 }
 
+// True when some counter of e_f is at least f_th (1 <= f_th <= 15): the
+// vertex is in an edge at the cutoff on one of its sides (edge_at of that side
+// is not E).  The even and the odd 4-bit counters as bytes v, each v + 128 -
+// f_th reaching bit 7 exactly when v >= f_th (no carry leaves a byte).
+__device__ __forceinline__ bool ef_active(uint32_t e_f, uint32_t f_th) {
+    const uint32_t add = (128u - f_th) * 0x01010101u;
+    return (((e_f & 0x0F0F0F0Fu) + add) | (((e_f >> 4) & 0x0F0F0F0Fu) + add)) & 0x80808080u;
+}
+
 }  // namespace cfgpu

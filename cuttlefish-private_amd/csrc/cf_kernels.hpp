@@ -111,7 +111,17 @@ struct ConstructArgs {
     uint64_t* spill_key;        // LDS kernel, all-pass items: per workgroup a region of
     uint32_t* spill_meta;       //   spill_cap entries (KW key words + one meta word each),
     uint32_t spill_cap;         //   P sub-lists of spill_cap / P entries for a P-pass item
+    uint32_t cutoff;            // LDS kernel: edge count threshold of the active / isolated split
+    uint64_t* seg;              // LDS kernel: per bucket SEG_MAX pass runs (start | active << 32)
+    uint32_t* iso;              // LDS kernel: per bucket isolated and phantom vertex counts
 };
+
+// The LDS kernel writes each pass's vertices as one run of the bucket's table,
+// the vertices in an edge at the cutoff (active) first; for buckets of at most
+// SEG_MAX passes it records each run's start and active count, and it counts
+// the bucket's isolated (and phantom: isolated and discontinuous) vertices, so
+// that contract reads only the active vertices.
+constexpr uint32_t SEG_MAX = 64;
 
 // Entry stride in bytes for KW key words: KW u64 key words, u32 e_f, u32 status.
 inline uint32_t entry_bytes(uint32_t KW) { return 8 * KW + 8; }
@@ -139,8 +149,8 @@ constexpr uint32_t LDS_ALL_PASS_MAX = 64;  // sub-list counters kept in LDS besi
 constexpr int lds_capacity(int KW) { return KW == 1 ? 12544 : KW == 2 ? 7744 : KW == 3 ? 5632 : 4416; }
 void launch_construct_lds(const ConstructArgs& a, uint32_t KW, uint32_t grid, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
-// n_vert[b] = overflow[b] = 0 for every b in list[0..n).
-void launch_zero_buckets(const uint32_t* list, uint32_t n, uint32_t* n_vert, uint32_t* overflow,
+// n_vert[b] = overflow[b] = 0 (and iso[2b] = iso[2b+1] = 0 when given) for every b in list[0..n).
+void launch_zero_buckets(const uint32_t* list, uint32_t n, uint32_t* n_vert, uint32_t* overflow, uint32_t* iso,
                          hipStream_t s);
 
 // Compact the occupied slots of bucket tables into key/e_f/status/dfa arrays.
@@ -243,6 +253,9 @@ struct ContractArgs {
     unsigned long long* cursors;   // [0] records, [1] label bytes (totals, even past capacity)
     uint32_t* retry;               // set when the output capacity was short
     uint32_t* error;               // set on an inconsistent table
+    const uint64_t* seg;           // per bucket SEG_MAX runs (start | active << 32) of the LDS construct
+    const uint32_t* nseg;          // per bucket runs recorded (0: scan the whole table)
+    const uint32_t* iso;           // per bucket isolated and phantom counts (buckets with runs)
 };
 // Three passes: LDS tables for every listed bucket that fits, dense tables
 // for the rest that fit those, then global scratch tables (the counts of the

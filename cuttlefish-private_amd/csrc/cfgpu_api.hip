@@ -129,6 +129,7 @@ struct cfgpu_handle {
     // construct state
     DevBuf d_tbl_ptr, d_cap, d_limit, d_nvert, d_overflow, d_items_b, d_items_f, d_items_c,
         d_qhead, d_round_list;
+    DevBuf d_seg, d_iso, d_nseg;  // LDS-built tables: pass runs, isolated / phantom counts (contract reads)
     std::vector<std::unique_ptr<DevBuf>> pools;  // one per sizing round, reused across calls
     uint32_t round_idx = 0;
     DevBuf spill_key, spill_meta;  // LDS construct: per-workgroup spill regions of all-pass items
@@ -146,6 +147,7 @@ struct cfgpu_handle {
     std::vector<uint64_t> h_tbl_ptr;
     std::vector<uint64_t> h_cap;
     std::vector<uint8_t> h_dense;             // 1: built in LDS, the table is its n_vert vertices
+    std::vector<uint32_t> h_passes;           // LDS-built tables: hash passes (runs of the table)
     std::vector<uint32_t> h_limit, h_nvert, h_flags;
     std::vector<float> h_pred;
     bool constructed = false;
@@ -359,6 +361,9 @@ ConstructArgs construct_args(cfgpu_handle* h) {
     a.item_first = h->d_items_f.as<uint32_t>();
     a.item_count = h->d_items_c.as<uint32_t>();
     a.queue_head = h->d_qhead.as<uint32_t>();
+    a.cutoff = h->prm.cutoff;
+    a.seg = h->d_seg.as<uint64_t>();
+    a.iso = h->d_iso.as<uint32_t>();
     if (h->dd_active) {
         a.att = h->dd_att.as<uint32_t>();
         a.labels = h->dd_lab.as<uint64_t>();
@@ -435,6 +440,7 @@ int prepare_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
         h->h_tbl_ptr[b] = reinterpret_cast<uint64_t>(pool->as<uint8_t>() + off * eb);
         h->h_cap[b] = caps[i];
         h->h_dense[b] = lds_cap ? 1 : 0;
+        h->h_passes[b] = lds_cap ? uint32_t(caps[i] / lds_cap) : 0u;
         h->h_limit[b] = limit_for(caps[i]);
         off += caps[i];
         if (lds_cap) {
@@ -485,7 +491,7 @@ int prepare_round(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
 // the sequence can be captured in a graph).
 void enqueue_round_reset(cfgpu_handle* h, uint32_t n_buckets, DevBuf* pool, uint64_t n_slots) {
     launch_zero_buckets(h->d_round_list.as<uint32_t>(), n_buckets, h->d_nvert.as<uint32_t>(),
-                        h->d_overflow.as<uint32_t>(), h->stream);
+                        h->d_overflow.as<uint32_t>(), h->d_iso.as<uint32_t>(), h->stream);
     launch_clear_tables(pool->as<uint8_t>(), n_slots, h->KW, h->stream);
 }
 
@@ -562,7 +568,7 @@ int run_round_batched(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
     auto reset = [&]() {
         launch_fill_u32(h->d_qhead.as<uint32_t>(), 0u, 1, h->stream);
         launch_zero_buckets(h->d_round_list.as<uint32_t>(), uint32_t(buckets.size()), h->d_nvert.as<uint32_t>(),
-                            h->d_overflow.as<uint32_t>(), h->stream);
+                            h->d_overflow.as<uint32_t>(), h->d_iso.as<uint32_t>(), h->stream);
         if (!lds) launch_clear_tables(pool->as<uint8_t>(), n_slots, h->KW, h->stream);  // LDS passes write every slot
     };
     auto kernel = [&]() {
@@ -586,7 +592,7 @@ int run_round_batched(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
     const void* ptrs[] = {pool->p, a.att, a.labels, a.mult, a.bucket_end, h->boff.p, h->d_tbl_ptr.p, h->d_cap.p,
                           h->d_limit.p, h->d_nvert.p, h->d_overflow.p, h->d_items_b.p,
                           h->d_items_f.p, h->d_items_c.p, h->d_qhead.p, h->d_round_list.p,
-                          a.spill_key, a.spill_meta};
+                          a.spill_key, a.spill_meta, a.seg, a.iso};
     std::vector<uint64_t> key = {n_slots, n_items, grid, h->KW, h->W, h->prm.k, h->tune.workgroup,
                                  buckets.size(), uint64_t(lds), a.spill_cap};
     for (const void* q : ptrs) key.push_back(reinterpret_cast<uint64_t>(q));
@@ -852,6 +858,7 @@ int construct_impl(cfgpu_handle* h, uint32_t mode, const uint32_t* subset = null
     h->h_tbl_ptr.assign(G, 0);
     h->h_cap.assign(G, 0);
     h->h_dense.assign(G, 0);
+    h->h_passes.assign(G, 0);
     h->h_limit.assign(G, 0);
     h->h_nvert.assign(G, 0);
     h->h_flags.assign(G, 0);
@@ -862,6 +869,8 @@ int construct_impl(cfgpu_handle* h, uint32_t mode, const uint32_t* subset = null
     HIPCK(h, h->d_nvert.ensure(G * 4));
     HIPCK(h, h->d_overflow.ensure(G * 4));
     HIPCK(h, h->d_qhead.ensure(4));
+    HIPCK(h, h->d_seg.ensure(G * SEG_MAX * 8));
+    HIPCK(h, h->d_iso.ensure(G * 8));
     launch_fill_u32(h->d_nvert.as<uint32_t>(), 0u, G, h->stream);
     launch_fill_u32(h->d_overflow.as<uint32_t>(), 0u, G, h->stream);
 
@@ -1511,6 +1520,12 @@ int cfgpu_contract(cfgpu_handle* h) {
     HIPCK(h, h->ct_counts.ensure(size_t(h->G) * sizeof(cfgpu_contract_stat)));
     HIPCK(h, h->ct_ctl.ensure(64));
     HIPCK(h, ensure_mem(h, h->ct_scratch, per_wg * std::max<uint32_t>(grid, 1), Phase::Contract));
+    // tables built in LDS passes (at most SEG_MAX) are read over their runs' active vertices
+    std::vector<uint32_t> nseg(h->G, 0u);
+    for (uint32_t g = 0; g < h->G; ++g)
+        if (h->h_tbl_ptr[g] && h->h_dense[g] && h->h_passes[g] <= SEG_MAX) nseg[g] = h->h_passes[g];
+    HIPCK(h, h->d_nseg.ensure(size_t(h->G) * 4));
+    HIPCK(h, h2d(h, h->d_nseg.p, nseg.data(), size_t(h->G) * 4));
     if (n_list) HIPCK(h, h2d(h, h->ct_list.p, list.data(), n_list * 4ull));
     if (n_big) HIPCK(h, h2d(h, h->ct_ovf.p, big.data(), n_big * 4ull));  // the dense pass's list starts with them
     // Output sized from the vertex count (most vertices of read sets are
@@ -1554,6 +1569,9 @@ int cfgpu_contract(cfgpu_handle* h) {
         a.rec_cap = rec_cap;
         a.labels = h->ct_labels.as<char>();
         a.lab_cap = lab_cap;
+        a.seg = h->d_seg.as<uint64_t>();
+        a.nseg = h->d_nseg.as<uint32_t>();
+        a.iso = h->d_iso.as<uint32_t>();
         launch_contract(a, KW, grid, grid, h->stream);
         HIPCK(h, hipGetLastError());
         HIPCK(h, hipMemcpyAsync(ctl, h->ct_ctl.p, sizeof ctl, hipMemcpyDeviceToHost, h->stream));

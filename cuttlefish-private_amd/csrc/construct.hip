@@ -218,11 +218,13 @@ __global__ __launch_bounds__(256) void k_clear(uint8_t* tables, uint64_t n) {
 }
 
 __global__ __launch_bounds__(256) void k_zero_buckets(const uint32_t* list, uint32_t n,
-                                                      uint32_t* n_vert, uint32_t* overflow) {
+                                                      uint32_t* n_vert, uint32_t* overflow, uint32_t* iso) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) {
-        n_vert[list[i]] = 0;
-        overflow[list[i]] = 0;
+        const uint32_t b = list[i];
+        n_vert[b] = 0;
+        overflow[b] = 0;
+        if (iso) iso[2 * b] = iso[2 * b + 1] = 0;
     }
 }
 
@@ -313,10 +315,10 @@ void launch_construct_persistent(const ConstructArgs& a, uint32_t KW, uint32_t g
     launch_any(a, KW, 0, grid, wg, 1, s);
 }
 
-void launch_zero_buckets(const uint32_t* list, uint32_t n, uint32_t* n_vert, uint32_t* overflow,
+void launch_zero_buckets(const uint32_t* list, uint32_t n, uint32_t* n_vert, uint32_t* overflow, uint32_t* iso,
                          hipStream_t s) {
     if (!n) return;
-    hipLaunchKernelGGL(k_zero_buckets, dim3((n + 255) / 256), dim3(256), 0, s, list, n, n_vert, overflow);
+    hipLaunchKernelGGL(k_zero_buckets, dim3((n + 255) / 256), dim3(256), 0, s, list, n, n_vert, overflow, iso);
 }
 
 void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {

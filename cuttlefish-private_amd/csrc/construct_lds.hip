@@ -41,11 +41,6 @@ constexpr int LWG = LDS_WG;
 // phase): 1 measured faster than 2 at 50M reads, k = 63 (92.5 vs 95.1 ms).
 constexpr uint32_t DEFER_TRIPS = 1;
 
-struct LdsProbe {
-    int slot;  // -1: every group is full; -2: max_trips group reads without a result
-    bool inserted;
-};
-
 // Pass and group choice of these tables: pass_hash32 of the key words.  Only
 // the spread of keys over passes and groups depends on it: keys are always
 // compared whole, and nothing looks a key up in a finished table by it.
@@ -54,104 +49,101 @@ __device__ __forceinline__ uint32_t pass_hash(const Kmer<KW>& x) {
     return pass_hash32<KW>(x.w);
 }
 
-// Find or insert `key` in an LDS table of C slots (KW u64 words each), probing
-// groups of 4 slots: the 4 top words are read together, and a group fills
-// from its first slot on (claims take the first empty slot; entries are never
-// removed), so a group with an empty slot ends an unsuccessful search.  The
-// group is scanned without branches (select chains, a candidate bit mask).
-// After max_trips group reads (re-reads included) without a result it returns
-// slot -2 with g the group to go on from: every group before g in the key's
-// probe order was full without the key, and stays so, so a later search from
-// g finds or inserts the key where a search from its home group would.
+constexpr int UPSERT_FULL = -1;      // every group is full without the key
+constexpr int UPSERT_DEFERRED = -2;  // max_trips group reads without a result
+
+// Add one k-mer instance to an LDS table of C slots (KW u64 words each): find
+// or insert `key`, add m to the counters of its front / back bases (saturating
+// at 15, Edge_Frequency::add_edge, include/State_Config.hpp:207-217) and OR
+// its discontinuity bits into the status byte.  Returns the slot, or
+// UPSERT_FULL / UPSERT_DEFERRED.
+//
+// Groups of 4 slots are probed: the 4 top words are read together, and a
+// group fills from its first slot on (claims take the first empty slot;
+// entries are never removed), so a group with an empty slot ends an
+// unsuccessful search.  The group is scanned without branches (select
+// chains, a candidate bit mask).  A slot is claimed by writing its top word
+// with BUSY set; its lower key words and its first counters are written, and
+// a release fence orders them before the plain top word publishes the slot.
+// A lane that finds the key reads its counters together with its lower key
+// words and updates them by CAS; a lane that meets a claimed slot with its own
+// top word re-reads the group.  After max_trips group reads (re-reads
+// included) without a result it returns UPSERT_DEFERRED with g the group to
+// go on from: every group before g in the key's probe order was full without
+// the key, and stays so, so a later search from g adds the instance where a
+// search from its home group would.
 template <int KW, int C>
-__device__ __forceinline__ LdsProbe lds_find_or_insert(uint64_t* s_key, const Kmer<KW>& key, uint32_t& g,
-                                                       uint32_t max_trips = ~0u) {
+__device__ __forceinline__ int lds_upsert(uint64_t* s_key, uint32_t* s_ef, uint32_t* s_st, const Kmer<KW>& key,
+                                          uint32_t& g, uint32_t front, uint32_t back, uint32_t disc, uint32_t m,
+                                          uint32_t& my_ins, uint32_t max_trips = ~0u) {
     constexpr uint32_t NG = uint32_t(C) / 4;
     const uint64_t top = key.w[KW - 1];
     for (uint32_t n = 0, trips = 0; n < NG;) {
-        if (trips++ == max_trips) return {-2, false};
+        if (trips++ == max_trips) return UPSERT_DEFERRED;
         uint64_t cur[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j)
             cur[j] = __hip_atomic_load(&s_key[(g * 4 + j) * KW + KW - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         const int empty = cur[0] == EMPTY ? 0 : cur[1] == EMPTY ? 1 : cur[2] == EMPTY ? 2 : cur[3] == EMPTY ? 3 : 4;
-        if constexpr (KW == 1) {
-            // an empty slot never equals a key (keys are < 2^62)
-            const int hit = cur[0] == top ? 0 : cur[1] == top ? 1 : cur[2] == top ? 2 : cur[3] == top ? 3 : -1;
-            if (hit >= 0) return {int(g * 4 + uint32_t(hit)), false};
-        } else {
-            uint32_t cand = 0;  // published slots whose top word matches
-            bool busy = false;  // a matching top word is still being published
+        uint32_t cand = 0;  // published slots whose top word matches
+        bool busy = false;  // a matching top word is still being published
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const bool c = j < empty && (cur[j] & ~BUSY) == top;
-                cand |= uint32_t(c && !(cur[j] & BUSY)) << j;
-                busy |= c && (cur[j] & BUSY);
-            }
-            while (cand) {
-                const uint32_t j = uint32_t(__builtin_ctz(cand));
-                cand &= cand - 1;
-                const uint64_t* kp = &s_key[(g * 4 + j) * KW];
-                bool eq = true;
-#pragma unroll
-                for (int w = 0; w < KW - 1; ++w)
-                    eq &= __hip_atomic_load(kp + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == key.w[w];
-                if (eq) return {int(g * 4 + j), false};
-            }
-            if (busy) continue;  // re-read the group once it is published
+        for (int j = 0; j < 4; ++j) {
+            const bool c = j < empty && (cur[j] & ~BUSY) == top;
+            cand |= uint32_t(c && !(cur[j] & BUSY)) << j;
+            busy |= c && (cur[j] & BUSY);
         }
-        if (empty == 4) {  // full group without the key: next group
+        int hit = -1;
+        uint32_t old = 0;
+        while (cand) {
+            const uint32_t slot = g * 4 + uint32_t(__builtin_ctz(cand));
+            cand &= cand - 1;
+            old = __hip_atomic_load(&s_ef[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            bool eq = true;
+#pragma unroll
+            for (int w = 0; w < KW - 1; ++w)
+                eq &= __hip_atomic_load(&s_key[slot * KW + w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) ==
+                      key.w[w];
+            if (eq) {
+                hit = int(slot);
+                break;
+            }
+        }
+        if (hit >= 0) {
+            // a stale counter word only costs a CAS retry
+            while (true) {
+                const uint32_t nw = ef_add(old, uint8_t(front), uint8_t(back), m);
+                if (nw == old) break;
+                const uint32_t prev = atomicCAS(&s_ef[hit], old, nw);
+                if (prev == old) break;
+                old = prev;
+            }
+            if (disc) atomicOr(&s_st[hit >> 2], disc << (8 * (hit & 3)));
+            return hit;
+        }
+        if (busy) continue;  // re-read the group once it is published
+        if (empty == 4) {    // full group without the key: next group
             g = g + 1 == NG ? 0 : g + 1;
             ++n;
             continue;
         }
         const uint32_t slot = g * 4 + uint32_t(empty);
         uint64_t* kp = &s_key[slot * KW];
-        const uint64_t claim = KW == 1 ? top : (top | BUSY);
-        const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(kp + KW - 1), EMPTY, claim);
-        if (old == EMPTY) {
-            if constexpr (KW > 1) {
+        if (atomicCAS(reinterpret_cast<unsigned long long*>(kp + KW - 1), EMPTY, top | BUSY) == EMPTY) {
 #pragma unroll
-                for (int w = 0; w < KW - 1; ++w)
-                    __hip_atomic_store(kp + w, key.w[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // LDS table: the workgroup is the only observer
-                __hip_atomic_store(kp + KW - 1, top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
-            return {int(slot), true};
+            for (int w = 0; w < KW - 1; ++w)
+                __hip_atomic_store(kp + w, key.w[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(&s_ef[slot], ef_add(0u, uint8_t(front), uint8_t(back), m), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // LDS table: the workgroup is the only observer
+            __hip_atomic_store(kp + KW - 1, top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (disc) atomicOr(&s_st[slot >> 2], disc << (8 * (slot & 3)));
+            ++my_ins;
+            return int(slot);
         }
-        if (KW == 1 && old == top) return {int(slot), false};
         // lost the slot to another key: re-read the group
     }
-    return {-1, false};
-}
-
-// Add one k-mer instance (counters of the front / back bases, m times;
-// discontinuity bits) to the slot pr found or inserted.
-__device__ __forceinline__ void lds_count(uint32_t* s_ef, uint32_t* s_st, const LdsProbe& pr, uint32_t front,
-                                          uint32_t back, uint32_t disc, uint32_t m, uint32_t& my_ins) {
-    // Saturating update (a fresh slot starts at 0; a stale value only costs
-    // a CAS retry).
-    uint32_t old = pr.inserted ? 0u : s_ef[pr.slot];
-    while (true) {
-        const uint32_t nw = ef_add(old, uint8_t(front), uint8_t(back), m);
-        if (nw == old) break;
-        const uint32_t prev = atomicCAS(&s_ef[pr.slot], old, nw);
-        if (prev == old) break;
-        old = prev;
-    }
-    if (disc) atomicOr(&s_st[pr.slot >> 2], disc << (8 * (pr.slot & 3)));
-    my_ins += pr.inserted;
-}
-
-// lds_find_or_insert + lds_count for one instance; false when every group is full.
-template <int KW, int C>
-__device__ __forceinline__ bool lds_add(uint64_t* s_key, uint32_t* s_ef, uint32_t* s_st, const Kmer<KW>& key,
-                                        uint32_t g0, uint32_t front, uint32_t back, uint32_t disc, uint32_t m,
-                                        uint32_t& my_ins) {
-    const LdsProbe pr = lds_find_or_insert<KW, C>(s_key, key, g0);
-    if (pr.slot < 0) return false;
-    lds_count(s_ef, s_st, pr, front, back, disc, m, my_ins);
-    return true;
+    return UPSERT_FULL;
 }
 
 // Spill region of a workgroup for one all-pass item: P sub-lists of c =
@@ -221,7 +213,8 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
     __shared__ uint64_t s_key[C * KW];
     __shared__ uint32_t s_ef[C];
     __shared__ uint32_t s_st[C / 4];  // one Vertex_Status byte per slot
-    __shared__ uint32_t s_item, s_cnt, s_ovf, s_base, s_wr, s_ndef;
+    __shared__ uint32_t s_item, s_cnt, s_ovf, s_base, s_wp, s_ndef;
+    __shared__ unsigned long long s_wai;  // write-out: active entries | isolated entries << 32
     __shared__ uint32_t s_sn[LDS_ALL_PASS_MAX];  // sub-list lengths of an all-pass item
 
     const int tid = threadIdx.x;
@@ -309,7 +302,9 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                                                 (g0 << 13),
                                             R, a.spill_cap, c_sub, s_sn, &s_ovf);
                     } else if (q == p) {
-                        if (!lds_add<KW, C>(s_key, s_ef, s_st, in.key, g0, in.front, in.back, in.disc, m, my_ins))
+                        uint32_t g = g0;
+                        if (lds_upsert<KW, C>(s_key, s_ef, s_st, in.key, g, in.front, in.back, in.disc, m, my_ins) ==
+                            UPSERT_FULL)
                             s_ovf = 1;
                     }
                     if (rw.i == rw.n) {  // record done: account inserts, start the prefetched record
@@ -373,27 +368,21 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                             meta_n = R.meta[e0 + e + LWG];
                         }
                         uint32_t g = meta >> 13;
-                        LdsProbe pr = lds_find_or_insert<KW, C>(s_key, key, g, max_trips);
-                        bool counted = true;
-                        if (pr.slot == -2) {
+                        const uint32_t fr = meta & 7u, bk = (meta >> 3) & 7u, ds = (meta >> 6) & 7u,
+                                       mm = (meta >> 9) & 15u;
+                        int r = lds_upsert<KW, C>(s_key, s_ef, s_st, key, g, fr, bk, ds, mm, my_ins, max_trips);
+                        if (r == UPSERT_DEFERRED) {
                             const uint32_t d = atomicAdd(&s_ndef, 1u);
                             if (d < room) {
                                 const uint64_t at = e0 + n_in + d;
 #pragma unroll
                                 for (int w = 0; w < KW; ++w) R.key[uint64_t(w) * a.spill_cap + at] = key.w[w];
                                 R.meta[at] = (meta & 0x1FFFu) | (g << 13);
-                                counted = false;
                             } else {
-                                pr = lds_find_or_insert<KW, C>(s_key, key, g);
+                                r = lds_upsert<KW, C>(s_key, s_ef, s_st, key, g, fr, bk, ds, mm, my_ins);
                             }
                         }
-                        if (counted) {
-                            if (pr.slot < 0)
-                                s_ovf = 1;
-                            else
-                                lds_count(s_ef, s_st, pr, meta & 7u, (meta >> 3) & 7u, (meta >> 6) & 7u,
-                                          (meta >> 9) & 15u, my_ins);
-                        }
+                        if (r == UPSERT_FULL) s_ovf = 1;
                         if ((++seen & 7u) == 0) {
                             flush();
                             if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
@@ -412,31 +401,54 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
             }
             // Append the pass's vertices (its occupied slots, s_cnt of them)
             // after those of the bucket's other passes: the bucket's table is
-            // its n_vert[b] vertices, dense, in no particular order.  Each wave
-            // reserves a run for its occupied slots with one LDS atomic.
+            // its n_vert[b] vertices, dense.  Within the pass's run the
+            // vertices in an edge at the cutoff (active) come first and the
+            // isolated ones fill it from its end; each wave reserves its
+            // entries of either kind with one LDS atomic.
             if (tid == 0) {
                 s_base = atomicAdd(&a.n_vert[b], s_cnt);
-                s_wr = 0;
+                s_wai = 0;
+                s_wp = 0;
             }
             __syncthreads();
+            const uint32_t n_run = s_cnt;
             typedef __attribute__((address_space(1))) uint64_t g_u64;
             g_u64* tbl = reinterpret_cast<g_u64*>(a.tbl_ptr[b]) + uint64_t(s_base) * (KW + 1);
             for (int i = tid; i < C; i += LWG) {  // C % 64 == 0: a wave's slots are all below C or none
                 const bool occ = s_key[i * KW + KW - 1] != EMPTY;
-                const unsigned long long mk = __ballot(occ);
-                uint32_t run = 0;
-                if (lane == 0 && mk) run = atomicAdd(&s_wr, uint32_t(__popcll(mk)));
-                run = uint32_t(__shfl(int(run), 0));
+                const uint32_t ef = s_ef[i];
+                const uint32_t st = (s_st[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+                const bool act = occ && ef_active(ef, a.cutoff);
+                const unsigned long long ma = __ballot(act), mi = __ballot(occ) & ~ma,
+                                         mp = __ballot(occ && !act && (st & 6u));
+                // one LDS atomic reserves the wave's active (low half) and isolated (high half) entries
+                unsigned long long rr = 0;
+                if (lane == 0) {
+                    const unsigned long long inc =
+                        (unsigned long long)__popcll(ma) | ((unsigned long long)__popcll(mi) << 32);
+                    if (inc) rr = atomicAdd(&s_wai, inc);
+                    if (mp) atomicAdd(&s_wp, uint32_t(__popcll(mp)));
+                }
+                const uint32_t ra = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(rr))));
+                const uint32_t ri = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(rr >> 32))));
                 if (!occ) continue;
-                const uint32_t pos =
-                    run + __builtin_amdgcn_mbcnt_hi(uint32_t(mk >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mk), 0u));
+                const unsigned long long mk = act ? ma : mi;
+                const uint32_t r = __builtin_amdgcn_mbcnt_hi(uint32_t(mk >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mk), 0u));
+                const uint32_t pos = act ? ra + r : n_run - 1 - (ri + r);
                 g_u64* e = tbl + uint64_t(pos) * (KW + 1);
 #pragma unroll
                 for (int w = 0; w < KW; ++w) e[w] = s_key[i * KW + w];
-                const uint32_t st = (s_st[i >> 2] >> (8 * (i & 3))) & 0xFFu;
-                e[KW] = uint64_t(s_ef[i]) | (uint64_t(st) << 32);
+                e[KW] = uint64_t(ef) | (uint64_t(st) << 32);
             }
             __syncthreads();  // table read out and counters used before the next pass clears them
+            if (tid == 0) {
+                const uint32_t n_act = uint32_t(s_wai), n_iso = uint32_t(s_wai >> 32);
+                if (a.seg && P <= SEG_MAX) a.seg[uint64_t(b) * SEG_MAX + p] = uint64_t(s_base) | (uint64_t(n_act) << 32);
+                if (a.iso) {
+                    if (n_iso) atomicAdd(&a.iso[2 * b], n_iso);
+                    if (s_wp) atomicAdd(&a.iso[2 * b + 1], s_wp);
+                }
+            }
         }
         if (s_ovf && tid == 0) atomicOr(&a.overflow[b], 1u);
     }

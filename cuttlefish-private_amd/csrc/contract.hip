@@ -265,20 +265,55 @@ __device__ __forceinline__ void put_kmer(char* out, const Kmer<KW>& z, uint32_t 
 // counted in s_cnt, the others inserted with their DFA byte (and their slot
 // appended to `list` when given).  Sets *s_ovf when more than `limit` do not
 // fit.  Several slots per thread are loaded before any is used, so each thread
-// keeps several HBM reads in flight.
+// keeps several HBM reads in flight.  A table whose pass runs were recorded by
+// the LDS construct (active vertices first in each run) is read only over the
+// runs' active parts, its isolated and phantom counts taken from the
+// construct's.  Every thread of the workgroup calls it (it synchronises).
 template <int KW, class Insert>
 __device__ __forceinline__ void load_bucket_with(const ContractArgs& a, uint32_t b, uint32_t limit, uint32_t* s_n,
                                                  uint32_t* s_ovf, uint32_t* s_cnt, Insert&& insert) {
     constexpr int U = KW <= 2 ? 8 : 4;
+    __shared__ uint32_t s_pre[SEG_MAX + 1];  // active entries before each run
+    __shared__ uint32_t s_beg[SEG_MAX];      // each run's first table entry
     const uint64_t* tbl = reinterpret_cast<const uint64_t*>(a.tbl_ptr[b]);
-    const uint64_t tcap = a.tbl_cap[b];
+    const uint32_t ns = a.nseg ? a.nseg[b] : 0u;
+    uint64_t n_idx = a.tbl_cap[b];
+    if (ns) {
+        if (threadIdx.x < 64) {  // one wave: the runs' active counts scanned
+            const uint32_t l = threadIdx.x;
+            const uint64_t sg = l < ns ? a.seg[uint64_t(b) * SEG_MAX + l] : 0ull;
+            uint32_t v = uint32_t(sg >> 32);
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t u = uint32_t(__shfl_up(int(v), d));
+                if (int(l) >= d) v += u;
+            }
+            s_beg[l] = uint32_t(sg);
+            s_pre[l + 1] = v;
+            if (l == 0) s_pre[0] = 0;
+        }
+        __syncthreads();
+        n_idx = s_pre[ns];
+    }
+    // table entry of the v-th entry read (the runs' active parts back to back)
+    const auto entry_of = [&](uint64_t v) -> uint64_t {
+        if (!ns) return v;
+        uint32_t lo = 0, hi = ns;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_pre[mid] <= uint32_t(v)) lo = mid;
+            else hi = mid;
+        }
+        return uint64_t(s_beg[lo]) + (v - s_pre[lo]);
+    };
     uint32_t iso = 0, ph = 0;
-    for (uint64_t s0 = threadIdx.x; s0 < tcap; s0 += uint64_t(U) * CT_WG) {
+    for (uint64_t s0 = threadIdx.x; s0 < n_idx; s0 += uint64_t(U) * CT_WG) {
         uint64_t e[U][KW + 1];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const uint64_t s = s0 + uint64_t(u) * CT_WG;
-            if (s < tcap) {
+            const uint64_t v = s0 + uint64_t(u) * CT_WG;
+            if (v < n_idx) {
+                const uint64_t s = entry_of(v);
 #pragma unroll
                 for (int w = 0; w <= KW; ++w) e[u][w] = tbl[s * (KW + 1) + w];
             } else {
@@ -308,6 +343,7 @@ __device__ __forceinline__ void load_bucket_with(const ContractArgs& a, uint32_t
         }
         if (stop || __hip_atomic_load(s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
     }
+    if (ns && threadIdx.x == 0) iso += a.iso[2 * b], ph += a.iso[2 * b + 1];
     if (iso) atomicAdd(&s_cnt[CT_ISOLATED], iso);
     if (ph) atomicAdd(&s_cnt[CT_PHANTOM], ph);
 }

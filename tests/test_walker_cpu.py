@@ -88,3 +88,12 @@ def test_walker_palindrome_fixture(harness):
     with open(os.path.join(HERE, "golden", "ref_data", "rob-palindrome.fa"), "rb") as f:
         seq, ro = O.parse(f.read(), fastq=False)
     assert compare(harness, seq, ro, 15, 7) == 1
+
+
+def test_ef_active_matches_edge_at(harness):
+    # the write-out's active test equals "edge_at is not E on some side" for
+    # every cutoff and every counter word built from boundary field values
+    out = subprocess.run([harness, "ef_active"], capture_output=True, check=True, text=True).stdout.split()
+    bad, n = int(out[0]), int(out[1])
+    assert n == 15 * 5 ** 8
+    assert bad == 0

@@ -12,6 +12,7 @@
 #include "../cuttlefish-private_amd/csrc/cf_instance.hpp"
 
 #include <array>
+#include <string>
 #include <cstdio>
 #include <map>
 #include <vector>
@@ -38,7 +39,28 @@ static void walk_all(uint32_t k, uint32_t W, const std::vector<uint32_t>& att,
     }
 }
 
-int main() {
+// ef_active (the active / isolated split of the construct's write-out)
+// against edge_at on both sides: every counter word whose eight fields are
+// drawn from {0, 1, f_th - 1, f_th, 15}, for every f_th.  Prints the
+// mismatches and the words checked.
+static int check_ef_active() {
+    uint64_t bad = 0, n = 0;
+    for (uint32_t f = 1; f <= 15; ++f) {
+        const uint32_t vals[5] = {0u, 1u, f - 1, f, 15u};
+        for (uint32_t c = 0; c < 390625u; ++c) {  // 5^8 words
+            uint32_t e = 0, x = c;
+            for (int i = 0; i < 8; ++i, x /= 5) e |= vals[x % 5] << (4 * i);
+            const bool want = edge_at(e, 0, f) != BASE_E || edge_at(e, 16, f) != BASE_E;
+            bad += ef_active(e, f) != want;
+            ++n;
+        }
+    }
+    printf("%llu %llu\n", (unsigned long long)bad, (unsigned long long)n);
+    return 0;
+}
+
+int main(int argc, char** argv) {
+    if (argc > 1 && std::string(argv[1]) == "ef_active") return check_ef_active();
     uint32_t k, W;
     uint64_t n;
     if (fread(&k, 4, 1, stdin) != 1 || fread(&W, 4, 1, stdin) != 1 || fread(&n, 8, 1, stdin) != 1) return 2;
