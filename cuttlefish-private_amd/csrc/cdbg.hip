@@ -372,7 +372,9 @@ __global__ __launch_bounds__(256) void k_place(CdbgArgs a) {
                 }
                 const uint32_t hd = uint32_t(fwd ? (E1 >> 1) : (E0 >> 1));
                 a.place_head[r] = hd;
-                a.place_pos[r] = (fwd ? c1 : c0) | (uint64_t(fwd ? 0u : 1u) << 63);
+                // the piece ends its chain when nothing follows it in that reading
+                const uint64_t last = (fwd ? c0 : c1) == 0 ? 1u : 0u;
+                a.place_pos[r] = (fwd ? c1 : c0) | (last << 62) | (uint64_t(fwd ? 0u : 1u) << 63);
                 head = fwd ? E1 == 2 * r + 1 : E0 == 2 * r;  // nothing before it in that reading
                 len = c0 + c1 + u.label_len;
             }
@@ -407,30 +409,59 @@ __global__ __launch_bounds__(256) void k_place(CdbgArgs a) {
     }
 }
 
-// Eight lanes per piece, each copying every eighth character.
+// Four characters at s (any alignment) as one little-endian word: two aligned
+// loads and a byte funnel shift (the label buffer has slack after its last
+// label, and s - (s & 3) is inside it).
+__device__ __forceinline__ uint32_t load4(const char* s) {
+    const uintptr_t ad = reinterpret_cast<uintptr_t>(s);
+    const uint32_t* q = reinterpret_cast<const uint32_t*>(ad & ~uintptr_t(3));
+    return __builtin_amdgcn_alignbyte(q[1], q[0], uint32_t(ad & 3u));  // ({q1, q0} >> 8 * (ad & 3))[31:0]
+}
+
+// Complement of four ASCII bases A C G T: C and G (bit 1 set) differ by
+// 0x04, A and T by 0x15.
+__device__ __forceinline__ uint32_t comp4(uint32_t v) {
+    return v ^ (0x15151515u ^ (((v >> 1) & 0x01010101u) * 0x11u));
+}
+
+// Eight lanes per piece.  Every aligned 4-byte word of the output that lies
+// inside a piece's range is written by that piece (neighbours overlap by k >=
+// 3 characters, so each aligned word inside a chain lies inside one of its
+// pieces, and two pieces writing one word write the same characters); the
+// bytes of the partial words at the chain's two ends are written one by one
+// by its first and its last piece.
 __global__ __launch_bounds__(256) void k_copy(CdbgArgs a) {
     const uint32_t j = threadIdx.x & 7u;
     for (uint64_t r = (uint64_t(blockIdx.x) * 256 + threadIdx.x) >> 3; r < a.n; r += (uint64_t(gridDim.x) * 256) >> 3) {
         const uint32_t hd = a.place_head[r];
         if (hd == NONE) continue;
         const UnitigRec u = a.recs[r];
-        const uint64_t pos = a.place_pos[r] & ~(1ull << 63);
-        const bool rev = a.place_pos[r] >> 63;
+        const uint64_t pp = a.place_pos[r];
+        const uint64_t pos = pp & ((1ull << 62) - 1);
+        const bool rev = pp >> 63, last = (pp >> 62) & 1u;
         const uint64_t hoff = a.head_off[hd];  // all-ones unless the head placed its chain
         if (hoff >= a.out_cap || pos >= a.out_cap - hoff || a.out_cap - hoff - pos < u.label_len) {
             if (j == 0) atomicOr(a.error, 4u);
             continue;
         }
         const char* src = a.labels + u.label_off;
-        char* dst = a.out + hoff + pos;
-        if (!rev) {
-            for (uint32_t i = j; i < u.label_len; i += 8) dst[i] = src[i];
-        } else {
-            for (uint32_t i = j; i < u.label_len; i += 8) {
-                const char c = src[u.label_len - 1 - i];
-                dst[i] = c == 'A' ? 'T' : c == 'C' ? 'G' : c == 'G' ? 'C' : 'A';
-            }
+        const uint32_t L = u.label_len;
+        const uint64_t d0 = hoff + pos, d1 = d0 + L;  // the piece's output bytes
+        const uint64_t w0 = (d0 + 3) & ~3ull, w1 = d1 & ~3ull;
+        for (uint64_t x = w0 + 4 * j; x < w1; x += 32) {
+            const uint32_t i = uint32_t(x - d0);  // output characters i .. i+3 of the piece
+            const uint32_t v = rev ? comp4(__builtin_bswap32(load4(src + (L - 4 - i)))) : load4(src + i);
+            *reinterpret_cast<uint32_t*>(a.out + x) = v;
         }
+        const auto put = [&](uint64_t x) {
+            const uint32_t i = uint32_t(x - d0);
+            const char c = rev ? src[L - 1 - i] : src[i];
+            a.out[x] = rev ? (c == 'A' ? 'T' : c == 'C' ? 'G' : c == 'G' ? 'C' : 'A') : c;
+        };
+        if (pos == 0)
+            for (uint64_t x = d0 + j; x < min(w0, d1); x += 8) put(x);
+        if (last)
+            for (uint64_t x = max(w1, d0) + j; x < d1; x += 8) put(x);
     }
 }
 
