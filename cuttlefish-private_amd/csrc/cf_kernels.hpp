@@ -89,6 +89,14 @@ void launch_merge_chunks(const ChunkSrc* src, uint32_t n_chunks, uint32_t G, uin
 // MLP feature vector (N_FEAT floats).
 void launch_bucket_features(const uint64_t* off, const uint32_t* att, const uint8_t* mbin,
                             uint32_t G, uint32_t k, uint64_t* stats, float* feat, hipStream_t s);
+// The same statistics and features for buckets that leave device memory chunk by
+// chunk: each chunk's per-bucket sums added into `sums` (SUM_WORDS u32 per bucket,
+// zeroed before the first chunk), then computed from the sums.
+constexpr uint32_t SUM_WORDS = 22;  // 16 histogram words, records, k-mer instances, bases (u64)
+void launch_bucket_sums(const uint64_t* off, const uint32_t* att, const uint8_t* mbin, uint32_t G, uint32_t k,
+                        uint32_t* sums, hipStream_t s);
+void launch_features_from_sums(const uint32_t* sums, uint32_t G, uint32_t k, uint64_t* stats, float* feat,
+                               hipStream_t s);
 
 // ---------------------------------------------------------------------------
 struct ConstructArgs {

@@ -13,9 +13,12 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <fcntl.h>
 #include <memory>
+#include <unistd.h>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "cf_kernels.hpp"
@@ -122,7 +125,28 @@ struct cfgpu_handle {
     };
     std::vector<std::unique_ptr<PartChunk>> chunks;
     DevBuf ch_src, ch_dst;  // gather arguments
+    // External bucket store (CFGPU_PARTITION_SPILL): per spilled chunk its
+    // records in bucket order (headers | labels | minimizer bins) in pinned
+    // host memory or in a file, and its bucket offsets; the per-bucket sums
+    // of every chunk (statistics and features) on the device.
+    struct SpillSeg {
+        uint64_t n_rec = 0, n_bases = 0, n_frag = 0;
+        std::vector<uint64_t> h_boff;
+        uint8_t* host = nullptr;
+        std::string path;
+        double t_partition = 0, t_minimizer = 0, t_count = 0, t_emit = 0, t_features = 0;
+    };
+    std::vector<SpillSeg> spill;
+    std::string spill_dir;
+    uint64_t spill_serial = 0;     // file names
+    DevBuf spill_sums;
+    bool spilled = false;          // the buckets live in the store
+    std::vector<uint8_t> resident; // spilled: buckets in device memory
+    uint8_t* spill_stage = nullptr;  // pinned staging of file segments and restores
+    size_t spill_stage_cap = 0;
+    double t_spill_ms = 0, t_restore_ms = 0;
     uint64_t n_bases = 0, n_frag = 0, n_records = 0;
+    uint64_t n_dev_records = 0;    // records in the device bucket arrays
     bool partitioned = false;
     std::vector<uint64_t> h_boff, h_stats;
 
@@ -199,6 +223,11 @@ struct cfgpu_handle {
     std::vector<uint32_t> na_list, na_cnt;
 
     ~cfgpu_handle() {
+        for (SpillSeg& sg : spill) {
+            if (sg.host) (void)hipHostFree(sg.host);
+            if (!sg.path.empty()) (void)unlink(sg.path.c_str());
+        }
+        if (spill_stage) (void)hipHostFree(spill_stage);
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (stream) (void)hipStreamDestroy(stream);
@@ -772,7 +801,7 @@ int dedup_buckets(cfgpu_handle* h, const std::vector<uint32_t>& all) {
     rs.t_dedup_ms = 0;
     if (!h->tune.dedup || all.empty()) return CFGPU_OK;
     const size_t G = h->G;
-    const uint64_t n = std::max<uint64_t>(h->n_records, 1);
+    const uint64_t n = std::max<uint64_t>(h->n_dev_records, 1);
     HIPCK(h, ensure_mem(h, h->dd_att, n * 4, Phase::Construct));
     HIPCK(h, ensure_mem(h, h->dd_lab, n * h->W * 8, Phase::Construct));
     HIPCK(h, ensure_mem(h, h->dd_mult, n * 4, Phase::Construct));
@@ -892,6 +921,10 @@ int construct_impl(cfgpu_handle* h, uint32_t mode, const uint32_t* subset = null
     } else {
         all = nonempty_buckets(h);
     }
+    if (h->spilled)
+        for (uint32_t b : all)
+            if (!h->resident[b])
+                return fail(h, CFGPU_E_STATE, "construct: bucket %u is in the bucket store (restore it first)", b);
     rs.n_buckets_nonempty = uint32_t(all.size());
     h->dd_active = false;
     rs.n_records_walked = h->n_records;
@@ -1098,6 +1131,166 @@ int construct_impl(cfgpu_handle* h, uint32_t mode, const uint32_t* subset = null
     return CFGPU_OK;
 }
 
+// ---------------------------------------------------------------------------
+// External bucket store.
+
+void spill_clear(cfgpu_handle* h) {
+    for (auto& sg : h->spill) {
+        if (sg.host) (void)hipHostFree(sg.host);
+        if (!sg.path.empty()) (void)unlink(sg.path.c_str());
+    }
+    h->spill.clear();
+    h->resident.clear();
+    h->t_spill_ms = h->t_restore_ms = 0;
+}
+
+hipError_t spill_stage_ensure(cfgpu_handle* h, size_t bytes) {
+    if (bytes <= h->spill_stage_cap) return hipSuccess;
+    if (h->spill_stage) (void)hipHostFree(h->spill_stage);
+    h->spill_stage = nullptr;
+    h->spill_stage_cap = 0;
+    const hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&h->spill_stage), bytes, hipHostMallocDefault);
+    if (e == hipSuccess) h->spill_stage_cap = bytes;
+    return e;
+}
+
+// Segment byte layout: headers (4 n) | labels (8 W n) | minimizer bins (n).
+uint64_t seg_bytes(const cfgpu_handle* h, uint64_t n) { return n * (4 + 8 * uint64_t(h->W) + 1); }
+
+bool write_all(int fd, const uint8_t* p, uint64_t n) {
+    while (n) {
+        const ssize_t w = write(fd, p, size_t(std::min<uint64_t>(n, 1ull << 30)));
+        if (w <= 0) return false;
+        p += w;
+        n -= uint64_t(w);
+    }
+    return true;
+}
+
+bool pread_all(int fd, uint8_t* p, uint64_t n, uint64_t off) {
+    while (n) {
+        const ssize_t r = pread(fd, p, size_t(std::min<uint64_t>(n, 1ull << 30)), off_t(off));
+        if (r <= 0) return false;
+        p += r;
+        n -= uint64_t(r);
+        off += uint64_t(r);
+    }
+    return true;
+}
+
+// The chunk just partitioned (its buckets in h->att / labels / mbin / boff,
+// offsets in h->h_boff) becomes a segment of the store; its per-bucket sums
+// are added to the store's.
+int spill_chunk(cfgpu_handle* h) {
+    const uint32_t G = h->G;
+    const uint64_t n = h->n_records, W = h->W;
+    const size_t sum_bytes = size_t(G) * SUM_WORDS * 4;
+    Event e0, e1;
+    HIPCK(h, hipEventRecord(e0.e, h->stream));
+    if (h->spill.empty()) {
+        HIPCK(h, h->spill_sums.ensure(sum_bytes));
+        launch_fill_u32(h->spill_sums.as<uint32_t>(), 0u, uint64_t(G) * SUM_WORDS, h->stream);
+    }
+    launch_bucket_sums(h->boff.as<uint64_t>(), h->att.as<uint32_t>(), h->mbin.as<uint8_t>(), G, h->prm.k,
+                       h->spill_sums.as<uint32_t>(), h->stream);
+    HIPCK(h, hipGetLastError());
+    cfgpu_handle::SpillSeg sg;
+    sg.n_rec = n;
+    sg.n_bases = h->n_bases;
+    sg.n_frag = h->n_frag;
+    sg.h_boff = h->h_boff;
+    const cfgpu_run_stats& rs = h->rs;
+    sg.t_partition = rs.t_partition_ms;
+    sg.t_minimizer = rs.t_minimizer_ms;
+    sg.t_count = rs.t_count_ms;
+    sg.t_emit = rs.t_emit_ms;
+    sg.t_features = rs.t_features_ms;
+    const uint64_t bytes = seg_bytes(h, n);
+    uint8_t* dst = nullptr;
+    if (h->spill_dir.empty()) {
+        HIPCK(h, hipHostMalloc(reinterpret_cast<void**>(&sg.host), std::max<uint64_t>(bytes, 1), hipHostMallocDefault));
+        dst = sg.host;
+    } else {
+        HIPCK(h, spill_stage_ensure(h, std::max<uint64_t>(bytes, 1)));
+        dst = h->spill_stage;
+    }
+    if (n) {
+        HIPCK(h, hipMemcpyAsync(dst, h->att.p, n * 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCK(h, hipMemcpyAsync(dst + n * 4, h->labels.p, n * W * 8, hipMemcpyDeviceToHost, h->stream));
+        HIPCK(h, hipMemcpyAsync(dst + n * (4 + 8 * W), h->mbin.p, n, hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCK(h, hipEventRecord(e1.e, h->stream));
+    HIPCK(h, sync_stream(h));
+    if (!h->spill_dir.empty()) {
+        char name[64];
+        std::snprintf(name, sizeof name, "/cfgpu_bucket_segment_%p_%llu.bin", static_cast<void*>(h),
+                      (unsigned long long)h->spill_serial++);
+        sg.path = h->spill_dir + name;
+        const int fd = open(sg.path.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0600);
+        if (fd < 0) return fail(h, CFGPU_E_ARG, "spill: cannot create %s", sg.path.c_str());
+        const bool ok = write_all(fd, dst, bytes);
+        const bool closed = close(fd) == 0;
+        if (!ok || !closed) {
+            (void)unlink(sg.path.c_str());
+            return fail(h, CFGPU_E_ARG, "spill: cannot write %s", sg.path.c_str());
+        }
+    }
+    h->t_spill_ms += elapsed_ms(e0, e1);
+    h->spill.push_back(std::move(sg));
+    h->rs.n_partition_chunks = uint32_t(h->spill.size());
+    return CFGPU_OK;
+}
+
+// Statistics and features of the store's buckets; none is in device memory.
+int spill_finish(cfgpu_handle* h) {
+    const uint32_t G = h->G;
+    HIPCK(h, h->stats.ensure(size_t(G) * 4 * 8));
+    HIPCK(h, h->feat.ensure(size_t(G) * N_FEAT * 4));
+    HIPCK(h, h->boff.ensure((G + 1) * 8));
+    launch_features_from_sums(h->spill_sums.as<uint32_t>(), G, h->prm.k, h->stats.as<uint64_t>(),
+                              h->feat.as<float>(), h->stream);
+    launch_fill_u32(h->boff.as<uint32_t>(), 0u, uint64_t(G + 1) * 2, h->stream);  // nothing resident
+    HIPCK(h, hipGetLastError());
+    h->h_stats.resize(size_t(G) * 4);
+    HIPCK(h, d2h(h, h->h_stats.data(), h->stats.p, size_t(G) * 32));
+    HIPCK(h, sync_stream(h));
+    // the device bucket arrays held the last chunk only
+    h->att.release();
+    h->labels.release();
+    h->mbin.release();
+    h->h_boff.assign(G + 1, 0);
+    h->resident.assign(G, 0);
+    h->n_dev_records = 0;
+    cfgpu_run_stats sum{};
+    uint64_t n = 0, nb = 0, nf = 0;
+    for (const auto& sg : h->spill) {
+        n += sg.n_rec;
+        nb += sg.n_bases;
+        nf += sg.n_frag;
+        sum.t_partition_ms += sg.t_partition;
+        sum.t_minimizer_ms += sg.t_minimizer;
+        sum.t_count_ms += sg.t_count;
+        sum.t_emit_ms += sg.t_emit;
+        sum.t_features_ms += sg.t_features;
+    }
+    h->n_bases = nb;
+    h->n_frag = nf;
+    h->n_records = n;
+    cfgpu_run_stats& rs = h->rs;
+    rs = sum;
+    rs.n_bases = nb;
+    rs.n_fragments = nf;
+    rs.n_records = n;
+    for (uint32_t b = 0; b < G; ++b) {
+        rs.n_kmers += h->h_stats[4 * b + 1];
+        rs.n_edges += h->h_stats[4 * b + 1] - h->h_stats[4 * b + 0];
+    }
+    rs.n_partition_chunks = uint32_t(h->spill.size());
+    h->spilled = true;
+    h->partitioned = true;
+    return CFGPU_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1165,10 +1358,15 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     HIPCK(h, hipSetDevice(h->prm.device));
     const uint32_t k = h->prm.k;
     if ((n_frag && !packed) || (!frag_off && n_frag) ||
-        (flags & ~uint32_t(CFGPU_INPUT_DEVICE | CFGPU_PARTITION_APPEND)))
+        (flags & ~uint32_t(CFGPU_INPUT_DEVICE | CFGPU_PARTITION_APPEND | CFGPU_PARTITION_SPILL)))
         return fail(h, CFGPU_E_ARG, "partition: bad pointers or flags");
     const bool dev = flags & CFGPU_INPUT_DEVICE;
-    const bool append = flags & CFGPU_PARTITION_APPEND;
+    const bool to_store = flags & CFGPU_PARTITION_SPILL;
+    const bool append = (flags & CFGPU_PARTITION_APPEND) || to_store;
+    if (to_store && !h->chunks.empty())
+        return fail(h, CFGPU_E_STATE, "partition: chunks kept in device memory are pending (finish them first)");
+    if (append && !to_store && !h->spill.empty() && !h->spilled)
+        return fail(h, CFGPU_E_STATE, "partition: spilled chunks are pending (finish them first)");
     if (!dev) {
         if (n_frag == 0 && n_bases != 0) return fail(h, CFGPU_E_ARG, "partition: bases without fragments");
         if (n_frag) {
@@ -1182,7 +1380,11 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     }
     h->partitioned = h->constructed = h->contracted = h->cdbg_done = false;
     h->mlp_trained = false;
-    if (!append) h->chunks.clear();
+    if (!append || h->spilled) {  // new input replaces a finished store
+        h->chunks.clear();
+        spill_clear(h);
+    }
+    h->spilled = false;
     const uint32_t G = h->G;
     const uint64_t n_words = (n_bases + 31) / 32 + 2;
     const uint64_t tiles = (n_bases + P_TILE - 1) / P_TILE;
@@ -1281,6 +1483,7 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
     rs.t_features_ms = elapsed_ms(e4, e5);
     rs.t_partition_ms = elapsed_ms(e0, e5);
     rs.n_partition_chunks = 1;
+    if (to_store) return spill_chunk(h);
     if (append) {
         // the chunk's buckets move aside; the next chunk allocates its own
         auto c = std::make_unique<cfgpu_handle::PartChunk>();
@@ -1302,11 +1505,16 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
         return CFGPU_OK;
     }
     h->partitioned = true;
+    h->n_dev_records = n_rec;
     return CFGPU_OK;
 }
 
 int cfgpu_partition_finish(cfgpu_handle* h) {
     if (!h) return CFGPU_E_ARG;
+    if (!h->spill.empty() && !h->spilled) {
+        HIPCK(h, hipSetDevice(h->prm.device));
+        return spill_finish(h);
+    }
     if (h->chunks.empty()) return fail(h, CFGPU_E_STATE, "partition_finish without appended chunks");
     HIPCK(h, hipSetDevice(h->prm.device));
     const uint32_t G = h->G, nc = uint32_t(h->chunks.size());
@@ -1376,6 +1584,137 @@ int cfgpu_partition_finish(cfgpu_handle* h) {
     rs.n_partition_chunks = nc;
     rs.t_partition_merge_ms = elapsed_ms(e0, e1);
     h->partitioned = true;
+    h->n_dev_records = n;
+    return CFGPU_OK;
+}
+
+int cfgpu_spill_dir(cfgpu_handle* h, const char* dir) {
+    if (!h) return CFGPU_E_ARG;
+    if (!h->spill.empty()) return fail(h, CFGPU_E_STATE, "spill_dir: the bucket store is not empty");
+    h->spill_dir = dir ? dir : "";
+    while (h->spill_dir.size() > 1 && h->spill_dir.back() == '/') h->spill_dir.pop_back();
+    if (!h->spill_dir.empty() && access(h->spill_dir.c_str(), W_OK) != 0)
+        return fail(h, CFGPU_E_ARG, "spill_dir: %s is not a writable directory", h->spill_dir.c_str());
+    return CFGPU_OK;
+}
+
+int cfgpu_buckets_restore(cfgpu_handle* h, const uint32_t* buckets, uint32_t n) {
+    if (!h || (n && !buckets)) return CFGPU_E_ARG;
+    if (!h->spilled) return fail(h, CFGPU_E_STATE, "restore: the handle's buckets are not in a bucket store");
+    const uint32_t G = h->G;
+    std::vector<uint8_t> want(G, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (buckets[i] >= G) return fail(h, CFGPU_E_ARG, "restore: bucket %u out of range", buckets[i]);
+        want[buckets[i]] = 1;
+    }
+    HIPCK(h, hipSetDevice(h->prm.device));
+    h->constructed = h->contracted = h->cdbg_done = false;
+    const uint64_t W = h->W;
+    // device layout: the listed buckets in bucket order, each one's records
+    // segment after segment (as cfgpu_partition_finish gathers kept chunks)
+    std::vector<uint64_t> boff(G + 1, 0);
+    for (uint32_t b = 0; b < G; ++b) boff[b + 1] = boff[b] + (want[b] ? h->h_stats[4 * b + 0] : 0);
+    const uint64_t tot = boff[G];
+    Event e0, e1;
+    HIPCK(h, hipEventRecord(e0.e, h->stream));
+    HIPCK(h, ensure_mem(h, h->att, std::max<uint64_t>(tot, 1) * 4, Phase::Input));
+    HIPCK(h, ensure_mem(h, h->labels, std::max<uint64_t>(tot, 1) * W * 8, Phase::Input));
+    HIPCK(h, ensure_mem(h, h->mbin, std::max<uint64_t>(tot, 1), Phase::Input));
+    HIPCK(h, spill_stage_ensure(h, std::max<uint64_t>(seg_bytes(h, tot), 1)));
+    uint8_t* const s_att = h->spill_stage;
+    uint8_t* const s_lab = s_att + tot * 4;
+    uint8_t* const s_mb = s_lab + tot * W * 8;
+    // Host threads gather bucket ranges (disjoint destinations); a file
+    // segment is read once per maximal run of listed buckets.
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::string> errs(nt);
+    const auto work = [&](unsigned t) {
+        const uint32_t b_lo = uint32_t(uint64_t(G) * t / nt), b_hi = uint32_t(uint64_t(G) * (t + 1) / nt);
+        std::vector<uint64_t> cur(boff.begin() + b_lo, boff.begin() + b_hi);  // next record of each bucket
+        std::vector<uint8_t> tmp;
+        for (const auto& sg : h->spill) {
+            const uint64_t m = sg.n_rec;
+            int fd = -1;
+            if (!sg.path.empty() && (fd = open(sg.path.c_str(), O_RDONLY)) < 0) {
+                errs[t] = "cannot open " + sg.path;
+                return;
+            }
+            for (uint32_t b1 = b_lo; b1 < b_hi;) {
+                if (!want[b1]) {
+                    ++b1;
+                    continue;
+                }
+                uint32_t b2 = b1;
+                while (b2 < b_hi && want[b2]) ++b2;
+                const uint64_t r_lo = sg.h_boff[b1], r_hi = sg.h_boff[b2], c = r_hi - r_lo;
+                const uint8_t *a_src, *l_src, *m_src;
+                if (fd >= 0) {
+                    tmp.resize(c * (4 + 8 * W + 1));
+                    if (c && (!pread_all(fd, tmp.data(), c * 4, r_lo * 4) ||
+                              !pread_all(fd, tmp.data() + c * 4, c * 8 * W, m * 4 + r_lo * 8 * W) ||
+                              !pread_all(fd, tmp.data() + c * (4 + 8 * W), c, m * (4 + 8 * W) + r_lo))) {
+                        errs[t] = "cannot read " + sg.path;
+                        close(fd);
+                        return;
+                    }
+                    a_src = tmp.data();
+                    l_src = tmp.data() + c * 4;
+                    m_src = tmp.data() + c * (4 + 8 * W);
+                } else {
+                    a_src = sg.host + r_lo * 4;
+                    l_src = sg.host + m * 4 + r_lo * 8 * W;
+                    m_src = sg.host + m * (4 + 8 * W) + r_lo;
+                }
+                for (uint32_t b = b1; b < b2; ++b) {
+                    const uint64_t q = sg.h_boff[b] - r_lo, cb = sg.h_boff[b + 1] - sg.h_boff[b];
+                    uint64_t& at = cur[b - b_lo];
+                    std::memcpy(s_att + at * 4, a_src + q * 4, cb * 4);
+                    std::memcpy(s_lab + at * 8 * W, l_src + q * 8 * W, cb * 8 * W);
+                    std::memcpy(s_mb + at, m_src + q, cb);
+                    at += cb;
+                }
+                b1 = b2;
+            }
+            if (fd >= 0) close(fd);
+        }
+    };
+    {
+        std::vector<std::thread> th;
+        for (unsigned t = 1; t < nt; ++t) th.emplace_back(work, t);
+        work(0);
+        for (auto& x : th) x.join();
+    }
+    for (const auto& e : errs)
+        if (!e.empty()) return fail(h, CFGPU_E_ARG, "restore: %s", e.c_str());
+    if (tot) {
+        HIPCK(h, hipMemcpyAsync(h->att.p, s_att, tot * 4, hipMemcpyHostToDevice, h->stream));
+        HIPCK(h, hipMemcpyAsync(h->labels.p, s_lab, tot * W * 8, hipMemcpyHostToDevice, h->stream));
+        HIPCK(h, hipMemcpyAsync(h->mbin.p, s_mb, tot, hipMemcpyHostToDevice, h->stream));
+    }
+    HIPCK(h, h2d(h, h->boff.p, boff.data(), (G + 1) * 8));
+    HIPCK(h, hipEventRecord(e1.e, h->stream));
+    HIPCK(h, sync_stream(h));
+    h->t_restore_ms = elapsed_ms(e0, e1);
+    h->h_boff = boff;
+    h->resident = want;
+    h->n_dev_records = tot;
+    return CFGPU_OK;
+}
+
+int cfgpu_spill_stats(cfgpu_handle* h, cfgpu_spill_stat* out) {
+    if (!h || !out) return CFGPU_E_ARG;
+    cfgpu_spill_stat s{};
+    s.n_segments = h->spill.size();
+    for (const auto& sg : h->spill) {
+        s.n_records += sg.n_rec;
+        s.bytes += seg_bytes(h, sg.n_rec);
+    }
+    s.in_files = !h->spill.empty() && !h->spill.front().path.empty();
+    for (uint8_t r : h->resident) s.n_resident += r;
+    s.n_resident_records = h->spilled ? h->n_dev_records : 0;
+    s.t_spill_ms = h->t_spill_ms;
+    s.t_restore_ms = h->t_restore_ms;
+    *out = s;
     return CFGPU_OK;
 }
 
@@ -1404,12 +1743,12 @@ int cfgpu_records_copy(cfgpu_handle* h, uint32_t* att, uint64_t* labels, uint64_
     if (!h) return CFGPU_E_ARG;
     if (!h->partitioned) return fail(h, CFGPU_E_STATE, "records_copy before partition");
     HIPCK(h, hipSetDevice(h->prm.device));
-    if (n_records) *n_records = h->n_records;
-    if (cap_records < h->n_records) return fail(h, CFGPU_E_ARG, "records_copy: capacity too small");
-    if (att && h->n_records)
-        HIPCK(h, hipMemcpyAsync(att, h->att.p, h->n_records * 4, hipMemcpyDeviceToHost, h->stream));
-    if (labels && h->n_records)
-        HIPCK(h, hipMemcpyAsync(labels, h->labels.p, h->n_records * h->W * 8, hipMemcpyDeviceToHost, h->stream));
+    const uint64_t nd = h->n_dev_records;  // spilled buckets: the restored ones
+    if (n_records) *n_records = nd;
+    if (cap_records < nd) return fail(h, CFGPU_E_ARG, "records_copy: capacity too small");
+    if (att && nd) HIPCK(h, hipMemcpyAsync(att, h->att.p, nd * 4, hipMemcpyDeviceToHost, h->stream));
+    if (labels && nd)
+        HIPCK(h, hipMemcpyAsync(labels, h->labels.p, nd * h->W * 8, hipMemcpyDeviceToHost, h->stream));
     if (bucket_off) std::memcpy(bucket_off, h->h_boff.data(), (h->G + 1) * 8);
     HIPCK(h, sync_stream(h));
     return CFGPU_OK;
@@ -1434,6 +1773,8 @@ int cfgpu_load_buckets(cfgpu_handle* h, const uint32_t* att, const uint64_t* lab
     }
     h->partitioned = h->constructed = h->contracted = h->cdbg_done = false;
     h->chunks.clear();
+    spill_clear(h);
+    h->spilled = false;
     const uint64_t n = bucket_off[G];
     HIPCK(h, h->att.ensure(n * 4));
     HIPCK(h, h->labels.ensure(n * h->W * 8));
@@ -1457,6 +1798,7 @@ int cfgpu_load_buckets(cfgpu_handle* h, const uint32_t* att, const uint64_t* lab
     h->n_bases = 0;
     h->n_frag = 0;
     h->n_records = n;
+    h->n_dev_records = n;
     cfgpu_run_stats& rs = h->rs;
     rs = cfgpu_run_stats{};
     rs.n_records = n;

@@ -810,7 +810,73 @@ __global__ __launch_bounds__(256) void k_bucket_features(const uint64_t* __restr
     }
 }
 
+// Per-bucket sums of one partition chunk added into sums (SUM_WORDS u32 words
+// per bucket: the 16-bin histogram of the 4 hash bits above g, then u64 record,
+// k-mer instance and base counts), for buckets that leave device memory
+// chunk by chunk; k_features_from_sums then gives what k_bucket_features gives
+// for the merged buckets.
+__global__ __launch_bounds__(256) void k_bucket_sums(const uint64_t* __restrict__ off,
+                                                     const uint32_t* __restrict__ att,
+                                                     const uint8_t* __restrict__ mbin, uint32_t k,
+                                                     uint32_t* __restrict__ sums) {
+    __shared__ uint32_t s_hist[16];
+    __shared__ unsigned long long s_km, s_bases;
+    const uint32_t b = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (tid < 16) s_hist[tid] = 0;
+    if (tid == 0) s_km = 0, s_bases = 0;
+    __syncthreads();
+    const uint64_t r0 = off[b], r1 = off[b + 1];
+    unsigned long long km = 0, bases = 0;
+    for (uint64_t r = r0 + tid; r < r1; r += 256) {
+        const uint32_t len = att[r] & 0xFFu;
+        km += len - (k - 1);
+        bases += len;
+        atomicAdd(&s_hist[mbin[r] & 15u], 1u);
+    }
+    atomicAdd(&s_km, km);
+    atomicAdd(&s_bases, bases);
+    __syncthreads();
+    uint32_t* sb = sums + size_t(b) * SUM_WORDS;
+    if (tid < 16 && s_hist[tid]) atomicAdd(&sb[tid], s_hist[tid]);
+    if (tid == 0 && r1 > r0) {
+        unsigned long long* s64 = reinterpret_cast<unsigned long long*>(sb + 16);
+        atomicAdd(&s64[0], (unsigned long long)(r1 - r0));
+        atomicAdd(&s64[1], s_km);
+        atomicAdd(&s64[2], s_bases);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_features_from_sums(const uint32_t* __restrict__ sums, uint32_t G,
+                                                            uint32_t k, uint64_t* __restrict__ stats,
+                                                            float* __restrict__ feat) {
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= G) return;
+    const uint32_t* sb = sums + size_t(b) * SUM_WORDS;
+    const uint64_t* s64 = reinterpret_cast<const uint64_t*>(sb + 16);
+    const uint64_t n = s64[0], km = s64[1], bases = s64[2];
+    stats[4 * b + 0] = n;
+    stats[4 * b + 1] = km;
+    stats[4 * b + 2] = bases;
+    stats[4 * b + 3] = 0;
+    float* f = feat + size_t(b) * N_FEAT;  // the expressions of k_bucket_features
+    for (int i = 0; i < 16; ++i) f[i] = log1pf(float(sb[i]));
+    f[16] = log1pf(float(n));
+    f[17] = log1pf(float(km));
+    f[18] = n ? float(bases) / float(n) / float(2 * k) : 0.f;
+}
+
 }  // namespace
+
+void launch_bucket_sums(const uint64_t* off, const uint32_t* att, const uint8_t* mbin, uint32_t G, uint32_t k,
+                        uint32_t* sums, hipStream_t s) {
+    hipLaunchKernelGGL(k_bucket_sums, dim3(G), dim3(256), 0, s, off, att, mbin, k, sums);
+}
+
+void launch_features_from_sums(const uint32_t* sums, uint32_t G, uint32_t k, uint64_t* stats, float* feat,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(k_features_from_sums, dim3((G + 255) / 256), dim3(256), 0, s, sums, G, k, stats, feat);
+}
 
 void launch_minimizer_g(const PartitionArgs& a, uint16_t* gbin, uint32_t* mbn, uint64_t* tile_lc, hipStream_t s) {
     const uint64_t tiles = (a.n_bases + P_TILE - 1) / P_TILE;

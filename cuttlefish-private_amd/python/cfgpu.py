@@ -100,6 +100,16 @@ class SizingModel(ctypes.Structure):
                [("lr", ctypes.c_float), ("loss", ctypes.c_float)]
 
 
+class SpillStat(ctypes.Structure):
+    _fields_ = [("n_segments", ctypes.c_uint64), ("n_records", ctypes.c_uint64), ("bytes", ctypes.c_uint64),
+                ("in_files", ctypes.c_uint32), ("n_resident", ctypes.c_uint32),
+                ("n_resident_records", ctypes.c_uint64), ("t_spill_ms", ctypes.c_double),
+                ("t_restore_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class IngestStats(ctypes.Structure):
     _fields_ = ([(n, ctypes.c_uint64) for n in ("n_bytes", "n_records", "n_bases", "n_fragments")]
                 + [("n_chunks", ctypes.c_uint32), ("threads", ctypes.c_uint32)]
@@ -124,6 +134,9 @@ SIGNATURES = {
     "cfgpu_partition": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
                                        ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32]),
     "cfgpu_partition_finish": (ctypes.c_int, [ctypes.c_void_p]),
+    "cfgpu_spill_dir": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "cfgpu_buckets_restore": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_uint32]),
+    "cfgpu_spill_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SpillStat)]),
     "cfgpu_bucket_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BucketStat),
                                           ctypes.c_uint32]),
     "cfgpu_records_copy": (ctypes.c_int, [ctypes.c_void_p, u32p, u64p, u64p, ctypes.c_uint64,
@@ -403,12 +416,28 @@ class Handle:
     def __exit__(self, *a):
         self.close()
 
-    def partition(self, inp: Input, append: bool = False):
+    def partition(self, inp: Input, append: bool = False, spill: bool = False):
+        """append: keep the chunk's buckets aside; spill: move them to the bucket store."""
+        flags = (2 if append else 0) | (4 if spill else 0)
         self._check(lib().cfgpu_partition(self._h, inp.packed.ctypes.data, inp.n_bases,
-                                          inp.frag_off.ctypes.data, inp.n_frag, 2 if append else 0), "partition")
+                                          inp.frag_off.ctypes.data, inp.n_frag, flags), "partition")
 
     def partition_finish(self):
         self._check(lib().cfgpu_partition_finish(self._h), "partition_finish")
+
+    def spill_dir(self, path):
+        """Bucket store location: None for pinned host memory, else a directory."""
+        self._check(lib().cfgpu_spill_dir(self._h, None if path is None else str(path).encode()), "spill_dir")
+
+    def restore(self, buckets):
+        """cfgpu_buckets_restore: the listed buckets into device memory."""
+        b = np.ascontiguousarray(buckets, dtype=np.uint32)
+        self._check(lib().cfgpu_buckets_restore(self._h, _p(b, u32p) if b.size else None, len(b)), "restore")
+
+    def spill_stats(self):
+        st = SpillStat()
+        self._check(lib().cfgpu_spill_stats(self._h, ctypes.byref(st)), "spill_stats")
+        return st.as_dict()
 
     def ingest_buffer(self, buf: bytes, fastq: bool, threads: int = 0, chunk_bytes: int = 0):
         """cfhost_partition_buffer: chunked parse + pack overlapped with the device partition."""

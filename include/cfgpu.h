@@ -76,7 +76,7 @@ int cfgpu_last_error(const cfgpu_handle* h, char* buf, size_t n);
  * (include/Subgraphs_Manager.hpp:93, 113; src/Subgraphs_Manager.cpp:84-95): the weak
  * super k-mers are left in 16384 device-resident buckets in the reference record
  * format (Super_Kmer_Attributes<false> + MSB-aligned label). */
-enum { CFGPU_INPUT_DEVICE = 1, CFGPU_PARTITION_APPEND = 2 };
+enum { CFGPU_INPUT_DEVICE = 1, CFGPU_PARTITION_APPEND = 2, CFGPU_PARTITION_SPILL = 4 };
 int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
                     const uint64_t* frag_off, uint64_t n_frag, uint32_t flags);
 /* Input in chunks: with CFGPU_PARTITION_APPEND, cfgpu_partition partitions one chunk of
@@ -88,6 +88,39 @@ int cfgpu_partition(cfgpu_handle* h, const uint64_t* packed, uint64_t n_bases,
  * 121-184).  Until the finish, the handle holds no buckets; a partition without APPEND
  * or a cfgpu_load_buckets drops the appended chunks. */
 int cfgpu_partition_finish(cfgpu_handle* h);
+
+/* External-memory buckets, for inputs whose buckets do not fit device memory.  With
+ * CFGPU_PARTITION_SPILL (appending, like CFGPU_PARTITION_APPEND), cfgpu_partition
+ * partitions one chunk and moves its buckets out of device memory into the handle's
+ * bucket store: one segment per chunk (its records in bucket order and the bucket
+ * offsets), in pinned host memory or, after cfgpu_spill_dir, in one file per segment.
+ * This replaces the reference's Super_Kmer_Bucket files, to which every partitioner
+ * worker flushes its per-bucket chunks (src/Super_Kmer_Bucket.cpp:27-40,
+ * include/Super_Kmer_Chunk.hpp:281-351; a segment holds every bucket's chunk of one
+ * input chunk, uncompressed: the labels are already 2-bit packed).  cfgpu_partition_finish
+ * then computes the bucket statistics and features from the segments without gathering
+ * them; the handle's buckets live in the store and none is in device memory.  A
+ * partition without SPILL or APPEND, a cfgpu_load_buckets or cfgpu_close empties the
+ * store (and removes its files).  Segments of spilled and kept chunks are not mixed. */
+int cfgpu_spill_dir(cfgpu_handle* h, const char* dir);  /* NULL or "": pinned host memory */
+/* Loads the listed buckets' records from the store into device memory, replacing the
+ * buckets there (every other bucket is then empty in device memory, and the tables,
+ * unitigs and compacted graph of the previous buckets are dropped): the reader side of
+ * the reference's bucket files (Super_Kmer_Bucket::Iterator::read_chunk,
+ * src/Super_Kmer_Bucket.cpp:85-95).  cfgpu_construct_buckets then builds any of them;
+ * a construct of a bucket of the store that is not loaded is CFGPU_E_STATE. */
+int cfgpu_buckets_restore(cfgpu_handle* h, const uint32_t* buckets, uint32_t n);
+typedef struct cfgpu_spill_stat {
+    uint64_t n_segments;    /* chunks in the store */
+    uint64_t n_records;     /* records in the store */
+    uint64_t bytes;         /* bytes of the store (headers, labels, minimizer bins) */
+    uint32_t in_files;      /* 1: the segments are files under the spill directory */
+    uint32_t n_resident;    /* buckets in device memory (the last restore's) */
+    uint64_t n_resident_records;
+    double t_spill_ms;      /* device-to-store time summed over the segments */
+    double t_restore_ms;    /* the last restore: store to device */
+} cfgpu_spill_stat;
+int cfgpu_spill_stats(cfgpu_handle* h, cfgpu_spill_stat* out);
 
 typedef struct cfgpu_bucket_stat {
     uint64_t n_records;   /* weak super k-mers in the bucket */
