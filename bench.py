@@ -42,7 +42,25 @@ WORKLOADS = {
     "c2": dict(genome=50_000_000, repeat_frac=0.0, divergence=0.0, reads=10_000_000, read_len=150,
                sub_rate=0.005, k=31, l=20, cutoff=2, seed=2026,
                traffic="partition_construct_reads10M_k31_hbm_traffic.txt", roofline_kernel="k_construct_lds<1, 2>"),
+    # reference genomes (one FASTA record each), not reads
+    "c3": dict(kind="genome", genome=500_000_000, repeat_frac=0.3, divergence=0.02, k=31, l=20, cutoff=1, seed=2026,
+               traffic="construct_genome500M_k31_hbm_traffic.txt", roofline_kernel="k_construct_lds<1, 2>"),
+    "c5": dict(kind="pangenome", genome=5_000_000, copies=64, snp=0.005, indel=0.01, indel_max=10, k=127, l=20,
+               cutoff=1, seed=2026, traffic="construct_pangenome64x5M_k127_hbm_traffic.txt",
+               roofline_kernel="k_construct_lds<4, 8>"),
 }
+
+
+def describe(name, w):
+    if w.get("kind") == "genome":
+        return (f"{name}: one {w['genome']} bp genome ({w['repeat_frac'] * 100:.0f}% repeats in families of "
+                f"300 bp - 6 kb, {w['divergence'] * 100:.0f}% diverged copies), one FASTA record")
+    if w.get("kind") == "pangenome":
+        return (f"{name}: {w['copies']} genomes derived from one {w['genome']} bp base genome "
+                f"({w['snp'] * 100:.1f}% SNPs, {w['indel'] * 100:.0f}% indels of 1-{w['indel_max']} bp), "
+                "one FASTA record each")
+    return (f"{name}: {w['reads']} x {w['read_len']} bp reads from a {w['genome']} bp genome "
+            f"({w['repeat_frac'] * 100:.0f}% repeats, both strands, {w['sub_rate'] * 100:.1f}% substitutions)")
 
 
 def measured_traffic(path, kernel):
@@ -138,11 +156,24 @@ def algorithmic_bytes(k, l, n_sk, n_e, n_km):
 
 
 def make_workload(w):
+    """(sequence bytes, record offsets, packed input) of a workload."""
     t = time.time()
-    g = C.synth_genome(w["genome"], w["repeat_frac"], 300, 6000, w["divergence"], seed=w["seed"])
-    reads = C.synth_reads(g, w["reads"], w["read_len"], w["sub_rate"], seed=w["seed"] + 1)
-    del g
-    rec_off = np.arange(w["reads"] + 1, dtype=np.uint64) * np.uint64(w["read_len"])
+    kind = w.get("kind", "reads")
+    if kind == "genome":
+        reads = C.synth_genome(w["genome"], w["repeat_frac"], 300, 6000, w["divergence"], seed=w["seed"])
+        rec_off = np.array([0, len(reads)], np.uint64)
+    elif kind == "pangenome":
+        base = C.synth_genome(w["genome"], 0.0, 300, 6000, 0.0, seed=w["seed"])
+        recs = [C.synth_mutate(base, w["snp"], w["indel"], w["indel_max"], seed=w["seed"] + 1 + i)
+                for i in range(w["copies"])]
+        reads = b"".join(recs)
+        rec_off = np.zeros(len(recs) + 1, np.uint64)
+        rec_off[1:] = np.cumsum([len(r) for r in recs])
+    else:
+        g = C.synth_genome(w["genome"], w["repeat_frac"], 300, 6000, w["divergence"], seed=w["seed"])
+        reads = C.synth_reads(g, w["reads"], w["read_len"], w["sub_rate"], seed=w["seed"] + 1)
+        del g
+        rec_off = np.arange(w["reads"] + 1, dtype=np.uint64) * np.uint64(w["read_len"])
     inp = C.Input.from_records(reads, rec_off, w["k"])
     log(f"workload ready in {time.time() - t:.1f}s: {inp.n_bases} bases in {inp.n_frag} fragments")
     return reads, rec_off, inp
@@ -199,6 +230,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ingest", action="store_true")
+    ap.add_argument("--external-batches", type=int, default=4,
+                    help="after the ingest: the FASTQ text partitioned into the bucket store (pinned host "
+                         "memory) and the graph built in this many restored bucket batches (0: skip)")
     ap.add_argument("--ingest-chunk-mib", type=int, default=1024,
                     help="chunk size of the pipelined FASTQ ingest (parse + pack overlapped with the partition)")
     ap.add_argument("--max-passes", type=int, default=-1,
@@ -329,7 +363,9 @@ def main():
         learned_construct = (time.perf_counter() - t0) * 1e3 / args.compare_steps
 
     ingest = None
-    if not args.no_ingest:
+    external = None
+    cst_unitigs = rs["n_unitigs"]
+    if not args.no_ingest and w.get("kind", "reads") == "reads":
         # the workload's reads as FASTQ text in host memory (4-line records),
         # to buckets: parse + pack everything, then one partition from pageable
         # memory; and the pipeline (chunks of whole records, parse + pack of the
@@ -375,6 +411,49 @@ def main():
                   "fastq_to_unitigs_kmers_per_s": n_km / ((t_pipe + rest) / 1e3),
                   "note": "FASTQ text already in host memory (file reading not included); the step after the "
                           "partition as in `value`"}
+        if args.external_batches > 0:
+            # external-memory buckets: every chunk's buckets leave HBM for the
+            # bucket store; then per batch of buckets: restore -> construct ->
+            # contract -> unitigs to the host
+            log(f"external memory: ingest into the bucket store, {args.external_batches} batches")
+            hip.sync()
+            t0 = time.perf_counter()
+            sts = h.ingest_buffer(text, True, threads=16, chunk_bytes=args.ingest_chunk_mib << 20, spill=True)
+            t1 = time.perf_counter()
+            sp = h.spill_stats()
+            nz = np.nonzero(h.bucket_stats()["n_records"])[0].astype(np.uint32)
+            tb = {"restore": 0.0, "construct": 0.0, "contract": 0.0, "unitigs_to_host": 0.0}
+            n_u = n_ub = 0
+            for bt in np.array_split(nz, args.external_batches):
+                a0 = time.perf_counter()
+                h.restore(bt)
+                a1 = time.perf_counter()
+                h.construct_buckets(bt, "learned")
+                hip.sync()
+                a2 = time.perf_counter()
+                h.contract()
+                hip.sync()
+                a3 = time.perf_counter()
+                recs, nu, buf, nb = h.unitigs_raw()
+                a4 = time.perf_counter()
+                tb["restore"] += (a1 - a0) * 1e3
+                tb["construct"] += (a2 - a1) * 1e3
+                tb["contract"] += (a3 - a2) * 1e3
+                tb["unitigs_to_host"] += (a4 - a3) * 1e3
+                n_u += nu
+                n_ub += nb
+                del recs, buf
+            t2 = time.perf_counter()
+            assert n_u == cst_unitigs, "batches and the one-pass construct disagree on the unitig count"
+            external = {"store": "pinned host memory", "segments": sp["n_segments"], "store_bytes": sp["bytes"],
+                        "batches": args.external_batches, "ingest_into_store_ms": (t1 - t0) * 1e3,
+                        "device_to_store_ms": sp["t_spill_ms"], "batch_ms": tb,
+                        "fastq_to_unitigs_ms": (t2 - t0) * 1e3,
+                        "fastq_to_unitigs_kmers_per_s": n_km / (t2 - t0),
+                        "unitigs": n_u, "unitig_bytes": n_ub,
+                        "note": "cfhost_partition_buffer_ex with CFGPU_PARTITION_SPILL, then per batch "
+                                "cfgpu_buckets_restore -> cfgpu_construct_buckets -> cfgpu_contract -> "
+                                "cfgpu_unitigs_copy (the per-subgraph unitigs; the host glues the batches)"}
         del text
 
     cpu = None
@@ -383,12 +462,23 @@ def main():
         import oracle_lib as O
         # the GPU box grants one GPU's share of its host cores: 16
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        nr = args.cpu_sample_reads or min(w["reads"], 2_000_000)
-        sample = reads[: nr * w["read_len"]]
-        log(f"cpu baseline: {nr} reads on {threads} threads, median of 3")
+        if w.get("kind", "reads") == "reads":
+            nr = args.cpu_sample_reads or min(w["reads"], 2_000_000)
+            s_off = rec_off[: nr + 1]
+            what = f"first {nr} of the {w['reads']} reads"
+        elif w["kind"] == "genome":  # a prefix of the record (the oracle partitions a record on one thread)
+            nr = 1
+            s_off = np.array([0, min(len(reads), 20_000_000)], np.uint64)
+            what = f"the first {int(s_off[1])} bases of the genome as one record"
+        else:
+            nr = min(len(rec_off) - 1, 16)
+            s_off = rec_off[: nr + 1]
+            what = f"the first {nr} of the {len(rec_off) - 1} genomes"
+        sample = reads[: int(s_off[-1])]
+        log(f"cpu baseline: {what} on {threads} threads, median of 3")
         runs = []
         for _ in range(3):
-            r = O.baseline(sample, rec_off[: nr + 1], k, l, cutoff, threads, threads)
+            r = O.baseline(sample, s_off, k, l, cutoff, threads, threads)
             runs.append((r["partition_ms"] + r["bucketize_ms"] + r["construct_ms"], r))
         runs.sort(key=lambda x: x[0])
         t_pc, r = runs[1]
@@ -396,18 +486,19 @@ def main():
         # the reference partitions one input file per worker: a single read
         # file on one thread (the rest of the pipeline on every worker)
         nf = max(1, nr // 4)
-        log(f"cpu baseline, one partition thread: {nf} reads")
-        rf = O.baseline(reads[: nf * w["read_len"]], rec_off[: nf + 1], k, l, cutoff, 1, threads)
+        log(f"cpu baseline, one partition thread: {nf} records")
+        f_off = s_off[: nf + 1] if nr > 1 else np.array([0, int(s_off[-1]) // 4], np.uint64)
+        rf = O.baseline(reads[: int(f_off[-1])], f_off, k, l, cutoff, 1, threads)
         tf = rf["partition_ms"] + rf["bucketize_ms"] + rf["construct_ms"]
         cpu = {"value": r["n_kmers"] / (t_pc / 1000.0), "unit": "k-mers/s", "cores": threads, "kind": "port",
                "cpu_model": cpu_model(),
-               "sample": f"first {nr} of the {w['reads']} reads ({r['n_kmers']} k-mer instances), median of 3: "
+               "sample": f"{what} ({r['n_kmers']} k-mer instances), median of 3: "
                          "the oracle's partition on every worker + bucketize + per-subgraph construct with a reusable "
                          "open-addressing map, largest buckets first; contract and gluing not in `value`",
                "phases_ms": {"partition": r["partition_ms"], "bucketize": r["bucketize_ms"],
                              "construct": r["construct_ms"], "contract": r["contract_ms"]},
                "to_unitigs_kmers_per_s": r["n_kmers"] / (t_all / 1000.0),
-               "per_file_partition": {"reads": nf, "kmers_per_s": rf["n_kmers"] / (tf / 1000.0),
+               "per_file_partition": {"records": nf, "kmers_per_s": rf["n_kmers"] / (tf / 1000.0),
                                       "partition_ms": rf["partition_ms"],
                                       "note": "partition on one thread, as the reference does for one input file "
                                               "(src/Graph_Partitioner.cpp:121-184); construct on every worker"}}
@@ -434,10 +525,7 @@ def main():
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {w['reads']} x {w['read_len']} bp reads from a {w['genome']} bp "
-                               f"genome ({w['repeat_frac'] * 100:.0f}% repeats, both strands, "
-                               f"{w['sub_rate'] * 100:.1f}% substitutions), k={k}, l={l}, cutoff={cutoff}, "
-                               "16384 subgraphs",
+        "config": {"workload": f"{describe(args.workload, w)}, k={k}, l={l}, cutoff={cutoff}, 16384 subgraphs",
                    "n_kmers": n_km, "n_records": rs["n_records"], "n_vertices": rs["n_vertices"],
                    "maximal_unitigs": n_cdbg, "unitig_bytes": cdbg_bytes},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -468,6 +556,7 @@ def main():
                       "fallback": ph["t_fallback_ms"], "contract": ph["t_contract_ms"], "cdbg": ph["t_cdbg_ms"],
                       "copy_to_host": wall["copy_to_host"], "wall": wall},
         "ingest": ingest,
+        "external_memory": external,
         "paths": {
             "vertex_states": {"ms": part_ms + learned_construct,
                               "kmers_per_s": n_km / ((part_ms + learned_construct) / 1000.0),

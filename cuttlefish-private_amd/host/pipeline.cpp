@@ -86,9 +86,9 @@ void pack_chunk(const char* buf, uint64_t n, int is_fastq, uint64_t b, uint64_t 
 
 extern "C" {
 
-int cfhost_partition_buffer(cfgpu_handle* h, const char* buf, uint64_t n, int is_fastq, uint32_t threads,
-                            uint64_t chunk_bytes, cfhost_ingest_stats* st) {
-    if (!h || (!buf && n)) return -1;
+int cfhost_partition_buffer_ex(cfgpu_handle* h, const char* buf, uint64_t n, int is_fastq, uint32_t threads,
+                               uint64_t chunk_bytes, uint32_t flags, cfhost_ingest_stats* st) {
+    if (!h || (!buf && n) || (flags & ~uint32_t(CFGPU_PARTITION_SPILL))) return -1;
     cfgpu_params prm{};
     if (cfgpu_params_get(h, &prm) != CFGPU_OK) return -1;
     const uint32_t T = pick_threads(threads);
@@ -106,7 +106,8 @@ int cfhost_partition_buffer(cfgpu_handle* h, const char* buf, uint64_t n, int is
         std::thread th;
         if (more) th = std::thread([&]() { pack_chunk(buf, n, is_fastq, cur.end, chunk_bytes, prm.k, T, nxt); });
         const double d0 = now_ms();
-        const int rc = cfgpu_partition(h, cur.packed.p, cur.n_bases, cur.frag.p, cur.n_frag, CFGPU_PARTITION_APPEND);
+        const int rc =
+            cfgpu_partition(h, cur.packed.p, cur.n_bases, cur.frag.p, cur.n_frag, CFGPU_PARTITION_APPEND | flags);
         t_dev += now_ms() - d0;
         if (th.joinable()) th.join();
         if (rc != CFGPU_OK) return rc;
@@ -136,8 +137,13 @@ int cfhost_partition_buffer(cfgpu_handle* h, const char* buf, uint64_t n, int is
     return 0;
 }
 
-int cfhost_partition_file(cfgpu_handle* h, const char* path, int is_fastq, uint32_t threads, uint64_t chunk_bytes,
-                          cfhost_ingest_stats* st) {
+int cfhost_partition_buffer(cfgpu_handle* h, const char* buf, uint64_t n, int is_fastq, uint32_t threads,
+                            uint64_t chunk_bytes, cfhost_ingest_stats* st) {
+    return cfhost_partition_buffer_ex(h, buf, n, is_fastq, threads, chunk_bytes, 0, st);
+}
+
+int cfhost_partition_file_ex(cfgpu_handle* h, const char* path, int is_fastq, uint32_t threads, uint64_t chunk_bytes,
+                             uint32_t flags, cfhost_ingest_stats* st) {
     if (!h || !path) return -1;
     const int fd = open(path, O_RDONLY);
     if (fd < 0) return -2;
@@ -159,10 +165,15 @@ int cfhost_partition_file(cfgpu_handle* h, const char* path, int is_fastq, uint3
         (void)madvise(map, n, MADV_SEQUENTIAL);
         buf = static_cast<const char*>(map);
     }
-    const int rc = cfhost_partition_buffer(h, buf, n, is_fastq, threads, chunk_bytes, st);
+    const int rc = cfhost_partition_buffer_ex(h, buf, n, is_fastq, threads, chunk_bytes, flags, st);
     if (map) munmap(map, n);
     close(fd);
     return rc;
+}
+
+int cfhost_partition_file(cfgpu_handle* h, const char* path, int is_fastq, uint32_t threads, uint64_t chunk_bytes,
+                          cfhost_ingest_stats* st) {
+    return cfhost_partition_file_ex(h, path, is_fastq, threads, chunk_bytes, 0, st);
 }
 
 }  // extern "C"

@@ -193,6 +193,11 @@ SIGNATURES = {
                                                ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(IngestStats)]),
     "cfhost_partition_file": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_uint32,
                                              ctypes.c_uint64, ctypes.POINTER(IngestStats)]),
+    "cfhost_partition_buffer_ex": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_int,
+                                                  ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
+                                                  ctypes.POINTER(IngestStats)]),
+    "cfhost_partition_file_ex": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_uint32,
+                                                ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(IngestStats)]),
     "cfhost_synth_genome": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_double, ctypes.c_uint32,
                                            ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64,
                                            ctypes.c_void_p]),
@@ -439,10 +444,12 @@ class Handle:
         self._check(lib().cfgpu_spill_stats(self._h, ctypes.byref(st)), "spill_stats")
         return st.as_dict()
 
-    def ingest_buffer(self, buf: bytes, fastq: bool, threads: int = 0, chunk_bytes: int = 0):
-        """cfhost_partition_buffer: chunked parse + pack overlapped with the device partition."""
+    def ingest_buffer(self, buf: bytes, fastq: bool, threads: int = 0, chunk_bytes: int = 0, spill: bool = False):
+        """cfhost_partition_buffer_ex: chunked parse + pack overlapped with the device partition
+        (spill: every chunk's buckets to the bucket store)."""
         st = IngestStats()
-        r = lib().cfhost_partition_buffer(self._h, buf, len(buf), int(fastq), threads, chunk_bytes, ctypes.byref(st))
+        r = lib().cfhost_partition_buffer_ex(self._h, buf, len(buf), int(fastq), threads, chunk_bytes,
+                                             4 if spill else 0, ctypes.byref(st))
         self._check(r, "ingest_buffer")
         return st.as_dict()
 

@@ -70,6 +70,14 @@ int cfhost_partition_buffer(cfgpu_handle* h, const char* buf, uint64_t n, int is
                             uint64_t chunk_bytes, cfhost_ingest_stats* stats);
 int cfhost_partition_file(cfgpu_handle* h, const char* path, int is_fastq, uint32_t threads, uint64_t chunk_bytes,
                           cfhost_ingest_stats* stats);
+/* The same with partition flags for the chunks: CFGPU_PARTITION_SPILL sends every
+ * chunk's buckets to the handle's bucket store (cfgpu_spill_dir), so that inputs whose
+ * buckets exceed device memory are partitioned as the reference partitions into its
+ * bucket files; 0 is cfhost_partition_buffer / _file. */
+int cfhost_partition_buffer_ex(cfgpu_handle* h, const char* buf, uint64_t n, int is_fastq, uint32_t threads,
+                               uint64_t chunk_bytes, uint32_t flags, cfhost_ingest_stats* stats);
+int cfhost_partition_file_ex(cfgpu_handle* h, const char* path, int is_fastq, uint32_t threads, uint64_t chunk_bytes,
+                             uint32_t flags, cfhost_ingest_stats* stats);
 
 /* Seeded synthetic sequences (ACGT only).  All are deterministic in `seed`
  * regardless of `threads`. */

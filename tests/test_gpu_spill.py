@@ -135,3 +135,23 @@ def test_spill_rejects_mixing_and_early_restore(reads):
         h.partition_finish()
         with pytest.raises(C.CfgpuError):
             h.restore(np.array([C.G], np.uint32))
+
+
+def test_pipelined_ingest_into_the_store(reads, tmp_path):
+    # FASTQ text through the chunked ingest pipeline, every chunk's buckets to
+    # files: after a restore of every bucket the records are one partition's
+    text = b"".join(b"@r\n" + reads[i * READ_LEN:(i + 1) * READ_LEN] + b"\n+\n" + b"I" * READ_LEN + b"\n"
+                    for i in range(N_READS))
+    with C.Handle(K, L, CUTOFF) as ref:
+        ref.partition(whole_input(reads))
+        s_ref, c_ref = bucket_sums(ref)
+    with C.Handle(K, L, CUTOFF) as h:
+        h.spill_dir(tmp_path)
+        st = h.ingest_buffer(text, True, threads=4, chunk_bytes=256 << 10, spill=True)
+        sp = h.spill_stats()
+        say(f"[case] pipelined ingest into the store: {st['n_chunks']} chunks, {sp['n_segments']} segments")
+        assert st["n_chunks"] > 4 and sp["n_segments"] == st["n_chunks"] and sp["in_files"]
+        h.restore(np.arange(C.G, dtype=np.uint32))
+        s, c = bucket_sums(h)
+        np.testing.assert_array_equal(c, c_ref)
+        np.testing.assert_array_equal(s, s_ref)
