@@ -200,6 +200,28 @@ class CdbgOut:
         return n.value, nb.value
 
 
+class UnitigOut:
+    """Pinned host buffers the per-subgraph unitigs are copied into (grown when short)."""
+
+    def __init__(self, hip):
+        self.hip, self.lab, self.rec, self.lab_cap, self.rec_cap = hip, None, None, 0, 0
+
+    def copy(self, h):
+        n, nb = ctypes.c_uint64(), ctypes.c_uint64()
+        L = C.lib()
+        h._check(L.cfgpu_unitigs_copy(h._h, None, 0, None, 0, ctypes.byref(n), ctypes.byref(nb)), "unitigs_copy")
+        if nb.value > self.lab_cap or n.value > self.rec_cap:
+            if self.lab is not None:
+                self.hip.L.hipHostFree(self.lab)
+                self.hip.L.hipHostFree(self.rec)
+            self.lab_cap, self.rec_cap = int(nb.value * 1.2) + 4096, int(n.value * 1.2) + 16
+            self.lab, self.rec = self.hip.pinned(self.lab_cap), self.hip.pinned(16 * self.rec_cap)
+        h._check(L.cfgpu_unitigs_copy(h._h, ctypes.cast(self.rec, ctypes.POINTER(C.Unitig)), self.rec_cap,
+                                      ctypes.cast(self.lab, ctypes.c_char_p), self.lab_cap, ctypes.byref(n),
+                                      ctypes.byref(nb)), "unitigs_copy")
+        return n.value, nb.value
+
+
 def run_step(h, dev, inp, out, mode="learned"):
     h.partition_device(dev[0].value, inp.n_bases, dev[1].value, inp.n_frag)
     h.construct(mode)
@@ -415,8 +437,12 @@ def main():
             # external-memory buckets: every chunk's buckets leave HBM for the
             # bucket store; then per batch of buckets: restore -> construct ->
             # contract -> unitigs to the host
+            # (a fresh handle: the one-pass handle's tables and scratch are freed first)
             log(f"external memory: ingest into the bucket store, {args.external_batches} batches")
+            h.close()
+            h = C.Handle(k, l, cutoff)
             hip.sync()
+            mem0 = hip.used_gib()
             t0 = time.perf_counter()
             sts = h.ingest_buffer(text, True, threads=16, chunk_bytes=args.ingest_chunk_mib << 20, spill=True)
             t1 = time.perf_counter()
@@ -424,6 +450,8 @@ def main():
             nz = np.nonzero(h.bucket_stats()["n_records"])[0].astype(np.uint32)
             tb = {"restore": 0.0, "construct": 0.0, "contract": 0.0, "unitigs_to_host": 0.0}
             n_u = n_ub = 0
+            mem = 0.0
+            uo = UnitigOut(hip)
             for bt in np.array_split(nz, args.external_batches):
                 a0 = time.perf_counter()
                 h.restore(bt)
@@ -434,15 +462,15 @@ def main():
                 h.contract()
                 hip.sync()
                 a3 = time.perf_counter()
-                recs, nu, buf, nb = h.unitigs_raw()
+                nu, nb = uo.copy(h)
                 a4 = time.perf_counter()
+                mem = max(mem, hip.used_gib() - mem0)
                 tb["restore"] += (a1 - a0) * 1e3
                 tb["construct"] += (a2 - a1) * 1e3
                 tb["contract"] += (a3 - a2) * 1e3
                 tb["unitigs_to_host"] += (a4 - a3) * 1e3
                 n_u += nu
                 n_ub += nb
-                del recs, buf
             t2 = time.perf_counter()
             assert n_u == cst_unitigs, "batches and the one-pass construct disagree on the unitig count"
             external = {"store": "pinned host memory", "segments": sp["n_segments"], "store_bytes": sp["bytes"],
@@ -451,9 +479,11 @@ def main():
                         "fastq_to_unitigs_ms": (t2 - t0) * 1e3,
                         "fastq_to_unitigs_kmers_per_s": n_km / (t2 - t0),
                         "unitigs": n_u, "unitig_bytes": n_ub,
+                        "device_memory_gib": mem,
                         "note": "cfhost_partition_buffer_ex with CFGPU_PARTITION_SPILL, then per batch "
                                 "cfgpu_buckets_restore -> cfgpu_construct_buckets -> cfgpu_contract -> "
-                                "cfgpu_unitigs_copy (the per-subgraph unitigs; the host glues the batches)"}
+                                "cfgpu_unitigs_copy (the per-subgraph unitigs; the host glues the batches); "
+                                "device_memory_gib: the handle's device memory after the batches (a fresh handle)"}
         del text
 
     cpu = None

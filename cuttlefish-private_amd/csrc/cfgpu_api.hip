@@ -140,6 +140,7 @@ struct cfgpu_handle {
     std::string spill_dir;
     uint64_t spill_serial = 0;     // file names
     DevBuf spill_sums;
+    DevBuf rst_att, rst_lab, rst_mb, rst_boff;  // restore: staged ranges of the segments
     bool spilled = false;          // the buckets live in the store
     std::vector<uint8_t> resident; // spilled: buckets in device memory
     uint8_t* spill_stage = nullptr;  // pinned staging of file segments and restores
@@ -1610,91 +1611,139 @@ int cfgpu_buckets_restore(cfgpu_handle* h, const uint32_t* buckets, uint32_t n) 
     HIPCK(h, hipSetDevice(h->prm.device));
     h->constructed = h->contracted = h->cdbg_done = false;
     const uint64_t W = h->W;
-    // device layout: the listed buckets in bucket order, each one's records
-    // segment after segment (as cfgpu_partition_finish gathers kept chunks)
-    std::vector<uint64_t> boff(G + 1, 0);
-    for (uint32_t b = 0; b < G; ++b) boff[b + 1] = boff[b] + (want[b] ? h->h_stats[4 * b + 0] : 0);
+    const uint32_t nc = uint32_t(h->spill.size());
+    // Device layout: the listed buckets in bucket order, each one's records
+    // segment after segment (as cfgpu_partition_finish gathers kept chunks).
+    // Each segment's records of the listed buckets (one range per run of
+    // consecutive listed buckets) are copied into a device staging area,
+    // segment after segment, and the chunk gather kernel puts them in place.
+    std::vector<uint64_t> boff(G + 1, 0), dst(size_t(nc) * G), sboff(size_t(nc) * (G + 1));
+    for (uint32_t b = 0; b < G; ++b) {
+        uint64_t at = boff[b];
+        for (uint32_t c = 0; c < nc; ++c) {
+            dst[size_t(c) * G + b] = at;
+            if (want[b]) at += h->spill[c].h_boff[b + 1] - h->spill[c].h_boff[b];
+        }
+        boff[b + 1] = at;
+    }
     const uint64_t tot = boff[G];
+    struct Run {
+        uint32_t c;
+        uint64_t r_lo, n, to;  // segment records [r_lo, r_lo + n) to staging record `to`
+    };
+    std::vector<Run> runs;
+    uint64_t st_at = 0;
+    for (uint32_t c = 0; c < nc; ++c) {
+        const auto& hb = h->spill[c].h_boff;
+        uint64_t* sb = &sboff[size_t(c) * (G + 1)];
+        sb[0] = st_at;
+        for (uint32_t b1 = 0; b1 < G;) {
+            if (!want[b1]) {
+                sb[b1 + 1] = sb[b1];
+                ++b1;
+                continue;
+            }
+            uint32_t b2 = b1;
+            while (b2 < G && want[b2]) {
+                sb[b2 + 1] = sb[b2] + (hb[b2 + 1] - hb[b2]);
+                ++b2;
+            }
+            if (hb[b2] > hb[b1]) runs.push_back({c, hb[b1], hb[b2] - hb[b1], sb[b1]});
+            b1 = b2;
+        }
+        st_at = sb[G];
+    }
     Event e0, e1;
     HIPCK(h, hipEventRecord(e0.e, h->stream));
-    HIPCK(h, ensure_mem(h, h->att, std::max<uint64_t>(tot, 1) * 4, Phase::Input));
-    HIPCK(h, ensure_mem(h, h->labels, std::max<uint64_t>(tot, 1) * W * 8, Phase::Input));
-    HIPCK(h, ensure_mem(h, h->mbin, std::max<uint64_t>(tot, 1), Phase::Input));
-    HIPCK(h, spill_stage_ensure(h, std::max<uint64_t>(seg_bytes(h, tot), 1)));
-    uint8_t* const s_att = h->spill_stage;
-    uint8_t* const s_lab = s_att + tot * 4;
-    uint8_t* const s_mb = s_lab + tot * W * 8;
-    // Host threads gather bucket ranges (disjoint destinations); a file
-    // segment is read once per maximal run of listed buckets.
-    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<std::string> errs(nt);
-    const auto work = [&](unsigned t) {
-        const uint32_t b_lo = uint32_t(uint64_t(G) * t / nt), b_hi = uint32_t(uint64_t(G) * (t + 1) / nt);
-        std::vector<uint64_t> cur(boff.begin() + b_lo, boff.begin() + b_hi);  // next record of each bucket
-        std::vector<uint8_t> tmp;
-        for (const auto& sg : h->spill) {
+    const uint64_t m1 = std::max<uint64_t>(tot, 1);
+    HIPCK(h, ensure_mem(h, h->att, m1 * 4, Phase::Input));
+    HIPCK(h, ensure_mem(h, h->labels, m1 * W * 8, Phase::Input));
+    HIPCK(h, ensure_mem(h, h->mbin, m1, Phase::Input));
+    HIPCK(h, ensure_mem(h, h->rst_att, m1 * 4, Phase::Input));
+    HIPCK(h, ensure_mem(h, h->rst_lab, m1 * W * 8, Phase::Input));
+    HIPCK(h, ensure_mem(h, h->rst_mb, m1, Phase::Input));
+    const bool files = nc && !h->spill.front().path.empty();
+    uint8_t* const d_att = h->rst_att.as<uint8_t>();
+    uint8_t* const d_lab = h->rst_lab.as<uint8_t>();
+    uint8_t* const d_mb = h->rst_mb.as<uint8_t>();
+    if (!files) {
+        // pinned segments: straight to the device
+        for (const Run& r : runs) {
+            const auto& sg = h->spill[r.c];
             const uint64_t m = sg.n_rec;
+            HIPCK(h, hipMemcpyAsync(d_att + r.to * 4, sg.host + r.r_lo * 4, r.n * 4, hipMemcpyHostToDevice,
+                                    h->stream));
+            HIPCK(h, hipMemcpyAsync(d_lab + r.to * 8 * W, sg.host + m * 4 + r.r_lo * 8 * W, r.n * 8 * W,
+                                    hipMemcpyHostToDevice, h->stream));
+            HIPCK(h, hipMemcpyAsync(d_mb + r.to, sg.host + m * (4 + 8 * W) + r.r_lo, r.n, hipMemcpyHostToDevice,
+                                    h->stream));
+        }
+    } else {
+        // file segments: read into pinned memory (host threads over the runs), then to the device
+        HIPCK(h, spill_stage_ensure(h, std::max<uint64_t>(seg_bytes(h, tot), 1)));
+        uint8_t* const s_att = h->spill_stage;
+        uint8_t* const s_lab = s_att + tot * 4;
+        uint8_t* const s_mb = s_lab + tot * W * 8;
+        const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::string> errs(nt);
+        const auto work = [&](unsigned t) {
             int fd = -1;
-            if (!sg.path.empty() && (fd = open(sg.path.c_str(), O_RDONLY)) < 0) {
-                errs[t] = "cannot open " + sg.path;
-                return;
-            }
-            for (uint32_t b1 = b_lo; b1 < b_hi;) {
-                if (!want[b1]) {
-                    ++b1;
-                    continue;
-                }
-                uint32_t b2 = b1;
-                while (b2 < b_hi && want[b2]) ++b2;
-                const uint64_t r_lo = sg.h_boff[b1], r_hi = sg.h_boff[b2], c = r_hi - r_lo;
-                const uint8_t *a_src, *l_src, *m_src;
-                if (fd >= 0) {
-                    tmp.resize(c * (4 + 8 * W + 1));
-                    if (c && (!pread_all(fd, tmp.data(), c * 4, r_lo * 4) ||
-                              !pread_all(fd, tmp.data() + c * 4, c * 8 * W, m * 4 + r_lo * 8 * W) ||
-                              !pread_all(fd, tmp.data() + c * (4 + 8 * W), c, m * (4 + 8 * W) + r_lo))) {
-                        errs[t] = "cannot read " + sg.path;
-                        close(fd);
+            uint32_t open_c = ~0u;
+            for (size_t i = t; i < runs.size(); i += nt) {
+                const Run& r = runs[i];
+                const auto& sg = h->spill[r.c];
+                if (r.c != open_c) {
+                    if (fd >= 0) close(fd);
+                    fd = open(sg.path.c_str(), O_RDONLY);
+                    open_c = r.c;
+                    if (fd < 0) {
+                        errs[t] = "cannot open " + sg.path;
                         return;
                     }
-                    a_src = tmp.data();
-                    l_src = tmp.data() + c * 4;
-                    m_src = tmp.data() + c * (4 + 8 * W);
-                } else {
-                    a_src = sg.host + r_lo * 4;
-                    l_src = sg.host + m * 4 + r_lo * 8 * W;
-                    m_src = sg.host + m * (4 + 8 * W) + r_lo;
                 }
-                for (uint32_t b = b1; b < b2; ++b) {
-                    const uint64_t q = sg.h_boff[b] - r_lo, cb = sg.h_boff[b + 1] - sg.h_boff[b];
-                    uint64_t& at = cur[b - b_lo];
-                    std::memcpy(s_att + at * 4, a_src + q * 4, cb * 4);
-                    std::memcpy(s_lab + at * 8 * W, l_src + q * 8 * W, cb * 8 * W);
-                    std::memcpy(s_mb + at, m_src + q, cb);
-                    at += cb;
+                const uint64_t m = sg.n_rec;
+                if (!pread_all(fd, s_att + r.to * 4, r.n * 4, r.r_lo * 4) ||
+                    !pread_all(fd, s_lab + r.to * 8 * W, r.n * 8 * W, m * 4 + r.r_lo * 8 * W) ||
+                    !pread_all(fd, s_mb + r.to, r.n, m * (4 + 8 * W) + r.r_lo)) {
+                    errs[t] = "cannot read " + sg.path;
+                    break;
                 }
-                b1 = b2;
             }
             if (fd >= 0) close(fd);
-        }
-    };
-    {
+        };
         std::vector<std::thread> th;
         for (unsigned t = 1; t < nt; ++t) th.emplace_back(work, t);
         work(0);
         for (auto& x : th) x.join();
+        for (const auto& e : errs)
+            if (!e.empty()) return fail(h, CFGPU_E_ARG, "restore: %s", e.c_str());
+        if (tot) {
+            HIPCK(h, hipMemcpyAsync(d_att, s_att, tot * 4, hipMemcpyHostToDevice, h->stream));
+            HIPCK(h, hipMemcpyAsync(d_lab, s_lab, tot * W * 8, hipMemcpyHostToDevice, h->stream));
+            HIPCK(h, hipMemcpyAsync(d_mb, s_mb, tot, hipMemcpyHostToDevice, h->stream));
+        }
     }
-    for (const auto& e : errs)
-        if (!e.empty()) return fail(h, CFGPU_E_ARG, "restore: %s", e.c_str());
-    if (tot) {
-        HIPCK(h, hipMemcpyAsync(h->att.p, s_att, tot * 4, hipMemcpyHostToDevice, h->stream));
-        HIPCK(h, hipMemcpyAsync(h->labels.p, s_lab, tot * W * 8, hipMemcpyHostToDevice, h->stream));
-        HIPCK(h, hipMemcpyAsync(h->mbin.p, s_mb, tot, hipMemcpyHostToDevice, h->stream));
-    }
-    HIPCK(h, h2d(h, h->boff.p, boff.data(), (G + 1) * 8));
+    // gather: segment c's staged records of bucket b, at sboff[c][b], go to dst[c][b]
+    std::vector<ChunkSrc> src(nc);
+    HIPCK(h, h->ch_src.ensure(std::max<size_t>(nc, 1) * sizeof(ChunkSrc)));
+    HIPCK(h, h->ch_dst.ensure(std::max<size_t>(dst.size(), 1) * 8));
+    HIPCK(h, h->rst_boff.ensure(std::max<size_t>(sboff.size(), 1) * 8));
+    for (uint32_t c = 0; c < nc; ++c)
+        src[c] = {h->rst_att.as<uint32_t>(), h->rst_lab.as<uint64_t>(), h->rst_mb.as<uint8_t>(),
+                  h->rst_boff.as<uint64_t>() + size_t(c) * (G + 1)};
+    HIPCK(h, hipMemcpyAsync(h->rst_boff.p, sboff.data(), sboff.size() * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCK(h, hipMemcpyAsync(h->ch_dst.p, dst.data(), dst.size() * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCK(h, hipMemcpyAsync(h->ch_src.p, src.data(), nc * sizeof(ChunkSrc), hipMemcpyHostToDevice, h->stream));
+    HIPCK(h, hipMemcpyAsync(h->boff.p, boff.data(), (G + 1) * 8, hipMemcpyHostToDevice, h->stream));
+    launch_merge_chunks(h->ch_src.as<ChunkSrc>(), nc, G, h->W, h->ch_dst.as<uint64_t>(), h->att.as<uint32_t>(),
+                        h->labels.as<uint64_t>(), h->mbin.as<uint8_t>(), h->stream);
+    HIPCK(h, hipGetLastError());
     HIPCK(h, hipEventRecord(e1.e, h->stream));
-    HIPCK(h, sync_stream(h));
+    HIPCK(h, sync_stream(h));  // the host arrays above are read by the copies until here
     h->t_restore_ms = elapsed_ms(e0, e1);
+    h->rst_att.release();  // staging is not kept between restores
+    h->rst_lab.release();
+    h->rst_mb.release();
     h->h_boff = boff;
     h->resident = want;
     h->n_dev_records = tot;
