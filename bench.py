@@ -606,6 +606,7 @@ def main():
         },
         "contract": {"unitigs": ph["n_unitigs"], "label_bytes": ph["n_unitig_bytes"],
                      "dense_table_subgraphs": ph["n_contract_dense"],
+                     "wide_table_subgraphs": ph["n_contract_wide"],
                      "global_table_subgraphs": ph["n_contract_global"], "runs": ph["n_contract_runs"],
                      "isolated": int(cst[:, 0].sum()), "phantom": int(cst[:, 1].sum()),
                      "trivial": int(cst[:, 2].sum()), "cycles": int(cst[:, 3].sum()),

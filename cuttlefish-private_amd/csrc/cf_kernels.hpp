@@ -229,6 +229,16 @@ __host__ __device__ constexpr uint32_t ct_big_max_vertices() { return 36864 * 4 
 __host__ __device__ constexpr uint64_t ct_big_scratch_bytes(uint32_t KW) {
     return uint64_t(ct_big_max_vertices()) * (8 * KW + 4 + 16 + 16);
 }
+// Wide dense tables (beyond ct_big_max_vertices(), up to ct_wide_max_vertices()):
+// vertex arrays, 64-bit ranking words and the key index all in the workgroup's
+// global scratch; index slots for nv vertices (load <= 0.8) and scratch bytes for
+// a workgroup holding up to nv vertices.
+__host__ __device__ constexpr uint32_t ct_wide_max_vertices() { return (1u << 20) - 1; }
+__host__ __device__ constexpr uint64_t ct_wide_index_slots(uint64_t nv) { return ((nv * 5 / 4 + 64) + 63) & ~63ull; }
+__host__ __device__ constexpr uint64_t ct_wide_scratch_bytes(uint32_t KW, uint64_t nv) {
+    // keys, state words, three ranking buffers of 2 nv u64, four u32 arrays, the index
+    return (((nv * (8 * KW + 4) + 7) & ~7ull) + 3 * 16 * nv + 16 * nv + 4 * ct_wide_index_slots(nv) + 255) & ~255ull;
+}
 // Global scratch table slots for a subgraph of nv vertices (load <= 0.7).
 __host__ __device__ constexpr uint64_t ct_global_capacity(uint64_t nv) { return ((nv * 10 / 7 + 64) + 63) & ~63ull; }
 struct UnitigRec {             // layout of cfgpu_unitig
@@ -251,6 +261,10 @@ struct ContractArgs {
     uint32_t* walk_head;           // global pass
     uint32_t* ovf2_list;           // buckets the dense pass could not hold
     uint32_t* ovf2_count;
+    uint32_t* wide_head;           // wide dense pass (over ovf2_list)
+    uint32_t* ovf3_list;           // buckets the wide dense pass could not hold (the global pass's)
+    uint32_t* ovf3_count;
+    uint64_t wide_cap;             // vertices a workgroup's scratch holds in the wide pass (0: no wide pass)
     uint8_t* scratch;              // dense and global passes: scratch_per_wg bytes per workgroup
     uint64_t scratch_per_wg;
     uint32_t* counts;              // per bucket 8 counters (CT_*), written by the pass that finishes it
@@ -265,10 +279,14 @@ struct ContractArgs {
     const uint32_t* nseg;          // per bucket runs recorded (0: scan the whole table)
     const uint32_t* iso;           // per bucket isolated and phantom counts (buckets with runs)
 };
-// Three passes: LDS tables for every listed bucket that fits, dense tables
-// for the rest that fit those, then global scratch tables (the counts of the
-// later passes are read on the device).
-void launch_contract(const ContractArgs& a, uint32_t KW, uint32_t grid_lds, uint32_t grid_global, hipStream_t s);
+// Four passes: LDS tables for every listed bucket that fits, dense tables for
+// the rest that fit those, wide dense tables for the rest that fit those
+// (wide_cap > 0), then global scratch tables (the counts of the later passes
+// are read on the device).
+// part 0: the LDS and dense passes; part 1: the wide dense pass (when
+// wide_cap > 0) and the global pass, over the dense pass's overflow.
+void launch_contract(const ContractArgs& a, uint32_t KW, uint32_t grid_lds, uint32_t grid_global, int part,
+                     hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // The compacted graph (cdbg.hip): contract's unitigs glued into the maximal

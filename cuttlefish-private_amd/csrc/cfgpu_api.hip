@@ -109,7 +109,7 @@ struct cfgpu_handle {
     hipStream_t stream = nullptr;
     std::string err;
     // target fill of an LDS pass table 0.8 (a table overflows at 7/8)
-    cfgpu_tuning tune{0.6f, 256, 2048, 8, 1.15f, 1, LDS_ALL_PASS_MAX, 0, 1, 120, 3e-3f, 0.8f};
+    cfgpu_tuning tune{0.6f, 256, 2048, 8, 1.15f, 1, LDS_ALL_PASS_MAX, 0, 1, 120, 3e-3f, 0.8f, 1};
 
     // partition state
     DevBuf packed, frag, gbin, mbn, tile_lc, tile_frag, boff, att, labels, mbin, stats, feat;
@@ -195,7 +195,7 @@ struct cfgpu_handle {
     DevBuf ex_keys, ex_ef, ex_st, ex_dfa, ex_off, ex_cursor, ex_list;
 
     // contract: per-subgraph counters, unitig records and labels
-    DevBuf ct_list, ct_ovf, ct_ovf2, ct_counts, ct_ctl, ct_scratch, ct_recs, ct_labels;
+    DevBuf ct_list, ct_ovf, ct_ovf2, ct_ovf3, ct_counts, ct_ctl, ct_scratch, ct_recs, ct_labels;
     uint64_t n_unitigs = 0, n_ubytes = 0;
     uint64_t ct_need[2] = {0, 0};  // records and label bytes the last contract needed
     bool contracted = false;
@@ -1908,6 +1908,7 @@ int cfgpu_contract(cfgpu_handle* h) {
     HIPCK(h, h->ct_list.ensure(n_list * 4ull));
     HIPCK(h, h->ct_ovf.ensure((n_list + n_big) * 4ull));
     HIPCK(h, h->ct_ovf2.ensure((n_list + n_big) * 4ull));
+    HIPCK(h, h->ct_ovf3.ensure((n_list + n_big) * 4ull));
     HIPCK(h, h->ct_counts.ensure(size_t(h->G) * sizeof(cfgpu_contract_stat)));
     HIPCK(h, h->ct_ctl.ensure(64));
     HIPCK(h, ensure_mem(h, h->ct_scratch, per_wg * std::max<uint32_t>(grid, 1), Phase::Contract));
@@ -1928,6 +1929,7 @@ int cfgpu_contract(cfgpu_handle* h) {
     HIPCK(h, hipEventRecord(e0.e, h->stream));
     uint32_t ctl[16];
     uint32_t run = 0;
+    uint32_t wide_cap = 0;
     for (;;) {
         ++run;
         HIPCK(h, ensure_mem(h, h->ct_recs, rec_cap * sizeof(cfgpu_unitig), Phase::Contract));
@@ -1963,7 +1965,34 @@ int cfgpu_contract(cfgpu_handle* h) {
         a.seg = h->d_seg.as<uint64_t>();
         a.nseg = h->d_nseg.as<uint32_t>();
         a.iso = h->d_iso.as<uint32_t>();
-        launch_contract(a, KW, grid, grid, h->stream);
+        a.wide_head = ctl_d + 7;
+        a.ovf3_list = h->ct_ovf3.as<uint32_t>();
+        a.ovf3_count = ctl_d + 12;
+        launch_contract(a, KW, grid, grid, 0, h->stream);
+        HIPCK(h, hipGetLastError());
+        // The dense pass's overflow: the wide dense pass takes the subgraphs
+        // its scratch holds (sized here for the largest of them), the global
+        // pass the rest.
+        uint32_t n2 = 0;
+        wide_cap = 0;
+        if (grid) {
+            HIPCK(h, hipMemcpyAsync(&n2, ctl_d + 6, 4, hipMemcpyDeviceToHost, h->stream));
+            HIPCK(h, sync_stream(h));
+        }
+        if (n2 && h->tune.contract_wide) {
+            std::vector<uint32_t> l2(n2);
+            HIPCK(h, hipMemcpyAsync(l2.data(), h->ct_ovf2.p, n2 * 4ull, hipMemcpyDeviceToHost, h->stream));
+            HIPCK(h, sync_stream(h));
+            uint32_t m2 = 0;
+            for (uint32_t g : l2) m2 = std::max(m2, h->h_nvert[g]);
+            wide_cap = std::min<uint32_t>(m2, ct_wide_max_vertices());
+            const uint64_t per_wg2 = std::max(per_wg, ct_wide_scratch_bytes(KW, wide_cap));
+            HIPCK(h, ensure_mem(h, h->ct_scratch, per_wg2 * std::max<uint32_t>(grid, 1), Phase::Contract));
+            a.scratch = h->ct_scratch.as<uint8_t>();
+            a.scratch_per_wg = per_wg2;
+            a.wide_cap = wide_cap;
+        }
+        if (n2) launch_contract(a, KW, grid, grid, 1, h->stream);
         HIPCK(h, hipGetLastError());
         HIPCK(h, hipMemcpyAsync(ctl, h->ct_ctl.p, sizeof ctl, hipMemcpyDeviceToHost, h->stream));
         HIPCK(h, sync_stream(h));
@@ -1975,8 +2004,9 @@ int cfgpu_contract(cfgpu_handle* h) {
         if (!ctl[3]) {
             h->n_unitigs = cur[0];
             h->n_ubytes = cur[1];
-            h->rs.n_contract_global = ctl[6];
+            h->rs.n_contract_global = wide_cap ? ctl[12] : ctl[6];
             h->rs.n_contract_dense = ctl[2] - ctl[6];
+            h->rs.n_contract_wide = wide_cap ? ctl[6] - ctl[12] : 0;
             break;
         }
         if (run == 2) return fail(h, CFGPU_E_OVERFLOW, "contract: output still short after resizing");
@@ -2386,7 +2416,8 @@ int cfgpu_tuning_set(cfgpu_handle* h, const cfgpu_tuning* in) {
         in->chunk_records > (1u << 24) || in->wgs_per_cu < 1 || in->wgs_per_cu > 32 ||
         !(in->margin >= 0.01f && in->margin <= 4.0f) || in->use_graph > 1 || in->max_passes > 4096 ||
         in->predictor > 2 || in->dedup > 2 || in->mlp_epochs < 1 || in->mlp_epochs > 10000 ||
-        !(in->mlp_lr > 0.f && in->mlp_lr <= 1.f) || !(in->lds_load >= 0.5f && in->lds_load <= 0.85f))
+        !(in->mlp_lr > 0.f && in->mlp_lr <= 1.f) || !(in->lds_load >= 0.5f && in->lds_load <= 0.85f) ||
+        in->contract_wide > 1)
         return fail(h, CFGPU_E_ARG, "tuning out of range");
     h->tune = *in;
     return CFGPU_OK;

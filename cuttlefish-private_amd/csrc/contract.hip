@@ -198,6 +198,74 @@ __device__ __forceinline__ void big_index(uint32_t* ht, const Kmer<KW>& x, uint3
     while (atomicCAS(&ht[s], HT_FREE, e) != HT_FREE) s = s + 1 == CB_HCAP ? 0 : s + 1;
 }
 
+// Dense vertex arrays beyond CB_NMAX vertices (the wide dense tier): the
+// key -> vertex index in the workgroup's global scratch (entry d | 12
+// fingerprint bits << CW_DBITS, all-ones = free), 64-bit ranking words.
+constexpr uint32_t CW_DBITS = 20;
+constexpr uint32_t CW_DMASK = (1u << CW_DBITS) - 1;
+static_assert(uint64_t(ct_wide_max_vertices()) <= CW_DMASK, "vertices fit the index entry");
+
+template <int KW>
+struct WideTab {
+    uint64_t* key;
+    uint32_t* meta;
+    uint32_t* aux;
+    uint32_t cap;          // vertices
+    uint32_t* ht;          // global index, hcap entries
+    uint32_t hcap;
+    const uint64_t* succ;  // per directed state its successor state | 1 << 32, or itself
+};
+
+template <int KW>
+__device__ __forceinline__ Kmer<KW> tab_key(const WideTab<KW>& t, uint32_t i) {
+    Kmer<KW> x;
+#pragma unroll
+    for (int w = 0; w < KW; ++w) x.w[w] = t.key[size_t(i) * KW + w];
+    return x;
+}
+
+template <int KW>
+__device__ __forceinline__ int tab_find(const WideTab<KW>& t, const Kmer<KW>& x) {
+    const uint64_t h = km_hash(x);
+    const uint32_t fp = uint32_t(h) & ~CW_DMASK;
+    uint32_t s = slot_of(h, t.hcap);
+    for (uint32_t n = 0; n < t.hcap; ++n) {
+        const uint32_t e = t.ht[s];
+        if (e == HT_FREE) return -1;
+        if ((e & ~CW_DMASK) == fp) {
+            const uint32_t d = e & CW_DMASK;
+            bool eq = true;
+#pragma unroll
+            for (int w = 0; w < KW; ++w) eq &= t.key[size_t(d) * KW + w] == x.w[w];
+            if (eq) return int(d);
+        }
+        s = s + 1 == t.hcap ? 0 : s + 1;
+    }
+    return -1;
+}
+
+template <int KW>
+__device__ __forceinline__ int walk_next(const WideTab<KW>& t, uint32_t cur, uint32_t m, uint32_t k, Kmer<KW>& z,
+                                         Kmer<KW>& zb, uint32_t& side, uint32_t& b) {
+    const uint32_t e = (m >> (3 * side)) & 7u;
+    b = side == 1 ? e : 3u - e;
+    z = km_append(z, b, k);
+    zb = km_prepend(zb, uint8_t(3u - b), k);
+    const uint32_t x = 2 * cur + side, st = uint32_t(t.succ[x]);
+    if (st == x) return -1;
+    side = st & 1u;
+    return int(st >> 1);
+}
+
+// Index vertex d under key x (keys are unique) in an index of hcap entries.
+template <int KW>
+__device__ __forceinline__ void wide_index(uint32_t* ht, uint32_t hcap, const Kmer<KW>& x, uint32_t d) {
+    const uint64_t h = km_hash(x);
+    const uint32_t e = d | (uint32_t(h) & ~CW_DMASK);
+    uint32_t s = slot_of(h, hcap);
+    while (atomicCAS(&ht[s], HT_FREE, e) != HT_FREE) s = s + 1 == hcap ? 0 : s + 1;
+}
+
 __device__ __forceinline__ uint32_t ld_meta(const uint32_t* m) {
     return __hip_atomic_load(m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
@@ -1015,6 +1083,298 @@ __global__ __launch_bounds__(CT_WG) void k_contract_big(ContractArgs a) {
     }
 }
 
+// Subgraphs beyond CB_NMAX non-isolated vertices (up to ct_wide_max_vertices()
+// and the vertices this workgroup's scratch holds): the dense tier's steps with
+// 32-bit directed states - the key -> vertex index in global scratch and
+// 64-bit ranking words (state | distance << 32) in three global buffers.
+constexpr uint32_t CW_ROUNDS = 22;  // > log2(2 * ct_wide_max_vertices()): every path converges
+static_assert((1ull << CW_ROUNDS) > 2ull * ct_wide_max_vertices(), "ranking rounds");
+__device__ __forceinline__ uint64_t cw_pack(uint32_t p, uint64_t d) { return uint64_t(p) | (d << 32); }
+
+template <int KW>
+__global__ __launch_bounds__(CT_WG) void k_contract_wide(ContractArgs a) {
+    __shared__ __align__(16) uint32_t s_buf[CB_JP_LDS];  // labels, when they fit
+    __shared__ uint32_t s_cand[CT_CAND];
+    __shared__ uint32_t s_cnt[8];
+    __shared__ uint32_t s_chg[CW_ROUNDS];
+    __shared__ uint32_t s_item, s_n, s_ovf, s_ncand, s_bytes, s_nrec, s_rctr, s_err, s_skip;
+    __shared__ unsigned long long s_brec, s_bbyte;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t k = a.k;
+    const uint32_t kt = k - 32 * (KW - 1);
+    // scratch (wide_cap vertices): key words, state, three ranking buffers of
+    // 64-bit words (state | distance << 32), path info (first vertex |
+    // forward << 30 | on a path << 31), position, the first vertex's vertex
+    // count | end flags << 29, label offsets, the key index
+    const uint64_t N = a.wide_cap;
+    uint8_t* base = a.scratch + uint64_t(blockIdx.x) * a.scratch_per_wg;
+    WideTab<KW> t;
+    t.key = reinterpret_cast<uint64_t*>(base);
+    t.meta = reinterpret_cast<uint32_t*>(base + N * KW * 8);
+    uint64_t* jp0 = reinterpret_cast<uint64_t*>(base + ((N * (8 * KW + 4) + 7) & ~7ull));
+    uint64_t* jp1 = jp0 + 2 * N;
+    uint64_t* jp2 = jp1 + 2 * N;
+    uint32_t* ia = reinterpret_cast<uint32_t*>(jp2 + 2 * N);
+    uint32_t* posv = ia + N;
+    uint32_t* wnv = posv + N;
+    uint32_t* offv = wnv + N;
+    t.aux = offv;
+    t.ht = offv + N;
+    t.succ = jp0;
+
+    while (true) {
+        __syncthreads();
+        if (tid == 0) {
+            s_item = atomicAdd(a.wide_head, 1u);
+            s_n = s_ovf = s_ncand = s_bytes = s_nrec = s_rctr = s_err = s_skip = 0;
+        }
+        if (tid < 8) s_cnt[tid] = 0;
+        if (tid < CW_ROUNDS) s_chg[tid] = 0;
+        __syncthreads();
+        if (s_item >= __hip_atomic_load(a.ovf2_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        const uint32_t b = a.ovf2_list[s_item];
+        // the index is sized by the subgraph's vertices (an upper bound on
+        // the ones it takes); a subgraph with more than this workgroup's
+        // scratch holds goes on to the global pass
+        const uint64_t nv = a.n_vert[b];
+        if (nv > N) {
+            if (tid == 0) a.ovf3_list[atomicAdd(a.ovf3_count, 1u)] = b;
+            continue;
+        }
+        t.hcap = uint32_t(ct_wide_index_slots(nv));
+        for (uint32_t i = tid; i < t.hcap; i += CT_WG) t.ht[i] = HT_FREE;
+        __syncthreads();
+        load_bucket_with<KW>(a, b, uint32_t(N), &s_n, &s_ovf, s_cnt, [&](uint32_t d, const Kmer<KW>& x, uint32_t m) {
+#pragma unroll
+            for (int w = 0; w < KW; ++w) t.key[size_t(d) * KW + w] = x.w[w];
+            t.meta[d] = m;
+            wide_index(t.ht, t.hcap, x, d);
+            return true;
+        });
+        __syncthreads();
+        if (s_ovf) {
+            if (tid == 0) a.ovf3_list[atomicAdd(a.ovf3_count, 1u)] = b;
+            continue;
+        }
+        const uint32_t n = s_n;
+        t.cap = n;
+
+        // link: each directed state points at its successor (1 step) or at itself (an end)
+        for (uint32_t d = tid; d < n; d += CT_WG) {
+            const Kmer<KW> x = tab_key<KW>(t, d);
+            const Kmer<KW> xr = km_rc(x, k);
+            const uint32_t m = ld_meta(&t.meta[d]);
+            uint32_t add = 0;
+            for (uint32_t sd = 0; sd < 2; ++sd) {
+                const int nx = link_of<KW>(t, d, sd, x, xr, m, k, &s_err);
+                jp0[2 * d + sd] = nx < 0 ? cw_pack(2 * d + sd, 0) : cw_pack(uint32_t(nx), 1);
+                if (nx >= 0) add |= M_LINK0 << sd;
+            }
+            // only this thread writes vertex d's state here; the others read
+            // its DFA bits, which the store keeps
+            if (add) __hip_atomic_store(&t.meta[d], m | add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        __syncthreads();
+
+        // rank: pointer jumping until no path state moves (cycle states stop
+        // counting once past the vertex count).  The successor words in jp0
+        // stay as they are (cycle walks follow them).
+        // in global scratch, three buffers (jp0 keeps the successor words,
+        // which cycle walks follow); CB_RU states per thread at a time, their
+        // loads issued before any store
+        const uint64_t* jp;
+        {
+            const uint64_t* src = jp0;
+            uint64_t* dst = jp1;
+            for (uint32_t r = 0; r < CW_ROUNDS; ++r) {
+                bool any = false;
+                for (uint32_t x0 = tid; x0 < 2 * n; x0 += CT_WG * CB_RU) {
+                    uint64_t w[CB_RU], w2[CB_RU];
+#pragma unroll
+                    for (uint32_t u = 0; u < CB_RU; ++u) {
+                        const uint32_t x = x0 + u * CT_WG;
+                        w[u] = x < 2 * n ? src[x] : x;
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < CB_RU; ++u) {
+                        const uint32_t x = x0 + u * CT_WG, p = uint32_t(w[u]);
+                        w2[u] = x < 2 * n && p != x ? src[p] : 0ull;
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < CB_RU; ++u) {
+                        const uint32_t x = x0 + u * CT_WG;
+                        if (x >= 2 * n) continue;
+                        const uint32_t p = uint32_t(w[u]);
+                        uint64_t nw = w[u];
+                        if (p != x) {
+                            const uint32_t p2 = uint32_t(w2[u]);
+                            const uint64_t nd = (w[u] >> 32) + (w2[u] >> 32);
+                            nw = cw_pack(p2, nd);
+                            any |= p2 != p && nd < n;
+                        }
+                        dst[x] = nw;
+                    }
+                }
+                if (any) s_chg[r] = 1;
+                __syncthreads();
+                src = dst;
+                dst = dst == jp1 ? jp2 : jp1;
+                if (!s_chg[r]) break;
+            }
+            jp = src;
+        }
+
+        // per vertex: the ends of its path, the output orientation and its position
+        for (uint32_t d = tid; d < n; d += CT_WG) {
+            const uint64_t wf = jp[2 * d], wb = jp[2 * d + 1];
+            const uint32_t Ff = uint32_t(wf), Fb = uint32_t(wb);
+            if (jp[Fb] != cw_pack(Fb, 0)) {  // not a path end: the vertex lies on a cycle
+                atomicOr(&t.meta[d], M_ONCYC);
+                const uint32_t c = atomicAdd(&s_ncand, 1u);
+                if (c < uint32_t(CT_CAND)) s_cand[c] = d;
+                ia[d] = 0;
+                continue;
+            }
+            const uint32_t df = uint32_t(wf >> 32), db = uint32_t(wb >> 32);
+            const uint32_t L = Ff >> 1, sL = Ff & 1u, R = Fb >> 1, sR = Fb & 1u;
+            const Kmer<KW> cL = tab_key<KW>(t, L), cR = tab_key<KW>(t, R);
+            const Kmer<KW> first = sL == 0 ? cL : km_rc(cL, k);
+            const Kmer<KW> rlast = sR == 1 ? km_rc(cR, k) : cR;
+            const bool fwd = km_less(first, rlast);
+            const uint32_t pos = fwd ? df : db;
+            const uint32_t A = fwd ? L : R;
+            ia[d] = A | (uint32_t(fwd) << 30) | (1u << 31);
+            posv[d] = pos;
+            if (pos == 0) {
+                const uint32_t sA = fwd ? sL : sR, B = fwd ? R : L, sB = fwd ? sR : sL;
+                const uint32_t fl = (side_exits(ld_meta(&t.meta[A]), sA) ? 2u : 0u) |
+                                    (side_exits(ld_meta(&t.meta[B]), sB) ? 4u : 0u);
+                wnv[d] = (df + db + 1) | (fl << 29);
+            }
+        }
+        __syncthreads();
+
+        // the first vertex of each path reserves its label
+        {
+            uint32_t nrec = 0, verts = 0, lbytes = 0, triv = 0, disc = 0;
+            for (uint32_t d = tid; d < n; d += CT_WG) {
+                if (!ia[d] || posv[d]) continue;
+                const uint32_t wn = wnv[d] & 0x1FFFFFFFu, fl = wnv[d] >> 29;
+                const uint32_t bytes = wn + k - 1;
+                offv[d] = atomicAdd(&s_bytes, bytes);
+                ++nrec;
+                verts += wn;
+                lbytes += bytes;
+                disc += fl != 0;
+                triv += fl == 0;
+            }
+            if (nrec) {
+                atomicAdd(&s_nrec, nrec);
+                atomicAdd(&s_cnt[CT_VERTICES], verts);
+                atomicAdd(&s_cnt[CT_LABEL_BYTES], lbytes);
+                atomicAdd(&s_cnt[CT_TRIVIAL], triv);
+                atomicAdd(&s_cnt[CT_DISC_EDGES], disc);
+            }
+        }
+        __syncthreads();
+        if (tid == 0 && s_ncand)
+            cycles_serial<KW>(t, k, s_cand, s_ncand, &s_bytes, &s_nrec, s_cnt, &s_err,
+                              [&](uint32_t slot) { return &offv[slot]; });
+        __syncthreads();
+
+        // reserve the bucket's output, publish its counters
+        if (tid == 0) {
+            s_brec = atomicAdd(&a.cursors[0], (unsigned long long)s_nrec);
+            s_bbyte = atomicAdd(&a.cursors[1], (unsigned long long)s_bytes);
+            s_skip = s_brec + s_nrec > a.rec_cap || s_bbyte + s_bytes > a.lab_cap;
+            if (s_skip) atomicOr(a.retry, 1u);
+            if (s_err) atomicOr(a.error, 1u);
+            s_cnt[CT_FLAGS] = 16u | (s_err ? 4u : 0u);
+        }
+        __syncthreads();
+        if (tid < 8) a.counts[size_t(b) * 8 + tid] = s_cnt[tid];
+        if (s_skip) continue;
+
+        // write: every path vertex stores its base, the first one its k-mer
+        // and the record.  The ranking words are no longer needed: when the
+        // subgraph's labels fit, they are assembled in that LDS buffer (at the
+        // offset that keeps the output's 16-byte alignment) and copied out
+        // with whole 16-byte stores.  Each thread loads CB_WU vertices'
+        // words before it writes any byte.
+        const uint64_t lab_end = s_bbyte + s_bytes;
+        const uint32_t sh = uint32_t(s_bbyte & 15u);
+        const bool staged = sh + s_bytes <= CB_STAGE;
+        char* s_lab = reinterpret_cast<char*>(s_buf);
+        char* out0 = staged ? s_lab + sh : a.labels + s_bbyte;  // label byte s_bbyte + j is out0[j]
+        for (uint32_t d0 = tid; d0 < n; d0 += CT_WG * CB_WU) {
+            uint32_t info[CB_WU], pos[CB_WU], offa[CB_WU];
+            Kmer<KW> x[CB_WU];
+#pragma unroll
+            for (uint32_t u = 0; u < CB_WU; ++u) {
+                const uint32_t d = d0 + u * CT_WG;
+                info[u] = d < n ? ia[d] : 0u;
+                pos[u] = d < n ? posv[d] : 0u;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < CB_WU; ++u) {
+                const uint32_t d = d0 + u * CT_WG;
+                offa[u] = info[u] ? offv[info[u] & 0x3FFFFFFFu] : 0u;
+                if (info[u]) x[u] = tab_key<KW>(t, d);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < CB_WU; ++u) {
+                const uint32_t d = d0 + u * CT_WG;
+                if (d >= n) continue;
+                if (!info[u]) {
+                    if ((ld_meta(&t.meta[d]) & (M_WRITER | M_CYCLE)) != (M_WRITER | M_CYCLE)) continue;
+                    const uint64_t off = s_bbyte + offv[d];
+                    const uint32_t nc = write_cycle<KW>(t, d, k, out0 + offv[d], off, lab_end, a.error);
+                    const uint32_t ri = atomicAdd(&s_rctr, 1u);
+                    if (ri < s_nrec) a.recs[s_brec + ri] = UnitigRec{off, nc + k - 1, uint16_t(b), 1};
+                    continue;
+                }
+                const bool fwd = (info[u] >> 30) & 1u;
+                const uint64_t off = s_bbyte + offa[u];
+                if (pos[u] == 0) {
+                    const uint32_t wn = wnv[d] & 0x1FFFFFFFu, fl = wnv[d] >> 29;
+                    if (off + wn + k - 1 > lab_end) {
+                        atomicOr(a.error, 1u);
+                        continue;
+                    }
+                    put_kmer(out0 + offa[u], fwd ? x[u] : km_rc(x[u], k), k);
+                    const uint32_t ri = atomicAdd(&s_rctr, 1u);
+                    if (ri < s_nrec) a.recs[s_brec + ri] = UnitigRec{off, wn + k - 1, uint16_t(b), uint16_t(fl)};
+                } else {
+                    const uint32_t c =
+                        fwd ? uint32_t(x[u].w[0] & 3u) : 3u - uint32_t((x[u].w[KW - 1] >> (2 * (kt - 1))) & 3u);
+                    if (off + pos[u] + k - 1 >= lab_end) {
+                        atomicOr(a.error, 1u);
+                        continue;
+                    }
+                    out0[offa[u] + pos[u] + k - 1] = char(base_char(c));
+                }
+            }
+        }
+        if (staged) {
+            // LDS bytes [sh, sh + s_bytes) go to labels [s_bbyte, lab_end);
+            // 16-byte chunk c covers LDS [16c, 16c + 16) and the same offsets
+            // from s_bbyte - sh (16-byte aligned)
+            __syncthreads();
+            char* g0 = a.labels + (s_bbyte - sh);
+            const uint32_t end = sh + s_bytes;
+            for (uint32_t c = tid; c < (end + 15) / 16; c += CT_WG) {
+                const uint32_t lo = 16 * c;
+                if (lo >= sh && lo + 16 <= end) {
+                    *reinterpret_cast<uint4*>(g0 + lo) = *reinterpret_cast<const uint4*>(s_lab + lo);
+                } else {
+                    for (uint32_t j = lo; j < lo + 16; ++j)
+                        if (j >= sh && j < end) g0[j] = s_lab[j];
+                }
+            }
+        }
+    }
+}
+
 // Subgraphs whose vertices do not fit an LDS table: a table in this
 // workgroup's global scratch region, sized by the subgraph's vertex count.
 template <int KW>
@@ -1189,22 +1549,34 @@ __global__ __launch_bounds__(CT_WG) void k_contract_walk(ContractArgs a) {
 }
 
 template <int KW>
-void launch_kw(const ContractArgs& a, uint32_t grid_lds, uint32_t grid_global, hipStream_t s) {
-    if (grid_lds) hipLaunchKernelGGL(k_contract_rank<KW>, dim3(grid_lds), dim3(CT_WG), 0, s, a);
-    if (grid_global) {
-        hipLaunchKernelGGL(k_contract_big<KW>, dim3(grid_global), dim3(CT_WG), 0, s, a);
+void launch_kw(const ContractArgs& a, uint32_t grid_lds, uint32_t grid_global, int part, hipStream_t s) {
+    if (part == 0) {
+        if (grid_lds) hipLaunchKernelGGL(k_contract_rank<KW>, dim3(grid_lds), dim3(CT_WG), 0, s, a);
+        if (grid_global) hipLaunchKernelGGL(k_contract_big<KW>, dim3(grid_global), dim3(CT_WG), 0, s, a);
+        return;
+    }
+    if (!grid_global) return;
+    if (a.wide_cap) {
+        // the wide pass takes the dense pass's overflow; the global pass its own
+        hipLaunchKernelGGL(k_contract_wide<KW>, dim3(grid_global), dim3(CT_WG), 0, s, a);
+        ContractArgs w = a;
+        w.ovf2_list = a.ovf3_list;
+        w.ovf2_count = a.ovf3_count;
+        hipLaunchKernelGGL(k_contract_walk<KW>, dim3(grid_global), dim3(CT_WG), 0, s, w);
+    } else {
         hipLaunchKernelGGL(k_contract_walk<KW>, dim3(grid_global), dim3(CT_WG), 0, s, a);
     }
 }
 
 }  // namespace
 
-void launch_contract(const ContractArgs& a, uint32_t KW, uint32_t grid_lds, uint32_t grid_global, hipStream_t s) {
+void launch_contract(const ContractArgs& a, uint32_t KW, uint32_t grid_lds, uint32_t grid_global, int part,
+                     hipStream_t s) {
     switch (KW) {
-        case 1: launch_kw<1>(a, grid_lds, grid_global, s); break;
-        case 2: launch_kw<2>(a, grid_lds, grid_global, s); break;
-        case 3: launch_kw<3>(a, grid_lds, grid_global, s); break;
-        default: launch_kw<4>(a, grid_lds, grid_global, s); break;
+        case 1: launch_kw<1>(a, grid_lds, grid_global, part, s); break;
+        case 2: launch_kw<2>(a, grid_lds, grid_global, part, s); break;
+        case 3: launch_kw<3>(a, grid_lds, grid_global, part, s); break;
+        default: launch_kw<4>(a, grid_lds, grid_global, part, s); break;
     }
 }
 

@@ -64,7 +64,7 @@ class RunStats(ctypes.Structure):
                    ("n_cdbg_unitigs", ctypes.c_uint64), ("n_cdbg_bytes", ctypes.c_uint64),
                    ("n_cdbg_rounds", ctypes.c_uint32), ("n_cdbg_cycle_pieces", ctypes.c_uint32),
                    ("n_contract_dense", ctypes.c_uint32), ("n_partition_chunks", ctypes.c_uint32),
-                   ("t_partition_merge_ms", ctypes.c_double)])
+                   ("t_partition_merge_ms", ctypes.c_double), ("n_contract_wide", ctypes.c_uint32)])
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -89,7 +89,7 @@ class Tuning(ctypes.Structure):
                 ("margin", ctypes.c_float), ("use_graph", ctypes.c_uint32),
                 ("max_passes", ctypes.c_uint32), ("predictor", ctypes.c_uint32),
                 ("dedup", ctypes.c_uint32), ("mlp_epochs", ctypes.c_uint32), ("mlp_lr", ctypes.c_float),
-                ("lds_load", ctypes.c_float)]
+                ("lds_load", ctypes.c_float), ("contract_wide", ctypes.c_uint32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -249,8 +249,9 @@ int cfgpu_contract(cfgpu_handle* h);
 /* Per-subgraph counters of the last contract (Subgraph's isolated, trivial_mtig_c,
  * icc_count, disc_edge_c, the vertices in unitigs and the label bytes;
  * include/Subgraph.hpp:178-186).  flags: 1 LDS table, 8 dense table (vertex arrays in
- * global memory, index in LDS), 2 global table, 0 not contracted; 4 is added when the
- * subgraph's table was inconsistent. */
+ * global memory, index in LDS), 16 wide dense table (32-bit states, index in global
+ * memory), 2 global table, 0 not contracted; 4 is added when the subgraph's table was
+ * inconsistent. */
 typedef struct cfgpu_contract_stat {
     uint32_t isolated, phantom, trivial, icc, disc_edges, vertices, label_bytes, flags;
 } cfgpu_contract_stat;
@@ -311,6 +312,7 @@ typedef struct cfgpu_run_stats {
     uint32_t n_contract_dense;        /* subgraphs contracted with dense tables (too large for LDS) */
     uint32_t n_partition_chunks;      /* input chunks the last partition gathered (1 without APPEND) */
     double t_partition_merge_ms;      /* cfgpu_partition_finish's gather of the chunks' buckets */
+    uint32_t n_contract_wide;         /* subgraphs contracted with wide dense tables (beyond the dense ones) */
 } cfgpu_run_stats;
 int cfgpu_run_stats_get(cfgpu_handle* h, cfgpu_run_stats* out);
 
@@ -355,6 +357,9 @@ typedef struct cfgpu_tuning {
     float mlp_lr;             /* learned sizing: Adam learning rate (0 < lr <= 1) */
     float lds_load;           /* batched paths: target fill of an LDS pass table (0.5 .. 0.85);
                                  fewer, fuller passes or more, emptier ones */
+    uint32_t contract_wide;   /* contract: 1 subgraphs beyond the dense tables take wide dense
+                                 tables (32-bit states, index in global memory), 0 they take
+                                 the global-table walk (results never change) */
 } cfgpu_tuning;
 int cfgpu_tuning_get(cfgpu_handle* h, cfgpu_tuning* out);
 int cfgpu_tuning_set(cfgpu_handle* h, const cfgpu_tuning* in);

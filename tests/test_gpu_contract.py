@@ -77,26 +77,32 @@ def test_contract_reference_k127():
     run("k127", buf, False, 127, 20, 1)
 
 
-def test_contract_large_subgraphs_use_dense_and_global_tables():
+@pytest.mark.parametrize("wide", [0, 1])
+def test_contract_large_subgraphs_use_dense_and_global_tables(wide):
     # Zipf-sized buckets of random records: the large ones hold more vertices
     # than an LDS table (dense tables: vertex arrays in global scratch, index
-    # in LDS), the largest more than a dense table (global-scratch walk).
+    # in LDS), the largest more than a dense table: wide dense tables (32-bit
+    # states, index in global scratch) or, with those off, the global-scratch
+    # walk.
     for k, l in ((31, 20), (63, 20)):
         att, lab, off = C.synth_bucket_workload(256, 150000, 1.1, k, l, seed=40 + k)
         st = O.construct(k, l, off, (att & 0xFFFF).astype(np.uint16), lab)
         want = O.contract(k, 1, st, normalise=True)
-        say(f"[case] zipf-contract k={k} vertices={len(st['g'])}")
+        say(f"[case] zipf-contract k={k} wide={wide} vertices={len(st['g'])}")
         with C.Handle(k, l, 1) as h:
+            h.set_tuning(contract_wide=wide)
             h.load_buckets(att, lab, off)
             h.construct("batched")
             h.contract()
             stats = h.contract_stats()
             compare("zipf", h.unitigs(), stats, want)
-            kinds = {f: int((stats[:, 7] == f).sum()) for f in (1, 8, 2)}
-            say(f"[case] zipf-contract k={k} subgraphs by table: {kinds}")
-            assert all(v > 0 for v in kinds.values())
+            kinds = {f: int((stats[:, 7] == f).sum()) for f in (1, 8, 16, 2)}
+            say(f"[case] zipf-contract k={k} wide={wide} subgraphs by table: {kinds}")
+            assert kinds[1] > 0 and kinds[8] > 0
+            assert (kinds[16] > 0 and kinds[2] == 0) if wide else (kinds[2] > 0 and kinds[16] == 0)
             rs = h.run_stats()
             assert rs["n_contract_global"] == kinds[2] and rs["n_contract_dense"] == kinds[8]
+            assert rs["n_contract_wide"] == kinds[16]
 
 
 def test_contract_after_release_skips_subgraph():
