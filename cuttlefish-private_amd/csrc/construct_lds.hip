@@ -37,9 +37,6 @@ namespace {
 constexpr uint64_t EMPTY = ~0ull;
 constexpr uint64_t BUSY = 1ull << 63;
 constexpr int LWG = LDS_WG;
-// Group reads after which a pass-phase search is set aside (see the pass
-// phase): 1 measured faster than 2 at 50M reads, k = 63 (92.5 vs 95.1 ms).
-constexpr uint32_t DEFER_TRIPS = 1;
 
 // Pass and group choice of these tables: pass_hash32 of the key words.  Only
 // the spread of keys over passes and groups depends on it: keys are always
@@ -49,37 +46,79 @@ __device__ __forceinline__ uint32_t pass_hash(const Kmer<KW>& x) {
     return pass_hash32<KW>(x.w);
 }
 
-constexpr int UPSERT_FULL = -1;      // every group is full without the key
-constexpr int UPSERT_DEFERRED = -2;  // max_trips group reads without a result
+constexpr int UPSERT_FULL = -1;  // every group is full without the key
+
+// Group probe order of a key: its home group g0, then g0 + s, g0 + 2s, ...
+// modulo the NG groups, for an odd stride s < 128 coprime to NG (so the order
+// visits every group) taken from 6 hash bits.  Keys that share a home group
+// mostly go on with different strides, so the full groups do not gather into
+// the runs that every later search through them must cross (the clusters of
+// a probe by consecutive groups).
+constexpr uint32_t gcd_u32(uint32_t a, uint32_t b) { return b ? gcd_u32(b, a % b) : a; }
+constexpr uint64_t stride_bad_mask(uint32_t ng, uint32_t half) {  // bit i: 64 half + i shares a factor with ng
+    uint64_t m = 0;
+    for (uint32_t i = 0; i < 64; ++i)
+        if (gcd_u32(64 * half + i, ng) != 1) m |= 1ull << i;
+    return m;
+}
+constexpr uint32_t STRIDE_BUMPS = 3;
+constexpr uint32_t stride_of(uint32_t ng, uint32_t x) {
+    uint32_t s = 2 * x + 1;
+    for (uint32_t i = 0; i < STRIDE_BUMPS; ++i)
+        if (gcd_u32(s, ng) != 1) s += 2;
+    return s;
+}
+constexpr bool strides_ok(uint32_t ng) {  // every stride coprime, below 128 and below ng
+    for (uint32_t x = 0; x < 64; ++x)
+        if (gcd_u32(stride_of(ng, x), ng) != 1 || stride_of(ng, x) >= 128 || stride_of(ng, x) >= ng) return false;
+    return true;
+}
+template <uint32_t NG>
+__device__ __forceinline__ uint32_t group_stride(uint32_t h) {
+    static_assert(strides_ok(NG), "group strides");
+    constexpr uint64_t B0 = stride_bad_mask(NG, 0), B1 = stride_bad_mask(NG, 1);
+    uint32_t s = 2 * (h & 63u) + 1;
+#pragma unroll
+    for (uint32_t i = 0; i < STRIDE_BUMPS; ++i) {
+        const bool bad = s < 64 ? (B0 >> s) & 1u : (B1 >> (s - 64)) & 1u;
+        s += bad ? 2u : 0u;
+    }
+    return s;
+}
 
 // Add one k-mer instance to an LDS table of C slots (KW u64 words each): find
 // or insert `key`, add m to the counters of its front / back bases (saturating
 // at 15, Edge_Frequency::add_edge, include/State_Config.hpp:207-217) and OR
 // its discontinuity bits into the status byte.  Returns the slot, or
-// UPSERT_FULL / UPSERT_DEFERRED.
+// UPSERT_FULL.
 //
-// Groups of 4 slots are probed: the 4 top words are read together, and a
-// group fills from its first slot on (claims take the first empty slot;
-// entries are never removed), so a group with an empty slot ends an
-// unsuccessful search.  The group is scanned without branches (select
-// chains, a candidate bit mask).  A slot is claimed by writing its top word
+// Groups of 4 slots are probed in the key's group order (group_stride): the
+// 4 top words are read together, and a group fills from its first slot on
+// (claims take the first empty slot; entries are never removed), so a group
+// with an empty slot ends an unsuccessful search.  The group is scanned
+// without branches (select chains, a candidate bit mask); a top word is below
+// 2^62 (k odd: at most 31 bases in the top word), so neither EMPTY nor a
+// claimed word of another key equals the key's plain or claimed top word.  A slot is claimed by writing its top word
 // with BUSY set; its lower key words and its first counters are written, and
 // a release fence orders them before the plain top word publishes the slot.
 // A lane that finds the key reads its counters together with its lower key
 // words and updates them by CAS; a lane that meets a claimed slot with its own
-// top word re-reads the group.  After max_trips group reads (re-reads
-// included) without a result it returns UPSERT_DEFERRED with g the group to
-// go on from: every group before g in the key's probe order was full without
-// the key, and stays so, so a later search from g adds the instance where a
-// search from its home group would.
+// top word re-reads the group.
+//
+// One group read is one step (upsert_step, STEP_DONE / STEP_AGAIN /
+// STEP_FULL): lds_upsert repeats it until the instance is added; the pass
+// phase's lanes each take one step per loop trip and a new instance when
+// theirs is added.  Every group before g in the key's probe order is full
+// without the key, and stays so, so a search that goes on from g adds the
+// instance where a search from its home group would.
+constexpr int STEP_DONE = 0, STEP_AGAIN = 1, STEP_FULL = 2;
 template <int KW, int C>
-__device__ __forceinline__ int lds_upsert(uint64_t* s_key, uint32_t* s_ef, uint32_t* s_st, const Kmer<KW>& key,
-                                          uint32_t& g, uint32_t front, uint32_t back, uint32_t disc, uint32_t m,
-                                          uint32_t& my_ins, uint32_t max_trips = ~0u) {
+__device__ __forceinline__ int upsert_step(uint64_t* s_key, uint32_t* s_ef, uint32_t* s_st, const Kmer<KW>& key,
+                                           uint32_t& g, uint32_t stride, uint32_t& n_full, uint32_t front,
+                                           uint32_t back, uint32_t disc, uint32_t m, uint32_t& my_ins) {
     constexpr uint32_t NG = uint32_t(C) / 4;
     const uint64_t top = key.w[KW - 1];
-    for (uint32_t n = 0, trips = 0; n < NG;) {
-        if (trips++ == max_trips) return UPSERT_DEFERRED;
+    {
         uint64_t cur[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j)
@@ -89,9 +128,8 @@ __device__ __forceinline__ int lds_upsert(uint64_t* s_key, uint32_t* s_ef, uint3
         bool busy = false;  // a matching top word is still being published
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const bool c = j < empty && (cur[j] & ~BUSY) == top;
-            cand |= uint32_t(c && !(cur[j] & BUSY)) << j;
-            busy |= c && (cur[j] & BUSY);
+            cand |= uint32_t(cur[j] == top) << j;
+            busy |= cur[j] == (top | BUSY);
         }
         int hit = -1;
         uint32_t old = 0;
@@ -119,13 +157,13 @@ __device__ __forceinline__ int lds_upsert(uint64_t* s_key, uint32_t* s_ef, uint3
                 old = prev;
             }
             if (disc) atomicOr(&s_st[hit >> 2], disc << (8 * (hit & 3)));
-            return hit;
+            return STEP_DONE;
         }
-        if (busy) continue;  // re-read the group once it is published
-        if (empty == 4) {    // full group without the key: next group
-            g = g + 1 == NG ? 0 : g + 1;
-            ++n;
-            continue;
+        if (busy) return STEP_AGAIN;  // re-read the group once it is published
+        if (empty == 4) {             // full group without the key: next group
+            g += stride;
+            g = g >= NG ? g - NG : g;
+            return ++n_full == NG ? STEP_FULL : STEP_AGAIN;
         }
         const uint32_t slot = g * 4 + uint32_t(empty);
         uint64_t* kp = &s_key[slot * KW];
@@ -139,19 +177,30 @@ __device__ __forceinline__ int lds_upsert(uint64_t* s_key, uint32_t* s_ef, uint3
             __hip_atomic_store(kp + KW - 1, top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (disc) atomicOr(&s_st[slot >> 2], disc << (8 * (slot & 3)));
             ++my_ins;
-            return int(slot);
+            return STEP_DONE;
         }
-        // lost the slot to another key: re-read the group
+        return STEP_AGAIN;  // lost the slot to another key: re-read the group
     }
-    return UPSERT_FULL;
+}
+
+template <int KW, int C>
+__device__ __forceinline__ int lds_upsert(uint64_t* s_key, uint32_t* s_ef, uint32_t* s_st, const Kmer<KW>& key,
+                                          uint32_t g, uint32_t stride, uint32_t front, uint32_t back, uint32_t disc,
+                                          uint32_t m, uint32_t& my_ins) {
+    uint32_t n_full = 0;
+    while (true) {
+        const int r = upsert_step<KW, C>(s_key, s_ef, s_st, key, g, stride, n_full, front, back, disc, m, my_ins);
+        if (r != STEP_AGAIN) return r == STEP_DONE ? 0 : UPSERT_FULL;
+    }
 }
 
 // Spill region of a workgroup for one all-pass item: P sub-lists of c =
 // spill_cap / P entries, sub-list q holding the instances of pass q.  Key
 // word w of entry e is at key[w * spill_cap + e]; meta[e] = front | back << 3
-// | disc << 6 | min(m, 15) << 9 | group << 13 (counters saturate at 15, so
-// adding min(m, 15) equals adding m; the group of the pass table, below
-// 2^12, is computed once by the walk).
+// | disc << 6 | min(m, 15) << 9 | group << 13 | stride << 25 (counters
+// saturate at 15, so adding min(m, 15) equals adding m; the group of the pass
+// table, below 2^12, and the group stride, below 2^7, are computed once by
+// the walk).
 struct SpillRegion {
     uint64_t* key;
     uint32_t* meta;
@@ -213,7 +262,8 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
     __shared__ uint64_t s_key[C * KW];
     __shared__ uint32_t s_ef[C];
     __shared__ uint32_t s_st[C / 4];  // one Vertex_Status byte per slot
-    __shared__ uint32_t s_item, s_cnt, s_ovf, s_base, s_wp, s_ndef;
+    __shared__ uint32_t s_item, s_cnt, s_ovf, s_base, s_wp;
+    __shared__ uint32_t s_next;  // pass phase: entries of the sub-list handed out
     __shared__ unsigned long long s_wai;  // write-out: active entries | isolated entries << 32
     __shared__ uint32_t s_sn[LDS_ALL_PASS_MAX];  // sub-list lengths of an all-pass item
 
@@ -253,7 +303,7 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                 for (int i = tid; i < C; i += LWG) s_ef[i] = 0u;
                 for (int i = tid; i < C / 4; i += LWG) s_st[i] = 0u;
             }
-            if (tid == 0) s_cnt = s_ndef = 0;
+            if (tid == 0) s_cnt = s_next = 0;
             __syncthreads();
             if (s_ovf) break;  // bucket already flagged
             uint32_t my_ins = 0;  // keys this lane inserted since its last flush into s_cnt
@@ -292,19 +342,21 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                 }
                 while (r < r_end) {
                     const Inst<KW> in = rw.next();
-                    // pass = top of h * P; the group comes from the fraction below it
-                    const uint64_t hp = uint64_t(pass_hash<KW>(in.key)) * P;
+                    // pass = top of h * P; the group comes from the fraction
+                    // below it, the group stride from the low bits of h
+                    const uint32_t h = pass_hash<KW>(in.key);
+                    const uint64_t hp = uint64_t(h) * P;
                     const uint32_t q = uint32_t(hp >> 32);
                     const uint32_t g0 = uint32_t((uint64_t(uint32_t(hp)) * NG) >> 32);
+                    const uint32_t sg = group_stride<NG>(h);
                     if (all) {
                         spill_push_pass<KW>(q, nbits, in.key,
                                             in.front | (in.back << 3) | (in.disc << 6) | (min(m, 15u) << 9) |
-                                                (g0 << 13),
+                                                (g0 << 13) | (sg << 25),
                                             R, a.spill_cap, c_sub, s_sn, &s_ovf);
                     } else if (q == p) {
-                        uint32_t g = g0;
-                        if (lds_upsert<KW, C>(s_key, s_ef, s_st, in.key, g, in.front, in.back, in.disc, m, my_ins) ==
-                            UPSERT_FULL)
+                        if (lds_upsert<KW, C>(s_key, s_ef, s_st, in.key, g0, sg, in.front, in.back, in.disc, m,
+                                              my_ins) == UPSERT_FULL)
                             s_ovf = 1;
                     }
                     if (rw.i == rw.n) {  // record done: account inserts, start the prefetched record
@@ -335,64 +387,65 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                 }
             }
             if (all) {
-                // Pass p over its own sub-list, one entry per lane, the
-                // lane's next entry loaded while this one is probed (a
-                // sub-list that overflowed has flagged the bucket: nothing
-                // past its capacity is read).  A wave's loop runs as long as
-                // its longest search, so a search that has not ended after
-                // DEFER_TRIPS group reads is set aside: the instance, with
-                // the group to go on from, is appended after the sub-list
-                // (its unused capacity), and the set-aside instances are
-                // searched after a barrier.  Counter adds commute, so the
-                // order of the adds never shows.  With no room left a search
-                // goes on in place.
+                // Pass p over its own sub-list (a sub-list that overflowed
+                // has flagged the bucket: nothing past its capacity is read).
+                // Each lane takes one group-read step of its current search
+                // per loop trip and, once the instance is added, goes on with
+                // the entry it prefetched, so a wave's trips follow the
+                // average search length, not its longest search.  The lanes
+                // that consumed their prefetched entry take the next ones
+                // from the sub-list together (one LDS atomic per wave, the
+                // entries consecutive).  Counter adds commute, so the order
+                // of the adds never shows.
                 const uint32_t n_in = min(s_sn[p], c_sub);
-                const uint32_t room = c_sub - n_in;
                 const uint64_t e0 = uint64_t(p) * c_sub;
-                const auto sweep = [&](uint32_t lo, uint32_t hi, uint32_t max_trips) {
-                    Kmer<KW> key_n;
-                    uint32_t meta_n = 0;
-                    if (lo + uint32_t(tid) < hi) {
-#pragma unroll
-                        for (int w = 0; w < KW; ++w) key_n.w[w] = R.key[uint64_t(w) * a.spill_cap + e0 + lo + tid];
-                        meta_n = R.meta[e0 + lo + tid];
-                    }
-                    uint32_t seen = 0;
-                    for (uint32_t e = lo + tid; e < hi; e += LWG) {
-                        const Kmer<KW> key = key_n;
-                        const uint32_t meta = meta_n;
-                        if (e + LWG < hi) {
-#pragma unroll
-                            for (int w = 0; w < KW; ++w)
-                                key_n.w[w] = R.key[uint64_t(w) * a.spill_cap + e0 + e + LWG];
-                            meta_n = R.meta[e0 + e + LWG];
-                        }
-                        uint32_t g = meta >> 13;
-                        const uint32_t fr = meta & 7u, bk = (meta >> 3) & 7u, ds = (meta >> 6) & 7u,
-                                       mm = (meta >> 9) & 15u;
-                        int r = lds_upsert<KW, C>(s_key, s_ef, s_st, key, g, fr, bk, ds, mm, my_ins, max_trips);
-                        if (r == UPSERT_DEFERRED) {
-                            const uint32_t d = atomicAdd(&s_ndef, 1u);
-                            if (d < room) {
-                                const uint64_t at = e0 + n_in + d;
-#pragma unroll
-                                for (int w = 0; w < KW; ++w) R.key[uint64_t(w) * a.spill_cap + at] = key.w[w];
-                                R.meta[at] = (meta & 0x1FFFu) | (g << 13);
-                            } else {
-                                r = lds_upsert<KW, C>(s_key, s_ef, s_st, key, g, fr, bk, ds, mm, my_ins);
-                            }
-                        }
-                        if (r == UPSERT_FULL) s_ovf = 1;
-                        if ((++seen & 7u) == 0) {
-                            flush();
-                            if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
-                        }
-                    }
-                    flush();
+                // entries for the lanes with `want` (all lanes of the wave call this)
+                const auto take = [&](bool want) -> uint32_t {
+                    const unsigned long long mw = __ballot(want);
+                    uint32_t base = 0;
+                    if (lane == 0 && mw) base = atomicAdd(&s_next, uint32_t(__popcll(mw)));
+                    base = uint32_t(__builtin_amdgcn_readfirstlane(int(base)));
+                    return base + __builtin_amdgcn_mbcnt_hi(uint32_t(mw >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mw), 0u));
                 };
-                sweep(0, n_in, DEFER_TRIPS);
-                __syncthreads();  // every set-aside instance written
-                if (!s_ovf) sweep(n_in, n_in + min(s_ndef, room), ~0u);
+                Kmer<KW> key_n, key;
+                uint32_t meta_n = 0, meta = 0, g = 0, n_full = 0;
+                bool pre = false, have = false;  // a lane with neither has no entries left to take
+                const auto fetch = [&](uint32_t e) {  // prefetch entry e, if it exists
+                    pre = e < n_in;
+                    if (pre) {
+#pragma unroll
+                        for (int w = 0; w < KW; ++w) key_n.w[w] = R.key[uint64_t(w) * a.spill_cap + e0 + e];
+                        meta_n = R.meta[e0 + e];
+                    }
+                };
+                fetch(take(true));
+                for (uint32_t trip = 1;; ++trip) {
+                    bool want = false;
+                    if (!have && pre) {  // start the prefetched instance, prefetch the next
+                        key = key_n;
+                        meta = meta_n;
+                        g = (meta >> 13) & 0xFFFu;
+                        n_full = 0;
+                        have = true;
+                        pre = false;
+                        want = true;
+                    }
+                    const uint32_t e = take(want);
+                    if (want) fetch(e);
+                    if (!__ballot(have)) break;  // every entry handed out, and this wave's are added
+                    if (have) {
+                        const int r = upsert_step<KW, C>(s_key, s_ef, s_st, key, g, meta >> 25, n_full, meta & 7u,
+                                                         (meta >> 3) & 7u, (meta >> 6) & 7u, (meta >> 9) & 15u,
+                                                         my_ins);
+                        if (r != STEP_AGAIN) have = false;
+                        if (r == STEP_FULL) s_ovf = 1;
+                    }
+                    if ((trip & 7u) == 0) {
+                        flush();
+                        if (__hip_atomic_load(&s_ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+                    }
+                }
+                flush();
             }
             __syncthreads();
             if (s_ovf) {
