@@ -20,7 +20,9 @@
 // reverse complement are rolled one base per instance), so no staging
 // barriers or per-instance searches are needed and the LDS holds only the
 // table: u64 key words, the u32 Edge_Frequency word and one Vertex_Status byte
-// per slot.
+// per slot.  The key words are split: the top words of all slots first (a
+// group's four top words are what every probe step reads), then the lower
+// words, slot by slot.
 //
 // The per-instance semantics are those of Subgraph::construct
 // (src/Subgraph.cpp:44-108); counters saturate exactly as
@@ -49,41 +51,25 @@ __device__ __forceinline__ uint32_t pass_hash(const Kmer<KW>& x) {
 constexpr int UPSERT_FULL = -1;  // every group is full without the key
 
 // Group probe order of a key: its home group g0, then g0 + s, g0 + 2s, ...
-// modulo the NG groups, for an odd stride s < 128 coprime to NG (so the order
-// visits every group) taken from 6 hash bits.  Keys that share a home group
-// mostly go on with different strides, so the full groups do not gather into
-// the runs that every later search through them must cross (the clusters of
-// a probe by consecutive groups).
+// modulo the NG groups, for an odd stride s < 64 coprime to NG (so the order
+// visits every group) taken from 5 hash bits; the few odd values that share a
+// factor with NG are replaced by 1.  Keys that share a home group mostly go
+// on with different strides, so the full groups do not gather into the runs
+// that every later search through them must cross (the clusters of a probe by
+// consecutive groups).
 constexpr uint32_t gcd_u32(uint32_t a, uint32_t b) { return b ? gcd_u32(b, a % b) : a; }
-constexpr uint64_t stride_bad_mask(uint32_t ng, uint32_t half) {  // bit i: 64 half + i shares a factor with ng
+constexpr uint64_t stride_bad_mask(uint32_t ng) {  // bit i: i shares a factor with ng
     uint64_t m = 0;
     for (uint32_t i = 0; i < 64; ++i)
-        if (gcd_u32(64 * half + i, ng) != 1) m |= 1ull << i;
+        if (gcd_u32(i, ng) != 1) m |= 1ull << i;
     return m;
-}
-constexpr uint32_t STRIDE_BUMPS = 3;
-constexpr uint32_t stride_of(uint32_t ng, uint32_t x) {
-    uint32_t s = 2 * x + 1;
-    for (uint32_t i = 0; i < STRIDE_BUMPS; ++i)
-        if (gcd_u32(s, ng) != 1) s += 2;
-    return s;
-}
-constexpr bool strides_ok(uint32_t ng) {  // every stride coprime, below 128 and below ng
-    for (uint32_t x = 0; x < 64; ++x)
-        if (gcd_u32(stride_of(ng, x), ng) != 1 || stride_of(ng, x) >= 128 || stride_of(ng, x) >= ng) return false;
-    return true;
 }
 template <uint32_t NG>
 __device__ __forceinline__ uint32_t group_stride(uint32_t h) {
-    static_assert(strides_ok(NG), "group strides");
-    constexpr uint64_t B0 = stride_bad_mask(NG, 0), B1 = stride_bad_mask(NG, 1);
-    uint32_t s = 2 * (h & 63u) + 1;
-#pragma unroll
-    for (uint32_t i = 0; i < STRIDE_BUMPS; ++i) {
-        const bool bad = s < 64 ? (B0 >> s) & 1u : (B1 >> (s - 64)) & 1u;
-        s += bad ? 2u : 0u;
-    }
-    return s;
+    static_assert(NG > 64, "strides below the group count");
+    constexpr uint64_t BAD = stride_bad_mask(NG);
+    const uint32_t s = 2 * (h & 31u) + 1;
+    return (BAD >> s) & 1u ? 1u : s;
 }
 
 // Add one k-mer instance to an LDS table of C slots (KW u64 words each): find
@@ -112,6 +98,21 @@ __device__ __forceinline__ uint32_t group_stride(uint32_t h) {
 // without the key, and stays so, so a search that goes on from g adds the
 // instance where a search from its home group would.
 constexpr int STEP_DONE = 0, STEP_AGAIN = 1, STEP_FULL = 2;
+// Slot j (0..3, the group's fill order) of group g is slot 4g + ((j + r) & 3)
+// for r = group_rot(g): the lanes of a wave read the j-th top words of
+// unrelated groups in one instruction, and with the rotation those words fall
+// on LDS banks spread over the whole bank set (unrotated, every group's j-th
+// top word would sit at the same offset modulo 64 bytes: a few banks, many
+// conflicts).
+__device__ __forceinline__ uint32_t group_rot(uint32_t g) { return ((g >> 2) + (g >> 3)) & 3u; }
+
+// Key words of slot s in a table of C slots: the top word at s_key[s], word w
+// < KW - 1 at s_key[C + s (KW - 1) + w].
+template <int KW, int C>
+__device__ __forceinline__ uint64_t* low_word(uint64_t* s_key, uint32_t s, int w) {
+    return &s_key[C + s * uint32_t(KW - 1) + uint32_t(w)];
+}
+
 template <int KW, int C>
 __device__ __forceinline__ int upsert_step(uint64_t* s_key, uint32_t* s_ef, uint32_t* s_st, const Kmer<KW>& key,
                                            uint32_t& g, uint32_t stride, uint32_t& n_full, uint32_t front,
@@ -119,10 +120,12 @@ __device__ __forceinline__ int upsert_step(uint64_t* s_key, uint32_t* s_ef, uint
     constexpr uint32_t NG = uint32_t(C) / 4;
     const uint64_t top = key.w[KW - 1];
     {
-        uint64_t cur[4];
+        const uint32_t rot = group_rot(g);
+        uint64_t cur[4];  // in the group's fill order
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-            cur[j] = __hip_atomic_load(&s_key[(g * 4 + j) * KW + KW - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            cur[j] = __hip_atomic_load(&s_key[g * 4 + ((uint32_t(j) + rot) & 3u)], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
         const int empty = cur[0] == EMPTY ? 0 : cur[1] == EMPTY ? 1 : cur[2] == EMPTY ? 2 : cur[3] == EMPTY ? 3 : 4;
         uint32_t cand = 0;  // published slots whose top word matches
         bool busy = false;  // a matching top word is still being published
@@ -134,14 +137,14 @@ __device__ __forceinline__ int upsert_step(uint64_t* s_key, uint32_t* s_ef, uint
         int hit = -1;
         uint32_t old = 0;
         while (cand) {
-            const uint32_t slot = g * 4 + uint32_t(__builtin_ctz(cand));
+            const uint32_t slot = g * 4 + ((uint32_t(__builtin_ctz(cand)) + rot) & 3u);
             cand &= cand - 1;
             old = __hip_atomic_load(&s_ef[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             bool eq = true;
 #pragma unroll
             for (int w = 0; w < KW - 1; ++w)
-                eq &= __hip_atomic_load(&s_key[slot * KW + w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) ==
-                      key.w[w];
+                eq &= __hip_atomic_load(low_word<KW, C>(s_key, slot, w), __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_WORKGROUP) == key.w[w];
             if (eq) {
                 hit = int(slot);
                 break;
@@ -165,16 +168,17 @@ __device__ __forceinline__ int upsert_step(uint64_t* s_key, uint32_t* s_ef, uint
             g = g >= NG ? g - NG : g;
             return ++n_full == NG ? STEP_FULL : STEP_AGAIN;
         }
-        const uint32_t slot = g * 4 + uint32_t(empty);
-        uint64_t* kp = &s_key[slot * KW];
-        if (atomicCAS(reinterpret_cast<unsigned long long*>(kp + KW - 1), EMPTY, top | BUSY) == EMPTY) {
+        const uint32_t slot = g * 4 + ((uint32_t(empty) + rot) & 3u);
+        uint64_t* kp = &s_key[slot];
+        if (atomicCAS(reinterpret_cast<unsigned long long*>(kp), EMPTY, top | BUSY) == EMPTY) {
 #pragma unroll
             for (int w = 0; w < KW - 1; ++w)
-                __hip_atomic_store(kp + w, key.w[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_store(low_word<KW, C>(s_key, slot, w), key.w[w], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
             __hip_atomic_store(&s_ef[slot], ef_add(0u, uint8_t(front), uint8_t(back), m), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WORKGROUP);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // LDS table: the workgroup is the only observer
-            __hip_atomic_store(kp + KW - 1, top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(kp, top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (disc) atomicOr(&s_st[slot >> 2], disc << (8 * (slot & 3)));
             ++my_ins;
             return STEP_DONE;
@@ -199,7 +203,7 @@ __device__ __forceinline__ int lds_upsert(uint64_t* s_key, uint32_t* s_ef, uint3
 // word w of entry e is at key[w * spill_cap + e]; meta[e] = front | back << 3
 // | disc << 6 | min(m, 15) << 9 | group << 13 | stride << 25 (counters
 // saturate at 15, so adding min(m, 15) equals adding m; the group of the pass
-// table, below 2^12, and the group stride, below 2^7, are computed once by
+// table, below 2^12, and the group stride, below 2^6, are computed once by
 // the walk).
 struct SpillRegion {
     uint64_t* key;
@@ -259,7 +263,7 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
     constexpr int C = lds_capacity(KW);
     static_assert(C % 64 == 0, "capacity");
     static_assert(C / 4 <= (1 << 12), "a group index fits the spill meta word");
-    __shared__ uint64_t s_key[C * KW];
+    __shared__ uint64_t s_key[C * KW];  // top words, then lower words (low_word)
     __shared__ uint32_t s_ef[C];
     __shared__ uint32_t s_st[C / 4];  // one Vertex_Status byte per slot
     __shared__ uint32_t s_item, s_cnt, s_ovf, s_base, s_wp;
@@ -299,7 +303,7 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
 
         for (uint32_t p = p_lo; p <= p_hi; ++p) {
             if (!(all && p == p_lo)) {  // the all-pass walk needs no table
-                for (int i = tid; i < C * KW; i += LWG) s_key[i] = EMPTY;
+                for (int i = tid; i < C; i += LWG) s_key[i] = EMPTY;  // top words (lower ones are written at a claim)
                 for (int i = tid; i < C; i += LWG) s_ef[i] = 0u;
                 for (int i = tid; i < C / 4; i += LWG) s_st[i] = 0u;
             }
@@ -380,7 +384,7 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                     // cleared here instead).
                     __syncthreads();
                     if (s_ovf) break;
-                    for (int i = tid; i < C * KW; i += LWG) s_key[i] = EMPTY;
+                    for (int i = tid; i < C; i += LWG) s_key[i] = EMPTY;
                     for (int i = tid; i < C; i += LWG) s_ef[i] = 0u;
                     for (int i = tid; i < C / 4; i += LWG) s_st[i] = 0u;
                     __syncthreads();
@@ -468,7 +472,7 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
             typedef __attribute__((address_space(1))) uint64_t g_u64;
             g_u64* tbl = reinterpret_cast<g_u64*>(a.tbl_ptr[b]) + uint64_t(s_base) * (KW + 1);
             for (int i = tid; i < C; i += LWG) {  // C % 64 == 0: a wave's slots are all below C or none
-                const bool occ = s_key[i * KW + KW - 1] != EMPTY;
+                const bool occ = s_key[i] != EMPTY;
                 const uint32_t ef = s_ef[i];
                 const uint32_t st = (s_st[i >> 2] >> (8 * (i & 3))) & 0xFFu;
                 const bool act = occ && ef_active(ef, a.cutoff);
@@ -490,7 +494,8 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                 const uint32_t pos = act ? ra + r : n_run - 1 - (ri + r);
                 g_u64* e = tbl + uint64_t(pos) * (KW + 1);
 #pragma unroll
-                for (int w = 0; w < KW; ++w) e[w] = s_key[i * KW + w];
+                for (int w = 0; w < KW - 1; ++w) e[w] = *low_word<KW, C>(s_key, uint32_t(i), w);
+                e[KW - 1] = s_key[i];
                 e[KW] = uint64_t(ef) | (uint64_t(st) << 32);
             }
             __syncthreads();  // table read out and counters used before the next pass clears them

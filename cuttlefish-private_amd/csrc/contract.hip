@@ -175,6 +175,28 @@ __device__ __forceinline__ int tab_find(const BigTab<KW>& t, const Kmer<KW>& x) 
     return -1;
 }
 
+// Lookup of a key known to be in the index (a neighbour through an edge at
+// the cutoff: the edge is counted at both endpoints, so the neighbour is not
+// isolated and was indexed): the probe chain is read to its end, and when one
+// entry alone carries the key's fingerprint it is the key's, with no read of
+// the vertex's key words.  Otherwise the keys are compared (tab_find).
+template <int KW>
+__device__ __forceinline__ int tab_find_present(const BigTab<KW>& t, const Kmer<KW>& x) {
+    const uint64_t h = km_hash(x);
+    const uint32_t fp = uint32_t(h) & 0xFFFF0000u;
+    uint32_t s = slot_of(h, CB_HCAP), n_fp = 0, d = 0;
+    for (uint32_t n = 0; n < CB_HCAP; ++n) {
+        const uint32_t e = t.ht[s];
+        if (e == HT_FREE) break;
+        if ((e & 0xFFFF0000u) == fp) {
+            d = e & 0xFFFFu;
+            if (++n_fp > 1) break;
+        }
+        s = s + 1 == CB_HCAP ? 0 : s + 1;
+    }
+    return n_fp == 1 ? int(d) : tab_find(t, x);
+}
+
 template <int KW>
 __device__ __forceinline__ int walk_next(const BigTab<KW>& t, uint32_t cur, uint32_t m, uint32_t k, Kmer<KW>& z,
                                          Kmer<KW>& zb, uint32_t& side, uint32_t& b) {
@@ -244,6 +266,13 @@ __device__ __forceinline__ int tab_find(const WideTab<KW>& t, const Kmer<KW>& x)
     return -1;
 }
 
+// The wide tier's index is in global memory: reading a probe chain to its end
+// costs more there than the key words of the first fingerprint match.
+template <int KW>
+__device__ __forceinline__ int tab_find_present(const WideTab<KW>& t, const Kmer<KW>& x) {
+    return tab_find(t, x);
+}
+
 template <int KW>
 __device__ __forceinline__ int walk_next(const WideTab<KW>& t, uint32_t cur, uint32_t m, uint32_t k, Kmer<KW>& z,
                                          Kmer<KW>& zb, uint32_t& side, uint32_t& b) {
@@ -283,6 +312,26 @@ __device__ __forceinline__ int walk_step(const Tab& t, uint32_t m, uint32_t k, K
     const bool can = km_less(z, zb);
     side = can ? 1u : 0u;  // entrance side front (canonical) -> leave through the back
     return tab_find(t, can ? z : zb);
+}
+
+// Tables whose keys are compared at every lookup.
+template <int KW>
+__device__ __forceinline__ int tab_find_present(const CtTab<KW>& t, const Kmer<KW>& x) {
+    return tab_find(t, x);
+}
+
+// walk_step for the link pass: the neighbour through an edge at the cutoff is
+// in the table (tab_find_present).
+template <int KW, class Tab>
+__device__ __forceinline__ int link_step(const Tab& t, uint32_t m, uint32_t k, Kmer<KW>& z, Kmer<KW>& zb,
+                                         uint32_t& side, uint32_t& b) {
+    const uint32_t e = side_edge(m, side);
+    b = side == 1 ? e : 3u - e;
+    z = km_append(z, b, k);
+    zb = km_prepend(zb, uint8_t(3u - b), k);
+    const bool can = km_less(z, zb);
+    side = can ? 1u : 0u;
+    return tab_find_present(t, can ? z : zb);
 }
 
 // One step of a walk that knows its current slot: the lookup of walk_step,
@@ -437,7 +486,7 @@ __device__ __forceinline__ int link_of(const Tab& t, uint32_t i, uint32_t s, con
     if (side_edge(m, s) >= 4) return -1;
     Kmer<KW> z = s ? x : xr, zb = s ? xr : x;
     uint32_t side = s, bb;
-    const int j = walk_step<KW>(t, m, k, z, zb, side, bb);
+    const int j = link_step<KW>(t, m, k, z, zb, side, bb);
     if (j < 0) {
         *err = 1;
         return -1;

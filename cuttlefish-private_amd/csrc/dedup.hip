@@ -25,6 +25,18 @@ namespace {
 constexpr uint64_t EMPTY = ~0ull;
 constexpr uint64_t BUSY = 1ull << 63;
 
+// Table layout: the top words of all C slots first, then the lower KR - 1
+// words slot by slot (dd_low).  Slot j (0..3, the fill order) of group g is
+// slot 4g + ((j + r) & 3) for r = dd_rot(g), so that the j-th top words of the
+// unrelated groups a wave probes in one instruction fall on LDS banks spread
+// over the whole bank set (at a fixed offset modulo the group size they would
+// share a few banks).
+__device__ __forceinline__ uint32_t dd_rot(uint32_t g) { return ((g >> 2) + (g >> 3)) & 3u; }
+template <int KR, int C>
+__device__ __forceinline__ uint64_t* dd_low(uint64_t* s_key, uint32_t s, int w) {
+    return &s_key[C + s * uint32_t(KR - 1) + uint32_t(w)];
+}
+
 // Find or insert a KR-word key (word KR-1 is the top word, < 2^63) in an LDS
 // table of C slots by 4-slot group probing; the claim protocol of the
 // construct tables (top word claimed with BUSY, lower words written, top word
@@ -36,10 +48,12 @@ __device__ __forceinline__ int dd_find_or_insert(uint64_t* s_key, const uint64_t
     const uint64_t top = key[KR - 1];
     inserted = false;
     for (uint32_t n = 0; n < NG;) {
-        uint64_t cur[4];
+        const uint32_t rot = dd_rot(g);
+        uint64_t cur[4];  // in the group's fill order
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-            cur[j] = __hip_atomic_load(&s_key[(g * 4 + j) * KR + KR - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            cur[j] = __hip_atomic_load(&s_key[g * 4 + ((uint32_t(j) + rot) & 3u)], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
         const int empty = cur[0] == EMPTY ? 0 : cur[1] == EMPTY ? 1 : cur[2] == EMPTY ? 2 : cur[3] == EMPTY ? 3 : 4;
         uint32_t cand = 0;  // published slots whose top word matches (scanned without branches)
         bool busy = false;  // a matching top word is still being published
@@ -50,14 +64,14 @@ __device__ __forceinline__ int dd_find_or_insert(uint64_t* s_key, const uint64_t
             busy |= c && (cur[j] & BUSY);
         }
         while (cand) {
-            const uint32_t j = uint32_t(__builtin_ctz(cand));
+            const uint32_t slot = g * 4 + ((uint32_t(__builtin_ctz(cand)) + rot) & 3u);
             cand &= cand - 1;
-            const uint64_t* kp = &s_key[(g * 4 + j) * KR];
             bool eq = true;
 #pragma unroll
             for (int w = 0; w < KR - 1; ++w)
-                eq &= __hip_atomic_load(kp + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == key[w];
-            if (eq) return int(g * 4 + j);
+                eq &= __hip_atomic_load(dd_low<KR, C>(s_key, slot, w), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) ==
+                      key[w];
+            if (eq) return int(slot);
         }
         if (busy) continue;
         if (empty == 4) {
@@ -65,14 +79,15 @@ __device__ __forceinline__ int dd_find_or_insert(uint64_t* s_key, const uint64_t
             ++n;
             continue;
         }
-        const uint32_t slot = g * 4 + uint32_t(empty);
-        uint64_t* kp = &s_key[slot * KR];
-        if (atomicCAS(reinterpret_cast<unsigned long long*>(kp + KR - 1), EMPTY, top | BUSY) == EMPTY) {
+        const uint32_t slot = g * 4 + ((uint32_t(empty) + rot) & 3u);
+        uint64_t* kp = &s_key[slot];
+        if (atomicCAS(reinterpret_cast<unsigned long long*>(kp), EMPTY, top | BUSY) == EMPTY) {
 #pragma unroll
             for (int w = 0; w < KR - 1; ++w)
-                __hip_atomic_store(kp + w, key[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_store(dd_low<KR, C>(s_key, slot, w), key[w], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // LDS table: the workgroup is the only observer
-            __hip_atomic_store(kp + KR - 1, top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(kp, top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             inserted = true;
             return int(slot);
         }
@@ -127,7 +142,7 @@ __global__ __launch_bounds__(DD_WG) void k_dedup(DedupArgs a) {
     constexpr int KR = PK ? W : W + 1;  // PK: len, disc bits and hash packed into the last label word
     constexpr int C = dedup_capacity(KR);
     static_assert(C % 64 == 0, "capacity");
-    __shared__ uint64_t s_key[C * KR];
+    __shared__ uint64_t s_key[C * KR];  // top words, then lower words (dd_low)
     __shared__ uint32_t s_cnt[C];
     __shared__ uint32_t s_item, s_n, s_ovf, s_base;
     __shared__ uint32_t s_len[256], s_wsum[4];  // per-len counts, then output positions
@@ -180,7 +195,7 @@ __global__ __launch_bounds__(DD_WG) void k_dedup(DedupArgs a) {
         }
         for (uint32_t p = all ? 0u : p_it; p <= (all ? P - 1 : p_it); ++p) {
             if (tid == 0) s_n = 0;
-            for (int i = tid; i < C * KR; i += DD_WG) s_key[i] = EMPTY;
+            for (int i = tid; i < C; i += DD_WG) s_key[i] = EMPTY;  // top words (lower ones are written at a claim)
             for (int i = tid; i < C; i += DD_WG) s_cnt[i] = 0;
             __syncthreads();
             if (s_ovf) break;
@@ -238,7 +253,7 @@ __global__ __launch_bounds__(DD_WG) void k_dedup(DedupArgs a) {
             if (tid < 256) s_len[tid] = 0;
             __syncthreads();
             for (int i = tid; i < C; i += DD_WG)
-                if (s_cnt[i]) atomicAdd(&s_len[s_key[i * KR + KR - 1] & 0xFFu], 1u);
+                if (s_cnt[i]) atomicAdd(&s_len[s_key[i] & 0xFFu], 1u);
             __syncthreads();
             // exclusive prefix over len = 255, 254, ..., 0 by the first 4 waves
             const int lane = tid & 63, wv = tid >> 6;
@@ -265,14 +280,15 @@ __global__ __launch_bounds__(DD_WG) void k_dedup(DedupArgs a) {
             for (int i = tid; i < C; i += DD_WG) {
                 const uint32_t c = s_cnt[i];
                 if (!c) continue;
-                const uint64_t* kp = &s_key[i * KR];
-                const uint64_t o = base + atomicAdd(&s_len[kp[KR - 1] & 0xFFu], 1u);
-                my_km += (kp[KR - 1] & 0xFFu) - (a.k - 1);
-                a.out_att[o] = uint32_t(kp[KR - 1] & 0x3FFu) | g16;
+                const uint64_t top = s_key[i];
+                const uint64_t* kp = dd_low<KR, C>(s_key, uint32_t(i), 0);  // the KR - 1 lower words
+                const uint64_t o = base + atomicAdd(&s_len[top & 0xFFu], 1u);
+                my_km += (top & 0xFFu) - (a.k - 1);
+                a.out_att[o] = uint32_t(top & 0x3FFu) | g16;
                 a.out_mult[o] = c;
 #pragma unroll
                 for (int q = 0; q < W; ++q)
-                    a.out_labels[o * W + q] = !PK ? kp[q] : q == 0 ? (kp[KR - 1] >> 32) << 33 : kp[q - 1];
+                    a.out_labels[o * W + q] = !PK ? (q < KR - 1 ? kp[q] : top) : q == 0 ? (top >> 32) << 33 : kp[q - 1];
             }
             if (my_km) atomicAdd(&s_km, my_km);
             __syncthreads();
