@@ -981,6 +981,32 @@ __global__ __launch_bounds__(CT_WG) void k_contract_big(ContractArgs a) {
             jp = src;
         }
 
+        // Path ends first: the end state F = (vertex L, side s) of a path
+        // with far end F' = (R, s') reads the path from F when E(F) <
+        // E(F'), for E(F) = the key of L, or its reverse complement when s =
+        // 1 (the first k-mer of the reading from F; E(F') is the reverse
+        // complement of its last).  Bit s of offv[L] keeps that choice (offv
+        // is free until the labels are reserved), so that the other vertices
+        // of the path need not read the keys of both its ends.
+        for (uint32_t d = tid; d < n; d += CT_WG) {
+            const uint32_t w0 = jp[2 * d], w1 = jp[2 * d + 1];
+            const bool e0 = w0 == cb_pack(2 * d, 0), e1 = w1 == cb_pack(2 * d + 1, 0);
+            uint32_t bits = 0;
+            if (e0 || e1) {
+                const Kmer<KW> x = tab_key<KW>(t, d), xr = km_rc(x, k);
+#pragma unroll
+                for (uint32_t sd = 0; sd < 2; ++sd) {
+                    if (!(sd ? e1 : e0)) continue;
+                    const uint32_t Fo = (sd ? w0 : w1) & CB_M16;  // reached through the other side
+                    const Kmer<KW> co = tab_key<KW>(t, Fo >> 1);
+                    const Kmer<KW> eo = (Fo & 1u) ? km_rc(co, k) : co;
+                    bits |= uint32_t(km_less(sd ? xr : x, eo)) << sd;
+                }
+            }
+            offv[d] = bits;
+        }
+        __syncthreads();
+
         // per vertex: the ends of its path, the output orientation and its position
         for (uint32_t d = tid; d < n; d += CT_WG) {
             const uint32_t wf = jp[2 * d], wb = jp[2 * d + 1];
@@ -994,10 +1020,7 @@ __global__ __launch_bounds__(CT_WG) void k_contract_big(ContractArgs a) {
             }
             const uint32_t df = wf >> 16, db = wb >> 16;
             const uint32_t L = Ff >> 1, sL = Ff & 1u, R = Fb >> 1, sR = Fb & 1u;
-            const Kmer<KW> cL = tab_key<KW>(t, L), cR = tab_key<KW>(t, R);
-            const Kmer<KW> first = sL == 0 ? cL : km_rc(cL, k);
-            const Kmer<KW> rlast = sR == 1 ? km_rc(cR, k) : cR;
-            const bool fwd = km_less(first, rlast);
+            const bool fwd = (offv[L] >> sL) & 1u;  // E(Ff) < E(Fb): Fb is Ff's far end
             const uint32_t pos = fwd ? df : db;
             const uint32_t A = fwd ? L : R;
             ia[d] = A | (uint32_t(fwd) << 16) | (1u << 17);
@@ -1273,6 +1296,26 @@ __global__ __launch_bounds__(CT_WG) void k_contract_wide(ContractArgs a) {
             jp = src;
         }
 
+        // path ends first: the reading of each end state, as in k_contract_big
+        for (uint32_t d = tid; d < n; d += CT_WG) {
+            const uint64_t w0 = jp[2 * d], w1 = jp[2 * d + 1];
+            const bool e0 = w0 == cw_pack(2 * d, 0), e1 = w1 == cw_pack(2 * d + 1, 0);
+            uint32_t bits = 0;
+            if (e0 || e1) {
+                const Kmer<KW> x = tab_key<KW>(t, d), xr = km_rc(x, k);
+#pragma unroll
+                for (uint32_t sd = 0; sd < 2; ++sd) {
+                    if (!(sd ? e1 : e0)) continue;
+                    const uint32_t Fo = uint32_t(sd ? w0 : w1);  // reached through the other side
+                    const Kmer<KW> co = tab_key<KW>(t, Fo >> 1);
+                    const Kmer<KW> eo = (Fo & 1u) ? km_rc(co, k) : co;
+                    bits |= uint32_t(km_less(sd ? xr : x, eo)) << sd;
+                }
+            }
+            offv[d] = bits;
+        }
+        __syncthreads();
+
         // per vertex: the ends of its path, the output orientation and its position
         for (uint32_t d = tid; d < n; d += CT_WG) {
             const uint64_t wf = jp[2 * d], wb = jp[2 * d + 1];
@@ -1286,10 +1329,7 @@ __global__ __launch_bounds__(CT_WG) void k_contract_wide(ContractArgs a) {
             }
             const uint32_t df = uint32_t(wf >> 32), db = uint32_t(wb >> 32);
             const uint32_t L = Ff >> 1, sL = Ff & 1u, R = Fb >> 1, sR = Fb & 1u;
-            const Kmer<KW> cL = tab_key<KW>(t, L), cR = tab_key<KW>(t, R);
-            const Kmer<KW> first = sL == 0 ? cL : km_rc(cL, k);
-            const Kmer<KW> rlast = sR == 1 ? km_rc(cR, k) : cR;
-            const bool fwd = km_less(first, rlast);
+            const bool fwd = (offv[L] >> sL) & 1u;  // E(Ff) < E(Fb): Fb is Ff's far end
             const uint32_t pos = fwd ? df : db;
             const uint32_t A = fwd ? L : R;
             ia[d] = A | (uint32_t(fwd) << 30) | (1u << 31);
