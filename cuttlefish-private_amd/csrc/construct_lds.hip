@@ -198,7 +198,8 @@ __device__ __forceinline__ int lds_upsert(uint64_t* s_key, uint32_t* s_ef, uint3
     }
 }
 
-// Spill region of a workgroup for one all-pass item: P sub-lists of c =
+// Spill region of a workgroup for one all-pass item (KW * spill_cap < 2^32
+// words: the host bounds the regions' bytes): P sub-lists of c =
 // spill_cap / P entries, sub-list q holding the instances of pass q.  Key
 // word w of entry e is at key[w * spill_cap + e]; meta[e] = front | back << 3
 // | disc << 6 | min(m, 15) << 9 | group << 13 | stride << 25 (counters
@@ -252,9 +253,9 @@ __device__ __forceinline__ void spill_push_pass(uint32_t q, uint32_t nbits, cons
         *s_ovf = 1;
         return;
     }
-    const uint64_t e = uint64_t(q) * c + idx;
+    const uint32_t e = q * c + idx;  // 32-bit offsets from the region's base: KW * cap < 2^32
 #pragma unroll
-    for (int w = 0; w < KW; ++w) R.key[uint64_t(w) * cap + e] = key.w[w];
+    for (int w = 0; w < KW; ++w) R.key[uint32_t(w) * cap + e] = key.w[w];
     R.meta[e] = meta;
 }
 
@@ -402,7 +403,7 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                 // entries consecutive).  Counter adds commute, so the order
                 // of the adds never shows.
                 const uint32_t n_in = min(s_sn[p], c_sub);
-                const uint64_t e0 = uint64_t(p) * c_sub;
+                const uint32_t e0 = p * c_sub;
                 // entries for the lanes with `want` (all lanes of the wave call this)
                 const auto take = [&](bool want) -> uint32_t {
                     const unsigned long long mw = __ballot(want);
@@ -418,7 +419,7 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                     pre = e < n_in;
                     if (pre) {
 #pragma unroll
-                        for (int w = 0; w < KW; ++w) key_n.w[w] = R.key[uint64_t(w) * a.spill_cap + e0 + e];
+                        for (int w = 0; w < KW; ++w) key_n.w[w] = R.key[uint32_t(w) * a.spill_cap + e0 + e];
                         meta_n = R.meta[e0 + e];
                     }
                 };
