@@ -420,7 +420,9 @@ uint64_t walked_kmers(const cfgpu_handle* h, uint32_t b) {
 // workgroups; otherwise one item per pass, each walking the records.
 constexpr uint64_t SPILL_BYTES_MAX = 16ull << 30;
 uint64_t spill_cap_max(const cfgpu_handle* h) {
-    return SPILL_BYTES_MAX / (uint64_t(std::max(h->cus, 1)) * (8 * h->KW + 4));
+    // entries are addressed with 32-bit word offsets within a region
+    return std::min<uint64_t>(SPILL_BYTES_MAX / (uint64_t(std::max(h->cus, 1)) * (8 * h->KW + 4)),
+                              0xFFFFFFFFull / (2 * h->KW + 1));
 }
 // Region entries an all-pass item of `inst` walked instances over P passes
 // needs: P sub-lists with 50 % headroom over the mean plus 8 standard
@@ -589,10 +591,8 @@ int run_round_batched(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(n_items, uint32_t(h->cus) * per_cu));
     if (lds && h->spill_need) {  // one region per workgroup, sized for the largest all-pass item
         const uint64_t cap = (h->spill_need + 63) & ~63ull;
-        HIPCK(h, ensure_mem(h, h->spill_key, uint64_t(grid) * cap * h->KW * 8, Phase::Construct));
-        HIPCK(h, ensure_mem(h, h->spill_meta, uint64_t(grid) * cap * 4, Phase::Construct));
+        HIPCK(h, ensure_mem(h, h->spill_key, uint64_t(grid) * cap * (h->KW * 8 + 4), Phase::Construct));
         a.spill_key = h->spill_key.as<uint64_t>();
-        a.spill_meta = h->spill_meta.as<uint32_t>();
         a.spill_cap = uint32_t(cap);
     }
     auto reset = [&]() {

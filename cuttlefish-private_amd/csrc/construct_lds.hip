@@ -198,23 +198,46 @@ __device__ __forceinline__ int lds_upsert(uint64_t* s_key, uint32_t* s_ef, uint3
     }
 }
 
-// Spill region of a workgroup for one all-pass item (KW * spill_cap < 2^32
-// words: the host bounds the regions' bytes): P sub-lists of c =
-// spill_cap / P entries, sub-list q holding the instances of pass q.  Key
-// word w of entry e is at key[w * spill_cap + e]; meta[e] = front | back << 3
-// | disc << 6 | min(m, 15) << 9 | group << 13 | stride << 25 (counters
-// saturate at 15, so adding min(m, 15) equals adding m; the group of the pass
-// table, below 2^12, and the group stride, below 2^6, are computed once by
-// the walk).
+// Spill region of a workgroup for one all-pass item ((2 KW + 1) spill_cap <
+// 2^32 words: the host bounds the regions' bytes): P sub-lists of c =
+// spill_cap / P entries, sub-list q holding the instances of pass q.  Entry e
+// is 2 KW + 1 consecutive u32 words at 4 (2 KW + 1) e bytes: the KW key words
+// (low half first), then meta = front | back << 3 | disc << 6 | min(m, 15) <<
+// 9 | group << 13 | stride << 25 (counters saturate at 15, so adding min(m,
+// 15) equals adding m; the group of the pass table, below 2^12, and the group
+// stride, below 2^6, are computed once by the walk).  A lane's entry is one
+// contiguous span, so a wave's appends to a few sub-lists touch few lines.
+template <int KW>
+struct SpillEnt {
+    uint32_t d[2 * KW + 1];
+};
 struct SpillRegion {
-    uint64_t* key;
-    uint32_t* meta;
+    uint32_t* base;
 };
 
 template <int KW>
 __device__ __forceinline__ SpillRegion spill_region(const ConstructArgs& a) {
-    const uint64_t cap = a.spill_cap;
-    return {a.spill_key + uint64_t(blockIdx.x) * KW * cap, a.spill_meta + uint64_t(blockIdx.x) * cap};
+    return {reinterpret_cast<uint32_t*>(a.spill_key) + uint64_t(blockIdx.x) * (2 * KW + 1) * a.spill_cap};
+}
+
+template <int KW>
+__device__ __forceinline__ void spill_store(const SpillRegion& R, uint32_t e, const Kmer<KW>& key, uint32_t meta) {
+    SpillEnt<KW> v;
+#pragma unroll
+    for (int w = 0; w < KW; ++w) {
+        v.d[2 * w] = uint32_t(key.w[w]);
+        v.d[2 * w + 1] = uint32_t(key.w[w] >> 32);
+    }
+    v.d[2 * KW] = meta;
+    *reinterpret_cast<SpillEnt<KW>*>(R.base + e * uint32_t(2 * KW + 1)) = v;
+}
+
+template <int KW>
+__device__ __forceinline__ void spill_load(const SpillRegion& R, uint32_t e, Kmer<KW>& key, uint32_t& meta) {
+    const SpillEnt<KW> v = *reinterpret_cast<const SpillEnt<KW>*>(R.base + e * uint32_t(2 * KW + 1));
+#pragma unroll
+    for (int w = 0; w < KW; ++w) key.w[w] = uint64_t(v.d[2 * w]) | (uint64_t(v.d[2 * w + 1]) << 32);
+    meta = v.d[2 * KW];
 }
 
 // Append the calling lanes' instances to their passes' sub-lists.  With more
@@ -253,10 +276,7 @@ __device__ __forceinline__ void spill_push_pass(uint32_t q, uint32_t nbits, cons
         *s_ovf = 1;
         return;
     }
-    const uint32_t e = q * c + idx;  // 32-bit offsets from the region's base: KW * cap < 2^32
-#pragma unroll
-    for (int w = 0; w < KW; ++w) R.key[uint32_t(w) * cap + e] = key.w[w];
-    R.meta[e] = meta;
+    spill_store<KW>(R, q * c + idx, key, meta);
 }
 
 template <int KW, int WMAX>
@@ -417,11 +437,7 @@ __global__ __launch_bounds__(LWG) void k_construct_lds(ConstructArgs a) {
                 bool pre = false, have = false;  // a lane with neither has no entries left to take
                 const auto fetch = [&](uint32_t e) {  // prefetch entry e, if it exists
                     pre = e < n_in;
-                    if (pre) {
-#pragma unroll
-                        for (int w = 0; w < KW; ++w) key_n.w[w] = R.key[uint32_t(w) * a.spill_cap + e0 + e];
-                        meta_n = R.meta[e0 + e];
-                    }
+                    if (pre) spill_load<KW>(R, e0 + e, key_n, meta_n);
                 };
                 fetch(take(true));
                 for (uint32_t trip = 1;; ++trip) {
