@@ -116,9 +116,9 @@ struct ConstructArgs {
     uint32_t* queue_head;       // persistent mode: next item
     const uint32_t* mult;       // per record multiplicity (null: every record counts once)
     const uint64_t* bucket_end; // per bucket end of its records (null: bucket_off[b + 1])
-    uint64_t* spill_key;        // LDS kernel, all-pass items: per workgroup a region of
-    uint32_t* spill_meta;       //   spill_cap entries (KW key words + one meta word each),
-    uint32_t spill_cap;         //   P sub-lists of spill_cap / P entries for a P-pass item
+    uint64_t* spill_key;        // LDS kernel, all-pass items: per workgroup a region of spill_cap
+    uint32_t spill_cap;         //   entries (KW key words + one meta word each), P sub-lists of
+                                //   spill_cap / P entries for a P-pass item
     uint32_t cutoff;            // LDS kernel: edge count threshold of the active / isolated split
     uint64_t* seg;              // LDS kernel: per bucket SEG_MAX pass runs (start | active << 32)
     uint32_t* iso;              // LDS kernel: per bucket isolated and phantom vertex counts

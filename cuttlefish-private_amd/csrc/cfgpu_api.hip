@@ -157,7 +157,7 @@ struct cfgpu_handle {
     DevBuf d_seg, d_iso, d_nseg;  // LDS-built tables: pass runs, isolated / phantom counts (contract reads)
     std::vector<std::unique_ptr<DevBuf>> pools;  // one per sizing round, reused across calls
     uint32_t round_idx = 0;
-    DevBuf spill_key, spill_meta;  // LDS construct: per-workgroup spill regions of all-pass items
+    DevBuf spill_key;  // LDS construct: per-workgroup spill regions of all-pass items
     uint64_t spill_need = 0;       // region entries the largest all-pass item of the round needs
     uint32_t n_all_pass = 0;       // all-pass items of the round
     uint8_t* stage = nullptr;      // pinned staging of small copies (h2d, d2h); reused after sync_stream
@@ -305,7 +305,7 @@ enum class Phase { Input, Partition, Construct, Contract, Cdbg };
 void trim(cfgpu_handle* h, Phase ph) {
     if (ph == Phase::Input) {
         for (auto& pool : h->pools) pool->release();
-        for (DevBuf* b : {&h->ex_keys, &h->ex_ef, &h->ex_st, &h->ex_dfa, &h->spill_key, &h->spill_meta, &h->dd_att,
+        for (DevBuf* b : {&h->ex_keys, &h->ex_ef, &h->ex_st, &h->ex_dfa, &h->spill_key, &h->dd_att,
                           &h->dd_lab, &h->dd_mult, &h->dd_region, &h->ct_scratch, &h->ct_recs, &h->ct_labels,
                           &h->cd_settled, &h->cd_partner, &h->cd_bhist, &h->cd_bkey, &h->cd_bid, &h->cd_kend,
                           &h->cd_jp, &h->cd_head, &h->cd_pos, &h->cd_hoff, &h->cd_out, &h->cd_rec})
@@ -322,7 +322,7 @@ void trim(cfgpu_handle* h, Phase ph) {
                           &h->cd_pos, &h->cd_hoff, &h->cd_out, &h->cd_rec})
             b->release();
     if (ph != Phase::Construct)
-        for (DevBuf* b : {&h->spill_key, &h->spill_meta, &h->dd_att, &h->dd_lab, &h->dd_mult, &h->dd_region}) b->release();
+        for (DevBuf* b : {&h->spill_key, &h->dd_att, &h->dd_lab, &h->dd_mult, &h->dd_region}) b->release();
     if (ph == Phase::Cdbg) h->ct_scratch.release();
 }
 
@@ -622,7 +622,7 @@ int run_round_batched(cfgpu_handle* h, const std::vector<uint32_t>& buckets,
     const void* ptrs[] = {pool->p, a.att, a.labels, a.mult, a.bucket_end, h->boff.p, h->d_tbl_ptr.p, h->d_cap.p,
                           h->d_limit.p, h->d_nvert.p, h->d_overflow.p, h->d_items_b.p,
                           h->d_items_f.p, h->d_items_c.p, h->d_qhead.p, h->d_round_list.p,
-                          a.spill_key, a.spill_meta, a.seg, a.iso};
+                          a.spill_key, a.seg, a.iso};
     std::vector<uint64_t> key = {n_slots, n_items, grid, h->KW, h->W, h->prm.k, h->tune.workgroup,
                                  buckets.size(), uint64_t(lds), a.spill_cap};
     for (const void* q : ptrs) key.push_back(reinterpret_cast<uint64_t>(q));
