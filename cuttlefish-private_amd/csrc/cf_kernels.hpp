@@ -225,7 +225,7 @@ constexpr int CT_WG = 1024;
 constexpr int ct_lds_capacity(int KW) { return KW == 1 ? 7360 : KW == 2 ? 5248 : KW == 3 ? 4096 : 3328; }
 // Dense tables (non-isolated vertices beyond an LDS table): at most this many
 // vertices, and per workgroup this many scratch bytes.
-__host__ __device__ constexpr uint32_t ct_big_max_vertices() { return 36864 * 4 / 5; }
+__host__ __device__ constexpr uint32_t ct_big_max_vertices() { return 32767; }
 __host__ __device__ constexpr uint64_t ct_big_scratch_bytes(uint32_t KW) {
     return uint64_t(ct_big_max_vertices()) * (8 * KW + 4 + 16 + 16);
 }

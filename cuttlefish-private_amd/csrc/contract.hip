@@ -131,8 +131,8 @@ __device__ __forceinline__ int tab_find(const CtTab<KW>& t, const Kmer<KW>& x) {
 // key words at key[d * KW ..] and its state word at meta[d]; the LDS hash
 // `ht` maps a key to d (entry d | 16 fingerprint bits of the key's hash << 16,
 // all-ones = free), so a lookup reads the one vertex whose fingerprint matches.
-constexpr uint32_t CB_HCAP = 36864;            // LDS index slots (u32 each)
-constexpr uint32_t CB_NMAX = CB_HCAP * 4 / 5;  // vertices an index takes (load <= 0.8)
+constexpr uint32_t CB_HCAP = 36864;  // LDS index slots (u32 each)
+constexpr uint32_t CB_NMAX = 32767;  // vertices an index takes (load <= 0.89): directed states fit 16 bits
 static_assert(CB_NMAX == uint32_t(ct_big_max_vertices()), "host and device agree on the dense table size");
 static_assert(2 * CB_NMAX < (1u << 16), "directed states fit 16 bits");
 constexpr uint32_t HT_FREE = 0xFFFFFFFFu;
@@ -179,9 +179,12 @@ __device__ __forceinline__ int tab_find(const BigTab<KW>& t, const Kmer<KW>& x) 
 // the cutoff: the edge is counted at both endpoints, so the neighbour is not
 // isolated and was indexed): the probe chain is read to its end, and when one
 // entry alone carries the key's fingerprint it is the key's, with no read of
-// the vertex's key words.  Otherwise the keys are compared (tab_find).
+// the vertex's key words.  Otherwise the keys are compared (tab_find), and
+// always in an index loaded past 0.7, whose chains run long.
+constexpr uint32_t CB_PRESENT_MAX = CB_HCAP * 7 / 10;
 template <int KW>
 __device__ __forceinline__ int tab_find_present(const BigTab<KW>& t, const Kmer<KW>& x) {
+    if (t.cap > CB_PRESENT_MAX) return tab_find(t, x);
     const uint64_t h = km_hash(x);
     const uint32_t fp = uint32_t(h) & 0xFFFF0000u;
     uint32_t s = slot_of(h, CB_HCAP), n_fp = 0, d = 0;
