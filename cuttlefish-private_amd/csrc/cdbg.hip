@@ -289,9 +289,8 @@ __global__ __launch_bounds__(1024) void k_ends_pair(CdbgArgs a) {
 }
 
 // State 2r + e: piece r entered at end e.  Its successor is the piece across
-// the vertex at r's other end, entered at its matching end.
-// State st has no partner across the end it leaves through: a chain end.
-__device__ __forceinline__ bool rank_end(const CdbgArgs& a, uint64_t st) { return a.partner[st ^ 1] == NONE; }
+// the vertex at r's other end, entered at its matching end; a state with no
+// partner across the end it leaves through is a chain end.
 
 __global__ __launch_bounds__(256) void k_rank_init(CdbgArgs a) {
     for (uint64_t st = uint64_t(blockIdx.x) * 256 + threadIdx.x; st < 2 * a.n; st += uint64_t(gridDim.x) * 256) {
@@ -337,74 +336,115 @@ __global__ __launch_bounds__(256) void k_rank_round(CdbgArgs a, uint32_t* change
     if (any) atomicOr(changed, 1u);
 }
 
-// Per piece: chain ends, orientation, offset and length of its chain; the
-// first piece of each chain reserves the output (wave-aggregated).
+// Per state, as bits of 64-state words: whether it is a chain end (no
+// partner across the end it leaves through) and, for an end E, whether the
+// chain read from E comes first: K(E) < K(F) for F = E's far end (where the
+// reading of E's piece the other way stops) and K(E) = the first k characters
+// of that reading (kend of E's piece on the side it leaves through backwards).
+// k_place then reads two bits per piece instead of the partners of both its
+// ends and the k-mers at both ends of its chain.
+template <int KW>
+__global__ __launch_bounds__(256) void k_chain_bits(CdbgArgs a) {
+    const Kmer<KW>* kend = reinterpret_cast<const Kmer<KW>*>(a.kend);
+    const uint32_t lane = threadIdx.x & 63u;
+    const auto K = [&](uint64_t e) { return kend[2 * (e >> 1) + 1 - (e & 1u)]; };
+    for (uint64_t s0 = uint64_t(blockIdx.x) * 256 + (threadIdx.x & ~63u); s0 < 2 * a.n;
+         s0 += uint64_t(gridDim.x) * 256) {  // wave-uniform, 64-state words
+        const uint64_t st = s0 + lane;
+        bool end = false, first = false;
+        if (st < 2 * a.n) {
+            end = a.partner[st ^ 1] == NONE;
+            if (end) first = km_less(K(st), K(a.jp[st ^ 1] & JT_MASK));
+        }
+        const unsigned long long me = __ballot(end), mf = __ballot(first);
+        if (lane == 0) {
+            a.end_bits[s0 >> 6] = me;
+            a.first_bits[s0 >> 6] = mf;
+        }
+    }
+}
+
+// Per piece: chain ends, orientation, offset and length of its chain.  The
+// first piece of each chain (its head) reserves the chain's output bytes and
+// record: a workgroup takes PL_Q pieces per thread, scans its heads' lengths
+// and makes one reservation per PL_Q * 256 pieces (reservations per wave made
+// the two counters' device-scope atomics the kernel's bound).
+constexpr uint32_t PL_Q = 8;
 template <int KW>
 __global__ __launch_bounds__(256) void k_place(CdbgArgs a) {
+    __shared__ unsigned long long s_wb[4], s_wc[4], s_base, s_rbase;
     const uint64_t* jp = a.jp;
-    const int lane = int(threadIdx.x & 63u);
-    for (uint64_t r0 = uint64_t(blockIdx.x) * 256; r0 < a.n; r0 += uint64_t(gridDim.x) * 256) {
-        const uint64_t r = r0 + threadIdx.x;
-        bool head = false;
-        uint64_t len = 0;
-        if (r < a.n) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6;
+    const auto is_end = [&](uint64_t e) { return (a.end_bits[e >> 6] >> (e & 63u)) & 1u; };
+    for (uint64_t r0 = uint64_t(blockIdx.x) * 256 * PL_Q; r0 < a.n; r0 += uint64_t(gridDim.x) * 256 * PL_Q) {
+        uint64_t len[PL_Q];
+        uint32_t heads = 0;  // bit i: piece r0 + 256 i + tid heads its chain
+        uint64_t my_bytes = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < PL_Q; ++i) {
+            const uint64_t r = r0 + 256 * i + tid;
+            len[i] = 0;
+            if (r >= a.n) continue;
             const UnitigRec u = a.recs[r];
             const uint64_t w0 = jp[2 * r], w1 = jp[2 * r + 1];
             const uint64_t E0 = w0 & JT_MASK, E1 = w1 & JT_MASK;
-            // E0 and E1 are where the piece's two readings stop: a chain's ends,
-            // or, on a cycle, wherever the jumping stopped (a cycle state can land
-            // on itself).  Ends are the states with no partner (as in k_rank_init).
-            if (!rank_end(a, E0) || !rank_end(a, E1)) {  // a cycle
+            // E0 and E1 are where the piece's two readings stop: a chain's
+            // ends, or, on a cycle, wherever the jumping stopped (a cycle
+            // state can land on itself)
+            if (!is_end(E0) || !is_end(E1)) {  // a cycle
                 a.place_head[r] = NONE;
                 atomicAdd(a.n_cycle_pieces, 1u);
-            } else {
-                const uint64_t c0 = w0 >> JT_BITS, c1 = w1 >> JT_BITS;
-                bool fwd = true;  // read the chain with this piece forward
-                if (!(u.flags & 1u)) {
-                    // first piece in that reading: the end state of direction 1,
-                    // read from its unpaired end; last piece: the end of direction 0
-                    // kend[2j] = first k characters of piece j, kend[2j+1] = rc of its last k
-                    const Kmer<KW>* kend = reinterpret_cast<const Kmer<KW>*>(a.kend);
-                    const uint32_t jf = E1 >> 1, af = 1u - (E1 & 1u), jl = E0 >> 1, al = E0 & 1u;
-                    const Kmer<KW> first = kend[2 * uint64_t(jf) + af];
-                    const Kmer<KW> rlast = kend[2 * uint64_t(jl) + (1u - al)];  // rc of the last k characters
-                    fwd = km_less(first, rlast);
-                }
-                const uint32_t hd = uint32_t(fwd ? (E1 >> 1) : (E0 >> 1));
-                a.place_head[r] = hd;
-                // the piece ends its chain when nothing follows it in that reading
-                const uint64_t last = (fwd ? c0 : c1) == 0 ? 1u : 0u;
-                a.place_pos[r] = (fwd ? c1 : c0) | (last << 62) | (uint64_t(fwd ? 0u : 1u) << 63);
-                head = fwd ? E1 == 2 * r + 1 : E0 == 2 * r;  // nothing before it in that reading
-                len = c0 + c1 + u.label_len;
+                continue;
+            }
+            const uint64_t c0 = w0 >> JT_BITS, c1 = w1 >> JT_BITS;
+            // read the chain with this piece forward when the reading from E1
+            // (the end of direction 1) comes first: its first k characters
+            // below the reverse complement of the last k at E0 (k_chain_bits);
+            // a cycle unitig of one subgraph is as written
+            const bool fwd = (u.flags & 1u) || ((a.first_bits[E1 >> 6] >> (E1 & 63u)) & 1u);
+            a.place_head[r] = uint32_t(fwd ? (E1 >> 1) : (E0 >> 1));
+            // the piece ends its chain when nothing follows it in that reading
+            const uint64_t last = (fwd ? c0 : c1) == 0 ? 1u : 0u;
+            a.place_pos[r] = (fwd ? c1 : c0) | (last << 62) | (uint64_t(fwd ? 0u : 1u) << 63);
+            if (fwd ? E1 == 2 * r + 1 : E0 == 2 * r) {  // nothing before it in that reading
+                len[i] = c0 + c1 + u.label_len;
+                heads |= 1u << i;
+                my_bytes += len[i];
             }
         }
-        // heads reserve their chain's bytes: one atomic per wave
-        uint64_t incl = head ? len : 0;
+        // exclusive prefix of the heads' bytes and counts over the workgroup
+        uint64_t ib = my_bytes, ic = uint64_t(__popc(heads));
+        const uint64_t my_cnt = ic;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
-            const uint64_t v = __shfl_up(incl, d);
-            if (lane >= d) incl += v;
+            const uint64_t vb = __shfl_up(ib, d), vc = __shfl_up(ic, d);
+            if (int(lane) >= d) ib += vb, ic += vc;
         }
-        const uint64_t tot = __shfl(incl, 63);
-        const unsigned long long hb = __ballot(head);
-        uint64_t base = 0, rbase = 0;
-        if (lane == 0 && tot) {
-            base = atomicAdd(&a.cursors[1], (unsigned long long)tot);
-            rbase = atomicAdd(&a.cursors[0], (unsigned long long)__popcll(hb));
+        if (lane == 63) s_wb[wv] = ib, s_wc[wv] = ic;
+        __syncthreads();
+        if (tid == 0) {
+            const unsigned long long tb = s_wb[0] + s_wb[1] + s_wb[2] + s_wb[3];
+            const unsigned long long tc = s_wc[0] + s_wc[1] + s_wc[2] + s_wc[3];
+            s_base = tb ? atomicAdd(&a.cursors[1], tb) : 0ull;
+            s_rbase = tc ? atomicAdd(&a.cursors[0], tc) : 0ull;
         }
-        base = __shfl(base, 0);
-        rbase = __shfl(rbase, 0);
-        if (head) {
-            const uint64_t off = base + incl - len;
-            const uint64_t ri = rbase + uint64_t(__popcll(hb & ((1ull << lane) - 1)));
+        __syncthreads();
+        uint64_t off = s_base + ib - my_bytes, ri = s_rbase + ic - my_cnt;
+        for (uint32_t wb = 0; wb < wv; ++wb) off += s_wb[wb], ri += s_wc[wb];
+        __syncthreads();  // s_wb, s_wc and the bases read before the next round writes them
+#pragma unroll
+        for (uint32_t i = 0; i < PL_Q; ++i) {
+            if (!((heads >> i) & 1u)) continue;
+            const uint64_t r = r0 + 256 * i + tid;
             a.head_off[r] = off;
-            if (off + len <= a.out_cap && ri < a.out_n_cap) {
+            if (off + len[i] <= a.out_cap && ri < a.out_n_cap) {
                 a.out_rec[2 * ri] = off;
-                a.out_rec[2 * ri + 1] = len;
+                a.out_rec[2 * ri + 1] = len[i];
             } else {
                 atomicOr(a.retry, 1u);
             }
+            off += len[i];
+            ++ri;
         }
     }
 }
@@ -484,7 +524,10 @@ void launch_kw(const CdbgArgs& a, uint32_t grid, int phase, uint32_t src, uint32
         }
         case 1: hipLaunchKernelGGL(k_rank_init, dim3(grid), dim3(256), 0, s, a); break;
         case 2: hipLaunchKernelGGL(k_rank_round, dim3(grid), dim3(256), 0, s, a, changed); break;
-        case 3: hipLaunchKernelGGL(k_place<KW>, dim3(grid), dim3(256), 0, s, a); break;
+        case 3:
+            hipLaunchKernelGGL(k_chain_bits<KW>, dim3(grid), dim3(256), 0, s, a);
+            hipLaunchKernelGGL(k_place<KW>, dim3(grid), dim3(256), 0, s, a);
+            break;
         default: hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, a); break;
     }
 }

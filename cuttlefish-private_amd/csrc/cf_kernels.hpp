@@ -311,6 +311,8 @@ struct CdbgArgs {
     uint64_t* kend;             // per piece: its first k characters and the rc of its last k (KW words each)
     uint64_t* jp;               // per state: the state it points at | label characters up to it << 30
     uint32_t* settled;          // a bit per state: settled (zeroed before the ranking passes)
+    uint64_t* end_bits;         // a bit per state: a chain end (k_chain_bits)
+    uint64_t* first_bits;       // a bit per state: an end whose reading comes first
     uint64_t total_chars;       // label bytes of all pieces
     uint32_t* place_head;       // per piece: first piece of its chain (all-ones: on a cycle)
     uint64_t* place_pos;        //   ... its offset in the chain | ends the chain << 62 | reversed << 63

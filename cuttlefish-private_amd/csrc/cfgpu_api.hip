@@ -2090,8 +2090,10 @@ int cfgpu_cdbg(cfgpu_handle* h) {
         a.error = c + 3;
         a.n_cycle_pieces = c + 4;
         const uint32_t grid = uint32_t(std::min<uint64_t>((2 * n + 255) / 256, uint64_t(h->cus) * 8));
-        HIPCK(h, ensure_mem(h, h->cd_settled, ((2 * n + 63) / 64) * 8, Phase::Cdbg));  // whole words of 64 states
+        HIPCK(h, ensure_mem(h, h->cd_settled, ((2 * n + 63) / 64) * 8 * 3, Phase::Cdbg));  // whole words of 64 states
         a.settled = h->cd_settled.as<uint32_t>();
+        a.end_bits = h->cd_settled.as<uint64_t>() + (2 * n + 63) / 64;
+        a.first_bits = a.end_bits + (2 * n + 63) / 64;
         launch_fill_u32(a.settled, 0u, ((2 * n + 63) / 64) * 2, h->stream);
         launch_cdbg_phase(a, KW, grid, 0, 0, nullptr, h->stream);
         launch_cdbg_phase(a, KW, grid, 1, 0, nullptr, h->stream);

@@ -35,15 +35,30 @@ constexpr int PER_T = (S_MAX + P_WG - 1) / P_WG;
 // tile_frag[t] = the fragment holding position max(t * P_TILE - 1, 0): the
 // largest f with frag_off[f] <= that position.  Fragment f covers the tiles
 // whose position falls in [frag_off[f], frag_off[f+1]) (frag_off[0] = 0,
-// strictly increasing), so every tile is written exactly once.
+// strictly increasing), so every tile is written exactly once.  One lane per
+// fragment; the tiles of a fragment spanning more than 8 tiles (a genome
+// record) are written by its whole wave.
 __global__ __launch_bounds__(256) void k_tile_frag(const uint64_t* __restrict__ frag_off, uint64_t n_frag,
                                                    uint64_t tiles, uint64_t* __restrict__ tile_frag) {
-    for (uint64_t f = uint64_t(blockIdx.x) * 256 + threadIdx.x; f < n_frag; f += uint64_t(gridDim.x) * 256) {
-        const uint64_t b = frag_off[f], e = frag_off[f + 1];
-        // tiles t >= 1 with t * P_TILE - 1 in [b, e); tile 0 (position 0) belongs to f = 0
-        const uint64_t t0 = f == 0 ? 0 : (b + P_TILE) / P_TILE;  // ceil((b + 1) / P_TILE)
-        const uint64_t t1 = min((e + P_TILE) / P_TILE, tiles);   // ceil((e + 1) / P_TILE)
-        for (uint64_t t = t0; t < t1; ++t) tile_frag[t] = f;
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint64_t fw = uint64_t(blockIdx.x) * 256 + (threadIdx.x & ~63u); fw < n_frag;
+         fw += uint64_t(gridDim.x) * 256) {  // wave-uniform
+        const uint64_t f = fw + lane;
+        uint64_t t0 = 0, t1 = 0;
+        if (f < n_frag) {
+            const uint64_t b = frag_off[f], e = frag_off[f + 1];
+            // tiles t >= 1 with t * P_TILE - 1 in [b, e); tile 0 (position 0) belongs to f = 0
+            t0 = f == 0 ? 0 : (b + P_TILE) / P_TILE;  // ceil((b + 1) / P_TILE)
+            t1 = max(t0, min((e + P_TILE) / P_TILE, tiles));  // ceil((e + 1) / P_TILE)
+        }
+        const bool wide = t1 - t0 > 8;
+        if (!wide)
+            for (uint64_t t = t0; t < t1; ++t) tile_frag[t] = f;
+        for (unsigned long long m = __ballot(wide); m; m &= m - 1) {
+            const int src = __ffsll((long long)m) - 1;
+            const uint64_t a0 = __shfl(t0, src), a1 = __shfl(t1, src);
+            for (uint64_t t = a0 + lane; t < a1; t += 64) tile_frag[t] = fw + uint64_t(src);
+        }
     }
 }
 
