@@ -448,31 +448,38 @@ def main():
             t1 = time.perf_counter()
             sp = h.spill_stats()
             nz = np.nonzero(h.bucket_stats()["n_records"])[0].astype(np.uint32)
-            tb = {"restore": 0.0, "construct": 0.0, "contract": 0.0, "unitigs_to_host": 0.0}
-            n_u = n_ub = 0
             mem = 0.0
             uo = UnitigOut(hip)
-            for bt in np.array_split(nz, args.external_batches):
-                a0 = time.perf_counter()
-                h.restore(bt)
-                a1 = time.perf_counter()
-                h.construct_buckets(bt, "learned")
-                hip.sync()
-                a2 = time.perf_counter()
-                h.contract()
-                hip.sync()
-                a3 = time.perf_counter()
-                nu, nb = uo.copy(h)
-                a4 = time.perf_counter()
-                mem = max(mem, hip.used_gib() - mem0)
-                tb["restore"] += (a1 - a0) * 1e3
-                tb["construct"] += (a2 - a1) * 1e3
-                tb["contract"] += (a3 - a2) * 1e3
-                tb["unitigs_to_host"] += (a4 - a3) * 1e3
-                n_u += nu
-                n_ub += nb
-            t2 = time.perf_counter()
-            assert n_u == cst_unitigs, "batches and the one-pass construct disagree on the unitig count"
+            # the batches twice, the faster pass reported (as the pipelined
+            # ingest above): the first pass also grows the handle's buffers,
+            # right after the one-pass handle's memory was given back
+            passes = []
+            for _ in range(2):
+                tb = {"restore": 0.0, "construct": 0.0, "contract": 0.0, "unitigs_to_host": 0.0}
+                n_u = n_ub = 0
+                for bt in np.array_split(nz, args.external_batches):
+                    a0 = time.perf_counter()
+                    h.restore(bt)
+                    a1 = time.perf_counter()
+                    h.construct_buckets(bt, "learned")
+                    hip.sync()
+                    a2 = time.perf_counter()
+                    h.contract()
+                    hip.sync()
+                    a3 = time.perf_counter()
+                    nu, nb = uo.copy(h)
+                    a4 = time.perf_counter()
+                    mem = max(mem, hip.used_gib() - mem0)
+                    tb["restore"] += (a1 - a0) * 1e3
+                    tb["construct"] += (a2 - a1) * 1e3
+                    tb["contract"] += (a3 - a2) * 1e3
+                    tb["unitigs_to_host"] += (a4 - a3) * 1e3
+                    n_u += nu
+                    n_ub += nb
+                assert n_u == cst_unitigs, "batches and the one-pass construct disagree on the unitig count"
+                passes.append((sum(tb.values()), tb))
+            t_batches, tb = min(passes, key=lambda x: x[0])
+            t2 = t1 + t_batches / 1e3
             external = {"store": "pinned host memory", "segments": sp["n_segments"], "store_bytes": sp["bytes"],
                         "batches": args.external_batches, "ingest_into_store_ms": (t1 - t0) * 1e3,
                         "device_to_store_ms": sp["t_spill_ms"], "batch_ms": tb,
@@ -482,8 +489,9 @@ def main():
                         "device_memory_gib": mem,
                         "note": "cfhost_partition_buffer_ex with CFGPU_PARTITION_SPILL, then per batch "
                                 "cfgpu_buckets_restore -> cfgpu_construct_buckets -> cfgpu_contract -> "
-                                "cfgpu_unitigs_copy (the per-subgraph unitigs; the host glues the batches); "
-                                "device_memory_gib: the handle's device memory after the batches (a fresh handle)"}
+                                "cfgpu_unitigs_copy (the per-subgraph unitigs; the host glues the batches); the "
+                                "faster of two passes over the batches; device_memory_gib: the handle's device "
+                                "memory after the batches (a fresh handle)"}
         del text
 
     cpu = None
