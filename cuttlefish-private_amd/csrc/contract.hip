@@ -818,8 +818,11 @@ constexpr uint32_t CB_M16 = 0xFFFFu;
 __device__ __forceinline__ uint32_t cb_pack(uint32_t p, uint32_t d) { return p | (min(d, CB_M16) << 16); }
 
 // Ranking words kept in LDS (over the index, no longer needed once linked)
-// when the directed states fit: 2n <= CB_JP_LDS.
+// when the directed states fit: 2n <= CB_JP_LDS; otherwise the successor ids
+// (u16) and a bit per state (from word CB_WALK_BITS on) for the path walks.
 constexpr uint32_t CB_JP_LDS = 39936;
+constexpr uint32_t CB_WALK_BITS = 32768;
+static_assert(CB_WALK_BITS * 2 >= 2 * CB_NMAX && CB_WALK_BITS + (2 * CB_NMAX + 31) / 32 <= CB_JP_LDS, "walk buffers");
 constexpr uint32_t CB_JQ = CB_JP_LDS / CT_WG;  // states per thread
 static_assert(CB_JP_LDS >= CB_HCAP && CB_JP_LDS % CT_WG == 0, "LDS ranking buffer");
 constexpr uint32_t CB_RU = 8;                        // states per thread in flight, global ranking
@@ -833,7 +836,7 @@ __global__ __launch_bounds__(CT_WG) void k_contract_big(ContractArgs a) {
     __shared__ uint32_t s_cand[CT_CAND];
     __shared__ uint32_t s_cnt[8];
     __shared__ uint32_t s_chg[CB_ROUNDS];
-    __shared__ uint32_t s_item, s_n, s_ovf, s_ncand, s_bytes, s_nrec, s_rctr, s_err, s_skip;
+    __shared__ uint32_t s_item, s_n, s_ovf, s_ncand, s_bytes, s_nrec, s_rctr, s_err, s_skip, s_wq;
     __shared__ unsigned long long s_brec, s_bbyte;
     const uint32_t tid = threadIdx.x;
     const uint32_t k = a.k;
@@ -937,50 +940,50 @@ __global__ __launch_bounds__(CT_WG) void k_contract_big(ContractArgs a) {
             }
             jp = s_buf;
         } else {
-            // in global scratch, two buffers; CB_RU states per thread at a
-            // time, their loads issued before any store (the stores could
-            // alias them as far as the compiler knows)
-            const uint32_t* src = jp0;
-            uint32_t* dst = jp1;
-            for (uint32_t r = 0; r < CB_ROUNDS; ++r) {
-                bool any = false;
-                for (uint32_t x0 = tid; x0 < 2 * n; x0 += CT_WG * CB_RU) {
-                    uint32_t w[CB_RU], w2[CB_RU];
-#pragma unroll
-                    for (uint32_t u = 0; u < CB_RU; ++u) {
-                        const uint32_t x = x0 + u * CT_WG;
-                        w[u] = x < 2 * n ? src[x] : x;
-                    }
-#pragma unroll
-                    for (uint32_t u = 0; u < CB_RU; ++u) {
-                        const uint32_t x = x0 + u * CT_WG, p = w[u] & CB_M16;
-                        w2[u] = x < 2 * n && p != x ? src[p] : 0u;
-                    }
-#pragma unroll
-                    for (uint32_t u = 0; u < CB_RU; ++u) {
-                        const uint32_t x = x0 + u * CT_WG;
-                        if (x >= 2 * n) continue;
-                        const uint32_t p = w[u] & CB_M16;
-                        uint32_t nw = w[u];
-                        if (p != x) {
-                            const uint32_t p2 = w2[u] & CB_M16, nd = (w[u] >> 16) + (w2[u] >> 16);
-                            nw = cb_pack(p2, nd);
-                            any |= p2 != p && nd < n;
-                        }
-                        dst[x] = nw;
-                    }
+            // 2n states beyond the LDS ranking buffer (2n <= 65534): every
+            // path is walked from its start (a state no other state leads to)
+            // over successor ids held in LDS (16 bits each), twice: to its end
+            // and length, then writing each state's end and distance.  States
+            // on cycles have no start and keep their link words (successor,
+            // distance 1), which the passes below tell from a path's.
+            // Pointer jumping over these states in global scratch read a
+            // random word per state and round.
+            uint16_t* s_succ = reinterpret_cast<uint16_t*>(s_buf);
+            uint32_t* s_led = s_buf + CB_WALK_BITS;  // bit x: some state leads to x
+            for (uint32_t x = tid; x < 2 * n; x += CT_WG) s_succ[x] = uint16_t(jp0[x] & CB_M16);
+            for (uint32_t i = tid; i < (2 * n + 31) / 32; i += CT_WG) s_led[i] = 0;
+            if (tid == 0) s_wq = 0;
+            __syncthreads();
+            for (uint32_t x = tid; x < 2 * n; x += CT_WG) {
+                const uint32_t y = s_succ[x];
+                if (y != x) atomicOr(&s_led[y >> 5], 1u << (y & 31u));
+                jp1[x] = jp0[x];
+            }
+            __syncthreads();
+            while (true) {  // a wave takes 64 states at a time and walks from those that start a path
+                uint32_t x0 = 0;
+                if ((tid & 63u) == 0) x0 = atomicAdd(&s_wq, 64u);
+                x0 = uint32_t(__builtin_amdgcn_readfirstlane(int(x0)));
+                if (x0 >= 2 * n) break;
+                const uint32_t x = x0 + (tid & 63u);
+                if (x >= 2 * n || ((s_led[x >> 5] >> (x & 31u)) & 1u)) continue;
+                uint32_t len = 1, y = x;
+                for (uint32_t z = s_succ[y]; z != y && len <= 2 * n; z = s_succ[y]) y = z, ++len;
+                if (len > 2 * n) {  // a path longer than the states: an inconsistent table
+                    s_err = 1;
+                    continue;
                 }
-                if (any) s_chg[r] = 1;
-                __syncthreads();
-                src = dst;
-                dst = dst == jp1 ? ia : jp1;  // ia (with posv) is free until the info pass: 2N words
-                if (!s_chg[r]) break;
+                const uint32_t F = y;
+                y = x;
+                for (uint32_t i = 0;; ++i) {
+                    jp1[y] = cb_pack(F, len - 1 - i);
+                    const uint32_t z = s_succ[y];
+                    if (z == y) break;
+                    y = z;
+                }
             }
-            if (src == ia) {  // the info pass below writes ia: move the words to jp1
-                for (uint32_t x = tid; x < 2 * n; x += CT_WG) jp1[x] = ia[x];
-                __syncthreads();
-                src = jp1;
-            }
+            __syncthreads();
+            const uint32_t* src = jp1;
             jp = src;
         }
 
