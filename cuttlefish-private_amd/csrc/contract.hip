@@ -129,13 +129,16 @@ __device__ __forceinline__ int tab_find(const CtTab<KW>& t, const Kmer<KW>& x) {
 // Dense vertex arrays in global scratch with an LDS index: subgraphs whose
 // non-isolated vertices exceed an LDS table.  Vertex d (0 <= d < cap) has its
 // key words at key[d * KW ..] and its state word at meta[d]; the LDS hash
-// `ht` maps a key to d (entry d | 16 fingerprint bits of the key's hash << 16,
-// all-ones = free), so a lookup reads the one vertex whose fingerprint matches.
+// `ht` maps a key to d (entry d | the edge codes of d's two sides << 16 | 10
+// fingerprint bits of the key's hash << 22, all-ones = free), so a lookup
+// reads the one vertex whose fingerprint matches, and the link pass reads a
+// neighbour's entrance side from the entry.
 constexpr uint32_t CB_HCAP = 36864;  // LDS index slots (u32 each)
 constexpr uint32_t CB_NMAX = 32767;  // vertices an index takes (load <= 0.89): directed states fit 16 bits
 static_assert(CB_NMAX == uint32_t(ct_big_max_vertices()), "host and device agree on the dense table size");
 static_assert(2 * CB_NMAX < (1u << 16), "directed states fit 16 bits");
 constexpr uint32_t HT_FREE = 0xFFFFFFFFu;
+constexpr uint32_t HT_FP = 0xFFC00000u;  // fingerprint bits of a dense-tier index entry
 
 template <int KW>
 struct BigTab {
@@ -155,24 +158,31 @@ __device__ __forceinline__ Kmer<KW> tab_key(const BigTab<KW>& t, uint32_t i) {
     return x;
 }
 
+// The index entry of key x (HT_FREE when x is not indexed), keys compared.
 template <int KW>
-__device__ __forceinline__ int tab_find(const BigTab<KW>& t, const Kmer<KW>& x) {
+__device__ __forceinline__ uint32_t big_entry(const BigTab<KW>& t, const Kmer<KW>& x) {
     const uint64_t h = km_hash(x);
-    const uint32_t fp = uint32_t(h) & 0xFFFF0000u;
+    const uint32_t fp = uint32_t(h) & HT_FP;
     uint32_t s = slot_of(h, CB_HCAP);
     for (uint32_t n = 0; n < CB_HCAP; ++n) {
         const uint32_t e = t.ht[s];
-        if (e == HT_FREE) return -1;
-        if ((e & 0xFFFF0000u) == fp) {
+        if (e == HT_FREE) return HT_FREE;
+        if ((e & HT_FP) == fp) {
             const uint32_t d = e & 0xFFFFu;
             bool eq = true;
 #pragma unroll
             for (int w = 0; w < KW; ++w) eq &= t.key[size_t(d) * KW + w] == x.w[w];
-            if (eq) return int(d);
+            if (eq) return e;
         }
         s = s + 1 == CB_HCAP ? 0 : s + 1;
     }
-    return -1;
+    return HT_FREE;
+}
+
+template <int KW>
+__device__ __forceinline__ int tab_find(const BigTab<KW>& t, const Kmer<KW>& x) {
+    const uint32_t e = big_entry(t, x);
+    return e == HT_FREE ? -1 : int(e & 0xFFFFu);
 }
 
 // Lookup of a key known to be in the index (a neighbour through an edge at
@@ -183,21 +193,27 @@ __device__ __forceinline__ int tab_find(const BigTab<KW>& t, const Kmer<KW>& x) 
 // always in an index loaded past 0.7, whose chains run long.
 constexpr uint32_t CB_PRESENT_MAX = CB_HCAP * 7 / 10;
 template <int KW>
-__device__ __forceinline__ int tab_find_present(const BigTab<KW>& t, const Kmer<KW>& x) {
-    if (t.cap > CB_PRESENT_MAX) return tab_find(t, x);
+__device__ __forceinline__ uint32_t big_entry_present(const BigTab<KW>& t, const Kmer<KW>& x) {
+    if (t.cap > CB_PRESENT_MAX) return big_entry(t, x);
     const uint64_t h = km_hash(x);
-    const uint32_t fp = uint32_t(h) & 0xFFFF0000u;
-    uint32_t s = slot_of(h, CB_HCAP), n_fp = 0, d = 0;
+    const uint32_t fp = uint32_t(h) & HT_FP;
+    uint32_t s = slot_of(h, CB_HCAP), n_fp = 0, one = HT_FREE;
     for (uint32_t n = 0; n < CB_HCAP; ++n) {
         const uint32_t e = t.ht[s];
         if (e == HT_FREE) break;
-        if ((e & 0xFFFF0000u) == fp) {
-            d = e & 0xFFFFu;
+        if ((e & HT_FP) == fp) {
+            one = e;
             if (++n_fp > 1) break;
         }
         s = s + 1 == CB_HCAP ? 0 : s + 1;
     }
-    return n_fp == 1 ? int(d) : tab_find(t, x);
+    return n_fp == 1 ? one : big_entry(t, x);
+}
+
+template <int KW>
+__device__ __forceinline__ int tab_find_present(const BigTab<KW>& t, const Kmer<KW>& x) {
+    const uint32_t e = big_entry_present(t, x);
+    return e == HT_FREE ? -1 : int(e & 0xFFFFu);
 }
 
 template <int KW>
@@ -213,12 +229,12 @@ __device__ __forceinline__ int walk_next(const BigTab<KW>& t, uint32_t cur, uint
     return int(st >> 1);
 }
 
-// Index vertex d under key x (keys are unique): the first free slot from the
-// key's home slot.
+// Index vertex d (state m) under key x (keys are unique): the first free
+// slot from the key's home slot.
 template <int KW>
-__device__ __forceinline__ void big_index(uint32_t* ht, const Kmer<KW>& x, uint32_t d) {
+__device__ __forceinline__ void big_index(uint32_t* ht, const Kmer<KW>& x, uint32_t d, uint32_t m) {
     const uint64_t h = km_hash(x);
-    const uint32_t e = d | (uint32_t(h) & 0xFFFF0000u);
+    const uint32_t e = d | ((m & 0x3Fu) << 16) | (uint32_t(h) & HT_FP);
     uint32_t s = slot_of(h, CB_HCAP);
     while (atomicCAS(&ht[s], HT_FREE, e) != HT_FREE) s = s + 1 == CB_HCAP ? 0 : s + 1;
 }
@@ -500,6 +516,31 @@ __device__ __forceinline__ int link_of(const Tab& t, uint32_t i, uint32_t s, con
     if (ej >= 4) return -1;                      // branching entrance: crossed
     if (uint32_t(j) == i && ent == s) return -1; // back through the same side
     return int((uint32_t(j) << 1) | side);
+}
+
+// link_of for the dense tier: the neighbour's entrance side read from its
+// index entry instead of its state word in global scratch.
+template <int KW>
+__device__ __forceinline__ int link_of_big(const BigTab<KW>& t, uint32_t i, uint32_t s, const Kmer<KW>& x,
+                                           const Kmer<KW>& xr, uint32_t m, uint32_t k, uint32_t* err) {
+    if (side_edge(m, s) >= 4) return -1;
+    Kmer<KW> z = s ? x : xr, zb = s ? xr : x;
+    const uint32_t e = side_edge(m, s), b = s == 1 ? e : 3u - e;
+    z = km_append(z, b, k);
+    zb = km_prepend(zb, uint8_t(3u - b), k);
+    const bool can = km_less(z, zb);
+    const uint32_t side = can ? 1u : 0u;  // entrance side front (canonical) -> leave through the back
+    const uint32_t ent_w = big_entry_present(t, can ? z : zb);
+    if (ent_w == HT_FREE) {
+        *err = 1;
+        return -1;
+    }
+    const uint32_t j = ent_w & 0xFFFFu, ent = 1u - side;
+    const uint32_t ej = side_edge(ent_w >> 16, ent);
+    if (ej == BASE_E) *err = 1;            // the edge is counted at both ends
+    if (ej >= 4) return -1;                // branching entrance: crossed
+    if (j == i && ent == s) return -1;     // back through the same side
+    return int((j << 1) | side);
 }
 
 // Thread 0: every cycle among the candidate slots (or all slots marked M_ONCYC
@@ -876,7 +917,7 @@ __global__ __launch_bounds__(CT_WG) void k_contract_big(ContractArgs a) {
 #pragma unroll
             for (int w = 0; w < KW; ++w) t.key[size_t(d) * KW + w] = x.w[w];
             t.meta[d] = m;
-            big_index(s_ht, x, d);
+            big_index(s_ht, x, d, m);
             return true;
         });
         __syncthreads();
@@ -894,7 +935,7 @@ __global__ __launch_bounds__(CT_WG) void k_contract_big(ContractArgs a) {
             const uint32_t m = ld_meta(&t.meta[d]);
             uint32_t add = 0;
             for (uint32_t sd = 0; sd < 2; ++sd) {
-                const int nx = link_of<KW>(t, d, sd, x, xr, m, k, &s_err);
+                const int nx = link_of_big<KW>(t, d, sd, x, xr, m, k, &s_err);
                 jp0[2 * d + sd] = nx < 0 ? cb_pack(2 * d + sd, 0) : cb_pack(uint32_t(nx), 1);
                 if (nx >= 0) add |= M_LINK0 << sd;
             }
