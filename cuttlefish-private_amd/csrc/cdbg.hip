@@ -51,16 +51,25 @@ __device__ __forceinline__ Kmer<KW> enc_kmer(const char* s, uint32_t k) {
     const uint32_t* p = reinterpret_cast<const uint32_t*>(reinterpret_cast<uintptr_t>(s) & ~uintptr_t(3));
     const uint32_t sh = uint32_t(reinterpret_cast<uintptr_t>(s) & 3u);
     const uint32_t nw = (sh + k + 3) / 4;
+    // every word is requested before the first is decoded (a loop over the
+    // runtime count would wait for each load in turn)
+    constexpr uint32_t NWM = 8 * KW + 1;  // (3 + 32 KW + 3) / 4 >= nw
+    uint32_t words[NWM];
+#pragma unroll
+    for (uint32_t j = 0; j < NWM; ++j) words[j] = j < nw ? p[j] : 0u;
     uint64_t acc[KW + 1];
 #pragma unroll
     for (int w = 0; w <= KW; ++w) acc[w] = 0;
-    for (uint32_t j = 0; j < nw; ++j) {
-        const uint32_t v = p[j];
+#pragma unroll
+    for (uint32_t j = 0; j < NWM; ++j) {
+        const uint32_t v = words[j];
         const uint32_t c = ((v >> 1) ^ (v >> 2)) & 0x03030303u;  // byte q: code of character 4j + q
         const uint32_t b8 = ((c & 3u) << 6) | (((c >> 8) & 3u) << 4) | (((c >> 16) & 3u) << 2) | (c >> 24);
+        if (j < nw) {
 #pragma unroll
-        for (int w = KW; w > 0; --w) acc[w] = (acc[w] << 8) | (acc[w - 1] >> 56);
-        acc[0] = (acc[0] << 8) | b8;
+            for (int w = KW; w > 0; --w) acc[w] = (acc[w] << 8) | (acc[w - 1] >> 56);
+            acc[0] = (acc[0] << 8) | b8;
+        }
     }
     // drop the characters read past the k-mer (fewer than 4), then the ones
     // before it (the mask)
