@@ -467,18 +467,29 @@ __device__ __forceinline__ uint32_t load4(const char* s) {
     return __builtin_amdgcn_alignbyte(q[1], q[0], uint32_t(ad & 3u));  // ({q1, q0} >> 8 * (ad & 3))[31:0]
 }
 
+// Sixteen characters at s (any alignment): five aligned loads and four byte
+// funnel shifts (the label buffer's slack covers the fifth word).
+__device__ __forceinline__ uint4 load16(const char* s) {
+    const uintptr_t ad = reinterpret_cast<uintptr_t>(s);
+    const uint32_t* q = reinterpret_cast<const uint32_t*>(ad & ~uintptr_t(3));
+    const uint32_t sh = uint32_t(ad & 3u);
+    const uint32_t q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4];
+    return make_uint4(__builtin_amdgcn_alignbyte(q1, q0, sh), __builtin_amdgcn_alignbyte(q2, q1, sh),
+                      __builtin_amdgcn_alignbyte(q3, q2, sh), __builtin_amdgcn_alignbyte(q4, q3, sh));
+}
+
 // Complement of four ASCII bases A C G T: C and G (bit 1 set) differ by
 // 0x04, A and T by 0x15.
 __device__ __forceinline__ uint32_t comp4(uint32_t v) {
     return v ^ (0x15151515u ^ (((v >> 1) & 0x01010101u) * 0x11u));
 }
 
-// Eight lanes per piece.  Every aligned 4-byte word of the output that lies
-// inside a piece's range is written by that piece (neighbours overlap by k >=
-// 3 characters, so each aligned word inside a chain lies inside one of its
-// pieces, and two pieces writing one word write the same characters); the
-// bytes of the partial words at the chain's two ends are written one by one
-// by its first and its last piece.
+// Eight lanes per piece.  Every aligned W-byte word of the output that lies
+// inside a piece's range is written by that piece (W = 16 for k >= 16, else
+// 4: neighbours overlap by k >= W - 1 characters, so each aligned word inside
+// a chain lies inside one of its pieces, and two pieces writing one word
+// write the same characters); the bytes of the partial words at the chain's
+// two ends are written one by one by its first and its last piece.
 __global__ __launch_bounds__(256) void k_copy(CdbgArgs a) {
     const uint32_t j = threadIdx.x & 7u;
     for (uint64_t r = (uint64_t(blockIdx.x) * 256 + threadIdx.x) >> 3; r < a.n; r += (uint64_t(gridDim.x) * 256) >> 3) {
@@ -496,11 +507,30 @@ __global__ __launch_bounds__(256) void k_copy(CdbgArgs a) {
         const char* src = a.labels + u.label_off;
         const uint32_t L = u.label_len;
         const uint64_t d0 = hoff + pos, d1 = d0 + L;  // the piece's output bytes
-        const uint64_t w0 = (d0 + 3) & ~3ull, w1 = d1 & ~3ull;
-        for (uint64_t x = w0 + 4 * j; x < w1; x += 32) {
-            const uint32_t i = uint32_t(x - d0);  // output characters i .. i+3 of the piece
-            const uint32_t v = rev ? comp4(__builtin_bswap32(load4(src + (L - 4 - i)))) : load4(src + i);
-            *reinterpret_cast<uint32_t*>(a.out + x) = v;
+        // whole aligned words of W bytes: 16 when neighbours overlap by k >= 16
+        // characters (every aligned 16-byte word inside a chain then lies
+        // inside one of its pieces), else 4
+        const uint64_t W = a.k >= 16 ? 16 : 4;
+        const uint64_t w0 = (d0 + W - 1) & ~(W - 1), w1 = d1 & ~(W - 1);
+        if (W == 16) {
+            for (uint64_t x = w0 + 16 * j; x < w1; x += 128) {
+                const uint32_t i = uint32_t(x - d0);  // output characters i .. i+15 of the piece
+                uint4 v;
+                if (rev) {
+                    const uint4 f = load16(src + (L - 16 - i));
+                    v = make_uint4(comp4(__builtin_bswap32(f.w)), comp4(__builtin_bswap32(f.z)),
+                                   comp4(__builtin_bswap32(f.y)), comp4(__builtin_bswap32(f.x)));
+                } else {
+                    v = load16(src + i);
+                }
+                *reinterpret_cast<uint4*>(a.out + x) = v;
+            }
+        } else {
+            for (uint64_t x = w0 + 4 * j; x < w1; x += 32) {
+                const uint32_t i = uint32_t(x - d0);  // output characters i .. i+3 of the piece
+                const uint32_t v = rev ? comp4(__builtin_bswap32(load4(src + (L - 4 - i)))) : load4(src + i);
+                *reinterpret_cast<uint32_t*>(a.out + x) = v;
+            }
         }
         const auto put = [&](uint64_t x) {
             const uint32_t i = uint32_t(x - d0);
