@@ -216,7 +216,11 @@ __global__ __launch_bounds__(256) void k_ends_scatter(CdbgArgs a) {
 // vertex, its partner.  A vertex side claimed twice is an inconsistent input.
 template <int KW>
 struct PairTab {
-    static constexpr uint32_t CAP = KW <= 3 ? 4096u : 3584u;
+    // 144 KiB of LDS for KW 1, 2 and 4 (keys + two ids per slot), 128 KiB for
+    // KW 3: the largest tables one workgroup's LDS holds, so that most bins
+    // take one pass
+    static constexpr uint32_t CAP = KW == 1 ? 9216u : KW == 2 ? 6144u : KW == 3 ? 4096u : 3584u;
+    static_assert(CAP * (8 * KW + 8) <= 147456, "pairing table in LDS");
 };
 
 template <int KW>
