@@ -118,6 +118,7 @@ __global__ __launch_bounds__(256) void k_ends_encode(CdbgArgs a) {
     const uint64_t r0 = uint64_t(blockIdx.x) * a.per_block, r1 = min(a.n, r0 + a.per_block);
     for (uint64_t r = r0 + threadIdx.x; r < r1; r += 256) {
         const UnitigRec u = a.recs[r];
+        a.piece_len[r] = u.label_len - a.k;  // for k_rank_init: 4 bytes per piece instead of a record
         const char* lab = a.labels + u.label_off;
         const Kmer<KW> A = enc_kmer<KW>(lab, a.k);
         const Kmer<KW> B = km_rc(enc_kmer<KW>(lab + u.label_len - a.k, a.k), a.k);
@@ -308,7 +309,7 @@ __global__ __launch_bounds__(1024) void k_ends_pair(CdbgArgs a) {
 __global__ __launch_bounds__(256) void k_rank_init(CdbgArgs a) {
     for (uint64_t st = uint64_t(blockIdx.x) * 256 + threadIdx.x; st < 2 * a.n; st += uint64_t(gridDim.x) * 256) {
         const uint32_t next = a.partner[st ^ 1];  // the end across the vertex it leaves through
-        a.jp[st] = next == NONE ? jword(st, 0) : jword(next, a.recs[next >> 1].label_len - a.k);
+        a.jp[st] = next == NONE ? jword(st, 0) : jword(next, a.piece_len[next >> 1]);
     }
 }
 

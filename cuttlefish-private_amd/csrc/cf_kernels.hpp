@@ -316,6 +316,7 @@ struct CdbgArgs {
     uint64_t total_chars;       // label bytes of all pieces
     uint32_t* place_head;       // per piece: first piece of its chain (all-ones: on a cycle)
     uint64_t* place_pos;        //   ... its offset in the chain | ends the chain << 62 | reversed << 63
+    uint32_t* piece_len;        // per piece: label characters after its first k-mer (place_pos's memory until k_place)
     uint64_t* head_off;         // per chain head: output offset
     char* out;
     uint64_t out_cap;

@@ -2080,6 +2080,7 @@ int cfgpu_cdbg(cfgpu_handle* h) {
         a.total_chars = h->n_ubytes;
         a.place_head = h->cd_head.as<uint32_t>();
         a.place_pos = h->cd_pos.as<uint64_t>();
+        a.piece_len = h->cd_pos.as<uint32_t>();  // n words of the n u64 place_pos words
         a.head_off = h->cd_hoff.as<uint64_t>();
         a.out = h->cd_out.as<char>();
         a.out_cap = h->n_ubytes;
